@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark: DoF x RK-stage updates per second of the fused FP64 wave DG
+RK4 stage on one MI355X.
+
+Workload (one "step" = one low-storage RK4 step = 5 fused stage kernels):
+512 x 512 quads on the unit square with interior vertices moved by
+U(-0.2h, 0.2h) (seeded), P = 7 (8 x 8 Gauss-Lobatto nodes), upwind flux,
+characteristic outflow on every boundary edge, p(0) = sum of 8 seeded
+Gaussian pulses (centres U[0.2, 0.8]^2, sigma U[0.02, 0.08], amplitudes
+N(0, 1)), u = v = 0, dt = 0.5 min_h / (P+1)^2, 2000 steps.  The state is in
+HBM before timing starts; the timed region is exactly `--steps` steps.
+
+Also reported on the same line: the FP32 path on the same workload with its
+relative L2 error against the FP64 result, the stage kernels' achieved HBM
+bandwidth (HIP events on the launch stream) against the 8 TB/s peak, and the
+CPU oracle timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(ndof: int, nel: int, s: int) -> float:
+    """Average algorithmic HBM bytes per fused stage launch.
+
+    Stage 0 reads u and writes u', du (3 words/DoF); stages 1-4 read u, du and
+    write u', du (4 words/DoF); every stage reads 64 B of corners and 32 B of
+    face connectivity per element.
+    """
+    return ndof * s * (3 + 4 * 4) / 5.0 + nel * 96.0
+
+
+def build_workload(nx: int):
+    from wavedg_amd import capi, meshgen
+
+    c = meshgen.config2(nx=nx)
+    mesh = capi.Mesh(c["vertices"], c["elements"])
+    n = c["n"]
+    bc = np.full(mesh.n_boundary, c["bc_code"], dtype=np.int32)
+    nodes = mesh.nodes(n)
+    p0 = meshgen.gaussian_sum(nodes, c["centers"], c["sigmas"], c["amps"])
+    u0 = meshgen.nodal_state(nodes, p0)
+    dt = 0.5 * mesh.min_h() / n**2
+    return c, mesh, n, bc, u0, dt
+
+
+def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0):
+    wave = capi.Wave2D(mesh, n, bc, precision=precision)
+    dtype = torch.float64 if precision == capi.F64 else torch.float32
+    u = torch.tensor(u0, dtype=dtype, device="cuda")
+    stream = torch.cuda.current_stream()
+    if warmup:
+        w = u.clone()
+        if graph:
+            wave.rk4_graph(w, dt, (warmup // graph) * graph or graph, graph)
+        else:
+            wave.rk4(w, dt, warmup)
+        del w
+    torch.cuda.synchronize()
+    wave.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if graph:
+        wave.rk4_graph(u, dt, steps, graph, stream=stream.cuda_stream)
+    else:
+        wave.rk4(u, dt, steps, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_kernels, n_kernels = wave.last_timing()
+    return wave, u, t1 - t0, ms_kernels, n_kernels
+
+
+def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
+    from oracle import oracle as O
+
+    om = O.Mesh(c["vertices"], c["elements"])
+    ow = O.Wave(om, n, bc)
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    ow.rk4(u0, dt, steps)
+    t1 = time.perf_counter()
+    return ow.ndof * 5 * steps / (t1 - t0), t1 - t0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=512, help="mesh size (512 = the benchmark workload)")
+    ap.add_argument("--graph", type=int, default=0, help="replay a HIP graph of this many steps (even)")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the element-partitioned multi-GPU "
+              "path is a design, not part of this build.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from wavedg_amd import capi
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    c, mesh, n, bc, u0, dt = build_workload(args.nx)
+    print(f"[bench] workload {c['name']}: nel={mesh.n_elem} n={n} dt={dt:.6e} steps={args.steps}", file=sys.stderr,
+          flush=True)
+
+    wave64, u64, sec64, ms_k64, nk64 = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F64, args.steps,
+                                                 args.warmup, args.graph)
+    ndof, nel = wave64.ndof, wave64.n_elem
+    value = ndof * 5 * args.steps / sec64
+    print(f"[bench] fp64: {sec64:.3f} s, {value:.4e} DoF*stage/s", file=sys.stderr, flush=True)
+    avg_ms = ms_k64 / max(nk64, 1)
+    bytes64 = stage_bytes(ndof, nel, 8)
+    achieved = bytes64 / (avg_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "DoF*RK-stage updates/s",
+        "value": value,
+        "unit": "DoF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec64 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded skewed mesh and Gaussian-pulse initial state)",
+        "config": {
+            "workload": f"wave2d {c['name']}: {args.nx}x{args.nx} skewed quads (+-0.2h), P=7, upwind, outflow, "
+                        f"8 Gaussian pulses, low-storage RK4 ({args.steps} steps)",
+            "n_elem": nel, "P": n - 1, "ndof": ndof, "dt": dt, "graph_steps": args.graph,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "wave_kernel<8,double,stage> (5 fused RK stages per step)",
+            "bytes_per_launch": bytes64,
+            "avg_launch_us": avg_ms * 1e3,
+        },
+    }
+
+    if not args.no_fp32:
+        wave32, u32, sec32, ms_k32, nk32 = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F32, args.steps,
+                                                     args.warmup, args.graph)
+        err = float(torch.linalg.norm(u32.double() - u64) / torch.linalg.norm(u64))
+        avg32 = ms_k32 / max(nk32, 1)
+        ach32 = stage_bytes(ndof, nel, 4) / (avg32 * 1e-3) / 1e9
+        out["fp32"] = {
+            "value": ndof * 5 * args.steps / sec32,
+            "ms_per_step": sec32 * 1e3 / args.steps,
+            "rel_l2_vs_fp64": err,
+            "achieved_GBs": ach32,
+            "frac": ach32 / HBM_PEAK_GBS,
+            "avg_launch_us": avg32 * 1e3,
+        }
+        print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
+        del wave32, u32
+
+    if args.cpu_steps > 0:
+        v1, s1 = cpu_baseline(c, n, bc, u0, dt, 1, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": v1, "unit": "DoF*stage/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (C, -O2), same workload, {args.cpu_steps} RK4 step(s) = "
+                      f"{5 * args.cpu_steps} stages, {s1:.1f} s",
+        }
+        vN, sN = cpu_baseline(c, n, bc, u0, dt, args.cpu_threads, args.cpu_steps)
+        out["cpu_baseline_all_cores"] = {
+            "value": vN, "unit": "DoF*stage/s", "cores": args.cpu_threads, "kind": "port",
+            "sample": f"same sample with {args.cpu_threads} OpenMP threads, {sN:.1f} s",
+        }
+        print(f"[bench] cpu: 1 thread {v1:.3e}, {args.cpu_threads} threads {vN:.3e}", file=sys.stderr, flush=True)
+
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,177 @@
+/*
+ * wavedg_hip.h -- C ABI of the MI355X (gfx950) implementation of the 2D wave
+ * DG operator and the WaveHoltz iteration of arotem3/waveDG.
+ *
+ * Plain pointers and sizes only.  Every function returns a status code
+ * (WDG_OK = 0); on failure wdg_hip_last_error() holds a message for the
+ * calling thread.  The C++ drop-in classes (wavedg_amd/include/wavedg_hip/dg.hpp)
+ * turn nonzero codes into std::runtime_error, as wdg_error does in the
+ * reference (source/wdg_error.cpp:5-20).
+ *
+ * Memory: arguments named d_* are device pointers (hipMalloc'd, on the
+ * handle's device); everything else is host memory.  `stream` is a
+ * hipStream_t (NULL = default stream); device work is enqueued on it and the
+ * call returns without synchronising, except functions that return a host
+ * value (norms, residuals), which synchronise the stream.
+ *
+ * Layouts (column-major, first index fastest, as in the reference,
+ * docs/docs.md:55-61):
+ *   state        (3, n, n, n_elem)  (p, u, v) per Gauss-Lobatto node
+ *   vertices     (2, n_vertices);   elements (4, n_elem), corners counter-clockwise
+ *   H            (2, n, n, n_elem)  real/imag part of the Helmholtz solution
+ *
+ * Precision: WDG_F64 runs everything in double.  WDG_F32 keeps the state, the
+ * right-hand side and the Runge-Kutta stages in float; WaveHoltz keeps its
+ * filter accumulation, fixed-point update and reductions in double (the
+ * WaveHoltz functions always take double arrays).
+ *
+ * Thread safety: a handle is used by one thread at a time (the reference's
+ * operators hold mutable scratch and are not re-entrant either,
+ * WaveEquation.hpp:61).  Calls on one handle must be serialised on one stream.
+ */
+#ifndef WAVEDG_HIP_H
+#define WAVEDG_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    WDG_OK = 0,
+    WDG_ERR_INVALID = 1,   /* bad argument or handle */
+    WDG_ERR_HIP = 2,       /* a HIP runtime call failed */
+    WDG_ERR_ALLOC = 3,     /* device allocation failed */
+    WDG_ERR_MESH = 4,      /* malformed or inverted mesh, missing file */
+    WDG_ERR_NONFINITE = 5  /* a residual or norm is NaN/Inf (e.g. dt above the CFL limit) */
+};
+
+enum { WDG_F64 = 0, WDG_F32 = 1 };
+enum { WDG_INTERIOR = 0, WDG_BOUNDARY = 1 };  /* FaceType, Mesh2D.hpp */
+enum { WDG_BC_ABSORB = 0, WDG_BC_REFLECT = 1, WDG_BC_DIRICHLET = 2 };
+
+typedef struct wdg_mesh_s *wdg_mesh;
+typedef struct wdg_wave2d_s *wdg_wave2d;
+typedef struct wdg_waveholtz_s *wdg_waveholtz;
+
+const char *wdg_hip_last_error(void);
+int wdg_hip_version(int *major, int *minor);
+
+/* ---- device memory helpers (plumbing for FFI callers without a HIP runtime) */
+int wdg_hip_malloc(void **d_ptr, size_t bytes);
+int wdg_hip_free(void *d_ptr);
+int wdg_hip_memcpy_htod(void *d_dst, const void *src, size_t bytes);
+int wdg_hip_memcpy_dtoh(void *dst, const void *d_src, size_t bytes);
+int wdg_hip_memset(void *d_ptr, int value, size_t bytes);
+int wdg_hip_synchronize(void *stream);
+
+/* ---- mesh (host topology) -------------------------------------------
+ * Replaces Mesh2D::from_vertices / uniform_rect / from_file
+ * (wavedg/Mesh2D.hpp:339-355; source/Mesh2D.cpp:5-163).  Edge numbering,
+ * interior/boundary lists and delta follow Mesh2D.cpp:5-92; the edge key is
+ * 64-bit. */
+int wdg_hip_mesh_from_vertices(int n_vertices, const double *x, int n_elem, const int *elements, wdg_mesh *out);
+int wdg_hip_mesh_uniform_rect(int nx, double ax, double bx, int ny, double ay, double by, wdg_mesh *out);
+int wdg_hip_mesh_from_file(const char *dir, wdg_mesh *out);
+int wdg_hip_mesh_destroy(wdg_mesh mesh);
+/* Mesh2D::n_elem, n_edges(type) (Mesh2D.hpp:230-252) */
+int wdg_hip_mesh_counts(wdg_mesh mesh, int *n_elem, int *n_interior_edges, int *n_boundary_edges);
+/* Mesh2D::edge(i, type) (Mesh2D.hpp:292-311): out is (5, n_edges) = el0, side0, el1, side1, delta */
+int wdg_hip_mesh_edges(wdg_mesh mesh, int face_type, int *out);
+/* edge_metrics(..).normals()/measures() at the edge midpoint and the midpoint
+ * itself (Mesh2D.cpp:702-749): out is (5, n_edges) = nx, ny, ds, xmid, ymid */
+int wdg_hip_mesh_edge_geometry(wdg_mesh mesh, int face_type, double *out);
+/* Mesh2D::min_h (Mesh2D.cpp:751-767) */
+int wdg_hip_mesh_min_h(wdg_mesh mesh, double *h);
+/* element_metrics(basis).physical_coordinates() (Mesh2D.cpp:687-697) on the
+ * n_colloc Gauss-Lobatto nodes: out is (2, n, n, n_elem) */
+int wdg_hip_mesh_nodes(wdg_mesh mesh, int n_colloc, double *out);
+
+/* ---- basis -----------------------------------------------------------
+ * QuadratureRule::quadrature_rule(n, GaussLobatto) (QuadratureRule.cpp:32-97,164-184)
+ * and lagrange_basis_deriv (lagrange_interpolation.cpp:33-50): x, w (n), D (n, n). */
+int wdg_hip_gll(int n, double *x, double *w, double *D);
+
+/* ---- wave operator ---------------------------------------------------
+ * One handle holds WaveEquation (WaveEquation.cpp:5-30), WaveBC
+ * (WaveEquation.cpp:70-105) and MassMatrix<true> (MassMatrix.cpp:6-28) on the
+ * current HIP device, for approx_quad = true and a Gauss-Lobatto basis of
+ * n_colloc points (2..9).  bc has one code per boundary edge (NULL = all
+ * reflecting).  (a, b) use the README/BASELINE convention: upwind a = 1,
+ * b = 1/2; central a = 1, b = 0 (the reference's EdgeFlux uses the negated
+ * pair, EdgeFlux.hpp:51).  The mesh may be destroyed after this call. */
+int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int *bc, double a, double b, int precision,
+                          wdg_wave2d *out);
+int wdg_hip_wave2d_destroy(wdg_wave2d op);
+/* ndof = 3 n^2 n_elem */
+int wdg_hip_wave2d_info(wdg_wave2d op, long long *ndof, int *n_colloc, int *n_elem, int *precision);
+/* WaveEquation::action (WaveEquation.cpp:54-68, Operator.hpp:19): d_y += A d_x.
+ * Volume divergence (Div<true>, Div.cpp:131-179), interior traces
+ * (lobatto_action_2d, FaceProlongator.cpp:565-590), upwind/central flux
+ * (flux_action_2d_c, EdgeFlux.cpp:289-325) and lift (lobatto_transpose_2d,
+ * FaceProlongator.cpp:650-672) in one kernel.  d_x and d_y must not overlap. */
+int wdg_hip_wave2d_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
+/* WaveBC::action (WaveEquation.cpp:190-215): d_y += B d_x (boundary flux and lift). */
+int wdg_hip_wavebc_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
+/* MassMatrix<true>::inv (MassMatrix.cpp:69-82): d_x <- M^{-1} d_x, in place. */
+int wdg_hip_mass_inv(wdg_wave2d op, void *d_x, void *stream);
+/* MassMatrix<true>::action (MassMatrix.cpp:52-67): d_y <- M d_x. */
+int wdg_hip_mass_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
+/* The time derivative of the examples and WaveHoltz (WaveHoltz.cpp:57-70):
+ * d_y <- M^{-1} (A d_x + B d_x + s d_F), fused.  d_F may be NULL; it has the
+ * handle's precision. */
+int wdg_hip_wave2d_rhs(wdg_wave2d op, const void *d_x, void *d_y, const void *d_F, double s, void *stream);
+/* n_steps of ode::RungeKutta4::step (ode.hpp:66-93) on u' = M^{-1}(A+B)u
+ * [- F sin(omega t)/omega if d_F != NULL and omega > 0, as in WaveHoltz::pi0],
+ * each stage one fused kernel.  *t is advanced by n_steps dt.  d_u is updated
+ * in place; d_F has the handle's precision. */
+int wdg_hip_rk4_steps(wdg_wave2d op, double dt, double *t, void *d_u, int n_steps, const void *d_F, double omega,
+                      void *stream);
+/* Same as wdg_hip_rk4_steps with d_F = NULL, captured once into a HIP graph
+ * of `steps_per_graph` steps and replayed (n_steps must be a multiple of it). */
+int wdg_hip_rk4_steps_graph(wdg_wave2d op, double dt, double *t, void *d_u, int n_steps, int steps_per_graph,
+                            void *stream);
+/* Average duration in milliseconds of the stage kernels launched by the last
+ * wdg_hip_rk4_steps call on this handle (HIP events around them on `stream`).
+ * Enabled by wdg_hip_wave2d_set_timing(op, 1). */
+int wdg_hip_wave2d_set_timing(wdg_wave2d op, int enable);
+int wdg_hip_wave2d_last_timing(wdg_wave2d op, double *ms_total, int *n_kernels);
+
+/* ---- WaveHoltz -------------------------------------------------------
+ * WaveHoltz(omega, ...) (WaveHoltz.cpp:5-30): T = 2 pi/omega,
+ * dt = 0.5 min_h / n^2, nt = ceil(T/dt), dt = T/nt. */
+int wdg_hip_waveholtz_create(wdg_wave2d op, double omega, wdg_waveholtz *out);
+int wdg_hip_waveholtz_destroy(wdg_waveholtz wh);
+int wdg_hip_waveholtz_info(wdg_waveholtz wh, double *T, double *dt, int *nt);
+/* WaveHoltz::pi0 (WaveHoltz.cpp:55-89): d_w <- Pi(0) for the load vector d_F. */
+int wdg_hip_waveholtz_pi0(wdg_waveholtz wh, double *d_w, const double *d_F, void *stream);
+/* WaveHoltz::S (WaveHoltz.cpp:91-121): d_w <- S d_w, in place. */
+int wdg_hip_waveholtz_S(wdg_waveholtz wh, double *d_w, void *stream);
+/* One fixed-point step of the caller loop (examples/waveholtz2d.cpp:165-185):
+ * d_w <- S d_w + d_pi0, *rel_change = ||w_new - w_old||_2 / pi0_norm
+ * (linalg.cpp:116-130), all in double on the device.  Synchronises `stream`. */
+int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double *d_w, const double *d_pi0, double pi0_norm,
+                              double *rel_change, void *stream);
+/* WaveHoltz::postprocess (WaveHoltz.cpp:123-145): d_H (2, n, n, n_elem). */
+int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double *d_H, const double *d_w, void *stream);
+
+/* ---- load vector -----------------------------------------------------
+ * LinearFunctional2D::action (LinearFunctional.hpp:136-202) with
+ * Gauss-Legendre(n_colloc) quadrature, for f = sum of Gaussians in variable
+ * `var` (0 = p): f(x) = sum_k amp_k exp(-|x - c_k|^2 / (2 sigma_k^2)).
+ * sources is (4, n_sources) host memory: cx, cy, sigma, amp.  d_F (double) is
+ * overwritten. */
+int wdg_hip_load_vector_gauss(wdg_wave2d op, int n_sources, const double *sources, int var, double *d_F,
+                              void *stream);
+
+/* ---- reductions (linalg.cpp:102-130), double, deterministic ---------- */
+int wdg_hip_norm(const double *d_x, long long n, double *out, void *stream);
+int wdg_hip_error(const double *d_x, const double *d_y, long long n, double *out, void *stream);
+/* Converts between the handle's precision and double (n values). */
+int wdg_hip_convert(const void *d_src, int src_precision, void *d_dst, int dst_precision, long long n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,287 @@
+"""ctypes wrapper around the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module, and only as the checker.  The product
+library (``wavedg_amd``) never loads it.
+
+Every array is a numpy float64/int32 array in the reference's column-major
+layout (see wdg_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "orc_gauss_lobatto": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+            "orc_gauss_legendre": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+            "orc_lagrange_basis": (None, [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp]),
+            "orc_lagrange_basis_deriv": (None, [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp]),
+            "orc_mesh_from_vertices": (_vp, [ctypes.c_int, _dp, ctypes.c_int, _ip]),
+            "orc_mesh_free": (None, [_vp]),
+            "orc_mesh_n_elem": (ctypes.c_int, [_vp]),
+            "orc_mesh_n_edges": (ctypes.c_int, [_vp, ctypes.c_int]),
+            "orc_mesh_edges": (None, [_vp, ctypes.c_int, _ip]),
+            "orc_mesh_edge_geometry": (None, [_vp, ctypes.c_int, _dp]),
+            "orc_mesh_min_h": (ctypes.c_double, [_vp]),
+            "orc_mesh_nodes": (None, [_vp, ctypes.c_int, _dp, _dp]),
+            "orc_mesh_detj": (None, [_vp, ctypes.c_int, _dp, _dp]),
+            "orc_wave_create": (_vp, [_vp, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double]),
+            "orc_wave_free": (None, [_vp]),
+            "orc_wave_ndof": (ctypes.c_longlong, [_vp]),
+            "orc_wave_action": (None, [_vp, _dp, _dp]),
+            "orc_wavebc_action": (None, [_vp, _dp, _dp]),
+            "orc_mass_action": (None, [_vp, ctypes.c_int, _dp, _dp]),
+            "orc_mass_inv": (None, [_vp, ctypes.c_int, _dp]),
+            "orc_rhs": (None, [_vp, _dp, _dp, _dp, ctypes.c_double]),
+            "orc_rk4_steps": (None, [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int, _dp, ctypes.c_double]),
+            "orc_set_threads": (None, [ctypes.c_int]),
+            "orc_waveholtz_create": (_vp, [_vp, ctypes.c_double]),
+            "orc_waveholtz_free": (None, [_vp]),
+            "orc_waveholtz_info": (None, [_vp, _dp, _dp, _ip]),
+            "orc_waveholtz_K": (ctypes.c_double, [_vp, ctypes.c_double]),
+            "orc_waveholtz_pi0": (None, [_vp, _dp, _dp]),
+            "orc_waveholtz_S": (None, [_vp, _dp]),
+            "orc_waveholtz_postprocess": (None, [_vp, _dp, _dp]),
+            "orc_load_vector_gauss": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]),
+            "orc_edge_flux": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
+            "orc_norm": (ctypes.c_double, [ctypes.c_longlong, _dp]),
+            "orc_error": (ctypes.c_double, [ctypes.c_longlong, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+# ---------------------------------------------------------------- basis
+def gauss_lobatto(n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().orc_gauss_lobatto(n, _d(x), _d(w)) != 0:
+        raise ValueError(f"gauss_lobatto: n={n} outside the tabulated range 2..9")
+    return x, w
+
+
+def gauss_legendre(n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().orc_gauss_legendre(n, _d(x), _d(w)) != 0:
+        raise ValueError(f"gauss_legendre: n={n} outside the tabulated range 1..10")
+    return x, w
+
+
+def lagrange_basis(xgrid, xeval):
+    xgrid = np.ascontiguousarray(xgrid, dtype=np.float64)
+    xeval = np.ascontiguousarray(xeval, dtype=np.float64)
+    B = np.zeros(len(xeval) * len(xgrid))
+    lib().orc_lagrange_basis(_d(B), len(xgrid), _d(xgrid), len(xeval), _d(xeval))
+    return B.reshape(len(xgrid), len(xeval)).T  # B[i, j] = phi_j(xeval_i)
+
+
+def lagrange_basis_deriv(xgrid, xeval):
+    xgrid = np.ascontiguousarray(xgrid, dtype=np.float64)
+    xeval = np.ascontiguousarray(xeval, dtype=np.float64)
+    D = np.zeros(len(xeval) * len(xgrid))
+    lib().orc_lagrange_basis_deriv(_d(D), len(xgrid), _d(xgrid), len(xeval), _d(xeval))
+    return D.reshape(len(xgrid), len(xeval)).T  # D[i, j] = phi_j'(xeval_i)
+
+
+# ---------------------------------------------------------------- mesh
+class Mesh:
+    """Oracle mesh from (2, nv) vertices and (4, nel) quads (column-major)."""
+
+    def __init__(self, vertices: np.ndarray, elements: np.ndarray):
+        v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 2))
+        e = np.ascontiguousarray(np.asarray(elements, dtype=np.int32).reshape(-1, 4))
+        self._v, self._e = v, e
+        self.h = lib().orc_mesh_from_vertices(v.shape[0], _d(v), e.shape[0], _i(e))
+        self.n_elem = lib().orc_mesh_n_elem(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+    def n_edges(self, type_: int) -> int:
+        return lib().orc_mesh_n_edges(self.h, type_)
+
+    def edges(self, type_: int) -> np.ndarray:
+        ne = self.n_edges(type_)
+        out = np.zeros(5 * ne, dtype=np.int32)
+        lib().orc_mesh_edges(self.h, type_, _i(out))
+        return out.reshape(ne, 5)
+
+    def edge_geometry(self, type_: int) -> np.ndarray:
+        ne = self.n_edges(type_)
+        out = np.zeros(5 * ne)
+        lib().orc_mesh_edge_geometry(self.h, type_, _d(out))
+        return out.reshape(ne, 5)
+
+    def min_h(self) -> float:
+        return lib().orc_mesh_min_h(self.h)
+
+    def nodes(self, xq) -> np.ndarray:
+        xq = np.ascontiguousarray(xq, dtype=np.float64)
+        nq = len(xq)
+        out = np.zeros(2 * nq * nq * self.n_elem)
+        lib().orc_mesh_nodes(self.h, nq, _d(xq), _d(out))
+        return out.reshape(self.n_elem, nq, nq, 2)  # [el, j, i, xy]
+
+    def detj(self, xq) -> np.ndarray:
+        xq = np.ascontiguousarray(xq, dtype=np.float64)
+        nq = len(xq)
+        out = np.zeros(nq * nq * self.n_elem)
+        lib().orc_mesh_detj(self.h, nq, _d(xq), _d(out))
+        return out.reshape(self.n_elem, nq, nq)
+
+    def load_vector_gauss(self, n: int, sources, var: int = 0, nq: int | None = None) -> np.ndarray:
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
+        F = np.zeros(3 * n * n * self.n_elem)
+        lib().orc_load_vector_gauss(self.h, n, nq or n, src.shape[0], _d(src), var, _d(F))
+        return F
+
+
+# ---------------------------------------------------------------- operator
+class Wave:
+    """Oracle WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on a mesh."""
+
+    def __init__(self, mesh: Mesh, n: int, bc=None, a: float = -1.0, b: float = -0.5):
+        self.mesh = mesh
+        self.n = n
+        nB = mesh.n_edges(1)
+        self._bc = np.ascontiguousarray(np.ones(nB, dtype=np.int32) if bc is None else np.asarray(bc, dtype=np.int32))
+        assert self._bc.shape == (nB,)
+        self.h = lib().orc_wave_create(mesh.h, n, _i(self._bc), a, b)
+        if not self.h:
+            raise ValueError("orc_wave_create failed")
+        self.ndof = lib().orc_wave_ndof(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_wave_free(self.h)
+            self.h = None
+
+    def action(self, u, du=None):
+        du = np.zeros(self.ndof) if du is None else du
+        lib().orc_wave_action(self.h, _d(np.ascontiguousarray(u)), _d(du))
+        return du
+
+    def bc_action(self, u, du=None):
+        du = np.zeros(self.ndof) if du is None else du
+        lib().orc_wavebc_action(self.h, _d(np.ascontiguousarray(u)), _d(du))
+        return du
+
+    def mass_action(self, x, nvar: int = 3):
+        y = np.zeros_like(x)
+        lib().orc_mass_action(self.h, nvar, _d(np.ascontiguousarray(x)), _d(y))
+        return y
+
+    def mass_inv(self, x, nvar: int = 3):
+        x = np.array(x, dtype=np.float64, copy=True)
+        lib().orc_mass_inv(self.h, nvar, _d(x))
+        return x
+
+    def rhs(self, u, F=None, s: float = 0.0):
+        vt = np.zeros(self.ndof)
+        Fp = _d(np.ascontiguousarray(F)) if F is not None else None
+        lib().orc_rhs(self.h, _d(np.ascontiguousarray(u)), _d(vt), Fp, s)
+        return vt
+
+    def rk4(self, u, dt: float, n_steps: int, t: float = 0.0, F=None, omega: float = 0.0):
+        u = np.array(u, dtype=np.float64, copy=True)
+        tt = ctypes.c_double(t)
+        Fp = _d(np.ascontiguousarray(F)) if F is not None else None
+        lib().orc_rk4_steps(self.h, dt, ctypes.byref(tt), _d(u), n_steps, Fp, omega)
+        return u, tt.value
+
+
+class WaveHoltz:
+    def __init__(self, wave: Wave, omega: float):
+        self.wave = wave
+        self.omega = omega
+        self.h = lib().orc_waveholtz_create(wave.h, omega)
+        T, dt, nt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        lib().orc_waveholtz_info(self.h, ctypes.byref(T), ctypes.byref(dt), ctypes.byref(nt))
+        self.T, self.dt, self.nt = T.value, dt.value, nt.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_waveholtz_free(self.h)
+            self.h = None
+
+    def K(self, t: float) -> float:
+        return lib().orc_waveholtz_K(self.h, t)
+
+    def pi0(self, F):
+        u = np.zeros(self.wave.ndof)
+        lib().orc_waveholtz_pi0(self.h, _d(u), _d(np.ascontiguousarray(F)))
+        return u
+
+    def S(self, w):
+        w = np.array(w, dtype=np.float64, copy=True)
+        lib().orc_waveholtz_S(self.h, _d(w))
+        return w
+
+    def postprocess(self, w):
+        H = np.zeros(2 * (self.wave.ndof // 3))
+        lib().orc_waveholtz_postprocess(self.h, _d(H), _d(np.ascontiguousarray(w)))
+        return H
+
+
+def edge_flux(n0: float, n1: float, a: float, b: float, side: int) -> np.ndarray:
+    """Fs[c, d] for the wave equation (reference sign convention)."""
+    Fs = np.zeros(9)
+    lib().orc_edge_flux(n0, n1, a, b, side, _d(Fs))
+    return Fs.reshape(3, 3).T
+
+
+def norm(x) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_norm(x.size, _d(x))
+
+
+def error(x, y) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_error(x.size, _d(x), _d(y))

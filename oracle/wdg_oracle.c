@@ -1,0 +1,997 @@
+/*
+ * wdg_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * Plain-C restatement of the reference's 2D wave DG path with approximate
+ * (collocated Gauss-Lobatto) quadrature.  Each routine names the reference
+ * file:line it follows.  It keeps the reference's data layouts and the order
+ * of its floating-point sums, so that its results can stand in for the
+ * reference (which cannot be built in this image: its headers include a
+ * cmake-generated wdg_config.hpp and it links LAPACK).
+ *
+ * Differences from the reference, all deliberate:
+ *   - absorbing BC computes p_ext after u_ext (the reference reads u_ext
+ *     before writing it, WaveEquation.hpp:77-84);
+ *   - BC code 2 (Dirichlet p = 0) is added;
+ *   - edge keys are 64-bit (the reference's `min + nv*max` in int overflows
+ *     above ~46k vertices, Mesh2D.cpp:34-38);
+ *   - the eigen-decomposition of n.A (LAPACK dgeevx_/dgels_ in
+ *     EdgeFlux.cpp:7-89) is done with a cyclic Jacobi rotation method, which is
+ *     valid because n.A is symmetric for the wave equation (checked at run
+ *     time);
+ *   - Legendre polynomials use the Bonnet recurrence (the reference calls
+ *     std::legendre); Gauss-Lobatto/Legendre node tables are those of
+ *     QuadratureRule.cpp:37-46,103-112 (n <= 9 / n <= 10 only).
+ */
+#include "wdg_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ===================================================================== */
+/* basis                                                                 */
+/* ===================================================================== */
+
+/* Bonnet recurrence for P_l(x). */
+static double legendre_p(int l, double x)
+{
+    if (l == 0) return 1.0;
+    double pm2 = 1.0, pm1 = x, p = x;
+    for (int k = 2; k <= l; ++k) {
+        p = ((2.0 * k - 1.0) * x * pm1 - (k - 1.0) * pm2) / k;
+        pm2 = pm1;
+        pm1 = p;
+    }
+    return p;
+}
+
+/* P_l'(x) for |x| < 1 via (x^2-1) P_l' = l (x P_l - P_{l-1}). */
+static double legendre_dp(int l, double x)
+{
+    return l * (x * legendre_p(l, x) - legendre_p(l - 1, x)) / (x * x - 1.0);
+}
+
+/* Node table: QuadratureRule.cpp:37-46 (values as printed there). */
+static const double GLL_NODES[10][9] = {
+    {0}, {0},
+    {-1, 1},
+    {-1, 0, 1},
+    {-1, -0.447213595499958, 0.447213595499958, 1},
+    {-1, -0.654653670707977, 0, 0.654653670707977, 1},
+    {-1, -0.765055323929465, -0.285231516480645, 0.285231516480645, 0.765055323929465, 1},
+    {-1, -0.830223896278567, -0.468848793470714, 0.0, 0.468848793470714, 0.830223896278567, 1},
+    {-1, -0.871740148509607, -0.591700181433142, -0.209299217902479, 0.2092992179024789, 0.591700181433142, 0.871740148509607, 1},
+    {-1, -0.899757995411460, -0.677186279510738, -0.363117463826178, 0, 0.363117463826178, 0.677186279510738, 0.899757995411460, 1}};
+
+/* QuadratureRule.cpp:32-97: nodes from the table, w_i = 2/(n(n-1) P_{n-1}(x_i)^2) (:94). */
+int orc_gauss_lobatto(int n, double *x, double *w)
+{
+    if (n < 2 || n > 9) return -1;
+    for (int i = 0; i < n; ++i) {
+        x[i] = GLL_NODES[n][i];
+        const double P = legendre_p(n - 1, x[i]);
+        w[i] = 2.0 / (n * (n - 1) * (P * P));
+    }
+    return 0;
+}
+
+/* Node table: QuadratureRule.cpp:103-112. */
+static const double GL_NODES[11][10] = {
+    {0},
+    {0.0},
+    {-0.577350269189625764509149, 0.577350269189625764509149},
+    {-0.774596669241483377035853, 0.0, 0.774596669241483377035853},
+    {-0.861136311594052575223946, -0.339981043584856264802666, 0.339981043584856264802666, 0.861136311594052575223946},
+    {-0.906179845938663992797627, -0.538469310105683091036314, 0.0, 0.538469310105683091036314, 0.906179845938663992797627},
+    {-0.932469514203152027812302, -0.661209386466264513661400, -0.238619186083196908630502, 0.238619186083196908630502, 0.661209386466264513661400, 0.932469514203152027812302},
+    {-0.949107912342758524526190, -0.741531185599394439863865, -0.405845151377397166906606, 0.0, 0.405845151377397166906606, 0.741531185599394439863865, 0.949107912342758524526190},
+    {-0.960289856497536231683561, -0.796666477413626739591554, -0.525532409916328985817739, -0.183434642495649804939476, 0.183434642495649804939476, 0.525532409916328985817739, 0.796666477413626739591554, 0.960289856497536231683561},
+    {-0.968160239507626089835576, -0.836031107326635794299430, -0.613371432700590397308702, -0.324253423403808929038538, 0.0, 0.324253423403808929038538, 0.613371432700590397308702, 0.836031107326635794299430, 0.968160239507626089835576},
+    {-0.973906528517171720077964, -0.865063366688984510732097, -0.679409568299024406234327, -0.433395394129247190799266, -0.148874338981631210884826, 0.148874338981631210884826, 0.433395394129247190799266, 0.679409568299024406234327, 0.865063366688984510732097, 0.973906528517171720077964}};
+
+/* QuadratureRule.cpp:99-162: w_i = 2 / ((1 - x_i^2) P_n'(x_i)^2). */
+int orc_gauss_legendre(int n, double *x, double *w)
+{
+    if (n < 1 || n > 10) return -1;
+    for (int i = 0; i < n; ++i) {
+        x[i] = GL_NODES[n][i];
+        const double dP = legendre_dp(n, x[i]);
+        w[i] = 2.0 / (1.0 - x[i] * x[i]) / (dP * dP);
+    }
+    return 0;
+}
+
+/* lagrange_interpolation.cpp:52-72 */
+static void barycentric_weights(double *w, const double *x, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        w[i] = 1.0;
+        for (int j = 0; j < n; ++j)
+            if (i != j) w[i] *= x[i] - x[j];
+        w[i] = 1.0 / w[i];
+    }
+    double wmin = w[0], wmax = w[0];
+    for (int i = 1; i < n; ++i) {
+        if (w[i] < wmin) wmin = w[i];
+        if (w[i] > wmax) wmax = w[i];
+    }
+    const double diff = wmax - wmin;
+    for (int i = 0; i < n; ++i) w[i] /= diff;
+}
+
+/* lagrange_interpolation.cpp:74-94 */
+static double lagrange_interp(double t, const double *x, const double *w, const double *y, int n)
+{
+    double A = 0.0, B = 0.0;
+    const double eps = 2.220446049250313e-16;
+    for (int i = 0; i < n; ++i) {
+        const double xdiff = t - x[i];
+        if (t == x[i] || fabs(xdiff) <= eps) return y[i];
+        const double C = w[i] / xdiff;
+        A += C * y[i];
+        B += C;
+    }
+    return A / B;
+}
+
+/* lagrange_interpolation.cpp:96-139 */
+static double lagrange_deriv(double x0, const double *x, const double *w, const double *y, int n)
+{
+    int atnode = 0, i = 0;
+    double A = 0.0, B = 0.0;
+    const double p = lagrange_interp(x0, x, w, y, n);
+    const double eps = 2.220446049250313e-16;
+    for (int j = 0; j < n; ++j) {
+        if (x0 == x[j] || fabs(x0 - x[j]) <= eps) {
+            atnode = 1;
+            B = -w[j];
+            i = j;
+        }
+    }
+    if (atnode) {
+        for (int j = 0; j < n; ++j)
+            if (j != i) A += w[j] * (p - y[j]) / (x0 - x[j]);
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const double t = w[j] / (x0 - x[j]);
+            A += t * (p - y[j]) / (x0 - x[j]);
+            B += t;
+        }
+    }
+    return A / B;
+}
+
+/* lagrange_interpolation.cpp:14-31: B(i, j) = phi_j(xeval_i), shape (neval, ngrid). */
+void orc_lagrange_basis(double *B, int ngrid, const double *xgrid, int neval, const double *xeval)
+{
+    double w[64], y[64];
+    barycentric_weights(w, xgrid, ngrid);
+    for (int j = 0; j < ngrid; ++j) y[j] = 0.0;
+    for (int j = 0; j < ngrid; ++j) {
+        y[j] = 1.0;
+        for (int i = 0; i < neval; ++i) B[i + neval * j] = lagrange_interp(xeval[i], xgrid, w, y, ngrid);
+        y[j] = 0.0;
+    }
+}
+
+/* lagrange_interpolation.cpp:33-50: D(i, j) = phi_j'(xeval_i), shape (neval, ngrid). */
+void orc_lagrange_basis_deriv(double *D, int ngrid, const double *xgrid, int neval, const double *xeval)
+{
+    double w[64], y[64];
+    barycentric_weights(w, xgrid, ngrid);
+    for (int j = 0; j < ngrid; ++j) y[j] = 0.0;
+    for (int j = 0; j < ngrid; ++j) {
+        y[j] = 1.0;
+        for (int i = 0; i < neval; ++i) D[i + neval * j] = lagrange_deriv(xeval[i], xgrid, w, y, ngrid);
+        y[j] = 0.0;
+    }
+}
+
+/* ===================================================================== */
+/* mesh                                                                  */
+/* ===================================================================== */
+
+typedef struct {
+    int el[2], side[2], delta, type; /* type: 0 interior, 1 boundary */
+    double n[2], meas, x0[2], dx[2];
+} orc_edge;
+
+struct orc_mesh {
+    int nv, nel, ne, nI, nB;
+    double *corners; /* (2, 4, nel) */
+    orc_edge *edges;
+    int *interior, *boundary;
+};
+
+/* Mesh2D.cpp:10-11 */
+static const int EMAP1[4] = {0, 1, 3, 0};
+static const int EMAP2[4] = {1, 2, 2, 3};
+
+/* Edge.hpp:107-122 (StraightEdge constructor). */
+static void straight_edge(orc_edge *E, const double *x0, const double *x1, int side)
+{
+    E->x0[0] = x0[0];
+    E->x0[1] = x0[1];
+    E->dx[0] = x1[0] - x0[0];
+    E->dx[1] = x1[1] - x0[1];
+    const double s = hypot(E->dx[0], E->dx[1]);
+    const double sgn = (side == 2 || side == 3) ? -1.0 : 1.0;
+    E->n[0] = sgn * E->dx[1] / s;
+    E->n[1] = -sgn * E->dx[0] / s;
+    E->meas = s / 2;
+}
+
+/* Mesh2D.cpp:5-92 (from_vertices), with a 64-bit open-addressing edge map. */
+orc_mesh *orc_mesh_from_vertices(int nv, const double *x, int nel, const int *elems)
+{
+    orc_mesh *m = (orc_mesh *)calloc(1, sizeof(orc_mesh));
+    m->nv = nv;
+    m->nel = nel;
+    m->corners = (double *)malloc(sizeof(double) * 8 * (size_t)nel);
+    for (int el = 0; el < nel; ++el)
+        for (int i = 0; i < 4; ++i) {
+            const int c = elems[i + 4 * el];
+            m->corners[0 + 2 * (i + 4 * el)] = x[2 * c];
+            m->corners[1 + 2 * (i + 4 * el)] = x[2 * c + 1];
+        }
+
+    size_t cap = 16;
+    while (cap < (size_t)8 * nel + 16) cap <<= 1;
+    int64_t *keys = (int64_t *)malloc(sizeof(int64_t) * cap);
+    int *vals = (int *)malloc(sizeof(int) * cap);
+    for (size_t i = 0; i < cap; ++i) keys[i] = -1;
+
+    m->edges = (orc_edge *)calloc((size_t)4 * nel + 1, sizeof(orc_edge));
+    int ne = 0;
+    for (int el = 0; el < nel; ++el) {
+        for (int s = 0; s < 4; ++s) {
+            const int C0 = elems[EMAP1[s] + 4 * el];
+            const int C1 = elems[EMAP2[s] + 4 * el];
+            const int64_t lo = C0 < C1 ? C0 : C1, hi = C0 < C1 ? C1 : C0;
+            const int64_t k = lo + (int64_t)nv * hi;
+            size_t h = (size_t)((uint64_t)k * 0x9E3779B97F4A7C15ull) & (cap - 1);
+            while (keys[h] != -1 && keys[h] != k) h = (h + 1) & (cap - 1);
+            if (keys[h] == -1) {
+                keys[h] = k;
+                vals[h] = ne;
+                orc_edge *E = &m->edges[ne++];
+                straight_edge(E, x + 2 * C0, x + 2 * C1, s);
+                E->el[0] = el;
+                E->side[0] = s;
+                E->el[1] = -1;
+                E->side[1] = -1;
+                E->type = 1;
+                E->delta = 1;
+            } else {
+                orc_edge *E = &m->edges[vals[h]];
+                const int e0 = E->el[0], s0 = E->side[0];
+                const int n1 = elems[EMAP1[s0] + 4 * e0];
+                E->el[1] = el;
+                E->side[1] = s;
+                E->type = 0;
+                E->delta = (C0 == n1) ? 1 : -1;
+            }
+        }
+    }
+    free(keys);
+    free(vals);
+    m->ne = ne;
+    m->interior = (int *)malloc(sizeof(int) * (ne + 1));
+    m->boundary = (int *)malloc(sizeof(int) * (ne + 1));
+    for (int e = 0; e < ne; ++e) {
+        if (m->edges[e].type == 1) m->boundary[m->nB++] = e;
+        else m->interior[m->nI++] = e;
+    }
+    return m;
+}
+
+void orc_mesh_free(orc_mesh *m)
+{
+    if (!m) return;
+    free(m->corners);
+    free(m->edges);
+    free(m->interior);
+    free(m->boundary);
+    free(m);
+}
+
+int orc_mesh_n_elem(const orc_mesh *m) { return m->nel; }
+int orc_mesh_n_edges(const orc_mesh *m, int type) { return type == 0 ? m->nI : m->nB; }
+
+static const orc_edge *edge_of(const orc_mesh *m, int i, int type)
+{
+    return &m->edges[type == 0 ? m->interior[i] : m->boundary[i]];
+}
+
+void orc_mesh_edges(const orc_mesh *m, int type, int *out)
+{
+    const int ne = orc_mesh_n_edges(m, type);
+    for (int i = 0; i < ne; ++i) {
+        const orc_edge *E = edge_of(m, i, type);
+        out[5 * i + 0] = E->el[0];
+        out[5 * i + 1] = E->side[0];
+        out[5 * i + 2] = E->el[1];
+        out[5 * i + 3] = E->side[1];
+        out[5 * i + 4] = E->delta;
+    }
+}
+
+void orc_mesh_edge_geometry(const orc_mesh *m, int type, double *out)
+{
+    const int ne = orc_mesh_n_edges(m, type);
+    for (int i = 0; i < ne; ++i) {
+        const orc_edge *E = edge_of(m, i, type);
+        out[5 * i + 0] = E->n[0];
+        out[5 * i + 1] = E->n[1];
+        out[5 * i + 2] = E->meas;
+        out[5 * i + 3] = E->x0[0] + E->dx[0] * 0.5; /* Edge.hpp:142-148 at xi = 0 */
+        out[5 * i + 4] = E->x0[1] + E->dx[1] * 0.5;
+    }
+}
+
+/* Mesh2D.cpp:751-767 */
+double orc_mesh_min_h(const orc_mesh *m)
+{
+    double h = INFINITY;
+    for (int e = 0; e < m->ne; ++e) {
+        const double len = 2.0 * m->edges[e].meas;
+        if (len < h) h = len;
+    }
+    return h;
+}
+
+/* Element.cpp:21-27 */
+static void quad_jacobian(const double *c, double xi, double eta, double *J)
+{
+    /* c = (2, 4) corners: c[0 + 2*k], c[1 + 2*k] */
+    J[0] = 0.25 * ((1.0 - eta) * (c[2] - c[0]) + (1.0 + eta) * (c[4] - c[6]));
+    J[1] = 0.25 * ((1.0 - eta) * (c[3] - c[1]) + (1.0 + eta) * (c[5] - c[7]));
+    J[2] = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
+    J[3] = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
+}
+
+/* Element.cpp:5-19 */
+static void quad_coords(const double *c, double xi, double eta, double *X)
+{
+    const double b[4] = {0.25 * (1.0 - xi) * (1.0 - eta), 0.25 * (1.0 + xi) * (1.0 - eta),
+                         0.25 * (1.0 + xi) * (1.0 + eta), 0.25 * (1.0 - xi) * (1.0 + eta)};
+    X[0] = 0.0;
+    X[1] = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        X[0] += c[2 * i] * b[i];
+        X[1] += c[2 * i + 1] * b[i];
+    }
+}
+
+/* Mesh2D.cpp:643-700 (set_element_metric + physical_coordinates) */
+void orc_mesh_nodes(const orc_mesh *m, int nq, const double *xq, double *out)
+{
+    for (int el = 0; el < m->nel; ++el)
+        for (int j = 0; j < nq; ++j)
+            for (int i = 0; i < nq; ++i)
+                quad_coords(m->corners + 8 * (size_t)el, xq[i], xq[j], out + 2 * (i + nq * (j + (size_t)nq * el)));
+}
+
+/* Element.hpp:56-61 (measure = det J) */
+void orc_mesh_detj(const orc_mesh *m, int nq, const double *xq, double *out)
+{
+    for (int el = 0; el < m->nel; ++el)
+        for (int j = 0; j < nq; ++j)
+            for (int i = 0; i < nq; ++i) {
+                double J[4];
+                quad_jacobian(m->corners + 8 * (size_t)el, xq[i], xq[j], J);
+                out[i + nq * (j + (size_t)nq * el)] = J[0] * J[3] - J[1] * J[2];
+            }
+}
+
+/* ===================================================================== */
+/* wave operator                                                         */
+/* ===================================================================== */
+
+/* Wave equation flux tensor A(c, d, k), WaveEquation.cpp:10-18. */
+static const double WAVE_A[18] = {0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                                  0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+
+struct orc_wave {
+    const orc_mesh *m;
+    int n;
+    double x[16], w[16], D[256];
+    int *bc;
+    double a, b;
+    int *v2eI, *v2eB; /* (n, 2, ne) */
+    double *FsI, *FsB; /* per edge and side: (3, 3, 2, ne) = Fs(c, d) */
+    double *uI, *uB;   /* face vectors (n, 3, 2, ne), FEMVector.cpp:81-87 */
+};
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (column-major):
+ * A = R diag(e) R^T.  Stands in for real_eig (eig.cpp:29-74). */
+static void sym_eig3(const double *A0, double *R, double *e)
+{
+    double A[9];
+    memcpy(A, A0, sizeof(A));
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        const double off = A[3] * A[3] + A[6] * A[6] + A[7] * A[7];
+        if (off < 1e-300) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = A[p + 3 * q];
+                if (fabs(apq) < 1e-300) continue;
+                const double app = A[p + 3 * p], aqq = A[q + 3 * q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) { /* A <- A G */
+                    const double akp = A[k + 3 * p], akq = A[k + 3 * q];
+                    A[k + 3 * p] = c * akp - s * akq;
+                    A[k + 3 * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) { /* A <- G^T A */
+                    const double apk = A[p + 3 * k], aqk = A[q + 3 * k];
+                    A[p + 3 * k] = c * apk - s * aqk;
+                    A[q + 3 * k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) { /* R <- R G */
+                    const double rkp = R[k + 3 * p], rkq = R[k + 3 * q];
+                    R[k + 3 * p] = c * rkp - s * rkq;
+                    R[k + 3 * q] = s * rkp + c * rkq;
+                }
+            }
+    }
+    for (int i = 0; i < 3; ++i) e[i] = A[i + 3 * i];
+}
+
+/* EdgeFlux.cpp:31-89 (_ComputeFlux::flux): Fs = (R diag(a/2 l + sgn b |l|) R^{-1})^T.
+ * nA is symmetric here, so R^{-1} = R^T and Fs(c, d) = sum_k R(c,k) coef_k R(d,k). */
+static void compute_flux(const double *nA, double a, double b, int side, double *Fs)
+{
+    const double sgn = (side == 0) ? 1.0 : -1.0;
+    if (b == 0.0) {
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) Fs[i + 3 * j] = 0.5 * a * nA[j + 3 * i];
+        return;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (nA[i + 3 * j] != nA[j + 3 * i]) abort(); /* symmetric n.A required */
+    double R[9], e[3];
+    sym_eig3(nA, R, e);
+    double coef[3];
+    for (int k = 0; k < 3; ++k) coef[k] = 0.5 * a * e[k] + sgn * b * fabs(e[k]);
+    for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += R[c + 3 * k] * coef[k] * R[d + 3 * k];
+            Fs[c + 3 * d] = s;
+        }
+}
+
+void orc_edge_flux(double n0, double n1, double a, double b, int side, double *Fs)
+{
+    double nA[9];
+    for (int d = 0; d < 9; ++d) nA[d] = n0 * WAVE_A[d] + n1 * WAVE_A[9 + d];
+    compute_flux(nA, a, b, side, Fs);
+}
+
+/* FaceProlongator.cpp:508-514 (mapV2E) */
+static int map_v2e(int nc, int i, int f, int el)
+{
+    const int mm = (f == 0 || f == 2) ? i : (f == 1) ? (nc - 1) : 0;
+    const int nn = (f == 1 || f == 3) ? i : (f == 2) ? (nc - 1) : 0;
+    return mm + nc * (nn + nc * el);
+}
+
+/* FaceProlongator.cpp:497-539 (serial Lobatto constructor) */
+static int *make_v2e(const orc_mesh *m, int n, int type)
+{
+    const int ne = orc_mesh_n_edges(m, type);
+    int *v2e = (int *)malloc(sizeof(int) * ((size_t)2 * n * ne + 1));
+    for (size_t k = 0; k < (size_t)2 * n * ne; ++k) v2e[k] = -1;
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(m, e, type);
+        for (int i = 0; i < n; ++i) v2e[i + n * (0 + 2 * (size_t)e)] = map_v2e(n, i, E->side[0], E->el[0]);
+        if (type == 0)
+            for (int i = 0; i < n; ++i) {
+                const int j = (E->delta > 0) ? i : (n - 1 - i);
+                v2e[i + n * (1 + 2 * (size_t)e)] = map_v2e(n, j, E->side[1], E->el[1]);
+            }
+    }
+    return v2e;
+}
+
+/* EdgeFlux.cpp:91-139 (setup_flux_2d_cc_c), one Fs per edge and side: straight
+ * edges have a constant normal, so every node of an edge gets the same Fs. The
+ * node weight w_i ds_i is applied in flux_apply exactly as the stored table
+ * F(i,c,d,s,e) = (W(i) ds(i,e)) Fs(c,d) would be. */
+static double *make_fs(const orc_mesh *m, int type, double a, double b)
+{
+    const int ne = orc_mesh_n_edges(m, type);
+    double *Fs = (double *)malloc(sizeof(double) * ((size_t)18 * ne + 1));
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(m, e, type);
+        double nA[9];
+        for (int d = 0; d < 9; ++d) nA[d] = E->n[0] * WAVE_A[d] + E->n[1] * WAVE_A[9 + d];
+        for (int s = 0; s < 2; ++s) compute_flux(nA, a, b, s, Fs + 9 * (s + 2 * (size_t)e));
+    }
+    return Fs;
+}
+
+orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, double b)
+{
+    orc_wave *W = (orc_wave *)calloc(1, sizeof(orc_wave));
+    W->m = m;
+    W->n = n;
+    W->a = a;
+    W->b = b;
+    if (orc_gauss_lobatto(n, W->x, W->w) != 0) {
+        free(W);
+        return NULL;
+    }
+    orc_lagrange_basis_deriv(W->D, n, W->x, n, W->x);
+    W->bc = (int *)malloc(sizeof(int) * (m->nB + 1));
+    for (int i = 0; i < m->nB; ++i) W->bc[i] = bc ? bc[i] : 1;
+    W->v2eI = make_v2e(m, n, 0);
+    W->v2eB = make_v2e(m, n, 1);
+    W->FsI = make_fs(m, 0, a, b);
+    W->FsB = make_fs(m, 1, a, b);
+    W->uI = (double *)malloc(sizeof(double) * ((size_t)6 * n * m->nI + 1));
+    W->uB = (double *)malloc(sizeof(double) * ((size_t)6 * n * m->nB + 1));
+    return W;
+}
+
+void orc_wave_free(orc_wave *W)
+{
+    if (!W) return;
+    free(W->bc);
+    free(W->v2eI);
+    free(W->v2eB);
+    free(W->FsI);
+    free(W->FsB);
+    free(W->uI);
+    free(W->uB);
+    free(W);
+}
+
+long long orc_wave_ndof(const orc_wave *W) { return 3LL * W->n * W->n * W->m->nel; }
+
+/* Div.cpp:5-53 (op table, built here per element on the fly) and
+ * Div.cpp:131-179 (div_2d_c): du(d,i,j) += sum_l D(l,i) F(l,d,j) + D(l,j) G(l,d,i). */
+static void div_2d(const orc_wave *W, const double *u, double *du)
+{
+    const int n = W->n, c2d = n * n, nel = W->m->nel;
+    const double *D = W->D, *w = W->w, *x = W->x;
+#pragma omp parallel
+    {
+        double Fq[2 * 16 * 3 * 16];
+        double op[2 * 3 * 3];
+#pragma omp for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const double *c = W->m->corners + 8 * (size_t)el;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const int node = i + n * j;
+                    double J[4];
+                    quad_jacobian(c, x[i], x[j], J);
+                    const double Wt = w[node / n] * w[node % n];
+                    const double Y_eta = J[3] * Wt, X_eta = J[2] * Wt;
+                    const double Y_xi = J[1] * Wt, X_xi = J[0] * Wt;
+                    for (int d = 0; d < 3; ++d)
+                        for (int cc = 0; cc < 3; ++cc) {
+                            const double a0 = WAVE_A[cc + 3 * d], a1 = WAVE_A[9 + cc + 3 * d];
+                            op[0 + 2 * (d + 3 * cc)] = Y_eta * a0 - X_eta * a1;
+                            op[1 + 2 * (d + 3 * cc)] = -Y_xi * a0 + X_xi * a1;
+                        }
+                    const double *uij = u + 3 * (node + (size_t)c2d * el);
+                    for (int d = 0; d < 3; ++d) {
+                        double Fd = 0.0, Gd = 0.0;
+                        for (int cc = 0; cc < 3; ++cc) {
+                            Fd += op[0 + 2 * (cc + 3 * d)] * uij[cc];
+                            Gd += op[1 + 2 * (cc + 3 * d)] * uij[cc];
+                        }
+                        Fq[0 + 2 * (i + n * (d + 3 * j))] = Fd;
+                        Fq[1 + 2 * (j + n * (d + 3 * i))] = Gd;
+                    }
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int d = 0; d < 3; ++d) {
+                        double divF = 0.0;
+                        for (int l = 0; l < n; ++l)
+                            divF += D[l + n * i] * Fq[0 + 2 * (l + n * (d + 3 * j))] +
+                                    D[l + n * j] * Fq[1 + 2 * (l + n * (d + 3 * i))];
+                        du[d + 3 * (i + n * (j + (size_t)n * el))] += divF;
+                    }
+        }
+    }
+}
+
+/* FaceProlongator.cpp:565-590 (lobatto_action_2d) */
+static void prolong(const orc_wave *W, int type, const double *u, double *uf)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    memset(uf, 0, sizeof(double) * (size_t)6 * n * ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < 3; ++d) uf[i + n * (d + 3 * (side + 2 * (size_t)e))] = u[d + 3 * (size_t)j];
+            }
+}
+
+/* EdgeFlux.cpp:289-325 (flux_action_2d_c), in place. */
+static void flux_apply(const orc_wave *W, int type, double *uf)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const double *Fs = type == 0 ? W->FsI : W->FsB;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        const double ds = edge_of(W->m, e, type)->meas;
+        double acc[16 * 3];
+        for (int k = 0; k < 3 * n; ++k) acc[k] = 0.0;
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < 3; ++d)
+                for (int c = 0; c < 3; ++c)
+                    for (int i = 0; i < n; ++i) {
+                        const double F = (W->w[i] * ds) * Fs[c + 3 * d + 9 * (s + 2 * (size_t)e)];
+                        acc[i + n * d] += F * uf[i + n * (c + 3 * (s + 2 * (size_t)e))];
+                    }
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i) {
+                uf[i + n * (d + 3 * (0 + 2 * (size_t)e))] = acc[i + n * d];
+                uf[i + n * (d + 3 * (1 + 2 * (size_t)e))] = -acc[i + n * d];
+            }
+    }
+}
+
+/* FaceProlongator.cpp:650-672 (lobatto_transpose_2d); serial: corner nodes
+ * receive contributions from two edges, in edge order. */
+static void lift(const orc_wave *W, int type, const double *uf, double *du)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    du[d + 3 * (size_t)j] += uf[i + n * (d + 3 * (side + 2 * (size_t)e))];
+                }
+}
+
+/* WaveEquation.cpp:54-68 */
+void orc_wave_action(const orc_wave *W, const double *u, double *du)
+{
+    div_2d(W, u, du);
+    prolong(W, 0, u, W->uI);
+    flux_apply(W, 0, W->uI);
+    lift(W, 0, W->uI, du);
+}
+
+/* WaveEquation.hpp:64-69 */
+static void reflect_2d(double *v_ext, const double *v_int, const double *n)
+{
+    v_ext[0] = v_int[0];
+    v_ext[1] = (n[1] * n[1] - n[0] * n[0]) * v_int[1] - 2.0 * n[0] * n[1] * v_int[2];
+    v_ext[2] = -2.0 * n[0] * n[1] * v_int[1] + (n[0] * n[0] - n[1] * n[1]) * v_int[2];
+}
+
+/* WaveEquation.hpp:77-84, with p_ext evaluated after u_ext. */
+static void absorb_2d(double *v_ext, const double *v_int, const double *n)
+{
+    const double outgoing = v_int[0] + n[0] * v_int[1] + n[1] * v_int[2];
+    const double tangential_velocity = n[1] * v_int[1] - n[0] * v_int[2];
+    v_ext[1] = n[1] * tangential_velocity + 0.5 * n[0] * outgoing;
+    v_ext[2] = -n[0] * tangential_velocity + 0.5 * n[1] * outgoing;
+    v_ext[0] = n[0] * v_ext[1] + n[1] * v_ext[2];
+}
+
+/* Dirichlet p = 0 by the mirror state (no reference counterpart). */
+static void dirichlet_2d(double *v_ext, const double *v_int, const double *n)
+{
+    (void)n;
+    v_ext[0] = -v_int[0];
+    v_ext[1] = v_int[1];
+    v_ext[2] = v_int[2];
+}
+
+/* WaveEquation.cpp:144-167 (apply_bc_2d) and :190-215 (WaveBC::action) */
+void orc_wavebc_action(const orc_wave *W, const double *u, double *du)
+{
+    const int n = W->n, nB = W->m->nB;
+    double *uB = W->uB;
+    prolong(W, 1, u, uB);
+    for (int e = 0; e < nB; ++e) {
+        const double *nrm = edge_of(W->m, e, 1)->n;
+        void (*ext)(double *, const double *, const double *) =
+            W->bc[e] == 0 ? absorb_2d : (W->bc[e] == 2 ? dirichlet_2d : reflect_2d);
+        for (int i = 0; i < n; ++i) {
+            double v_int[3], v_ext[3];
+            for (int d = 0; d < 3; ++d) v_int[d] = uB[i + n * (d + 3 * (0 + 2 * (size_t)e))];
+            ext(v_ext, v_int, nrm);
+            for (int d = 0; d < 3; ++d) uB[i + n * (d + 3 * (1 + 2 * (size_t)e))] = v_ext[d];
+        }
+    }
+    flux_apply(W, 1, uB);
+    lift(W, 1, uB, du);
+}
+
+/* MassMatrix.cpp:6-28 (m = w_i w_j detJ) and :52-67 (action) */
+void orc_mass_action(const orc_wave *W, int nvar, const double *xv, double *y)
+{
+    const int n = W->n, nel = W->m->nel;
+#pragma omp parallel for schedule(static)
+    for (int el = 0; el < nel; ++el)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double J[4];
+                quad_jacobian(W->m->corners + 8 * (size_t)el, W->x[i], W->x[j], J);
+                const double detJ = J[0] * J[3] - J[1] * J[2];
+                const double mm = W->w[i] * W->w[j] * detJ;
+                const size_t node = i + n * (j + (size_t)n * el);
+                for (int d = 0; d < nvar; ++d) y[d + nvar * node] = mm * xv[d + nvar * node];
+            }
+}
+
+/* MassMatrix.cpp:69-82 (inv: x /= m) */
+void orc_mass_inv(const orc_wave *W, int nvar, double *xv)
+{
+    const int n = W->n, nel = W->m->nel;
+#pragma omp parallel for schedule(static)
+    for (int el = 0; el < nel; ++el)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double J[4];
+                quad_jacobian(W->m->corners + 8 * (size_t)el, W->x[i], W->x[j], J);
+                const double detJ = J[0] * J[3] - J[1] * J[2];
+                const double mm = W->w[i] * W->w[j] * detJ;
+                const size_t node = i + n * (j + (size_t)n * el);
+                for (int d = 0; d < nvar; ++d) xv[d + nvar * node] /= mm;
+            }
+}
+
+/* WaveHoltz.cpp:57-70 (time_derivative lambda) */
+void orc_rhs(const orc_wave *W, const double *u, double *vt, const double *F, double s)
+{
+    const long long nd = orc_wave_ndof(W);
+    memset(vt, 0, sizeof(double) * (size_t)nd);
+    orc_wave_action(W, u, vt);
+    orc_wavebc_action(W, u, vt);
+    if (F)
+        for (long long i = 0; i < nd; ++i) vt[i] += F[i] * s;
+    orc_mass_inv(W, 3, vt);
+}
+
+/* ode.hpp:66-93 (RungeKutta4::step, Carpenter-Kennedy (5,4) 2N-storage) */
+static const double RK4A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double RK4B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+static const double RK4C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                               2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+typedef struct {
+    double *p, *du;
+} rk_work;
+
+static void rk_step(const orc_wave *W, rk_work *wk, double dt, double *t, double *u, const double *F,
+                    double omega)
+{
+    const long long nd = orc_wave_ndof(W);
+    double *p = wk->p, *du = wk->du;
+    for (int stage = 0; stage < 5; ++stage) {
+        const double ts = (stage == 0) ? *t : *t + RK4C[stage] * dt;
+        const double s = (F && omega > 0) ? -1.0 / omega * sin(omega * ts) : 0.0;
+        orc_rhs(W, u, p, (F && omega > 0) ? F : NULL, s);
+        if (stage == 0) {
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < nd; ++i) {
+                du[i] = dt * p[i];
+                u[i] += RK4B[0] * du[i];
+            }
+        } else {
+            const double a = RK4A[stage], b = RK4B[stage];
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < nd; ++i) {
+                du[i] = a * du[i] + dt * p[i];
+                u[i] += b * du[i];
+            }
+        }
+    }
+    *t += dt;
+}
+
+void orc_rk4_steps(const orc_wave *W, double dt, double *t, double *u, int n_steps, const double *F,
+                   double omega)
+{
+    const long long nd = orc_wave_ndof(W);
+    rk_work wk;
+    wk.p = (double *)malloc(sizeof(double) * (size_t)nd);
+    wk.du = (double *)malloc(sizeof(double) * (size_t)nd);
+    for (int k = 0; k < n_steps; ++k) rk_step(W, &wk, dt, t, u, F, omega);
+    free(wk.p);
+    free(wk.du);
+}
+
+/* ===================================================================== */
+/* WaveHoltz                                                             */
+/* ===================================================================== */
+
+struct orc_wh {
+    const orc_wave *W;
+    double omega, T, dt;
+    int nt;
+};
+
+/* WaveHoltz.cpp:5-30 */
+orc_wh *orc_waveholtz_create(const orc_wave *W, double omega)
+{
+    orc_wh *H = (orc_wh *)calloc(1, sizeof(orc_wh));
+    H->W = W;
+    H->omega = omega;
+    H->T = 2.0 * M_PI / omega;
+    const double h = orc_mesh_min_h(W->m);
+    const double p = W->n;
+    H->dt = 0.5 * h / (p * p);
+    H->nt = (int)ceil(H->T / H->dt);
+    H->dt = H->T / H->nt;
+    return H;
+}
+
+void orc_waveholtz_free(orc_wh *H) { free(H); }
+
+void orc_waveholtz_info(const orc_wh *H, double *T, double *dt, int *nt)
+{
+    *T = H->T;
+    *dt = H->dt;
+    *nt = H->nt;
+}
+
+/* WaveHoltz.hpp:48-51 */
+double orc_waveholtz_K(const orc_wh *H, double t) { return H->omega / M_PI * (cos(H->omega * t) - 0.25); }
+
+/* WaveHoltz.cpp:55-89 */
+void orc_waveholtz_pi0(const orc_wh *H, double *u, const double *F)
+{
+    const long long nd = orc_wave_ndof(H->W);
+    double *p = (double *)calloc((size_t)nd, sizeof(double));
+    rk_work wk;
+    wk.p = (double *)malloc(sizeof(double) * (size_t)nd);
+    wk.du = (double *)malloc(sizeof(double) * (size_t)nd);
+    for (long long i = 0; i < nd; ++i) u[i] = 0.0;
+    double t = 0.0;
+    for (int it = 1; it <= H->nt; ++it) {
+        rk_step(H->W, &wk, H->dt, &t, p, F, H->omega);
+        const double c = ((it == H->nt) ? 0.5 : 1.0) * H->dt * orc_waveholtz_K(H, t);
+        for (long long i = 0; i < nd; ++i) u[i] += c * p[i];
+    }
+    free(p);
+    free(wk.p);
+    free(wk.du);
+}
+
+/* WaveHoltz.cpp:91-121 */
+void orc_waveholtz_S(const orc_wh *H, double *u)
+{
+    const long long nd = orc_wave_ndof(H->W);
+    double *p = (double *)malloc(sizeof(double) * (size_t)nd);
+    rk_work wk;
+    wk.p = (double *)malloc(sizeof(double) * (size_t)nd);
+    wk.du = (double *)malloc(sizeof(double) * (size_t)nd);
+    const double c0 = 0.5 * H->dt * orc_waveholtz_K(H, 0.0);
+    for (long long i = 0; i < nd; ++i) {
+        p[i] = u[i];
+        u[i] *= c0;
+    }
+    double t = 0.0;
+    for (int it = 1; it <= H->nt; ++it) {
+        rk_step(H->W, &wk, H->dt, &t, p, NULL, 0.0);
+        const double c = ((it == H->nt) ? 0.5 : 1.0) * H->dt * orc_waveholtz_K(H, t);
+        for (long long i = 0; i < nd; ++i) u[i] += c * p[i];
+    }
+    free(p);
+    free(wk.p);
+    free(wk.du);
+}
+
+/* WaveHoltz.cpp:123-145: H(0,i) = p, H(1,i) = -(1/omega) (M^{-1}(A+B)U)_p */
+void orc_waveholtz_postprocess(const orc_wh *H, double *Hout, const double *u)
+{
+    const long long nd = orc_wave_ndof(H->W), npt = nd / 3;
+    double *P = (double *)malloc(sizeof(double) * (size_t)nd);
+    orc_rhs(H->W, u, P, NULL, 0.0);
+    for (long long i = 0; i < npt; ++i) {
+        Hout[2 * i] = u[3 * i];
+        Hout[2 * i + 1] = -1.0 / H->omega * P[3 * i];
+    }
+    free(P);
+}
+
+/* ===================================================================== */
+/* load vector                                                           */
+/* ===================================================================== */
+
+/* LinearFunctional.hpp:62-74 (constructor) and :136-202 (2D action). */
+void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src, int var, double *F)
+{
+    double xb[16], wb[16], xq[16], wq[16], B[256];
+    orc_gauss_lobatto(n, xb, wb);
+    orc_gauss_legendre(nq, xq, wq);
+    orc_lagrange_basis(B, n, xb, nq, xq); /* B(l, j) = phi_j(xq_l), (nq, n) */
+    const int nel = m->nel;
+#pragma omp parallel for schedule(static)
+    for (int el = 0; el < nel; ++el) {
+        const double *c = m->corners + 8 * (size_t)el;
+        double Fq[3 * 16 * 16], ProjF[3 * 16 * 16];
+        for (int l = 0; l < nq; ++l)
+            for (int k = 0; k < nq; ++k) {
+                double X[2], J[4];
+                quad_coords(c, xq[k], xq[l], X);
+                quad_jacobian(c, xq[k], xq[l], J);
+                double fv[3] = {0.0, 0.0, 0.0};
+                for (int s = 0; s < nsrc; ++s) {
+                    const double dx = X[0] - src[4 * s], dy = X[1] - src[4 * s + 1], sg = src[4 * s + 2];
+                    fv[var] += src[4 * s + 3] * exp(-(dx * dx + dy * dy) / (2.0 * sg * sg));
+                }
+                const double detJ = J[0] * J[3] - J[1] * J[2];
+                const double wt = detJ * wq[k] * wq[l];
+                for (int d = 0; d < 3; ++d) Fq[d + 3 * (k + nq * l)] = fv[d] * wt;
+            }
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < nq; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    double s = 0.0;
+                    for (int l = 0; l < nq; ++l) s += Fq[d + 3 * (k + nq * l)] * B[l + nq * j];
+                    ProjF[d + 3 * (k + nq * j)] = s;
+                }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    double s = 0.0;
+                    for (int k = 0; k < nq; ++k) s += ProjF[d + 3 * (k + nq * j)] * B[k + nq * i];
+                    F[d + 3 * (i + n * (j + (size_t)n * el))] = s;
+                }
+    }
+}
+
+/* ===================================================================== */
+/* reductions (linalg.cpp:102-130)                                       */
+/* ===================================================================== */
+
+double orc_norm(long long n, const double *x)
+{
+    double r = 0.0;
+    for (long long i = 0; i < n; ++i) r += x[i] * x[i];
+    return sqrt(r);
+}
+
+double orc_error(long long n, const double *x, const double *y)
+{
+    double r = 0.0;
+    for (long long i = 0; i < n; ++i) {
+        const double d = x[i] - y[i];
+        r += d * d;
+    }
+    return sqrt(r);
+}

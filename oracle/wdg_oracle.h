@@ -1,0 +1,91 @@
+/*
+ * wdg_oracle.h -- CPU oracle for the 2D wave DG operator and WaveHoltz.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (arotem3/waveDG, approx_quad = true, Gauss-Lobatto basis) used to
+ * check the HIP implementation.  Only tests, the smoke check and the CPU
+ * baseline leg of bench.py may load it.  The product library never links it.
+ *
+ * Layouts follow the reference (column-major, first index fastest):
+ *   state u        (3, n, n, n_elem)   p, u, v per GLL node      (docs/docs.md:55-61)
+ *   vertices       (2, n_vertices)
+ *   elements       (4, n_elem)         counter-clockwise corners (Mesh2D.cpp:5-29)
+ *   edges          (5, n_edges)        el0, side0, el1, side1, delta
+ *
+ * Boundary condition codes: 0 absorbing (characteristic outflow, with the
+ * exterior pressure computed after the exterior velocity), 1 reflecting,
+ * 2 Dirichlet p = 0 (mirror state p_ext = -p, u_ext = u).
+ *
+ * Pinning status of each function is listed in oracle/README.md.
+ */
+#ifndef WDG_ORACLE_H
+#define WDG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- basis ---------------------------------------------------------- */
+int orc_gauss_lobatto(int n, double *x, double *w);
+int orc_gauss_legendre(int n, double *x, double *w);
+void orc_lagrange_basis(double *B, int ngrid, const double *xgrid, int neval, const double *xeval);
+void orc_lagrange_basis_deriv(double *D, int ngrid, const double *xgrid, int neval, const double *xeval);
+
+/* ---- mesh ----------------------------------------------------------- */
+typedef struct orc_mesh orc_mesh;
+orc_mesh *orc_mesh_from_vertices(int nv, const double *x, int nel, const int *elems);
+void orc_mesh_free(orc_mesh *m);
+int orc_mesh_n_elem(const orc_mesh *m);
+int orc_mesh_n_edges(const orc_mesh *m, int type); /* type 0 interior, 1 boundary */
+void orc_mesh_edges(const orc_mesh *m, int type, int *out /* (5, ne) */);
+void orc_mesh_edge_geometry(const orc_mesh *m, int type, double *out /* (5, ne): nx, ny, meas, xmid, ymid */);
+double orc_mesh_min_h(const orc_mesh *m);
+void orc_mesh_nodes(const orc_mesh *m, int nq, const double *xq, double *out /* (2, nq, nq, nel) */);
+void orc_mesh_detj(const orc_mesh *m, int nq, const double *xq, double *out /* (nq, nq, nel) */);
+
+/* ---- wave operator -------------------------------------------------- */
+/* Fs(c, d) (3x3 column-major) of EdgeFlux for the wave equation at unit
+ * normal (n0, n1) and side 0/1 (EdgeFlux.cpp:31-89), reference (a, b) sign. */
+void orc_edge_flux(double n0, double n1, double a, double b, int side, double *Fs);
+typedef struct orc_wave orc_wave;
+/* a, b follow the reference EdgeFlux sign convention (upwind: a=-1, b=-0.5). */
+orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, double b);
+void orc_wave_free(orc_wave *W);
+long long orc_wave_ndof(const orc_wave *W);
+void orc_wave_action(const orc_wave *W, const double *u, double *du);   /* du += A u   */
+void orc_wavebc_action(const orc_wave *W, const double *u, double *du); /* du += B u   */
+void orc_mass_action(const orc_wave *W, int nvar, const double *x, double *y);
+void orc_mass_inv(const orc_wave *W, int nvar, double *x);
+/* vt = M^{-1} (A u + B u + F s); F may be NULL. */
+void orc_rhs(const orc_wave *W, const double *u, double *vt, const double *F, double s);
+/* n_steps of the low-storage RK4 on u' = M^{-1}(A+B)u (+ forcing F*sin-term if
+ * omega > 0 and F != NULL, as WaveHoltz::pi0). t is advanced. */
+void orc_rk4_steps(const orc_wave *W, double dt, double *t, double *u, int n_steps,
+                   const double *F, double omega);
+void orc_set_threads(int n);
+
+/* ---- WaveHoltz ------------------------------------------------------ */
+typedef struct orc_wh orc_wh;
+orc_wh *orc_waveholtz_create(const orc_wave *W, double omega);
+void orc_waveholtz_free(orc_wh *H);
+void orc_waveholtz_info(const orc_wh *H, double *T, double *dt, int *nt);
+double orc_waveholtz_K(const orc_wh *H, double t);
+void orc_waveholtz_pi0(const orc_wh *H, double *u, const double *F);
+void orc_waveholtz_S(const orc_wh *H, double *u);
+void orc_waveholtz_postprocess(const orc_wh *H, double *Hout, const double *u);
+
+/* ---- linear functional (load vector) ----------------------------------
+ * (f, phi_ij) with Gauss-Legendre(nq) quadrature, f = sum of Gaussians in
+ * variable `var`: f(x) = sum_k amp_k exp(-|x - c_k|^2 / (2 sigma_k^2)).
+ * src is (4, nsrc): cx, cy, sigma, amp. */
+void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src,
+                           int var, double *F);
+
+/* ---- reductions ------------------------------------------------------ */
+double orc_norm(long long n, const double *x);
+double orc_error(long long n, const double *x, const double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

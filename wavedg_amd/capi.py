@@ -1,0 +1,322 @@
+"""ctypes binding of libwavedg_hip.so (include/wavedg_hip.h).
+
+Device arrays are passed as torch CUDA tensors (or raw integer device
+pointers); torch supplies device memory and the current stream only.  Every
+call checks the returned status and raises ``WdgError`` with the library's
+message.  Nothing here falls back to CPU code: without the built library,
+``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwavedg_hip.so")
+
+F64, F32 = 0, 1
+INTERIOR, BOUNDARY = 0, 1
+BC_ABSORB, BC_REFLECT, BC_DIRICHLET = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_ll = ctypes.c_longlong
+
+# name -> argument types (every function returns int status)
+_SIGS = {
+    "wdg_hip_version": [_ip, _ip],
+    "wdg_hip_malloc": [ctypes.POINTER(_vp), ctypes.c_size_t],
+    "wdg_hip_free": [_vp],
+    "wdg_hip_memcpy_htod": [_vp, _vp, ctypes.c_size_t],
+    "wdg_hip_memcpy_dtoh": [_vp, _vp, ctypes.c_size_t],
+    "wdg_hip_memset": [_vp, _i, ctypes.c_size_t],
+    "wdg_hip_synchronize": [_vp],
+    "wdg_hip_mesh_from_vertices": [_i, _dp, _i, _ip, ctypes.POINTER(_vp)],
+    "wdg_hip_mesh_uniform_rect": [_i, _d, _d, _i, _d, _d, ctypes.POINTER(_vp)],
+    "wdg_hip_mesh_from_file": [ctypes.c_char_p, ctypes.POINTER(_vp)],
+    "wdg_hip_mesh_destroy": [_vp],
+    "wdg_hip_mesh_counts": [_vp, _ip, _ip, _ip],
+    "wdg_hip_mesh_edges": [_vp, _i, _ip],
+    "wdg_hip_mesh_edge_geometry": [_vp, _i, _dp],
+    "wdg_hip_mesh_min_h": [_vp, _dp],
+    "wdg_hip_mesh_nodes": [_vp, _i, _dp],
+    "wdg_hip_gll": [_i, _dp, _dp, _dp],
+    "wdg_hip_wave2d_create": [_vp, _i, _ip, _d, _d, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_wave2d_destroy": [_vp],
+    "wdg_hip_wave2d_info": [_vp, ctypes.POINTER(_ll), _ip, _ip, _ip],
+    "wdg_hip_wave2d_action": [_vp, _vp, _vp, _vp],
+    "wdg_hip_wavebc_action": [_vp, _vp, _vp, _vp],
+    "wdg_hip_mass_inv": [_vp, _vp, _vp],
+    "wdg_hip_mass_action": [_vp, _vp, _vp, _vp],
+    "wdg_hip_wave2d_rhs": [_vp, _vp, _vp, _vp, _d, _vp],
+    "wdg_hip_rk4_steps": [_vp, _d, _dp, _vp, _i, _vp, _d, _vp],
+    "wdg_hip_rk4_steps_graph": [_vp, _d, _dp, _vp, _i, _i, _vp],
+    "wdg_hip_wave2d_set_timing": [_vp, _i],
+    "wdg_hip_wave2d_last_timing": [_vp, _dp, _ip],
+    "wdg_hip_waveholtz_create": [_vp, _d, ctypes.POINTER(_vp)],
+    "wdg_hip_waveholtz_destroy": [_vp],
+    "wdg_hip_waveholtz_info": [_vp, _dp, _dp, _ip],
+    "wdg_hip_waveholtz_pi0": [_vp, _vp, _vp, _vp],
+    "wdg_hip_waveholtz_S": [_vp, _vp, _vp],
+    "wdg_hip_waveholtz_iterate": [_vp, _vp, _vp, _d, _dp, _vp],
+    "wdg_hip_waveholtz_postprocess": [_vp, _vp, _vp, _vp],
+    "wdg_hip_load_vector_gauss": [_vp, _i, _dp, _i, _vp, _vp],
+    "wdg_hip_norm": [_vp, _ll, _dp, _vp],
+    "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
+    "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
+}
+
+_lib = None
+
+
+class WdgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[wdg status {code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Loads the built HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m wavedg_amd._build` "
+                               "(the HIP implementation has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.wdg_hip_last_error.restype = ctypes.c_char_p
+        L.wdg_hip_last_error.argtypes = []
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def header_symbols() -> list[str]:
+    """Every function declared in include/wavedg_hip.h."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "wavedg_hip.h")
+    with open(hdr) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(wdg_hip_\w+)\s*\(", txt)))
+
+
+def _call(name: str, *args) -> None:
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise WdgError(rc, lib().wdg_hip_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def dev_ptr(t) -> int | None:
+    """Device address of a torch CUDA tensor (or an int address, or None)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def cur_stream() -> int | None:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+def gll(n: int):
+    x, w, D = np.zeros(n), np.zeros(n), np.zeros(n * n)
+    _call("wdg_hip_gll", n, _dptr(x), _dptr(w), _dptr(D))
+    return x, w, D.reshape(n, n).T  # D[i, j] = phi_j'(x_i)
+
+
+class Mesh:
+    """Host topology (Mesh2D equivalent)."""
+
+    def __init__(self, vertices=None, elements=None, *, handle=None):
+        if handle is not None:
+            self.h = handle
+        else:
+            v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 2))
+            e = np.ascontiguousarray(np.asarray(elements, dtype=np.int32).reshape(-1, 4))
+            h = _vp()
+            _call("wdg_hip_mesh_from_vertices", v.shape[0], _dptr(v), e.shape[0], _iptr(e), ctypes.byref(h))
+            self.h = h
+        ne, ni, nb = _i(), _i(), _i()
+        _call("wdg_hip_mesh_counts", self.h, ctypes.byref(ne), ctypes.byref(ni), ctypes.byref(nb))
+        self.n_elem, self.n_interior, self.n_boundary = ne.value, ni.value, nb.value
+
+    @classmethod
+    def uniform_rect(cls, nx, ax, bx, ny, ay, by):
+        h = _vp()
+        _call("wdg_hip_mesh_uniform_rect", nx, ax, bx, ny, ay, by, ctypes.byref(h))
+        return cls(handle=h)
+
+    @classmethod
+    def from_file(cls, path: str):
+        h = _vp()
+        _call("wdg_hip_mesh_from_file", path.encode(), ctypes.byref(h))
+        return cls(handle=h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_mesh_destroy(self.h)
+            self.h = None
+
+    def edges(self, face_type: int) -> np.ndarray:
+        ne = self.n_interior if face_type == INTERIOR else self.n_boundary
+        out = np.zeros(5 * ne, dtype=np.int32)
+        _call("wdg_hip_mesh_edges", self.h, face_type, _iptr(out))
+        return out.reshape(ne, 5)
+
+    def edge_geometry(self, face_type: int) -> np.ndarray:
+        ne = self.n_interior if face_type == INTERIOR else self.n_boundary
+        out = np.zeros(5 * ne)
+        _call("wdg_hip_mesh_edge_geometry", self.h, face_type, _dptr(out))
+        return out.reshape(ne, 5)
+
+    def min_h(self) -> float:
+        h = _d()
+        _call("wdg_hip_mesh_min_h", self.h, ctypes.byref(h))
+        return h.value
+
+    def nodes(self, n: int) -> np.ndarray:
+        out = np.zeros(2 * n * n * self.n_elem)
+        _call("wdg_hip_mesh_nodes", self.h, n, _dptr(out))
+        return out.reshape(self.n_elem, n, n, 2)
+
+
+class Wave2D:
+    """WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on the GPU."""
+
+    def __init__(self, mesh: Mesh, n: int, bc=None, a: float = 1.0, b: float = 0.5, precision: int = F64):
+        bcp = None
+        if bc is not None:
+            self._bc = np.ascontiguousarray(np.asarray(bc, dtype=np.int32))
+            if self._bc.shape != (mesh.n_boundary,):
+                raise ValueError("bc must have one code per boundary edge")
+            bcp = _iptr(self._bc)
+        h = _vp()
+        _call("wdg_hip_wave2d_create", mesh.h, n, bcp, a, b, precision, ctypes.byref(h))
+        self.h = h
+        nd, nc, ne, pr = _ll(), _i(), _i(), _i()
+        _call("wdg_hip_wave2d_info", self.h, ctypes.byref(nd), ctypes.byref(nc), ctypes.byref(ne), ctypes.byref(pr))
+        self.ndof, self.n, self.n_elem, self.precision = nd.value, nc.value, ne.value, pr.value
+        self.min_h = mesh.min_h()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_wave2d_destroy(self.h)
+            self.h = None
+
+    @property
+    def torch_dtype(self):
+        import torch
+
+        return torch.float32 if self.precision == F32 else torch.float64
+
+    def action(self, x, y, stream=None):
+        _call("wdg_hip_wave2d_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+
+    def bc_action(self, x, y, stream=None):
+        _call("wdg_hip_wavebc_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+
+    def mass_inv(self, x, stream=None):
+        _call("wdg_hip_mass_inv", self.h, dev_ptr(x), stream if stream is not None else cur_stream())
+
+    def mass_action(self, x, y, stream=None):
+        _call("wdg_hip_mass_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+
+    def rhs(self, x, y, F=None, s: float = 0.0, stream=None):
+        _call("wdg_hip_wave2d_rhs", self.h, dev_ptr(x), dev_ptr(y), dev_ptr(F), s,
+              stream if stream is not None else cur_stream())
+
+    def rk4(self, u, dt: float, n_steps: int, t: float = 0.0, F=None, omega: float = 0.0, stream=None) -> float:
+        tt = _d(t)
+        _call("wdg_hip_rk4_steps", self.h, dt, ctypes.byref(tt), dev_ptr(u), n_steps, dev_ptr(F), omega,
+              stream if stream is not None else cur_stream())
+        return tt.value
+
+    def rk4_graph(self, u, dt: float, n_steps: int, steps_per_graph: int, t: float = 0.0, stream=None) -> float:
+        tt = _d(t)
+        _call("wdg_hip_rk4_steps_graph", self.h, dt, ctypes.byref(tt), dev_ptr(u), n_steps, steps_per_graph,
+              stream if stream is not None else cur_stream())
+        return tt.value
+
+    def set_timing(self, on: bool) -> None:
+        _call("wdg_hip_wave2d_set_timing", self.h, 1 if on else 0)
+
+    def last_timing(self):
+        ms, nk = _d(), _i()
+        _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
+        return ms.value, nk.value
+
+    def load_vector_gauss(self, sources, F, var: int = 0, stream=None):
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
+        _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, dev_ptr(F),
+              stream if stream is not None else cur_stream())
+
+
+class WaveHoltz:
+    def __init__(self, wave: Wave2D, omega: float):
+        h = _vp()
+        _call("wdg_hip_waveholtz_create", wave.h, omega, ctypes.byref(h))
+        self.h = h
+        self.wave = wave
+        self.omega = omega
+        T, dt, nt = _d(), _d(), _i()
+        _call("wdg_hip_waveholtz_info", self.h, ctypes.byref(T), ctypes.byref(dt), ctypes.byref(nt))
+        self.T, self.dt, self.nt = T.value, dt.value, nt.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_waveholtz_destroy(self.h)
+            self.h = None
+
+    def pi0(self, w, F, stream=None):
+        _call("wdg_hip_waveholtz_pi0", self.h, dev_ptr(w), dev_ptr(F), stream if stream is not None else cur_stream())
+
+    def S(self, w, stream=None):
+        _call("wdg_hip_waveholtz_S", self.h, dev_ptr(w), stream if stream is not None else cur_stream())
+
+    def iterate(self, w, pi0, pi0_norm: float, stream=None) -> float:
+        r = _d()
+        _call("wdg_hip_waveholtz_iterate", self.h, dev_ptr(w), dev_ptr(pi0), pi0_norm, ctypes.byref(r),
+              stream if stream is not None else cur_stream())
+        return r.value
+
+    def postprocess(self, H, w, stream=None):
+        _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),
+              stream if stream is not None else cur_stream())
+
+
+def norm(x, stream=None) -> float:
+    r = _d()
+    _call("wdg_hip_norm", dev_ptr(x), x.numel(), ctypes.byref(r), stream if stream is not None else cur_stream())
+    return r.value
+
+
+def error(x, y, stream=None) -> float:
+    r = _d()
+    _call("wdg_hip_error", dev_ptr(x), dev_ptr(y), x.numel(), ctypes.byref(r),
+          stream if stream is not None else cur_stream())
+    return r.value

@@ -1,0 +1,156 @@
+#include "host_basis.hpp"
+
+#include <cmath>
+
+namespace wdg {
+
+namespace {
+
+// P_l(x) by the three-term recurrence.
+double legendre(int l, double x)
+{
+    double a = 1.0, b = x;
+    if (l == 0) return a;
+    for (int k = 1; k < l; ++k) {
+        const double c = ((2 * k + 1) * x * b - k * a) / (k + 1);
+        a = b;
+        b = c;
+    }
+    return b;
+}
+
+// Table values: QuadratureRule.cpp:37-46.
+const double kGll4[] = {-1, -0.447213595499958, 0.447213595499958, 1};
+const double kGll5[] = {-1, -0.654653670707977, 0, 0.654653670707977, 1};
+const double kGll6[] = {-1, -0.765055323929465, -0.285231516480645, 0.285231516480645, 0.765055323929465, 1};
+const double kGll7[] = {-1, -0.830223896278567, -0.468848793470714, 0.0, 0.468848793470714, 0.830223896278567, 1};
+const double kGll8[] = {-1, -0.871740148509607, -0.591700181433142, -0.209299217902479,
+                        0.2092992179024789, 0.591700181433142, 0.871740148509607, 1};
+const double kGll9[] = {-1, -0.899757995411460, -0.677186279510738, -0.363117463826178, 0,
+                        0.363117463826178, 0.677186279510738, 0.899757995411460, 1};
+
+// Table values: QuadratureRule.cpp:103-112.
+const double kGl[11][10] = {
+    {},
+    {0.0},
+    {-0.577350269189625764509149, 0.577350269189625764509149},
+    {-0.774596669241483377035853, 0.0, 0.774596669241483377035853},
+    {-0.861136311594052575223946, -0.339981043584856264802666, 0.339981043584856264802666, 0.861136311594052575223946},
+    {-0.906179845938663992797627, -0.538469310105683091036314, 0.0, 0.538469310105683091036314, 0.906179845938663992797627},
+    {-0.932469514203152027812302, -0.661209386466264513661400, -0.238619186083196908630502, 0.238619186083196908630502,
+     0.661209386466264513661400, 0.932469514203152027812302},
+    {-0.949107912342758524526190, -0.741531185599394439863865, -0.405845151377397166906606, 0.0,
+     0.405845151377397166906606, 0.741531185599394439863865, 0.949107912342758524526190},
+    {-0.960289856497536231683561, -0.796666477413626739591554, -0.525532409916328985817739, -0.183434642495649804939476,
+     0.183434642495649804939476, 0.525532409916328985817739, 0.796666477413626739591554, 0.960289856497536231683561},
+    {-0.968160239507626089835576, -0.836031107326635794299430, -0.613371432700590397308702, -0.324253423403808929038538,
+     0.0, 0.324253423403808929038538, 0.613371432700590397308702, 0.836031107326635794299430, 0.968160239507626089835576},
+    {-0.973906528517171720077964, -0.865063366688984510732097, -0.679409568299024406234327, -0.433395394129247190799266,
+     -0.148874338981631210884826, 0.148874338981631210884826, 0.433395394129247190799266, 0.679409568299024406234327,
+     0.865063366688984510732097, 0.973906528517171720077964}};
+
+// Barycentric weights lambda_j = 1 / prod_{k != j} (x_j - x_k).
+std::vector<double> bary(int n, const double* x)
+{
+    std::vector<double> lam(n, 1.0);
+    for (int j = 0; j < n; ++j) {
+        for (int k = 0; k < n; ++k)
+            if (k != j) lam[j] *= (x[j] - x[k]);
+        lam[j] = 1.0 / lam[j];
+    }
+    return lam;
+}
+
+}  // namespace
+
+bool gll_rule(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    const double* tab = nullptr;
+    static const double k2[] = {-1, 1}, k3[] = {-1, 0, 1};
+    switch (n) {
+    case 2: tab = k2; break;
+    case 3: tab = k3; break;
+    case 4: tab = kGll4; break;
+    case 5: tab = kGll5; break;
+    case 6: tab = kGll6; break;
+    case 7: tab = kGll7; break;
+    case 8: tab = kGll8; break;
+    case 9: tab = kGll9; break;
+    default: return false;
+    }
+    x.assign(tab, tab + n);
+    w.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const double P = legendre(n - 1, x[i]);
+        w[i] = 2.0 / (n * (n - 1) * (P * P));
+    }
+    return true;
+}
+
+bool gauss_legendre_rule(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    if (n < 1 || n > 10) return false;
+    x.assign(kGl[n], kGl[n] + n);
+    w.resize(n);
+    for (int i = 0; i < n; ++i) {
+        // P_n'(x) = n (x P_n - P_{n-1}) / (x^2 - 1)
+        const double xi = x[i];
+        const double dP = n * (xi * legendre(n, xi) - legendre(n - 1, xi)) / (xi * xi - 1.0);
+        w[i] = 2.0 / ((1.0 - xi * xi) * dP * dP);
+    }
+    return true;
+}
+
+void lagrange_derivative_matrix(int ngrid, const double* xg, int neval, const double* xe, double* D)
+{
+    const std::vector<double> lam = bary(ngrid, xg);
+    for (int i = 0; i < neval; ++i) {
+        const double t = xe[i];
+        int at = -1;
+        for (int k = 0; k < ngrid; ++k)
+            if (t == xg[k]) at = k;
+        if (at >= 0) {
+            // phi_j'(x_a) = (lam_j / lam_a) / (x_a - x_j), j != a; diagonal by the row-sum identity.
+            double diag = 0.0;
+            for (int j = 0; j < ngrid; ++j) {
+                if (j == at) continue;
+                const double v = (lam[j] / lam[at]) / (xg[at] - xg[j]);
+                D[i + neval * j] = v;
+                diag -= v;
+            }
+            D[i + neval * at] = diag;
+        } else {
+            // phi_j(t) = l(t) lam_j / (t - x_j); phi_j' = phi_j (sum_k 1/(t - x_k) - 1/(t - x_j))
+            double l = 1.0, s = 0.0;
+            for (int k = 0; k < ngrid; ++k) {
+                l *= (t - xg[k]);
+                s += 1.0 / (t - xg[k]);
+            }
+            for (int j = 0; j < ngrid; ++j) {
+                const double phi = l * lam[j] / (t - xg[j]);
+                D[i + neval * j] = phi * (s - 1.0 / (t - xg[j]));
+            }
+        }
+    }
+}
+
+void lagrange_interpolation_matrix(int ngrid, const double* xg, int neval, const double* xe, double* B)
+{
+    const std::vector<double> lam = bary(ngrid, xg);
+    for (int i = 0; i < neval; ++i) {
+        const double t = xe[i];
+        int at = -1;
+        for (int k = 0; k < ngrid; ++k)
+            if (t == xg[k]) at = k;
+        if (at >= 0) {
+            for (int j = 0; j < ngrid; ++j) B[i + neval * j] = (j == at) ? 1.0 : 0.0;
+            continue;
+        }
+        // second barycentric form: phi_j(t) = (lam_j/(t-x_j)) / sum_k lam_k/(t-x_k)
+        double den = 0.0;
+        for (int k = 0; k < ngrid; ++k) den += lam[k] / (t - xg[k]);
+        for (int j = 0; j < ngrid; ++j) B[i + neval * j] = (lam[j] / (t - xg[j])) / den;
+    }
+}
+
+}  // namespace wdg

@@ -1,0 +1,1046 @@
+// C ABI (include/wavedg_hip.h): handles, drivers and the small kernels that
+// are not part of the fused operator (mass matrix, reductions, WaveHoltz
+// bookkeeping, precision conversion).
+#include "../../include/wavedg_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "host_basis.hpp"
+#include "host_mesh.hpp"
+#include "wave2d_kernels.hpp"
+
+using wdg::KArgs;
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local std::string g_last_error;
+
+struct Status : std::exception {
+    int code;
+    std::string msg;
+    Status(int c, std::string m) : code(c), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Status(code, msg); }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(WDG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return WDG_OK;
+    } catch (const Status& s) {
+        g_last_error = s.msg;
+        return s.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return WDG_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return WDG_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return WDG_ERR_INVALID;
+    }
+}
+
+void* dmalloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (bytes == 0) bytes = 8;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(WDG_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// Carpenter-Kennedy (5,4) 2N-storage coefficients (ode.hpp:68-70).
+constexpr double RK4A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                            -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+constexpr double RK4B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                            1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                            2277821191437.0 / 14882151754819.0};
+constexpr double RK4C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                            2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+}  // namespace
+
+// ------------------------------------------------------------------ handles
+struct wdg_mesh_s {
+    wdg::HostMesh mesh;
+};
+
+struct wdg_wave2d_s {
+    int dev = 0;
+    int n = 0, nel = 0, prec = WDG_F64;
+    long long ndof = 0;
+    size_t rsz = 8;
+    double ha = -0.5, hb = -0.5;
+    double min_h = 0.0;
+    std::vector<double> x, w;
+    double* d_corners = nullptr;
+    int32_t* d_nbr = nullptr;
+    int32_t* d_info = nullptr;
+    double* d_basis = nullptr;
+    double* d_lvtab = nullptr;
+    void* d_tmp = nullptr;  // RK ping-pong partner of the caller's state
+    void* d_du = nullptr;   // RK register
+    void* d_r1 = nullptr;   // scratch state-sized buffers (postprocess, conversions)
+    void* d_r2 = nullptr;
+    double* d_partial = nullptr;
+    double* d_result = nullptr;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_n = 0;
+    hipStream_t cap_stream = nullptr;
+    std::map<std::tuple<void*, int, double>, hipGraphExec_t> graphs;
+
+    ~wdg_wave2d_s()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        dfree(d_corners);
+        dfree(d_nbr);
+        dfree(d_info);
+        dfree(d_basis);
+        dfree(d_lvtab);
+        dfree(d_tmp);
+        dfree(d_du);
+        dfree(d_r1);
+        dfree(d_r2);
+        dfree(d_partial);
+        dfree(d_result);
+    }
+
+    void use_device() const
+    {
+        int cur = -1;
+        hip_check(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    size_t state_bytes() const { return (size_t)ndof * rsz; }
+    void* ensure(void*& p, size_t bytes)
+    {
+        if (!p) p = dmalloc(bytes);
+        return p;
+    }
+    KArgs base_args() const
+    {
+        KArgs a{};
+        a.corners = d_corners;
+        a.nbr = d_nbr;
+        a.info = d_info;
+        a.basis = d_basis;
+        a.ha = ha;
+        a.hb = hb;
+        a.nel = nel;
+        return a;
+    }
+};
+
+struct wdg_waveholtz_s {
+    wdg_wave2d op = nullptr;
+    double omega = 0, T = 0, dt = 0;
+    int nt = 0;
+    void* d_p = nullptr;       // RK state (handle precision)
+    double* d_wprev = nullptr;
+    void* d_F = nullptr;       // forcing in handle precision (F32 only)
+    ~wdg_waveholtz_s()
+    {
+        dfree(d_p);
+        dfree(d_wprev);
+        dfree(d_F);
+    }
+    double K(double t) const { return omega / M_PI * (std::cos(omega * t) - 0.25); }  // WaveHoltz.hpp:48-51
+};
+
+namespace {
+
+void check_op(wdg_wave2d op)
+{
+    if (!op) fail(WDG_ERR_INVALID, "null wave2d handle");
+    op->use_device();
+}
+
+void launch_wave(int n, int kid, int f32, const KArgs& a, hipStream_t s)
+{
+    switch (n) {
+    case 2: wdg::launch_wave_n2(kid, f32, a, s); break;
+    case 3: wdg::launch_wave_n3(kid, f32, a, s); break;
+    case 4: wdg::launch_wave_n4(kid, f32, a, s); break;
+    case 5: wdg::launch_wave_n5(kid, f32, a, s); break;
+    case 6: wdg::launch_wave_n6(kid, f32, a, s); break;
+    case 7: wdg::launch_wave_n7(kid, f32, a, s); break;
+    case 8: wdg::launch_wave_n8(kid, f32, a, s); break;
+    case 9: wdg::launch_wave_n9(kid, f32, a, s); break;
+    default: fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+    }
+    hip_check(hipGetLastError(), "wave kernel launch");
+}
+
+// ------------------------------------------------------------ small kernels
+template <typename R>
+__global__ void k_mass(const R* __restrict__ x, R* __restrict__ y, const double* __restrict__ corners,
+                       const double* __restrict__ basis, int N, long long nnodes, int inv)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnodes) return;
+    const int NN = N * N;
+    const long long e = q / NN;
+    const int node = (int)(q - e * NN);
+    const int i = node % N, j = node / N;
+    const double* c = corners + 8 * e;
+    const double xi = basis[i], eta = basis[j];
+    const double J0 = 0.25 * ((1.0 - eta) * (c[2] - c[0]) + (1.0 + eta) * (c[4] - c[6]));
+    const double J1 = 0.25 * ((1.0 - eta) * (c[3] - c[1]) + (1.0 + eta) * (c[5] - c[7]));
+    const double J2 = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
+    const double J3 = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
+    const double m = basis[N + i] * basis[N + j] * (J0 * J3 - J1 * J2);
+    for (int d = 0; d < 3; ++d) {
+        if (inv)
+            y[3 * q + d] = (R)((double)x[3 * q + d] / m);
+        else
+            y[3 * q + d] = (R)(m * (double)x[3 * q + d]);
+    }
+}
+
+__device__ double block_sum(double v)
+{
+    __shared__ double sh[kRedThreads];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sumsq(const double* __restrict__ x, const double* __restrict__ y,
+                                                       long long n, double* __restrict__ partial)
+{
+    double s = 0.0;
+    for (long long k = (long long)blockIdx.x * kRedThreads + threadIdx.x; k < n;
+         k += (long long)gridDim.x * kRedThreads) {
+        const double d = y ? x[k] - y[k] : x[k];
+        s += d * d;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_final(const double* __restrict__ partial, int np,
+                                                       double* __restrict__ out)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += kRedThreads) s += partial[k];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// S initialisation (WaveHoltz.cpp:103-107): p = w; [wprev = w]; w *= c0.
+template <typename R>
+__global__ void k_wh_init(double* __restrict__ w, R* __restrict__ p, double* __restrict__ wprev, double c0,
+                          long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double v = w[k];
+    p[k] = (R)v;
+    if (wprev) wprev[k] = v;
+    w[k] = v * c0;
+}
+
+// Fixed-point update and residual partial sums (waveholtz2d.cpp:172-175).
+__global__ __launch_bounds__(kRedThreads) void k_wh_finalize(double* __restrict__ w, const double* __restrict__ pi0,
+                                                             const double* __restrict__ wprev, long long n,
+                                                             double* __restrict__ partial)
+{
+    double s = 0.0;
+    for (long long k = (long long)blockIdx.x * kRedThreads + threadIdx.x; k < n;
+         k += (long long)gridDim.x * kRedThreads) {
+        const double v = w[k] + pi0[k];
+        w[k] = v;
+        const double d = v - wprev[k];
+        s += d * d;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <typename S, typename D>
+__global__ void k_convert(const S* __restrict__ src, D* __restrict__ dst, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = (D)src[k];
+}
+
+template <typename R>
+__global__ void k_zero(R* __restrict__ x, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = (R)0;
+}
+
+// postprocess (WaveHoltz.cpp:138-144): H(0,i) = U(0,i), H(1,i) = -1/omega P(0,i).
+template <typename R>
+__global__ void k_post(const double* __restrict__ U, const R* __restrict__ P, double* __restrict__ H, double omega,
+                       long long npts)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    H[2 * k] = U[3 * k];
+    H[2 * k + 1] = -1.0 / omega * (double)P[3 * k];
+}
+
+unsigned grid_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+double reduce_sumsq(const double* x, const double* y, long long n, double* partial, double* result, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sumsq, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, y, n, partial);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks, result);
+    hip_check(hipGetLastError(), "reduction launch");
+    double h = 0.0;
+    hip_check(hipMemcpyAsync(&h, result, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return h;
+}
+
+template <typename R>
+void zero_state(R* x, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero<R>, dim3(grid_for(n)), dim3(256), 0, s, x, n);
+}
+
+void zero_any(void* x, int prec, long long n, hipStream_t s)
+{
+    if (prec == WDG_F32)
+        zero_state(static_cast<float*>(x), n, s);
+    else
+        zero_state(static_cast<double*>(x), n, s);
+}
+
+// Runs n_steps RK4 steps starting from state `u` (ping-pong with op->d_tmp).
+// force: load vector in handle precision or nullptr (then omega unused).
+// acc: WaveHoltz accumulator or nullptr; coef(it, t_after) gives its weight.
+// Returns the buffer holding the final state.
+template <typename Coef>
+void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const void* force, double omega,
+                double* acc, Coef coef, hipStream_t s)
+{
+    op->ensure(op->d_tmp, op->state_bytes());
+    op->ensure(op->d_du, op->state_bytes());
+    const int f32 = op->prec == WDG_F32;
+    KArgs a = op->base_args();
+    a.du = op->d_du;
+    a.dt = dt;
+    a.force = force;
+    void* cur = u;
+    void* nxt = op->d_tmp;
+    for (int it = 0; it < n_steps; ++it) {
+        const double t_after = t + dt;
+        for (int st = 0; st < 5; ++st) {
+            a.u_in = cur;
+            a.u_out = nxt;
+            a.rk_a = RK4A[st];
+            a.rk_b = RK4B[st];
+            int kid;
+            if (st == 0) {
+                kid = force ? wdg::K_STAGE0_F : wdg::K_STAGE0;
+            } else if (st == 4 && acc) {
+                kid = force ? wdg::K_STAGE_F_ACC : wdg::K_STAGE_ACC;
+                a.acc = acc;
+                a.ac = coef(it, t_after);
+            } else {
+                kid = force ? wdg::K_STAGE_F : wdg::K_STAGE;
+            }
+            if (force) {
+                const double ts = (st == 0) ? t : t + RK4C[st] * dt;
+                a.fs = -1.0 / omega * std::sin(omega * ts);  // WaveHoltz.cpp:65
+            }
+            launch_wave(op->n, kid, f32, a, s);
+            std::swap(cur, nxt);
+        }
+        t += dt;
+    }
+    return cur;
+}
+
+struct NoCoef {
+    double operator()(int, double) const { return 0.0; }
+};
+
+void start_timing(wdg_wave2d op, hipStream_t s)
+{
+    if (!op->timing) return;
+    if (!op->ev0) hip_check(hipEventCreate(&op->ev0), "hipEventCreate");
+    if (!op->ev1) hip_check(hipEventCreate(&op->ev1), "hipEventCreate");
+    hip_check(hipEventRecord(op->ev0, s), "hipEventRecord");
+}
+
+void stop_timing(wdg_wave2d op, hipStream_t s, int n_kernels)
+{
+    if (!op->timing) return;
+    hip_check(hipEventRecord(op->ev1, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(op->ev1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, op->ev0, op->ev1), "hipEventElapsedTime");
+    op->last_ms = ms;
+    op->last_n = n_kernels;
+}
+
+void check_wh(wdg_waveholtz wh)
+{
+    if (!wh) fail(WDG_ERR_INVALID, "null waveholtz handle");
+    check_op(wh->op);
+}
+
+// S(w) with optional wprev copy; accumulation in w.
+void wh_apply_S(wdg_waveholtz wh, double* d_w, double* d_wprev, hipStream_t s)
+{
+    wdg_wave2d op = wh->op;
+    wh->op->ensure(wh->d_p, op->state_bytes());
+    const double c0 = 0.5 * wh->dt * wh->K(0.0);
+    const long long n = op->ndof;
+    if (op->prec == WDG_F32)
+        hipLaunchKernelGGL(k_wh_init<float>, dim3(grid_for(n)), dim3(256), 0, s, d_w, static_cast<float*>(wh->d_p),
+                           d_wprev, c0, n);
+    else
+        hipLaunchKernelGGL(k_wh_init<double>, dim3(grid_for(n)), dim3(256), 0, s, d_w,
+                           static_cast<double*>(wh->d_p), d_wprev, c0, n);
+    hip_check(hipGetLastError(), "k_wh_init launch");
+    double t = 0.0;
+    const int nt = wh->nt;
+    const double dt = wh->dt;
+    auto coef = [wh, nt, dt](int it, double t_after) {
+        return ((it + 1 == nt) ? 0.5 : 1.0) * dt * wh->K(t_after);  // WaveHoltz.cpp:115-119
+    };
+    run_steps(op, dt, t, wh->d_p, nt, nullptr, 0.0, d_w, coef, s);
+}
+
+}  // namespace
+
+// ====================================================================== API
+extern "C" {
+
+const char* wdg_hip_last_error(void) { return g_last_error.c_str(); }
+
+int wdg_hip_version(int* major, int* minor)
+{
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return WDG_OK;
+}
+
+int wdg_hip_malloc(void** d_ptr, size_t bytes)
+{
+    return guarded([&] {
+        if (!d_ptr) fail(WDG_ERR_INVALID, "null output pointer");
+        *d_ptr = dmalloc(bytes);
+    });
+}
+
+int wdg_hip_free(void* d_ptr)
+{
+    return guarded([&] { hip_check(hipFree(d_ptr), "hipFree"); });
+}
+
+int wdg_hip_memcpy_htod(void* d_dst, const void* src, size_t bytes)
+{
+    return guarded([&] { hip_check(hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D"); });
+}
+
+int wdg_hip_memcpy_dtoh(void* dst, const void* d_src, size_t bytes)
+{
+    return guarded([&] { hip_check(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H"); });
+}
+
+int wdg_hip_memset(void* d_ptr, int value, size_t bytes)
+{
+    return guarded([&] { hip_check(hipMemset(d_ptr, value, bytes), "hipMemset"); });
+}
+
+int wdg_hip_synchronize(void* stream)
+{
+    return guarded([&] { hip_check(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize"); });
+}
+
+// ---------------------------------------------------------------- mesh
+int wdg_hip_mesh_from_vertices(int n_vertices, const double* x, int n_elem, const int* elements, wdg_mesh* out)
+{
+    return guarded([&] {
+        if (!out) fail(WDG_ERR_INVALID, "null output handle");
+        try {
+            *out = new wdg_mesh_s{wdg::HostMesh::from_vertices(n_vertices, x, n_elem, elements)};
+        } catch (const std::invalid_argument& e) {
+            fail(WDG_ERR_MESH, e.what());
+        }
+    });
+}
+
+int wdg_hip_mesh_uniform_rect(int nx, double ax, double bx, int ny, double ay, double by, wdg_mesh* out)
+{
+    return guarded([&] {
+        if (!out) fail(WDG_ERR_INVALID, "null output handle");
+        try {
+            *out = new wdg_mesh_s{wdg::HostMesh::uniform_rect(nx, ax, bx, ny, ay, by)};
+        } catch (const std::invalid_argument& e) {
+            fail(WDG_ERR_MESH, e.what());
+        }
+    });
+}
+
+int wdg_hip_mesh_from_file(const char* dir, wdg_mesh* out)
+{
+    return guarded([&] {
+        if (!out || !dir) fail(WDG_ERR_INVALID, "null argument");
+        try {
+            *out = new wdg_mesh_s{wdg::HostMesh::from_file(dir)};
+        } catch (const std::exception& e) {
+            fail(WDG_ERR_MESH, e.what());
+        }
+    });
+}
+
+int wdg_hip_mesh_destroy(wdg_mesh mesh)
+{
+    delete mesh;
+    return WDG_OK;
+}
+
+int wdg_hip_mesh_counts(wdg_mesh mesh, int* n_elem, int* n_interior, int* n_boundary)
+{
+    return guarded([&] {
+        if (!mesh) fail(WDG_ERR_INVALID, "null mesh handle");
+        if (n_elem) *n_elem = mesh->mesh.n_elem();
+        if (n_interior) *n_interior = mesh->mesh.n_interior();
+        if (n_boundary) *n_boundary = mesh->mesh.n_boundary();
+    });
+}
+
+int wdg_hip_mesh_edges(wdg_mesh mesh, int face_type, int* out)
+{
+    return guarded([&] {
+        if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
+        const bool interior = face_type == WDG_INTERIOR;
+        const int ne = interior ? mesh->mesh.n_interior() : mesh->mesh.n_boundary();
+        for (int i = 0; i < ne; ++i) {
+            const wdg::HostEdge& E = mesh->mesh.edge(i, interior);
+            out[5 * i + 0] = E.el[0];
+            out[5 * i + 1] = E.side[0];
+            out[5 * i + 2] = E.el[1];
+            out[5 * i + 3] = E.side[1];
+            out[5 * i + 4] = E.delta;
+        }
+    });
+}
+
+int wdg_hip_mesh_edge_geometry(wdg_mesh mesh, int face_type, double* out)
+{
+    return guarded([&] {
+        if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
+        const bool interior = face_type == WDG_INTERIOR;
+        const int ne = interior ? mesh->mesh.n_interior() : mesh->mesh.n_boundary();
+        for (int i = 0; i < ne; ++i) {
+            const wdg::HostEdge& E = mesh->mesh.edge(i, interior);
+            out[5 * i + 0] = E.normal[0];
+            out[5 * i + 1] = E.normal[1];
+            out[5 * i + 2] = E.half_length;
+            out[5 * i + 3] = E.x0[0] + E.dx[0] * 0.5;
+            out[5 * i + 4] = E.x0[1] + E.dx[1] * 0.5;
+        }
+    });
+}
+
+int wdg_hip_mesh_min_h(wdg_mesh mesh, double* h)
+{
+    return guarded([&] {
+        if (!mesh || !h) fail(WDG_ERR_INVALID, "null argument");
+        *h = mesh->mesh.min_h();
+    });
+}
+
+int wdg_hip_mesh_nodes(wdg_mesh mesh, int n_colloc, double* out)
+{
+    return guarded([&] {
+        if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
+        std::vector<double> x, w;
+        if (!wdg::gll_rule(n_colloc, x, w)) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+        mesh->mesh.nodes(n_colloc, x.data(), out);
+    });
+}
+
+int wdg_hip_gll(int n, double* x, double* w, double* D)
+{
+    return guarded([&] {
+        std::vector<double> xv, wv;
+        if (!wdg::gll_rule(n, xv, wv)) fail(WDG_ERR_INVALID, "n out of range 2..9");
+        if (x) std::memcpy(x, xv.data(), sizeof(double) * n);
+        if (w) std::memcpy(w, wv.data(), sizeof(double) * n);
+        if (D) wdg::lagrange_derivative_matrix(n, xv.data(), n, xv.data(), D);
+    });
+}
+
+// ---------------------------------------------------------------- operator
+int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int precision,
+                          wdg_wave2d* out)
+{
+    return guarded([&] {
+        if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
+        if (precision != WDG_F64 && precision != WDG_F32) fail(WDG_ERR_INVALID, "precision must be WDG_F64 or WDG_F32");
+        if (!std::isfinite(a) || !std::isfinite(b)) fail(WDG_ERR_INVALID, "flux coefficients must be finite");
+        const wdg::HostMesh& m = mesh->mesh;
+        auto op = std::make_unique<wdg_wave2d_s>();
+        hip_check(hipGetDevice(&op->dev), "hipGetDevice");
+        if (!wdg::gll_rule(n_colloc, op->x, op->w)) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+        if (m.min_corner_detj() <= 0.0) fail(WDG_ERR_MESH, "mesh has an inverted or degenerate element (det J <= 0)");
+        wdg::FaceTables ft;
+        std::string err;
+        if (!m.face_tables(bc, ft, err)) fail(WDG_ERR_INVALID, err);
+        op->n = n_colloc;
+        op->nel = m.n_elem();
+        op->ndof = 3LL * n_colloc * n_colloc * op->nel;
+        op->prec = precision;
+        op->rsz = precision == WDG_F32 ? 4 : 8;
+        op->ha = -0.5 * a;  // reference sign convention (EdgeFlux.hpp:51)
+        op->hb = -b;
+        op->min_h = m.min_h();
+
+        const int n = n_colloc;
+        std::vector<double> basis(2 * n + n * n);
+        std::memcpy(basis.data(), op->x.data(), sizeof(double) * n);
+        std::memcpy(basis.data() + n, op->w.data(), sizeof(double) * n);
+        wdg::lagrange_derivative_matrix(n, op->x.data(), n, op->x.data(), basis.data() + 2 * n);
+        std::vector<double> xq, wq;
+        wdg::gauss_legendre_rule(n, xq, wq);
+        std::vector<double> lv(2 * n + n * n);
+        std::memcpy(lv.data(), xq.data(), sizeof(double) * n);
+        std::memcpy(lv.data() + n, wq.data(), sizeof(double) * n);
+        wdg::lagrange_interpolation_matrix(n, op->x.data(), n, xq.data(), lv.data() + 2 * n);
+
+        op->d_corners = static_cast<double*>(dmalloc(sizeof(double) * m.corners().size()));
+        op->d_nbr = static_cast<int32_t*>(dmalloc(sizeof(int32_t) * ft.nbr.size()));
+        op->d_info = static_cast<int32_t*>(dmalloc(sizeof(int32_t) * ft.info.size()));
+        op->d_basis = static_cast<double*>(dmalloc(sizeof(double) * basis.size()));
+        op->d_lvtab = static_cast<double*>(dmalloc(sizeof(double) * lv.size()));
+        op->d_partial = static_cast<double*>(dmalloc(sizeof(double) * kRedBlocks));
+        op->d_result = static_cast<double*>(dmalloc(sizeof(double)));
+        hip_check(hipMemcpy(op->d_corners, m.corners().data(), sizeof(double) * m.corners().size(),
+                            hipMemcpyHostToDevice), "upload corners");
+        hip_check(hipMemcpy(op->d_nbr, ft.nbr.data(), sizeof(int32_t) * ft.nbr.size(), hipMemcpyHostToDevice),
+                  "upload nbr");
+        hip_check(hipMemcpy(op->d_info, ft.info.data(), sizeof(int32_t) * ft.info.size(), hipMemcpyHostToDevice),
+                  "upload info");
+        hip_check(hipMemcpy(op->d_basis, basis.data(), sizeof(double) * basis.size(), hipMemcpyHostToDevice),
+                  "upload basis");
+        hip_check(hipMemcpy(op->d_lvtab, lv.data(), sizeof(double) * lv.size(), hipMemcpyHostToDevice),
+                  "upload load-vector table");
+        *out = op.release();
+    });
+}
+
+int wdg_hip_wave2d_destroy(wdg_wave2d op)
+{
+    if (op) {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != op->dev) (void)hipSetDevice(op->dev);
+        delete op;
+        if (cur >= 0) (void)hipSetDevice(cur);
+    }
+    return WDG_OK;
+}
+
+int wdg_hip_wave2d_info(wdg_wave2d op, long long* ndof, int* n_colloc, int* n_elem, int* precision)
+{
+    return guarded([&] {
+        if (!op) fail(WDG_ERR_INVALID, "null wave2d handle");
+        if (ndof) *ndof = op->ndof;
+        if (n_colloc) *n_colloc = op->n;
+        if (n_elem) *n_elem = op->nel;
+        if (precision) *precision = op->prec;
+    });
+}
+
+static int op_launch(wdg_wave2d op, int kid, const void* x, void* y, const void* F, double s, void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!x || !y) fail(WDG_ERR_INVALID, "null device pointer");
+        if (x == y) fail(WDG_ERR_INVALID, "input and output must not alias");
+        KArgs a = op->base_args();
+        a.u_in = x;
+        a.u_out = y;
+        a.force = F;
+        a.fs = s;
+        launch_wave(op->n, kid, op->prec == WDG_F32, a, as_stream(stream));
+    });
+}
+
+int wdg_hip_wave2d_action(wdg_wave2d op, const void* d_x, void* d_y, void* stream)
+{
+    return op_launch(op, wdg::K_ACTION, d_x, d_y, nullptr, 0.0, stream);
+}
+
+int wdg_hip_wavebc_action(wdg_wave2d op, const void* d_x, void* d_y, void* stream)
+{
+    return op_launch(op, wdg::K_BC, d_x, d_y, nullptr, 0.0, stream);
+}
+
+int wdg_hip_wave2d_rhs(wdg_wave2d op, const void* d_x, void* d_y, const void* d_F, double s, void* stream)
+{
+    return op_launch(op, d_F ? wdg::K_RHS_F : wdg::K_RHS, d_x, d_y, d_F, s, stream);
+}
+
+static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!x || !y) fail(WDG_ERR_INVALID, "null device pointer");
+        const long long nn = op->ndof / 3;
+        const hipStream_t s = as_stream(stream);
+        if (op->prec == WDG_F32)
+            hipLaunchKernelGGL(k_mass<float>, dim3(grid_for(nn)), dim3(256), 0, s, static_cast<const float*>(x),
+                               static_cast<float*>(y), op->d_corners, op->d_basis, op->n, nn, inv);
+        else
+            hipLaunchKernelGGL(k_mass<double>, dim3(grid_for(nn)), dim3(256), 0, s, static_cast<const double*>(x),
+                               static_cast<double*>(y), op->d_corners, op->d_basis, op->n, nn, inv);
+        hip_check(hipGetLastError(), "mass kernel launch");
+    });
+}
+
+int wdg_hip_mass_inv(wdg_wave2d op, void* d_x, void* stream) { return mass_launch(op, d_x, d_x, 1, stream); }
+
+int wdg_hip_mass_action(wdg_wave2d op, const void* d_x, void* d_y, void* stream)
+{
+    return mass_launch(op, d_x, d_y, 0, stream);
+}
+
+int wdg_hip_rk4_steps(wdg_wave2d op, double dt, double* t, void* d_u, int n_steps, const void* d_F, double omega,
+                      void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!t || !d_u) fail(WDG_ERR_INVALID, "null argument");
+        if (n_steps < 0) fail(WDG_ERR_INVALID, "n_steps must be >= 0");
+        if (!(dt > 0.0) || !std::isfinite(dt)) fail(WDG_ERR_INVALID, "dt must be positive and finite");
+        if (d_F && !(omega > 0.0)) fail(WDG_ERR_INVALID, "forcing requires omega > 0");
+        const hipStream_t s = as_stream(stream);
+        start_timing(op, s);
+        double tt = *t;
+        void* fin = run_steps(op, dt, tt, d_u, n_steps, d_F, omega, nullptr, NoCoef{}, s);
+        stop_timing(op, s, 5 * n_steps);
+        if (fin != d_u)
+            hip_check(hipMemcpyAsync(d_u, fin, op->state_bytes(), hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+        *t = tt;
+    });
+}
+
+int wdg_hip_rk4_steps_graph(wdg_wave2d op, double dt, double* t, void* d_u, int n_steps, int steps_per_graph,
+                            void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!t || !d_u) fail(WDG_ERR_INVALID, "null argument");
+        if (steps_per_graph <= 0 || (steps_per_graph & 1))
+            fail(WDG_ERR_INVALID, "steps_per_graph must be a positive even number");
+        if (n_steps < 0 || n_steps % steps_per_graph)
+            fail(WDG_ERR_INVALID, "n_steps must be a non-negative multiple of steps_per_graph");
+        if (!(dt > 0.0) || !std::isfinite(dt)) fail(WDG_ERR_INVALID, "dt must be positive and finite");
+        const hipStream_t s = as_stream(stream);
+        const auto key = std::make_tuple(d_u, steps_per_graph, dt);
+        auto it = op->graphs.find(key);
+        if (it == op->graphs.end()) {
+            if (!op->cap_stream) hip_check(hipStreamCreateWithFlags(&op->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+            op->ensure(op->d_tmp, op->state_bytes());
+            op->ensure(op->d_du, op->state_bytes());
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(op->cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            double tc = 0.0;
+            try {
+                run_steps(op, dt, tc, d_u, steps_per_graph, nullptr, 0.0, nullptr, NoCoef{}, op->cap_stream);
+            } catch (...) {
+                (void)hipStreamEndCapture(op->cap_stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(op->cap_stream, &g), "hipStreamEndCapture");
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(ie, "hipGraphInstantiate");
+            it = op->graphs.emplace(key, ex).first;
+        }
+        start_timing(op, s);
+        for (int k = 0; k < n_steps / steps_per_graph; ++k) hip_check(hipGraphLaunch(it->second, s), "hipGraphLaunch");
+        stop_timing(op, s, 5 * n_steps);
+        double tt = *t;
+        for (int k = 0; k < n_steps; ++k) tt += dt;
+        *t = tt;
+    });
+}
+
+int wdg_hip_wave2d_set_timing(wdg_wave2d op, int enable)
+{
+    return guarded([&] {
+        if (!op) fail(WDG_ERR_INVALID, "null wave2d handle");
+        op->timing = enable != 0;
+    });
+}
+
+int wdg_hip_wave2d_last_timing(wdg_wave2d op, double* ms_total, int* n_kernels)
+{
+    return guarded([&] {
+        if (!op) fail(WDG_ERR_INVALID, "null wave2d handle");
+        if (ms_total) *ms_total = op->last_ms;
+        if (n_kernels) *n_kernels = op->last_n;
+    });
+}
+
+// ---------------------------------------------------------------- WaveHoltz
+int wdg_hip_waveholtz_create(wdg_wave2d op, double omega, wdg_waveholtz* out)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!out) fail(WDG_ERR_INVALID, "null output handle");
+        if (!(omega > 0.0) || !std::isfinite(omega)) fail(WDG_ERR_INVALID, "omega must be positive and finite");
+        auto wh = std::make_unique<wdg_waveholtz_s>();
+        wh->op = op;
+        wh->omega = omega;
+        // WaveHoltz.cpp:24-29
+        wh->T = 2.0 * M_PI / omega;
+        const double h = op->min_h;
+        const double p = op->n;
+        wh->dt = 0.5 * h / (p * p);
+        wh->nt = (int)std::ceil(wh->T / wh->dt);
+        wh->dt = wh->T / wh->nt;
+        *out = wh.release();
+    });
+}
+
+int wdg_hip_waveholtz_destroy(wdg_waveholtz wh)
+{
+    delete wh;
+    return WDG_OK;
+}
+
+int wdg_hip_waveholtz_info(wdg_waveholtz wh, double* T, double* dt, int* nt)
+{
+    return guarded([&] {
+        if (!wh) fail(WDG_ERR_INVALID, "null waveholtz handle");
+        if (T) *T = wh->T;
+        if (dt) *dt = wh->dt;
+        if (nt) *nt = wh->nt;
+    });
+}
+
+int wdg_hip_waveholtz_pi0(wdg_waveholtz wh, double* d_w, const double* d_F, void* stream)
+{
+    return guarded([&] {
+        check_wh(wh);
+        if (!d_w || !d_F) fail(WDG_ERR_INVALID, "null device pointer");
+        wdg_wave2d op = wh->op;
+        const hipStream_t s = as_stream(stream);
+        const long long n = op->ndof;
+        op->ensure(wh->d_p, op->state_bytes());
+        const void* force = d_F;
+        if (op->prec == WDG_F32) {
+            op->ensure(wh->d_F, op->state_bytes());
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, s, d_F,
+                               static_cast<float*>(wh->d_F), n);
+            force = wh->d_F;
+        }
+        zero_any(wh->d_p, op->prec, n, s);
+        zero_state(d_w, n, s);
+        hip_check(hipGetLastError(), "pi0 init launch");
+        double t = 0.0;
+        const int nt = wh->nt;
+        const double dt = wh->dt;
+        auto coef = [wh, nt, dt](int it, double t_after) { return ((it + 1 == nt) ? 0.5 : 1.0) * dt * wh->K(t_after); };
+        run_steps(op, dt, t, wh->d_p, nt, force, wh->omega, d_w, coef, s);
+    });
+}
+
+int wdg_hip_waveholtz_S(wdg_waveholtz wh, double* d_w, void* stream)
+{
+    return guarded([&] {
+        check_wh(wh);
+        if (!d_w) fail(WDG_ERR_INVALID, "null device pointer");
+        wh_apply_S(wh, d_w, nullptr, as_stream(stream));
+    });
+}
+
+int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double* d_w, const double* d_pi0, double pi0_norm,
+                              double* rel_change, void* stream)
+{
+    return guarded([&] {
+        check_wh(wh);
+        if (!d_w || !d_pi0 || !rel_change) fail(WDG_ERR_INVALID, "null argument");
+        if (!(pi0_norm > 0.0)) fail(WDG_ERR_INVALID, "pi0_norm must be positive");
+        wdg_wave2d op = wh->op;
+        const hipStream_t s = as_stream(stream);
+        if (!wh->d_wprev) wh->d_wprev = static_cast<double*>(dmalloc(sizeof(double) * op->ndof));
+        wh_apply_S(wh, d_w, wh->d_wprev, s);
+        hipLaunchKernelGGL(k_wh_finalize, dim3(kRedBlocks), dim3(kRedThreads), 0, s, d_w, d_pi0, wh->d_wprev,
+                           op->ndof, op->d_partial);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kRedThreads), 0, s, op->d_partial, kRedBlocks, op->d_result);
+        hip_check(hipGetLastError(), "finalize launch");
+        double ss = 0.0;
+        hip_check(hipMemcpyAsync(&ss, op->d_result, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        *rel_change = std::sqrt(ss) / pi0_norm;
+        if (!std::isfinite(*rel_change)) fail(WDG_ERR_NONFINITE, "WaveHoltz residual is not finite (check dt / CFL)");
+    });
+}
+
+int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double* d_H, const double* d_w, void* stream)
+{
+    return guarded([&] {
+        check_wh(wh);
+        if (!d_H || !d_w) fail(WDG_ERR_INVALID, "null device pointer");
+        wdg_wave2d op = wh->op;
+        const hipStream_t s = as_stream(stream);
+        const long long n = op->ndof;
+        op->ensure(op->d_r2, op->state_bytes());
+        const void* x = d_w;
+        if (op->prec == WDG_F32) {
+            op->ensure(op->d_r1, op->state_bytes());
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, s, d_w,
+                               static_cast<float*>(op->d_r1), n);
+            x = op->d_r1;
+        }
+        KArgs a = op->base_args();
+        a.u_in = x;
+        a.u_out = op->d_r2;
+        launch_wave(op->n, wdg::K_RHS, op->prec == WDG_F32, a, s);
+        if (op->prec == WDG_F32)
+            hipLaunchKernelGGL(k_post<float>, dim3(grid_for(n / 3)), dim3(256), 0, s, d_w,
+                               static_cast<const float*>(op->d_r2), d_H, wh->omega, n / 3);
+        else
+            hipLaunchKernelGGL(k_post<double>, dim3(grid_for(n / 3)), dim3(256), 0, s, d_w,
+                               static_cast<const double*>(op->d_r2), d_H, wh->omega, n / 3);
+        hip_check(hipGetLastError(), "postprocess launch");
+    });
+}
+
+// ---------------------------------------------------------------- load vector
+int wdg_hip_load_vector_gauss(wdg_wave2d op, int n_sources, const double* sources, int var, double* d_F,
+                              void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!d_F || (n_sources > 0 && !sources)) fail(WDG_ERR_INVALID, "null argument");
+        if (n_sources < 0 || var < 0 || var > 2) fail(WDG_ERR_INVALID, "bad n_sources or var");
+        for (int k = 0; k < n_sources; ++k)
+            if (!(sources[4 * k + 2] > 0.0)) fail(WDG_ERR_INVALID, "source sigma must be positive");
+        const hipStream_t s = as_stream(stream);
+        double* d_src = static_cast<double*>(dmalloc(sizeof(double) * 4 * (n_sources > 0 ? n_sources : 1)));
+        if (n_sources > 0)
+            hip_check(hipMemcpy(d_src, sources, sizeof(double) * 4 * n_sources, hipMemcpyHostToDevice), "upload sources");
+        switch (op->n) {
+        case 2: wdg::launch_load_vector_n2(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 3: wdg::launch_load_vector_n3(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 4: wdg::launch_load_vector_n4(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 5: wdg::launch_load_vector_n5(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 6: wdg::launch_load_vector_n6(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 7: wdg::launch_load_vector_n7(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 8: wdg::launch_load_vector_n8(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        case 9: wdg::launch_load_vector_n9(op->d_corners, op->d_lvtab, d_src, n_sources, var, d_F, op->nel, s); break;
+        default: break;
+        }
+        const hipError_t le = hipGetLastError();
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(d_src);
+        hip_check(le, "load vector launch");
+    });
+}
+
+// ---------------------------------------------------------------- reductions
+int wdg_hip_norm(const double* d_x, long long n, double* out, void* stream)
+{
+    return guarded([&] {
+        if (!d_x || !out || n < 0) fail(WDG_ERR_INVALID, "bad argument");
+        double* buf = static_cast<double*>(dmalloc(sizeof(double) * (kRedBlocks + 1)));
+        double v = 0.0;
+        try {
+            v = reduce_sumsq(d_x, nullptr, n, buf, buf + kRedBlocks, as_stream(stream));
+        } catch (...) {
+            (void)hipFree(buf);
+            throw;
+        }
+        (void)hipFree(buf);
+        *out = std::sqrt(v);
+        if (!std::isfinite(*out)) fail(WDG_ERR_NONFINITE, "norm is not finite");
+    });
+}
+
+int wdg_hip_error(const double* d_x, const double* d_y, long long n, double* out, void* stream)
+{
+    return guarded([&] {
+        if (!d_x || !d_y || !out || n < 0) fail(WDG_ERR_INVALID, "bad argument");
+        double* buf = static_cast<double*>(dmalloc(sizeof(double) * (kRedBlocks + 1)));
+        double v = 0.0;
+        try {
+            v = reduce_sumsq(d_x, d_y, n, buf, buf + kRedBlocks, as_stream(stream));
+        } catch (...) {
+            (void)hipFree(buf);
+            throw;
+        }
+        (void)hipFree(buf);
+        *out = std::sqrt(v);
+        if (!std::isfinite(*out)) fail(WDG_ERR_NONFINITE, "error norm is not finite");
+    });
+}
+
+int wdg_hip_convert(const void* d_src, int src_precision, void* d_dst, int dst_precision, long long n, void* stream)
+{
+    return guarded([&] {
+        if (!d_src || !d_dst || n < 0) fail(WDG_ERR_INVALID, "bad argument");
+        const hipStream_t s = as_stream(stream);
+        const unsigned g = grid_for(n);
+        if (src_precision == WDG_F64 && dst_precision == WDG_F32)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, s, static_cast<const double*>(d_src),
+                               static_cast<float*>(d_dst), n);
+        else if (src_precision == WDG_F32 && dst_precision == WDG_F64)
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(256), 0, s, static_cast<const float*>(d_src),
+                               static_cast<double*>(d_dst), n);
+        else if (src_precision == dst_precision && (src_precision == WDG_F64 || src_precision == WDG_F32))
+            hip_check(hipMemcpyAsync(d_dst, d_src, (size_t)n * (src_precision == WDG_F32 ? 4 : 8),
+                                     hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+        else
+            fail(WDG_ERR_INVALID, "precision must be WDG_F64 or WDG_F32");
+        hip_check(hipGetLastError(), "convert launch");
+    });
+}
+
+}  // extern "C"

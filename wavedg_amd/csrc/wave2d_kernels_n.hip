@@ -1,0 +1,63 @@
+// Instantiates the wave kernels for one polynomial order: compiled once per
+// N = WDG_N in 2..9 so the eight orders build in parallel.
+#include "wave2d_kernels.hpp"
+
+#ifndef WDG_N
+#error "compile with -DWDG_N=<n_colloc>"
+#endif
+
+namespace wdg {
+
+namespace {
+
+template <typename R>
+void launch_typed(int kid, const KArgs& a, hipStream_t s)
+{
+    constexpr int N = WDG_N;
+    const int nblk = (a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
+    const dim3 grid(nblk), block(Shape<N>::NT);
+    switch (kid) {
+#define WDG_CASE(K)                                                              \
+    case K:                                                                      \
+        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K]>), grid, block, 0, s, a); \
+        break;
+        WDG_CASE(K_ACTION)
+        WDG_CASE(K_BC)
+        WDG_CASE(K_RHS)
+        WDG_CASE(K_RHS_F)
+        WDG_CASE(K_STAGE0)
+        WDG_CASE(K_STAGE0_F)
+        WDG_CASE(K_STAGE)
+        WDG_CASE(K_STAGE_F)
+        WDG_CASE(K_STAGE_ACC)
+        WDG_CASE(K_STAGE_F_ACC)
+#undef WDG_CASE
+    default: break;
+    }
+}
+
+}  // namespace
+
+#define WDG_CAT2(a, b) a##b
+#define WDG_CAT(a, b) WDG_CAT2(a, b)
+
+void WDG_CAT(launch_wave_n, WDG_N)(int kid, int f32, const KArgs& a, hipStream_t s)
+{
+    if (a.nel <= 0) return;
+    if (f32)
+        launch_typed<float>(kid, a, s);
+    else
+        launch_typed<double>(kid, a, s);
+}
+
+void WDG_CAT(launch_load_vector_n, WDG_N)(const double* corners, const double* tab, const double* src, int nsrc,
+                                           int var, double* F, int nel, hipStream_t s)
+{
+    constexpr int N = WDG_N;
+    if (nel <= 0) return;
+    const int nblk = (nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
+    hipLaunchKernelGGL((load_vector_kernel<N>), dim3(nblk), dim3(Shape<N>::NT), 0, s, corners, tab, src, nsrc, var,
+                       F, nel);
+}
+
+}  // namespace wdg
