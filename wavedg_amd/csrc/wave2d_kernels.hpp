@@ -115,67 +115,148 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
     constexpr int NN = N * N;
     constexpr int EPB = Shape<N>::EPB;
     constexpr int NT = Shape<N>::NT;
-    constexpr int SZ = EPB * 3 * NN;
+    constexpr int NF = 4 * N;                  // face nodes per element
+    constexpr int FP = (NF + NN - 1) / NN;     // face nodes per thread (1 for N >= 4)
     constexpr bool VOL = (MODE & M_VOL) != 0;
     constexpr bool IFACE = (MODE & M_IF) != 0;
     constexpr bool BFACE = (MODE & M_BF) != 0;
+    constexpr bool FACE = IFACE || BFACE;
     constexpr int OUT = MODE & M_OUTMASK;
     constexpr bool FORCE = (MODE & M_FORCE) != 0;
     constexpr bool ACC = (MODE & M_ACC) != 0;
+    constexpr bool READ_DU = OUT == M_OUT_STAGE;
 
-    __shared__ R s_u[SZ];
-    __shared__ R s_F[VOL ? SZ : 1];
-    __shared__ R s_G[VOL ? SZ : 1];
+    __shared__ R s_u[EPB * 3 * NN];
+    __shared__ R s_F[VOL ? EPB * 3 * NN : 1];
+    __shared__ R s_G[VOL ? EPB * 3 * NN : 1];
+    __shared__ R s_S[FACE ? EPB * NF * 3 : 1];  // signed face contributions [el][side][k][var]
     __shared__ R s_D[NN];
-    __shared__ double s_x[N], s_w[N];
+    __shared__ double s_xw[2 * N];  // x[N], w[N]
+    const double* s_x = s_xw;
+    const double* s_w = s_xw + N;
 
     const R* __restrict__ u_in = static_cast<const R*>(a.u_in);
     const int tid = threadIdx.x;
     const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
     const int nloc = min(EPB, a.nel - e0);
 
+    // ---- prologue: issue every global load before the first LDS write so
+    // that they are all in flight together (clamped indices keep the
+    // unconditional loads of a partial last block in bounds).
+    const int cnt = nloc * 3 * NN;
+    R ld[3];
     {
-        const size_t gbase = (size_t)e0 * 3 * NN;
-        const int cnt = nloc * 3 * NN;
-        for (int k = tid; k < cnt; k += NT) s_u[k] = u_in[gbase + k];
-        for (int k = tid; k < NN; k += NT) s_D[k] = (R)a.basis[2 * N + k];
-        if (tid < N) {
-            s_x[tid] = a.basis[tid];
-            s_w[tid] = a.basis[N + tid];
-        }
+        const R* ub = u_in + (size_t)e0 * 3 * NN;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ld[r] = ub[min(tid + r * NT, cnt - 1)];
     }
-    __syncthreads();
-
+    const R dval = (R)a.basis[2 * N + min(tid, NN - 1)];
+    const double xwv = a.basis[min(tid, 2 * N - 1)];
     const int el = tid / NN;
     const int node = tid - el * NN;
     const int i = node % N, j = node / N;
     const bool active = el < nloc;
-    const int e = e0 + el;
+    int e = e0 + (active ? el : 0);
+    if constexpr (NN % 64 == 0) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave
+    const size_t g = ((size_t)e * NN + node) * 3;
 
-    double c[8];
+    // Corners (element-uniform); bilinear map x = a + h xi eta + ... gives
+    // dx/dxi = ax + hx eta, dx/deta = cx + hx xi (same for y) (Element.cpp:21-27).
+    const double2* cp = reinterpret_cast<const double2*>(a.corners + 8 * (size_t)e);
+    const double2 c01 = cp[0], c23 = cp[1], c45 = cp[2], c67 = cp[3];
+    const double c[8] = {c01.x, c01.y, c23.x, c23.y, c45.x, c45.y, c67.x, c67.y};
+
+    // Face lanes: face node f = node + p NN (side f / N, position f % N).
+    R nbv[FP][3];
+    int finfo[FP], fnb[FP];
+    double sxa[FP], sya[FP], sxb[FP], syb[FP];  // own side's end points (traversal order)
+    if constexpr (FACE) {
+        int4 nb4 = make_int4(0, 0, 0, 0), in4 = make_int4(0, 0, 0, 0);
+        if constexpr (NN % 64 == 0) {
+            nb4 = *reinterpret_cast<const int4*>(a.nbr + 4 * (size_t)e);
+            in4 = *reinterpret_cast<const int4*>(a.info + 4 * (size_t)e);
+        }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = active ? a.corners[8 * (size_t)e + k] : 0.0;
-
-    // Bilinear map Jacobian at (x_i, x_j): [dx/dxi, dy/dxi, dx/deta, dy/deta] (Element.cpp:21-27).
-    const double xi = s_x[i], eta = s_x[j];
-    const double J0 = 0.25 * ((1.0 - eta) * (c[2] - c[0]) + (1.0 + eta) * (c[4] - c[6]));
-    const double J1 = 0.25 * ((1.0 - eta) * (c[3] - c[1]) + (1.0 + eta) * (c[5] - c[7]));
-    const double J2 = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
-    const double J3 = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
-
-    R p = 0, uu = 0, vv = 0;
-    if (active) {
-        p = s_u[el * 3 * NN + 3 * node + 0];
-        uu = s_u[el * 3 * NN + 3 * node + 1];
-        vv = s_u[el * 3 * NN + 3 * node + 2];
+        for (int p = 0; p < FP; ++p) {
+            const int f = node + p * NN;
+            const bool fon = active && f < NF;
+            const int s = fon ? f / N : 0, k = fon ? f - (f / N) * N : 0;
+            if constexpr (NN % 64 == 0) {
+                fnb[p] = s == 0 ? nb4.x : s == 1 ? nb4.y : s == 2 ? nb4.z : nb4.w;
+                finfo[p] = s == 0 ? in4.x : s == 1 ? in4.y : s == 2 ? in4.z : in4.w;
+            } else {
+                fnb[p] = a.nbr[4 * (size_t)e + s];
+                finfo[p] = a.info[4 * (size_t)e + s];
+            }
+            const int va = (s == 0 || s == 3) ? 0 : (s == 1 ? 1 : 3);
+            const int vb = (s == 0) ? 1 : (s == 3 ? 3 : 2);
+            const double* ce = a.corners + 8 * (size_t)e;
+            sxa[p] = ce[2 * va];
+            sya[p] = ce[2 * va + 1];
+            sxb[p] = ce[2 * vb];
+            syb[p] = ce[2 * vb + 1];
+            nbv[p][0] = nbv[p][1] = nbv[p][2] = (R)0;
+            if (IFACE && fon && fnb[p] >= 0) {
+                const int inf = finfo[p];
+                const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                const R* q = u_in + ((size_t)fnb[p] * NN + side_node<N>(inf & 3, kk)) * 3;
+                nbv[p][0] = q[0];
+                nbv[p][1] = q[1];
+                nbv[p][2] = q[2];
+            }
+        }
     }
-    R r0 = 0, r1 = 0, r2 = 0;
+    R pre0 = 0, pre1 = 0, pre2 = 0;  // du (stage), y (accumulate)
+    if (active) {
+        if constexpr (READ_DU) {
+            const R* du = static_cast<const R*>(a.du);
+            pre0 = du[g + 0];
+            pre1 = du[g + 1];
+            pre2 = du[g + 2];
+        } else if constexpr (OUT == M_OUT_ACCUM) {
+            const R* y = static_cast<const R*>(a.u_out);
+            pre0 = y[g + 0];
+            pre1 = y[g + 1];
+            pre2 = y[g + 2];
+        }
+    }
+    R fv0 = 0, fv1 = 0, fv2 = 0;
+    if constexpr (FORCE) {
+        if (active) {
+            const R* Fv = static_cast<const R*>(a.force);
+            fv0 = Fv[g + 0];
+            fv1 = Fv[g + 1];
+            fv2 = Fv[g + 2];
+        }
+    }
+    double ac0 = 0, ac1 = 0, ac2 = 0;
+    if constexpr (ACC) {
+        if (active) {
+            ac0 = a.acc[g + 0];
+            ac1 = a.acc[g + 1];
+            ac2 = a.acc[g + 2];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) s_u[tid + r * NT] = ld[r];
+    if (tid < NN) s_D[tid] = dval;
+    if (tid < 2 * N) s_xw[tid] = xwv;
+    __syncthreads();
+
+    const double hx = 0.25 * ((c[0] - c[2]) + (c[4] - c[6])), hy = 0.25 * ((c[1] - c[3]) + (c[5] - c[7]));
+    const double ax = 0.25 * ((c[2] - c[0]) + (c[4] - c[6])), ay = 0.25 * ((c[3] - c[1]) + (c[5] - c[7]));
+    const double cx = 0.25 * ((c[6] - c[0]) + (c[4] - c[2])), cy = 0.25 * ((c[7] - c[1]) + (c[5] - c[3]));
+    const double xi = s_x[i], eta = s_x[j];
+    const double J0 = ax + hx * eta, J1 = ay + hy * eta, J2 = cx + hx * xi, J3 = cy + hy * xi;
+    const double Wt = s_w[j] * s_w[i];
+
+    const R p = s_u[el * 3 * NN + 3 * node + 0];
+    const R uu = s_u[el * 3 * NN + 3 * node + 1];
+    const R vv = s_u[el * 3 * NN + 3 * node + 2];
 
     if constexpr (VOL) {
-        // Weak-form volume term (Div.cpp:5-53, 131-179): contravariant fluxes
-        // F = W (y_eta f - x_eta g), G = W (-y_xi f + x_xi g) with f = (u, p, 0),
-        // g = (v, 0, p), then du(i,j) += sum_l D(l,i) F(l,j) + D(l,j) G(i,l).
-        const double Wt = s_w[j] * s_w[i];
+        // Contravariant fluxes F = W (y_eta f - x_eta g), G = W (-y_xi f + x_xi g),
+        // f = (u, p, 0), g = (v, 0, p) (Div.cpp:5-53).
         const R Ye = (R)(J3 * Wt), Xe = (R)(J2 * Wt), Yx = (R)(J1 * Wt), Xx = (R)(J0 * Wt);
         R* F = s_F + el * 3 * NN;
         R* G = s_G + el * 3 * NN;
@@ -185,9 +266,83 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
         G[0 * NN + i * N + j] = Xx * vv - Yx * uu;
         G[1 * NN + i * N + j] = -(Yx * p);
         G[2 * NN + i * N + j] = Xx * p;
-        __syncthreads();
-        const R* Fr = F + j * N;
-        const R* Gc = G + i * N;
+    }
+
+    if constexpr (FACE) {
+        // One lane per face node: closed form of EdgeFlux<true> for the wave
+        // equation (EdgeFlux.cpp:31-139, 289-325; |n.A| = (n.A)^2 for unit n)
+        // lifted into this element (lobatto_transpose_2d,
+        // FaceProlongator.cpp:650-672).  Written with this element's outward
+        // normal n (= -n of el0 on the el1 side, Edge.hpp:107-122), own trace
+        // A and the other trace B, the contribution is
+        //   c_p = w ds [ha n.(uA+uB) + hb (pA-pB)],  c_u = n w ds [ha (pA+pB) + hb n.(uA-uB)]
+        // which equals +f on el0 and -f on el1 for the reference's edge flux f.
+        const R ha = (R)a.ha, hb = (R)a.hb;
+#pragma unroll
+        for (int pp = 0; pp < FP; ++pp) {
+            const int f = node + pp * NN;
+            if (!(active && f < NF)) continue;
+            const int s = f / N, k = f - s * N;
+            const int inf = finfo[pp];
+            const bool interior = fnb[pp] >= 0;
+            R c0 = 0, c1 = 0, c2 = 0;
+            if (interior ? IFACE : BFACE) {
+                const bool flip = (inf >> 2) & 1, first = (inf >> 3) & 1;
+                // own side traversed first corner -> second; sides 2, 3 run clockwise
+                const double dx = sxb[pp] - sxa[pp], dy = syb[pp] - sya[pp];
+                const double l2 = dx * dx + dy * dy;
+                double il = __builtin_amdgcn_rsq(l2);  // 1/len, two Newton steps
+                il = il * (1.5 - 0.5 * l2 * il * il);
+                il = il * (1.5 - 0.5 * l2 * il * il);
+                const double sil = (s >= 2) ? -il : il;
+                const R n0 = (R)(dy * sil), n1 = (R)(-dx * sil);
+                const int ie = (!first && flip) ? (N - 1 - k) : k;  // node index in el0's order
+                const R wds = (R)(s_w[ie] * (0.5 * (l2 * il)));
+                const int ln = side_node<N>(s, k);
+                const R pA = s_u[el * 3 * NN + 3 * ln + 0];
+                const R uA = s_u[el * 3 * NN + 3 * ln + 1];
+                const R vA = s_u[el * 3 * NN + 3 * ln + 2];
+                R pB, uB, vB;
+                if (interior) {
+                    pB = nbv[pp][0];
+                    uB = nbv[pp][1];
+                    vB = nbv[pp][2];
+                } else {
+                    const int code = (inf >> 8) & 0xff;
+                    if (code == 1) {  // reflecting (WaveEquation.hpp:64-69)
+                        pB = pA;
+                        uB = (n1 * n1 - n0 * n0) * uA - (R)2 * n0 * n1 * vA;
+                        vB = -(R)2 * n0 * n1 * uA + (n0 * n0 - n1 * n1) * vA;
+                    } else if (code == 0) {  // characteristic outflow, p_ext after u_ext (WaveEquation.hpp:77-84)
+                        const R out = pA + n0 * uA + n1 * vA;
+                        const R tang = n1 * uA - n0 * vA;
+                        uB = n1 * tang + (R)0.5 * n0 * out;
+                        vB = -n0 * tang + (R)0.5 * n1 * out;
+                        pB = n0 * uB + n1 * vB;
+                    } else {  // Dirichlet p = 0 (mirror state)
+                        pB = -pA;
+                        uB = uA;
+                        vB = vA;
+                    }
+                }
+                c0 = wds * (ha * (n0 * (uA + uB) + n1 * (vA + vB)) + hb * (pA - pB));
+                const R cf = wds * (ha * (pA + pB) + hb * (n0 * (uA - uB) + n1 * (vA - vB)));
+                c1 = n0 * cf;
+                c2 = n1 * cf;
+            }
+            R* S = s_S + (el * NF + f) * 3;
+            S[0] = c0;
+            S[1] = c1;
+            S[2] = c2;
+        }
+    }
+    __syncthreads();
+
+    R r0 = 0, r1 = 0, r2 = 0;
+    if constexpr (VOL) {
+        // du(i,j) += sum_l D(l,i) F(l,j) + D(l,j) G(i,l) (Div.cpp:131-179)
+        const R* Fr = s_F + el * 3 * NN + j * N;
+        const R* Gc = s_G + el * 3 * NN + i * N;
 #pragma unroll
         for (int l = 0; l < N; ++l) {
             const R di = s_D[l + N * i], dj = s_D[l + N * j];
@@ -196,103 +351,46 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
             r2 += di * Fr[2 * NN + l] + dj * Gc[2 * NN + l];
         }
     }
-
-    if constexpr (IFACE || BFACE) {
-        if (active && (i == 0 || i == N - 1 || j == 0 || j == N - 1)) {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const bool on = (s == 0) ? (j == 0) : (s == 1) ? (i == N - 1) : (s == 2) ? (j == N - 1) : (i == 0);
-                if (!on) continue;
-                const int nb = a.nbr[4 * (size_t)e + s];
-                if (nb >= 0 ? !IFACE : !BFACE) continue;
-                const int inf = a.info[4 * (size_t)e + s];
-                const int k = (s == 0 || s == 2) ? i : j;
-                const int first = (inf >> 3) & 1, flip = (inf >> 2) & 1, s0 = (inf >> 4) & 3;
-                // Side corners in traversal order (Mesh2D.cpp:10-11); the edge
-                // normal and length are taken from el0's traversal exactly as
-                // StraightEdge does (Edge.hpp:107-122), so both elements of an
-                // edge see bitwise identical n and ds.
-                const int va = (s == 0 || s == 3) ? 0 : (s == 1 ? 1 : 3);
-                const int vb = (s == 0) ? 1 : (s == 3 ? 3 : 2);
-                double dx = c[2 * vb] - c[2 * va], dy = c[2 * vb + 1] - c[2 * va + 1];
-                if (!first && flip) {
-                    dx = -dx;
-                    dy = -dy;
-                }
-                const double len = hypot(dx, dy);
-                const double sg = (s0 >= 2) ? -1.0 : 1.0;
-                const double nx = sg * dy / len, ny = -sg * dx / len;
-                const int kk = flip ? (N - 1 - k) : k;  // matching node on the other side
-                const int ie = first ? k : kk;          // node index in el0's order
-                const R wds = (R)(s_w[ie] * (len / 2));
-                const R n0 = (R)nx, n1 = (R)ny;
-
-                R pB, uB, vB;
-                if (nb >= 0) {
-                    const R* q = u_in + ((size_t)nb * NN + side_node<N>(inf & 3, kk)) * 3;
-                    pB = q[0];
-                    uB = q[1];
-                    vB = q[2];
-                } else {
-                    const int code = (inf >> 8) & 0xff;
-                    if (code == 1) {  // reflecting (WaveEquation.hpp:64-69)
-                        pB = p;
-                        uB = (n1 * n1 - n0 * n0) * uu - (R)2 * n0 * n1 * vv;
-                        vB = -(R)2 * n0 * n1 * uu + (n0 * n0 - n1 * n1) * vv;
-                    } else if (code == 0) {  // characteristic outflow (WaveEquation.hpp:77-84, p_ext after u_ext)
-                        const R out = p + n0 * uu + n1 * vv;
-                        const R tang = n1 * uu - n0 * vv;
-                        uB = n1 * tang + (R)0.5 * n0 * out;
-                        vB = -n0 * tang + (R)0.5 * n1 * out;
-                        pB = n0 * uB + n1 * vB;
-                    } else {  // Dirichlet p = 0 (mirror state)
-                        pB = -p;
-                        uB = uu;
-                        vB = vv;
-                    }
-                }
-                // (u0, u1) in the edge's orientation: u0 belongs to el0.
-                const R p0 = first ? p : pB, p1 = first ? pB : p;
-                const R u0 = first ? uu : uB, u1 = first ? uB : uu;
-                const R v0 = first ? vv : vB, v1 = first ? vB : vv;
-                // Closed form of EdgeFlux<true> for the wave equation (EdgeFlux.cpp:31-139,
-                // 289-325): with C = n.A, |C| = C^2 for unit n, so
-                //   f_p = w ds [ha n.(u0+u1) + hb (p0-p1)]
-                //   f_u = w ds n [ha (p0+p1) + hb n.(u0-u1)]
-                const R ha = (R)a.ha, hb = (R)a.hb;
-                const R fp = wds * (ha * (n0 * (u0 + u1) + n1 * (v0 + v1)) + hb * (p0 - p1));
-                const R fc = wds * (ha * (p0 + p1) + hb * (n0 * (u0 - u1) + n1 * (v0 - v1)));
-                if (first) {
-                    r0 += fp;
-                    r1 += n0 * fc;
-                    r2 += n1 * fc;
-                } else {
-                    r0 -= fp;
-                    r1 -= n0 * fc;
-                    r2 -= n1 * fc;
-                }
-            }
+    if constexpr (FACE) {
+        // add this node's face contributions in side order 0..3
+        const R* S = s_S + el * NF * 3;
+        if (j == 0) {
+            r0 += S[(0 * N + i) * 3 + 0];
+            r1 += S[(0 * N + i) * 3 + 1];
+            r2 += S[(0 * N + i) * 3 + 2];
+        }
+        if (i == N - 1) {
+            r0 += S[(1 * N + j) * 3 + 0];
+            r1 += S[(1 * N + j) * 3 + 1];
+            r2 += S[(1 * N + j) * 3 + 2];
+        }
+        if (j == N - 1) {
+            r0 += S[(2 * N + i) * 3 + 0];
+            r1 += S[(2 * N + i) * 3 + 1];
+            r2 += S[(2 * N + i) * 3 + 2];
+        }
+        if (i == 0) {
+            r0 += S[(3 * N + j) * 3 + 0];
+            r1 += S[(3 * N + j) * 3 + 1];
+            r2 += S[(3 * N + j) * 3 + 2];
         }
     }
 
     if (!active) return;
-    const size_t g = ((size_t)e * NN + node) * 3;
     if constexpr (OUT == M_OUT_ACCUM) {
         R* y = static_cast<R*>(a.u_out);
-        y[g + 0] += r0;
-        y[g + 1] += r1;
-        y[g + 2] += r2;
+        y[g + 0] = pre0 + r0;
+        y[g + 1] = pre1 + r1;
+        y[g + 2] = pre2 + r2;
     } else {
-        // MassMatrix<true>: m = w_i w_j det J (MassMatrix.cpp:6-28).
-        const double m = s_w[i] * s_w[j] * (J0 * J3 - J1 * J2);
+        // MassMatrix<true>: m = w_i w_j det J (MassMatrix.cpp:6-28)
+        const R im = (R)(1.0 / (Wt * (J0 * J3 - J1 * J2)));
         if constexpr (FORCE) {
-            const R* Fv = static_cast<const R*>(a.force);
             const R fs = (R)a.fs;
-            r0 += Fv[g + 0] * fs;
-            r1 += Fv[g + 1] * fs;
-            r2 += Fv[g + 2] * fs;
+            r0 += fv0 * fs;
+            r1 += fv1 * fs;
+            r2 += fv2 * fs;
         }
-        const R im = (R)(1.0 / m);
         r0 *= im;
         r1 *= im;
         r2 *= im;
@@ -302,7 +400,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
             y[g + 1] = r1;
             y[g + 2] = r2;
         } else {
-            // Low-storage RK stage (ode.hpp:74-90).
+            // low-storage RK stage (ode.hpp:74-90)
             R* du = static_cast<R*>(a.du);
             R* uo = static_cast<R*>(a.u_out);
             const R dt = (R)a.dt, rb = (R)a.rk_b;
@@ -313,9 +411,9 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
                 d2 = dt * r2;
             } else {
                 const R ra = (R)a.rk_a;
-                d0 = ra * du[g + 0] + dt * r0;
-                d1 = ra * du[g + 1] + dt * r1;
-                d2 = ra * du[g + 2] + dt * r2;
+                d0 = ra * pre0 + dt * r0;
+                d1 = ra * pre1 + dt * r1;
+                d2 = ra * pre2 + dt * r2;
             }
             du[g + 0] = d0;
             du[g + 1] = d1;
@@ -325,11 +423,11 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
             uo[g + 1] = w1;
             uo[g + 2] = w2;
             if constexpr (ACC) {
-                // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87).
-                const double ac = a.ac;
-                a.acc[g + 0] += ac * (double)w0;
-                a.acc[g + 1] += ac * (double)w1;
-                a.acc[g + 2] += ac * (double)w2;
+                // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
+                const double acw = a.ac;
+                a.acc[g + 0] = ac0 + acw * (double)w0;
+                a.acc[g + 1] = ac1 + acw * (double)w1;
+                a.acc[g + 2] = ac2 + acw * (double)w2;
             }
         }
     }
