@@ -35,10 +35,10 @@ def stage_bytes(ndof: int, nel: int, s: int) -> float:
     """Average algorithmic HBM bytes per fused stage launch.
 
     Stage 0 reads u and writes u', du (3 words/DoF); stages 1-4 read u, du and
-    write u', du (4 words/DoF); every stage reads 64 B of corners and 32 B of
-    face connectivity per element.
+    write u', du (4 words/DoF); every stage reads 48 B of bilinear-map
+    coefficients and 32 B of face connectivity per element.
     """
-    return ndof * s * (3 + 4 * 4) / 5.0 + nel * 96.0
+    return ndof * s * (3 + 4 * 4) / 5.0 + nel * 80.0
 
 
 def build_workload(nx: int):
@@ -81,6 +81,55 @@ def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph:
     return wave, u, t1 - t0, ms_kernels, n_kernels
 
 
+# Roofline set: ~50M DoF per order, larger than the 256 MB Infinity Cache.
+SWEEP = [(4, 1024), (5, 800), (6, 680), (7, 580), (8, 512)]
+
+
+def sweep(torch, capi, steps: int, precisions):
+    """Per-order stage time and achieved bandwidth on skewed nx^2 meshes."""
+    from wavedg_amd import meshgen
+
+    rows = []
+    for n, nx in SWEEP:
+        v, e = meshgen.perturbed_rect(nx, nx, 0.2, 4000 + n)
+        mesh = capi.Mesh(v, e)
+        bc = np.zeros(mesh.n_boundary, dtype=np.int32)
+        nodes = mesh.nodes(n)
+        p0 = meshgen.gaussian_sum(nodes, [(0.4, 0.5), (0.6, 0.55)], [0.05, 0.08], [1.0, -0.5])
+        u0 = meshgen.nodal_state(nodes, p0)
+        dt = 0.5 * mesh.min_h() / n**2
+        for prec in precisions:
+            wave, u, sec, ms_k, nk = time_path(torch, capi, mesh, n, bc, u0, dt, prec, steps, 2)
+            s = 8 if prec == capi.F64 else 4
+            avg = ms_k / max(nk, 1)
+            gbs = stage_bytes(wave.ndof, wave.n_elem, s) / (avg * 1e-3) / 1e9
+            rows.append({"P": n - 1, "nx": nx, "ndof": wave.ndof, "dtype": "f64" if s == 8 else "f32",
+                         "avg_launch_us": avg * 1e3, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                         "dof_stage_per_s": wave.ndof * 5 * steps / sec})
+            print(f"[sweep] P={n-1} {nx}^2 {rows[-1]['dtype']}: {avg*1e3:.1f} us/stage, {gbs:.0f} GB/s "
+                  f"({100*gbs/HBM_PEAK_GBS:.1f}%)", file=sys.stderr, flush=True)
+            del wave, u
+        del mesh
+    return rows
+
+
+def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Device-to-device copy of nbytes (read + write) with torch's copy kernel."""
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    ev1.record()
+    torch.cuda.synchronize()
+    sec = ev0.elapsed_time(ev1) * 1e-3 / reps
+    del x, y
+    return 2.0 * nbytes / sec / 1e9
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
     from oracle import oracle as O
 
@@ -103,6 +152,9 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
+    ap.add_argument("--only-sweep", action="store_true", help="skip the headline workload")
+    ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the copy-bandwidth calibration (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,6 +170,13 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.only_sweep:
+        rows = sweep(torch, capi, max(args.sweep, 1), [capi.F32, capi.F64])
+        print(json.dumps({"sweep": rows}))
+        return 0
+    copy_gbs = copy_bandwidth(torch, int(args.copy_gib * (1 << 30))) if args.copy_gib > 0 else None
+    if copy_gbs:
+        print(f"[bench] copy bandwidth {copy_gbs:.0f} GB/s ({args.copy_gib} GiB)", file=sys.stderr, flush=True)
     c, mesh, n, bc, u0, dt = build_workload(args.nx)
     print(f"[bench] workload {c['name']}: nel={mesh.n_elem} n={n} dt={dt:.6e} steps={args.steps}", file=sys.stderr,
           flush=True)
@@ -159,6 +218,7 @@ def main() -> int:
             "kernel": "wave_kernel<8,double,stage> (5 fused RK stages per step)",
             "bytes_per_launch": bytes64,
             "avg_launch_us": avg_ms * 1e3,
+            "copy_GBs": copy_gbs,
         },
     }
 
@@ -178,6 +238,9 @@ def main() -> int:
         }
         print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
         del wave32, u32
+
+    if args.sweep > 0:
+        out["roofline_by_P"] = sweep(torch, capi, args.sweep, [capi.F32, capi.F64])
 
     if args.cpu_steps > 0:
         v1, s1 = cpu_baseline(c, n, bc, u0, dt, 1, args.cpu_steps)

@@ -13,8 +13,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-OUT_DIR = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libwavedg_hip.so")
+OUT_DIR = os.environ.get("WDG_BUILD_DIR", os.path.join(HERE, "_build"))
+LIB = os.environ.get("WDG_LIB_OUT", os.path.join(HERE, "libwavedg_hip.so"))
 ORDERS = range(2, 10)  # n_colloc = P + 1
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -24,7 +24,7 @@ HIP_FLAGS = [
     # no packed-FP32 instructions in generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
-]
+] + os.environ.get("WDG_EXTRA_HIPFLAGS", "").split()
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
 

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwavedg_hip.so")
+LIB_PATH = os.environ.get("WDG_LIB", os.path.join(_HERE, "libwavedg_hip.so"))
 
 F64, F32 = 0, 1
 INTERIOR, BOUNDARY = 0, 1

@@ -107,6 +107,7 @@ struct wdg_wave2d_s {
     double min_h = 0.0;
     std::vector<double> x, w;
     double* d_corners = nullptr;
+    double* d_geom = nullptr;
     int32_t* d_nbr = nullptr;
     int32_t* d_info = nullptr;
     double* d_basis = nullptr;
@@ -131,6 +132,7 @@ struct wdg_wave2d_s {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         dfree(d_corners);
+        dfree(d_geom);
         dfree(d_nbr);
         dfree(d_info);
         dfree(d_basis);
@@ -158,7 +160,7 @@ struct wdg_wave2d_s {
     KArgs base_args() const
     {
         KArgs a{};
-        a.corners = d_corners;
+        a.geom = d_geom;
         a.nbr = d_nbr;
         a.info = d_info;
         a.basis = d_basis;
@@ -649,6 +651,21 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
         std::memcpy(lv.data() + n, wq.data(), sizeof(double) * n);
         wdg::lagrange_interpolation_matrix(n, op->x.data(), n, xq.data(), lv.data() + 2 * n);
 
+        // bilinear map coefficients a, c, h per element (see wave2d_kernels.hpp)
+        std::vector<double> geom(6 * (size_t)op->nel);
+        for (int e = 0; e < op->nel; ++e) {
+            const double* c = &m.corners()[8 * (size_t)e];
+            double* gq = &geom[6 * (size_t)e];
+            gq[0] = 0.25 * ((c[2] - c[0]) + (c[4] - c[6]));
+            gq[1] = 0.25 * ((c[3] - c[1]) + (c[5] - c[7]));
+            gq[2] = 0.25 * ((c[6] - c[0]) + (c[4] - c[2]));
+            gq[3] = 0.25 * ((c[7] - c[1]) + (c[5] - c[3]));
+            gq[4] = 0.25 * ((c[0] - c[2]) + (c[4] - c[6]));
+            gq[5] = 0.25 * ((c[1] - c[3]) + (c[5] - c[7]));
+        }
+        op->d_geom = static_cast<double*>(dmalloc(sizeof(double) * geom.size()));
+        hip_check(hipMemcpy(op->d_geom, geom.data(), sizeof(double) * geom.size(), hipMemcpyHostToDevice),
+                  "upload geometry");
         op->d_corners = static_cast<double*>(dmalloc(sizeof(double) * m.corners().size()));
         op->d_nbr = static_cast<int32_t*>(dmalloc(sizeof(int32_t) * ft.nbr.size()));
         op->d_info = static_cast<int32_t*>(dmalloc(sizeof(int32_t) * ft.info.size()));

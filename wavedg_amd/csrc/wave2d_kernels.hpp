@@ -30,7 +30,7 @@ struct KArgs {
     const void* u_in;       // state read by this element and its neighbours
     void* u_out;            // y (action/bc/rhs) or the next stage state
     void* du;               // RK register (stage modes)
-    const double* corners;  // (2, 4, nel)
+    const double* geom;     // (6, nel): ax, ay, cx, cy, hx, hy of the bilinear map
     const int32_t* nbr;     // (4, nel)
     const int32_t* info;    // (4, nel), see host_mesh.hpp
     const double* basis;    // x[N], w[N], D[N*N] (column-major)
@@ -84,6 +84,25 @@ constexpr int kModeOf[K_COUNT] = {
     M_VOL | M_IF | M_BF | M_OUT_STAGE | M_FORCE | M_ACC,
 };
 
+#ifndef WDG_MIN_WAVES
+#define WDG_MIN_WAVES 1  // occupancy hint: minimum waves per SIMD
+#endif
+#ifndef WDG_NT_STORES
+#define WDG_NT_STORES 0  // 1: stream stage outputs with non-temporal stores
+#endif
+
+// Store that bypasses cache allocation for data no kernel reads before the
+// next launch.
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v)
+{
+#if WDG_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 template <int N>
 struct Shape {
     static constexpr int NN = N * N;
@@ -100,6 +119,16 @@ __device__ __forceinline__ int xcd_block(int b, int nblk)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// 1/x to full double precision: hardware reciprocal estimate and two Newton
+// steps (x > 0 here: it is a mass-matrix entry).
+__device__ __forceinline__ double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 // Node (k along side s, counted from the side's first corner) -> flat node.
 template <int N>
 __device__ __forceinline__ int side_node(int s, int k)
@@ -110,7 +139,7 @@ __device__ __forceinline__ int side_node(int s, int k)
 }
 
 template <int N, typename R, int MODE>
-__global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
+__global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const KArgs a)
 {
     constexpr int NN = N * N;
     constexpr int EPB = Shape<N>::EPB;
@@ -160,16 +189,16 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
     if constexpr (NN % 64 == 0) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave
     const size_t g = ((size_t)e * NN + node) * 3;
 
-    // Corners (element-uniform); bilinear map x = a + h xi eta + ... gives
-    // dx/dxi = ax + hx eta, dx/deta = cx + hx xi (same for y) (Element.cpp:21-27).
-    const double2* cp = reinterpret_cast<const double2*>(a.corners + 8 * (size_t)e);
-    const double2 c01 = cp[0], c23 = cp[1], c45 = cp[2], c67 = cp[3];
-    const double c[8] = {c01.x, c01.y, c23.x, c23.y, c45.x, c45.y, c67.x, c67.y};
+    // Bilinear element map x(xi, eta) = x0 + a xi + c eta + h xi eta (per
+    // coordinate), so dx/dxi = a + h eta, dx/deta = c + h xi (Element.cpp:21-27)
+    // and side s runs along 2(a -+ h) (s = 0, 2) or 2(c +- h) (s = 1, 3).
+    const double2* gp = reinterpret_cast<const double2*>(a.geom + 6 * (size_t)e);
+    const double2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
+    const double ax = g_a.x, ay = g_a.y, cx = g_c.x, cy = g_c.y, hx = g_h.x, hy = g_h.y;
 
     // Face lanes: face node f = node + p NN (side f / N, position f % N).
     R nbv[FP][3];
     int finfo[FP], fnb[FP];
-    double sxa[FP], sya[FP], sxb[FP], syb[FP];  // own side's end points (traversal order)
     if constexpr (FACE) {
         int4 nb4 = make_int4(0, 0, 0, 0), in4 = make_int4(0, 0, 0, 0);
         if constexpr (NN % 64 == 0) {
@@ -188,13 +217,6 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
                 fnb[p] = a.nbr[4 * (size_t)e + s];
                 finfo[p] = a.info[4 * (size_t)e + s];
             }
-            const int va = (s == 0 || s == 3) ? 0 : (s == 1 ? 1 : 3);
-            const int vb = (s == 0) ? 1 : (s == 3 ? 3 : 2);
-            const double* ce = a.corners + 8 * (size_t)e;
-            sxa[p] = ce[2 * va];
-            sya[p] = ce[2 * va + 1];
-            sxb[p] = ce[2 * vb];
-            syb[p] = ce[2 * vb + 1];
             nbv[p][0] = nbv[p][1] = nbv[p][2] = (R)0;
             if (IFACE && fon && fnb[p] >= 0) {
                 const int inf = finfo[p];
@@ -243,9 +265,6 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
     if (tid < 2 * N) s_xw[tid] = xwv;
     __syncthreads();
 
-    const double hx = 0.25 * ((c[0] - c[2]) + (c[4] - c[6])), hy = 0.25 * ((c[1] - c[3]) + (c[5] - c[7]));
-    const double ax = 0.25 * ((c[2] - c[0]) + (c[4] - c[6])), ay = 0.25 * ((c[3] - c[1]) + (c[5] - c[7]));
-    const double cx = 0.25 * ((c[6] - c[0]) + (c[4] - c[2])), cy = 0.25 * ((c[7] - c[1]) + (c[5] - c[3]));
     const double xi = s_x[i], eta = s_x[j];
     const double J0 = ax + hx * eta, J1 = ay + hy * eta, J2 = cx + hx * xi, J3 = cy + hy * xi;
     const double Wt = s_w[j] * s_w[i];
@@ -288,8 +307,10 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
             R c0 = 0, c1 = 0, c2 = 0;
             if (interior ? IFACE : BFACE) {
                 const bool flip = (inf >> 2) & 1, first = (inf >> 3) & 1;
-                // own side traversed first corner -> second; sides 2, 3 run clockwise
-                const double dx = sxb[pp] - sxa[pp], dy = syb[pp] - sya[pp];
+                // own side vector in traversal order; sides 2, 3 run clockwise
+                const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
+                const double dx = 2.0 * ((s & 1) ? cx + sh * hx : ax + sh * hx);
+                const double dy = 2.0 * ((s & 1) ? cy + sh * hy : ay + sh * hy);
                 const double l2 = dx * dx + dy * dy;
                 double il = __builtin_amdgcn_rsq(l2);  // 1/len, two Newton steps
                 il = il * (1.5 - 0.5 * l2 * il * il);
@@ -346,9 +367,12 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
 #pragma unroll
         for (int l = 0; l < N; ++l) {
             const R di = s_D[l + N * i], dj = s_D[l + N * j];
-            r0 += di * Fr[l] + dj * Gc[l];
-            r1 += di * Fr[NN + l] + dj * Gc[NN + l];
-            r2 += di * Fr[2 * NN + l] + dj * Gc[2 * NN + l];
+            r0 = fma(di, Fr[l], r0);
+            r1 = fma(di, Fr[NN + l], r1);
+            r2 = fma(di, Fr[2 * NN + l], r2);
+            r0 = fma(dj, Gc[l], r0);
+            r1 = fma(dj, Gc[NN + l], r1);
+            r2 = fma(dj, Gc[2 * NN + l], r2);
         }
     }
     if constexpr (FACE) {
@@ -384,7 +408,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
         y[g + 2] = pre2 + r2;
     } else {
         // MassMatrix<true>: m = w_i w_j det J (MassMatrix.cpp:6-28)
-        const R im = (R)(1.0 / (Wt * (J0 * J3 - J1 * J2)));
+        const R im = (R)recip(Wt * (J0 * J3 - J1 * J2));
         if constexpr (FORCE) {
             const R fs = (R)a.fs;
             r0 += fv0 * fs;
@@ -415,13 +439,13 @@ __global__ __launch_bounds__(Shape<N>::NT) void wave_kernel(const KArgs a)
                 d1 = ra * pre1 + dt * r1;
                 d2 = ra * pre2 + dt * r2;
             }
-            du[g + 0] = d0;
-            du[g + 1] = d1;
-            du[g + 2] = d2;
+            st_stream(du + g + 0, d0);
+            st_stream(du + g + 1, d1);
+            st_stream(du + g + 2, d2);
             const R w0 = p + rb * d0, w1 = uu + rb * d1, w2 = vv + rb * d2;
-            uo[g + 0] = w0;
-            uo[g + 1] = w1;
-            uo[g + 2] = w2;
+            st_stream(uo + g + 0, w0);
+            st_stream(uo + g + 1, w1);
+            st_stream(uo + g + 2, w2);
             if constexpr (ACC) {
                 // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
                 const double acw = a.ac;
