@@ -12,6 +12,14 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    # Build the HIP library once if it is missing and a compiler is present
+    # (a GPU box runs the prebuilt library and has nothing to build).
+    from wavedg_amd import capi
+
+    if not os.path.exists(capi.LIB_PATH) and os.path.exists("/opt/rocm/bin/hipcc"):
+        from wavedg_amd import _build
+
+        _build.build()
 
 
 @pytest.fixture(scope="session")

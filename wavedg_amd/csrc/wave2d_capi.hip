@@ -624,12 +624,13 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
         if (!std::isfinite(a) || !std::isfinite(b)) fail(WDG_ERR_INVALID, "flux coefficients must be finite");
         const wdg::HostMesh& m = mesh->mesh;
         auto op = std::make_unique<wdg_wave2d_s>();
-        hip_check(hipGetDevice(&op->dev), "hipGetDevice");
+        // validate everything on the host before touching the device
         if (!wdg::gll_rule(n_colloc, op->x, op->w)) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
         if (m.min_corner_detj() <= 0.0) fail(WDG_ERR_MESH, "mesh has an inverted or degenerate element (det J <= 0)");
         wdg::FaceTables ft;
         std::string err;
         if (!m.face_tables(bc, ft, err)) fail(WDG_ERR_INVALID, err);
+        hip_check(hipGetDevice(&op->dev), "hipGetDevice");
         op->n = n_colloc;
         op->nel = m.n_elem();
         op->ndof = 3LL * n_colloc * n_colloc * op->nel;
