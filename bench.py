@@ -130,6 +130,40 @@ def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 20) -> float:
     return 2.0 * nbytes / sec / 1e9
 
 
+def waveholtz_solve(torch, capi, precision, tol: float, max_iter: int, n_grid: int = 384):
+    """Plain WaveHoltz on the L-shaped domain until ||W_k - W_{k-1}|| / ||pi0|| <= tol.
+
+    Timed from the load vector through pi0 and every S application; mesh and
+    operator setup are excluded.  Returns (W, stats)."""
+    from wavedg_amd import meshgen
+
+    c = meshgen.config3(n=n_grid)
+    mesh = capi.Mesh(c["vertices"], c["elements"])
+    bc = np.full(mesh.n_boundary, c["bc_code"], dtype=np.int32)
+    wave = capi.Wave2D(mesh, c["n"], bc, precision=precision)
+    wh = capi.WaveHoltz(wave, c["omega"])
+    F = torch.empty(wave.ndof, dtype=torch.float64, device="cuda")
+    pi0 = torch.empty_like(F)
+    W = torch.zeros_like(F)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wave.load_vector_gauss(c["sources"], F)
+    wh.pi0(pi0, F)
+    pn = capi.norm(pi0)
+    hist = []
+    for _ in range(max_iter):
+        hist.append(wh.iterate(W, pi0, pn))
+        if hist[-1] <= tol:
+            break
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    stats = {"dtype": "f64" if precision == capi.F64 else "f32 state / f64 filter", "iterations": len(hist),
+             "seconds": sec, "final_residual": hist[-1], "converged": hist[-1] <= tol, "nt": wh.nt,
+             "ndof": wave.ndof, "dof_stage_per_s": wave.ndof * 5.0 * wh.nt * (len(hist) + 1) / sec,
+             "residual_history_tail": hist[-5:]}
+    return W, stats
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
     from oracle import oracle as O
 
@@ -155,6 +189,11 @@ def main() -> int:
     ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
     ap.add_argument("--only-sweep", action="store_true", help="skip the headline workload")
     ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the copy-bandwidth calibration (0 = skip)")
+    ap.add_argument("--waveholtz", action="store_true", help="also time WaveHoltz to --tol on the L-shaped domain")
+    ap.add_argument("--only-waveholtz", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iter", type=int, default=400)
+    ap.add_argument("--wh-grid", type=int, default=384, help="L-shape grid size (384 = the benchmark workload)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -170,6 +209,15 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.only_waveholtz:
+        out = {}
+        W64, out["f64"] = waveholtz_solve(torch, capi, capi.F64, args.tol, args.max_iter, args.wh_grid)
+        print(f"[waveholtz] f64: {out['f64']}", file=sys.stderr, flush=True)
+        W32, out["f32"] = waveholtz_solve(torch, capi, capi.F32, args.tol, args.max_iter, args.wh_grid)
+        out["f32"]["rel_l2_vs_f64"] = float(torch.linalg.norm(W32 - W64) / torch.linalg.norm(W64))
+        print(f"[waveholtz] f32: {out['f32']}", file=sys.stderr, flush=True)
+        print(json.dumps({"waveholtz": out}))
+        return 0
     if args.only_sweep:
         rows = sweep(torch, capi, max(args.sweep, 1), [capi.F32, capi.F64])
         print(json.dumps({"sweep": rows}))
@@ -238,6 +286,14 @@ def main() -> int:
         }
         print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
         del wave32, u32
+
+    if args.waveholtz:
+        wh = {}
+        W64, wh["f64"] = waveholtz_solve(torch, capi, capi.F64, args.tol, args.max_iter, args.wh_grid)
+        W32, wh["f32"] = waveholtz_solve(torch, capi, capi.F32, args.tol, args.max_iter, args.wh_grid)
+        wh["f32"]["rel_l2_vs_f64"] = float(torch.linalg.norm(W32 - W64) / torch.linalg.norm(W64))
+        out["waveholtz"] = wh
+        del W64, W32
 
     if args.sweep > 0:
         out["roofline_by_P"] = sweep(torch, capi, args.sweep, [capi.F32, capi.F64])

@@ -1,0 +1,100 @@
+// Drives the dg::hip drop-in classes the way examples/waveholtz2d.cpp and
+// examples/wave2d.cpp drive the reference, and writes raw float64 results for
+// tests/test_gpu_dropin.py to compare with the oracle.
+//
+// usage: dropin_driver <out_dir> <mesh_dir>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "wavedg_amd/include/wavedg_hip/dg.hpp"
+
+using namespace dg::hip;
+
+static void to_file(const std::string& fname, size_t n, const double* u)
+{
+    std::ofstream out(fname, std::ios::out | std::ios::binary);
+    out.write(reinterpret_cast<const char*>(u), n * sizeof(double));
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <out_dir> <mesh_dir>\n", argv[0]);
+        return 2;
+    }
+    const std::string dir = argv[1], mesh_dir = argv[2];
+    try {
+        // --- WaveHoltz, as examples/waveholtz2d.cpp:92-190 (smaller mesh)
+        const int n_colloc = 5;
+        auto basis = QuadratureRule::quadrature_rule(n_colloc);
+        Mesh2D mesh = Mesh2D::uniform_rect(8, -1.0, 1.0, 8, -1.0, 1.0);
+        const int nB = mesh.n_edges(FaceType::BOUNDARY);
+        std::vector<double> geo = mesh.edge_geometry(FaceType::BOUNDARY);
+        std::vector<int> bc(nB);
+        for (int e = 0; e < nB; ++e) {
+            const bool left = std::abs(geo[5 * e + 3] + 1.0) < 1e-12, bottom = std::abs(geo[5 * e + 4] + 1.0) < 1e-12;
+            bc[e] = (left || bottom) ? 1 : 0;  // reflecting on the left and bottom walls, absorbing elsewhere
+        }
+        const double omega = 10.0;
+        WaveHoltz WH(omega, mesh, basis, bc.data(), true);
+        const long long ndof = WH.ndof();
+        // forcing (w/pi) exp(-w r^2), w = (10 pi)^2, centred at (-0.7, -0.1) (waveholtz2d.cpp:41-47)
+        const double w2 = std::pow(10.0 * M_PI, 2);
+        const double src[4] = {-0.7, -0.1, 1.0 / std::sqrt(2.0 * w2), w2 / M_PI};
+        LinearFunctional2D L(mesh, basis);
+        std::vector<double> F(ndof), pi0(ndof), W(ndof, 0.0), Wprev(ndof), H(2 * ndof / 3);
+        L.gaussians(1, src, 0, F.data());
+        WH.pi0(pi0.data(), F.data());
+        double pi_zero = 0;
+        for (double v : pi0) pi_zero += v * v;
+        pi_zero = std::sqrt(pi_zero);
+        std::FILE* log = std::fopen((dir + "/errs.txt").c_str(), "w");
+        for (int it = 1; it <= 3; ++it) {
+            Wprev = W;
+            WH.S(W.data());
+            double err = 0;
+            for (long long i = 0; i < ndof; ++i) {
+                W[i] += pi0[i];
+                err += (W[i] - Wprev[i]) * (W[i] - Wprev[i]);
+            }
+            std::fprintf(log, "%.17g\n", std::sqrt(err) / pi_zero);
+        }
+        std::fclose(log);
+        WH.postprocess(H.data(), W.data());
+        to_file(dir + "/F.bin", ndof, F.data());
+        to_file(dir + "/pi0.bin", ndof, pi0.data());
+        to_file(dir + "/W.bin", ndof, W.data());
+        to_file(dir + "/H.bin", H.size(), H.data());
+        std::vector<double> bcd(bc.begin(), bc.end());
+        to_file(dir + "/bc.bin", bcd.size(), bcd.data());
+
+        // --- wave2d time derivative as examples/wave2d.cpp:164-177, host arrays
+        Mesh2D mesh2 = Mesh2D::from_file(mesh_dir);
+        auto basis2 = QuadratureRule::quadrature_rule(4);
+        const int nB2 = mesh2.n_edges(FaceType::BOUNDARY);
+        std::vector<int> bc2(nB2, 1);
+        WaveEquation a(mesh2, basis2, true);
+        WaveBC b(mesh2, bc2.data(), basis2, true);
+        MassMatrix<true> m(mesh2, basis2);
+        const size_t nd2 = 3 * (size_t)16 * mesh2.n_elem();
+        std::vector<double> u(nd2), dudt(nd2);
+        for (size_t i = 0; i < nd2; ++i) u[i] = std::sin(0.37 * i) + 0.25 * std::cos(1.3 * i);
+        for (size_t i = 0; i < nd2; ++i) dudt[i] = 0.0;
+        a.action(u.data(), dudt.data());
+        b.action(u.data(), dudt.data());
+        m.inv(3, dudt.data());
+        to_file(dir + "/u2.bin", nd2, u.data());
+        to_file(dir + "/dudt2.bin", nd2, dudt.data());
+        std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
+    } catch (const Error& e) {
+        std::fprintf(stderr, "dg::hip::Error %d: %s\n", e.code, e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -64,5 +64,19 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+def build_dropin_driver(verbose: bool = False) -> str:
+    """Builds tests/cpp/dropin_driver.cpp, a C++ consumer of the dg::hip
+    drop-in classes (C ABI only, no HIP headers), next to the objects."""
+    root = os.path.dirname(HERE)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    exe = os.path.join(OUT_DIR, "dropin_driver")
+    _run([CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{root}", os.path.join(root, "tests", "cpp", "dropin_driver.cpp"),
+          f"-L{os.path.dirname(LIB)}", "-lwavedg_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe])
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_dropin_driver(verbose=True)

@@ -1,0 +1,448 @@
+// C++ drop-in classes for the 2D wave path of waveDG on MI355X.
+//
+// Same names, constructor arguments and semantics as the reference
+// (wavedg/Operator.hpp, WaveEquation.hpp, MassMatrix.hpp, WaveHoltz.hpp,
+// Mesh2D.hpp, QuadratureRule.hpp), in namespace dg::hip, implemented on the
+// C ABI of libwavedg_hip.so only (include/wavedg_hip.h).  Errors are thrown as
+// dg::hip::Error (a std::runtime_error), as wdg_error throws in the
+// reference's serial build (source/wdg_error.cpp:5-20).
+//
+// Arrays passed to the reference-style methods (action, inv, pi0, S,
+// postprocess) are host arrays in the reference layout (3, n, n, n_elem); each
+// call copies them to the device and back.  The *_device methods take device
+// pointers (hipMalloc'd) and an optional hipStream_t and do no copies; they
+// are the ones to use in time loops.
+//
+// Only approx_quad = true with a Gauss-Lobatto basis is implemented on the
+// GPU; asking for approx_quad = false throws.
+#pragma once
+
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/wavedg_hip.h"
+
+namespace dg {
+namespace hip {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+    int code;
+};
+
+inline void check(int rc)
+{
+    if (rc != WDG_OK) throw Error(rc, wdg_hip_last_error());
+}
+
+enum class FaceType { INTERIOR = WDG_INTERIOR, BOUNDARY = WDG_BOUNDARY };
+enum class Precision { F64 = WDG_F64, F32 = WDG_F32 };
+
+// Device buffer owned by the drop-in objects for host-array calls.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes) { resize(bytes); }
+    ~DeviceBuffer()
+    {
+        if (p_) wdg_hip_free(p_);
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    void resize(size_t bytes)
+    {
+        if (bytes <= n_) return;
+        if (p_) wdg_hip_free(p_);
+        p_ = nullptr;
+        check(wdg_hip_malloc(&p_, bytes));
+        n_ = bytes;
+    }
+    void* get() const { return p_; }
+    void upload(const void* src, size_t bytes) { check(wdg_hip_memcpy_htod(p_, src, bytes)); }
+    void download(void* dst, size_t bytes) const { check(wdg_hip_memcpy_dtoh(dst, p_, bytes)); }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// QuadratureRule::quadrature_rule(n, GaussLobatto) (QuadratureRule.cpp:164-184).
+class QuadratureRule {
+public:
+    enum QuadratureType { GaussLegendre, GaussLobatto };
+    static const QuadratureRule* quadrature_rule(int n, QuadratureType type = GaussLobatto)
+    {
+        if (type != GaussLobatto) throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
+        static std::unordered_map<int, std::unique_ptr<QuadratureRule>> cache;
+        auto it = cache.find(n);
+        if (it == cache.end()) it = cache.emplace(n, std::unique_ptr<QuadratureRule>(new QuadratureRule(n))).first;
+        return it->second.get();
+    }
+    const int n;
+    const double* x;
+    const double* w;
+    const QuadratureType type = GaussLobatto;
+
+private:
+    explicit QuadratureRule(int n_) : n(n_), x_(n_), w_(n_), D_(n_ * n_)
+    {
+        check(wdg_hip_gll(n_, x_.data(), w_.data(), D_.data()));
+        x = x_.data();
+        w = w_.data();
+    }
+    std::vector<double> x_, w_, D_;
+};
+
+// Mesh2D (Mesh2D.hpp:339-355, Mesh2D.cpp:5-163).
+class Mesh2D {
+public:
+    static Mesh2D from_vertices(int nx, const double* x, int nel, const int* elems)
+    {
+        wdg_mesh m = nullptr;
+        check(wdg_hip_mesh_from_vertices(nx, x, nel, elems, &m));
+        return Mesh2D(m);
+    }
+    static Mesh2D uniform_rect(int nx, double ax, double bx, int ny, double ay, double by)
+    {
+        wdg_mesh m = nullptr;
+        check(wdg_hip_mesh_uniform_rect(nx, ax, bx, ny, ay, by, &m));
+        return Mesh2D(m);
+    }
+    static Mesh2D from_file(const std::string& dir)
+    {
+        wdg_mesh m = nullptr;
+        check(wdg_hip_mesh_from_file(dir.c_str(), &m));
+        return Mesh2D(m);
+    }
+    int n_elem() const { return counts()[0]; }
+    int n_edges(FaceType t) const { return counts()[t == FaceType::INTERIOR ? 1 : 2]; }
+    double min_h() const
+    {
+        double h = 0;
+        check(wdg_hip_mesh_min_h(h_.get(), &h));
+        return h;
+    }
+    // element_metrics(basis).physical_coordinates(): (2, n, n, n_elem)
+    std::vector<double> physical_coordinates(const QuadratureRule* basis) const
+    {
+        std::vector<double> x(2 * (size_t)basis->n * basis->n * n_elem());
+        check(wdg_hip_mesh_nodes(h_.get(), basis->n, x.data()));
+        return x;
+    }
+    // (5, n_edges): el0, side0, el1, side1, delta in Mesh2D::edge(i, type) order
+    std::vector<int> edges(FaceType t) const
+    {
+        std::vector<int> out(5 * (size_t)n_edges(t));
+        check(wdg_hip_mesh_edges(h_.get(), (int)t, out.data()));
+        return out;
+    }
+    // (5, n_edges): nx, ny, ds, x_mid, y_mid
+    std::vector<double> edge_geometry(FaceType t) const
+    {
+        std::vector<double> out(5 * (size_t)n_edges(t));
+        check(wdg_hip_mesh_edge_geometry(h_.get(), (int)t, out.data()));
+        return out;
+    }
+    wdg_mesh handle() const { return h_.get(); }
+
+private:
+    explicit Mesh2D(wdg_mesh m) : h_(m, [](wdg_mesh p) { wdg_hip_mesh_destroy(p); }) {}
+    std::vector<int> counts() const
+    {
+        std::vector<int> c(3);
+        check(wdg_hip_mesh_counts(h_.get(), &c[0], &c[1], &c[2]));
+        return c;
+    }
+    std::shared_ptr<wdg_mesh_s> h_;
+};
+
+// Operator / InvertibleOperator (Operator.hpp:10-43).
+class Operator {
+public:
+    virtual ~Operator() = default;
+    virtual void action(const double* x, double* y) const = 0;
+    virtual void action(int n_var, const double* x, double* y) const = 0;
+};
+
+class InvertibleOperator : public Operator {
+public:
+    virtual void inv(double* x) const = 0;
+    virtual void inv(int n_var, double* x) const = 0;
+};
+
+namespace detail {
+// One device discretisation (geometry, connectivity, basis) per object.
+class Wave2D {
+public:
+    Wave2D(const Mesh2D& mesh, const QuadratureRule* basis, const int* bc, double a, double b, Precision p)
+    {
+        wdg_wave2d h = nullptr;
+        check(wdg_hip_wave2d_create(mesh.handle(), basis->n, bc, a, b, (int)p, &h));
+        h_.reset(h);
+        check(wdg_hip_wave2d_info(h, &ndof_, nullptr, nullptr, nullptr));
+        rsz_ = p == Precision::F32 ? 4 : 8;
+    }
+    wdg_wave2d get() const { return h_.get(); }
+    long long ndof() const { return ndof_; }
+    size_t state_bytes() const { return (size_t)ndof_ * rsz_; }
+    mutable DeviceBuffer x, y;
+
+private:
+    struct Del {
+        void operator()(wdg_wave2d p) const { wdg_hip_wave2d_destroy(p); }
+    };
+    std::unique_ptr<wdg_wave2d_s, Del> h_;
+    long long ndof_ = 0;
+    size_t rsz_ = 8;
+};
+
+inline void require_approx(bool approx_quad)
+{
+    if (!approx_quad)
+        throw std::invalid_argument("dg::hip: only approx_quad = true (collocated Gauss-Lobatto quadrature) is implemented");
+}
+}  // namespace detail
+
+// WaveEquation (WaveEquation.hpp:28-52): Dw <- Dw + A w, interior faces only.
+class WaveEquation : public Operator {
+public:
+    WaveEquation(const Mesh2D& mesh, const QuadratureRule* basis, bool approx_quad = false,
+                 const QuadratureRule* quad = nullptr, double a = 1.0, double b = 0.5)
+        : op_((detail::require_approx(approx_quad), mesh), basis, nullptr, a, b, Precision::F64)
+    {
+        (void)quad;
+    }
+    void action(const double* w, double* Dw) const override
+    {
+        const size_t bytes = op_.state_bytes();
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.x.upload(w, bytes);
+        op_.y.upload(Dw, bytes);
+        action_device(op_.x.get(), op_.y.get());
+        op_.y.download(Dw, bytes);
+    }
+    void action(int, const double* w, double* Dw) const override { action(w, Dw); }
+    void action_device(const void* d_w, void* d_Dw, void* stream = nullptr) const
+    {
+        check(wdg_hip_wave2d_action(op_.get(), d_w, d_Dw, stream));
+    }
+
+private:
+    detail::Wave2D op_;
+};
+
+// WaveBC (WaveEquation.hpp:102-122): Bw <- Bw + B w.  bc codes per boundary
+// edge: 0 absorbing, 1 reflecting, 2 Dirichlet p = 0.
+class WaveBC : public Operator {
+public:
+    WaveBC(const Mesh2D& mesh, const int* bc, const QuadratureRule* basis, bool approx_quad,
+           const QuadratureRule* quad = nullptr, double a = 1.0, double b = 0.5)
+        : op_((detail::require_approx(approx_quad), mesh), basis, bc, a, b, Precision::F64)
+    {
+        (void)quad;
+    }
+    void action(const double* w, double* Bw) const override
+    {
+        const size_t bytes = op_.state_bytes();
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.x.upload(w, bytes);
+        op_.y.upload(Bw, bytes);
+        action_device(op_.x.get(), op_.y.get());
+        op_.y.download(Bw, bytes);
+    }
+    void action(int, const double* w, double* Bw) const override { action(w, Bw); }
+    void action_device(const void* d_w, void* d_Bw, void* stream = nullptr) const
+    {
+        check(wdg_hip_wavebc_action(op_.get(), d_w, d_Bw, stream));
+    }
+
+private:
+    detail::Wave2D op_;
+};
+
+// MassMatrix<true> (MassMatrix.hpp:77, MassMatrix.cpp:6-82); n_var is 3.
+template <bool ApproxQuadrature>
+class MassMatrix : public InvertibleOperator {
+    static_assert(ApproxQuadrature, "dg::hip: only MassMatrix<true> is implemented");
+
+public:
+    MassMatrix(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+    {
+        (void)quad;
+    }
+    void action(const double* x, double* y) const override
+    {
+        const size_t bytes = op_.state_bytes();
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.x.upload(x, bytes);
+        check(wdg_hip_mass_action(op_.get(), op_.x.get(), op_.y.get(), nullptr));
+        op_.y.download(y, bytes);
+    }
+    void action(int, const double* x, double* y) const override { action(x, y); }
+    void inv(double* x) const override
+    {
+        const size_t bytes = op_.state_bytes();
+        op_.x.resize(bytes);
+        op_.x.upload(x, bytes);
+        inv_device(op_.x.get());
+        op_.x.download(x, bytes);
+    }
+    void inv(int, double* x) const override { inv(x); }
+    void inv_device(void* d_x, void* stream = nullptr) const { check(wdg_hip_mass_inv(op_.get(), d_x, stream)); }
+
+private:
+    detail::Wave2D op_;
+};
+
+// The semi-discrete wave system u' = M^{-1}(A + B) u advanced by the
+// low-storage RK4 of ode::RungeKutta4 (ode.hpp:46-93), one fused kernel per
+// stage.  Arrays are device arrays of the chosen precision.
+class WaveSystem {
+public:
+    WaveSystem(const Mesh2D& mesh, const QuadratureRule* basis, const int* bc, Precision p = Precision::F64,
+               double a = 1.0, double b = 0.5)
+        : op_(mesh, basis, bc, a, b, p)
+    {
+    }
+    long long ndof() const { return op_.ndof(); }
+    // d_y <- M^{-1}(A d_x + B d_x + s d_F)
+    void rhs_device(const void* d_x, void* d_y, const void* d_F = nullptr, double s = 0.0,
+                    void* stream = nullptr) const
+    {
+        check(wdg_hip_wave2d_rhs(op_.get(), d_x, d_y, d_F, s, stream));
+    }
+    // n RungeKutta4::step calls; t advances
+    void rk4_steps_device(double dt, double& t, void* d_u, int n, void* stream = nullptr) const
+    {
+        check(wdg_hip_rk4_steps(op_.get(), dt, &t, d_u, n, nullptr, 0.0, stream));
+    }
+    wdg_wave2d handle() const { return op_.get(); }
+
+private:
+    detail::Wave2D op_;
+};
+
+// WaveHoltz (WaveHoltz.hpp:35-90, WaveHoltz.cpp:5-145).  With Precision::F32
+// the RK state runs in float while the filter accumulation, the fixed-point
+// update and the norms stay in double; the arrays passed in are always double.
+class WaveHoltz {
+public:
+    WaveHoltz(double omega, const Mesh2D& mesh, const QuadratureRule* basis, const int* boundary_conditions,
+              bool approx_quad = false, Precision p = Precision::F64)
+        : omega_(omega), op_((detail::require_approx(approx_quad), mesh), basis, boundary_conditions, 1.0, 0.5, p)
+    {
+        wdg_waveholtz h = nullptr;
+        check(wdg_hip_waveholtz_create(op_.get(), omega, &h));
+        wh_.reset(h);
+        check(wdg_hip_waveholtz_info(h, &T_, &dt_, &nt_));
+    }
+    double K(double t) const { return omega_ / M_PI * (std::cos(omega_ * t) - 0.25); }
+    double T() const { return T_; }
+    double dt() const { return dt_; }
+    int nt() const { return nt_; }
+    long long ndof() const { return op_.ndof(); }
+
+    void pi0(double* w, const double* F) const
+    {
+        const size_t bytes = 8 * (size_t)op_.ndof();
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.y.upload(F, bytes);
+        pi0_device(static_cast<double*>(op_.x.get()), static_cast<const double*>(op_.y.get()));
+        op_.x.download(w, bytes);
+    }
+    void S(double* w) const
+    {
+        const size_t bytes = 8 * (size_t)op_.ndof();
+        op_.x.resize(bytes);
+        op_.x.upload(w, bytes);
+        S_device(static_cast<double*>(op_.x.get()));
+        op_.x.download(w, bytes);
+    }
+    void postprocess(double* H, const double* w) const
+    {
+        const size_t bytes = 8 * (size_t)op_.ndof();
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.x.upload(w, bytes);
+        check(wdg_hip_waveholtz_postprocess(wh_.get(), static_cast<double*>(op_.y.get()),
+                                            static_cast<const double*>(op_.x.get()), nullptr));
+        op_.y.download(H, 2 * bytes / 3);
+    }
+
+    void pi0_device(double* d_w, const double* d_F, void* stream = nullptr) const
+    {
+        check(wdg_hip_waveholtz_pi0(wh_.get(), d_w, d_F, stream));
+    }
+    void S_device(double* d_w, void* stream = nullptr) const { check(wdg_hip_waveholtz_S(wh_.get(), d_w, stream)); }
+    // d_w <- S d_w + d_pi0; returns ||w_new - w_old|| / pi0_norm (waveholtz2d.cpp:165-185)
+    double iterate_device(double* d_w, const double* d_pi0, double pi0_norm, void* stream = nullptr) const
+    {
+        double r = 0.0;
+        check(wdg_hip_waveholtz_iterate(wh_.get(), d_w, d_pi0, pi0_norm, &r, stream));
+        return r;
+    }
+    wdg_wave2d wave_handle() const { return op_.get(); }
+
+private:
+    struct Del {
+        void operator()(wdg_waveholtz p) const { wdg_hip_waveholtz_destroy(p); }
+    };
+    double omega_;
+    detail::Wave2D op_;
+    std::unique_ptr<wdg_waveholtz_s, Del> wh_;
+    double T_ = 0, dt_ = 0;
+    int nt_ = 0;
+};
+
+// LinearFunctional2D (LinearFunctional.hpp:136-202) with Gauss-Legendre(n)
+// quadrature, for forcing functions that are sums of Gaussians in one
+// variable: f(x) = sum_k amp_k exp(-|x - c_k|^2 / (2 sigma_k^2)).  sources is
+// (4, n_sources): cx, cy, sigma, amp.
+class LinearFunctional2D {
+public:
+    LinearFunctional2D(const Mesh2D& mesh, const QuadratureRule* basis)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+    {
+    }
+    void gaussians(int n_sources, const double* sources, int var, double* F) const
+    {
+        const size_t bytes = 8 * (size_t)op_.ndof();
+        op_.x.resize(bytes);
+        gaussians_device(n_sources, sources, var, static_cast<double*>(op_.x.get()));
+        op_.x.download(F, bytes);
+    }
+    void gaussians_device(int n_sources, const double* sources, int var, double* d_F, void* stream = nullptr) const
+    {
+        check(wdg_hip_load_vector_gauss(op_.get(), n_sources, sources, var, d_F, stream));
+    }
+
+private:
+    detail::Wave2D op_;
+};
+
+// norm / error (linalg.cpp:102-130) of device arrays, in double.
+inline double norm_device(long long n, const double* d_x, void* stream = nullptr)
+{
+    double r = 0;
+    check(wdg_hip_norm(d_x, n, &r, stream));
+    return r;
+}
+inline double error_device(long long n, const double* d_x, const double* d_y, void* stream = nullptr)
+{
+    double r = 0;
+    check(wdg_hip_error(d_x, d_y, n, &r, stream));
+    return r;
+}
+
+}  // namespace hip
+}  // namespace dg
