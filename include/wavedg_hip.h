@@ -153,6 +153,17 @@ int wdg_hip_waveholtz_S(wdg_waveholtz wh, double *d_w, void *stream);
  * (linalg.cpp:116-130), all in double on the device.  Synchronises `stream`. */
 int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double *d_w, const double *d_pi0, double pi0_norm,
                               double *rel_change, void *stream);
+/* The caller loop of examples/waveholtz2d.cpp:165-185 run to a tolerance:
+ * iterate with `wh` until rel_change <= tol, or, when `wh_refine` is not NULL,
+ * until the residual stops decreasing (it stalls at a precision floor when
+ * `wh` runs an FP32 state), then continue with `wh_refine` (an FP64 handle on
+ * the same mesh, order and omega) until rel_change <= tol.  iters[0] counts
+ * the `wh` iterations, iters[1] the refinement iterations; *final_residual is
+ * the last rel_change.  Returns WDG_OK even if max_iter is reached (check
+ * *final_residual). */
+int wdg_hip_waveholtz_solve(wdg_waveholtz wh, wdg_waveholtz wh_refine, double *d_w, const double *d_pi0,
+                            double pi0_norm, double tol, int max_iter, int *iters, double *final_residual,
+                            void *stream);
 /* WaveHoltz::postprocess (WaveHoltz.cpp:123-145): d_H (2, n, n, n_elem). */
 int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double *d_H, const double *d_w, void *stream);
 

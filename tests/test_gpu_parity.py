@@ -223,3 +223,31 @@ def test_errors_are_raised(cuda):
         gw.action(x, x)
     with pytest.raises(capi.WdgError):
         gw.rk4(x, -1.0, 1)
+
+
+def test_waveholtz_solve_mixed_refines_to_fp64_fixed_point(cuda):
+    torch = cuda
+    v, e = meshgen.l_shape(12)
+    gm = capi.Mesh(v, e)
+    bc = np.zeros(gm.n_boundary, dtype=np.int32)
+    n, omega = 5, 4.0 * np.pi
+    w64 = capi.Wave2D(gm, n, bc)
+    w32 = capi.Wave2D(gm, n, bc, precision=capi.F32)
+    h64, h32 = capi.WaveHoltz(w64, omega), capi.WaveHoltz(w32, omega)
+    print(f"solve case nt={h64.nt}", flush=True)
+    F = torch.empty(w64.ndof, dtype=torch.float64, device="cuda")
+    w64.load_vector_gauss([[0.25, 0.3, 0.05, 1.0]], F)
+    pi0 = torch.zeros_like(F)
+    h64.pi0(pi0, F)
+    pn = capi.norm(pi0)
+    A = torch.zeros_like(F)
+    it_a, it_ar, ra = h64.solve(A, pi0, pn, 1e-9, 400)
+    B = torch.zeros_like(F)
+    it_b, it_br, rb = h32.solve(B, pi0, pn, 1e-9, 400, refine=h64)
+    print(f"f64: {it_a} its r={ra:.2e}; mixed: {it_b}+{it_br} its r={rb:.2e}")
+    assert ra <= 1e-9 and rb <= 1e-9 and it_ar == 0 and it_br > 0
+    assert float(torch.linalg.norm(A - B) / torch.linalg.norm(A)) < 1e-7
+    # the first phase alone stalls above the tolerance
+    C = torch.zeros_like(F)
+    it_c, _, rc = h32.solve(C, pi0, pn, 1e-9, 60)
+    assert rc > 1e-9

@@ -65,6 +65,7 @@ _SIGS = {
     "wdg_hip_waveholtz_S": [_vp, _vp, _vp],
     "wdg_hip_waveholtz_iterate": [_vp, _vp, _vp, _d, _dp, _vp],
     "wdg_hip_waveholtz_postprocess": [_vp, _vp, _vp, _vp],
+    "wdg_hip_waveholtz_solve": [_vp, _vp, _vp, _vp, _d, _d, _i, _ip, _dp, _vp],
     "wdg_hip_load_vector_gauss": [_vp, _i, _dp, _i, _vp, _vp],
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
@@ -303,6 +304,17 @@ class WaveHoltz:
         _call("wdg_hip_waveholtz_iterate", self.h, dev_ptr(w), dev_ptr(pi0), pi0_norm, ctypes.byref(r),
               stream if stream is not None else cur_stream())
         return r.value
+
+    def solve(self, w, pi0, pi0_norm: float, tol: float, max_iter: int, refine: "WaveHoltz | None" = None,
+              stream=None):
+        """Iterate to tol; with `refine` (an FP64 WaveHoltz on the same problem)
+        switch to it when this handle's residual stalls.  Returns
+        (iterations, refinement iterations, final residual)."""
+        it = (ctypes.c_int * 2)()
+        r = _d()
+        _call("wdg_hip_waveholtz_solve", self.h, refine.h if refine is not None else None, dev_ptr(w), dev_ptr(pi0),
+              pi0_norm, tol, max_iter, it, ctypes.byref(r), stream if stream is not None else cur_stream())
+        return it[0], it[1], r.value
 
     def postprocess(self, H, w, stream=None):
         _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),

@@ -941,6 +941,49 @@ int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double* d_w, const double* d_pi0
     });
 }
 
+int wdg_hip_waveholtz_solve(wdg_waveholtz wh, wdg_waveholtz wh_refine, double* d_w, const double* d_pi0,
+                            double pi0_norm, double tol, int max_iter, int* iters, double* final_residual,
+                            void* stream)
+{
+    return guarded([&] {
+        check_wh(wh);
+        if (!d_w || !d_pi0 || !iters || !final_residual) fail(WDG_ERR_INVALID, "null argument");
+        if (!(tol > 0.0) || max_iter < 1) fail(WDG_ERR_INVALID, "tol must be > 0 and max_iter >= 1");
+        if (wh_refine) {
+            check_wh(wh_refine);
+            if (wh_refine->op->ndof != wh->op->ndof || wh_refine->nt != wh->nt || wh_refine->omega != wh->omega)
+                fail(WDG_ERR_INVALID, "refinement handle must discretise the same problem");
+        }
+        iters[0] = iters[1] = 0;
+        double r = 0.0, best = INFINITY;
+        int since_best = 0;
+        int rc = WDG_OK;
+        // first phase: stop at tol, or (with a refinement handle) when the
+        // residual has not improved on its best value for 3 iterations
+        while (iters[0] + iters[1] < max_iter) {
+            rc = wdg_hip_waveholtz_iterate(wh, d_w, d_pi0, pi0_norm, &r, stream);
+            if (rc != WDG_OK) fail(rc, g_last_error);
+            ++iters[0];
+            if (r <= tol) break;
+            if (r < 0.97 * best) {
+                best = r;
+                since_best = 0;
+            } else if (++since_best >= 3 && wh_refine) {
+                break;
+            }
+        }
+        if (wh_refine && r > tol) {
+            while (iters[0] + iters[1] < max_iter) {
+                rc = wdg_hip_waveholtz_iterate(wh_refine, d_w, d_pi0, pi0_norm, &r, stream);
+                if (rc != WDG_OK) fail(rc, g_last_error);
+                ++iters[1];
+                if (r <= tol) break;
+            }
+        }
+        *final_residual = r;
+    });
+}
+
 int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double* d_H, const double* d_w, void* stream)
 {
     return guarded([&] {
