@@ -146,3 +146,11 @@ def test_host_mesh_errors():
     with pytest.raises(capi.WdgError) as ei:
         capi.Mesh.from_file("/nonexistent/mesh")
     assert ei.value.code == 4
+
+
+@pytest.mark.parametrize("name", MESHES)
+def test_min_h_is_shortest_edge(name):
+    v, e = mesh_case(name)
+    m = O.Mesh(v, e)
+    lengths = [np.hypot(*(v[q[(k + 1) % 4]] - v[q[k]])) for q in e for k in range(4)]
+    assert m.min_h() == min(lengths)
