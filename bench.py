@@ -41,6 +41,17 @@ def stage_bytes(ndof: int, nel: int, s: int) -> float:
     return ndof * s * (3 + 4 * 4) / 5.0 + nel * 80.0
 
 
+def measured_traffic(nx: int):
+    """HBM bytes per stage launch (average of the 5 stages of a step) from the
+    committed rocprofv3 PMC summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "wave_stage_p7_skewed512_traffic.json")
+    if nx != 512 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        b = json.load(f)["bytes_per_launch"]
+    return (4 * b["f64_stage"] + b["f64_stage0"]) / 5.0
+
+
 def build_workload(nx: int):
     from wavedg_amd import capi, meshgen
 
@@ -267,7 +278,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.nx),
             "kernel": "wave_kernel<8,double,stage> (5 fused RK stages per step)",
             "bytes_per_launch": bytes64,
             "avg_launch_us": avg_ms * 1e3,
