@@ -167,6 +167,40 @@ int wdg_hip_waveholtz_solve(wdg_waveholtz wh, wdg_waveholtz wh_refine, double *d
 /* WaveHoltz::postprocess (WaveHoltz.cpp:123-145): d_H (2, n, n, n_elem). */
 int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double *d_H, const double *d_w, void *stream);
 
+/* ---- batched WaveHoltz -------------------------------------------------
+ * Many independent WaveHoltz problems (own mesh, order, bc, omega, forcing)
+ * solved together: one kernel launch per RK stage covers every problem,
+ * with steps replayed from a captured HIP graph.  Per problem the sequence
+ * is the reference's: pi0 (WaveHoltz.cpp:55-89), then W <- S(W) + pi0 with
+ * the residual ||W_k - W_{k-1}|| / ||pi0|| (examples/waveholtz2d.cpp:165-185)
+ * until it is <= tol, in double for the filter, update and residual (the RK
+ * state in the batch precision).  Upwind flux.  The first update gives
+ * W_1 = S(0) + pi0 = pi0 without applying S to zero, or, after
+ * wdg_hip_batch_set_initial, W_1 = the given initial iterate (warm start).
+ * Iteration counts are the number of W updates, as in the example loop. */
+typedef struct wdg_batch_s *wdg_batch;
+typedef struct {
+    wdg_mesh mesh;          /* borrowed for the create call only */
+    int n_colloc;           /* Gauss-Lobatto points per direction, 2..9 */
+    const int *bc;          /* one code per boundary edge, NULL = reflecting */
+    double omega;
+    int n_sources;          /* Gaussian sources of the p-equation load vector */
+    const double *sources;  /* (4, n_sources): cx, cy, sigma, amp */
+} wdg_batch_problem;
+int wdg_hip_batch_create(int n_problems, const wdg_batch_problem *problems, int precision, wdg_batch *out);
+int wdg_hip_batch_destroy(wdg_batch b);
+int wdg_hip_batch_info(wdg_batch b, int k, long long *ndof, int *nt, int *n_colloc);
+/* Initial iterate of problem k (device, double, ndof); NULL clears it. */
+int wdg_hip_batch_set_initial(wdg_batch b, int k, const double *d_W0);
+/* Runs every problem to tol (or max_iter updates), steps_per_graph RK steps
+ * per graph replay.  Synchronises `stream`. */
+int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_graph, void *stream);
+/* Solution (device, double, ndof; may be NULL), number of W updates and the
+ * last residual of problem k. */
+int wdg_hip_batch_result(wdg_batch b, int k, double *d_W, int *iterations, double *residual);
+/* Wall time (HIP events) and RK steps of the last wdg_hip_batch_solve. */
+int wdg_hip_batch_timing(wdg_batch b, double *seconds, long long *steps);
+
 /* ---- load vector -----------------------------------------------------
  * LinearFunctional2D::action (LinearFunctional.hpp:136-202) with
  * Gauss-Legendre(n_colloc) quadrature, for f = sum of Gaussians in variable

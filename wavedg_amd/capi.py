@@ -67,6 +67,13 @@ _SIGS = {
     "wdg_hip_waveholtz_postprocess": [_vp, _vp, _vp, _vp],
     "wdg_hip_waveholtz_solve": [_vp, _vp, _vp, _vp, _d, _d, _i, _ip, _dp, _vp],
     "wdg_hip_load_vector_gauss": [_vp, _i, _dp, _i, _vp, _vp],
+    "wdg_hip_batch_create": [_i, _vp, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_batch_destroy": [_vp],
+    "wdg_hip_batch_info": [_vp, _i, ctypes.POINTER(_ll), _ip, _ip],
+    "wdg_hip_batch_set_initial": [_vp, _i, _vp],
+    "wdg_hip_batch_solve": [_vp, _d, _i, _i, _vp],
+    "wdg_hip_batch_result": [_vp, _i, _vp, _ip, _dp],
+    "wdg_hip_batch_timing": [_vp, _dp, ctypes.POINTER(_ll)],
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
     "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
@@ -319,6 +326,57 @@ class WaveHoltz:
     def postprocess(self, H, w, stream=None):
         _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),
               stream if stream is not None else cur_stream())
+
+
+class _BatchProblem(ctypes.Structure):
+    _fields_ = [("mesh", _vp), ("n_colloc", _i), ("bc", _ip), ("omega", _d), ("n_sources", _i), ("sources", _dp)]
+
+
+class Batch:
+    """Batched WaveHoltz over independent problems.  Each problem is a dict
+    with keys mesh (Mesh), n, omega, sources ((k, 4) array) and optional bc."""
+
+    def __init__(self, problems, precision: int = F64):
+        arr = (_BatchProblem * len(problems))()
+        self._keep = []
+        for k, pb in enumerate(problems):
+            src = np.ascontiguousarray(np.asarray(pb["sources"], dtype=np.float64).reshape(-1, 4))
+            bc = pb.get("bc")
+            bcp = None
+            if bc is not None:
+                bc = np.ascontiguousarray(np.asarray(bc, dtype=np.int32))
+                bcp = _iptr(bc)
+            self._keep += [src, bc]
+            arr[k] = _BatchProblem(pb["mesh"].h, pb["n"], bcp, pb["omega"], src.shape[0], _dptr(src))
+        h = _vp()
+        _call("wdg_hip_batch_create", len(problems), ctypes.cast(arr, _vp), precision, ctypes.byref(h))
+        self.h = h
+        self.n = len(problems)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_batch_destroy(self.h)
+            self.h = None
+
+    def info(self, k: int):
+        nd, nt, n = _ll(), _i(), _i()
+        _call("wdg_hip_batch_info", self.h, k, ctypes.byref(nd), ctypes.byref(nt), ctypes.byref(n))
+        return nd.value, nt.value, n.value
+
+    def set_initial(self, k: int, W0=None):
+        _call("wdg_hip_batch_set_initial", self.h, k, dev_ptr(W0))
+
+    def solve(self, tol: float, max_iter: int, steps_per_graph: int = 16, stream=None):
+        _call("wdg_hip_batch_solve", self.h, tol, max_iter, steps_per_graph,
+              stream if stream is not None else cur_stream())
+        sec, steps = _d(), _ll()
+        _call("wdg_hip_batch_timing", self.h, ctypes.byref(sec), ctypes.byref(steps))
+        return sec.value, steps.value
+
+    def result(self, k: int, W=None):
+        it, r = _i(), _d()
+        _call("wdg_hip_batch_result", self.h, k, dev_ptr(W), ctypes.byref(it), ctypes.byref(r))
+        return it.value, r.value
 
 
 def norm(x, stream=None) -> float:

@@ -59,4 +59,12 @@ private:
     std::vector<int> interior_, boundary_;
 };
 
+// Thread-local message for the C ABI's wdg_hip_last_error().
+extern thread_local std::string g_last_error;
+
 }  // namespace wdg
+
+// Opaque mesh handle of the C ABI.
+struct wdg_mesh_s {
+    wdg::HostMesh mesh;
+};

@@ -22,8 +22,12 @@
 using wdg::KArgs;
 
 // ------------------------------------------------------------------ errors
-namespace {
+namespace wdg {
 thread_local std::string g_last_error;
+}
+using wdg::g_last_error;
+
+namespace {
 
 struct Status : std::exception {
     int code;
@@ -94,9 +98,6 @@ constexpr int kRedThreads = 256;
 }  // namespace
 
 // ------------------------------------------------------------------ handles
-struct wdg_mesh_s {
-    wdg::HostMesh mesh;
-};
 
 struct wdg_wave2d_s {
     int dev = 0;

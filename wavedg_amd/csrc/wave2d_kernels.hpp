@@ -138,36 +138,62 @@ __device__ __forceinline__ int side_node(int s, int k)
     return i + N * j;
 }
 
-template <int N, typename R, int MODE>
-__global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const KArgs a)
+// LDS carve-up of apply_operator for one workgroup of Shape<N> elements.
+template <int N, typename R, bool VOL, bool FACE>
+struct OpLds {
+    static constexpr int NN = N * N, EPB = Shape<N>::EPB, NF = 4 * N;
+    static constexpr size_t n_u = (size_t)EPB * 3 * NN;
+    static constexpr size_t n_F = VOL ? n_u : 0, n_G = VOL ? n_u : 0;
+    static constexpr size_t n_S = FACE ? (size_t)EPB * NF * 3 : 0;
+    static constexpr size_t off_xw = 0;                                  // double x[N], w[N]
+    static constexpr size_t off_u = ((2 * N * 8 + 15) / 16) * 16;
+    static constexpr size_t off_F = off_u + n_u * sizeof(R);
+    static constexpr size_t off_G = off_F + n_F * sizeof(R);
+    static constexpr size_t off_S = off_G + n_G * sizeof(R);
+    static constexpr size_t off_D = off_S + n_S * sizeof(R);
+    static constexpr size_t bytes = ((off_D + NN * sizeof(R) + 15) / 16) * 16;
+};
+
+// Result of the operator at one node: own state, the weak-form right-hand
+// side r = (A + B) u before M^{-1}, and the mass-matrix entry w_i w_j det J.
+template <typename R>
+struct NodeResult {
+    R p, uu, vv;
+    R r0, r1, r2;
+    double m;
+};
+
+// The operator for one workgroup of elements [e0, e0 + nloc) of one mesh,
+// run by threads 0 .. Shape<N>::NT-1 of the workgroup (threads beyond only
+// join the barriers).  Element indices in nbr are local to u_in / geom.
+// `prefetch(active, g)` is called after the state loads are issued and
+// before the first barrier, so the caller's extra loads overlap them.
+template <int N, typename R, bool VOL, bool IFACE, bool BFACE, typename Prefetch>
+__device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const double* __restrict__ geom,
+                                                        const int32_t* __restrict__ nbr,
+                                                        const int32_t* __restrict__ info,
+                                                        const double* __restrict__ basis, double ha_, double hb_,
+                                                        int e0, int nloc, unsigned char* lds, int& el_out,
+                                                        size_t& g_out, bool& active_out, Prefetch&& prefetch)
 {
     constexpr int NN = N * N;
     constexpr int EPB = Shape<N>::EPB;
     constexpr int NT = Shape<N>::NT;
-    constexpr int NF = 4 * N;                  // face nodes per element
-    constexpr int FP = (NF + NN - 1) / NN;     // face nodes per thread (1 for N >= 4)
-    constexpr bool VOL = (MODE & M_VOL) != 0;
-    constexpr bool IFACE = (MODE & M_IF) != 0;
-    constexpr bool BFACE = (MODE & M_BF) != 0;
+    constexpr int NF = 4 * N;               // face nodes per element
+    constexpr int FP = (NF + NN - 1) / NN;  // face nodes per thread (1 for N >= 4)
     constexpr bool FACE = IFACE || BFACE;
-    constexpr int OUT = MODE & M_OUTMASK;
-    constexpr bool FORCE = (MODE & M_FORCE) != 0;
-    constexpr bool ACC = (MODE & M_ACC) != 0;
-    constexpr bool READ_DU = OUT == M_OUT_STAGE;
-
-    __shared__ R s_u[EPB * 3 * NN];
-    __shared__ R s_F[VOL ? EPB * 3 * NN : 1];
-    __shared__ R s_G[VOL ? EPB * 3 * NN : 1];
-    __shared__ R s_S[FACE ? EPB * NF * 3 : 1];  // signed face contributions [el][side][k][var]
-    __shared__ R s_D[NN];
-    __shared__ double s_xw[2 * N];  // x[N], w[N]
+    using L = OpLds<N, R, VOL, FACE>;
+    double* s_xw = reinterpret_cast<double*>(lds + L::off_xw);
+    R* s_u = reinterpret_cast<R*>(lds + L::off_u);
+    R* s_F = reinterpret_cast<R*>(lds + L::off_F);
+    R* s_G = reinterpret_cast<R*>(lds + L::off_G);
+    R* s_S = reinterpret_cast<R*>(lds + L::off_S);
+    R* s_D = reinterpret_cast<R*>(lds + L::off_D);
     const double* s_x = s_xw;
     const double* s_w = s_xw + N;
 
-    const R* __restrict__ u_in = static_cast<const R*>(a.u_in);
     const int tid = threadIdx.x;
-    const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
-    const int nloc = min(EPB, a.nel - e0);
+    const bool worker = tid < NT;
 
     // ---- prologue: issue every global load before the first LDS write so
     // that they are all in flight together (clamped indices keep the
@@ -179,20 +205,23 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
 #pragma unroll
         for (int r = 0; r < 3; ++r) ld[r] = ub[min(tid + r * NT, cnt - 1)];
     }
-    const R dval = (R)a.basis[2 * N + min(tid, NN - 1)];
-    const double xwv = a.basis[min(tid, 2 * N - 1)];
-    const int el = tid / NN;
-    const int node = tid - el * NN;
+    const R dval = (R)basis[2 * N + min(tid, NN - 1)];
+    const double xwv = basis[min(tid, 2 * N - 1)];
+    const int el = worker ? tid / NN : 0;
+    const int node = worker ? tid - el * NN : 0;
     const int i = node % N, j = node / N;
-    const bool active = el < nloc;
+    const bool active = worker && el < nloc;
     int e = e0 + (active ? el : 0);
     if constexpr (NN % 64 == 0) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave
-    const size_t g = ((size_t)e * NN + node) * 3;
+    const size_t g = ((size_t)e * NN + (active ? node : 0)) * 3;
+    el_out = el;
+    g_out = g;
+    active_out = active;
 
     // Bilinear element map x(xi, eta) = x0 + a xi + c eta + h xi eta (per
     // coordinate), so dx/dxi = a + h eta, dx/deta = c + h xi (Element.cpp:21-27)
     // and side s runs along 2(a -+ h) (s = 0, 2) or 2(c +- h) (s = 1, 3).
-    const double2* gp = reinterpret_cast<const double2*>(a.geom + 6 * (size_t)e);
+    const double2* gp = reinterpret_cast<const double2*>(geom + 6 * (size_t)e);
     const double2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
     const double ax = g_a.x, ay = g_a.y, cx = g_c.x, cy = g_c.y, hx = g_h.x, hy = g_h.y;
 
@@ -202,8 +231,8 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
     if constexpr (FACE) {
         int4 nb4 = make_int4(0, 0, 0, 0), in4 = make_int4(0, 0, 0, 0);
         if constexpr (NN % 64 == 0) {
-            nb4 = *reinterpret_cast<const int4*>(a.nbr + 4 * (size_t)e);
-            in4 = *reinterpret_cast<const int4*>(a.info + 4 * (size_t)e);
+            nb4 = *reinterpret_cast<const int4*>(nbr + 4 * (size_t)e);
+            in4 = *reinterpret_cast<const int4*>(info + 4 * (size_t)e);
         }
 #pragma unroll
         for (int p = 0; p < FP; ++p) {
@@ -214,8 +243,8 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
                 fnb[p] = s == 0 ? nb4.x : s == 1 ? nb4.y : s == 2 ? nb4.z : nb4.w;
                 finfo[p] = s == 0 ? in4.x : s == 1 ? in4.y : s == 2 ? in4.z : in4.w;
             } else {
-                fnb[p] = a.nbr[4 * (size_t)e + s];
-                finfo[p] = a.info[4 * (size_t)e + s];
+                fnb[p] = nbr[4 * (size_t)e + s];
+                finfo[p] = info[4 * (size_t)e + s];
             }
             nbv[p][0] = nbv[p][1] = nbv[p][2] = (R)0;
             if (IFACE && fon && fnb[p] >= 0) {
@@ -228,63 +257,41 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
             }
         }
     }
-    R pre0 = 0, pre1 = 0, pre2 = 0;  // du (stage), y (accumulate)
-    if (active) {
-        if constexpr (READ_DU) {
-            const R* du = static_cast<const R*>(a.du);
-            pre0 = du[g + 0];
-            pre1 = du[g + 1];
-            pre2 = du[g + 2];
-        } else if constexpr (OUT == M_OUT_ACCUM) {
-            const R* y = static_cast<const R*>(a.u_out);
-            pre0 = y[g + 0];
-            pre1 = y[g + 1];
-            pre2 = y[g + 2];
-        }
-    }
-    R fv0 = 0, fv1 = 0, fv2 = 0;
-    if constexpr (FORCE) {
-        if (active) {
-            const R* Fv = static_cast<const R*>(a.force);
-            fv0 = Fv[g + 0];
-            fv1 = Fv[g + 1];
-            fv2 = Fv[g + 2];
-        }
-    }
-    double ac0 = 0, ac1 = 0, ac2 = 0;
-    if constexpr (ACC) {
-        if (active) {
-            ac0 = a.acc[g + 0];
-            ac1 = a.acc[g + 1];
-            ac2 = a.acc[g + 2];
-        }
-    }
+    prefetch(active, g);
+
+    if (worker) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) s_u[tid + r * NT] = ld[r];
-    if (tid < NN) s_D[tid] = dval;
-    if (tid < 2 * N) s_xw[tid] = xwv;
+        for (int r = 0; r < 3; ++r) s_u[tid + r * NT] = ld[r];
+        if (tid < NN) s_D[tid] = dval;
+        if (tid < 2 * N) s_xw[tid] = xwv;
+    }
     __syncthreads();
 
     const double xi = s_x[i], eta = s_x[j];
     const double J0 = ax + hx * eta, J1 = ay + hy * eta, J2 = cx + hx * xi, J3 = cy + hy * xi;
     const double Wt = s_w[j] * s_w[i];
 
-    const R p = s_u[el * 3 * NN + 3 * node + 0];
-    const R uu = s_u[el * 3 * NN + 3 * node + 1];
-    const R vv = s_u[el * 3 * NN + 3 * node + 2];
+    NodeResult<R> out;
+    out.p = s_u[el * 3 * NN + 3 * node + 0];
+    out.uu = s_u[el * 3 * NN + 3 * node + 1];
+    out.vv = s_u[el * 3 * NN + 3 * node + 2];
+    out.m = Wt * (J0 * J3 - J1 * J2);  // MassMatrix<true>: w_i w_j det J (MassMatrix.cpp:6-28)
+    const R p = out.p, uu = out.uu, vv = out.vv;
 
     if constexpr (VOL) {
         // Contravariant fluxes F = W (y_eta f - x_eta g), G = W (-y_xi f + x_xi g),
         // f = (u, p, 0), g = (v, 0, p) (Div.cpp:5-53).
-        const R Ye = (R)(J3 * Wt), Xe = (R)(J2 * Wt), Yx = (R)(J1 * Wt), Xx = (R)(J0 * Wt);
-        R* F = s_F + el * 3 * NN;
-        R* G = s_G + el * 3 * NN;
-        F[0 * NN + j * N + i] = Ye * uu - Xe * vv;
-        F[1 * NN + j * N + i] = Ye * p;
-        F[2 * NN + j * N + i] = -(Xe * p);
-        G[0 * NN + i * N + j] = Xx * vv - Yx * uu;
-        G[1 * NN + i * N + j] = -(Yx * p);
-        G[2 * NN + i * N + j] = Xx * p;
+        if (worker) {
+            const R Ye = (R)(J3 * Wt), Xe = (R)(J2 * Wt), Yx = (R)(J1 * Wt), Xx = (R)(J0 * Wt);
+            R* F = s_F + el * 3 * NN;
+            R* G = s_G + el * 3 * NN;
+            F[0 * NN + j * N + i] = Ye * uu - Xe * vv;
+            F[1 * NN + j * N + i] = Ye * p;
+            F[2 * NN + j * N + i] = -(Xe * p);
+            G[0 * NN + i * N + j] = Xx * vv - Yx * uu;
+            G[1 * NN + i * N + j] = -(Yx * p);
+            G[2 * NN + i * N + j] = Xx * p;
+        }
     }
 
     if constexpr (FACE) {
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
         // A and the other trace B, the contribution is
         //   c_p = w ds [ha n.(uA+uB) + hb (pA-pB)],  c_u = n w ds [ha (pA+pB) + hb n.(uA-uB)]
         // which equals +f on el0 and -f on el1 for the reference's edge flux f.
-        const R ha = (R)a.ha, hb = (R)a.hb;
+        const R ha = (R)ha_, hb = (R)hb_;
 #pragma unroll
         for (int pp = 0; pp < FP; ++pp) {
             const int f = node + pp * NN;
@@ -335,10 +342,10 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
                         uB = (n1 * n1 - n0 * n0) * uA - (R)2 * n0 * n1 * vA;
                         vB = -(R)2 * n0 * n1 * uA + (n0 * n0 - n1 * n1) * vA;
                     } else if (code == 0) {  // characteristic outflow, p_ext after u_ext (WaveEquation.hpp:77-84)
-                        const R out = pA + n0 * uA + n1 * vA;
+                        const R o = pA + n0 * uA + n1 * vA;
                         const R tang = n1 * uA - n0 * vA;
-                        uB = n1 * tang + (R)0.5 * n0 * out;
-                        vB = -n0 * tang + (R)0.5 * n1 * out;
+                        uB = n1 * tang + (R)0.5 * n0 * o;
+                        vB = -n0 * tang + (R)0.5 * n1 * o;
                         pB = n0 * uB + n1 * vB;
                     } else {  // Dirichlet p = 0 (mirror state)
                         pB = -pA;
@@ -399,16 +406,72 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
             r2 += S[(3 * N + j) * 3 + 2];
         }
     }
+    out.r0 = r0;
+    out.r1 = r1;
+    out.r2 = r2;
+    return out;
+}
 
+template <int N, typename R, int MODE>
+__global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const KArgs a)
+{
+    constexpr int EPB = Shape<N>::EPB;
+    constexpr bool VOL = (MODE & M_VOL) != 0;
+    constexpr bool IFACE = (MODE & M_IF) != 0;
+    constexpr bool BFACE = (MODE & M_BF) != 0;
+    constexpr int OUT = MODE & M_OUTMASK;
+    constexpr bool FORCE = (MODE & M_FORCE) != 0;
+    constexpr bool ACC = (MODE & M_ACC) != 0;
+    constexpr bool READ_DU = OUT == M_OUT_STAGE;
+    using L = OpLds<N, R, VOL, IFACE || BFACE>;
+    __shared__ __align__(16) unsigned char lds[L::bytes];
+
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
+    const int nloc = min(EPB, a.nel - e0);
+
+    // RK register / output / forcing / accumulator values, loaded early
+    R pre0 = 0, pre1 = 0, pre2 = 0, fv0 = 0, fv1 = 0, fv2 = 0;
+    double ac0 = 0, ac1 = 0, ac2 = 0;
+    auto prefetch = [&](bool act, size_t g) {
+        if (!act) return;
+        if constexpr (READ_DU) {
+            const R* du = static_cast<const R*>(a.du);
+            pre0 = du[g + 0];
+            pre1 = du[g + 1];
+            pre2 = du[g + 2];
+        } else if constexpr (OUT == M_OUT_ACCUM) {
+            const R* y = static_cast<const R*>(a.u_out);
+            pre0 = y[g + 0];
+            pre1 = y[g + 1];
+            pre2 = y[g + 2];
+        }
+        if constexpr (FORCE) {
+            const R* Fv = static_cast<const R*>(a.force);
+            fv0 = Fv[g + 0];
+            fv1 = Fv[g + 1];
+            fv2 = Fv[g + 2];
+        }
+        if constexpr (ACC) {
+            ac0 = a.acc[g + 0];
+            ac1 = a.acc[g + 1];
+            ac2 = a.acc[g + 2];
+        }
+    };
+    int el;
+    size_t g;
+    bool active;
+    NodeResult<R> q = apply_operator<N, R, VOL, IFACE, BFACE>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info,
+                                                              a.basis, a.ha, a.hb, e0, nloc, lds, el, g, active,
+                                                              prefetch);
     if (!active) return;
+    R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
         R* y = static_cast<R*>(a.u_out);
         y[g + 0] = pre0 + r0;
         y[g + 1] = pre1 + r1;
         y[g + 2] = pre2 + r2;
     } else {
-        // MassMatrix<true>: m = w_i w_j det J (MassMatrix.cpp:6-28)
-        const R im = (R)recip(Wt * (J0 * J3 - J1 * J2));
+        const R im = (R)recip(q.m);
         if constexpr (FORCE) {
             const R fs = (R)a.fs;
             r0 += fv0 * fs;
@@ -442,7 +505,7 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
             st_stream(du + g + 0, d0);
             st_stream(du + g + 1, d1);
             st_stream(du + g + 2, d2);
-            const R w0 = p + rb * d0, w1 = uu + rb * d1, w2 = vv + rb * d2;
+            const R w0 = q.p + rb * d0, w1 = q.uu + rb * d1, w2 = q.vv + rb * d2;
             st_stream(uo + g + 0, w0);
             st_stream(uo + g + 1, w1);
             st_stream(uo + g + 2, w2);
