@@ -179,6 +179,53 @@ def waveholtz_solve(torch, capi, mode: str, tol: float, max_iter: int, n_grid: i
     return W, stats
 
 
+def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_per_graph):
+    """config-4 style batch: every problem solved alone with the plain driver
+    (pi0, then W <- S(W) + pi0 to tol), then all of them in one batch."""
+    from wavedg_amd import meshgen
+
+    cases = meshgen.config4(count, nmin, nmax)
+    probs = []
+    for c in cases:
+        m = capi.Mesh(c["vertices"], c["elements"])
+        probs.append(dict(mesh=m, n=c["n"], omega=c["omega"], sources=c["sources"],
+                          bc=np.zeros(m.n_boundary, dtype=np.int32)))
+    # sequential plain solves
+    seq_W, seq_it = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for pb in probs:
+        w = capi.Wave2D(pb["mesh"], pb["n"], pb["bc"])
+        wh = capi.WaveHoltz(w, pb["omega"])
+        F = torch.empty(w.ndof, dtype=torch.float64, device="cuda")
+        w.load_vector_gauss(pb["sources"], F)
+        pi0 = torch.zeros_like(F)
+        wh.pi0(pi0, F)
+        W = torch.zeros_like(F)
+        it, _, _ = wh.solve(W, pi0, capi.norm(pi0), tol, max_iter)
+        seq_W.append(W)
+        seq_it.append(it)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    b = capi.Batch(probs)
+    t0 = time.perf_counter()
+    sec, steps = b.solve(tol, max_iter, steps_per_graph)
+    t_bat = time.perf_counter() - t0
+    worst, its = 0.0, []
+    for k in range(len(probs)):
+        W = torch.empty_like(seq_W[k])
+        it, r = b.result(k, W)
+        its.append(it)
+        worst = max(worst, float(torch.linalg.norm(W - seq_W[k]) / torch.linalg.norm(seq_W[k])))
+    ndofs = [b.info(k)[0] for k in range(len(probs))]
+    nts = [b.info(k)[1] for k in range(len(probs))]
+    return {"problems": len(probs), "mesh_range": [nmin, nmax], "ndof_range": [min(ndofs), max(ndofs)],
+            "nt_range": [min(nts), max(nts)], "tol": tol, "sequential_seconds": t_seq, "batched_seconds": t_bat,
+            "batched_gpu_seconds": sec, "speedup": t_seq / t_bat, "batched_steps": steps,
+            "updates_total": int(sum(its)), "updates_equal_to_sequential": its == seq_it,
+            "max_rel_l2_batched_vs_sequential": worst, "steps_per_graph": steps_per_graph}
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
     from oracle import oracle as O
 
@@ -209,6 +256,11 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=400)
     ap.add_argument("--wh-grid", type=int, default=384, help="L-shape grid size (384 = the benchmark workload)")
+    ap.add_argument("--only-batched", action="store_true", help="batched vs sequential WaveHoltz (uneven batch)")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-nmin", type=int, default=32)
+    ap.add_argument("--batch-nmax", type=int, default=256)
+    ap.add_argument("--batch-graph", type=int, default=16)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -224,6 +276,12 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.only_batched:
+        r = batched_vs_sequential(torch, capi, args.batch_count, args.batch_nmin, args.batch_nmax, args.tol,
+                                  args.max_iter, args.batch_graph)
+        print(f"[batched] {r}", file=sys.stderr, flush=True)
+        print(json.dumps({"batched": r}))
+        return 0
     if args.only_waveholtz:
         out = {}
         W64, out["f64"] = waveholtz_solve(torch, capi, "f64", args.tol, args.max_iter, args.wh_grid)

@@ -144,3 +144,23 @@ def config3(n: int = 384, seed: int = 3000):
             centers.append(c)
     return dict(name=f"lshape{n}_P5_w20pi", vertices=v, elements=e, n=6, bc_code=0, omega=20.0 * np.pi,
                 sources=np.array([[c[0], c[1], 0.01, 1.0] for c in centers]))
+
+
+def config4(count: int = 64, nmin: int = 32, nmax: int = 256, seed: int = 4000):
+    """Uneven batch: square meshes nmin^2..nmax^2 quads, P uniform in 3..7,
+    omega uniform in [5 pi, 25 pi], 1-4 Gaussian sources (sigma 0.01-0.05),
+    characteristic outflow on every wall."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(count):
+        nx = int(rng.integers(1, nmin, nmax)[0])
+        P = int(rng.integers(1, 3, 7)[0])
+        omega = float(rng.uniform(1, 5.0 * np.pi, 25.0 * np.pi)[0])
+        ns = int(rng.integers(1, 1, 4)[0])
+        c = rng.uniform(2 * ns, 0.15, 0.85).reshape(ns, 2)
+        sig = rng.uniform(ns, 0.01, 0.05)
+        amp = rng.normal(ns)
+        v, e = perturbed_rect(nx, nx, 0.2, seed + 100 + k)
+        out.append(dict(name=f"p{k}_{nx}x{nx}_P{P}", vertices=v, elements=e, n=P + 1, omega=omega,
+                        sources=np.column_stack([c, sig, amp])))
+    return out
