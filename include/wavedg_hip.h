@@ -33,6 +33,7 @@
 #define WAVEDG_HIP_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -132,6 +133,27 @@ int wdg_hip_rk4_steps(wdg_wave2d op, double dt, double *t, void *d_u, int n_step
  * of `steps_per_graph` steps and replayed (n_steps must be a multiple of it). */
 int wdg_hip_rk4_steps_graph(wdg_wave2d op, double dt, double *t, void *d_u, int n_steps, int steps_per_graph,
                             void *stream);
+/* ---- RK-stage schedules and their autotuner (no counterpart in the
+ * reference, whose operators run on the CPU).  A schedule sets the
+ * workgroup size of the RK stage kernels: about threads[k] threads, i.e.
+ * threads[k] / n^2 whole elements per workgroup (one wavefront per element
+ * for n = 8 and 64 threads).  Every schedule computes the same values in the
+ * same order; the results are bitwise identical. */
+int wdg_hip_schedule_count(int *count, int *threads);
+int wdg_hip_wave2d_set_schedule(wdg_wave2d op, int schedule);
+int wdg_hip_wave2d_get_schedule(wdg_wave2d op, int *schedule, int *threads_per_workgroup);
+/* Times `reps` launches of the RK stage kernel under every schedule on
+ * private buffers (the handle's state is untouched), admits only schedules
+ * whose output equals schedule 0 bit for bit, selects the fastest for this
+ * handle and records it for (architecture, n_colloc, precision): handles
+ * created afterwards in this process start from it.  ms (may be NULL) gets
+ * the average launch time per schedule, -1 for a schedule not admitted.
+ * Synchronises the device. */
+int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int *schedule, double *ms);
+/* The tuned-schedule table as a text file: "arch n_colloc precision schedule
+ * threads" per line. */
+int wdg_hip_tune_cache_save(const char *path);
+int wdg_hip_tune_cache_load(const char *path);
 /* Average duration in milliseconds of the stage kernels launched by the last
  * wdg_hip_rk4_steps call on this handle (HIP events around them on `stream`).
  * Enabled by wdg_hip_wave2d_set_timing(op, 1). */
@@ -215,6 +237,50 @@ int wdg_hip_norm(const double *d_x, long long n, double *out, void *stream);
 int wdg_hip_error(const double *d_x, const double *d_y, long long n, double *out, void *stream);
 /* Converts between the handle's precision and double (n values). */
 int wdg_hip_convert(const void *d_src, int src_precision, void *d_dst, int dst_precision, long long n, void *stream);
+
+/* ---- learned warm start -------------------------------------------------
+ * No counterpart in the reference.  A four-level convolutional
+ * encoder-decoder (widths base, 2 base, 4 base, 8 base; 3x3 convolutions
+ * with bias and ReLU, 2x2 average pooling, nearest upsampling, skip
+ * concatenation, 1x1 output layer) on M x M element images: pixel e of an
+ * image is element e = ex + M ey of a uniform M x M quad grid, its channels
+ * the element's nodal values.  Images are NHWC, [batch][M][M][ld]; inputs
+ * bf16 (uint16 bit patterns) with in_ld channels, outputs and targets fp32
+ * with out_ld = out_channels rounded up to 16 channels (the padding is
+ * ignored by the loss).  Weights are bf16 with an fp32 master copy, all
+ * products on bf16 MFMA with fp32 accumulation; training minimises the mean
+ * over the batch of ||y - t||^2 / ||t||^2 with Adam (0.9, 0.999, 1e-8).
+ * The parameter vector holds, per layer in network order, weights
+ * [tap][cout][cin] then bias [cout]. */
+typedef struct wdg_cnn_s *wdg_cnn;
+int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int max_batch, wdg_cnn *out);
+int wdg_hip_cnn_destroy(wdg_cnn n);
+int wdg_hip_cnn_info(wdg_cnn n, long long *n_params, int *in_ld, int *out_ld);
+/* Host fp32 parameter vectors; set_params also resets the Adam state. */
+int wdg_hip_cnn_set_params(wdg_cnn n, const float *params);
+int wdg_hip_cnn_get_params(wdg_cnn n, float *params);
+/* d_y (device, fp32) = network(d_x) for `batch` images. */
+int wdg_hip_cnn_forward(wdg_cnn n, int batch, const uint16_t *d_x, float *d_y, void *stream);
+/* Loss (host, may be NULL) and its gradient w.r.t. the parameters (host
+ * fp32, n_params, may be NULL) for targets d_t.  Synchronises `stream`. */
+int wdg_hip_cnn_loss_grad(wdg_cnn n, int batch, const uint16_t *d_x, const float *d_t, float *loss, float *grad,
+                          void *stream);
+/* One Adam step on the loss of this batch; loss (host) may be NULL, which
+ * leaves the call asynchronous. */
+int wdg_hip_cnn_train_step(wdg_cnn n, int batch, const uint16_t *d_x, const float *d_t, float lr, float *loss,
+                           void *stream);
+int wdg_hip_f32_to_bf16(const float *d_src, uint16_t *d_dst, long long n, void *stream);
+/* Input image of one problem: channels 0..n^2-1 are gain * f(node i + n j)
+ * with f the sum of Gaussian sources (device (4, n_sources): cx, cy, sigma,
+ * amp), channel n^2 is omega_feature, n^2+1 and n^2+2 the element centre
+ * mapped from box = {x0, y0, x1, y1} to [-1, 1], the rest 0.  d_nodes are
+ * the GLL nodes as from wdg_hip_mesh_nodes, on the device. */
+int wdg_hip_ws_features(const double *d_nodes, int n_colloc, int nel, int n_sources, const double *d_sources,
+                        double gain, double omega_feature, const double *box, int ld, uint16_t *d_x, void *stream);
+/* d_t[e][c] = d_W[e][c] / scale for the 3 n^2 values of element e, padding 0. */
+int wdg_hip_ws_dg_to_image(const double *d_W, int n_colloc, int nel, double scale, int ld, float *d_t, void *stream);
+/* d_W[e][c] = scale * d_y[e][c]: a network output as a WaveHoltz iterate. */
+int wdg_hip_ws_image_to_dg(const float *d_y, int ld, int n_colloc, int nel, double scale, double *d_W, void *stream);
 
 #ifdef __cplusplus
 }

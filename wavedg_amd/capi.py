@@ -26,6 +26,7 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
 _ll = ctypes.c_longlong
+_fp = ctypes.POINTER(ctypes.c_float)
 
 # name -> argument types (every function returns int status)
 _SIGS = {
@@ -77,6 +78,24 @@ _SIGS = {
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
     "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
+    "wdg_hip_schedule_count": [_ip, _ip],
+    "wdg_hip_wave2d_set_schedule": [_vp, _i],
+    "wdg_hip_wave2d_get_schedule": [_vp, _ip, _ip],
+    "wdg_hip_wave2d_autotune": [_vp, _i, _ip, _dp],
+    "wdg_hip_tune_cache_save": [ctypes.c_char_p],
+    "wdg_hip_tune_cache_load": [ctypes.c_char_p],
+    "wdg_hip_cnn_create": [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_cnn_destroy": [_vp],
+    "wdg_hip_cnn_info": [_vp, ctypes.POINTER(_ll), _ip, _ip],
+    "wdg_hip_cnn_set_params": [_vp, _fp],
+    "wdg_hip_cnn_get_params": [_vp, _fp],
+    "wdg_hip_cnn_forward": [_vp, _i, _vp, _vp, _vp],
+    "wdg_hip_cnn_loss_grad": [_vp, _i, _vp, _vp, _fp, _fp, _vp],
+    "wdg_hip_cnn_train_step": [_vp, _i, _vp, _vp, ctypes.c_float, _fp, _vp],
+    "wdg_hip_f32_to_bf16": [_vp, _vp, _ll, _vp],
+    "wdg_hip_ws_features": [_vp, _i, _i, _i, _vp, _d, _d, _dp, _i, _vp, _vp],
+    "wdg_hip_ws_dg_to_image": [_vp, _i, _i, _d, _i, _vp, _vp],
+    "wdg_hip_ws_image_to_dg": [_vp, _i, _i, _i, _d, _vp, _vp],
 }
 
 _lib = None
@@ -278,6 +297,23 @@ class Wave2D:
         _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
         return ms.value, nk.value
 
+    def set_schedule(self, k: int) -> None:
+        _call("wdg_hip_wave2d_set_schedule", self.h, k)
+
+    def schedule(self):
+        """(schedule index, threads per workgroup) of the RK stage kernels."""
+        k, t = _i(), _i()
+        _call("wdg_hip_wave2d_get_schedule", self.h, ctypes.byref(k), ctypes.byref(t))
+        return k.value, t.value
+
+    def autotune(self, reps: int = 20):
+        """Picks the fastest bitwise-equal schedule; returns (index, ms per schedule)."""
+        n = schedule_count()[0]
+        ms = np.zeros(n)
+        k = _i()
+        _call("wdg_hip_wave2d_autotune", self.h, reps, ctypes.byref(k), _dptr(ms))
+        return k.value, ms
+
     def load_vector_gauss(self, sources, F, var: int = 0, stream=None):
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
         _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, dev_ptr(F),
@@ -326,6 +362,22 @@ class WaveHoltz:
     def postprocess(self, H, w, stream=None):
         _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),
               stream if stream is not None else cur_stream())
+
+
+def schedule_count():
+    """(number of schedules, threads-per-workgroup target of each)."""
+    c = _i()
+    t = (ctypes.c_int * 16)()
+    _call("wdg_hip_schedule_count", ctypes.byref(c), t)
+    return c.value, list(t[:c.value])
+
+
+def tune_cache_save(path: str) -> None:
+    _call("wdg_hip_tune_cache_save", path.encode())
+
+
+def tune_cache_load(path: str) -> None:
+    _call("wdg_hip_tune_cache_load", path.encode())
 
 
 class _BatchProblem(ctypes.Structure):
@@ -390,3 +442,73 @@ def error(x, y, stream=None) -> float:
     _call("wdg_hip_error", dev_ptr(x), dev_ptr(y), x.numel(), ctypes.byref(r),
           stream if stream is not None else cur_stream())
     return r.value
+
+
+class CNN:
+    """The warm-start encoder-decoder (wdg_hip_cnn_*) on M x M images with
+    in_ld bf16 input channels and out_channels fp32 outputs."""
+
+    def __init__(self, M: int, in_ld: int, out_channels: int, base: int = 32, max_batch: int = 16):
+        h = _vp()
+        _call("wdg_hip_cnn_create", M, in_ld, out_channels, base, max_batch, ctypes.byref(h))
+        self.h = h
+        self.M, self.base, self.max_batch, self.out_channels = M, base, max_batch, out_channels
+        n, il, ol = _ll(), _i(), _i()
+        _call("wdg_hip_cnn_info", h, ctypes.byref(n), ctypes.byref(il), ctypes.byref(ol))
+        self.n_params, self.in_ld, self.out_ld = n.value, il.value, ol.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_cnn_destroy(self.h)
+            self.h = None
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        _call("wdg_hip_cnn_set_params", self.h, p.ctypes.data_as(_fp))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        _call("wdg_hip_cnn_get_params", self.h, p.ctypes.data_as(_fp))
+        return p
+
+    def forward(self, x, y, batch: int | None = None, stream=None):
+        """x: uint16 (bf16 bits) [B, M, M, in_ld] device tensor; y: float32 [B, M, M, out_ld]."""
+        B = batch if batch is not None else x.shape[0]
+        _call("wdg_hip_cnn_forward", self.h, B, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+
+    def loss_grad(self, x, t, batch: int | None = None, want_grad: bool = True, stream=None):
+        B = batch if batch is not None else x.shape[0]
+        loss = ctypes.c_float()
+        g = np.zeros(self.n_params, dtype=np.float32) if want_grad else None
+        _call("wdg_hip_cnn_loss_grad", self.h, B, dev_ptr(x), dev_ptr(t), ctypes.byref(loss),
+              g.ctypes.data_as(_fp) if g is not None else None, stream if stream is not None else cur_stream())
+        return loss.value, g
+
+    def train_step(self, x, t, lr: float, batch: int | None = None, want_loss: bool = True, stream=None):
+        B = batch if batch is not None else x.shape[0]
+        loss = ctypes.c_float()
+        _call("wdg_hip_cnn_train_step", self.h, B, dev_ptr(x), dev_ptr(t), lr,
+              ctypes.byref(loss) if want_loss else None, stream if stream is not None else cur_stream())
+        return loss.value if want_loss else None
+
+
+def f32_to_bf16(src, dst, stream=None):
+    _call("wdg_hip_f32_to_bf16", dev_ptr(src), dev_ptr(dst), src.numel(), stream if stream is not None else cur_stream())
+
+
+def ws_features(d_nodes, n: int, nel: int, d_sources, n_sources: int, gain: float, omega_feature: float, box, ld: int,
+                d_x, stream=None):
+    b = np.ascontiguousarray(np.asarray(box, dtype=np.float64))
+    _call("wdg_hip_ws_features", dev_ptr(d_nodes), n, nel, n_sources, dev_ptr(d_sources), gain, omega_feature,
+          _dptr(b), ld, dev_ptr(d_x), stream if stream is not None else cur_stream())
+
+
+def ws_dg_to_image(d_W, n: int, nel: int, scale: float, ld: int, d_t, stream=None):
+    _call("wdg_hip_ws_dg_to_image", dev_ptr(d_W), n, nel, scale, ld, dev_ptr(d_t),
+          stream if stream is not None else cur_stream())
+
+
+def ws_image_to_dg(d_y, ld: int, n: int, nel: int, scale: float, d_W, stream=None):
+    _call("wdg_hip_ws_image_to_dg", dev_ptr(d_y), ld, n, nel, scale, dev_ptr(d_W),
+          stream if stream is not None else cur_stream())

@@ -38,7 +38,7 @@
 namespace {
 
 struct BatchBlock {
-    int prob, N, e0, nloc;
+    int prob, N, e0, nloc, gid;  // gid: index in the full table (residual partials)
 };
 
 struct BatchProb {
@@ -227,7 +227,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) a.partial[blockIdx.x] = red[0];
+        if (threadIdx.x == 0) a.partial[blk.gid] = red[0];
     }
 }
 
@@ -256,12 +256,13 @@ __global__ __launch_bounds__(kBT) void batch_stage_kernel(const BatchArgs a)
 
 // One workgroup per problem: finish the residual of problems whose period
 // ended at step G, retire converged ones; the last workgroup advances G.
-__global__ __launch_bounds__(kBT) void batch_reduce_kernel(const BatchProb* probs, const double* partial, int* active,
-                                                           long long* G, unsigned int* done, double* pn, double* hist,
-                                                           int* iters, int hist_len, double tol, int max_iter)
+__global__ __launch_bounds__(kBT) void batch_reduce_kernel(const BatchProb* probs, const int* prob_ids,
+                                                           const double* partial, int* active, long long* G,
+                                                           unsigned int* done, double* pn, double* hist, int* iters,
+                                                           int hist_len, double tol, int max_iter)
 {
     __shared__ double red[kBT];
-    const int k = blockIdx.x;
+    const int k = prob_ids[blockIdx.x];
     const long long g = *G;
     const BatchProb pr = probs[k];
     if (active[k] && (g + 1) % pr.nt == 0) {
@@ -384,6 +385,9 @@ struct wdg_batch_s {
     size_t rsz = 8, lds = 0;
     long long ntot = 0;  // total state length
     std::vector<BatchProb> probs;
+    std::vector<BatchBlock> all_blocks;  // full workgroup table
+    int n_run_blocks = 0, n_run_probs = 0;  // compacted tables of the problems still running
+    int* d_prob_ids = nullptr;
     std::vector<long long> ndof;
     std::vector<double> omega;
     BatchBlock* d_blocks = nullptr;
@@ -412,15 +416,13 @@ struct wdg_batch_s {
     unsigned int* d_done = nullptr;
     hipStream_t cap = nullptr;
     hipGraphExec_t graph = nullptr;
-    int graph_steps = 0, graph_max_iter = 0;
-    double graph_tol = 0.0;
     double last_seconds = 0.0;
     long long last_steps = 0;
     ~wdg_batch_s()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
-        void* ptrs[] = {d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
+        void* ptrs[] = {d_prob_ids, d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
                         d_acc, d_wprev, d_pi0, d_winit, d_warm, d_active, d_iters, d_partial, d_pn, d_hist, d_G,
                         d_done};
         for (void* p : ptrs)
@@ -463,14 +465,49 @@ void launch_step(const wdg_batch_s& b, hipStream_t s, double tol, int max_iter)
     for (int st = 0; st < 5; ++st) {
         const BatchArgs a = b.args(st);
         if (st == 0)
-            hipLaunchKernelGGL((batch_stage_kernel<R, 0>), dim3(b.nblk), dim3(kBT), b.lds, s, a);
+            hipLaunchKernelGGL((batch_stage_kernel<R, 0>), dim3(b.n_run_blocks), dim3(kBT), b.lds, s, a);
         else if (st < 4)
-            hipLaunchKernelGGL((batch_stage_kernel<R, 1>), dim3(b.nblk), dim3(kBT), b.lds, s, a);
+            hipLaunchKernelGGL((batch_stage_kernel<R, 1>), dim3(b.n_run_blocks), dim3(kBT), b.lds, s, a);
         else
-            hipLaunchKernelGGL((batch_stage_kernel<R, 2>), dim3(b.nblk), dim3(kBT), b.lds, s, a);
+            hipLaunchKernelGGL((batch_stage_kernel<R, 2>), dim3(b.n_run_blocks), dim3(kBT), b.lds, s, a);
     }
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(b.nprob), dim3(kBT), 0, s, b.d_probs, b.d_partial, b.d_active, b.d_G,
-                       b.d_done, b.d_pn, b.d_hist, b.d_iters, b.hist_len, tol, max_iter);
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(b.n_run_probs), dim3(kBT), 0, s, b.d_probs, b.d_prob_ids, b.d_partial,
+                       b.d_active, b.d_G, b.d_done, b.d_pn, b.d_hist, b.d_iters, b.hist_len, tol, max_iter);
+}
+
+// Tables of the problems still running (host flags), uploaded for the next capture.
+void compact(wdg_batch_s& b, const std::vector<int>& run, hipStream_t s)
+{
+    std::vector<BatchBlock> blocks;
+    std::vector<int> ids;
+    for (int k = 0; k < b.nprob; ++k) {
+        if (!run[k]) continue;
+        ids.push_back(k);
+        const BatchProb& pr = b.probs[k];
+        blocks.insert(blocks.end(), b.all_blocks.begin() + pr.blk_off, b.all_blocks.begin() + pr.blk_off + pr.nblk);
+    }
+    b.n_run_blocks = (int)blocks.size();
+    b.n_run_probs = (int)ids.size();
+    if (blocks.empty()) return;
+    bcheck(hipMemcpyAsync(b.d_blocks, blocks.data(), sizeof(BatchBlock) * blocks.size(), hipMemcpyHostToDevice, s),
+           "upload blocks");
+    bcheck(hipMemcpyAsync(b.d_prob_ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, s), "upload ids");
+    bcheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+template <typename Fn>
+void capture(wdg_batch_s& b, int steps, Fn&& step)
+{
+    if (b.graph) (void)hipGraphExecDestroy(b.graph);
+    b.graph = nullptr;
+    if (!b.cap) bcheck(hipStreamCreateWithFlags(&b.cap, hipStreamNonBlocking), "hipStreamCreate");
+    hipGraph_t gph = nullptr;
+    bcheck(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    for (int k = 0; k < steps; ++k) step(b.cap);
+    bcheck(hipStreamEndCapture(b.cap, &gph), "hipStreamEndCapture");
+    const hipError_t ie = hipGraphInstantiate(&b.graph, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    bcheck(ie, "hipGraphInstantiate");
 }
 
 }  // namespace
@@ -550,7 +587,8 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
             }
             ctab.push_back(dt);
             const int epb = 256 / NN > 0 ? 256 / NN : 1;
-            for (int e0 = 0; e0 < nel; e0 += epb) blocks.push_back(BatchBlock{k, n, e0, std::min(epb, nel - e0)});
+            for (int e0 = 0; e0 < nel; e0 += epb)
+                blocks.push_back(BatchBlock{k, n, e0, std::min(epb, nel - e0), (int)blocks.size()});
             pr.nblk = (int)blocks.size() - pr.blk_off;
             for (int e = 0; e < nel; ++e) {
                 const double* c = &m.corners()[8 * (size_t)e];
@@ -576,6 +614,8 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         }
         b->ntot = soff;
         b->nblk = (int)blocks.size();
+        b->all_blocks = blocks;
+        b->d_prob_ids = dalloc<int>(n_problems);
         // device arrays
         b->d_blocks = dalloc<BatchBlock>(blocks.size());
         b->d_basis = dalloc<double>(basis_all.size());
@@ -719,28 +759,15 @@ int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_gra
         hipLaunchKernelGGL(batch_init_state, dim3((b->nprob + 255) / 256), dim3(256), 0, s, b->d_active, b->nprob,
                            b->d_G, b->d_done, b->d_iters);
         bcheck(hipGetLastError(), "batch reset");
-        // capture steps_per_graph steps once per (tol, max_iter, steps) setting
-        if (!b->graph || b->graph_steps != steps_per_graph || b->graph_tol != tol || b->graph_max_iter != max_iter) {
-            if (b->graph) (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            if (!b->cap) bcheck(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "hipStreamCreate");
-            hipGraph_t gph = nullptr;
-            bcheck(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (int k = 0; k < steps_per_graph; ++k) {
-                if (b->prec == WDG_F32)
-                    launch_step<float>(*b, b->cap, tol, max_iter);
-                else
-                    launch_step<double>(*b, b->cap, tol, max_iter);
-            }
-            bcheck(hipStreamEndCapture(b->cap, &gph), "hipStreamEndCapture");
-            const hipError_t ie = hipGraphInstantiate(&b->graph, gph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(gph);
-            bcheck(ie, "hipGraphInstantiate");
-            b->graph_steps = steps_per_graph;
-            b->graph_tol = tol;
-            b->graph_max_iter = max_iter;
-        }
-        std::vector<int> act(b->nprob);
+        std::vector<int> act(b->nprob, 1), run(b->nprob, 1);
+        auto step = [&](hipStream_t cs) {
+            if (b->prec == WDG_F32)
+                launch_step<float>(*b, cs, tol, max_iter);
+            else
+                launch_step<double>(*b, cs, tol, max_iter);
+        };
+        compact(*b, run, s);
+        capture(*b, steps_per_graph, step);
         long long max_steps = 0;
         for (const BatchProb& pr : b->probs) max_steps = std::max(max_steps, (long long)pr.nt * (max_iter + 1));
         hipEvent_t e0, e1;
@@ -753,9 +780,14 @@ int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_gra
             steps += steps_per_graph;
             bcheck(hipMemcpyAsync(act.data(), b->d_active, sizeof(int) * b->nprob, hipMemcpyDeviceToHost, s), "copy");
             bcheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+            if (act == run) continue;
+            run = act;
             bool any = false;
-            for (int v : act) any = any || v;
+            for (int v : run) any = any || v;
             if (!any) break;
+            // problems retired: drop their workgroups from the launches
+            compact(*b, run, s);
+            capture(*b, steps_per_graph, step);
         }
         bcheck(hipEventRecord(e1, s), "hipEventRecord");
         bcheck(hipEventSynchronize(e1), "hipEventSynchronize");

@@ -1,0 +1,958 @@
+// Learned warm start for WaveHoltz: a four-level convolutional
+// encoder-decoder (U-Net) on element images, trained and evaluated with the
+// kernels in this file.  The reference has no learned component; the
+// network only proposes the initial iterate W_1, and the WaveHoltz
+// iteration decides the final residual.
+//
+// Images.  On an M x M grid of quads numbered e = ex + M ey (the uniform
+// rectangle numbering) element e is pixel (ey, ex); its channels are the
+// element's nodal values, so the DG block of element e (3 n^2 doubles,
+// (3, n, n) column-major) is the pixel's channel vector.
+//
+// Numerics.  Activations and weights are bf16 (round to nearest even),
+// products run on v_mfma_f32_16x16x16bf16_1k with fp32 accumulation, the
+// master weights, Adam moments, gradients, bias terms, loss and the
+// output layer are fp32.  Every reduction runs in a fixed order (per-chunk
+// partials, then a sequential sum over chunks): results are bitwise
+// reproducible.
+//
+// Network (C = base width, channel counts x1, x2, x4, x8):
+//   e1 = conv(conv(x))            M     C       p1 = avgpool(e1)
+//   e2 = conv(conv(p1))           M/2   2C      p2 = avgpool(e2)
+//   e3 = conv(conv(p2))           M/4   4C      p3 = avgpool(e3)
+//   b  = conv(conv(p3))           M/8   8C
+//   d3 = conv(conv([up(b),  e3])) M/4   4C
+//   d2 = conv(conv([up(d3), e2])) M/2   2C
+//   d1 = conv(conv([up(d2), e1])) M     C
+//   y  = conv1x1(d1)                    out channels, no activation
+// conv = 3x3, zero padding, bias, ReLU; up = nearest; [a, b] = channel concat.
+#include "../../include/wavedg_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_basis.hpp"
+#include "host_mesh.hpp"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;     // waves per workgroup of the conv kernels
+constexpr int kChunk = 1024;  // pixels per weight-gradient partial
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ------------------------------------------------------------------ conv
+// out[p, co] = act(bias[co] + sum_{tap, ci} in[p + off(tap), ci] w[tap][co][ci])
+// as a GEMM with M = output channels, N = pixels, K = taps x input channels.
+// A wave owns TA x TB tiles of 16 channels x 16 pixels.  MFMA operands:
+// A lane l holds channel co0 + l%16 and k = 4(l/16)..4(l/16)+3, i.e. four
+// consecutive input channels of one weight row (one 8-byte load); B lane l
+// holds pixel l%16 and the same four input channels (one 8-byte load from
+// the NHWC image).  The accumulator lane l holds pixel l%16 and output
+// channels 4(l/16)..+3, stored as one 8-byte (bf16) or 16-byte (fp32) write.
+struct ConvArgs {
+    const uint16_t* in;
+    const uint16_t* w;   // [ks*ks][cout][cin] bf16
+    const float* bias;   // [cout] or null
+    void* out;
+    int in_ld, in_off, cin;
+    int out_ld, out_off, cout;
+    int B, H, W, ks, relu;
+};
+
+template <int TA, int TB, bool F32OUT>
+__global__ __launch_bounds__(64 * kWaves) void conv_kernel(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const long long npix = (long long)a.B * a.H * a.W;
+    const long long p0 = ((long long)blockIdx.x * kWaves + wave) * (16 * TB);
+    const int co0 = blockIdx.y * (16 * TA);
+    if (p0 >= npix) return;  // no barriers below
+
+    int pb[TB], py[TB], pxx[TB];
+    bool pok[TB];
+#pragma unroll
+    for (int j = 0; j < TB; ++j) {
+        const long long p = p0 + 16 * j + ln;
+        pok[j] = p < npix;
+        const long long pc = pok[j] ? p : 0;
+        pxx[j] = (int)(pc % a.W);
+        const long long t = pc / a.W;
+        py[j] = (int)(t % a.H);
+        pb[j] = (int)(t / a.H);
+    }
+    f32x4 acc[TA][TB];
+#pragma unroll
+    for (int i = 0; i < TA; ++i)
+#pragma unroll
+        for (int j = 0; j < TB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int r = a.ks >> 1;
+    for (int tap = 0; tap < a.ks * a.ks; ++tap) {
+        const int dy = tap / a.ks - r, dx = tap % a.ks - r;
+        const uint16_t* src[TB];
+        bool ok[TB];
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            const int ny = py[j] + dy, nx = pxx[j] + dx;
+            ok[j] = pok[j] && ny >= 0 && ny < a.H && nx >= 0 && nx < a.W;
+            const long long q = ok[j] ? ((long long)pb[j] * a.H + ny) * a.W + nx : 0;
+            src[j] = a.in + q * a.in_ld + a.in_off + 4 * kq;
+        }
+        const uint16_t* wsrc = a.w + ((long long)tap * a.cout + co0 + ln) * a.cin + 4 * kq;
+        for (int c0 = 0; c0 < a.cin; c0 += 16) {
+            bf16x4 av[TA], bv[TB];
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                uint2 v = *reinterpret_cast<const uint2*>(src[j] + c0);
+                if (!ok[j]) v = make_uint2(0u, 0u);
+                bv[j] = __builtin_bit_cast(bf16x4, v);
+            }
+#pragma unroll
+            for (int i = 0; i < TA; ++i) {
+                const uint2 v = *reinterpret_cast<const uint2*>(wsrc + (long long)16 * i * a.cin + c0);
+                av[i] = __builtin_bit_cast(bf16x4, v);
+            }
+#pragma unroll
+            for (int i = 0; i < TA; ++i)
+#pragma unroll
+                for (int j = 0; j < TB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TA; ++i) {
+        const int co = co0 + 16 * i + 4 * kq;
+        float bias[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.bias) {
+            const float4 b4 = *reinterpret_cast<const float4*>(a.bias + co);
+            bias[0] = b4.x, bias[1] = b4.y, bias[2] = b4.z, bias[3] = b4.w;
+        }
+#pragma unroll
+        for (int j = 0; j < TB; ++j) {
+            if (!pok[j]) continue;
+            const long long p = p0 + 16 * j + ln;
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                v[q] = acc[i][j][q] + bias[q];
+                if (a.relu) v[q] = fmaxf(v[q], 0.f);
+            }
+            const long long o = p * a.out_ld + a.out_off + co;
+            if constexpr (F32OUT) {
+                *reinterpret_cast<float4*>(static_cast<float*>(a.out) + o) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out) + o) = make_uint2(lo, hi);
+            }
+        }
+    }
+}
+
+// dw[tap][co][ci] = sum_p dout[p, co] in[p + off(tap), ci]: a GEMM with
+// M = output channels, N = input channels, K = pixels.  One wave owns one
+// 16 x 16 tile of one tap over one chunk of pixels and writes its partial
+// sum to partial[chunk][tap][co][ci].
+struct WgradArgs {
+    const uint16_t* in;
+    const uint16_t* dout;
+    float* partial;
+    int in_ld, in_off, cin;
+    int dout_ld, cout;
+    int B, H, W, ks;
+};
+
+__global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const int nco = a.cout >> 4, nci = a.cin >> 4, ntap = a.ks * a.ks;
+    const int tile = blockIdx.y * kWaves + wave;
+    if (tile >= ntap * nco * nci) return;
+    const int tap = tile / (nco * nci), rem = tile % (nco * nci);
+    const int cob = rem / nci, cib = rem % nci;
+    const int dy = tap / a.ks - (a.ks >> 1), dx = tap % a.ks - (a.ks >> 1);
+    const long long npix = (long long)a.B * a.H * a.W;
+    const long long pbeg = (long long)blockIdx.x * kChunk;
+    const long long pend = min(npix, pbeg + kChunk);
+    const int co = cob * 16 + ln, ci = cib * 16 + ln;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long long p = pbeg; p < pend; p += 16) {
+        bf16x4 av, bv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long q = p + 4 * kq + r;
+            const bool okq = q < pend;
+            const long long qc = okq ? q : pbeg;
+            const int x = (int)(qc % a.W);
+            const long long t = qc / a.W;
+            const int y = (int)(t % a.H), b = (int)(t / a.H);
+            const uint16_t dv = a.dout[qc * a.dout_ld + co];
+            av[r] = (short)(okq ? dv : 0);
+            const int ny = y + dy, nx = x + dx;
+            const bool ok = okq && ny >= 0 && ny < a.H && nx >= 0 && nx < a.W;
+            const long long src = ok ? ((long long)b * a.H + ny) * a.W + nx : 0;
+            const uint16_t iv = a.in[src * a.in_ld + a.in_off + ci];
+            bv[r] = (short)(ok ? iv : 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    float* dst = a.partial + ((long long)blockIdx.x * ntap + tap) * a.cout * a.cin;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[(long long)(cob * 16 + 4 * kq + r) * a.cin + cib * 16 + ln] = acc[r];
+}
+
+// out[i] = sum_{c < chunks} partial[c n + i], in chunk order.
+__global__ void reduce_chunks_kernel(const float* partial, long long n, int chunks, float* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * n + i];
+    out[i] = s;
+}
+
+// partial[chunk][c] = sum over the chunk's pixels of d[p, c]
+__global__ void bias_partial_kernel(const uint16_t* d, int C, long long npix, float* partial)
+{
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const long long pbeg = (long long)blockIdx.x * kChunk, pend = min(npix, pbeg + kChunk);
+    float s = 0.f;
+    for (long long p = pbeg; p < pend; ++p) s += bf2f(d[p * C + c]);
+    partial[(long long)blockIdx.x * C + c] = s;
+}
+
+// ------------------------------------------------------------- elementwise
+// 2 x 2 average pool; out has H x W pixels, in 2H x 2W.
+__global__ void pool_kernel(const uint16_t* in, int in_ld, int in_off, uint16_t* out, int B, int H, int W, int C)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)B * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const long long p = i / C;
+    const int x = (int)(p % W);
+    const long long t = p / W;
+    const int y = (int)(t % H), b = (int)(t / H);
+    const long long q = ((long long)b * 2 * H + 2 * y) * (2 * W) + 2 * x;
+    const long long s0 = q * in_ld + in_off + c, s1 = (q + 2 * W) * in_ld + in_off + c;
+    const float v = (bf2f(in[s0]) + bf2f(in[s0 + in_ld])) + (bf2f(in[s1]) + bf2f(in[s1 + in_ld]));
+    out[i] = f2bf(0.25f * v);
+}
+
+// nearest upsample; in has H/2 x W/2 pixels (contiguous, C channels)
+__global__ void upsample_kernel(const uint16_t* in, uint16_t* out, int out_ld, int out_off, int B, int H, int W, int C)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)B * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    const long long p = i / C;
+    const int x = (int)(p % W);
+    const long long t = p / W;
+    const int y = (int)(t % H), b = (int)(t / H);
+    const long long q = ((long long)b * (H / 2) + y / 2) * (W / 2) + x / 2;
+    out[p * out_ld + out_off + c] = in[q * C + c];
+}
+
+// ReLU backward with the incoming gradient assembled from its sources:
+//   g = g1[p, off1 + c]                        (up = 0), or
+//   g = sum of g1 over the 2 x 2 children of p (up = 1, g1 at 2H x 2W);
+//   plus 0.25 g2[parent(p), c] if g2 (average-pool backward, g2 at H/2 x W/2);
+// out[p, c] = bf16(g) where act[p, c] > 0, else 0 (contiguous, C channels).
+struct PrepArgs {
+    const float* g1;
+    const float* g2;
+    const uint16_t* act;
+    uint16_t* out;
+    int g1_ld, g1_off, up, g2_ld, act_ld, act_off;
+    int B, H, W, C;
+};
+
+__global__ void relu_grad_kernel(PrepArgs a)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)a.B * a.H * a.W * a.C;
+    if (i >= n) return;
+    const int c = (int)(i % a.C);
+    const long long p = i / a.C;
+    const int x = (int)(p % a.W);
+    const long long t = p / a.W;
+    const int y = (int)(t % a.H), b = (int)(t / a.H);
+    float g;
+    if (a.up) {
+        const long long q = ((long long)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x;
+        const long long s0 = q * a.g1_ld + a.g1_off + c, s1 = (q + 2 * a.W) * a.g1_ld + a.g1_off + c;
+        g = (a.g1[s0] + a.g1[s0 + a.g1_ld]) + (a.g1[s1] + a.g1[s1 + a.g1_ld]);
+    } else {
+        g = a.g1[p * a.g1_ld + a.g1_off + c];
+    }
+    if (a.g2) {
+        const long long q = ((long long)b * (a.H / 2) + y / 2) * (a.W / 2) + x / 2;
+        g += 0.25f * a.g2[q * a.g2_ld + c];
+    }
+    const bool on = bf2f(a.act[p * a.act_ld + a.act_off + c]) > 0.f;
+    a.out[i] = on ? f2bf(g) : (uint16_t)0;
+}
+
+// Per-sample relative squared error ||y - t||^2 / ||t||^2 over the first
+// creal channels, and dy = 2 (y - t) / (B ||t||^2) as bf16 (0 on padding).
+__global__ __launch_bounds__(256) void loss_kernel(const float* y, const float* t, int ld, int creal, long long npix,
+                                                   int B, float* loss, uint16_t* dy)
+{
+    __shared__ double red[2][256];
+    const int b = blockIdx.x;
+    const long long n = npix * ld, base = (long long)b * n;
+    double tt = 0.0, ee = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) {
+        if ((int)(i % ld) >= creal) continue;
+        const double tv = t[base + i], d = (double)y[base + i] - tv;
+        tt += tv * tv;
+        ee += d * d;
+    }
+    red[0][threadIdx.x] = tt;
+    red[1][threadIdx.x] = ee;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const double tn = red[0][0] > 1e-300 ? red[0][0] : 1e-300;
+    if (threadIdx.x == 0) loss[b] = (float)(red[1][0] / tn);
+    const float s = (float)(2.0 / (B * tn));
+    for (long long i = threadIdx.x; i < n; i += 256) {
+        const bool real = (int)(i % ld) < creal;
+        dy[base + i] = real ? f2bf(s * (y[base + i] - t[base + i])) : (uint16_t)0;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master copy; c1 = 1/(1-b1^t),
+// c2 = 1/(1-b2^t).  Writes the bf16 copy the forward kernels read.
+__global__ void adam_kernel(float* w, float* m, float* v, const float* g, uint16_t* wbf, long long n, float lr, float b1,
+                            float b2, float eps, float c1, float c2)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    wbf[i] = f2bf(wi);
+}
+
+__global__ void to_bf16_kernel(const float* src, uint16_t* dst, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = f2bf(src[i]);
+}
+
+// Backward-data weights: wb[t][ci][co] = bf16(w[ntap-1-t][co][ci]).
+__global__ void wflip_kernel(const float* w, int cout, int cin, int ntap, uint16_t* wb)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)ntap * cout * cin;
+    if (i >= n) return;
+    const int co = (int)(i % cout);
+    const long long r = i / cout;
+    const int ci = (int)(r % cin), t = (int)(r / cin);
+    wb[i] = f2bf(w[((long long)(ntap - 1 - t) * cout + co) * cin + ci]);
+}
+
+// ------------------------------------------------------- DG <-> image maps
+// Input channels of element e: 0..n^2-1 forcing at node i + n j times gain;
+// n^2: omega feature; n^2+1, n^2+2: element centre mapped to [-1, 1]
+// (x0, y0, x1, y1 = bounding box); the rest 0.
+__global__ void features_kernel(const double* nodes, int n, int nel, const double* src, int ns, double gain,
+                                float omega_feat, double x0, double y0, double x1, double y1, uint16_t* x, int ld)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nel * ld) return;
+    const int c = (int)(i % ld);
+    const long long e = i / ld;
+    const int nn = n * n;
+    float v = 0.f;
+    if (c < nn) {
+        const double px = nodes[2 * (c + nn * e)], py = nodes[2 * (c + nn * e) + 1];
+        double f = 0.0;
+        for (int k = 0; k < ns; ++k) {
+            const double dx = px - src[4 * k], dy = py - src[4 * k + 1], s = src[4 * k + 2];
+            f += src[4 * k + 3] * exp(-(dx * dx + dy * dy) / (2.0 * s * s));
+        }
+        v = (float)(gain * f);
+    } else if (c == nn) {
+        v = omega_feat;
+    } else if (c == nn + 1 || c == nn + 2) {
+        const int d = c - nn - 1;
+        double m = 0.0;
+        for (int k = 0; k < nn; ++k) m += nodes[2 * (k + nn * e) + d];
+        m /= nn;
+        const double lo = d ? y0 : x0, hi = d ? y1 : x1;
+        v = (float)(2.0 * (m - lo) / (hi - lo) - 1.0);
+    }
+    x[i] = f2bf(v);
+}
+
+__global__ void dg_to_image_kernel(const double* W, int nd, long long nel, double inv_scale, float* t, int ld)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nel * ld) return;
+    const int c = (int)(i % ld);
+    const long long e = i / ld;
+    t[i] = c < nd ? (float)(W[e * nd + c] * inv_scale) : 0.f;
+}
+
+__global__ void image_to_dg_kernel(const float* y, int ld, int nd, long long nel, double scale, double* W)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nel * nd) return;
+    const int c = (int)(i % nd);
+    const long long e = i / nd;
+    W[i] = (double)y[e * ld + c] * scale;
+}
+
+// ------------------------------------------------------------------- host
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+void ck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Fail{WDG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <typename F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return WDG_OK;
+    } catch (const Fail& x) {
+        wdg::g_last_error = x.msg;
+        return x.code;
+    } catch (const std::exception& x) {
+        wdg::g_last_error = x.what();
+        return WDG_ERR_INVALID;
+    }
+}
+
+template <typename T>
+T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Fail{WDG_ERR_ALLOC, "hipMalloc failed (" + std::to_string(sizeof(T) * n) + " bytes)"};
+    }
+    return static_cast<T*>(p);
+}
+
+inline unsigned blocks_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Layer {
+    int cin, cout, ks, level;
+    long long w, b, wb;  // offsets: weights and bias in the parameter vector, flipped bf16 weights
+};
+
+// Layer indices in the parameter vector.
+enum { E1A, E1B, E2A, E2B, E3A, E3B, BA, BB, D3A, D3B, D2A, D2B, D1A, D1B, OUT, NLAYER };
+
+}  // namespace
+
+struct wdg_cnn_s {
+    int dev = 0, M = 0, in_ld = 0, out_ld = 0, out_real = 0, maxB = 0;
+    int C[4] = {0, 0, 0, 0};
+    long long nparam = 0, nflip = 0, step = 0;
+    Layer L[NLAYER];
+    float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+    uint16_t *wbf = nullptr, *wflip = nullptr;
+    // activations (bf16), per level l = 0..3 at resolution M >> l
+    uint16_t *e1a = nullptr, *cat1 = nullptr, *d1a = nullptr, *d1 = nullptr;
+    uint16_t *p1 = nullptr, *e2a = nullptr, *cat2 = nullptr, *d2a = nullptr, *d2 = nullptr;
+    uint16_t *p2 = nullptr, *e3a = nullptr, *cat3 = nullptr, *d3a = nullptr, *d3 = nullptr;
+    uint16_t *p3 = nullptr, *ba = nullptr, *bb = nullptr;
+    float* y = nullptr;
+    // gradients
+    uint16_t *dy = nullptr, *dpre = nullptr;
+    float *gA = nullptr, *gcat1 = nullptr, *gcat2 = nullptr, *gcat3 = nullptr, *gp1 = nullptr, *gp2 = nullptr,
+          *gp3 = nullptr;
+    float *partial = nullptr, *loss = nullptr;
+    long long partial_len = 0;
+    std::vector<void*> owned;
+
+    template <typename T>
+    T* alloc(size_t n)
+    {
+        T* p = dalloc<T>(n);
+        owned.push_back(p);
+        return p;
+    }
+    ~wdg_cnn_s()
+    {
+        for (void* p : owned) (void)hipFree(p);
+    }
+    int res(int level) const { return M >> level; }
+};
+
+namespace {
+
+void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_ld, int in_off, int cin,
+                 const uint16_t* w, const float* bias, void* out, int out_ld, int out_off, int cout, bool f32out,
+                 int B, int H, int W, int ks, bool relu)
+{
+    (void)n;
+    ConvArgs a{in, w, bias, out, in_ld, in_off, cin, out_ld, out_off, cout, B, H, W, ks, relu ? 1 : 0};
+    const long long npix = (long long)B * H * W;
+    const int nb = cout / 16;
+    const int TA = nb % 4 == 0 ? 4 : (nb % 2 == 0 ? 2 : 1);
+    int TB = 4;
+    while (TB > 1 && (long long)((npix + 64 * TB - 1) / (64 * TB)) * (nb / TA) < 512) TB >>= 1;
+    const dim3 grid((unsigned)((npix + 16LL * kWaves * TB - 1) / (16LL * kWaves * TB)), (unsigned)(nb / TA));
+    const dim3 blk(64 * kWaves);
+#define WDG_CONV_CASE(ta, tb)                                                                      \
+    if (TA == ta && TB == tb) {                                                                    \
+        if (f32out)                                                                                \
+            hipLaunchKernelGGL((conv_kernel<ta, tb, true>), grid, blk, 0, s, a);                   \
+        else                                                                                       \
+            hipLaunchKernelGGL((conv_kernel<ta, tb, false>), grid, blk, 0, s, a);                  \
+        return;                                                                                    \
+    }
+    WDG_CONV_CASE(1, 1) WDG_CONV_CASE(1, 2) WDG_CONV_CASE(1, 4)
+    WDG_CONV_CASE(2, 1) WDG_CONV_CASE(2, 2) WDG_CONV_CASE(2, 4)
+    WDG_CONV_CASE(4, 1) WDG_CONV_CASE(4, 2) WDG_CONV_CASE(4, 4)
+#undef WDG_CONV_CASE
+}
+
+// Forward of one conv layer (bias + ReLU, bf16 out).
+void fwd(const wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int in_off, uint16_t* out,
+         int out_ld, int out_off, int B)
+{
+    const Layer& L = n.L[li];
+    const int H = n.res(L.level);
+    launch_conv(n, s, in, in_ld, in_off, L.cin, n.wbf + L.w, n.w + L.b, out, out_ld, out_off, L.cout, false, B, H, H,
+                L.ks, true);
+}
+
+// Parameter gradients of layer li from its input and dpre (bf16, contiguous
+// cout channels), then (if gin) the input gradient (fp32, cin channels).
+void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int in_off, const uint16_t* dpre,
+         float* gin, int B)
+{
+    const Layer& L = n.L[li];
+    const int H = n.res(L.level);
+    const long long npix = (long long)B * H * H;
+    const int chunks = (int)((npix + kChunk - 1) / kChunk);
+    const int ntap = L.ks * L.ks;
+    const long long nw = (long long)ntap * L.cout * L.cin;
+    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks};
+    const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
+    hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, chunks, n.g + L.w);
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks, (L.cout + 63) / 64), dim3(64), 0, s, dpre, L.cout, npix,
+                       n.partial);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(256), 0, s, n.partial, (long long)L.cout,
+                       chunks, n.g + L.b);
+    if (gin)
+        launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H, L.ks,
+                    false);
+}
+
+void relu_grad(wdg_cnn_s& n, hipStream_t s, int level, int C, const float* g1, int g1_ld, int g1_off, bool up,
+               const float* g2, int g2_ld, const uint16_t* act, int act_ld, int act_off, int B)
+{
+    const int H = n.res(level);
+    PrepArgs a{g1, g2, act, n.dpre, g1_ld, g1_off, up ? 1 : 0, g2_ld, act_ld, act_off, B, H, H, C};
+    hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, a);
+}
+
+void pool(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, int in_ld, int in_off, uint16_t* out, int C,
+          int B)
+{
+    const int H = n.res(level_out);
+    hipLaunchKernelGGL(pool_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, in, in_ld, in_off, out,
+                       B, H, H, C);
+}
+
+void upsample(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, uint16_t* out, int out_ld, int C, int B)
+{
+    const int H = n.res(level_out);
+    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, in, out, out_ld, 0,
+                       B, H, H, C);
+}
+
+void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
+{
+    const int C1 = n.C[0], C2 = n.C[1], C3 = n.C[2], C4 = n.C[3];
+    fwd(n, s, E1A, x, n.in_ld, 0, n.e1a, C1, 0, B);
+    fwd(n, s, E1B, n.e1a, C1, 0, n.cat1, C2 + C1, C2, B);
+    pool(n, s, 1, n.cat1, C2 + C1, C2, n.p1, C1, B);
+    fwd(n, s, E2A, n.p1, C1, 0, n.e2a, C2, 0, B);
+    fwd(n, s, E2B, n.e2a, C2, 0, n.cat2, C3 + C2, C3, B);
+    pool(n, s, 2, n.cat2, C3 + C2, C3, n.p2, C2, B);
+    fwd(n, s, E3A, n.p2, C2, 0, n.e3a, C3, 0, B);
+    fwd(n, s, E3B, n.e3a, C3, 0, n.cat3, C4 + C3, C4, B);
+    pool(n, s, 3, n.cat3, C4 + C3, C4, n.p3, C3, B);
+    fwd(n, s, BA, n.p3, C3, 0, n.ba, C4, 0, B);
+    fwd(n, s, BB, n.ba, C4, 0, n.bb, C4, 0, B);
+    upsample(n, s, 2, n.bb, n.cat3, C4 + C3, C4, B);
+    fwd(n, s, D3A, n.cat3, C4 + C3, 0, n.d3a, C3, 0, B);
+    fwd(n, s, D3B, n.d3a, C3, 0, n.d3, C3, 0, B);
+    upsample(n, s, 1, n.d3, n.cat2, C3 + C2, C3, B);
+    fwd(n, s, D2A, n.cat2, C3 + C2, 0, n.d2a, C2, 0, B);
+    fwd(n, s, D2B, n.d2a, C2, 0, n.d2, C2, 0, B);
+    upsample(n, s, 0, n.d2, n.cat1, C2 + C1, C2, B);
+    fwd(n, s, D1A, n.cat1, C2 + C1, 0, n.d1a, C1, 0, B);
+    fwd(n, s, D1B, n.d1a, C1, 0, n.d1, C1, 0, B);
+    const Layer& L = n.L[OUT];
+    launch_conv(n, s, n.d1, C1, 0, C1, n.wbf + L.w, n.w + L.b, n.y, n.out_ld, 0, n.out_ld, true, B, n.M, n.M, 1, false);
+}
+
+// Gradients of the mean per-sample loss w.r.t. every parameter (into n.g),
+// given n.dy from loss_kernel and the activations of forward().
+void backward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
+{
+    const int C1 = n.C[0], C2 = n.C[1], C3 = n.C[2], C4 = n.C[3];
+    // output 1x1 layer: dpre is dy itself (no activation)
+    bwd(n, s, OUT, n.d1, C1, 0, n.dy, n.gA, B);
+    relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.d1, C1, 0, B);
+    bwd(n, s, D1B, n.d1a, C1, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.d1a, C1, 0, B);
+    bwd(n, s, D1A, n.cat1, C2 + C1, 0, n.dpre, n.gcat1, B);
+    // level 1 decoder: d2 feeds up() into cat1[:, 0:C2]
+    relu_grad(n, s, 1, C2, n.gcat1, C2 + C1, 0, true, nullptr, 0, n.d2, C2, 0, B);
+    bwd(n, s, D2B, n.d2a, C2, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 1, C2, n.gA, C2, 0, false, nullptr, 0, n.d2a, C2, 0, B);
+    bwd(n, s, D2A, n.cat2, C3 + C2, 0, n.dpre, n.gcat2, B);
+    relu_grad(n, s, 2, C3, n.gcat2, C3 + C2, 0, true, nullptr, 0, n.d3, C3, 0, B);
+    bwd(n, s, D3B, n.d3a, C3, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 2, C3, n.gA, C3, 0, false, nullptr, 0, n.d3a, C3, 0, B);
+    bwd(n, s, D3A, n.cat3, C4 + C3, 0, n.dpre, n.gcat3, B);
+    // bottleneck
+    relu_grad(n, s, 3, C4, n.gcat3, C4 + C3, 0, true, nullptr, 0, n.bb, C4, 0, B);
+    bwd(n, s, BB, n.ba, C4, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 3, C4, n.gA, C4, 0, false, nullptr, 0, n.ba, C4, 0, B);
+    bwd(n, s, BA, n.p3, C3, 0, n.dpre, n.gp3, B);
+    // encoder, deepest first: e_l receives the skip gradient and the pool gradient
+    relu_grad(n, s, 2, C3, n.gcat3, C4 + C3, C4, false, n.gp3, C3, n.cat3, C4 + C3, C4, B);
+    bwd(n, s, E3B, n.e3a, C3, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 2, C3, n.gA, C3, 0, false, nullptr, 0, n.e3a, C3, 0, B);
+    bwd(n, s, E3A, n.p2, C2, 0, n.dpre, n.gp2, B);
+    relu_grad(n, s, 1, C2, n.gcat2, C3 + C2, C3, false, n.gp2, C2, n.cat2, C3 + C2, C3, B);
+    bwd(n, s, E2B, n.e2a, C2, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 1, C2, n.gA, C2, 0, false, nullptr, 0, n.e2a, C2, 0, B);
+    bwd(n, s, E2A, n.p1, C1, 0, n.dpre, n.gp1, B);
+    relu_grad(n, s, 0, C1, n.gcat1, C2 + C1, C2, false, n.gp1, C1, n.cat1, C2 + C1, C2, B);
+    bwd(n, s, E1B, n.e1a, C1, 0, n.dpre, n.gA, B);
+    relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.e1a, C1, 0, B);
+    bwd(n, s, E1A, x, n.in_ld, 0, n.dpre, nullptr, B);
+}
+
+void refresh_copies(wdg_cnn_s& n, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n.nparam)), dim3(256), 0, s, n.w, n.wbf, n.nparam);
+    for (const Layer& L : n.L) {
+        const long long nw = (long long)L.ks * L.ks * L.cout * L.cin;
+        hipLaunchKernelGGL(wflip_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.w + L.w, L.cout, L.cin, L.ks * L.ks,
+                           n.wflip + L.wb);
+    }
+}
+
+void loss_grad(wdg_cnn_s& n, hipStream_t s, int B, const uint16_t* x, const float* t)
+{
+    forward(n, s, x, B);
+    hipLaunchKernelGGL(loss_kernel, dim3(B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, (long long)n.M * n.M, B,
+                       n.loss, n.dy);
+    backward(n, s, x, B);
+}
+
+float read_loss(wdg_cnn_s& n, hipStream_t s, int B)
+{
+    std::vector<float> l(B);
+    ck(hipMemcpyAsync(l.data(), n.loss, sizeof(float) * B, hipMemcpyDeviceToHost, s), "copy loss");
+    ck(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double sum = 0.0;
+    for (float v : l) sum += v;
+    return (float)(sum / B);
+}
+
+void check_batch(const wdg_cnn_s* n, int B)
+{
+    if (!n) throw Fail{WDG_ERR_INVALID, "null network handle"};
+    if (B < 1 || B > n->maxB) throw Fail{WDG_ERR_INVALID, "batch size outside 1..max_batch"};
+}
+
+void set_device(const wdg_cnn_s& n)
+{
+    int cur = -1;
+    ck(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != n.dev) ck(hipSetDevice(n.dev), "hipSetDevice");
+}
+
+}  // namespace
+
+extern "C" {
+
+int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int max_batch, wdg_cnn* out)
+{
+    return guard([&] {
+        if (!out) throw Fail{WDG_ERR_INVALID, "null output handle"};
+        *out = nullptr;
+        if (M < 8 || M % 8 != 0) throw Fail{WDG_ERR_INVALID, "image size must be a positive multiple of 8"};
+        if (in_ld < 16 || in_ld % 16 != 0) throw Fail{WDG_ERR_INVALID, "input channels must be a multiple of 16"};
+        if (out_channels < 1) throw Fail{WDG_ERR_INVALID, "output channels must be positive"};
+        if (base_width < 16 || base_width % 16 != 0) throw Fail{WDG_ERR_INVALID, "base width must be a multiple of 16"};
+        if (max_batch < 1) throw Fail{WDG_ERR_INVALID, "max_batch must be positive"};
+        auto n = std::make_unique<wdg_cnn_s>();
+        ck(hipGetDevice(&n->dev), "hipGetDevice");
+        n->M = M;
+        n->in_ld = in_ld;
+        n->out_real = out_channels;
+        n->out_ld = (out_channels + 15) / 16 * 16;
+        n->maxB = max_batch;
+        for (int l = 0; l < 4; ++l) n->C[l] = base_width << l;
+        const int C1 = n->C[0], C2 = n->C[1], C3 = n->C[2], C4 = n->C[3];
+        const int spec[NLAYER][4] = {
+            {in_ld, C1, 3, 0}, {C1, C1, 3, 0}, {C1, C2, 3, 1},      {C2, C2, 3, 1}, {C2, C3, 3, 2},
+            {C3, C3, 3, 2},    {C3, C4, 3, 3}, {C4, C4, 3, 3},      {C4 + C3, C3, 3, 2}, {C3, C3, 3, 2},
+            {C3 + C2, C2, 3, 1}, {C2, C2, 3, 1}, {C2 + C1, C1, 3, 0}, {C1, C1, 3, 0}, {C1, n->out_ld, 1, 0}};
+        long long off = 0, foff = 0, maxpart = 0;
+        for (int i = 0; i < NLAYER; ++i) {
+            Layer& L = n->L[i];
+            L.cin = spec[i][0];
+            L.cout = spec[i][1];
+            L.ks = spec[i][2];
+            L.level = spec[i][3];
+            const long long nw = (long long)L.ks * L.ks * L.cout * L.cin;
+            L.w = off;
+            L.b = off + nw;
+            off += nw + L.cout;
+            L.wb = foff;
+            foff += nw;
+            const int H = M >> L.level;
+            const long long chunks = ((long long)max_batch * H * H + kChunk - 1) / kChunk;
+            maxpart = std::max(maxpart, chunks * std::max<long long>(nw, L.cout));
+        }
+        n->nparam = off;
+        n->nflip = foff;
+        n->partial_len = maxpart;
+        n->w = n->alloc<float>(off);
+        n->m = n->alloc<float>(off);
+        n->v = n->alloc<float>(off);
+        n->g = n->alloc<float>(off);
+        n->wbf = n->alloc<uint16_t>(off);
+        n->wflip = n->alloc<uint16_t>(foff);
+        n->partial = n->alloc<float>(maxpart);
+        n->loss = n->alloc<float>(max_batch);
+        const long long B = max_batch;
+        auto px = [&](int level) { return B * (long long)(M >> level) * (M >> level); };
+        n->e1a = n->alloc<uint16_t>(px(0) * C1);
+        n->cat1 = n->alloc<uint16_t>(px(0) * (C2 + C1));
+        n->d1a = n->alloc<uint16_t>(px(0) * C1);
+        n->d1 = n->alloc<uint16_t>(px(0) * C1);
+        n->p1 = n->alloc<uint16_t>(px(1) * C1);
+        n->e2a = n->alloc<uint16_t>(px(1) * C2);
+        n->cat2 = n->alloc<uint16_t>(px(1) * (C3 + C2));
+        n->d2a = n->alloc<uint16_t>(px(1) * C2);
+        n->d2 = n->alloc<uint16_t>(px(1) * C2);
+        n->p2 = n->alloc<uint16_t>(px(2) * C2);
+        n->e3a = n->alloc<uint16_t>(px(2) * C3);
+        n->cat3 = n->alloc<uint16_t>(px(2) * (C4 + C3));
+        n->d3a = n->alloc<uint16_t>(px(2) * C3);
+        n->d3 = n->alloc<uint16_t>(px(2) * C3);
+        n->p3 = n->alloc<uint16_t>(px(3) * C3);
+        n->ba = n->alloc<uint16_t>(px(3) * C4);
+        n->bb = n->alloc<uint16_t>(px(3) * C4);
+        n->y = n->alloc<float>(px(0) * n->out_ld);
+        n->dy = n->alloc<uint16_t>(px(0) * n->out_ld);
+        long long maxg = 0;
+        for (int l = 0; l < 4; ++l) maxg = std::max(maxg, px(l) * (long long)n->C[l]);
+        n->dpre = n->alloc<uint16_t>(std::max(maxg, px(0) * n->out_ld));
+        n->gA = n->alloc<float>(maxg);
+        n->gcat1 = n->alloc<float>(px(0) * (C2 + C1));
+        n->gcat2 = n->alloc<float>(px(1) * (C3 + C2));
+        n->gcat3 = n->alloc<float>(px(2) * (C4 + C3));
+        n->gp1 = n->alloc<float>(px(1) * C1);
+        n->gp2 = n->alloc<float>(px(2) * C2);
+        n->gp3 = n->alloc<float>(px(3) * C3);
+        ck(hipMemset(n->w, 0, sizeof(float) * off), "hipMemset");
+        ck(hipMemset(n->m, 0, sizeof(float) * off), "hipMemset");
+        ck(hipMemset(n->v, 0, sizeof(float) * off), "hipMemset");
+        refresh_copies(*n, nullptr);
+        ck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        *out = n.release();
+    });
+}
+
+int wdg_hip_cnn_destroy(wdg_cnn n)
+{
+    delete n;
+    return WDG_OK;
+}
+
+int wdg_hip_cnn_info(wdg_cnn n, long long* n_params, int* in_ld, int* out_ld)
+{
+    return guard([&] {
+        if (!n) throw Fail{WDG_ERR_INVALID, "null network handle"};
+        if (n_params) *n_params = n->nparam;
+        if (in_ld) *in_ld = n->in_ld;
+        if (out_ld) *out_ld = n->out_ld;
+    });
+}
+
+int wdg_hip_cnn_set_params(wdg_cnn n, const float* params)
+{
+    return guard([&] {
+        if (!n || !params) throw Fail{WDG_ERR_INVALID, "null argument"};
+        set_device(*n);
+        ck(hipMemcpy(n->w, params, sizeof(float) * n->nparam, hipMemcpyHostToDevice), "upload params");
+        ck(hipMemset(n->m, 0, sizeof(float) * n->nparam), "hipMemset");
+        ck(hipMemset(n->v, 0, sizeof(float) * n->nparam), "hipMemset");
+        n->step = 0;
+        refresh_copies(*n, nullptr);
+        ck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
+int wdg_hip_cnn_get_params(wdg_cnn n, float* params)
+{
+    return guard([&] {
+        if (!n || !params) throw Fail{WDG_ERR_INVALID, "null argument"};
+        set_device(*n);
+        ck(hipMemcpy(params, n->w, sizeof(float) * n->nparam, hipMemcpyDeviceToHost), "download params");
+    });
+}
+
+int wdg_hip_cnn_forward(wdg_cnn n, int batch, const uint16_t* d_x, float* d_y, void* stream)
+{
+    return guard([&] {
+        check_batch(n, batch);
+        if (!d_x || !d_y) throw Fail{WDG_ERR_INVALID, "null device pointer"};
+        set_device(*n);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        forward(*n, s, d_x, batch);
+        ck(hipMemcpyAsync(d_y, n->y, sizeof(float) * batch * (long long)n->M * n->M * n->out_ld,
+                          hipMemcpyDeviceToDevice, s),
+           "copy output");
+        ck(hipGetLastError(), "forward");
+    });
+}
+
+int wdg_hip_cnn_loss_grad(wdg_cnn n, int batch, const uint16_t* d_x, const float* d_t, float* loss, float* grad,
+                          void* stream)
+{
+    return guard([&] {
+        check_batch(n, batch);
+        if (!d_x || !d_t) throw Fail{WDG_ERR_INVALID, "null device pointer"};
+        set_device(*n);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        loss_grad(*n, s, batch, d_x, d_t);
+        ck(hipGetLastError(), "loss_grad");
+        if (grad)
+            ck(hipMemcpyAsync(grad, n->g, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, s), "download gradient");
+        const float l = read_loss(*n, s, batch);
+        if (loss) *loss = l;
+    });
+}
+
+int wdg_hip_cnn_train_step(wdg_cnn n, int batch, const uint16_t* d_x, const float* d_t, float lr, float* loss,
+                           void* stream)
+{
+    return guard([&] {
+        check_batch(n, batch);
+        if (!d_x || !d_t) throw Fail{WDG_ERR_INVALID, "null device pointer"};
+        if (!(lr > 0.f)) throw Fail{WDG_ERR_INVALID, "learning rate must be positive"};
+        set_device(*n);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        loss_grad(*n, s, batch, d_x, d_t);
+        n->step += 1;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)n->step)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)n->step)));
+        hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n->nparam)), dim3(256), 0, s, n->w, n->m, n->v, n->g, n->wbf,
+                           n->nparam, lr, b1, b2, eps, c1, c2);
+        for (const Layer& L : n->L) {
+            const long long nw = (long long)L.ks * L.ks * L.cout * L.cin;
+            hipLaunchKernelGGL(wflip_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n->w + L.w, L.cout, L.cin,
+                               L.ks * L.ks, n->wflip + L.wb);
+        }
+        ck(hipGetLastError(), "train_step");
+        if (loss) *loss = read_loss(*n, s, batch);
+    });
+}
+
+int wdg_hip_f32_to_bf16(const float* d_src, uint16_t* d_dst, long long n, void* stream)
+{
+    return guard([&] {
+        if (n < 0 || (n && (!d_src || !d_dst))) throw Fail{WDG_ERR_INVALID, "bad arguments"};
+        if (n == 0) return;
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream), d_src,
+                           d_dst, n);
+        ck(hipGetLastError(), "to_bf16");
+    });
+}
+
+int wdg_hip_ws_features(const double* d_nodes, int n_colloc, int nel, int n_sources, const double* d_sources,
+                        double gain, double omega_feature, const double* box, int ld, uint16_t* d_x, void* stream)
+{
+    return guard([&] {
+        if (!d_nodes || !d_x || !box || (n_sources > 0 && !d_sources)) throw Fail{WDG_ERR_INVALID, "null argument"};
+        if (n_colloc < 2 || nel < 1 || n_sources < 0 || ld < n_colloc * n_colloc + 3)
+            throw Fail{WDG_ERR_INVALID, "bad sizes (ld must hold n^2 + 3 channels)"};
+        if (!(box[2] > box[0]) || !(box[3] > box[1])) throw Fail{WDG_ERR_INVALID, "empty bounding box"};
+        hipLaunchKernelGGL(features_kernel, dim3(blocks_for((long long)nel * ld)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), d_nodes, n_colloc, nel, d_sources, n_sources, gain,
+                           (float)omega_feature, box[0], box[1], box[2], box[3], d_x, ld);
+        ck(hipGetLastError(), "features");
+    });
+}
+
+int wdg_hip_ws_dg_to_image(const double* d_W, int n_colloc, int nel, double scale, int ld, float* d_t, void* stream)
+{
+    return guard([&] {
+        const int nd = 3 * n_colloc * n_colloc;
+        if (!d_W || !d_t || nel < 1 || ld < nd || !(scale > 0.0)) throw Fail{WDG_ERR_INVALID, "bad arguments"};
+        hipLaunchKernelGGL(dg_to_image_kernel, dim3(blocks_for((long long)nel * ld)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), d_W, nd, (long long)nel, 1.0 / scale, d_t, ld);
+        ck(hipGetLastError(), "dg_to_image");
+    });
+}
+
+int wdg_hip_ws_image_to_dg(const float* d_y, int ld, int n_colloc, int nel, double scale, double* d_W, void* stream)
+{
+    return guard([&] {
+        const int nd = 3 * n_colloc * n_colloc;
+        if (!d_W || !d_y || nel < 1 || ld < nd) throw Fail{WDG_ERR_INVALID, "bad arguments"};
+        hipLaunchKernelGGL(image_to_dg_kernel, dim3(blocks_for((long long)nel * nd)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), d_y, ld, nd, (long long)nel, scale, d_W);
+        ck(hipGetLastError(), "image_to_dg");
+    });
+}
+
+}  // extern "C"

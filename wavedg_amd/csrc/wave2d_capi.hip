@@ -8,8 +8,11 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -83,6 +86,15 @@ void dfree(void* p)
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+// Device buffer owned by one scope.
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) : p(dmalloc(bytes)) {}
+    ~DevBuf() { dfree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
 // Carpenter-Kennedy (5,4) 2N-storage coefficients (ode.hpp:68-70).
 constexpr double RK4A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
                             -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
@@ -125,6 +137,8 @@ struct wdg_wave2d_s {
     int last_n = 0;
     hipStream_t cap_stream = nullptr;
     std::map<std::tuple<void*, int, double>, hipGraphExec_t> graphs;
+    int sched = 0;     // workgroup schedule of the RK stage kernels (wdg::kSchedTgt)
+    std::string arch;  // device architecture name, key of the tuned-schedule table
 
     ~wdg_wave2d_s()
     {
@@ -196,17 +210,17 @@ void check_op(wdg_wave2d op)
     op->use_device();
 }
 
-void launch_wave(int n, int kid, int f32, const KArgs& a, hipStream_t s)
+void launch_wave(int n, int kid, int f32, int sched, const KArgs& a, hipStream_t s)
 {
     switch (n) {
-    case 2: wdg::launch_wave_n2(kid, f32, a, s); break;
-    case 3: wdg::launch_wave_n3(kid, f32, a, s); break;
-    case 4: wdg::launch_wave_n4(kid, f32, a, s); break;
-    case 5: wdg::launch_wave_n5(kid, f32, a, s); break;
-    case 6: wdg::launch_wave_n6(kid, f32, a, s); break;
-    case 7: wdg::launch_wave_n7(kid, f32, a, s); break;
-    case 8: wdg::launch_wave_n8(kid, f32, a, s); break;
-    case 9: wdg::launch_wave_n9(kid, f32, a, s); break;
+    case 2: wdg::launch_wave_n2(kid, f32, sched, a, s); break;
+    case 3: wdg::launch_wave_n3(kid, f32, sched, a, s); break;
+    case 4: wdg::launch_wave_n4(kid, f32, sched, a, s); break;
+    case 5: wdg::launch_wave_n5(kid, f32, sched, a, s); break;
+    case 6: wdg::launch_wave_n6(kid, f32, sched, a, s); break;
+    case 7: wdg::launch_wave_n7(kid, f32, sched, a, s); break;
+    case 8: wdg::launch_wave_n8(kid, f32, sched, a, s); break;
+    case 9: wdg::launch_wave_n9(kid, f32, sched, a, s); break;
     default: fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
     }
     hip_check(hipGetLastError(), "wave kernel launch");
@@ -392,7 +406,7 @@ void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const
                 const double ts = (st == 0) ? t : t + RK4C[st] * dt;
                 a.fs = -1.0 / omega * std::sin(omega * ts);  // WaveHoltz.cpp:65
             }
-            launch_wave(op->n, kid, f32, a, s);
+            launch_wave(op->n, kid, f32, op->sched, a, s);
             std::swap(cur, nxt);
         }
         t += dt;
@@ -403,6 +417,74 @@ void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const
 struct NoCoef {
     double operator()(int, double) const { return 0.0; }
 };
+
+// ------------------------------------------------------- schedule autotuner
+// Tuned RK-stage schedules, process-wide: (architecture, n_colloc,
+// precision) -> index into wdg::kSchedTgt.  New handles start from it.
+std::mutex g_tune_mu;
+std::map<std::tuple<std::string, int, int>, int> g_tune;
+
+std::string device_arch(int dev)
+{
+    hipDeviceProp_t p;
+    hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+    std::string a = p.gcnArchName;
+    const size_t c = a.find(':');
+    return c == std::string::npos ? a : a.substr(0, c);
+}
+
+int tuned_schedule(const std::string& arch, int n, int prec)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find({arch, n, prec});
+    return it == g_tune.end() ? 0 : it->second;
+}
+
+// Deterministic pseudo-random fill in [-1, 1] (SplitMix64 of the index).
+template <typename R>
+__global__ void k_fill_hash(R* x, long long n, unsigned long long seed)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = seed + (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (R)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+}
+
+// Counts positions where a and b differ bitwise.
+template <typename U>
+__global__ void k_count_diff(const U* a, const U* b, long long n, unsigned int* count)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != b[i]) atomicAdd(count, 1u);
+}
+
+void fill_hash(int prec, void* x, long long n, unsigned long long seed, hipStream_t s)
+{
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (prec == WDG_F32)
+        hipLaunchKernelGGL(k_fill_hash<float>, dim3(g), dim3(256), 0, s, static_cast<float*>(x), n, seed);
+    else
+        hipLaunchKernelGGL(k_fill_hash<double>, dim3(g), dim3(256), 0, s, static_cast<double*>(x), n, seed);
+}
+
+unsigned int count_diff(int prec, const void* a, const void* b, long long n, unsigned int* d_count, hipStream_t s)
+{
+    hip_check(hipMemsetAsync(d_count, 0, sizeof(unsigned int), s), "hipMemsetAsync");
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (prec == WDG_F32)
+        hipLaunchKernelGGL(k_count_diff<uint32_t>, dim3(g), dim3(256), 0, s, static_cast<const uint32_t*>(a),
+                           static_cast<const uint32_t*>(b), n, d_count);
+    else
+        hipLaunchKernelGGL(k_count_diff<uint64_t>, dim3(g), dim3(256), 0, s, static_cast<const uint64_t*>(a),
+                           static_cast<const uint64_t*>(b), n, d_count);
+    unsigned int h = 0;
+    hip_check(hipMemcpyAsync(&h, d_count, sizeof(h), hipMemcpyDeviceToHost, s), "copy count");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return h;
+}
 
 void start_timing(wdg_wave2d op, hipStream_t s)
 {
@@ -640,6 +722,8 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
         op->ha = -0.5 * a;  // reference sign convention (EdgeFlux.hpp:51)
         op->hb = -b;
         op->min_h = m.min_h();
+        op->arch = device_arch(op->dev);
+        op->sched = tuned_schedule(op->arch, n_colloc, precision);
 
         const int n = n_colloc;
         std::vector<double> basis(2 * n + n * n);
@@ -722,7 +806,7 @@ static int op_launch(wdg_wave2d op, int kid, const void* x, void* y, const void*
         a.u_out = y;
         a.force = F;
         a.fs = s;
-        launch_wave(op->n, kid, op->prec == WDG_F32, a, as_stream(stream));
+        launch_wave(op->n, kid, op->prec == WDG_F32, op->sched, a, as_stream(stream));
     });
 }
 
@@ -1004,7 +1088,7 @@ int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double* d_H, const double* d
         KArgs a = op->base_args();
         a.u_in = x;
         a.u_out = op->d_r2;
-        launch_wave(op->n, wdg::K_RHS, op->prec == WDG_F32, a, s);
+        launch_wave(op->n, wdg::K_RHS, op->prec == WDG_F32, op->sched, a, s);
         if (op->prec == WDG_F32)
             hipLaunchKernelGGL(k_post<float>, dim3(grid_for(n / 3)), dim3(256), 0, s, d_w,
                                static_cast<const float*>(op->d_r2), d_H, wh->omega, n / 3);
@@ -1102,6 +1186,137 @@ int wdg_hip_convert(const void* d_src, int src_precision, void* d_dst, int dst_p
         else
             fail(WDG_ERR_INVALID, "precision must be WDG_F64 or WDG_F32");
         hip_check(hipGetLastError(), "convert launch");
+    });
+}
+
+
+int wdg_hip_schedule_count(int* count, int* threads)
+{
+    return guarded([&] {
+        if (count) *count = wdg::kSchedules;
+        if (threads)
+            for (int k = 0; k < wdg::kSchedules; ++k) threads[k] = wdg::kSchedTgt[k];
+    });
+}
+
+int wdg_hip_wave2d_set_schedule(wdg_wave2d op, int schedule)
+{
+    return guarded([&] {
+        check_op(op);
+        if (schedule < 0 || schedule >= wdg::kSchedules) fail(WDG_ERR_INVALID, "schedule out of range");
+        op->sched = schedule;
+        for (auto& kv : op->graphs) (void)hipGraphExecDestroy(kv.second);
+        op->graphs.clear();
+    });
+}
+
+int wdg_hip_wave2d_get_schedule(wdg_wave2d op, int* schedule, int* threads_per_workgroup)
+{
+    return guarded([&] {
+        check_op(op);
+        if (schedule) *schedule = op->sched;
+        if (threads_per_workgroup) {
+            const int nn = op->n * op->n, epb = std::max(1, wdg::kSchedTgt[op->sched] / nn);
+            *threads_per_workgroup = epb * nn;
+        }
+    });
+}
+
+int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int* schedule, double* ms)
+{
+    return guarded([&] {
+        check_op(op);
+        if (reps < 1) fail(WDG_ERR_INVALID, "reps must be positive");
+        const hipStream_t s = nullptr;
+        const long long n = op->ndof;
+        const size_t bytes = op->state_bytes();
+        const int f32 = op->prec == WDG_F32;
+        // private buffers: input state, RK register, outputs of the default schedule
+        DevBuf u(bytes), du(bytes), du_ref(bytes), out(bytes), out_ref(bytes), cnt(sizeof(unsigned int));
+        KArgs a = op->base_args();
+        a.u_in = u.p;
+        a.u_out = out.p;
+        a.du = du.p;
+        a.rk_a = RK4A[2];
+        a.rk_b = RK4B[2];
+        a.dt = 0.25 * op->min_h / (op->n * op->n);
+        fill_hash(op->prec, u.p, n, 11, s);
+        std::vector<double> t(wdg::kSchedules, -1.0);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        int best = 0;
+        for (int k = 0; k < wdg::kSchedules; ++k) {
+            // parity: one stage from the same input must equal the default schedule bit for bit
+            fill_hash(op->prec, du.p, n, 23, s);
+            launch_wave(op->n, wdg::K_STAGE, f32, k, a, s);
+            if (k == 0) {
+                hip_check(hipMemcpyAsync(out_ref.p, out.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+                hip_check(hipMemcpyAsync(du_ref.p, du.p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+            } else {
+                const unsigned d1 = count_diff(op->prec, out.p, out_ref.p, n, static_cast<unsigned*>(cnt.p), s);
+                const unsigned d2 = count_diff(op->prec, du.p, du_ref.p, n, static_cast<unsigned*>(cnt.p), s);
+                if (d1 || d2) continue;  // not eligible
+            }
+            for (int w = 0; w < 2; ++w) launch_wave(op->n, wdg::K_STAGE, f32, k, a, s);
+            hip_check(hipEventRecord(e0, s), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) launch_wave(op->n, wdg::K_STAGE, f32, k, a, s);
+            hip_check(hipEventRecord(e1, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float el = 0.f;
+            hip_check(hipEventElapsedTime(&el, e0, e1), "hipEventElapsedTime");
+            t[k] = el / reps;
+            if (t[k] < t[best]) best = k;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        op->sched = best;
+        for (auto& kv : op->graphs) (void)hipGraphExecDestroy(kv.second);
+        op->graphs.clear();
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[{op->arch, op->n, op->prec}] = best;
+        }
+        if (schedule) *schedule = best;
+        if (ms)
+            for (int k = 0; k < wdg::kSchedules; ++k) ms[k] = t[k];
+    });
+}
+
+int wdg_hip_tune_cache_save(const char* path)
+{
+    return guarded([&] {
+        if (!path) fail(WDG_ERR_INVALID, "null path");
+        std::ofstream f(path);
+        if (!f) fail(WDG_ERR_INVALID, std::string("cannot write ") + path);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        f << "# arch n_colloc precision(0=f64,1=f32) schedule threads\n";
+        for (const auto& kv : g_tune)
+            f << std::get<0>(kv.first) << ' ' << std::get<1>(kv.first) << ' ' << std::get<2>(kv.first) << ' '
+              << kv.second << ' ' << wdg::kSchedTgt[kv.second] << '\n';
+    });
+}
+
+int wdg_hip_tune_cache_load(const char* path)
+{
+    return guarded([&] {
+        if (!path) fail(WDG_ERR_INVALID, "null path");
+        std::ifstream f(path);
+        if (!f) fail(WDG_ERR_INVALID, std::string("cannot read ") + path);
+        std::map<std::tuple<std::string, int, int>, int> add;
+        std::string line;
+        while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream is(line);
+            std::string arch;
+            int n = 0, prec = 0, sched = 0;
+            if (!(is >> arch >> n >> prec >> sched) || n < 2 || n > 9 || (prec != WDG_F64 && prec != WDG_F32) ||
+                sched < 0 || sched >= wdg::kSchedules)
+                fail(WDG_ERR_INVALID, "malformed tune cache line: " + line);
+            add[{arch, n, prec}] = sched;
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto& kv : add) g_tune[kv.first] = kv.second;
     });
 }
 

@@ -103,12 +103,19 @@ __device__ __forceinline__ void st_stream(T* p, T v)
 #endif
 }
 
-template <int N>
+// Workgroup schedule: about TGT threads, i.e. EPB = TGT / n^2 whole elements
+// (at least one).  TGT = 64 at n = 8 is one wavefront per element; smaller
+// orders pack several elements into each wavefront.
+template <int N, int TGT = 256>
 struct Shape {
     static constexpr int NN = N * N;
-    static constexpr int EPB = (256 / NN) > 0 ? (256 / NN) : 1;  // elements per workgroup
+    static constexpr int EPB = (TGT / NN) > 0 ? (TGT / NN) : 1;  // elements per workgroup
     static constexpr int NT = EPB * NN;                          // threads per workgroup
 };
+
+// Schedules the stage kernels can be launched with (index -> TGT).
+constexpr int kSchedules = 4;
+constexpr int kSchedTgt[kSchedules] = {256, 512, 128, 64};
 
 // Workgroups are dealt to the 8 XCDs round-robin (b -> XCD b mod 8).  Give
 // each XCD one contiguous run of elements so that an element's neighbours
@@ -138,10 +145,10 @@ __device__ __forceinline__ int side_node(int s, int k)
     return i + N * j;
 }
 
-// LDS carve-up of apply_operator for one workgroup of Shape<N> elements.
-template <int N, typename R, bool VOL, bool FACE>
+// LDS carve-up of apply_operator for one workgroup of Shape<N, TGT> elements.
+template <int N, typename R, bool VOL, bool FACE, int TGT = 256>
 struct OpLds {
-    static constexpr int NN = N * N, EPB = Shape<N>::EPB, NF = 4 * N;
+    static constexpr int NN = N * N, EPB = Shape<N, TGT>::EPB, NF = 4 * N;
     static constexpr size_t n_u = (size_t)EPB * 3 * NN;
     static constexpr size_t n_F = VOL ? n_u : 0, n_G = VOL ? n_u : 0;
     static constexpr size_t n_S = FACE ? (size_t)EPB * NF * 3 : 0;
@@ -164,11 +171,11 @@ struct NodeResult {
 };
 
 // The operator for one workgroup of elements [e0, e0 + nloc) of one mesh,
-// run by threads 0 .. Shape<N>::NT-1 of the workgroup (threads beyond only
+// run by threads 0 .. Shape<N, TGT>::NT-1 of the workgroup (threads beyond only
 // join the barriers).  Element indices in nbr are local to u_in / geom.
 // `prefetch(active, g)` is called after the state loads are issued and
 // before the first barrier, so the caller's extra loads overlap them.
-template <int N, typename R, bool VOL, bool IFACE, bool BFACE, typename Prefetch>
+template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, typename Prefetch>
 __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const double* __restrict__ geom,
                                                         const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ info,
@@ -177,12 +184,12 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                                                         size_t& g_out, bool& active_out, Prefetch&& prefetch)
 {
     constexpr int NN = N * N;
-    constexpr int EPB = Shape<N>::EPB;
-    constexpr int NT = Shape<N>::NT;
+    constexpr int EPB = Shape<N, TGT>::EPB;
+    constexpr int NT = Shape<N, TGT>::NT;
     constexpr int NF = 4 * N;               // face nodes per element
     constexpr int FP = (NF + NN - 1) / NN;  // face nodes per thread (1 for N >= 4)
     constexpr bool FACE = IFACE || BFACE;
-    using L = OpLds<N, R, VOL, FACE>;
+    using L = OpLds<N, R, VOL, FACE, TGT>;
     double* s_xw = reinterpret_cast<double*>(lds + L::off_xw);
     R* s_u = reinterpret_cast<R*>(lds + L::off_u);
     R* s_F = reinterpret_cast<R*>(lds + L::off_F);
@@ -412,10 +419,10 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     return out;
 }
 
-template <int N, typename R, int MODE>
-__global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const KArgs a)
+template <int N, typename R, int MODE, int TGT = 256>
+__global__ __launch_bounds__((Shape<N, TGT>::NT), WDG_MIN_WAVES) void wave_kernel(const KArgs a)
 {
-    constexpr int EPB = Shape<N>::EPB;
+    constexpr int EPB = Shape<N, TGT>::EPB;
     constexpr bool VOL = (MODE & M_VOL) != 0;
     constexpr bool IFACE = (MODE & M_IF) != 0;
     constexpr bool BFACE = (MODE & M_BF) != 0;
@@ -423,7 +430,7 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
     constexpr bool FORCE = (MODE & M_FORCE) != 0;
     constexpr bool ACC = (MODE & M_ACC) != 0;
     constexpr bool READ_DU = OUT == M_OUT_STAGE;
-    using L = OpLds<N, R, VOL, IFACE || BFACE>;
+    using L = OpLds<N, R, VOL, IFACE || BFACE, TGT>;
     __shared__ __align__(16) unsigned char lds[L::bytes];
 
     const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(Shape<N>::NT, WDG_MIN_WAVES) void wave_kernel(const
     int el;
     size_t g;
     bool active;
-    NodeResult<R> q = apply_operator<N, R, VOL, IFACE, BFACE>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info,
+    NodeResult<R> q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info,
                                                               a.basis, a.ha, a.hb, e0, nloc, lds, el, g, active,
                                                               prefetch);
     if (!active) return;
@@ -580,7 +587,7 @@ __global__ __launch_bounds__(Shape<N>::NT) void load_vector_kernel(const double*
 
 // Host-side launchers, one translation unit per N (wave2d_kernels_n.hip).
 #define WDG_DECLARE_LAUNCH(NV)                                                                              \
-    void launch_wave_n##NV(int kid, int f32, const KArgs& a, hipStream_t s);                              \
+    void launch_wave_n##NV(int kid, int f32, int sched, const KArgs& a, hipStream_t s);                              \
     void launch_load_vector_n##NV(const double* corners, const double* tab, const double* src, int nsrc, \
                                   int var, double* F, int nel, hipStream_t s);
 WDG_DECLARE_LAUNCH(2)

@@ -10,21 +10,17 @@ namespace wdg {
 
 namespace {
 
-template <typename R>
-void launch_typed(int kid, const KArgs& a, hipStream_t s)
+template <typename R, int TGT>
+void launch_sched(int kid, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
-    const int nblk = (a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
-    const dim3 grid(nblk), block(Shape<N>::NT);
+    using S = Shape<N, TGT>;
+    const dim3 grid((a.nel + S::EPB - 1) / S::EPB), block(S::NT);
     switch (kid) {
-#define WDG_CASE(K)                                                              \
-    case K:                                                                      \
-        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K]>), grid, block, 0, s, a); \
+#define WDG_CASE(K)                                                                   \
+    case K:                                                                           \
+        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K], TGT>), grid, block, 0, s, a); \
         break;
-        WDG_CASE(K_ACTION)
-        WDG_CASE(K_BC)
-        WDG_CASE(K_RHS)
-        WDG_CASE(K_RHS_F)
         WDG_CASE(K_STAGE0)
         WDG_CASE(K_STAGE0_F)
         WDG_CASE(K_STAGE)
@@ -36,18 +32,54 @@ void launch_typed(int kid, const KArgs& a, hipStream_t s)
     }
 }
 
+template <typename R>
+void launch_typed(int kid, int sched, const KArgs& a, hipStream_t s)
+{
+    constexpr int N = WDG_N;
+    switch (kid) {
+    case K_STAGE0:
+    case K_STAGE0_F:
+    case K_STAGE:
+    case K_STAGE_F:
+    case K_STAGE_ACC:
+    case K_STAGE_F_ACC:
+        // the RK stage kernels run with the schedule picked for this order
+        switch (sched) {
+        case 1: launch_sched<R, kSchedTgt[1]>(kid, a, s); return;
+        case 2: launch_sched<R, kSchedTgt[2]>(kid, a, s); return;
+        case 3: launch_sched<R, kSchedTgt[3]>(kid, a, s); return;
+        default: launch_sched<R, kSchedTgt[0]>(kid, a, s); return;
+        }
+    default: break;
+    }
+    const int nblk = (a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
+    const dim3 grid(nblk), block(Shape<N>::NT);
+    switch (kid) {
+#define WDG_CASE(K)                                                              \
+    case K:                                                                      \
+        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K]>), grid, block, 0, s, a); \
+        break;
+        WDG_CASE(K_ACTION)
+        WDG_CASE(K_BC)
+        WDG_CASE(K_RHS)
+        WDG_CASE(K_RHS_F)
+#undef WDG_CASE
+    default: break;
+    }
+}
+
 }  // namespace
 
 #define WDG_CAT2(a, b) a##b
 #define WDG_CAT(a, b) WDG_CAT2(a, b)
 
-void WDG_CAT(launch_wave_n, WDG_N)(int kid, int f32, const KArgs& a, hipStream_t s)
+void WDG_CAT(launch_wave_n, WDG_N)(int kid, int f32, int sched, const KArgs& a, hipStream_t s)
 {
     if (a.nel <= 0) return;
     if (f32)
-        launch_typed<float>(kid, a, s);
+        launch_typed<float>(kid, sched, a, s);
     else
-        launch_typed<double>(kid, a, s);
+        launch_typed<double>(kid, sched, a, s);
 }
 
 void WDG_CAT(launch_load_vector_n, WDG_N)(const double* corners, const double* tab, const double* src, int nsrc,

@@ -164,3 +164,16 @@ def config4(count: int = 64, nmin: int = 32, nmax: int = 256, seed: int = 4000):
         out.append(dict(name=f"p{k}_{nx}x{nx}_P{P}", vertices=v, elements=e, n=P + 1, omega=omega,
                         sources=np.column_stack([c, sig, amp])))
     return out
+
+
+def config5_problem(seed: int):
+    """Forcing of one learned-warm-start problem on the unit square: 1-4
+    Gaussian sources with centres uniform in [0.1, 0.9]^2, sigma uniform in
+    [0.005, 0.03] and N(0, 1) amplitudes; omega uniform in [5 pi, 20 pi]."""
+    rng = SplitMix64(seed)
+    omega = float(rng.uniform(1, 5.0 * np.pi, 20.0 * np.pi)[0])
+    ns = int(rng.integers(1, 1, 4)[0])
+    c = rng.uniform(2 * ns, 0.1, 0.9).reshape(ns, 2)
+    sig = rng.uniform(ns, 0.005, 0.03)
+    amp = rng.normal(ns)
+    return dict(omega=omega, sources=np.column_stack([c, sig, amp]))

@@ -1,0 +1,340 @@
+"""Learned warm start for WaveHoltz (the reference has no learned component).
+
+Problems live on the unit square meshed by a uniform M x M quad grid, with
+n = 5 Gauss-Lobatto points per direction (P = 4), characteristic outflow on
+every wall, and omega and 1-4 Gaussian sources drawn by
+``meshgen.config5_problem(seed)``.  The network (``capi.CNN``, kernels in
+csrc/cnn.hip) reads one image per problem -- pixel e is element e, channels
+0..24 the forcing at the element's nodes scaled by h^2 / s, channel 25 omega
+mapped to [-1, 1], channels 26-27 the element centre -- and predicts W / s
+with W the WaveHoltz solution in the DG layout (75 values per element) and s
+the total source mass sum_k |amp_k| 2 pi sigma_k^2.  Prediction times s is
+handed to the batched solver as the first iterate (wdg_hip_batch_set_initial);
+the WaveHoltz iteration then runs to the requested residual as without the
+warm start, so the network never decides the answer.
+
+Training data come from the batched WaveHoltz solver itself
+(``generate``), training runs the network's own forward/backward/Adam kernels
+(``train``), and ``evaluate`` compares iterations and solve time to the
+tolerance with and without the warm start on held-out seeds.
+
+Command line::
+
+    python -m wavedg_amd.warmstart train --M 64 --n-train 1024 --steps 3000 --out weights.npz
+    python -m wavedg_amd.warmstart eval --weights weights.npz --n-eval 128
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import capi, meshgen
+
+N_COLLOC = 5
+IN_LD = 32
+OMEGA_LO, OMEGA_HI = 5.0 * math.pi, 20.0 * math.pi
+TRAIN_SEED0 = 50000  # training seeds TRAIN_SEED0 + k
+EVAL_SEED0 = 90000  # held-out seeds EVAL_SEED0 + k
+
+
+def source_scale(sources) -> float:
+    s = np.asarray(sources, dtype=np.float64).reshape(-1, 4)
+    return float(np.sum(np.abs(s[:, 3]) * 2.0 * math.pi * s[:, 2] ** 2))
+
+
+def init_params(base: int, seed: int = 0, n: int = N_COLLOC) -> np.ndarray:
+    """He-normal weights (std sqrt(2 / fan_in); sqrt(1 / fan_in) for the linear
+    output layer), zero biases, in the parameter layout of wdg_hip_cnn_*:
+    per layer in network order, weights [tap][cout][cin] then bias [cout]."""
+    c = [base, 2 * base, 4 * base, 8 * base]
+    out_ld = (3 * n * n + 15) // 16 * 16
+    layers = [(IN_LD, c[0], 3), (c[0], c[0], 3), (c[0], c[1], 3), (c[1], c[1], 3), (c[1], c[2], 3), (c[2], c[2], 3),
+              (c[2], c[3], 3), (c[3], c[3], 3), (c[3] + c[2], c[2], 3), (c[2], c[2], 3), (c[2] + c[1], c[1], 3),
+              (c[1], c[1], 3), (c[1] + c[0], c[0], 3), (c[0], c[0], 3), (c[0], out_ld, 1)]
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i, (cin, cout, ks) in enumerate(layers):
+        fan_in = ks * ks * cin
+        std = math.sqrt((1.0 if i == len(layers) - 1 else 2.0) / fan_in)
+        parts.append((rng.standard_normal(ks * ks * cout * cin) * std).astype(np.float32))
+        parts.append(np.zeros(cout, dtype=np.float32))
+    return np.concatenate(parts)
+
+
+def omega_feature(omega: float) -> float:
+    return (omega - 0.5 * (OMEGA_LO + OMEGA_HI)) / (0.5 * (OMEGA_HI - OMEGA_LO))
+
+
+class WarmStart:
+    """Network plus the DG <-> image maps for one grid size M."""
+
+    def __init__(self, M: int, base: int = 32, max_batch: int = 32, n: int = N_COLLOC):
+        import torch
+
+        self.torch = torch
+        self.M, self.n, self.base, self.max_batch = M, n, base, max_batch
+        self.nel = M * M
+        self.ndof = 3 * n * n * self.nel
+        self.mesh = capi.Mesh.uniform_rect(M, 0.0, 1.0, M, 0.0, 1.0)
+        self.bc = np.zeros(self.mesh.n_boundary, dtype=np.int32)
+        self.nodes = torch.tensor(self.mesh.nodes(n).reshape(-1), device="cuda")
+        self.net = capi.CNN(M, IN_LD, 3 * n * n, base, max_batch)
+        self.out_ld = self.net.out_ld
+
+    def problem(self, spec) -> dict:
+        return dict(mesh=self.mesh, n=self.n, omega=spec["omega"], sources=spec["sources"], bc=self.bc)
+
+    def features(self, specs, out=None):
+        torch = self.torch
+        if out is None:
+            out = torch.empty((len(specs), self.M, self.M, IN_LD), dtype=torch.bfloat16, device="cuda")
+        for b, spec in enumerate(specs):
+            src = np.asarray(spec["sources"], dtype=np.float64).reshape(-1, 4)
+            sd = torch.tensor(src.reshape(-1), device="cuda")
+            gain = (1.0 / self.M) ** 2 / source_scale(src)
+            capi.ws_features(self.nodes, self.n, self.nel, sd, src.shape[0], gain, omega_feature(spec["omega"]),
+                             (0.0, 0.0, 1.0, 1.0), IN_LD, out[b])
+        return out
+
+    def targets(self, Ws, specs, out=None):
+        torch = self.torch
+        if out is None:
+            out = torch.empty((len(specs), self.M, self.M, self.out_ld), dtype=torch.float32, device="cuda")
+        for b, (W, spec) in enumerate(zip(Ws, specs)):
+            capi.ws_dg_to_image(W, self.n, self.nel, source_scale(spec["sources"]), self.out_ld, out[b])
+        return out
+
+    def predict(self, specs):
+        """Warm-start iterates (float64 device vectors in the DG layout)."""
+        torch = self.torch
+        out = []
+        for c0 in range(0, len(specs), self.max_batch):
+            chunk = specs[c0:c0 + self.max_batch]
+            x = self.features(chunk)
+            y = torch.empty((len(chunk), self.M, self.M, self.out_ld), dtype=torch.float32, device="cuda")
+            self.net.forward(x, y)
+            for b, spec in enumerate(chunk):
+                W = torch.empty(self.ndof, dtype=torch.float64, device="cuda")
+                capi.ws_image_to_dg(y[b], self.out_ld, self.n, self.nel, source_scale(spec["sources"]), W)
+                out.append(W)
+        return out
+
+    def save(self, path: str, meta: dict | None = None) -> None:
+        np.savez_compressed(path, params=self.net.get_params(), M=self.M, base=self.base, n=self.n,
+                            meta=json.dumps(meta or {}))
+
+    @classmethod
+    def load(cls, path: str, max_batch: int = 32) -> "WarmStart":
+        z = np.load(path, allow_pickle=False)
+        ws = cls(int(z["M"]), int(z["base"]), max_batch, int(z["n"]))
+        ws.net.set_params(z["params"])
+        return ws
+
+
+def specs_for(seeds):
+    return [meshgen.config5_problem(int(s)) for s in seeds]
+
+
+def generate(ws: WarmStart, seeds, tol: float, max_iter: int, precision: int = capi.F64, chunk: int = 64,
+             steps_per_graph: int = 16, log=print):
+    """Solves every seed's problem with the batched WaveHoltz solver; returns
+    the input images, target images and per-problem solver statistics."""
+    torch = ws.torch
+    seeds = list(seeds)
+    N = len(seeds)
+    X = torch.empty((N, ws.M, ws.M, IN_LD), dtype=torch.bfloat16, device="cuda")
+    T = torch.empty((N, ws.M, ws.M, ws.out_ld), dtype=torch.float32, device="cuda")
+    its, res, secs = [], [], 0.0
+    for c0 in range(0, N, chunk):
+        specs = specs_for(seeds[c0:c0 + chunk])
+        b = capi.Batch([ws.problem(s) for s in specs], precision)
+        sec, _ = b.solve(tol, max_iter, steps_per_graph)
+        secs += sec
+        Ws = []
+        for k in range(len(specs)):
+            W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+            it, r = b.result(k, W)
+            its.append(it)
+            res.append(r)
+            Ws.append(W)
+        ws.features(specs, X[c0:c0 + len(specs)])
+        ws.targets(Ws, specs, T[c0:c0 + len(specs)])
+        del b, Ws
+        log(f"generated {min(c0 + chunk, N)}/{N} solves, {secs:.1f} s solver time, "
+            f"mean iterations {np.mean(its):.1f}")
+    return X, T, dict(seconds=secs, iterations=its, residuals=res)
+
+
+def train(ws: WarmStart, X, T, steps: int, batch: int, lr: float = 1e-3, lr_min_frac: float = 0.05, seed: int = 0,
+          log_every: int = 100, log=print):
+    """Adam with a cosine learning-rate schedule over shuffled epochs."""
+    torch = ws.torch
+    N = X.shape[0]
+    batch = min(batch, ws.max_batch, N)
+    g = torch.Generator().manual_seed(seed)
+    perm, pos = torch.randperm(N, generator=g), 0
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        if pos + batch > N:
+            perm, pos = torch.randperm(N, generator=g), 0
+        idx = perm[pos:pos + batch].cuda()
+        pos += batch
+        xb = X.index_select(0, idx).contiguous()
+        tb = T.index_select(0, idx).contiguous()
+        frac = step / max(steps - 1, 1)
+        lr_t = lr * (lr_min_frac + (1.0 - lr_min_frac) * 0.5 * (1.0 + math.cos(math.pi * frac)))
+        want = step % log_every == 0 or step == steps - 1
+        loss = ws.net.train_step(xb, tb, lr_t, want_loss=want)
+        if want:
+            losses.append((step, loss))
+            log(f"step {step} loss {loss:.4f} lr {lr_t:.2e} ({time.perf_counter() - t0:.1f} s)")
+    torch.cuda.synchronize()
+    return losses, time.perf_counter() - t0
+
+
+def validation_loss(ws: WarmStart, X, T) -> float:
+    tot, n = 0.0, X.shape[0]
+    for c0 in range(0, n, ws.max_batch):
+        xb, tb = X[c0:c0 + ws.max_batch].contiguous(), T[c0:c0 + ws.max_batch].contiguous()
+        loss, _ = ws.net.loss_grad(xb, tb, want_grad=False)
+        tot += loss * xb.shape[0]
+    return tot / n
+
+
+def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: int = 16, sequential: bool = False,
+             log=print) -> dict:
+    """Iterations and time to tol with and without the warm start (both
+    batched), plus optionally the plain per-problem GPU solves."""
+    torch = ws.torch
+    specs = specs_for(seeds)
+    probs = [ws.problem(s) for s in specs]
+    out = {"problems": len(specs), "M": ws.M, "P": ws.n - 1, "tol": tol}
+    if sequential:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        seq_it = []
+        for pb in probs:
+            w = capi.Wave2D(pb["mesh"], pb["n"], pb["bc"])
+            wh = capi.WaveHoltz(w, pb["omega"])
+            F = torch.empty(w.ndof, dtype=torch.float64, device="cuda")
+            w.load_vector_gauss(pb["sources"], F)
+            pi0 = torch.zeros_like(F)
+            wh.pi0(pi0, F)
+            W = torch.zeros_like(F)
+            it, _, _ = wh.solve(W, pi0, capi.norm(pi0), tol, max_iter)
+            seq_it.append(it)
+        torch.cuda.synchronize()
+        out["sequential_plain_seconds"] = time.perf_counter() - t0
+        out["sequential_plain_iterations"] = seq_it
+    b = capi.Batch(probs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sec_plain, _ = b.solve(tol, max_iter, steps_per_graph)
+    wall_plain = time.perf_counter() - t0
+    plain_W, plain_it, plain_res = [], [], []
+    for k in range(len(probs)):
+        W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+        it, r = b.result(k, W)
+        plain_W.append(W)
+        plain_it.append(it)
+        plain_res.append(r)
+    del b
+    b = capi.Batch(probs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Wp = ws.predict(specs)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t0
+    for k, W in enumerate(Wp):
+        b.set_initial(k, W)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sec_warm, _ = b.solve(tol, max_iter, steps_per_graph)
+    wall_warm = time.perf_counter() - t0
+    warm_it, warm_res, pred_err, final_diff = [], [], [], []
+    for k in range(len(probs)):
+        W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+        it, r = b.result(k, W)
+        warm_it.append(it)
+        warm_res.append(r)
+        pn = float(torch.linalg.norm(plain_W[k]))
+        pred_err.append(float(torch.linalg.norm(Wp[k] - plain_W[k])) / pn)
+        final_diff.append(float(torch.linalg.norm(W - plain_W[k])) / pn)
+    out.update({
+        "plain_iterations_mean": float(np.mean(plain_it)), "warm_iterations_mean": float(np.mean(warm_it)),
+        "plain_iterations": plain_it, "warm_iterations": warm_it,
+        "plain_batched_seconds": wall_plain, "warm_batched_seconds": wall_warm, "inference_seconds": t_inf,
+        "plain_batched_gpu_seconds": sec_plain, "warm_batched_gpu_seconds": sec_warm,
+        "iteration_gain": float(np.mean(plain_it) / np.mean(warm_it)),
+        "time_gain_batched": wall_plain / (wall_warm + t_inf),
+        "prediction_rel_error_median": float(np.median(pred_err)),
+        "prediction_rel_error_mean": float(np.mean(pred_err)),
+        "warm_vs_plain_solution_max_rel_diff": float(np.max(final_diff)),
+        "plain_max_residual": float(np.max(plain_res)), "warm_max_residual": float(np.max(warm_res)),
+    })
+    if sequential:
+        out["speedup_batched_warm_vs_sequential_plain"] = out["sequential_plain_seconds"] / (wall_warm + t_inf)
+        out["speedup_batched_plain_vs_sequential_plain"] = out["sequential_plain_seconds"] / wall_plain
+    log(json.dumps({k: v for k, v in out.items() if not isinstance(v, list)}))
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="learned WaveHoltz warm start: train or evaluate")
+    ap.add_argument("mode", choices=["train", "eval"])
+    ap.add_argument("--M", type=int, default=64)
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--n-train", type=int, default=1024)
+    ap.add_argument("--n-val", type=int, default=64)
+    ap.add_argument("--n-eval", type=int, default=128)
+    ap.add_argument("--gen-tol", type=float, default=1e-6)
+    ap.add_argument("--gen-precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iter", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--init-seed", type=int, default=0)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--sequential", action="store_true", help="also time plain per-problem solves")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if a.mode == "train":
+        ws = WarmStart(a.M, a.base, max_batch=max(a.batch, 16))
+        ws.net.set_params(init_params(a.base, seed=a.init_seed))
+        prec = capi.F64 if a.gen_precision == "f64" else capi.F32
+        X, T, st = generate(ws, range(TRAIN_SEED0, TRAIN_SEED0 + a.n_train + a.n_val), a.gen_tol, a.max_iter, prec,
+                            log=log)
+        Xt, Tt, Xv, Tv = X[:a.n_train], T[:a.n_train], X[a.n_train:], T[a.n_train:]
+        v0 = validation_loss(ws, Xv, Tv)
+        losses, t_train = train(ws, Xt, Tt, a.steps, a.batch, a.lr, log=log)
+        v1 = validation_loss(ws, Xv, Tv)
+        meta = dict(n_train=a.n_train, n_val=a.n_val, gen_tol=a.gen_tol, gen_precision=a.gen_precision,
+                    gen_seconds=st["seconds"], gen_iterations_mean=float(np.mean(st["iterations"])),
+                    steps=a.steps, batch=a.batch, lr=a.lr, train_seconds=t_train, val_loss_init=v0,
+                    val_loss_final=v1, train_loss_final=losses[-1][1])
+        log(json.dumps(meta))
+        if a.out:
+            ws.save(a.out, meta)
+        if a.n_eval > 0:
+            evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log)
+        return 0
+    if not a.weights:
+        ap.error("eval needs --weights")
+    ws = WarmStart.load(a.weights)
+    evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
