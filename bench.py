@@ -261,6 +261,13 @@ def main() -> int:
     ap.add_argument("--batch-nmin", type=int, default=32)
     ap.add_argument("--batch-nmax", type=int, default=256)
     ap.add_argument("--batch-graph", type=int, default=16)
+    ap.add_argument("--only-learned", action="store_true",
+                    help="WaveHoltz to --tol with and without the learned warm start (held-out seeds)")
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "wavedg_amd",
+                                                      "weights", "warmstart_m64_p4.npz"))
+    ap.add_argument("--learned-count", type=int, default=64)
+    ap.add_argument("--learned-sequential", type=int, default=1,
+                    help="also time the plain per-problem GPU solves (1) or not (0)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -281,6 +288,15 @@ def main() -> int:
                                   args.max_iter, args.batch_graph)
         print(f"[batched] {r}", file=sys.stderr, flush=True)
         print(json.dumps({"batched": r}))
+        return 0
+    if args.only_learned:
+        from wavedg_amd import warmstart
+
+        ws = warmstart.WarmStart.load(args.weights)
+        seeds = range(warmstart.EVAL_SEED0, warmstart.EVAL_SEED0 + args.learned_count)
+        r = warmstart.evaluate(ws, seeds, args.tol, args.max_iter, args.batch_graph,
+                               sequential=bool(args.learned_sequential), log=lambda m: print(m, file=sys.stderr))
+        print(json.dumps({"learned": r}))
         return 0
     if args.only_waveholtz:
         out = {}

@@ -270,13 +270,19 @@ int wdg_hip_cnn_loss_grad(wdg_cnn n, int batch, const uint16_t *d_x, const float
 int wdg_hip_cnn_train_step(wdg_cnn n, int batch, const uint16_t *d_x, const float *d_t, float lr, float *loss,
                            void *stream);
 int wdg_hip_f32_to_bf16(const float *d_src, uint16_t *d_dst, long long n, void *stream);
-/* Input image of one problem: channels 0..n^2-1 are gain * f(node i + n j)
- * with f the sum of Gaussian sources (device (4, n_sources): cx, cy, sigma,
- * amp), channel n^2 is omega_feature, n^2+1 and n^2+2 the element centre
- * mapped from box = {x0, y0, x1, y1} to [-1, 1], the rest 0.  d_nodes are
- * the GLL nodes as from wdg_hip_mesh_nodes, on the device. */
+/* Input image of one problem (nn = n^2 channels per block):
+ *   [0, nn)        gain * f(node i + n j), f the sum of the Gaussian sources
+ *                  (device (4, n_sources): cx, cy, sigma, amp);
+ *   [nn, 3nn)      only if green_gain != 0: green_gain * sum_k m_k
+ *                  exp(-omega^2 sigma_k^2 / 2) J0 (then Y0) of omega r_k,
+ *                  m_k = amp_k 2 pi sigma_k^2, r_k = sqrt(|x - c_k|^2 + sigma_k^2):
+ *                  the free-space Helmholtz response of the sources;
+ *   then omega_feature, and the element centre mapped from box = {x0, y0,
+ *   x1, y1} to [-1, 1] (two channels); the rest 0.
+ * d_nodes are the GLL nodes as from wdg_hip_mesh_nodes, on the device. */
 int wdg_hip_ws_features(const double *d_nodes, int n_colloc, int nel, int n_sources, const double *d_sources,
-                        double gain, double omega_feature, const double *box, int ld, uint16_t *d_x, void *stream);
+                        double gain, double omega, double omega_feature, double green_gain, const double *box, int ld,
+                        uint16_t *d_x, void *stream);
 /* d_t[e][c] = d_W[e][c] / scale for the 3 n^2 values of element e, padding 0. */
 int wdg_hip_ws_dg_to_image(const double *d_W, int n_colloc, int nel, double scale, int ld, float *d_t, void *stream);
 /* d_W[e][c] = scale * d_y[e][c]: a network output as a WaveHoltz iterate. */

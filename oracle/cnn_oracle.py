@@ -233,7 +233,30 @@ class Net:
         return self.loss(y, t), flat
 
 
-def features(nodes: np.ndarray, n: int, sources: np.ndarray, gain: float, omega_feature: float, box, ld: int):
+def bessel_j0_y0(x: np.ndarray):
+    """J0 and Y0 by Abramowitz & Stegun 9.4.1-9.4.3 (the approximation the
+    feature kernel uses); pinned against scipy.special in the tests."""
+    x = np.asarray(x, dtype=np.float64)
+    j = np.empty_like(x)
+    yv = np.empty_like(x)
+    s = x <= 3.0
+    t = (x[s] / 3.0) ** 2
+    j[s] = 1 + t * (-2.2499997 + t * (1.2656208 + t * (-0.3163866 + t * (0.0444479 + t * (-0.0039444 + t * 0.0002100)))))
+    yv[s] = (2 / np.pi) * np.log(0.5 * x[s]) * j[s] + 0.36746691 + t * (0.60559366 + t * (-0.74350384 + t * (
+        0.25300117 + t * (-0.04261214 + t * (0.00427916 - t * 0.00024846)))))
+    z = 3.0 / x[~s]
+    f0 = 0.79788456 + z * (-0.00000077 + z * (-0.00552740 + z * (-0.00009512 + z * (0.00137237 + z * (
+        -0.00072805 + z * 0.00014476)))))
+    t0 = x[~s] - 0.78539816 + z * (-0.04166397 + z * (-0.00003954 + z * (0.00262573 + z * (-0.00054125 + z * (
+        -0.00029333 + z * 0.00013558)))))
+    a = f0 / np.sqrt(x[~s])
+    j[~s] = a * np.cos(t0)
+    yv[~s] = a * np.sin(t0)
+    return j, yv
+
+
+def features(nodes: np.ndarray, n: int, sources: np.ndarray, gain: float, omega_feature: float, box, ld: int,
+             omega: float = 0.0, green_gain: float = 0.0):
     """Input image (nel, ld) float64: see wdg_hip_ws_features.  nodes is (nel, n, n, 2)
     in the (2, n, n, nel) column-major layout, i.e. nodes[e, j, i, :]."""
     nel = nodes.shape[0]
@@ -241,11 +264,24 @@ def features(nodes: np.ndarray, n: int, sources: np.ndarray, gain: float, omega_
     xy = nodes.reshape(nel, nn, 2)
     out = np.zeros((nel, ld))
     f = np.zeros((nel, nn))
+    gj = np.zeros((nel, nn))
+    gy = np.zeros((nel, nn))
     for cx, cy, s, amp in np.asarray(sources, dtype=np.float64).reshape(-1, 4):
-        f += amp * np.exp(-((xy[..., 0] - cx) ** 2 + (xy[..., 1] - cy) ** 2) / (2.0 * s * s))
+        r2 = (xy[..., 0] - cx) ** 2 + (xy[..., 1] - cy) ** 2
+        f += amp * np.exp(-r2 / (2.0 * s * s))
+        if green_gain != 0.0:
+            jv, yv = bessel_j0_y0(omega * np.sqrt(r2 + s * s))
+            m = amp * 2.0 * np.pi * s * s * np.exp(-0.5 * omega * omega * s * s)
+            gj += m * jv
+            gy += m * yv
     out[:, :nn] = gain * f
-    out[:, nn] = omega_feature
+    base = nn
+    if green_gain != 0.0:
+        out[:, nn:2 * nn] = green_gain * gj
+        out[:, 2 * nn:3 * nn] = green_gain * gy
+        base = 3 * nn
+    out[:, base] = omega_feature
     centre = xy.mean(axis=1)
-    out[:, nn + 1] = 2.0 * (centre[:, 0] - box[0]) / (box[2] - box[0]) - 1.0
-    out[:, nn + 2] = 2.0 * (centre[:, 1] - box[1]) / (box[3] - box[1]) - 1.0
+    out[:, base + 1] = 2.0 * (centre[:, 0] - box[0]) / (box[2] - box[0]) - 1.0
+    out[:, base + 2] = 2.0 * (centre[:, 1] - box[1]) / (box[3] - box[1]) - 1.0
     return out

@@ -112,7 +112,7 @@ def test_dg_image_maps(cuda):
     xd = torch.empty((M * M, ld), dtype=torch.bfloat16, device="cuda")
     nd = torch.tensor(nodes.reshape(-1), device="cuda")
     sd = torch.tensor(src.reshape(-1), device="cuda")
-    capi.ws_features(nd, n, M * M, sd, 2, 0.37, -0.25, (0.0, 0.0, 1.0, 1.0), ld, xd)
+    capi.ws_features(nd, n, M * M, sd, 2, 0.37, 0.0, -0.25, 0.0, (0.0, 0.0, 1.0, 1.0), ld, xd)
     torch.cuda.synchronize()
     got = xd.float().cpu().numpy().astype(np.float64)
     want = co.bf16(co.features(nodes, n, src, 0.37, -0.25, (0.0, 0.0, 1.0, 1.0), ld))
@@ -122,6 +122,14 @@ def test_dg_image_maps(cuda):
     ex, ey = np.arange(M * M) % M, np.arange(M * M) // M
     np.testing.assert_allclose(got[:, n * n + 1], co.bf16(2 * (ex + 0.5) / M - 1), atol=0)
     np.testing.assert_allclose(got[:, n * n + 2], co.bf16(2 * (ey + 0.5) / M - 1), atol=0)
+    # with the free-space Green's-function channels
+    ld2 = 80
+    xg = torch.empty((M * M, ld2), dtype=torch.bfloat16, device="cuda")
+    capi.ws_features(nd, n, M * M, sd, 2, 0.37, 40.0, 0.5, 3.0, (0.0, 0.0, 1.0, 1.0), ld2, xg)
+    torch.cuda.synchronize()
+    got = xg.float().cpu().numpy().astype(np.float64)
+    want = co.bf16(co.features(nodes, n, src, 0.37, 0.5, (0.0, 0.0, 1.0, 1.0), ld2, omega=40.0, green_gain=3.0))
+    assert np.all(np.abs(got - want) <= 2.0 ** -7 * np.abs(want) + 1e-6)
     # DG <-> image round trip
     nd3 = 3 * n * n
     W = torch.randn(M * M * nd3, dtype=torch.float64, device="cuda")

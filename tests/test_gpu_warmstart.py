@@ -1,0 +1,40 @@
+"""What the trained network gains: WaveHoltz to 1e-6 on held-out problems
+(64 x 64 quads, P = 4, seeds never used in training), batched, with and
+without the learned warm start.  The warm-started solve must converge to the
+same fixed point (the plain iteration judges the residual) in fewer
+updates."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from wavedg_amd import warmstart
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "wavedg_amd", "weights",
+                       "warmstart_m64_p4.npz")
+
+
+def test_warm_start_reduces_iterations(cuda):
+    torch = cuda
+    ws = warmstart.WarmStart.load(WEIGHTS)
+    specs = warmstart.specs_for(range(warmstart.EVAL_SEED0 + 500, warmstart.EVAL_SEED0 + 508))
+    tol = 1e-6
+    print("warm start: plain batched solve", flush=True)
+    Wp, itp, _, resp, _, tp = warmstart.solve_batched(ws, specs, tol, 300, warm=False)
+    print(f"plain: iterations {itp}, {tp:.1f} s", flush=True)
+    Ww, itw, _, resw, t_inf, tw = warmstart.solve_batched(ws, specs, tol, 300, warm=True)
+    print(f"warm: iterations {itw}, {tw:.1f} s (+{t_inf * 1e3:.1f} ms inference)", flush=True)
+    assert max(resp) <= tol and max(resw) <= tol
+    assert np.mean(itw) < 0.95 * np.mean(itp)
+    # the same fixed point, to the accuracy the residual tolerance allows
+    for a, b in zip(Ww, Wp):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 2e-5
+    # the prediction itself is within 20 % of the converged solution
+    pred = ws.predict(specs)
+    err = [float(torch.linalg.norm(p - b) / torch.linalg.norm(b)) for p, b in zip(pred, Wp)]
+    print(f"prediction relative errors {np.round(err, 3)}", flush=True)
+    assert np.median(err) < 0.2

@@ -152,6 +152,34 @@ def test_features_oracle_channels():
     assert np.all(f[:, n * n + 3:] == 0.0)
 
 
+def test_bessel_approximation_against_scipy():
+    sp = pytest.importorskip("scipy.special")
+    x = np.concatenate([np.linspace(1e-3, 3.0, 3001), np.linspace(3.0, 200.0, 20001)])
+    j, y = co.bessel_j0_y0(x)
+    assert np.max(np.abs(j - sp.j0(x))) < 1e-7
+    assert np.max(np.abs(y - sp.y0(x))) < 1e-7
+
+
+def test_features_green_channels():
+    n, M = 2, 2
+    nodes = np.zeros((M * M, n, n, 2))
+    for e in range(M * M):
+        ex, ey = e % M, e // M
+        for j in range(n):
+            for i in range(n):
+                nodes[e, j, i] = [(ex + i) / M, (ey + j) / M]
+    src = np.array([[0.5, 0.5, 0.02, 1.5]])
+    om = 30.0
+    f = co.features(nodes, n, src, 1.0, 0.1, (0, 0, 1, 1), 16, omega=om, green_gain=2.0)
+    sp = pytest.importorskip("scipy.special")
+    r = np.hypot(nodes[..., 0] - 0.5, nodes[..., 1] - 0.5).reshape(M * M, n * n)
+    m = 1.5 * 2 * np.pi * 0.02 ** 2 * np.exp(-0.5 * om * om * 0.02 ** 2)
+    np.testing.assert_allclose(f[:, 4:8], 2.0 * m * sp.j0(om * np.sqrt(r ** 2 + 0.02 ** 2)), atol=1e-7)
+    np.testing.assert_allclose(f[:, 8:12], 2.0 * m * sp.y0(om * np.sqrt(r ** 2 + 0.02 ** 2)), atol=1e-7)
+    assert np.all(f[:, 12] == 0.1)
+    assert np.all(f[:, 15:] == 0.0)
+
+
 def test_product_init_matches_oracle_layout():
     """The package's initialiser and the oracle agree on the parameter layout
     (same draws, same order)."""
@@ -159,6 +187,6 @@ def test_product_init_matches_oracle_layout():
 
     for base in (16, 32):
         a = warmstart.init_params(base, seed=3)
-        b = co.init_params(32, 75, base, seed=3)
+        b = co.init_params(80, 75, base, seed=3)
         assert a.shape == b.shape
         np.testing.assert_array_equal(a, b)

@@ -93,7 +93,7 @@ _SIGS = {
     "wdg_hip_cnn_loss_grad": [_vp, _i, _vp, _vp, _fp, _fp, _vp],
     "wdg_hip_cnn_train_step": [_vp, _i, _vp, _vp, ctypes.c_float, _fp, _vp],
     "wdg_hip_f32_to_bf16": [_vp, _vp, _ll, _vp],
-    "wdg_hip_ws_features": [_vp, _i, _i, _i, _vp, _d, _d, _dp, _i, _vp, _vp],
+    "wdg_hip_ws_features": [_vp, _i, _i, _i, _vp, _d, _d, _d, _d, _dp, _i, _vp, _vp],
     "wdg_hip_ws_dg_to_image": [_vp, _i, _i, _d, _i, _vp, _vp],
     "wdg_hip_ws_image_to_dg": [_vp, _i, _i, _i, _d, _vp, _vp],
 }
@@ -497,11 +497,11 @@ def f32_to_bf16(src, dst, stream=None):
     _call("wdg_hip_f32_to_bf16", dev_ptr(src), dev_ptr(dst), src.numel(), stream if stream is not None else cur_stream())
 
 
-def ws_features(d_nodes, n: int, nel: int, d_sources, n_sources: int, gain: float, omega_feature: float, box, ld: int,
-                d_x, stream=None):
+def ws_features(d_nodes, n: int, nel: int, d_sources, n_sources: int, gain: float, omega: float, omega_feature: float,
+                green_gain: float, box, ld: int, d_x, stream=None):
     b = np.ascontiguousarray(np.asarray(box, dtype=np.float64))
-    _call("wdg_hip_ws_features", dev_ptr(d_nodes), n, nel, n_sources, dev_ptr(d_sources), gain, omega_feature,
-          _dptr(b), ld, dev_ptr(d_x), stream if stream is not None else cur_stream())
+    _call("wdg_hip_ws_features", dev_ptr(d_nodes), n, nel, n_sources, dev_ptr(d_sources), gain, omega, omega_feature,
+          green_gain, _dptr(b), ld, dev_ptr(d_x), stream if stream is not None else cur_stream())
 
 
 def ws_dg_to_image(d_W, n: int, nel: int, scale: float, ld: int, d_t, stream=None):

@@ -192,34 +192,51 @@ __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
     const int tap = tile / (nco * nci), rem = tile % (nco * nci);
     const int cob = rem / nci, cib = rem % nci;
     const int dy = tap / a.ks - (a.ks >> 1), dx = tap % a.ks - (a.ks >> 1);
-    const long long npix = (long long)a.B * a.H * a.W;
-    const long long pbeg = (long long)blockIdx.x * kChunk;
-    const long long pend = min(npix, pbeg + kChunk);
+    const int npix = a.B * a.H * a.W;
+    const int pbeg = blockIdx.x * kChunk;
+    const int pend = min(npix, pbeg + kChunk);
     const int co = cob * 16 + ln, ci = cib * 16 + ln;
+    // this lane's pixels are q = p + 4 kq + r, r = 0..3; their (b, y, x)
+    // advance by 16 pixels per step
+    int qb[4], qy[4], qx[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int q = pbeg + 4 * kq + r;
+        qx[r] = q % a.W;
+        const int t = q / a.W;
+        qy[r] = t % a.H;
+        qb[r] = t / a.H;
+    }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long long p = pbeg; p < pend; p += 16) {
+    for (int p = pbeg; p < pend; p += 16) {
         bf16x4 av, bv;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const long long q = p + 4 * kq + r;
+            const int q = p + 4 * kq + r;
             const bool okq = q < pend;
-            const long long qc = okq ? q : pbeg;
-            const int x = (int)(qc % a.W);
-            const long long t = qc / a.W;
-            const int y = (int)(t % a.H), b = (int)(t / a.H);
-            const uint16_t dv = a.dout[qc * a.dout_ld + co];
+            const int qc = okq ? q : pbeg;
+            const uint16_t dv = a.dout[(size_t)qc * a.dout_ld + co];
             av[r] = (short)(okq ? dv : 0);
-            const int ny = y + dy, nx = x + dx;
+            const int ny = qy[r] + dy, nx = qx[r] + dx;
             const bool ok = okq && ny >= 0 && ny < a.H && nx >= 0 && nx < a.W;
-            const long long src = ok ? ((long long)b * a.H + ny) * a.W + nx : 0;
-            const uint16_t iv = a.in[src * a.in_ld + a.in_off + ci];
+            const int src = ok ? (qb[r] * a.H + ny) * a.W + nx : 0;
+            const uint16_t iv = a.in[(size_t)src * a.in_ld + a.in_off + ci];
             bv[r] = (short)(ok ? iv : 0);
+            // advance to pixel q + 16
+            qx[r] += 16;
+            while (qx[r] >= a.W) {
+                qx[r] -= a.W;
+                if (++qy[r] == a.H) {
+                    qy[r] = 0;
+                    ++qb[r];
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
-    float* dst = a.partial + ((long long)blockIdx.x * ntap + tap) * a.cout * a.cin;
+    float* dst = a.partial + ((size_t)blockIdx.x * ntap + tap) * a.cout * a.cin;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) dst[(long long)(cob * 16 + 4 * kq + r) * a.cin + cib * 16 + ln] = acc[r];
+    for (int r = 0; r < 4; ++r) dst[(size_t)(cob * 16 + 4 * kq + r) * a.cin + cib * 16 + ln] = acc[r];
 }
 
 // out[i] = sum_{c < chunks} partial[c n + i], in chunk order.
@@ -233,13 +250,13 @@ __global__ void reduce_chunks_kernel(const float* partial, long long n, int chun
 }
 
 // partial[chunk][c] = sum over the chunk's pixels of d[p, c]
-__global__ void bias_partial_kernel(const uint16_t* d, int C, long long npix, float* partial)
+__global__ void bias_partial_kernel(const uint16_t* d, int C, int npix, float* partial)
 {
     const int c = blockIdx.y * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    const long long pbeg = (long long)blockIdx.x * kChunk, pend = min(npix, pbeg + kChunk);
+    const int pbeg = blockIdx.x * kChunk, pend = min(npix, pbeg + kChunk);
     float s = 0.f;
-    for (long long p = pbeg; p < pend; ++p) s += bf2f(d[p * C + c]);
+    for (int p = pbeg; p < pend; ++p) s += bf2f(d[(size_t)p * C + c]);
     partial[(long long)blockIdx.x * C + c] = s;
 }
 
@@ -247,16 +264,13 @@ __global__ void bias_partial_kernel(const uint16_t* d, int C, long long npix, fl
 // 2 x 2 average pool; out has H x W pixels, in 2H x 2W.
 __global__ void pool_kernel(const uint16_t* in, int in_ld, int in_off, uint16_t* out, int B, int H, int W, int C)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)B * H * W * C;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const long long p = i / C;
-    const int x = (int)(p % W);
-    const long long t = p / W;
-    const int y = (int)(t % H), b = (int)(t / H);
-    const long long q = ((long long)b * 2 * H + 2 * y) * (2 * W) + 2 * x;
-    const long long s0 = q * in_ld + in_off + c, s1 = (q + 2 * W) * in_ld + in_off + c;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * H * W * C) return;
+    const int c = i % C, p = i / C;
+    const int x = p % W, t = p / W;
+    const int y = t % H, b = t / H;
+    const size_t q = ((size_t)b * 2 * H + 2 * y) * (2 * W) + 2 * x;
+    const size_t s0 = q * in_ld + in_off + c, s1 = (q + 2 * W) * in_ld + in_off + c;
     const float v = (bf2f(in[s0]) + bf2f(in[s0 + in_ld])) + (bf2f(in[s1]) + bf2f(in[s1 + in_ld]));
     out[i] = f2bf(0.25f * v);
 }
@@ -264,16 +278,13 @@ __global__ void pool_kernel(const uint16_t* in, int in_ld, int in_off, uint16_t*
 // nearest upsample; in has H/2 x W/2 pixels (contiguous, C channels)
 __global__ void upsample_kernel(const uint16_t* in, uint16_t* out, int out_ld, int out_off, int B, int H, int W, int C)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)B * H * W * C;
-    if (i >= n) return;
-    const int c = (int)(i % C);
-    const long long p = i / C;
-    const int x = (int)(p % W);
-    const long long t = p / W;
-    const int y = (int)(t % H), b = (int)(t / H);
-    const long long q = ((long long)b * (H / 2) + y / 2) * (W / 2) + x / 2;
-    out[p * out_ld + out_off + c] = in[q * C + c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * H * W * C) return;
+    const int c = i % C, p = i / C;
+    const int x = p % W, t = p / W;
+    const int y = t % H, b = t / H;
+    const size_t q = ((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2;
+    out[(size_t)p * out_ld + out_off + c] = in[q * C + c];
 }
 
 // ReLU backward with the incoming gradient assembled from its sources:
@@ -292,27 +303,24 @@ struct PrepArgs {
 
 __global__ void relu_grad_kernel(PrepArgs a)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)a.B * a.H * a.W * a.C;
-    if (i >= n) return;
-    const int c = (int)(i % a.C);
-    const long long p = i / a.C;
-    const int x = (int)(p % a.W);
-    const long long t = p / a.W;
-    const int y = (int)(t % a.H), b = (int)(t / a.H);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.B * a.H * a.W * a.C) return;
+    const int c = i % a.C, p = i / a.C;
+    const int x = p % a.W, t = p / a.W;
+    const int y = t % a.H, b = t / a.H;
     float g;
     if (a.up) {
-        const long long q = ((long long)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x;
-        const long long s0 = q * a.g1_ld + a.g1_off + c, s1 = (q + 2 * a.W) * a.g1_ld + a.g1_off + c;
+        const size_t q = ((size_t)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x;
+        const size_t s0 = q * a.g1_ld + a.g1_off + c, s1 = (q + 2 * a.W) * a.g1_ld + a.g1_off + c;
         g = (a.g1[s0] + a.g1[s0 + a.g1_ld]) + (a.g1[s1] + a.g1[s1 + a.g1_ld]);
     } else {
-        g = a.g1[p * a.g1_ld + a.g1_off + c];
+        g = a.g1[(size_t)p * a.g1_ld + a.g1_off + c];
     }
     if (a.g2) {
-        const long long q = ((long long)b * (a.H / 2) + y / 2) * (a.W / 2) + x / 2;
+        const size_t q = ((size_t)b * (a.H / 2) + y / 2) * (a.W / 2) + x / 2;
         g += 0.25f * a.g2[q * a.g2_ld + c];
     }
-    const bool on = bf2f(a.act[p * a.act_ld + a.act_off + c]) > 0.f;
+    const bool on = bf2f(a.act[(size_t)p * a.act_ld + a.act_off + c]) > 0.f;
     a.out[i] = on ? f2bf(g) : (uint16_t)0;
 }
 
@@ -323,10 +331,11 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* y, const float* 
 {
     __shared__ double red[2][256];
     const int b = blockIdx.x;
-    const long long n = npix * ld, base = (long long)b * n;
+    const int n = (int)(npix * ld);
+    const size_t base = (size_t)b * n;
     double tt = 0.0, ee = 0.0;
-    for (long long i = threadIdx.x; i < n; i += 256) {
-        if ((int)(i % ld) >= creal) continue;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        if (i % ld >= creal) continue;
         const double tv = t[base + i], d = (double)y[base + i] - tv;
         tt += tv * tv;
         ee += d * d;
@@ -344,8 +353,8 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* y, const float* 
     const double tn = red[0][0] > 1e-300 ? red[0][0] : 1e-300;
     if (threadIdx.x == 0) loss[b] = (float)(red[1][0] / tn);
     const float s = (float)(2.0 / (B * tn));
-    for (long long i = threadIdx.x; i < n; i += 256) {
-        const bool real = (int)(i % ld) < creal;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const bool real = i % ld < creal;
         dy[base + i] = real ? f2bf(s * (y[base + i] - t[base + i])) : (uint16_t)0;
     }
 }
@@ -386,32 +395,71 @@ __global__ void wflip_kernel(const float* w, int cout, int cin, int ntap, uint16
 }
 
 // ------------------------------------------------------- DG <-> image maps
-// Input channels of element e: 0..n^2-1 forcing at node i + n j times gain;
-// n^2: omega feature; n^2+1, n^2+2: element centre mapped to [-1, 1]
-// (x0, y0, x1, y1 = bounding box); the rest 0.
-__global__ void features_kernel(const double* nodes, int n, int nel, const double* src, int ns, double gain,
-                                float omega_feat, double x0, double y0, double x1, double y1, uint16_t* x, int ld)
+// Bessel functions J0 and Y0 of the first and second kind, order 0, from the
+// polynomial approximations of Abramowitz & Stegun 9.4.1-9.4.3 (absolute
+// error below 1e-7; ample for input features).
+__device__ __forceinline__ void bessel_j0_y0(double x, double& j0, double& y0)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)nel * ld) return;
-    const int c = (int)(i % ld);
-    const long long e = i / ld;
+    if (x <= 3.0) {
+        const double y = (x / 3.0) * (x / 3.0);
+        j0 = 1.0 + y * (-2.2499997 + y * (1.2656208 + y * (-0.3163866 + y * (0.0444479 + y * (-0.0039444 + y * 0.0002100)))));
+        y0 = 0.63661977236758134 * log(0.5 * x) * j0 + 0.36746691 +
+             y * (0.60559366 + y * (-0.74350384 + y * (0.25300117 + y * (-0.04261214 + y * (0.00427916 - y * 0.00024846)))));
+    } else {
+        const double z = 3.0 / x;
+        const double f0 = 0.79788456 + z * (-0.00000077 + z * (-0.00552740 + z * (-0.00009512 + z * (0.00137237 +
+                          z * (-0.00072805 + z * 0.00014476)))));
+        const double t0 = x - 0.78539816 + z * (-0.04166397 + z * (-0.00003954 + z * (0.00262573 + z * (-0.00054125 +
+                          z * (-0.00029333 + z * 0.00013558)))));
+        const double a = f0 / sqrt(x);
+        j0 = a * cos(t0);
+        y0 = a * sin(t0);
+    }
+}
+
+// Input channels of element e (nn = n^2, G = green_gain != 0):
+//   [0, nn)          gain * f(node i + n j), f = sum of the Gaussian sources
+//   [nn, 2nn)   (G)  green_gain * sum_k m_k exp(-omega^2 sigma_k^2 / 2) J0(omega r_k)
+//   [2nn, 3nn)  (G)  the same with Y0: the free-space Helmholtz response of
+//                    point sources of mass m_k = amp_k 2 pi sigma_k^2, damped as
+//                    the Gaussian's Fourier transform at wavenumber omega, with
+//                    r_k = sqrt(|x - c_k|^2 + sigma_k^2) to stay finite
+//   base = nn (or 3nn with G): omega_feature; base+1, base+2: element centre
+//   mapped from the box to [-1, 1]; the rest 0.
+__global__ void features_kernel(const double* nodes, int n, int nel, const double* src, int ns, double gain,
+                                double omega, float omega_feat, double green_gain, double x0, double y0, double x1,
+                                double y1, uint16_t* x, int ld)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nel * ld) return;
+    const int c = i % ld, e = i / ld;
     const int nn = n * n;
+    const bool green = green_gain != 0.0;
+    const int base = green ? 3 * nn : nn;
     float v = 0.f;
-    if (c < nn) {
-        const double px = nodes[2 * (c + nn * e)], py = nodes[2 * (c + nn * e) + 1];
+    if (c < base) {
+        const int k0 = c % nn, part = c / nn;
+        const double px = nodes[2 * ((size_t)k0 + (size_t)nn * e)], py = nodes[2 * ((size_t)k0 + (size_t)nn * e) + 1];
         double f = 0.0;
         for (int k = 0; k < ns; ++k) {
-            const double dx = px - src[4 * k], dy = py - src[4 * k + 1], s = src[4 * k + 2];
-            f += src[4 * k + 3] * exp(-(dx * dx + dy * dy) / (2.0 * s * s));
+            const double dx = px - src[4 * k], dy = py - src[4 * k + 1], sg = src[4 * k + 2], amp = src[4 * k + 3];
+            const double r2 = dx * dx + dy * dy;
+            if (part == 0) {
+                f += amp * exp(-r2 / (2.0 * sg * sg));
+            } else {
+                double jv, yv;
+                bessel_j0_y0(omega * sqrt(r2 + sg * sg), jv, yv);
+                const double m = amp * 6.283185307179586 * sg * sg * exp(-0.5 * omega * omega * sg * sg);
+                f += m * (part == 1 ? jv : yv);
+            }
         }
-        v = (float)(gain * f);
-    } else if (c == nn) {
+        v = (float)((part == 0 ? gain : green_gain) * f);
+    } else if (c == base) {
         v = omega_feat;
-    } else if (c == nn + 1 || c == nn + 2) {
-        const int d = c - nn - 1;
+    } else if (c == base + 1 || c == base + 2) {
+        const int d = c - base - 1;
         double m = 0.0;
-        for (int k = 0; k < nn; ++k) m += nodes[2 * (k + nn * e) + d];
+        for (int k = 0; k < nn; ++k) m += nodes[2 * ((size_t)k + (size_t)nn * e) + d];
         m /= nn;
         const double lo = d ? y0 : x0, hi = d ? y1 : x1;
         v = (float)(2.0 * (m - lo) / (hi - lo) - 1.0);
@@ -575,7 +623,7 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
     hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, chunks, n.g + L.w);
-    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks, (L.cout + 63) / 64), dim3(64), 0, s, dpre, L.cout, npix,
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks, (L.cout + 63) / 64), dim3(64), 0, s, dpre, L.cout, (int)npix,
                        n.partial);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(256), 0, s, n.partial, (long long)L.cout,
                        chunks, n.g + L.b);
@@ -729,6 +777,12 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
         if (out_channels < 1) throw Fail{WDG_ERR_INVALID, "output channels must be positive"};
         if (base_width < 16 || base_width % 16 != 0) throw Fail{WDG_ERR_INVALID, "base width must be a multiple of 16"};
         if (max_batch < 1) throw Fail{WDG_ERR_INVALID, "max_batch must be positive"};
+        {
+            // 32-bit element indices inside the kernels
+            const long long widest = std::max<long long>({3LL * base_width, (long long)in_ld, out_channels + 15LL});
+            if ((long long)max_batch * M * M * widest >= (1LL << 31))
+                throw Fail{WDG_ERR_INVALID, "max_batch * M^2 * channels must stay below 2^31"};
+        }
         auto n = std::make_unique<wdg_cnn_s>();
         ck(hipGetDevice(&n->dev), "hipGetDevice");
         n->M = M;
@@ -919,16 +973,19 @@ int wdg_hip_f32_to_bf16(const float* d_src, uint16_t* d_dst, long long n, void* 
 }
 
 int wdg_hip_ws_features(const double* d_nodes, int n_colloc, int nel, int n_sources, const double* d_sources,
-                        double gain, double omega_feature, const double* box, int ld, uint16_t* d_x, void* stream)
+                        double gain, double omega, double omega_feature, double green_gain, const double* box, int ld,
+                        uint16_t* d_x, void* stream)
 {
     return guard([&] {
         if (!d_nodes || !d_x || !box || (n_sources > 0 && !d_sources)) throw Fail{WDG_ERR_INVALID, "null argument"};
-        if (n_colloc < 2 || nel < 1 || n_sources < 0 || ld < n_colloc * n_colloc + 3)
-            throw Fail{WDG_ERR_INVALID, "bad sizes (ld must hold n^2 + 3 channels)"};
+        const int nn = n_colloc * n_colloc, need = (green_gain != 0.0 ? 3 * nn : nn) + 3;
+        if (n_colloc < 2 || nel < 1 || n_sources < 0 || ld < need)
+            throw Fail{WDG_ERR_INVALID, "bad sizes (ld must hold every feature channel)"};
+        if ((long long)nel * ld >= (1LL << 31)) throw Fail{WDG_ERR_INVALID, "image too large"};
         if (!(box[2] > box[0]) || !(box[3] > box[1])) throw Fail{WDG_ERR_INVALID, "empty bounding box"};
         hipLaunchKernelGGL(features_kernel, dim3(blocks_for((long long)nel * ld)), dim3(256), 0,
-                           static_cast<hipStream_t>(stream), d_nodes, n_colloc, nel, d_sources, n_sources, gain,
-                           (float)omega_feature, box[0], box[1], box[2], box[3], d_x, ld);
+                           static_cast<hipStream_t>(stream), d_nodes, n_colloc, nel, d_sources, n_sources, gain, omega,
+                           (float)omega_feature, green_gain, box[0], box[1], box[2], box[3], d_x, ld);
         ck(hipGetLastError(), "features");
     });
 }

@@ -4,11 +4,13 @@ Problems live on the unit square meshed by a uniform M x M quad grid, with
 n = 5 Gauss-Lobatto points per direction (P = 4), characteristic outflow on
 every wall, and omega and 1-4 Gaussian sources drawn by
 ``meshgen.config5_problem(seed)``.  The network (``capi.CNN``, kernels in
-csrc/cnn.hip) reads one image per problem -- pixel e is element e, channels
-0..24 the forcing at the element's nodes scaled by h^2 / s, channel 25 omega
-mapped to [-1, 1], channels 26-27 the element centre -- and predicts W / s
-with W the WaveHoltz solution in the DG layout (75 values per element) and s
-the total source mass sum_k |amp_k| 2 pi sigma_k^2.  Prediction times s is
+csrc/cnn.hip) reads one image per problem -- pixel e is element e; channels
+0..24 the forcing at the element's nodes scaled by h^2 / s, 25..74 the
+free-space Helmholtz response of the sources at the nodes (J0 and Y0 parts,
+scaled by 1 / s), 75 omega mapped to [-1, 1], 76-77 the element centre --
+and predicts g W / s, with W the WaveHoltz solution in the DG layout (75
+values per element), s the total source mass sum_k |amp_k| 2 pi sigma_k^2 and
+g a fixed gain set from the training targets (unit RMS).  Prediction times s is
 handed to the batched solver as the first iterate (wdg_hip_batch_set_initial);
 the WaveHoltz iteration then runs to the requested residual as without the
 warm start, so the network never decides the answer.
@@ -36,7 +38,7 @@ import numpy as np
 from . import capi, meshgen
 
 N_COLLOC = 5
-IN_LD = 32
+IN_LD = 80  # 3 x 25 node channels + omega + 2 coordinates, padded to 80
 OMEGA_LO, OMEGA_HI = 5.0 * math.pi, 20.0 * math.pi
 TRAIN_SEED0 = 50000  # training seeds TRAIN_SEED0 + k
 EVAL_SEED0 = 90000  # held-out seeds EVAL_SEED0 + k
@@ -73,11 +75,14 @@ def omega_feature(omega: float) -> float:
 class WarmStart:
     """Network plus the DG <-> image maps for one grid size M."""
 
-    def __init__(self, M: int, base: int = 32, max_batch: int = 32, n: int = N_COLLOC):
+    def __init__(self, M: int, base: int = 32, max_batch: int = 32, n: int = N_COLLOC, tgt_gain: float = 1.0,
+                 green: bool = True):
         import torch
 
         self.torch = torch
         self.M, self.n, self.base, self.max_batch = M, n, base, max_batch
+        self.tgt_gain = tgt_gain
+        self.green = green  # False: no free-space response channels (they read 0)
         self.nel = M * M
         self.ndof = 3 * n * n * self.nel
         self.mesh = capi.Mesh.uniform_rect(M, 0.0, 1.0, M, 0.0, 1.0)
@@ -96,9 +101,10 @@ class WarmStart:
         for b, spec in enumerate(specs):
             src = np.asarray(spec["sources"], dtype=np.float64).reshape(-1, 4)
             sd = torch.tensor(src.reshape(-1), device="cuda")
-            gain = (1.0 / self.M) ** 2 / source_scale(src)
-            capi.ws_features(self.nodes, self.n, self.nel, sd, src.shape[0], gain, omega_feature(spec["omega"]),
-                             (0.0, 0.0, 1.0, 1.0), IN_LD, out[b])
+            s = source_scale(src)
+            capi.ws_features(self.nodes, self.n, self.nel, sd, src.shape[0], (1.0 / self.M) ** 2 / s, spec["omega"],
+                             omega_feature(spec["omega"]), 1.0 / s if self.green else 0.0, (0.0, 0.0, 1.0, 1.0),
+                             IN_LD, out[b])
         return out
 
     def targets(self, Ws, specs, out=None):
@@ -106,7 +112,8 @@ class WarmStart:
         if out is None:
             out = torch.empty((len(specs), self.M, self.M, self.out_ld), dtype=torch.float32, device="cuda")
         for b, (W, spec) in enumerate(zip(Ws, specs)):
-            capi.ws_dg_to_image(W, self.n, self.nel, source_scale(spec["sources"]), self.out_ld, out[b])
+            capi.ws_dg_to_image(W, self.n, self.nel, source_scale(spec["sources"]) / self.tgt_gain, self.out_ld,
+                                out[b])
         return out
 
     def predict(self, specs):
@@ -120,18 +127,19 @@ class WarmStart:
             self.net.forward(x, y)
             for b, spec in enumerate(chunk):
                 W = torch.empty(self.ndof, dtype=torch.float64, device="cuda")
-                capi.ws_image_to_dg(y[b], self.out_ld, self.n, self.nel, source_scale(spec["sources"]), W)
+                capi.ws_image_to_dg(y[b], self.out_ld, self.n, self.nel, source_scale(spec["sources"]) / self.tgt_gain,
+                                    W)
                 out.append(W)
         return out
 
     def save(self, path: str, meta: dict | None = None) -> None:
         np.savez_compressed(path, params=self.net.get_params(), M=self.M, base=self.base, n=self.n,
-                            meta=json.dumps(meta or {}))
+                            tgt_gain=self.tgt_gain, green=int(self.green), meta=json.dumps(meta or {}))
 
     @classmethod
     def load(cls, path: str, max_batch: int = 32) -> "WarmStart":
         z = np.load(path, allow_pickle=False)
-        ws = cls(int(z["M"]), int(z["base"]), max_batch, int(z["n"]))
+        ws = cls(int(z["M"]), int(z["base"]), max_batch, int(z["n"]), float(z["tgt_gain"]), bool(int(z["green"])))
         ws.net.set_params(z["params"])
         return ws
 
@@ -168,6 +176,16 @@ def generate(ws: WarmStart, seeds, tol: float, max_iter: int, precision: int = c
         log(f"generated {min(c0 + chunk, N)}/{N} solves, {secs:.1f} s solver time, "
             f"mean iterations {np.mean(its):.1f}")
     return X, T, dict(seconds=secs, iterations=its, residuals=res)
+
+
+def set_target_gain(ws: WarmStart, T, n_real: int) -> float:
+    """Rescales the target images to unit RMS over their real channels and
+    records the factor in ws (it scales predictions back)."""
+    rms = float(ws.torch.sqrt(ws.torch.mean(T[..., :n_real].double() ** 2)))
+    g = 1.0 / rms
+    T.mul_(g)
+    ws.tgt_gain *= g
+    return g
 
 
 def train(ws: WarmStart, X, T, steps: int, batch: int, lr: float = 1e-3, lr_min_frac: float = 0.05, seed: int = 0,
@@ -207,10 +225,56 @@ def validation_loss(ws: WarmStart, X, T) -> float:
     return tot / n
 
 
+def solve_batched(ws: WarmStart, specs, tol: float, max_iter: int, warm: bool = True, mixed: bool = False,
+                  tol_switch: float = 1e-5, steps_per_graph: int = 16):
+    """Batched WaveHoltz on the problems `specs`, optionally warm-started by the
+    network and optionally in two precisions: an FP32-state batch to
+    tol_switch, then an FP64 batch started from its iterates to tol.  The
+    final FP64 iteration decides the residual either way.  Returns the
+    solutions, per-problem update counts (FP32 phase, FP64 phase), the last
+    residuals and the wall times (inference, solves)."""
+    torch = ws.torch
+    probs = [ws.problem(s) for s in specs]
+    first = capi.Batch(probs, capi.F32 if mixed else capi.F64)
+    final = capi.Batch(probs, capi.F64) if mixed else first
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    init = ws.predict(specs) if warm else None
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t0
+    if init is not None:
+        for k, W in enumerate(init):
+            first.set_initial(k, W)
+    t0 = time.perf_counter()
+    first.solve(tol_switch if mixed else tol, max_iter, steps_per_graph)
+    it1 = []
+    Ws = []
+    for k in range(len(probs)):
+        W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+        it, _ = first.result(k, W)
+        it1.append(it)
+        Ws.append(W)
+    it2, res = [0] * len(probs), []
+    if mixed:
+        for k, W in enumerate(Ws):
+            final.set_initial(k, W)
+        final.solve(tol, max_iter, steps_per_graph)
+        it2 = []
+        for k in range(len(probs)):
+            it, r = final.result(k, Ws[k])
+            it2.append(it)
+            res.append(r)
+    else:
+        res = [first.result(k)[1] for k in range(len(probs))]
+    torch.cuda.synchronize()
+    return Ws, it1, it2, res, t_inf, time.perf_counter() - t0
+
+
 def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: int = 16, sequential: bool = False,
-             log=print) -> dict:
+             log=print, plain_cache: dict | None = None) -> dict:
     """Iterations and time to tol with and without the warm start (both
-    batched), plus optionally the plain per-problem GPU solves."""
+    batched), plus optionally the plain per-problem GPU solves.  plain_cache
+    (a dict) keeps the plain batched results between calls on the same seeds."""
     torch = ws.torch
     specs = specs_for(seeds)
     probs = [ws.problem(s) for s in specs]
@@ -232,19 +296,26 @@ def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: i
         torch.cuda.synchronize()
         out["sequential_plain_seconds"] = time.perf_counter() - t0
         out["sequential_plain_iterations"] = seq_it
-    b = capi.Batch(probs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sec_plain, _ = b.solve(tol, max_iter, steps_per_graph)
-    wall_plain = time.perf_counter() - t0
-    plain_W, plain_it, plain_res = [], [], []
-    for k in range(len(probs)):
-        W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
-        it, r = b.result(k, W)
-        plain_W.append(W)
-        plain_it.append(it)
-        plain_res.append(r)
-    del b
+    if plain_cache is not None and plain_cache.get("key") == (tuple(seeds), tol, max_iter):
+        plain_W, plain_it, plain_res = plain_cache["W"], plain_cache["it"], plain_cache["res"]
+        sec_plain, wall_plain = plain_cache["sec"], plain_cache["wall"]
+    else:
+        b = capi.Batch(probs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sec_plain, _ = b.solve(tol, max_iter, steps_per_graph)
+        wall_plain = time.perf_counter() - t0
+        plain_W, plain_it, plain_res = [], [], []
+        for k in range(len(probs)):
+            W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+            it, r = b.result(k, W)
+            plain_W.append(W)
+            plain_it.append(it)
+            plain_res.append(r)
+        del b
+        if plain_cache is not None:
+            plain_cache.update(key=(tuple(seeds), tol, max_iter), W=plain_W, it=plain_it, res=plain_res,
+                               sec=sec_plain, wall=wall_plain)
     b = capi.Batch(probs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -278,9 +349,19 @@ def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: i
         "warm_vs_plain_solution_max_rel_diff": float(np.max(final_diff)),
         "plain_max_residual": float(np.max(plain_res)), "warm_max_residual": float(np.max(warm_res)),
     })
+    Wm, i32, i64, rm, t_inf_m, wall_m = solve_batched(ws, specs, tol, max_iter, warm=True, mixed=True,
+                                                      steps_per_graph=steps_per_graph)
+    out.update({
+        "warm_mixed_iterations_f32_mean": float(np.mean(i32)), "warm_mixed_iterations_f64_mean": float(np.mean(i64)),
+        "warm_mixed_batched_seconds": wall_m + t_inf_m, "warm_mixed_max_residual": float(np.max(rm)),
+        "warm_mixed_vs_plain_solution_max_rel_diff": float(max(
+            float(torch.linalg.norm(Wm[k] - plain_W[k]) / torch.linalg.norm(plain_W[k])) for k in range(len(specs)))),
+    })
+    out["time_gain_batched_mixed"] = wall_plain / (wall_m + t_inf_m)
     if sequential:
         out["speedup_batched_warm_vs_sequential_plain"] = out["sequential_plain_seconds"] / (wall_warm + t_inf)
         out["speedup_batched_plain_vs_sequential_plain"] = out["sequential_plain_seconds"] / wall_plain
+        out["speedup_batched_warm_mixed_vs_sequential_plain"] = out["sequential_plain_seconds"] / (wall_m + t_inf_m)
     log(json.dumps({k: v for k, v in out.items() if not isinstance(v, list)}))
     return out
 
@@ -293,8 +374,9 @@ def main(argv=None) -> int:
     ap.add_argument("--n-train", type=int, default=1024)
     ap.add_argument("--n-val", type=int, default=64)
     ap.add_argument("--n-eval", type=int, default=128)
-    ap.add_argument("--gen-tol", type=float, default=1e-6)
-    ap.add_argument("--gen-precision", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--gen-tol", type=float, default=1e-5)
+    ap.add_argument("--gen-precision", choices=["f64", "f32"], default="f32")
+    ap.add_argument("--gen-max-iter", type=int, default=150)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=400)
     ap.add_argument("--steps", type=int, default=3000)
@@ -313,13 +395,16 @@ def main(argv=None) -> int:
         ws = WarmStart(a.M, a.base, max_batch=max(a.batch, 16))
         ws.net.set_params(init_params(a.base, seed=a.init_seed))
         prec = capi.F64 if a.gen_precision == "f64" else capi.F32
-        X, T, st = generate(ws, range(TRAIN_SEED0, TRAIN_SEED0 + a.n_train + a.n_val), a.gen_tol, a.max_iter, prec,
-                            log=log)
+        X, T, st = generate(ws, range(TRAIN_SEED0, TRAIN_SEED0 + a.n_train + a.n_val), a.gen_tol, a.gen_max_iter,
+                            prec, log=log)
+        set_target_gain(ws, T[:a.n_train], 3 * N_COLLOC ** 2)
+        T[a.n_train:].mul_(ws.tgt_gain)
         Xt, Tt, Xv, Tv = X[:a.n_train], T[:a.n_train], X[a.n_train:], T[a.n_train:]
         v0 = validation_loss(ws, Xv, Tv)
         losses, t_train = train(ws, Xt, Tt, a.steps, a.batch, a.lr, log=log)
         v1 = validation_loss(ws, Xv, Tv)
-        meta = dict(n_train=a.n_train, n_val=a.n_val, gen_tol=a.gen_tol, gen_precision=a.gen_precision,
+        meta = dict(M=a.M, base=a.base, n_train=a.n_train, n_val=a.n_val, gen_tol=a.gen_tol,
+                    gen_precision=a.gen_precision, tgt_gain=ws.tgt_gain,
                     gen_seconds=st["seconds"], gen_iterations_mean=float(np.mean(st["iterations"])),
                     steps=a.steps, batch=a.batch, lr=a.lr, train_seconds=t_train, val_loss_init=v0,
                     val_loss_final=v1, train_loss_final=losses[-1][1])
