@@ -66,8 +66,12 @@ def build_workload(nx: int):
     return c, mesh, n, bc, u0, dt
 
 
-def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0):
+def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0, autotune: bool = False):
     wave = capi.Wave2D(mesh, n, bc, precision=precision)
+    if autotune:
+        best, ms = wave.autotune(reps=10)
+        wave.tuned = {"schedule": best, "threads_per_workgroup": wave.schedule()[1],
+                      "ms_per_schedule": [round(float(t), 5) for t in ms]}
     dtype = torch.float64 if precision == capi.F64 else torch.float32
     u = torch.tensor(u0, dtype=dtype, device="cuda")
     stream = torch.cuda.current_stream()
@@ -96,7 +100,7 @@ def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph:
 SWEEP = [(4, 1024), (5, 800), (6, 680), (7, 580), (8, 512)]
 
 
-def sweep(torch, capi, steps: int, precisions):
+def sweep(torch, capi, steps: int, precisions, autotune: bool = True):
     """Per-order stage time and achieved bandwidth on skewed nx^2 meshes."""
     from wavedg_amd import meshgen
 
@@ -110,13 +114,14 @@ def sweep(torch, capi, steps: int, precisions):
         u0 = meshgen.nodal_state(nodes, p0)
         dt = 0.5 * mesh.min_h() / n**2
         for prec in precisions:
-            wave, u, sec, ms_k, nk = time_path(torch, capi, mesh, n, bc, u0, dt, prec, steps, 2)
+            wave, u, sec, ms_k, nk = time_path(torch, capi, mesh, n, bc, u0, dt, prec, steps, 2, autotune=autotune)
             s = 8 if prec == capi.F64 else 4
             avg = ms_k / max(nk, 1)
             gbs = stage_bytes(wave.ndof, wave.n_elem, s) / (avg * 1e-3) / 1e9
             rows.append({"P": n - 1, "nx": nx, "ndof": wave.ndof, "dtype": "f64" if s == 8 else "f32",
                          "avg_launch_us": avg * 1e3, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
-                         "dof_stage_per_s": wave.ndof * 5 * steps / sec})
+                         "dof_stage_per_s": wave.ndof * 5 * steps / sec,
+                         "schedule": getattr(wave, "tuned", None)})
             print(f"[sweep] P={n-1} {nx}^2 {rows[-1]['dtype']}: {avg*1e3:.1f} us/stage, {gbs:.0f} GB/s "
                   f"({100*gbs/HBM_PEAK_GBS:.1f}%)", file=sys.stderr, flush=True)
             del wave, u
@@ -226,6 +231,60 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
             "max_rel_l2_batched_vs_sequential": worst, "steps_per_graph": steps_per_graph}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+
+
+def cnn_flops(M: int, B: int, base: int, in_ld: int, out_ld: int):
+    """Multiply-add flops (2 per MAC) of the warm-start network's convolutions:
+    forward, and for training also backward-data (all but the first layer)
+    and weight gradients."""
+    c = [base, 2 * base, 4 * base, 8 * base]
+    layers = [(in_ld, c[0], 3, 0), (c[0], c[0], 3, 0), (c[0], c[1], 3, 1), (c[1], c[1], 3, 1), (c[1], c[2], 3, 2),
+              (c[2], c[2], 3, 2), (c[2], c[3], 3, 3), (c[3], c[3], 3, 3), (c[3] + c[2], c[2], 3, 2), (c[2], c[2], 3, 2),
+              (c[2] + c[1], c[1], 3, 1), (c[1], c[1], 3, 1), (c[1] + c[0], c[0], 3, 0), (c[0], c[0], 3, 0),
+              (c[0], out_ld, 1, 0)]
+    fwd = train = 0
+    for i, (cin, cout, ks, lev) in enumerate(layers):
+        f = 2.0 * B * (M >> lev) ** 2 * cout * cin * ks * ks
+        fwd += f
+        train += f * (3 if i > 0 else 2)
+    return fwd, train
+
+
+def cnn_rates(torch, capi, M: int = 64, B: int = 16, steps: int = 50):
+    """Training-step and inference rates of the warm-start network on
+    synthetic images, against the bf16 MFMA peak."""
+    from wavedg_amd import warmstart
+
+    net = capi.CNN(M, warmstart.IN_LD, 75, 32, B)
+    net.set_params(warmstart.init_params(32, seed=1))
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn((B, M, M, warmstart.IN_LD), device="cuda", generator=g).to(torch.bfloat16).contiguous()
+    t = torch.randn((B, M, M, net.out_ld), device="cuda", generator=g).contiguous()
+    y = torch.empty_like(t)
+    for _ in range(5):
+        net.train_step(x, t, 1e-4, want_loss=False)
+        net.forward(x, y)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(steps):
+        net.train_step(x, t, 1e-4, want_loss=False)
+    ev[1].record()
+    ev[2].record()
+    for _ in range(steps):
+        net.forward(x, y)
+    ev[3].record()
+    torch.cuda.synchronize()
+    ms_train = ev[0].elapsed_time(ev[1]) / steps
+    ms_fwd = ev[2].elapsed_time(ev[3]) / steps
+    f_fwd, f_train = cnn_flops(M, B, 32, warmstart.IN_LD, net.out_ld)
+    return {"M": M, "batch": B, "params": net.n_params, "ms_per_train_step": ms_train, "ms_per_forward": ms_fwd,
+            "train_TFLOPs": f_train / (ms_train * 1e-3) / 1e12, "forward_TFLOPs": f_fwd / (ms_fwd * 1e-3) / 1e12,
+            "train_frac_of_bf16_peak": f_train / (ms_train * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+            "flops_per_train_step": f_train, "flops_per_forward": f_fwd}
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
     from oracle import oracle as O
 
@@ -245,6 +304,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=512, help="mesh size (512 = the benchmark workload)")
     ap.add_argument("--graph", type=int, default=0, help="replay a HIP graph of this many steps (even)")
+    ap.add_argument("--only-cnn", action="store_true", help="training/inference rate of the warm-start network")
+    ap.add_argument("--cnn-M", type=int, default=64)
+    ap.add_argument("--cnn-batch", type=int, default=16)
+    ap.add_argument("--cnn-steps", type=int, default=50)
+    ap.add_argument("--autotune", type=int, default=1, help="pick the RK-stage schedule per order first (1) or not")
+    ap.add_argument("--learned", type=int, default=16,
+                    help="held-out problems for the plain vs batched+learned WaveHoltz line (0 = skip)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
@@ -289,6 +355,11 @@ def main() -> int:
         print(f"[batched] {r}", file=sys.stderr, flush=True)
         print(json.dumps({"batched": r}))
         return 0
+    if args.only_cnn:
+        r = cnn_rates(torch, capi, args.cnn_M, args.cnn_batch, args.cnn_steps)
+        print(f"[cnn] {r}", file=sys.stderr, flush=True)
+        print(json.dumps({"cnn": r}))
+        return 0
     if args.only_learned:
         from wavedg_amd import warmstart
 
@@ -320,7 +391,7 @@ def main() -> int:
           flush=True)
 
     wave64, u64, sec64, ms_k64, nk64 = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F64, args.steps,
-                                                 args.warmup, args.graph)
+                                                 args.warmup, args.graph, autotune=bool(args.autotune))
     ndof, nel = wave64.ndof, wave64.n_elem
     value = ndof * 5 * args.steps / sec64
     print(f"[bench] fp64: {sec64:.3f} s, {value:.4e} DoF*stage/s", file=sys.stderr, flush=True)
@@ -357,12 +428,13 @@ def main() -> int:
             "bytes_per_launch": bytes64,
             "avg_launch_us": avg_ms * 1e3,
             "copy_GBs": copy_gbs,
+            "schedule": getattr(wave64, "tuned", None),
         },
     }
 
     if not args.no_fp32:
         wave32, u32, sec32, ms_k32, nk32 = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F32, args.steps,
-                                                     args.warmup, args.graph)
+                                                     args.warmup, args.graph, autotune=bool(args.autotune))
         err = float(torch.linalg.norm(u32.double() - u64) / torch.linalg.norm(u64))
         avg32 = ms_k32 / max(nk32, 1)
         ach32 = stage_bytes(ndof, nel, 4) / (avg32 * 1e-3) / 1e9
@@ -373,9 +445,24 @@ def main() -> int:
             "achieved_GBs": ach32,
             "frac": ach32 / HBM_PEAK_GBS,
             "avg_launch_us": avg32 * 1e3,
+            "schedule": getattr(wave32, "tuned", None),
         }
         print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
         del wave32, u32
+
+    if args.learned > 0:
+        from wavedg_amd import warmstart
+
+        ws = warmstart.WarmStart.load(args.weights)
+        seeds = range(warmstart.EVAL_SEED0, warmstart.EVAL_SEED0 + args.learned)
+        r = warmstart.evaluate(ws, seeds, args.tol, args.max_iter, args.batch_graph, sequential=True,
+                               log=lambda m: print(f"[learned] {m}", file=sys.stderr, flush=True))
+        out["waveholtz_learned"] = {
+            "workload": f"{args.learned} held-out WaveHoltz problems: 64x64 quads on the unit square, P=4, outflow, "
+                        "omega in [5pi, 20pi], 1-4 Gaussian sources; tol 1e-6",
+            **{k: v for k, v in r.items() if not isinstance(v, list)},
+        }
+        del ws
 
     if args.waveholtz:
         wh = {}

@@ -249,15 +249,26 @@ __global__ void reduce_chunks_kernel(const float* partial, long long n, int chun
     out[i] = s;
 }
 
-// partial[chunk][c] = sum over the chunk's pixels of d[p, c]
-__global__ void bias_partial_kernel(const uint16_t* d, int C, int npix, float* partial)
+// partial[chunk][c] = sum over the chunk's pixels of d[p, c].  Thread
+// (g, c) of the block (g < G = 256 / C groups) sums pixels p = g mod G for
+// channel c (consecutive threads read consecutive channels); the G group
+// sums are then added in group order.
+__global__ __launch_bounds__(256) void bias_partial_kernel(const uint16_t* d, int C, int npix, float* partial)
 {
-    const int c = blockIdx.y * blockDim.x + threadIdx.x;
-    if (c >= C) return;
+    __shared__ float red[256];
+    const int G = max(1, 256 / C);
+    const int t = threadIdx.x, g = t / C, c = t % C;
     const int pbeg = blockIdx.x * kChunk, pend = min(npix, pbeg + kChunk);
     float s = 0.f;
-    for (int p = pbeg; p < pend; ++p) s += bf2f(d[(size_t)p * C + c]);
-    partial[(long long)blockIdx.x * C + c] = s;
+    if (g < G)
+        for (int p = pbeg + g; p < pend; p += G) s += bf2f(d[(size_t)p * C + c]);
+    red[t] = s;
+    __syncthreads();
+    if (t < C) {
+        float tot = 0.f;
+        for (int k = 0; k < G; ++k) tot += red[k * C + t];
+        partial[(size_t)blockIdx.x * C + t] = tot;
+    }
 }
 
 // ------------------------------------------------------------- elementwise
@@ -325,20 +336,24 @@ __global__ void relu_grad_kernel(PrepArgs a)
 }
 
 // Per-sample relative squared error ||y - t||^2 / ||t||^2 over the first
-// creal channels, and dy = 2 (y - t) / (B ||t||^2) as bf16 (0 on padding).
-__global__ __launch_bounds__(256) void loss_kernel(const float* y, const float* t, int ld, int creal, long long npix,
-                                                   int B, float* loss, uint16_t* dy)
+// creal channels and dy = 2 (y - t) / (B ||t||^2) as bf16 (0 on padding), in
+// three deterministic phases: per-(sample, chunk) partial sums, per-sample
+// totals in chunk order, then dy.
+constexpr int kLossChunk = 4096;  // elements per partial
+
+__global__ __launch_bounds__(256) void loss_partial_kernel(const float* y, const float* t, int ld, int creal, int n,
+                                                           double* partial)
 {
     __shared__ double red[2][256];
-    const int b = blockIdx.x;
-    const int n = (int)(npix * ld);
+    const int b = blockIdx.y, chunk = blockIdx.x;
     const size_t base = (size_t)b * n;
+    const int i0 = chunk * kLossChunk, i1 = min(n, i0 + kLossChunk);
     double tt = 0.0, ee = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) {
         if (i % ld >= creal) continue;
-        const double tv = t[base + i], d = (double)y[base + i] - tv;
+        const double tv = t[base + i], dv = (double)y[base + i] - tv;
         tt += tv * tv;
-        ee += d * d;
+        ee += dv * dv;
     }
     red[0][threadIdx.x] = tt;
     red[1][threadIdx.x] = ee;
@@ -350,13 +365,33 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* y, const float* 
         }
         __syncthreads();
     }
-    const double tn = red[0][0] > 1e-300 ? red[0][0] : 1e-300;
-    if (threadIdx.x == 0) loss[b] = (float)(red[1][0] / tn);
-    const float s = (float)(2.0 / (B * tn));
-    for (int i = threadIdx.x; i < n; i += 256) {
-        const bool real = i % ld < creal;
-        dy[base + i] = real ? f2bf(s * (y[base + i] - t[base + i])) : (uint16_t)0;
+    if (threadIdx.x == 0) {
+        partial[2 * ((size_t)b * gridDim.x + chunk)] = red[0][0];
+        partial[2 * ((size_t)b * gridDim.x + chunk) + 1] = red[1][0];
     }
+}
+
+__global__ void loss_final_kernel(const double* partial, int chunks, int B, float* loss, float* scale)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double tt = 0.0, ee = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        tt += partial[2 * ((size_t)b * chunks + c)];
+        ee += partial[2 * ((size_t)b * chunks + c) + 1];
+    }
+    const double tn = tt > 1e-300 ? tt : 1e-300;
+    loss[b] = (float)(ee / tn);
+    scale[b] = (float)(2.0 / (B * tn));
+}
+
+__global__ void loss_grad_kernel(const float* y, const float* t, int ld, int creal, int n, int B, const float* scale,
+                                 uint16_t* dy)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)B * n) return;
+    const int b = (int)(i / n), c = (int)(i % n) % ld;
+    dy[i] = c < creal ? f2bf(scale[b] * (y[i] - t[i])) : (uint16_t)0;
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master copy; c1 = 1/(1-b1^t),
@@ -623,8 +658,7 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
     hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, chunks, n.g + L.w);
-    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks, (L.cout + 63) / 64), dim3(64), 0, s, dpre, L.cout, (int)npix,
-                       n.partial);
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks), dim3(256), 0, s, dpre, L.cout, (int)npix, n.partial);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(256), 0, s, n.partial, (long long)L.cout,
                        chunks, n.g + L.b);
     if (gin)
@@ -735,8 +769,13 @@ void refresh_copies(wdg_cnn_s& n, hipStream_t s)
 void loss_grad(wdg_cnn_s& n, hipStream_t s, int B, const uint16_t* x, const float* t)
 {
     forward(n, s, x, B);
-    hipLaunchKernelGGL(loss_kernel, dim3(B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, (long long)n.M * n.M, B,
-                       n.loss, n.dy);
+    const int nper = n.M * n.M * n.out_ld;
+    const int chunks = (nper + kLossChunk - 1) / kLossChunk;
+    double* part = reinterpret_cast<double*>(n.partial);  // free between the weight-gradient passes
+    hipLaunchKernelGGL(loss_partial_kernel, dim3(chunks, B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, nper, part);
+    hipLaunchKernelGGL(loss_final_kernel, dim3((B + 63) / 64), dim3(64), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for((long long)B * nper)), dim3(256), 0, s, n.y, t, n.out_ld,
+                       n.out_real, nper, B, n.loss + n.maxB, n.dy);
     backward(n, s, x, B);
 }
 
@@ -813,6 +852,9 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
             const long long chunks = ((long long)max_batch * H * H + kChunk - 1) / kChunk;
             maxpart = std::max(maxpart, chunks * std::max<long long>(nw, L.cout));
         }
+        // the loss partials (two doubles per sample and chunk) reuse the buffer
+        const long long loss_chunks = ((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk;
+        maxpart = std::max(maxpart, 4LL * max_batch * loss_chunks);
         n->nparam = off;
         n->nflip = foff;
         n->partial_len = maxpart;
@@ -823,7 +865,7 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
         n->wbf = n->alloc<uint16_t>(off);
         n->wflip = n->alloc<uint16_t>(foff);
         n->partial = n->alloc<float>(maxpart);
-        n->loss = n->alloc<float>(max_batch);
+        n->loss = n->alloc<float>(2 * (size_t)max_batch);  // per-sample loss, then dy scale
         const long long B = max_batch;
         auto px = [&](int level) { return B * (long long)(M >> level) * (M >> level); };
         n->e1a = n->alloc<uint16_t>(px(0) * C1);

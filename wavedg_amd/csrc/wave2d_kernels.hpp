@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace wdg {
 
@@ -87,6 +88,9 @@ constexpr int kModeOf[K_COUNT] = {
 #ifndef WDG_MIN_WAVES
 #define WDG_MIN_WAVES 1  // occupancy hint: minimum waves per SIMD
 #endif
+#ifndef WDG_FACE_GEOM_DOUBLE
+#define WDG_FACE_GEOM_DOUBLE 0  // 1: face normals and weights in double for every precision
+#endif
 #ifndef WDG_NT_STORES
 #define WDG_NT_STORES 0  // 1: stream stage outputs with non-temporal stores
 #endif
@@ -136,6 +140,27 @@ __device__ __forceinline__ double recip(double x)
     return r;
 }
 
+// 1/x in single precision: reciprocal estimate and one Newton step.
+__device__ __forceinline__ float recip(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
+// 1/sqrt(x) from the hardware estimate and Newton steps (two in double,
+// one in single precision).
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * (1.5 - 0.5 * x * r * r);
+    return r * (1.5 - 0.5 * x * r * r);
+}
+__device__ __forceinline__ float rsqrt_nr(float x)
+{
+    const float r = __builtin_amdgcn_rsqf(x);
+    return r * (1.5f - 0.5f * x * r * r);
+}
+
 // Node (k along side s, counted from the side's first corner) -> flat node.
 template <int N>
 __device__ __forceinline__ int side_node(int s, int k)
@@ -167,7 +192,7 @@ template <typename R>
 struct NodeResult {
     R p, uu, vv;
     R r0, r1, r2;
-    double m;
+    R m;
 };
 
 // The operator for one workgroup of elements [e0, e0 + nloc) of one mesh,
@@ -228,9 +253,13 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     // Bilinear element map x(xi, eta) = x0 + a xi + c eta + h xi eta (per
     // coordinate), so dx/dxi = a + h eta, dx/deta = c + h xi (Element.cpp:21-27)
     // and side s runs along 2(a -+ h) (s = 0, 2) or 2(c +- h) (s = 1, 3).
+    // Volume metrics are evaluated in the state precision R (double for FP64
+    // handles); face normals and weights in FG.
+    using FG = typename std::conditional<WDG_FACE_GEOM_DOUBLE, double, R>::type;
     const double2* gp = reinterpret_cast<const double2*>(geom + 6 * (size_t)e);
     const double2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
-    const double ax = g_a.x, ay = g_a.y, cx = g_c.x, cy = g_c.y, hx = g_h.x, hy = g_h.y;
+    const R ax = (R)g_a.x, ay = (R)g_a.y, cx = (R)g_c.x, cy = (R)g_c.y, hx = (R)g_h.x, hy = (R)g_h.y;
+    const FG fax = (FG)g_a.x, fay = (FG)g_a.y, fcx = (FG)g_c.x, fcy = (FG)g_c.y, fhx = (FG)g_h.x, fhy = (FG)g_h.y;
 
     // Face lanes: face node f = node + p NN (side f / N, position f % N).
     R nbv[FP][3];
@@ -274,9 +303,9 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     }
     __syncthreads();
 
-    const double xi = s_x[i], eta = s_x[j];
-    const double J0 = ax + hx * eta, J1 = ay + hy * eta, J2 = cx + hx * xi, J3 = cy + hy * xi;
-    const double Wt = s_w[j] * s_w[i];
+    const R xi = (R)s_x[i], eta = (R)s_x[j];
+    const R J0 = ax + hx * eta, J1 = ay + hy * eta, J2 = cx + hx * xi, J3 = cy + hy * xi;
+    const R Wt = (R)s_w[j] * (R)s_w[i];
 
     NodeResult<R> out;
     out.p = s_u[el * 3 * NN + 3 * node + 0];
@@ -289,7 +318,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
         // Contravariant fluxes F = W (y_eta f - x_eta g), G = W (-y_xi f + x_xi g),
         // f = (u, p, 0), g = (v, 0, p) (Div.cpp:5-53).
         if (worker) {
-            const R Ye = (R)(J3 * Wt), Xe = (R)(J2 * Wt), Yx = (R)(J1 * Wt), Xx = (R)(J0 * Wt);
+            const R Ye = J3 * Wt, Xe = J2 * Wt, Yx = J1 * Wt, Xx = J0 * Wt;
             R* F = s_F + el * 3 * NN;
             R* G = s_G + el * 3 * NN;
             F[0 * NN + j * N + i] = Ye * uu - Xe * vv;
@@ -322,17 +351,15 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
             if (interior ? IFACE : BFACE) {
                 const bool flip = (inf >> 2) & 1, first = (inf >> 3) & 1;
                 // own side vector in traversal order; sides 2, 3 run clockwise
-                const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
-                const double dx = 2.0 * ((s & 1) ? cx + sh * hx : ax + sh * hx);
-                const double dy = 2.0 * ((s & 1) ? cy + sh * hy : ay + sh * hy);
-                const double l2 = dx * dx + dy * dy;
-                double il = __builtin_amdgcn_rsq(l2);  // 1/len, two Newton steps
-                il = il * (1.5 - 0.5 * l2 * il * il);
-                il = il * (1.5 - 0.5 * l2 * il * il);
-                const double sil = (s >= 2) ? -il : il;
+                const FG sh = (s == 0 || s == 3) ? (FG)-1 : (FG)1;
+                const FG dx = (FG)2 * ((s & 1) ? fcx + sh * fhx : fax + sh * fhx);
+                const FG dy = (FG)2 * ((s & 1) ? fcy + sh * fhy : fay + sh * fhy);
+                const FG l2 = dx * dx + dy * dy;
+                const FG il = rsqrt_nr(l2);  // 1/len
+                const FG sil = (s >= 2) ? -il : il;
                 const R n0 = (R)(dy * sil), n1 = (R)(-dx * sil);
                 const int ie = (!first && flip) ? (N - 1 - k) : k;  // node index in el0's order
-                const R wds = (R)(s_w[ie] * (0.5 * (l2 * il)));
+                const R wds = (R)((FG)s_w[ie] * ((FG)0.5 * (l2 * il)));
                 const int ln = side_node<N>(s, k);
                 const R pA = s_u[el * 3 * NN + 3 * ln + 0];
                 const R uA = s_u[el * 3 * NN + 3 * ln + 1];
@@ -478,7 +505,7 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), WDG_MIN_WAVES) void wave_kerne
         y[g + 1] = pre1 + r1;
         y[g + 2] = pre2 + r2;
     } else {
-        const R im = (R)recip(q.m);
+        const R im = recip(q.m);
         if constexpr (FORCE) {
             const R fs = (R)a.fs;
             r0 += fv0 * fs;
