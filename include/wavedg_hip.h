@@ -217,6 +217,11 @@ int wdg_hip_batch_set_initial(wdg_batch b, int k, const double *d_W0);
 /* Runs every problem to tol (or max_iter updates), steps_per_graph RK steps
  * per graph replay.  Synchronises `stream`. */
 int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_graph, void *stream);
+/* Retire a problem also when its residual has not improved by 3 % for
+ * `window` consecutive updates (0, the default, disables this).  For an
+ * FP32 batch whose rounding floor lies above the tolerance: its iterates
+ * then start an FP64 batch (wdg_hip_batch_set_initial). */
+int wdg_hip_batch_set_stall_window(wdg_batch b, int window);
 /* Solution (device, double, ndof; may be NULL), number of W updates and the
  * last residual of problem k. */
 int wdg_hip_batch_result(wdg_batch b, int k, double *d_W, int *iterations, double *residual);

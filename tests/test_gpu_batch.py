@@ -94,3 +94,18 @@ def test_batch_warm_start_and_resolve(cuda):
         W3 = torch.empty_like(W)
         it, _ = b.result(k, W3)
         assert it == it_cold and torch.equal(W3, W)
+
+
+def test_stall_window_retires_fp32_at_its_floor(cuda):
+    """An FP32 batch asked for an unreachable residual stops once the residual
+    stops improving (stall window), long before max_iter; without the window
+    it would run to max_iter."""
+    torch = cuda
+    probs = _problems()[:3]
+    b = capi.Batch(probs, capi.F32)
+    b.set_stall_window(3)
+    b.solve(1e-14, 400, 16)
+    for k in range(len(probs)):
+        it, r = b.result(k)
+        print(f"fp32 stall: problem {k} stopped after {it} updates at residual {r:.2e}", flush=True)
+        assert it < 400 and r > 1e-14

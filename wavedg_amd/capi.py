@@ -74,6 +74,7 @@ _SIGS = {
     "wdg_hip_batch_set_initial": [_vp, _i, _vp],
     "wdg_hip_batch_solve": [_vp, _d, _i, _i, _vp],
     "wdg_hip_batch_result": [_vp, _i, _vp, _ip, _dp],
+    "wdg_hip_batch_set_stall_window": [_vp, _i],
     "wdg_hip_batch_timing": [_vp, _dp, ctypes.POINTER(_ll)],
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
@@ -414,6 +415,9 @@ class Batch:
         nd, nt, n = _ll(), _i(), _i()
         _call("wdg_hip_batch_info", self.h, k, ctypes.byref(nd), ctypes.byref(nt), ctypes.byref(n))
         return nd.value, nt.value, n.value
+
+    def set_stall_window(self, window: int) -> None:
+        _call("wdg_hip_batch_set_stall_window", self.h, window)
 
     def set_initial(self, k: int, W0=None):
         _call("wdg_hip_batch_set_initial", self.h, k, dev_ptr(W0))

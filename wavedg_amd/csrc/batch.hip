@@ -259,7 +259,8 @@ __global__ __launch_bounds__(kBT) void batch_stage_kernel(const BatchArgs a)
 __global__ __launch_bounds__(kBT) void batch_reduce_kernel(const BatchProb* probs, const int* prob_ids,
                                                            const double* partial, int* active, long long* G,
                                                            unsigned int* done, double* pn, double* hist, int* iters,
-                                                           int hist_len, double tol, int max_iter)
+                                                           int hist_len, double tol, int max_iter, double* best,
+                                                           int* stall, int stall_window)
 {
     __shared__ double red[kBT];
     const int k = prob_ids[blockIdx.x];
@@ -285,7 +286,17 @@ __global__ __launch_bounds__(kBT) void batch_reduce_kernel(const BatchProb* prob
             }
             if (it < hist_len) hist[(long long)k * hist_len + it] = res;
             iters[k] = (int)it + 1;
-            if ((it >= 1 && res <= tol) || it + 1 >= max_iter || !(res == res)) active[k] = 0;
+            bool stalled = false;
+            if (stall_window > 0 && it >= 1) {
+                // retire a problem whose residual stopped improving (3 % over the window)
+                if (res < 0.97 * best[k]) {
+                    best[k] = res;
+                    stall[k] = 0;
+                } else {
+                    stalled = ++stall[k] >= stall_window;
+                }
+            }
+            if ((it >= 1 && res <= tol) || it + 1 >= max_iter || !(res == res) || stalled) active[k] = 0;
         }
     }
     if (threadIdx.x == 0) {
@@ -319,12 +330,15 @@ __global__ void batch_convert(const S* x, D* y, long long n)
     if (k < n) y[k] = (D)x[k];
 }
 
-__global__ void batch_init_state(int* active, int n, long long* G, unsigned int* done, int* iters)
+__global__ void batch_init_state(int* active, int n, long long* G, unsigned int* done, int* iters, double* best,
+                                 int* stall)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
         active[k] = 1;
         iters[k] = 0;
+        best[k] = 1e300;
+        stall[k] = 0;
     }
     if (k == 0) {
         *G = 0;
@@ -409,6 +423,9 @@ struct wdg_batch_s {
     int* d_warm = nullptr;
     int* d_active = nullptr;
     int* d_iters = nullptr;
+    double* d_best = nullptr;  // best residual so far (stall detection)
+    int* d_stall = nullptr;    // iterations since the last 3 % improvement
+    int stall_window = 0;      // 0: no stall exit
     double* d_partial = nullptr;
     double* d_pn = nullptr;
     double* d_hist = nullptr;
@@ -422,7 +439,7 @@ struct wdg_batch_s {
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
-        void* ptrs[] = {d_prob_ids, d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
+        void* ptrs[] = {d_best, d_stall, d_prob_ids, d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
                         d_acc, d_wprev, d_pi0, d_winit, d_warm, d_active, d_iters, d_partial, d_pn, d_hist, d_G,
                         d_done};
         for (void* p : ptrs)
@@ -472,7 +489,8 @@ void launch_step(const wdg_batch_s& b, hipStream_t s, double tol, int max_iter)
             hipLaunchKernelGGL((batch_stage_kernel<R, 2>), dim3(b.n_run_blocks), dim3(kBT), b.lds, s, a);
     }
     hipLaunchKernelGGL(batch_reduce_kernel, dim3(b.n_run_probs), dim3(kBT), 0, s, b.d_probs, b.d_prob_ids, b.d_partial,
-                       b.d_active, b.d_G, b.d_done, b.d_pn, b.d_hist, b.d_iters, b.hist_len, tol, max_iter);
+                       b.d_active, b.d_G, b.d_done, b.d_pn, b.d_hist, b.d_iters, b.hist_len, tol, max_iter, b.d_best,
+                       b.d_stall, b.stall_window);
 }
 
 // Tables of the problems still running (host flags), uploaded for the next capture.
@@ -646,6 +664,8 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         b->d_warm = dalloc<int>(n_problems);
         b->d_active = dalloc<int>(n_problems);
         b->d_iters = dalloc<int>(n_problems);
+        b->d_best = dalloc<double>(n_problems);
+        b->d_stall = dalloc<int>(n_problems);
         b->d_partial = dalloc<double>(b->nblk);
         b->d_pn = dalloc<double>(n_problems);
         b->d_G = dalloc<long long>(1);
@@ -757,7 +777,7 @@ int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_gra
         hipLaunchKernelGGL(batch_zero_kernel, dim3(gr), dim3(256), 0, s, b->d_acc, b->ntot);
         hipLaunchKernelGGL(batch_zero_kernel, dim3(gr), dim3(256), 0, s, b->d_wprev, b->ntot);
         hipLaunchKernelGGL(batch_init_state, dim3((b->nprob + 255) / 256), dim3(256), 0, s, b->d_active, b->nprob,
-                           b->d_G, b->d_done, b->d_iters);
+                           b->d_G, b->d_done, b->d_iters, b->d_best, b->d_stall);
         bcheck(hipGetLastError(), "batch reset");
         std::vector<int> act(b->nprob, 1), run(b->nprob, 1);
         auto step = [&](hipStream_t cs) {
@@ -797,6 +817,14 @@ int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_gra
         (void)hipEventDestroy(e1);
         b->last_seconds = ms * 1e-3;
         b->last_steps = steps;
+    });
+}
+
+int wdg_hip_batch_set_stall_window(wdg_batch b, int window)
+{
+    return bguard([&] {
+        if (!b || window < 0) throw Fail{WDG_ERR_INVALID, "bad batch handle or window"};
+        b->stall_window = window;
     });
 }
 

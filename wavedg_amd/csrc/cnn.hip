@@ -169,6 +169,114 @@ __global__ __launch_bounds__(64 * kWaves) void conv_kernel(ConvArgs a)
     }
 }
 
+// LDS-tiled 3x3 convolution (same operands and accumulator layout as
+// conv_kernel).  A workgroup owns CO_T = 32 TA output channels of one tile of
+// 64 pixels: TH rows x TW columns of one image, TW = min(W, 64), W a power of
+// two.  For each slice of 16 input channels it stages the tile with its
+// one-pixel halo (zero outside the image, at most 3 x 66 pixels) and the
+// slice's 9 x CO_T x 16 weights in LDS with 16-byte loads, so every input
+// value is fetched once per slice instead of once per tap; then each wave
+// runs 9 taps x TA x 2 MFMAs on 16 TA channels x 32 pixels.
+constexpr int kHaloMax = 198;  // (TH + 2)(TW + 2) for TW in {1, ..., 64}, TH = 64 / TW
+
+template <int TA, bool F32OUT>
+__global__ __launch_bounds__(256) void conv3_tiled_kernel(ConvArgs a)
+{
+    constexpr int CO_T = 32 * TA;
+    __shared__ __align__(16) uint16_t s_in[kHaloMax * 16];
+    __shared__ __align__(16) uint16_t s_w[9 * CO_T * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const int TW = min(a.W, 64), TH = 64 / TW;
+    const int tiles_x = a.W / TW, tiles_y = (a.H + TH - 1) / TH;
+    const int t = blockIdx.x;
+    const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int co0 = blockIdx.y * CO_T;
+    const int cw = (wave & 1) * (16 * TA);  // the wave's channels within the tile
+    const int pw = (wave >> 1) * 32;        // the wave's first pixel within the tile
+    const int HW2 = TW + 2, NH = (TH + 2) * HW2;
+    int py[2], px[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int p = pw + 16 * j + ln;
+        py[j] = p / TW;
+        px[j] = p - py[j] * TW;
+    }
+    f32x4 acc[TA][2];
+#pragma unroll
+    for (int i = 0; i < TA; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < a.cin; c0 += 16) {
+        __syncthreads();  // the previous slice is consumed
+        for (int q = tid; q < NH * 2; q += 256) {
+            const int hp = q >> 1, half = q & 1;
+            const int hy = hp / HW2, hx = hp - hy * HW2;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
+                                                    c0 + 8 * half);
+            *reinterpret_cast<uint4*>(s_in + hp * 16 + 8 * half) = v;
+        }
+        for (int q = tid; q < 9 * CO_T * 2; q += 256) {
+            const int row = q >> 1, half = q & 1;
+            const int tap = row / CO_T, co = row - tap * CO_T;
+            *reinterpret_cast<uint4*>(s_w + row * 16 + 8 * half) = *reinterpret_cast<const uint4*>(
+                a.w + ((size_t)tap * a.cout + co0 + co) * a.cin + c0 + 8 * half);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int hy = tap / 3, hx = tap % 3;  // halo offsets dy + 1, dx + 1
+            bf16x4 av[TA], bv[2];
+#pragma unroll
+            for (int i = 0; i < TA; ++i)
+                av[i] = __builtin_bit_cast(
+                    bf16x4, *reinterpret_cast<const uint2*>(s_w + (tap * CO_T + cw + 16 * i + ln) * 16 + 4 * kq));
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                bv[j] = __builtin_bit_cast(
+                    bf16x4, *reinterpret_cast<const uint2*>(s_in + ((py[j] + hy) * HW2 + px[j] + hx) * 16 + 4 * kq));
+#pragma unroll
+            for (int i = 0; i < TA; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TA; ++i) {
+        const int co = co0 + cw + 16 * i + 4 * kq;
+        float bias[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.bias) {
+            const float4 b4 = *reinterpret_cast<const float4*>(a.bias + co);
+            bias[0] = b4.x, bias[1] = b4.y, bias[2] = b4.z, bias[3] = b4.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int y = y0 + py[j], x = x0 + px[j];
+            if (y >= a.H) continue;
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                v[q] = acc[i][j][q] + bias[q];
+                if (a.relu) v[q] = fmaxf(v[q], 0.f);
+            }
+            const size_t o = ((size_t)(b * a.H + y) * a.W + x) * a.out_ld + a.out_off + co;
+            if constexpr (F32OUT) {
+                *reinterpret_cast<float4*>(static_cast<float*>(a.out) + o) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out) + o) = make_uint2(lo, hi);
+            }
+        }
+    }
+}
+
 // dw[tap][co][ci] = sum_p dout[p, co] in[p + off(tap), ci]: a GEMM with
 // M = output channels, N = input channels, K = pixels.  One wave owns one
 // 16 x 16 tile of one tap over one chunk of pixels and writes its partial
@@ -180,6 +288,7 @@ struct WgradArgs {
     int in_ld, in_off, cin;
     int dout_ld, cout;
     int B, H, W, ks;
+    int tiles_per_chunk;  // wgrad3_tiled_kernel only
 };
 
 __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
@@ -239,30 +348,143 @@ __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
     for (int r = 0; r < 4; ++r) dst[(size_t)(cob * 16 + 4 * kq + r) * a.cin + cib * 16 + ln] = acc[r];
 }
 
-// out[i] = sum_{c < chunks} partial[c n + i], in chunk order.
+// Weight gradient with LDS tiles (3x3, W a power of two; same 64-pixel tiles
+// as conv3_tiled_kernel).  A workgroup owns 32 output x CI_T input channels
+// for all 9 taps and a chunk of kWgTiles tiles.  Per tile it stages the
+// output gradient transposed ([co][pixel], so an A fragment is one 8-byte
+// read) and the input with its halo transposed ([ci][halo pixel]); wave w
+// accumulates the (tap, co block, ci block) MFMA tiles w, w + 4, ... over the
+// chunk (a.tiles_per_chunk tiles) and writes its partial sums to
+// partial[chunk][tap][co][ci].
+
+template <int CI_T>
+__global__ __launch_bounds__(256) void wgrad3_tiled_kernel(WgradArgs a)
+{
+    constexpr int NO = 9 * 2 * (CI_T / 16);  // output MFMA tiles
+    constexpr int NQ = (NO + 3) / 4;         // per wave
+    __shared__ __align__(16) uint16_t s_d[32 * 64];
+    __shared__ __align__(16) uint16_t s_x[CI_T * kHaloMax];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const int TW = min(a.W, 64), TH = 64 / TW;
+    const int sh = __builtin_ctz((unsigned)TW);
+    const int tiles_x = a.W / TW, tiles_y = (a.H + TH - 1) / TH;
+    const int ntiles = a.B * tiles_y * tiles_x;
+    const int HW2 = TW + 2, NH = (TH + 2) * HW2;
+    const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * CI_T;
+    const int t0 = blockIdx.x * a.tiles_per_chunk, t1 = min(ntiles, t0 + a.tiles_per_chunk);
+    f32x4 acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = t0; t < t1; ++t) {
+        const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
+        const int y0 = ty * TH, x0 = tx * TW;
+        __syncthreads();  // the previous tile is consumed
+        {
+            // output gradient: 64 pixels x 32 channels, four 8-channel parts per pixel
+            const int pp = tid >> 2, part = tid & 3;
+            const int y = y0 + (pp >> sh), x = x0 + (pp & (TW - 1));
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y < a.H)
+                v = *reinterpret_cast<const uint4*>(a.dout + ((size_t)(b * a.H + y) * a.W + x) * a.dout_ld + co0 +
+                                                    8 * part);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                s_d[(8 * part + k) * 64 + pp] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+        }
+        for (int q = tid; q < NH * (CI_T / 8); q += 256) {
+            const int hp = q / (CI_T / 8), part = q - hp * (CI_T / 8);
+            const int hy = hp / HW2, hx = hp - hy * HW2;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
+                                                    ci0 + 8 * part);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                s_x[(8 * part + k) * kHaloMax + hp] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int ot = wave + 4 * q;
+            if (ot >= NO) break;  // uniform per wave
+            const int tap = ot / (2 * (CI_T / 16)), rem = ot - tap * (2 * (CI_T / 16));
+            const int cb = rem / (CI_T / 16), ib = rem - cb * (CI_T / 16);
+            const int hy = tap / 3, hx = tap % 3;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int pb = 16 * k + 4 * kq;  // this lane's pixels pb .. pb + 3
+                const bf16x4 av =
+                    __builtin_bit_cast(bf16x4, *reinterpret_cast<const uint2*>(s_d + (cb * 16 + ln) * 64 + pb));
+                bf16x4 bv;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int pr = pb + r;
+                    bv[r] = (short)s_x[(ib * 16 + ln) * kHaloMax + ((pr >> sh) + hy) * HW2 + (pr & (TW - 1)) + hx];
+                }
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int ot = wave + 4 * q;
+        if (ot >= NO) break;
+        const int tap = ot / (2 * (CI_T / 16)), rem = ot - tap * (2 * (CI_T / 16));
+        const int cb = rem / (CI_T / 16), ib = rem - cb * (CI_T / 16);
+        float* dst = a.partial + ((size_t)blockIdx.x * 9 + tap) * a.cout * a.cin;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            dst[(size_t)(co0 + cb * 16 + 4 * kq + r) * a.cin + ci0 + ib * 16 + ln] = acc[q][r];
+    }
+}
+
+// out[i] = sum_{c < chunks} partial[c n + i]: four interleaved running sums
+// (chunks c = 0, 1, 2, 3 mod 4) combined in a fixed order, so the result does
+// not depend on the launch and several loads are in flight per thread.
 __global__ void reduce_chunks_kernel(const float* partial, long long n, int chunks, float* out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * n + i];
-    out[i] = s;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int c = 0;
+    for (; c + 4 <= chunks; c += 4) {
+        s0 += partial[(size_t)c * n + i];
+        s1 += partial[(size_t)(c + 1) * n + i];
+        s2 += partial[(size_t)(c + 2) * n + i];
+        s3 += partial[(size_t)(c + 3) * n + i];
+    }
+    if (c < chunks) s0 += partial[(size_t)c * n + i];
+    if (c + 1 < chunks) s1 += partial[(size_t)(c + 1) * n + i];
+    if (c + 2 < chunks) s2 += partial[(size_t)(c + 2) * n + i];
+    out[i] = (s0 + s1) + (s2 + s3);
 }
 
-// partial[chunk][c] = sum over the chunk's pixels of d[p, c].  Thread
-// (g, c) of the block (g < G = 256 / C groups) sums pixels p = g mod G for
-// channel c (consecutive threads read consecutive channels); the G group
-// sums are then added in group order.
+// partial[chunk][c] = sum over the chunk's kBiasChunk pixels of d[p, c].
+// Thread (g, c) of the block (g < G = 256 / C groups) sums pixels g, g + G,
+// ... for channel c (consecutive threads read consecutive channels) with two
+// interleaved running sums; the G group sums are then added in group order.
+constexpr int kBiasChunk = 256;
+
 __global__ __launch_bounds__(256) void bias_partial_kernel(const uint16_t* d, int C, int npix, float* partial)
 {
     __shared__ float red[256];
     const int G = max(1, 256 / C);
     const int t = threadIdx.x, g = t / C, c = t % C;
-    const int pbeg = blockIdx.x * kChunk, pend = min(npix, pbeg + kChunk);
-    float s = 0.f;
-    if (g < G)
-        for (int p = pbeg + g; p < pend; p += G) s += bf2f(d[(size_t)p * C + c]);
-    red[t] = s;
+    const int pbeg = blockIdx.x * kBiasChunk, pend = min(npix, pbeg + kBiasChunk);
+    float s0 = 0.f, s1 = 0.f;
+    if (g < G) {
+        int p = pbeg + g;
+        for (; p + G < pend; p += 2 * G) {
+            s0 += bf2f(d[(size_t)p * C + c]);
+            s1 += bf2f(d[(size_t)(p + G) * C + c]);
+        }
+        if (p < pend) s0 += bf2f(d[(size_t)p * C + c]);
+    }
+    red[t] = s0 + s1;
     __syncthreads();
     if (t < C) {
         float tot = 0.f;
@@ -606,6 +828,26 @@ struct wdg_cnn_s {
 
 namespace {
 
+// The tiled weight-gradient kernel covers 3x3 layers with 32k output and 16k
+// input channels on power-of-two images.
+bool wgrad_tiled_ok(const Layer& L, int H, int in_ld, int in_off)
+{
+    return L.ks == 3 && H > 0 && (H & (H - 1)) == 0 && L.cout % 32 == 0 && L.cin % 16 == 0 && in_ld % 8 == 0 &&
+           in_off % 8 == 0;
+}
+
+// Tiles per chunk of the tiled weight gradient: enough chunks for about 512
+// workgroups, at most 16 tiles each.  Returns the chunk count.
+int wgrad_tiled_chunks(const Layer& L, int B, int H, int* tiles_per_chunk)
+{
+    const int TW = std::min(H, 64), TH = 64 / TW;
+    const long long tiles = (long long)B * ((H + TH - 1) / TH) * (H / TW);
+    const long long per = (long long)(L.cout / 32) * (L.cin / (L.cin % 32 == 0 ? 32 : 16));
+    const int tpc = (int)std::max<long long>(1, std::min<long long>(16, tiles * per / 512));
+    if (tiles_per_chunk) *tiles_per_chunk = tpc;
+    return (int)((tiles + tpc - 1) / tpc);
+}
+
 void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_ld, int in_off, int cin,
                  const uint16_t* w, const float* bias, void* out, int out_ld, int out_off, int cout, bool f32out,
                  int B, int H, int W, int ks, bool relu)
@@ -613,6 +855,25 @@ void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_l
     (void)n;
     ConvArgs a{in, w, bias, out, in_ld, in_off, cin, out_ld, out_off, cout, B, H, W, ks, relu ? 1 : 0};
     const long long npix = (long long)B * H * W;
+    const bool pow2w = W > 0 && (W & (W - 1)) == 0;
+    if (ks == 3 && pow2w && cout % 32 == 0 && cin % 16 == 0 && in_ld % 8 == 0 && in_off % 8 == 0) {
+        const int TW = std::min(W, 64), TH = 64 / TW;
+        const long long tiles = (long long)B * ((H + TH - 1) / TH) * (W / TW);
+        const int TA = cout % 64 == 0 ? 2 : 1;
+        const dim3 grid((unsigned)tiles, (unsigned)(cout / (32 * TA)));
+        if (TA == 2) {
+            if (f32out)
+                hipLaunchKernelGGL((conv3_tiled_kernel<2, true>), grid, dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((conv3_tiled_kernel<2, false>), grid, dim3(256), 0, s, a);
+        } else {
+            if (f32out)
+                hipLaunchKernelGGL((conv3_tiled_kernel<1, true>), grid, dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((conv3_tiled_kernel<1, false>), grid, dim3(256), 0, s, a);
+        }
+        return;
+    }
     const int nb = cout / 16;
     const int TA = nb % 4 == 0 ? 4 : (nb % 2 == 0 ? 2 : 1);
     int TB = 4;
@@ -654,13 +915,25 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int chunks = (int)((npix + kChunk - 1) / kChunk);
     const int ntap = L.ks * L.ks;
     const long long nw = (long long)ntap * L.cout * L.cin;
-    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks};
-    const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
-    hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, chunks, n.g + L.w);
-    hipLaunchKernelGGL(bias_partial_kernel, dim3(chunks), dim3(256), 0, s, dpre, L.cout, (int)npix, n.partial);
+    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1};
+    int wchunks = chunks;
+    if (wgrad_tiled_ok(L, H, in_ld, in_off)) {
+        wchunks = wgrad_tiled_chunks(L, B, H, &wa.tiles_per_chunk);
+        const int cit = L.cin % 32 == 0 ? 32 : 16;
+        const dim3 grid(wchunks, L.cout / 32, L.cin / cit);
+        if (cit == 32)
+            hipLaunchKernelGGL(wgrad3_tiled_kernel<32>, grid, dim3(256), 0, s, wa);
+        else
+            hipLaunchKernelGGL(wgrad3_tiled_kernel<16>, grid, dim3(256), 0, s, wa);
+    } else {
+        const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
+        hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
+    }
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, wchunks, n.g + L.w);
+    const int bchunks = (int)((npix + kBiasChunk - 1) / kBiasChunk);
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(bchunks), dim3(256), 0, s, dpre, L.cout, (int)npix, n.partial);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(256), 0, s, n.partial, (long long)L.cout,
-                       chunks, n.g + L.b);
+                       bchunks, n.g + L.b);
     if (gin)
         launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H, L.ks,
                     false);
@@ -849,8 +1122,13 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
             L.wb = foff;
             foff += nw;
             const int H = M >> L.level;
-            const long long chunks = ((long long)max_batch * H * H + kChunk - 1) / kChunk;
-            maxpart = std::max(maxpart, chunks * std::max<long long>(nw, L.cout));
+            // the largest chunk count any batch size up to max_batch uses
+            long long tchunks = 0;
+            for (int bb = 1; bb <= max_batch; ++bb) tchunks = std::max<long long>(tchunks, wgrad_tiled_chunks(L, bb, H, nullptr));
+            const long long chunks =
+                std::max<long long>(((long long)max_batch * H * H + kChunk - 1) / kChunk, tchunks);
+            const long long bchunks = ((long long)max_batch * H * H + kBiasChunk - 1) / kBiasChunk;
+            maxpart = std::max({maxpart, chunks * nw, bchunks * L.cout});
         }
         // the loss partials (two doubles per sample and chunk) reuse the buffer
         const long long loss_chunks = ((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk;

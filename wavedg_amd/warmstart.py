@@ -226,10 +226,11 @@ def validation_loss(ws: WarmStart, X, T) -> float:
 
 
 def solve_batched(ws: WarmStart, specs, tol: float, max_iter: int, warm: bool = True, mixed: bool = False,
-                  tol_switch: float = 1e-5, steps_per_graph: int = 16):
+                  tol_switch: float | None = None, stall_window: int = 3, steps_per_graph: int = 16):
     """Batched WaveHoltz on the problems `specs`, optionally warm-started by the
     network and optionally in two precisions: an FP32-state batch to
-    tol_switch, then an FP64 batch started from its iterates to tol.  The
+    tol_switch (default tol) or until its residual stalls at the FP32
+    rounding floor, then an FP64 batch started from its iterates to tol.  The
     final FP64 iteration decides the residual either way.  Returns the
     solutions, per-problem update counts (FP32 phase, FP64 phase), the last
     residuals and the wall times (inference, solves)."""
@@ -245,8 +246,10 @@ def solve_batched(ws: WarmStart, specs, tol: float, max_iter: int, warm: bool = 
     if init is not None:
         for k, W in enumerate(init):
             first.set_initial(k, W)
+    if mixed:
+        first.set_stall_window(stall_window)
     t0 = time.perf_counter()
-    first.solve(tol_switch if mixed else tol, max_iter, steps_per_graph)
+    first.solve((tol_switch or tol) if mixed else tol, max_iter, steps_per_graph)
     it1 = []
     Ws = []
     for k in range(len(probs)):
