@@ -113,6 +113,14 @@ int wdg_hip_wave2d_info(wdg_wave2d op, long long *ndof, int *n_colloc, int *n_el
  * (flux_action_2d_c, EdgeFlux.cpp:289-325) and lift (lobatto_transpose_2d,
  * FaceProlongator.cpp:650-672) in one kernel.  d_x and d_y must not overlap. */
 int wdg_hip_wave2d_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
+/* DivF2D<true>::action (wavedg/DivF.hpp:207-274) for the linear flux functor
+ * f(x, u) = (A u, B u): d_divF += (F, grad v) over the handle's mesh and
+ * basis, for n_var <= 8 variables in the layout (n_var, n, n, n_elem) and
+ * the handle's precision.  A and B are host (n_var, n_var) column-major
+ * matrices (x- and y-flux).  With the wave equation's matrices it is the
+ * volume term of WaveEquation::action. */
+int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double *A, const double *B, const void *d_u, void *d_divF,
+                          void *stream);
 /* WaveBC::action (WaveEquation.cpp:190-215): d_y += B d_x (boundary flux and lift). */
 int wdg_hip_wavebc_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
 /* MassMatrix<true>::inv (MassMatrix.cpp:69-82): d_x <- M^{-1} d_x, in place. */

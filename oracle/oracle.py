@@ -54,6 +54,8 @@ def lib():
             "orc_wave_free": (None, [_vp]),
             "orc_wave_ndof": (ctypes.c_longlong, [_vp]),
             "orc_wave_action": (None, [_vp, _dp, _dp]),
+            "orc_wave_div": (None, [_vp, _dp, _dp]),
+            "orc_divf_linear": (None, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
             "orc_wavebc_action": (None, [_vp, _dp, _dp]),
             "orc_mass_action": (None, [_vp, ctypes.c_int, _dp, _dp]),
             "orc_mass_inv": (None, [_vp, ctypes.c_int, _dp]),
@@ -206,6 +208,21 @@ class Wave:
         du = np.zeros(self.ndof) if du is None else du
         lib().orc_wave_action(self.h, _d(np.ascontiguousarray(u)), _d(du))
         return du
+
+    def div(self, u, du=None):
+        """Div<true>::action alone (the volume term of action)."""
+        du = np.zeros(self.ndof) if du is None else du
+        lib().orc_wave_div(self.h, _d(np.ascontiguousarray(u)), _d(du))
+        return du
+
+    def divf_linear(self, nvar: int, A, B, u, divF=None):
+        """DivF2D<true>::action with f(x, u) = (A u, B u); u is (nvar, n, n, nel)."""
+        A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(nvar, nvar).T.reshape(-1))  # column-major
+        B = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(nvar, nvar).T.reshape(-1))
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        divF = np.zeros(u.size) if divF is None else divF
+        lib().orc_divf_linear(self.h, nvar, _d(A), _d(B), _d(u), _d(divF))
+        return divF
 
     def bc_action(self, u, du=None):
         du = np.zeros(self.ndof) if du is None else du

@@ -622,6 +622,61 @@ static void div_2d(const orc_wave *W, const double *u, double *du)
     }
 }
 
+void orc_wave_div(const orc_wave *W, const double *u, double *du) { div_2d(W, u, du); }
+
+/* DivF2D<true>::action (DivF.hpp:207-274) for the linear flux functor
+ * f(x, u) = (A u, B u): per node the physical flux Fq(d, 0) = (A u)_d,
+ * Fq(d, 1) = (B u)_d, the contravariant fluxes F(0, i, d, j) = Y_eta Fq(d, 0)
+ * - X_eta Fq(d, 1) and F(1, j, d, i) = -Y_xi Fq(d, 0) + X_xi Fq(d, 1) (times
+ * w_i w_j), then divF(d, i, j) += sum_l D(l, i) F(0, l, d, j) + D(l, j) F(1, l, d, i).
+ * A and B are (nvar, nvar) column-major; u, divF are (nvar, n, n, nel). */
+void orc_divf_linear(const orc_wave *W, int nvar, const double *A, const double *B, const double *u, double *divF)
+{
+    const int n = W->n, nn = n * n, nel = W->m->nel;
+    const double *D = W->D, *w = W->w, *x = W->x;
+#pragma omp parallel
+    {
+        double *F = (double *)malloc(sizeof(double) * 2 * (size_t)nn * nvar);
+        double Fq[2 * 16];
+#pragma omp for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const double *c = W->m->corners + 8 * (size_t)el;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double J[4];
+                    quad_jacobian(c, x[i], x[j], J);
+                    const double w_ij = w[i] * w[j];
+                    const double Y_eta = J[3] * w_ij, X_eta = J[2] * w_ij;
+                    const double Y_xi = J[1] * w_ij, X_xi = J[0] * w_ij;
+                    const double *Uq = u + (size_t)nvar * (i + n * (j + (size_t)n * el));
+                    for (int d = 0; d < nvar; ++d) {
+                        double fx = 0.0, fy = 0.0;
+                        for (int e = 0; e < nvar; ++e) {
+                            fx += A[d + nvar * e] * Uq[e];
+                            fy += B[d + nvar * e] * Uq[e];
+                        }
+                        Fq[d] = fx;
+                        Fq[nvar + d] = fy;
+                    }
+                    for (int d = 0; d < nvar; ++d) {
+                        F[0 + 2 * (i + n * (d + nvar * j))] = Y_eta * Fq[d] - X_eta * Fq[nvar + d];
+                        F[1 + 2 * (j + n * (d + nvar * i))] = -Y_xi * Fq[d] + X_xi * Fq[nvar + d];
+                    }
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int d = 0; d < nvar; ++d) {
+                        double dF = 0.0;
+                        for (int l = 0; l < n; ++l)
+                            dF += D[l + n * i] * F[0 + 2 * (l + n * (d + nvar * j))] +
+                                  D[l + n * j] * F[1 + 2 * (l + n * (d + nvar * i))];
+                        divF[d + nvar * (i + n * (j + (size_t)n * el))] += dF;
+                    }
+        }
+        free(F);
+    }
+}
+
 /* FaceProlongator.cpp:565-590 (lobatto_action_2d) */
 static void prolong(const orc_wave *W, int type, const double *u, double *uf)
 {

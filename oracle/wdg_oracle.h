@@ -54,6 +54,10 @@ void orc_wave_free(orc_wave *W);
 long long orc_wave_ndof(const orc_wave *W);
 void orc_wave_action(const orc_wave *W, const double *u, double *du);   /* du += A u   */
 void orc_wavebc_action(const orc_wave *W, const double *u, double *du); /* du += B u   */
+/* Div<true>::action alone (volume term of WaveEquation::action). */
+void orc_wave_div(const orc_wave *W, const double *u, double *du);
+/* DivF2D<true>::action with the linear flux f(x, u) = (A u, B u), nvar <= 16. */
+void orc_divf_linear(const orc_wave *W, int nvar, const double *A, const double *B, const double *u, double *divF);
 void orc_mass_action(const orc_wave *W, int nvar, const double *x, double *y);
 void orc_mass_inv(const orc_wave *W, int nvar, double *x);
 /* vt = M^{-1} (A u + B u + F s); F may be NULL. */

@@ -79,6 +79,7 @@ _SIGS = {
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
     "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
+    "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
     "wdg_hip_schedule_count": [_ip, _ip],
     "wdg_hip_wave2d_set_schedule": [_vp, _i],
     "wdg_hip_wave2d_get_schedule": [_vp, _ip, _ip],
@@ -297,6 +298,13 @@ class Wave2D:
         ms, nk = _d(), _i()
         _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
         return ms.value, nk.value
+
+    def divf_linear(self, n_var: int, A, B, u, out, stream=None) -> None:
+        """out += DivF2D(u) for the flux (A u, B u); A, B as (n_var, n_var) arrays."""
+        Ac = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(n_var, n_var).T.reshape(-1))
+        Bc = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(n_var, n_var).T.reshape(-1))
+        _call("wdg_hip_divf2d_linear", self.h, n_var, _dptr(Ac), _dptr(Bc), dev_ptr(u), dev_ptr(out),
+              stream if stream is not None else cur_stream())
 
     def set_schedule(self, k: int) -> None:
         _call("wdg_hip_wave2d_set_schedule", self.h, k)

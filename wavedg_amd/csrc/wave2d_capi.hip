@@ -1190,6 +1190,30 @@ int wdg_hip_convert(const void* d_src, int src_precision, void* d_dst, int dst_p
 }
 
 
+int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const double* B, const void* d_u, void* d_divF,
+                          void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (!A || !B || !d_u || !d_divF) fail(WDG_ERR_INVALID, "null argument");
+        wdg::DivFArgs a{};
+        a.geom = op->d_geom;
+        a.basis = op->d_basis;
+        a.u = d_u;
+        a.out = d_divF;
+        a.nel = op->nel;
+        a.nv = n_var;
+        for (int c = 0; c < n_var; ++c)
+            for (int r = 0; r < n_var; ++r) {
+                a.A[r + wdg::DivFArgs::kMaxVar * c] = A[r + n_var * c];
+                a.B[r + wdg::DivFArgs::kMaxVar * c] = B[r + n_var * c];
+            }
+        wdg::launch_divf_linear(op->n, op->prec == WDG_F32, a, as_stream(stream));
+        hip_check(hipGetLastError(), "divf kernel launch");
+    });
+}
+
 int wdg_hip_schedule_count(int* count, int* threads)
 {
     return guarded([&] {

@@ -612,6 +612,19 @@ __global__ __launch_bounds__(Shape<N>::NT) void load_vector_kernel(const double*
     }
 }
 
+// DivF2D<true> with the linear flux f(x, u) = (A u, B u) (divf.hip): A, B
+// are (kMaxVar, kMaxVar) column-major with the leading (nv, nv) block used.
+struct DivFArgs {
+    static constexpr int kMaxVar = 8;
+    const double* geom;   // (6, nel) bilinear map coefficients
+    const double* basis;  // x[N], w[N], D[N*N]
+    const void* u;        // (nv, n, n, nel), handle precision
+    void* out;            // divF += ...
+    int nel, nv;
+    double A[kMaxVar * kMaxVar], B[kMaxVar * kMaxVar];
+};
+void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
+
 // Host-side launchers, one translation unit per N (wave2d_kernels_n.hip).
 #define WDG_DECLARE_LAUNCH(NV)                                                                              \
     void launch_wave_n##NV(int kid, int f32, int sched, const KArgs& a, hipStream_t s);                              \
