@@ -245,6 +245,23 @@ int wdg_hip_batch_timing(wdg_batch b, double *seconds, long long *steps);
 int wdg_hip_load_vector_gauss(wdg_wave2d op, int n_sources, const double *sources, int var, double *d_F,
                               void *stream);
 
+/* ---- files ------------------------------------------------------------
+ * The examples' to_file (examples/examples.hpp:14-19): n values of a device
+ * array written as raw native-endian float64 (FP32 data converted), the
+ * format of solution/x.00000, solution/u*.00000 (examples/wave2d.cpp:191-204,
+ * examples/waveholtz2d.cpp:194-197).  from_file reads such a file (its size
+ * must be n values) into a device array of the given precision. */
+int wdg_hip_to_file(const char *path, const void *d_x, long long n, int precision);
+int wdg_hip_from_file(const char *path, void *d_x, long long n, int precision);
+/* WaveHoltz checkpoint (no counterpart in the reference): the iterate W as
+ * a to_file record at `path` and a text record `path`.meta holding the
+ * order, element count, ndof, omega, iteration count and residual.  resume
+ * checks order, mesh size and omega against the handle and loads W; passing
+ * it to wdg_hip_waveholtz_solve continues the iteration where it stopped. */
+int wdg_hip_waveholtz_checkpoint(wdg_waveholtz wh, const char *path, const double *d_W, int iterations,
+                                 double residual);
+int wdg_hip_waveholtz_resume(wdg_waveholtz wh, const char *path, double *d_W, int *iterations, double *residual);
+
 /* ---- reductions (linalg.cpp:102-130), double, deterministic ---------- */
 int wdg_hip_norm(const double *d_x, long long n, double *out, void *stream);
 int wdg_hip_error(const double *d_x, const double *d_y, long long n, double *out, void *stream);

@@ -1,0 +1,76 @@
+"""Raw float64 files (the reference examples' to_file format) and WaveHoltz
+checkpoint/resume: a solve stopped, written, read back by a fresh handle and
+continued must end bitwise where the uninterrupted solve ends."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from tests.cases import bc_codes, mesh_case
+from wavedg_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def test_to_file_from_file_round_trip(cuda, tmp_path):
+    torch = cuda
+    x = torch.randn(12345, dtype=torch.float64, device="cuda")
+    p = str(tmp_path / "u.00000")
+    capi.to_file(p, x)
+    assert os.path.getsize(p) == 8 * 12345
+    np.testing.assert_array_equal(np.fromfile(p, dtype=np.float64), x.cpu().numpy())
+    y = torch.empty_like(x)
+    capi.from_file(p, y)
+    assert torch.equal(x, y)
+    # FP32 data is written as float64 and read back rounded to float32
+    x32 = x.float()
+    capi.to_file(p, x32)
+    np.testing.assert_array_equal(np.fromfile(p, dtype=np.float64), x32.double().cpu().numpy())
+    y32 = torch.empty_like(x32)
+    capi.from_file(p, y32)
+    assert torch.equal(x32, y32)
+    with pytest.raises(capi.WdgError):
+        capi.from_file(p, torch.empty(100, dtype=torch.float64, device="cuda"))
+
+
+def _setup(torch, v, e, n, omega):
+    m = capi.Mesh(v, e)
+    w = capi.Wave2D(m, n, bc_codes(m.n_boundary, "absorb"))
+    wh = capi.WaveHoltz(w, omega)
+    F = torch.empty(w.ndof, dtype=torch.float64, device="cuda")
+    w.load_vector_gauss([[0.3, 0.4, 0.06, 1.0]], F)
+    pi0 = torch.zeros_like(F)
+    wh.pi0(pi0, F)
+    return w, wh, pi0
+
+
+def test_waveholtz_checkpoint_resume_bitwise(cuda, tmp_path):
+    torch = cuda
+    v, e = mesh_case("lshape8")
+    n, omega, tol = 4, 5.0 * np.pi, 1e-8
+    w, wh, pi0 = _setup(torch, v, e, n, omega)
+    pn = capi.norm(pi0)
+    W_full = torch.zeros_like(pi0)
+    it_full, _, res_full = wh.solve(W_full, pi0, pn, tol, 500)
+    print(f"uninterrupted: {it_full} iterations, residual {res_full:.2e}", flush=True)
+    assert it_full > 6
+    W = torch.zeros_like(pi0)
+    it5, _, res5 = wh.solve(W, pi0, pn, tol, 5)
+    assert it5 == 5
+    p = str(tmp_path / "W.ckpt")
+    wh.checkpoint(p, W, it5, res5)
+    del w, wh
+    # a fresh handle picks up the iteration
+    w2, wh2, pi0b = _setup(torch, v, e, n, omega)
+    W2 = torch.empty_like(pi0b)
+    it_r, res_r = wh2.resume(p, W2)
+    assert it_r == 5 and res_r == res5 and torch.equal(W2, W)
+    it_rest, _, res_end = wh2.solve(W2, pi0b, capi.norm(pi0b), tol, 500)
+    assert it_r + it_rest == it_full and res_end == res_full
+    assert torch.equal(W2, W_full)
+    # a checkpoint of another problem is refused
+    w3, wh3, _ = _setup(torch, v, e, n, 6.0 * np.pi)
+    with pytest.raises(capi.WdgError):
+        wh3.resume(p, torch.empty_like(W2))

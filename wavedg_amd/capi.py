@@ -79,6 +79,10 @@ _SIGS = {
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
     "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
+    "wdg_hip_to_file": [ctypes.c_char_p, _vp, _ll, _i],
+    "wdg_hip_from_file": [ctypes.c_char_p, _vp, _ll, _i],
+    "wdg_hip_waveholtz_checkpoint": [_vp, ctypes.c_char_p, _vp, _i, _d],
+    "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
     "wdg_hip_schedule_count": [_ip, _ip],
     "wdg_hip_wave2d_set_schedule": [_vp, _i],
@@ -368,9 +372,31 @@ class WaveHoltz:
               pi0_norm, tol, max_iter, it, ctypes.byref(r), stream if stream is not None else cur_stream())
         return it[0], it[1], r.value
 
+    def checkpoint(self, path: str, W, iterations: int, residual: float) -> None:
+        _call("wdg_hip_waveholtz_checkpoint", self.h, path.encode(), dev_ptr(W), iterations, residual)
+
+    def resume(self, path: str, W):
+        """Loads a checkpoint into W; returns (iterations, residual) recorded with it."""
+        it, r = _i(), _d()
+        _call("wdg_hip_waveholtz_resume", self.h, path.encode(), dev_ptr(W), ctypes.byref(it), ctypes.byref(r))
+        return it.value, r.value
+
     def postprocess(self, H, w, stream=None):
         _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),
               stream if stream is not None else cur_stream())
+
+
+def to_file(path: str, x) -> None:
+    """Raw float64 file of a device tensor (float32 tensors converted)."""
+    import torch
+
+    _call("wdg_hip_to_file", path.encode(), dev_ptr(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
+
+
+def from_file(path: str, x) -> None:
+    import torch
+
+    _call("wdg_hip_from_file", path.encode(), dev_ptr(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
 
 
 def schedule_count():

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <fstream>
@@ -1211,6 +1212,126 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const doubl
             }
         wdg::launch_divf_linear(op->n, op->prec == WDG_F32, a, as_stream(stream));
         hip_check(hipGetLastError(), "divf kernel launch");
+    });
+}
+
+// ------------------------------------------------------------ file formats
+int wdg_hip_to_file(const char* path, const void* d_x, long long n, int precision)
+{
+    return guarded([&] {
+        if (!path || (n > 0 && !d_x) || n < 0) fail(WDG_ERR_INVALID, "bad arguments");
+        if (precision != WDG_F64 && precision != WDG_F32) fail(WDG_ERR_INVALID, "bad precision");
+        FILE* f = std::fopen(path, "wb");
+        if (!f) fail(WDG_ERR_INVALID, std::string("cannot open ") + path + " for writing");
+        const long long chunk = 1LL << 22;  // values per host staging buffer
+        std::vector<double> buf((size_t)std::min(std::max(n, 1LL), chunk));
+        std::vector<float> fbuf(precision == WDG_F32 ? buf.size() : 0);
+        bool ok = true;
+        for (long long o = 0; o < n && ok; o += chunk) {
+            const long long m = std::min(chunk, n - o);
+            hipError_t e;
+            if (precision == WDG_F32) {
+                e = hipMemcpy(fbuf.data(), static_cast<const float*>(d_x) + o, sizeof(float) * m, hipMemcpyDeviceToHost);
+                for (long long k = 0; k < m; ++k) buf[k] = fbuf[k];
+            } else {
+                e = hipMemcpy(buf.data(), static_cast<const double*>(d_x) + o, sizeof(double) * m,
+                              hipMemcpyDeviceToHost);
+            }
+            if (e != hipSuccess) {
+                std::fclose(f);
+                hip_check(e, "copy to host");
+            }
+            ok = std::fwrite(buf.data(), sizeof(double), (size_t)m, f) == (size_t)m;
+        }
+        if (std::fclose(f) != 0 || !ok) fail(WDG_ERR_INVALID, std::string("write failed: ") + path);
+    });
+}
+
+int wdg_hip_from_file(const char* path, void* d_x, long long n, int precision)
+{
+    return guarded([&] {
+        if (!path || (n > 0 && !d_x) || n < 0) fail(WDG_ERR_INVALID, "bad arguments");
+        if (precision != WDG_F64 && precision != WDG_F32) fail(WDG_ERR_INVALID, "bad precision");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) fail(WDG_ERR_INVALID, std::string("cannot open ") + path);
+        std::fseek(f, 0, SEEK_END);
+        const long long bytes = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        if (bytes != n * (long long)sizeof(double)) {
+            std::fclose(f);
+            fail(WDG_ERR_INVALID, std::string(path) + " holds " + std::to_string(bytes / 8) + " values, expected " +
+                                      std::to_string(n));
+        }
+        const long long chunk = 1LL << 22;
+        std::vector<double> buf((size_t)std::min(std::max(n, 1LL), chunk));
+        std::vector<float> fbuf(precision == WDG_F32 ? buf.size() : 0);
+        for (long long o = 0; o < n; o += chunk) {
+            const long long m = std::min(chunk, n - o);
+            if (std::fread(buf.data(), sizeof(double), (size_t)m, f) != (size_t)m) {
+                std::fclose(f);
+                fail(WDG_ERR_INVALID, std::string("read failed: ") + path);
+            }
+            hipError_t e;
+            if (precision == WDG_F32) {
+                for (long long k = 0; k < m; ++k) fbuf[k] = (float)buf[k];
+                e = hipMemcpy(static_cast<float*>(d_x) + o, fbuf.data(), sizeof(float) * m, hipMemcpyHostToDevice);
+            } else {
+                e = hipMemcpy(static_cast<double*>(d_x) + o, buf.data(), sizeof(double) * m, hipMemcpyHostToDevice);
+            }
+            if (e != hipSuccess) {
+                std::fclose(f);
+                hip_check(e, "copy to device");
+            }
+        }
+        std::fclose(f);
+    });
+}
+
+int wdg_hip_waveholtz_checkpoint(wdg_waveholtz wh, const char* path, const double* d_W, int iterations,
+                                 double residual)
+{
+    return guarded([&] {
+        if (!wh || !path || !d_W) fail(WDG_ERR_INVALID, "null argument");
+        const wdg_wave2d op = wh->op;
+        const int rc = wdg_hip_to_file(path, d_W, op->ndof, WDG_F64);
+        if (rc) fail(rc, g_last_error);
+        const std::string meta = std::string(path) + ".meta";
+        FILE* f = std::fopen(meta.c_str(), "w");
+        if (!f) fail(WDG_ERR_INVALID, "cannot open " + meta + " for writing");
+        std::fprintf(f, "waveholtz-checkpoint 1\nn_colloc %d\nn_elem %d\nndof %lld\nomega %.17g\n"
+                        "iterations %d\nresidual %.17g\n",
+                     op->n, op->nel, op->ndof, wh->omega, iterations, residual);
+        if (std::fclose(f) != 0) fail(WDG_ERR_INVALID, "write failed: " + meta);
+    });
+}
+
+int wdg_hip_waveholtz_resume(wdg_waveholtz wh, const char* path, double* d_W, int* iterations, double* residual)
+{
+    return guarded([&] {
+        if (!wh || !path || !d_W) fail(WDG_ERR_INVALID, "null argument");
+        const wdg_wave2d op = wh->op;
+        const std::string meta = std::string(path) + ".meta";
+        std::ifstream f(meta);
+        if (!f) fail(WDG_ERR_INVALID, "cannot open " + meta);
+        std::map<std::string, std::string> kv;
+        std::string k, v;
+        while (f >> k >> v) kv[k] = v;
+        auto need = [&](const char* key) {
+            auto it = kv.find(key);
+            if (it == kv.end()) fail(WDG_ERR_INVALID, meta + ": missing " + key);
+            return it->second;
+        };
+        if (need("waveholtz-checkpoint") != "1") fail(WDG_ERR_INVALID, meta + ": unknown version");
+        if (std::stoi(need("n_colloc")) != op->n || std::stoi(need("n_elem")) != op->nel ||
+            std::stoll(need("ndof")) != op->ndof)
+            fail(WDG_ERR_INVALID, meta + ": mesh or order differs from the handle");
+        const double om = std::stod(need("omega"));
+        if (std::fabs(om - wh->omega) > 1e-12 * std::fabs(wh->omega))
+            fail(WDG_ERR_INVALID, meta + ": omega differs from the handle");
+        const int rc = wdg_hip_from_file(path, d_W, op->ndof, WDG_F64);
+        if (rc) fail(rc, g_last_error);
+        if (iterations) *iterations = std::stoi(need("iterations"));
+        if (residual) *residual = std::stod(need("residual"));
     });
 }
 

@@ -391,6 +391,18 @@ public:
         check(wdg_hip_waveholtz_iterate(wh_.get(), d_w, d_pi0, pi0_norm, &r, stream));
         return r;
     }
+    // Checkpoint of a device iterate (raw float64 + a .meta record) and its
+    // resumption; returns the recorded iteration count.
+    void checkpoint_device(const std::string& path, const double* d_W, int iterations, double residual) const
+    {
+        check(wdg_hip_waveholtz_checkpoint(wh_.get(), path.c_str(), d_W, iterations, residual));
+    }
+    int resume_device(const std::string& path, double* d_W, double* residual = nullptr) const
+    {
+        int it = 0;
+        check(wdg_hip_waveholtz_resume(wh_.get(), path.c_str(), d_W, &it, residual));
+        return it;
+    }
     wdg_wave2d wave_handle() const { return op_.get(); }
 
 private:
@@ -431,6 +443,13 @@ private:
 };
 
 // norm / error (linalg.cpp:102-130) of device arrays, in double.
+// The examples' to_file (examples/examples.hpp:14-19) for a device array:
+// raw native-endian float64.
+inline void to_file_device(const std::string& fname, long long n, const double* d_u)
+{
+    check(wdg_hip_to_file(fname.c_str(), d_u, n, WDG_F64));
+}
+
 inline double norm_device(long long n, const double* d_x, void* stream = nullptr)
 {
     double r = 0;
