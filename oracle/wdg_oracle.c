@@ -409,6 +409,14 @@ void orc_mesh_detj(const orc_mesh *m, int nq, const double *xq, double *out)
 static const double WAVE_A[18] = {0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0,
                                   0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0};
 
+/* approx_quad = false tables (exact-quadrature operators) */
+struct orc_exact {
+    int nqv, nqe, nqm;
+    double xv[16], wv[16], Pv[16 * 16], Dv[16 * 16]; /* Div<false> rule; P(i, k) = phi_k(xq_i), (nq, n) */
+    double xe[16], we[16], Pe[16 * 16];              /* EdgeFlux<false> rule */
+    double *L;                                       /* MassMatrix<false> Cholesky factors (n^2, n^2, nel) */
+};
+
 struct orc_wave {
     const orc_mesh *m;
     int n;
@@ -418,6 +426,7 @@ struct orc_wave {
     int *v2eI, *v2eB; /* (n, 2, ne) */
     double *FsI, *FsB; /* per edge and side: (3, 3, 2, ne) = Fs(c, d) */
     double *uI, *uB;   /* face vectors (n, 3, 2, ne), FEMVector.cpp:81-87 */
+    struct orc_exact *ex; /* approx_quad = false tables, or NULL */
 };
 
 /* Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (column-major):
@@ -566,6 +575,8 @@ void orc_wave_free(orc_wave *W)
     free(W->FsB);
     free(W->uI);
     free(W->uB);
+    if (W->ex) free(W->ex->L);
+    free(W->ex);
     free(W);
 }
 
@@ -622,7 +633,14 @@ static void div_2d(const orc_wave *W, const double *u, double *du)
     }
 }
 
-void orc_wave_div(const orc_wave *W, const double *u, double *du) { div_2d(W, u, du); }
+static void div_2d_q(const orc_wave *W, const double *u, double *du);
+void orc_wave_div(const orc_wave *W, const double *u, double *du)
+{
+    if (W->ex)
+        div_2d_q(W, u, du);
+    else
+        div_2d(W, u, du);
+}
 
 /* DivF2D<true>::action (DivF.hpp:207-274) for the linear flux functor
  * f(x, u) = (A u, B u): per node the physical flux Fq(d, 0) = (A u)_d,
@@ -734,9 +752,237 @@ static void lift(const orc_wave *W, int type, const double *uf, double *du)
                 }
 }
 
+/* ===================================================================== */
+/* exact quadrature (approx_quad = false)                                 */
+/* ===================================================================== */
+
+/* linalg.cpp:5-30 (chol): lower-triangular factor in place, column-major. */
+static int chol(int m, double *a)
+{
+    for (int k = 0; k < m; ++k) {
+        for (int j = k + 1; j < m; ++j) {
+            const double s = a[j + m * k] / a[k + m * k];
+            for (int i = j; i < m; ++i) a[i + m * j] -= s * a[i + m * k];
+        }
+        if (a[k + m * k] == 0) return 0;
+        const double s = 1.0 / sqrt(a[k + m * k]);
+        for (int i = k; i < m; ++i) a[i + m * k] *= s;
+    }
+    return 1;
+}
+
+/* linalg.cpp:32-66 (solve_chol): x (n, m) <- (L L')^{-1} x. */
+static void solve_chol(int m, const double *a, int n, double *x)
+{
+    for (int j = 0; j < m; ++j) {
+        for (int d = 0; d < n; ++d) x[d + n * j] /= a[j + m * j];
+        for (int i = j + 1; i < m; ++i)
+            for (int d = 0; d < n; ++d) x[d + n * i] -= a[i + m * j] * x[d + n * j];
+    }
+    for (int j = m - 1; j >= 0; --j) {
+        for (int d = 0; d < n; ++d) x[d + n * j] /= a[j + m * j];
+        for (int i = 0; i < j; ++i)
+            for (int d = 0; d < n; ++d) x[d + n * i] -= a[j + m * i] * x[d + n * j];
+    }
+}
+
+/* linalg.cpp:68-98 (mult_chol): x (n, m) <- L L' x. */
+static void mult_chol(int m, const double *a, int n, double *x)
+{
+    for (int i = 0; i < m; ++i)
+        for (int d = 0; d < n; ++d) {
+            double y = 0.0;
+            for (int j = i; j < m; ++j) y += a[j + m * i] * x[d + n * j];
+            x[d + n * i] = y;
+        }
+    for (int i = m - 1; i >= 0; --i)
+        for (int d = 0; d < n; ++d) {
+            double y = 0.0;
+            for (int j = 0; j <= i; ++j) y += a[i + m * j] * x[d + n * j];
+            x[d + n * i] = y;
+        }
+}
+
+/* Div.cpp:469-588 (div_2d_q) with the constant-coefficient op of
+ * Div.cpp:198-240 (op_2d_cc) evaluated per quadrature point. */
+static void div_2d_q(const orc_wave *W, const double *u, double *du)
+{
+    const struct orc_exact *X = W->ex;
+    const int n = W->n, nq = X->nqv, nel = W->m->nel;
+    const double *P = X->Pv, *D = X->Dv, *w = X->wv;
+#pragma omp parallel
+    {
+        double Pu[16 * 3 * 16], Uq[3 * 16 * 16], Fq[2 * 3 * 16 * 16], Df[16 * 3 * 16], Pg[16 * 3 * 16];
+#pragma omp for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const double *c = W->m->corners + 8 * (size_t)el;
+            const double *ue = u + 3 * (size_t)n * n * el;
+            /* Pu(l, d, i) = sum_k Pt(k, i) u(d, k, l) */
+            for (int i = 0; i < nq; ++i)
+                for (int l = 0; l < n; ++l)
+                    for (int d = 0; d < 3; ++d) {
+                        double pu = 0.0;
+                        for (int k = 0; k < n; ++k) pu += P[i + nq * k] * ue[d + 3 * (k + n * l)];
+                        Pu[l + n * (d + 3 * i)] = pu;
+                    }
+            /* Uq(d, i, j) = sum_l Pt(l, j) Pu(l, d, i) */
+            for (int j = 0; j < nq; ++j)
+                for (int i = 0; i < nq; ++i)
+                    for (int d = 0; d < 3; ++d) {
+                        double uq = 0.0;
+                        for (int l = 0; l < n; ++l) uq += P[j + nq * l] * Pu[l + n * (d + 3 * i)];
+                        Uq[d + 3 * (i + nq * j)] = uq;
+                    }
+            /* contravariant flux with op(k, c, d, i, j) = W (Y_eta a(d,c,0) - X_eta a(d,c,1)), ... */
+            for (int j = 0; j < nq; ++j)
+                for (int i = 0; i < nq; ++i) {
+                    double J[4];
+                    quad_jacobian(c, X->xv[i], X->xv[j], J);
+                    const double Wt = w[i] * w[j];
+                    const double X_xi = J[0] * Wt, Y_xi = J[1] * Wt, X_eta = J[2] * Wt, Y_eta = J[3] * Wt;
+                    for (int d = 0; d < 3; ++d) {
+                        double Fd = 0.0, Gd = 0.0;
+                        for (int cc = 0; cc < 3; ++cc) {
+                            const double aq0 = WAVE_A[d + 3 * cc], aq1 = WAVE_A[9 + d + 3 * cc];
+                            const double op0 = Y_eta * aq0 - X_eta * aq1, op1 = -Y_xi * aq0 + X_xi * aq1;
+                            Fd += op0 * Uq[cc + 3 * (i + nq * j)];
+                            Gd += op1 * Uq[cc + 3 * (i + nq * j)];
+                        }
+                        Fq[0 + 2 * (d + 3 * (i + nq * j))] = Fd;
+                        Fq[1 + 2 * (d + 3 * (i + nq * j))] = Gd;
+                    }
+                }
+            /* Df(j, d, k) = sum_i D(i, k) F(i, j); Pg(j, d, k) = sum_i P(i, k) G(i, j) */
+            for (int j = 0; j < nq; ++j)
+                for (int k = 0; k < n; ++k)
+                    for (int d = 0; d < 3; ++d) {
+                        double df = 0.0, pg = 0.0;
+                        for (int i = 0; i < nq; ++i) {
+                            df += D[i + nq * k] * Fq[0 + 2 * (d + 3 * (i + nq * j))];
+                            pg += P[i + nq * k] * Fq[1 + 2 * (d + 3 * (i + nq * j))];
+                        }
+                        Df[j + nq * (d + 3 * k)] = df;
+                        Pg[j + nq * (d + 3 * k)] = pg;
+                    }
+            for (int l = 0; l < n; ++l)
+                for (int k = 0; k < n; ++k)
+                    for (int d = 0; d < 3; ++d) {
+                        double divF = 0.0;
+                        for (int j = 0; j < nq; ++j)
+                            divF += P[j + nq * l] * Df[j + nq * (d + 3 * k)] + D[j + nq * l] * Pg[j + nq * (d + 3 * k)];
+                        du[d + 3 * (k + n * (l + (size_t)n * el))] += divF;
+                    }
+        }
+    }
+}
+
+/* EdgeFlux.cpp:548-620 (flux_action_2d_q) with the constant-coefficient
+ * table of EdgeFlux.cpp:375-420 (F = W(i) ds Fs), on Lobatto face traces. */
+static void flux_apply_q(const orc_wave *W, int type, double *uf)
+{
+    const struct orc_exact *X = W->ex;
+    const int n = W->n, nq = X->nqe, ne = orc_mesh_n_edges(W->m, type);
+    const double *Fs = type == 0 ? W->FsI : W->FsB, *P = X->Pe;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        const double ds = edge_of(W->m, e, type)->meas;
+        double acc[16 * 3], Uq[3 * 16];
+        for (int k = 0; k < 3 * nq; ++k) acc[k] = 0.0;
+        for (int s = 0; s < 2; ++s) {
+            for (int i = 0; i < nq; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    double v = 0.0;
+                    for (int j = 0; j < n; ++j) v += uf[j + n * (d + 3 * (s + 2 * (size_t)e))] * P[i + nq * j];
+                    Uq[d + 3 * i] = v;
+                }
+            for (int i = 0; i < nq; ++i) {
+                const double wq = X->we[i] * ds;
+                for (int d = 0; d < 3; ++d) {
+                    double flx = 0.0;
+                    for (int cc = 0; cc < 3; ++cc)
+                        flx += (wq * Fs[cc + 3 * d + 9 * (s + 2 * (size_t)e)]) * Uq[cc + 3 * i];
+                    acc[i + nq * d] += flx;
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i) {
+                double flx = 0.0;
+                for (int j = 0; j < nq; ++j) flx += acc[j + nq * d] * P[j + nq * i];
+                uf[i + n * (d + 3 * (0 + 2 * (size_t)e))] = flx;
+                uf[i + n * (d + 3 * (1 + 2 * (size_t)e))] = -flx;
+            }
+    }
+}
+
+orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double a, double b, int nqv, int nqe,
+                                int nqm)
+{
+    /* default rules of Div<false>, EdgeFlux<false>, MassMatrix<false> for
+     * bilinear elements (Div.cpp:320-327, EdgeFlux.cpp:503-510, MassMatrix.cpp:92-97) */
+    if (nqv <= 0) nqv = 1 + ((n - 2) + (n - 1) + 1) / 2;
+    if (nqe <= 0) nqe = 1 + (2 * (n - 1) + 2) / 2;
+    if (nqm <= 0) nqm = 1 + (2 * (n - 1) + 2) / 2;
+    if (nqv > 10 || nqe > 10 || nqm > 10) return NULL;
+    orc_wave *W = orc_wave_create(m, n, bc, a, b);
+    if (!W) return NULL;
+    struct orc_exact *X = (struct orc_exact *)calloc(1, sizeof(struct orc_exact));
+    X->nqv = nqv;
+    X->nqe = nqe;
+    X->nqm = nqm;
+    orc_gauss_legendre(nqv, X->xv, X->wv);
+    orc_lagrange_basis(X->Pv, n, W->x, nqv, X->xv);
+    orc_lagrange_basis_deriv(X->Dv, n, W->x, nqv, X->xv);
+    orc_gauss_legendre(nqe, X->xe, X->we);
+    orc_lagrange_basis(X->Pe, n, W->x, nqe, X->xe);
+    /* MassMatrix.cpp:85-133: lower triangle of M = (B_i B_j w detJ) summed over
+     * the tensor rule, then Cholesky per element */
+    double xm[16], wm[16], B[16 * 16];
+    orc_gauss_legendre(nqm, xm, wm);
+    orc_lagrange_basis(B, n, W->x, nqm, xm);
+    const int c2d = n * n, q2d = nqm * nqm, nel = m->nel;
+    X->L = (double *)calloc((size_t)c2d * c2d * nel, sizeof(double));
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (int el = 0; el < nel; ++el) {
+        double *M = X->L + (size_t)c2d * c2d * el;
+        double detJ[16 * 16];
+        for (int p = 0; p < q2d; ++p) {
+            double J[4];
+            quad_jacobian(m->corners + 8 * (size_t)el, xm[p % nqm], xm[p / nqm], J);
+            detJ[p] = J[0] * J[3] - J[1] * J[2];
+        }
+        for (int j = 0; j < c2d; ++j)
+            for (int i = j; i < c2d; ++i) {
+                double mij = 0.0;
+                for (int p = 0; p < q2d; ++p) {
+                    const double bi = B[p % nqm + nqm * (i % n)] * B[p / nqm + nqm * (i / n)];
+                    const double bj = B[p % nqm + nqm * (j % n)] * B[p / nqm + nqm * (j / n)];
+                    const double w = detJ[p] * wm[p % nqm] * wm[p / nqm];
+                    mij += bi * bj * w;
+                }
+                M[i + c2d * j] = mij;
+            }
+        ok = ok && chol(c2d, M);
+    }
+    W->ex = X;
+    if (!ok) {
+        orc_wave_free(W);
+        return NULL;
+    }
+    return W;
+}
+
 /* WaveEquation.cpp:54-68 */
 void orc_wave_action(const orc_wave *W, const double *u, double *du)
 {
+    if (W->ex) {
+        div_2d_q(W, u, du);
+        prolong(W, 0, u, W->uI);
+        flux_apply_q(W, 0, W->uI);
+        lift(W, 0, W->uI, du);
+        return;
+    }
     div_2d(W, u, du);
     prolong(W, 0, u, W->uI);
     flux_apply(W, 0, W->uI);
@@ -787,7 +1033,10 @@ void orc_wavebc_action(const orc_wave *W, const double *u, double *du)
             for (int d = 0; d < 3; ++d) uB[i + n * (d + 3 * (1 + 2 * (size_t)e))] = v_ext[d];
         }
     }
-    flux_apply(W, 1, uB);
+    if (W->ex)
+        flux_apply_q(W, 1, uB);
+    else
+        flux_apply(W, 1, uB);
     lift(W, 1, uB, du);
 }
 
@@ -795,6 +1044,16 @@ void orc_wavebc_action(const orc_wave *W, const double *u, double *du)
 void orc_mass_action(const orc_wave *W, int nvar, const double *xv, double *y)
 {
     const int n = W->n, nel = W->m->nel;
+    if (W->ex) { /* MassMatrix.cpp:193-210 */
+        const int c2d = n * n;
+#pragma omp parallel for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const size_t o = (size_t)nvar * c2d * el;
+            for (int i = 0; i < nvar * c2d; ++i) y[o + i] = xv[o + i];
+            mult_chol(c2d, W->ex->L + (size_t)c2d * c2d * el, nvar, y + o);
+        }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int el = 0; el < nel; ++el)
         for (int j = 0; j < n; ++j)
@@ -812,6 +1071,13 @@ void orc_mass_action(const orc_wave *W, int nvar, const double *xv, double *y)
 void orc_mass_inv(const orc_wave *W, int nvar, double *xv)
 {
     const int n = W->n, nel = W->m->nel;
+    if (W->ex) { /* MassMatrix.cpp:212-227 */
+        const int c2d = n * n;
+#pragma omp parallel for schedule(static)
+        for (int el = 0; el < nel; ++el)
+            solve_chol(c2d, W->ex->L + (size_t)c2d * c2d * el, nvar, xv + (size_t)nvar * c2d * el);
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int el = 0; el < nel; ++el)
         for (int j = 0; j < n; ++j)
