@@ -104,6 +104,17 @@ int wdg_hip_gll(int n, double *x, double *w, double *D);
  * pair, EdgeFlux.hpp:51).  The mesh may be destroyed after this call. */
 int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int *bc, double a, double b, int precision,
                           wdg_wave2d *out);
+/* The same handle with approx_quad = false: Div<false> (Div.cpp:312-360,
+ * 469-588), EdgeFlux<false> (EdgeFlux.cpp:375-620) and MassMatrix<false>
+ * (MassMatrix.cpp:85-227: block mass matrix, Cholesky per element), FP64,
+ * with Gauss-Legendre rules of nq_volume, nq_edge and nq_mass points (<= 10;
+ * 0 = the reference's default for bilinear elements: n, n + 1, n + 1).  Every
+ * operator, RK and WaveHoltz entry point then uses these operators (DivF and
+ * the schedule autotuner are for approx_quad = true handles).  Memory: n^4
+ * doubles per element for the mass-matrix factors. */
+int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int *bc, double a, double b, int nq_volume,
+                                int nq_edge, int nq_mass, wdg_wave2d *out);
+int wdg_hip_wave2d_quadrature(wdg_wave2d op, int *exact, int *nq_volume, int *nq_edge, int *nq_mass);
 int wdg_hip_wave2d_destroy(wdg_wave2d op);
 /* ndof = 3 n^2 n_elem */
 int wdg_hip_wave2d_info(wdg_wave2d op, long long *ndof, int *n_colloc, int *n_elem, int *precision);

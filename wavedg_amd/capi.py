@@ -48,6 +48,8 @@ _SIGS = {
     "wdg_hip_mesh_nodes": [_vp, _i, _dp],
     "wdg_hip_gll": [_i, _dp, _dp, _dp],
     "wdg_hip_wave2d_create": [_vp, _i, _ip, _d, _d, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_wave2d_create_exact": [_vp, _i, _ip, _d, _d, _i, _i, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_wave2d_quadrature": [_vp, _ip, _ip, _ip, _ip],
     "wdg_hip_wave2d_destroy": [_vp],
     "wdg_hip_wave2d_info": [_vp, ctypes.POINTER(_ll), _ip, _ip, _ip],
     "wdg_hip_wave2d_action": [_vp, _vp, _vp, _vp],
@@ -241,7 +243,10 @@ class Mesh:
 class Wave2D:
     """WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on the GPU."""
 
-    def __init__(self, mesh: Mesh, n: int, bc=None, a: float = 1.0, b: float = 0.5, precision: int = F64):
+    def __init__(self, mesh: Mesh, n: int, bc=None, a: float = 1.0, b: float = 0.5, precision: int = F64,
+                 approx_quad: bool = True, quad=(0, 0, 0)):
+        """approx_quad=False: exact-quadrature operators (FP64) with Gauss-Legendre
+        rules quad = (volume, edge, mass) points, 0 = the reference default."""
         bcp = None
         if bc is not None:
             self._bc = np.ascontiguousarray(np.asarray(bc, dtype=np.int32))
@@ -249,7 +254,12 @@ class Wave2D:
                 raise ValueError("bc must have one code per boundary edge")
             bcp = _iptr(self._bc)
         h = _vp()
-        _call("wdg_hip_wave2d_create", mesh.h, n, bcp, a, b, precision, ctypes.byref(h))
+        if approx_quad:
+            _call("wdg_hip_wave2d_create", mesh.h, n, bcp, a, b, precision, ctypes.byref(h))
+        else:
+            if precision != F64:
+                raise ValueError("exact-quadrature operators run in FP64")
+            _call("wdg_hip_wave2d_create_exact", mesh.h, n, bcp, a, b, *quad, ctypes.byref(h))
         self.h = h
         nd, nc, ne, pr = _ll(), _i(), _i(), _i()
         _call("wdg_hip_wave2d_info", self.h, ctypes.byref(nd), ctypes.byref(nc), ctypes.byref(ne), ctypes.byref(pr))
@@ -309,6 +319,12 @@ class Wave2D:
         Bc = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(n_var, n_var).T.reshape(-1))
         _call("wdg_hip_divf2d_linear", self.h, n_var, _dptr(Ac), _dptr(Bc), dev_ptr(u), dev_ptr(out),
               stream if stream is not None else cur_stream())
+
+    def quadrature(self):
+        """(exact, volume, edge, mass rule sizes)."""
+        e, v, ed, m = _i(), _i(), _i(), _i()
+        _call("wdg_hip_wave2d_quadrature", self.h, ctypes.byref(e), ctypes.byref(v), ctypes.byref(ed), ctypes.byref(m))
+        return bool(e.value), v.value, ed.value, m.value
 
     def set_schedule(self, k: int) -> None:
         _call("wdg_hip_wave2d_set_schedule", self.h, k)

@@ -140,6 +140,17 @@ struct wdg_wave2d_s {
     std::map<std::tuple<void*, int, double>, hipGraphExec_t> graphs;
     int sched = 0;     // workgroup schedule of the RK stage kernels (wdg::kSchedTgt)
     std::string arch;  // device architecture name, key of the tuned-schedule table
+    // approx_quad = false: exact-quadrature operators (FP64)
+    bool exact = false;
+    int nqv = 0, nqe = 0, nqm = 0;
+    double* d_extab = nullptr;  // xv, wv, Pv, Dv | xe, we, Pe | xm, wm, Bm (10 + 10 + 100 + 100 each block)
+    double* d_L = nullptr;      // Cholesky factors of the element mass matrices (n^4 per element)
+    void* d_exr = nullptr;      // operator output before M^{-1}
+    wdg::ExactTables tables() const
+    {
+        const double* t = d_extab;
+        return wdg::ExactTables{t, t + 10, t + 20, t + 120, t + 220, t + 230, t + 240, nqv, nqe};
+    }
 
     ~wdg_wave2d_s()
     {
@@ -153,6 +164,9 @@ struct wdg_wave2d_s {
         dfree(d_info);
         dfree(d_basis);
         dfree(d_lvtab);
+        dfree(d_extab);
+        dfree(d_L);
+        dfree(d_exr);
         dfree(d_tmp);
         dfree(d_du);
         dfree(d_r1);
@@ -225,6 +239,32 @@ void launch_wave(int n, int kid, int f32, int sched, const KArgs& a, hipStream_t
     default: fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
     }
     hip_check(hipGetLastError(), "wave kernel launch");
+}
+
+// One operator / RK-stage evaluation: the fused kernel, or for exact-
+// quadrature handles volume + face kernels into a scratch output and the
+// block mass-matrix kernel that finishes the right-hand side or stage.
+void apply_op(wdg_wave2d op, int kid, const KArgs& a, hipStream_t s)
+{
+    if (!op->exact) {
+        launch_wave(op->n, kid, op->prec == WDG_F32, op->sched, a, s);
+        return;
+    }
+    const wdg::ExactTables t = op->tables();
+    if (kid == wdg::K_ACTION) {
+        wdg::launch_exact_volume(op->n, a, t, 1, s);
+        wdg::launch_exact_face(op->n, a, t, 1, 0, s);
+    } else if (kid == wdg::K_BC) {
+        wdg::launch_exact_face(op->n, a, t, 0, 1, s);
+    } else {
+        op->ensure(op->d_exr, op->state_bytes());
+        KArgs r = a;
+        r.u_out = op->d_exr;
+        wdg::launch_exact_volume(op->n, r, t, 0, s);
+        wdg::launch_exact_face(op->n, r, t, 1, 1, s);
+        wdg::launch_exact_mass(op->n, a, static_cast<const double*>(op->d_exr), op->d_L, kid, s);
+    }
+    hip_check(hipGetLastError(), "exact-quadrature kernel launch");
 }
 
 // ------------------------------------------------------------ small kernels
@@ -407,7 +447,7 @@ void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const
                 const double ts = (st == 0) ? t : t + RK4C[st] * dt;
                 a.fs = -1.0 / omega * std::sin(omega * ts);  // WaveHoltz.cpp:65
             }
-            launch_wave(op->n, kid, f32, op->sched, a, s);
+            apply_op(op, kid, a, s);
             std::swap(cur, nxt);
         }
         t += dt;
@@ -774,6 +814,74 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
     });
 }
 
+int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int nq_volume,
+                                int nq_edge, int nq_mass, wdg_wave2d* out)
+{
+    if (!out) {
+        g_last_error = "null argument";
+        return WDG_ERR_INVALID;
+    }
+    *out = nullptr;
+    // the reference's default rules for bilinear elements (Div.cpp:320-327,
+    // EdgeFlux.cpp:503-510, MassMatrix.cpp:92-97)
+    if (nq_volume <= 0) nq_volume = 1 + ((n_colloc - 2) + (n_colloc - 1) + 1) / 2;
+    if (nq_edge <= 0) nq_edge = 1 + (2 * (n_colloc - 1) + 2) / 2;
+    if (nq_mass <= 0) nq_mass = 1 + (2 * (n_colloc - 1) + 2) / 2;
+    if (nq_volume > 10 || nq_edge > 10 || nq_mass > 10) {
+        g_last_error = "quadrature rules have at most 10 points";
+        return WDG_ERR_INVALID;
+    }
+    wdg_wave2d op = nullptr;
+    const int rc = wdg_hip_wave2d_create(mesh, n_colloc, bc, a, b, WDG_F64, &op);
+    if (rc) return rc;
+    const int rc2 = guarded([&] {
+        const int n = n_colloc;
+        // volume rule @0 (x, w, P, D), edge rule @220 (x, w, P), mass rule @340 (x, w, B)
+        std::vector<double> tab(460, 0.0);
+        auto rule = [&](int nq, int off, bool deriv) {
+            std::vector<double> xq, wq;
+            if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "no Gauss-Legendre rule of that size");
+            std::copy(xq.begin(), xq.end(), tab.begin() + off);
+            std::copy(wq.begin(), wq.end(), tab.begin() + off + 10);
+            wdg::lagrange_interpolation_matrix(n, op->x.data(), nq, xq.data(), tab.data() + off + 20);
+            if (deriv) wdg::lagrange_derivative_matrix(n, op->x.data(), nq, xq.data(), tab.data() + off + 120);
+        };
+        rule(nq_volume, 0, true);
+        rule(nq_edge, 220, false);
+        rule(nq_mass, 340, false);
+        op->exact = true;
+        op->nqv = nq_volume;
+        op->nqe = nq_edge;
+        op->nqm = nq_mass;
+        op->d_extab = static_cast<double*>(dmalloc(sizeof(double) * tab.size()));
+        hip_check(hipMemcpy(op->d_extab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice),
+                  "upload quadrature tables");
+        const size_t nn2 = (size_t)n * n * n * n;
+        op->d_L = static_cast<double*>(dmalloc(sizeof(double) * nn2 * op->nel));
+        const wdg::ExactMassTables m{op->d_extab + 340, op->d_extab + 350, op->d_extab + 360, nq_mass};
+        wdg::launch_exact_mass_setup(n, op->base_args(), m, op->d_L, nullptr);
+        hip_check(hipGetLastError(), "mass-matrix setup launch");
+        hip_check(hipDeviceSynchronize(), "mass-matrix setup");
+    });
+    if (rc2) {
+        wdg_hip_wave2d_destroy(op);
+        return rc2;
+    }
+    *out = op;
+    return WDG_OK;
+}
+
+int wdg_hip_wave2d_quadrature(wdg_wave2d op, int* exact, int* nq_volume, int* nq_edge, int* nq_mass)
+{
+    return guarded([&] {
+        check_op(op);
+        if (exact) *exact = op->exact ? 1 : 0;
+        if (nq_volume) *nq_volume = op->nqv;
+        if (nq_edge) *nq_edge = op->nqe;
+        if (nq_mass) *nq_mass = op->nqm;
+    });
+}
+
 int wdg_hip_wave2d_destroy(wdg_wave2d op)
 {
     if (op) {
@@ -807,7 +915,7 @@ static int op_launch(wdg_wave2d op, int kid, const void* x, void* y, const void*
         a.u_out = y;
         a.force = F;
         a.fs = s;
-        launch_wave(op->n, kid, op->prec == WDG_F32, op->sched, a, as_stream(stream));
+        apply_op(op, kid, a, as_stream(stream));
     });
 }
 
@@ -833,6 +941,14 @@ static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* str
         if (!x || !y) fail(WDG_ERR_INVALID, "null device pointer");
         const long long nn = op->ndof / 3;
         const hipStream_t s = as_stream(stream);
+        if (op->exact) {
+            KArgs a = op->base_args();
+            a.u_in = x;
+            a.u_out = y;
+            wdg::launch_exact_mass(op->n, a, nullptr, op->d_L, inv ? -2 : -1, s);
+            hip_check(hipGetLastError(), "mass kernel launch");
+            return;
+        }
         if (op->prec == WDG_F32)
             hipLaunchKernelGGL(k_mass<float>, dim3(grid_for(nn)), dim3(256), 0, s, static_cast<const float*>(x),
                                static_cast<float*>(y), op->d_corners, op->d_basis, op->n, nn, inv);
@@ -1089,7 +1205,7 @@ int wdg_hip_waveholtz_postprocess(wdg_waveholtz wh, double* d_H, const double* d
         KArgs a = op->base_args();
         a.u_in = x;
         a.u_out = op->d_r2;
-        launch_wave(op->n, wdg::K_RHS, op->prec == WDG_F32, op->sched, a, s);
+        apply_op(op, wdg::K_RHS, a, s);
         if (op->prec == WDG_F32)
             hipLaunchKernelGGL(k_post<float>, dim3(grid_for(n / 3)), dim3(256), 0, s, d_w,
                                static_cast<const float*>(op->d_r2), d_H, wh->omega, n / 3);
@@ -1197,6 +1313,7 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const doubl
     return guarded([&] {
         check_op(op);
         if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (op->exact) fail(WDG_ERR_INVALID, "DivF2D is implemented for approx_quad = true handles");
         if (!A || !B || !d_u || !d_divF) fail(WDG_ERR_INVALID, "null argument");
         wdg::DivFArgs a{};
         a.geom = op->d_geom;
@@ -1371,6 +1488,7 @@ int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int* schedule, double* ms)
 {
     return guarded([&] {
         check_op(op);
+        if (op->exact) fail(WDG_ERR_INVALID, "schedules apply to the approx_quad = true kernels only");
         if (reps < 1) fail(WDG_ERR_INVALID, "reps must be positive");
         const hipStream_t s = nullptr;
         const long long n = op->ndof;

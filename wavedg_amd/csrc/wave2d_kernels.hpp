@@ -625,6 +625,34 @@ struct DivFArgs {
 };
 void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 
+// Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
+// column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
+struct ExactTables {
+    const double* xv;  // volume rule (Div<false>)
+    const double* wv;
+    const double* Pv;
+    const double* Dv;
+    const double* xe;  // edge rule (EdgeFlux<false>)
+    const double* we;
+    const double* Pe;
+    int nqv, nqe;
+};
+struct ExactMassTables {  // mass-matrix rule (MassMatrix<false>)
+    const double* x;
+    const double* w;
+    const double* B;
+    int nq;
+};
+// a.u_out (+)= Div<false>(a.u_in)
+void launch_exact_volume(int n, const KArgs& a, const ExactTables& t, int accumulate, hipStream_t s);
+// a.u_out += lifted EdgeFlux<false> of interior and/or boundary sides
+void launch_exact_face(int n, const KArgs& a, const ExactTables& t, int interior, int boundary, hipStream_t s);
+// Cholesky factors of the exact element mass matrices into L (n^4 per element)
+void launch_exact_mass_setup(int n, const KArgs& a, const ExactMassTables& m, double* L, hipStream_t s);
+// kid -1: y = M x, -2: y = M^{-1} x; kid >= 0: the RHS / RK-stage epilogue of
+// that kernel id applied to the operator output r
+void launch_exact_mass(int n, const KArgs& a, const double* r, const double* L, int kid, hipStream_t s);
+
 // Host-side launchers, one translation unit per N (wave2d_kernels_n.hip).
 #define WDG_DECLARE_LAUNCH(NV)                                                                              \
     void launch_wave_n##NV(int kid, int f32, int sched, const KArgs& a, hipStream_t s);                              \
