@@ -93,6 +93,9 @@ int wdg_hip_mesh_nodes(wdg_mesh mesh, int n_colloc, double *out);
  * QuadratureRule::quadrature_rule(n, GaussLobatto) (QuadratureRule.cpp:32-97,164-184)
  * and lagrange_basis_deriv (lagrange_interpolation.cpp:33-50): x, w (n), D (n, n). */
 int wdg_hip_gll(int n, double *x, double *w, double *D);
+/* QuadratureRule::quadrature_rule(n, GaussLegendre) (QuadratureRule.cpp:99-148,164-184):
+ * the rule the approx_quad = false operators integrate with; x, w (n), n in 1..10. */
+int wdg_hip_gauss_legendre(int n, double *x, double *w);
 
 /* ---- wave operator ---------------------------------------------------
  * One handle holds WaveEquation (WaveEquation.cpp:5-30), WaveBC

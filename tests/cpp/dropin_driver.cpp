@@ -4,8 +4,10 @@
 //
 // usage: dropin_driver <out_dir> <mesh_dir>
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -29,7 +31,7 @@ int main(int argc, char** argv)
     try {
         // --- WaveHoltz, as examples/waveholtz2d.cpp:92-190 (smaller mesh)
         const int n_colloc = 5;
-        auto basis = QuadratureRule::quadrature_rule(n_colloc);
+        auto basis = QuadratureRule::quadrature_rule(n_colloc, QuadratureRule::GaussLobatto);
         Mesh2D mesh = Mesh2D::uniform_rect(8, -1.0, 1.0, 8, -1.0, 1.0);
         const int nB = mesh.n_edges(FaceType::BOUNDARY);
         std::vector<double> geo = mesh.edge_geometry(FaceType::BOUNDARY);
@@ -73,7 +75,7 @@ int main(int argc, char** argv)
 
         // --- wave2d time derivative as examples/wave2d.cpp:164-177, host arrays
         Mesh2D mesh2 = Mesh2D::from_file(mesh_dir);
-        auto basis2 = QuadratureRule::quadrature_rule(4);
+        auto basis2 = QuadratureRule::quadrature_rule(4, QuadratureRule::GaussLobatto);
         const int nB2 = mesh2.n_edges(FaceType::BOUNDARY);
         std::vector<int> bc2(nB2, 1);
         WaveEquation a(mesh2, basis2, true);
@@ -88,6 +90,49 @@ int main(int argc, char** argv)
         m.inv(3, dudt.data());
         to_file(dir + "/u2.bin", nd2, u.data());
         to_file(dir + "/dudt2.bin", nd2, dudt.data());
+
+        // --- the same with approx_quad = false, the reference's default
+        // (Div<false>, EdgeFlux<false>, MassMatrix<false>)
+        WaveEquation ax(mesh2, basis2);
+        WaveBC bx(mesh2, bc2.data(), basis2, false);
+        MassMatrix<false> mx(mesh2, basis2);
+        std::fill(dudt.begin(), dudt.end(), 0.0);
+        ax.action(u.data(), dudt.data());
+        bx.action(u.data(), dudt.data());
+        mx.inv(3, dudt.data());
+        to_file(dir + "/dudt2_exact.bin", nd2, dudt.data());
+        // a user-given 6-point Gauss-Legendre rule for the volume and edge
+        // integrals, and a 7-point rule for the mass matrix
+        auto q6 = QuadratureRule::quadrature_rule(6);
+        WaveEquation aq(mesh2, basis2, false, q6);
+        WaveBC bq(mesh2, bc2.data(), basis2, false, q6);
+        std::fill(dudt.begin(), dudt.end(), 0.0);
+        aq.action(u.data(), dudt.data());
+        bq.action(u.data(), dudt.data());
+        to_file(dir + "/AB2_q6.bin", nd2, dudt.data());
+        MassMatrix<false> mq(mesh2, basis2, QuadratureRule::quadrature_rule(7));
+        mq.action(u.data(), dudt.data());
+        to_file(dir + "/M2_q7.bin", nd2, dudt.data());
+        bool refused = false;
+        try {
+            WaveEquation bad(mesh2, basis2, false, QuadratureRule::quadrature_rule(6, QuadratureRule::GaussLobatto));
+        } catch (const std::invalid_argument&) {
+            refused = true;
+        }
+        if (!refused) {
+            std::fprintf(stderr, "a Gauss-Lobatto quad for approx_quad = false was accepted\n");
+            return 1;
+        }
+        // WaveHoltz with its default approx_quad = false: pi0 and two iterations
+        WaveHoltz WHx(omega, mesh, basis, bc.data());
+        std::vector<double> pi0x(ndof), Wx(ndof, 0.0);
+        WHx.pi0(pi0x.data(), F.data());
+        for (int it = 0; it < 2; ++it) {
+            WHx.S(Wx.data());
+            for (long long i = 0; i < ndof; ++i) Wx[i] += pi0x[i];
+        }
+        to_file(dir + "/pi0_exact.bin", ndof, pi0x.data());
+        to_file(dir + "/W_exact.bin", ndof, Wx.data());
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {
         std::fprintf(stderr, "dg::hip::Error %d: %s\n", e.code, e.what());

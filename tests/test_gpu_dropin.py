@@ -56,3 +56,19 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     W2 = O.Wave(O.Mesh(v2, e2), 4)
     u2 = _read(d, "u2.bin")
     assert rel_l2(_read(d, "dudt2.bin"), W2.rhs(u2)) < 1e-12
+    # approx_quad = false through the same classes: default rules, a 6-point
+    # Gauss-Legendre quad for volume and edges, a 7-point rule for the mass
+    Wx = O.Wave(O.Mesh(v2, e2), 4, approx_quad=False)
+    assert rel_l2(_read(d, "dudt2_exact.bin"), Wx.rhs(u2)) < 1e-12
+    Wq = O.Wave(O.Mesh(v2, e2), 4, approx_quad=False, quad=(6, 6, 0))
+    assert rel_l2(_read(d, "AB2_q6.bin"), Wq.action(u2) + Wq.bc_action(u2)) < 1e-12
+    Wm = O.Wave(O.Mesh(v2, e2), 4, approx_quad=False, quad=(0, 0, 7))
+    assert rel_l2(_read(d, "M2_q7.bin"), Wm.mass_action(u2)) < 1e-12
+    # WaveHoltz with the reference's default approx_quad = false
+    Hx = O.WaveHoltz(O.Wave(m, 5, bc, approx_quad=False), 10.0)
+    pi0x = Hx.pi0(F)
+    assert rel_l2(_read(d, "pi0_exact.bin"), pi0x) < 1e-12
+    Wk = np.zeros_like(pi0x)
+    for _ in range(2):
+        Wk = Hx.S(Wk) + pi0x
+    assert rel_l2(_read(d, "W_exact.bin"), Wk) < 1e-12

@@ -47,6 +47,7 @@ _SIGS = {
     "wdg_hip_mesh_min_h": [_vp, _dp],
     "wdg_hip_mesh_nodes": [_vp, _i, _dp],
     "wdg_hip_gll": [_i, _dp, _dp, _dp],
+    "wdg_hip_gauss_legendre": [_i, _dp, _dp],
     "wdg_hip_wave2d_create": [_vp, _i, _ip, _d, _d, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_create_exact": [_vp, _i, _ip, _d, _d, _i, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_quadrature": [_vp, _ip, _ip, _ip, _ip],
@@ -182,6 +183,13 @@ def gll(n: int):
     x, w, D = np.zeros(n), np.zeros(n), np.zeros(n * n)
     _call("wdg_hip_gll", n, _dptr(x), _dptr(w), _dptr(D))
     return x, w, D.reshape(n, n).T  # D[i, j] = phi_j'(x_i)
+
+
+def gauss_legendre(n: int):
+    """The n-point Gauss-Legendre rule the exact-quadrature operators use."""
+    x, w = np.zeros(n), np.zeros(n)
+    _call("wdg_hip_gauss_legendre", n, _dptr(x), _dptr(w))
+    return x, w
 
 
 class Mesh:

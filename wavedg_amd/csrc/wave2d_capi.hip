@@ -738,6 +738,16 @@ int wdg_hip_gll(int n, double* x, double* w, double* D)
     });
 }
 
+int wdg_hip_gauss_legendre(int n, double* x, double* w)
+{
+    return guarded([&] {
+        std::vector<double> xv, wv;
+        if (!wdg::gauss_legendre_rule(n, xv, wv)) fail(WDG_ERR_INVALID, "n out of range 1..10");
+        if (x) std::memcpy(x, xv.data(), sizeof(double) * n);
+        if (w) std::memcpy(w, wv.data(), sizeof(double) * n);
+    });
+}
+
 // ---------------------------------------------------------------- operator
 int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int precision,
                           wdg_wave2d* out)

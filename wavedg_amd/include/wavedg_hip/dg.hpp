@@ -13,8 +13,10 @@
 // pointers (hipMalloc'd) and an optional hipStream_t and do no copies; they
 // are the ones to use in time loops.
 //
-// Only approx_quad = true with a Gauss-Lobatto basis is implemented on the
-// GPU; asking for approx_quad = false throws.
+// Bases are Gauss-Lobatto.  approx_quad = true collocates the integrals on the
+// basis nodes (FP64 or FP32); approx_quad = false integrates them exactly with
+// Gauss-Legendre rules, chosen as the reference chooses them when quad is null
+// (FP64 only).
 #pragma once
 
 #include <cmath>
@@ -71,31 +73,38 @@ private:
     size_t n_ = 0;
 };
 
-// QuadratureRule::quadrature_rule(n, GaussLobatto) (QuadratureRule.cpp:164-184).
+// QuadratureRule::quadrature_rule(n, type) (QuadratureRule.hpp:77,
+// QuadratureRule.cpp:164-184): cached rules, Gauss-Legendre by default as in
+// the reference.  Gauss-Lobatto rules serve as bases; Gauss-Legendre rules as
+// the quad argument of the approx_quad = false operators.
 class QuadratureRule {
 public:
     enum QuadratureType { GaussLegendre, GaussLobatto };
-    static const QuadratureRule* quadrature_rule(int n, QuadratureType type = GaussLobatto)
+    static const QuadratureRule* quadrature_rule(int n, QuadratureType type = GaussLegendre)
     {
-        if (type != GaussLobatto) throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
         static std::unordered_map<int, std::unique_ptr<QuadratureRule>> cache;
-        auto it = cache.find(n);
-        if (it == cache.end()) it = cache.emplace(n, std::unique_ptr<QuadratureRule>(new QuadratureRule(n))).first;
+        const int key = 2 * n + (type == GaussLobatto ? 1 : 0);
+        auto it = cache.find(key);
+        if (it == cache.end())
+            it = cache.emplace(key, std::unique_ptr<QuadratureRule>(new QuadratureRule(n, type))).first;
         return it->second.get();
     }
     const int n;
     const double* x;
     const double* w;
-    const QuadratureType type = GaussLobatto;
+    const QuadratureType type;
 
 private:
-    explicit QuadratureRule(int n_) : n(n_), x_(n_), w_(n_), D_(n_ * n_)
+    QuadratureRule(int n_, QuadratureType t) : n(n_), x(nullptr), w(nullptr), type(t), x_(n_ > 0 ? n_ : 0), w_(x_.size())
     {
-        check(wdg_hip_gll(n_, x_.data(), w_.data(), D_.data()));
+        if (t == GaussLobatto)
+            check(wdg_hip_gll(n_, x_.data(), w_.data(), nullptr));
+        else
+            check(wdg_hip_gauss_legendre(n_, x_.data(), w_.data()));
         x = x_.data();
         w = w_.data();
     }
-    std::vector<double> x_, w_, D_;
+    std::vector<double> x_, w_;
 };
 
 // Mesh2D (Mesh2D.hpp:339-355, Mesh2D.cpp:5-163).
@@ -130,6 +139,8 @@ public:
     // element_metrics(basis).physical_coordinates(): (2, n, n, n_elem)
     std::vector<double> physical_coordinates(const QuadratureRule* basis) const
     {
+        if (basis->type != QuadratureRule::GaussLobatto)
+            throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
         std::vector<double> x(2 * (size_t)basis->n * basis->n * n_elem());
         check(wdg_hip_mesh_nodes(h_.get(), basis->n, x.data()));
         return x;
@@ -176,13 +187,37 @@ public:
 };
 
 namespace detail {
-// One device discretisation (geometry, connectivity, basis) per object.
+// Size of a user-given rule for the exact-quadrature operators; 0 (null rule)
+// lets the library choose as the reference does (Div.cpp:319-327,
+// EdgeFlux.cpp:503-513, MassMatrix.cpp:92-98).
+inline int exact_rule_size(const QuadratureRule* q)
+{
+    if (!q) return 0;
+    if (q->type != QuadratureRule::GaussLegendre)
+        throw std::invalid_argument("dg::hip: approx_quad = false integrates with Gauss-Legendre rules only");
+    return q->n;
+}
+
+// One device discretisation (geometry, connectivity, basis) per object.  With
+// approx_quad = false, quad sets the volume and edge rules (WaveEquation.cpp:
+// 25-29 hands the same rule to Div<false> and EdgeFlux<false>) and mass_quad
+// the MassMatrix<false> rule.
 class Wave2D {
 public:
-    Wave2D(const Mesh2D& mesh, const QuadratureRule* basis, const int* bc, double a, double b, Precision p)
+    Wave2D(const Mesh2D& mesh, const QuadratureRule* basis, const int* bc, double a, double b, Precision p,
+           bool approx_quad = true, const QuadratureRule* quad = nullptr, const QuadratureRule* mass_quad = nullptr)
     {
+        if (!basis) throw std::invalid_argument("dg::hip: null basis");
+        if (basis->type != QuadratureRule::GaussLobatto)
+            throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
         wdg_wave2d h = nullptr;
-        check(wdg_hip_wave2d_create(mesh.handle(), basis->n, bc, a, b, (int)p, &h));
+        if (approx_quad) {
+            check(wdg_hip_wave2d_create(mesh.handle(), basis->n, bc, a, b, (int)p, &h));
+        } else {
+            if (p != Precision::F64) throw std::invalid_argument("dg::hip: approx_quad = false runs in FP64 only");
+            const int nq = exact_rule_size(quad);
+            check(wdg_hip_wave2d_create_exact(mesh.handle(), basis->n, bc, a, b, nq, nq, exact_rule_size(mass_quad), &h));
+        }
         h_.reset(h);
         check(wdg_hip_wave2d_info(h, &ndof_, nullptr, nullptr, nullptr));
         rsz_ = p == Precision::F32 ? 4 : 8;
@@ -200,12 +235,6 @@ private:
     long long ndof_ = 0;
     size_t rsz_ = 8;
 };
-
-inline void require_approx(bool approx_quad)
-{
-    if (!approx_quad)
-        throw std::invalid_argument("dg::hip: only approx_quad = true (collocated Gauss-Lobatto quadrature) is implemented");
-}
 }  // namespace detail
 
 // WaveEquation (WaveEquation.hpp:28-52): Dw <- Dw + A w, interior faces only.
@@ -213,9 +242,8 @@ class WaveEquation : public Operator {
 public:
     WaveEquation(const Mesh2D& mesh, const QuadratureRule* basis, bool approx_quad = false,
                  const QuadratureRule* quad = nullptr, double a = 1.0, double b = 0.5)
-        : op_((detail::require_approx(approx_quad), mesh), basis, nullptr, a, b, Precision::F64)
+        : op_(mesh, basis, nullptr, a, b, Precision::F64, approx_quad, quad)
     {
-        (void)quad;
     }
     void action(const double* w, double* Dw) const override
     {
@@ -243,9 +271,8 @@ class WaveBC : public Operator {
 public:
     WaveBC(const Mesh2D& mesh, const int* bc, const QuadratureRule* basis, bool approx_quad,
            const QuadratureRule* quad = nullptr, double a = 1.0, double b = 0.5)
-        : op_((detail::require_approx(approx_quad), mesh), basis, bc, a, b, Precision::F64)
+        : op_(mesh, basis, bc, a, b, Precision::F64, approx_quad, quad)
     {
-        (void)quad;
     }
     void action(const double* w, double* Bw) const override
     {
@@ -267,16 +294,16 @@ private:
     detail::Wave2D op_;
 };
 
-// MassMatrix<true> (MassMatrix.hpp:77, MassMatrix.cpp:6-82); n_var is 3.
+// MassMatrix<true> (MassMatrix.hpp:77, MassMatrix.cpp:6-82), the diagonal
+// w_i w_j det J, and MassMatrix<false> (MassMatrix.cpp:85-227), the exact
+// block matrix held as per-element Cholesky factors; n_var is 3.  quad is
+// ignored for <true>, as in the reference.
 template <bool ApproxQuadrature>
 class MassMatrix : public InvertibleOperator {
-    static_assert(ApproxQuadrature, "dg::hip: only MassMatrix<true> is implemented");
-
 public:
     MassMatrix(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr)
-        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64, ApproxQuadrature, nullptr, ApproxQuadrature ? nullptr : quad)
     {
-        (void)quad;
     }
     void action(const double* x, double* y) const override
     {
@@ -309,8 +336,8 @@ private:
 class WaveSystem {
 public:
     WaveSystem(const Mesh2D& mesh, const QuadratureRule* basis, const int* bc, Precision p = Precision::F64,
-               double a = 1.0, double b = 0.5)
-        : op_(mesh, basis, bc, a, b, p)
+               double a = 1.0, double b = 0.5, bool approx_quad = true)
+        : op_(mesh, basis, bc, a, b, p, approx_quad)
     {
     }
     long long ndof() const { return op_.ndof(); }
@@ -334,11 +361,12 @@ private:
 // WaveHoltz (WaveHoltz.hpp:35-90, WaveHoltz.cpp:5-145).  With Precision::F32
 // the RK state runs in float while the filter accumulation, the fixed-point
 // update and the norms stay in double; the arrays passed in are always double.
+// approx_quad defaults to false (exact quadrature) as in the reference.
 class WaveHoltz {
 public:
     WaveHoltz(double omega, const Mesh2D& mesh, const QuadratureRule* basis, const int* boundary_conditions,
               bool approx_quad = false, Precision p = Precision::F64)
-        : omega_(omega), op_((detail::require_approx(approx_quad), mesh), basis, boundary_conditions, 1.0, 0.5, p)
+        : omega_(omega), op_(mesh, basis, boundary_conditions, 1.0, 0.5, p, approx_quad)
     {
         wdg_waveholtz h = nullptr;
         check(wdg_hip_waveholtz_create(op_.get(), omega, &h));
