@@ -74,6 +74,42 @@ def test_exact_rhs_and_rk4(cuda, name, n, bckind):
     assert rel_l2(du.cpu().numpy(), ref_u) < TOL
 
 
+@pytest.mark.parametrize("n", [2, 3, 5, 7, 8, 9])
+def test_exact_stage_kernel_all_orders(cuda, n):
+    """The fused exact stage kernel (default rules) for every order: forced
+    right-hand side and two forced RK4 steps on a mesh whose element count is
+    not a multiple of the elements per workgroup."""
+    torch = cuda
+    print(f"exact stage n={n}", flush=True)
+    om, ow, gm, gw = _setup("skewed6x5", n, "mixed")
+    u = random_state(gw.ndof, 40 + n)
+    F = om.load_vector_gauss(n, [[0.3, 0.2, 0.1, 1.5]])
+    y = torch.empty(gw.ndof, dtype=torch.float64, device="cuda")
+    gw.rhs(_dev(torch, u), y, _dev(torch, F), 0.7)
+    torch.cuda.synchronize()
+    assert rel_l2(y.cpu().numpy(), ow.rhs(u, F, 0.7)) < TOL
+    dt = 0.25 * gm.min_h() / n**2
+    ref_u, _ = ow.rk4(u, dt, 2, t=0.0, F=F, omega=3.0)
+    du = _dev(torch, u)
+    gw.rk4(du, dt, 2, t=0.0, F=_dev(torch, F), omega=3.0)
+    torch.cuda.synchronize()
+    assert rel_l2(du.cpu().numpy(), ref_u) < TOL
+
+
+def test_exact_custom_rules_rk4(cuda):
+    """Custom rules take the unfused kernels (volume, faces, mass)."""
+    torch = cuda
+    om, ow, gm, gw = _setup("L", 4, "absorb", quad=(6, 7, 5))
+    u = random_state(gw.ndof, 17)
+    F = om.load_vector_gauss(4, [[0.1, 0.2, 0.2, 1.0]])
+    dt = 0.5 * gm.min_h() / 16
+    ref_u, _ = ow.rk4(u, dt, 3, t=0.0, F=F, omega=5.0)
+    du = _dev(torch, u)
+    gw.rk4(du, dt, 3, t=0.0, F=_dev(torch, F), omega=5.0)
+    torch.cuda.synchronize()
+    assert rel_l2(du.cpu().numpy(), ref_u) < TOL
+
+
 def test_exact_custom_rules(cuda):
     torch = cuda
     _, ow, _, gw = _setup("skewed6x5", 4, "mixed", quad=(6, 7, 5))
@@ -115,3 +151,5 @@ def test_exact_rejects_f32_and_schedules(cuda):
     gw = capi.Wave2D(gm, 4, None, approx_quad=False)
     with pytest.raises(capi.WdgError):
         gw.autotune(2)
+    with pytest.raises(capi.WdgError):  # a mass rule too small for the basis
+        capi.Wave2D(gm, 4, None, approx_quad=False, quad=(0, 0, 3))

@@ -13,10 +13,12 @@
 //    is evaluated there (own outward normal: +f on el0, -f on el1 of the
 //    reference's edge flux) and projected onto the side's nodes; corner
 //    nodes add their two sides in side order, so every output has one writer;
-//  * mass matrix: a setup kernel integrates M = (phi_i phi_j det J) with the
-//    mass rule and factors it (Cholesky, linalg.cpp:5-30) in LDS; the apply
-//    kernel solves or multiplies with the factor (linalg.cpp:32-98) and
-//    finishes the RK stage or right-hand side in place.
+//  * mass matrix: det J of a bilinear element is affine in (xi, eta), so the
+//    exact element matrix is d0 A(x)A + d1 B(x)A + d2 A(x)B with 1D matrices
+//    A, B of the mass rule; it is applied and inverted (by fast
+//    diagonalisation) with tensor passes from three numbers per element
+//    instead of the reference's stored Cholesky factors (linalg.cpp:5-98), and
+//    the kernel finishes the RK stage or right-hand side in place.
 #include <hip/hip_runtime.h>
 
 #include "wave2d_kernels.hpp"
@@ -26,6 +28,41 @@ namespace {
 
 constexpr int kQMax = 10;  // points of any rule
 constexpr int kBlk = 128;  // threads per element workgroup
+
+// Own outward unit normal (n0, n1) and ds = |side| / 2 of side s of the
+// element with bilinear map coefficients g (its sides are straight).
+__device__ __forceinline__ void side_normal(const double* g, int s, double& n0, double& n1, double& ds)
+{
+    const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
+    const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
+    const double dy = 2.0 * ((s & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
+    const double l2 = dx * dx + dy * dy, il = rsqrt_nr(l2);
+    const double sil = (s >= 2) ? -il : il;
+    n0 = dy * sil;
+    n1 = -dx * sil;
+    ds = 0.5 * (l2 * il);
+}
+
+// Exterior state at a boundary node for boundary code `code` and outward
+// normal (n0, n1).
+__device__ __forceinline__ void boundary_state(int code, double n0, double n1, double pA, double uA, double vA,
+                                               double& pB, double& uB, double& vB)
+{
+    if (code == 1) {  // reflecting (WaveEquation.hpp:64-69)
+        pB = pA;
+        uB = (n1 * n1 - n0 * n0) * uA - 2.0 * n0 * n1 * vA;
+        vB = -2.0 * n0 * n1 * uA + (n0 * n0 - n1 * n1) * vA;
+    } else if (code == 0) {  // characteristic outflow, p_ext after u_ext
+        const double o = pA + n0 * uA + n1 * vA, tang = n1 * uA - n0 * vA;
+        uB = n1 * tang + 0.5 * n0 * o;
+        vB = -n0 * tang + 0.5 * n1 * o;
+        pB = n0 * uB + n1 * vB;
+    } else {  // Dirichlet p = 0 (mirror state)
+        pB = -pA;
+        uB = uA;
+        vB = vA;
+    }
+}
 
 template <int N>
 __global__ __launch_bounds__(kBlk) void exact_volume_kernel(const KArgs a, const ExactTables t, int accumulate)
@@ -112,21 +149,15 @@ __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const Exa
     for (int k = tid; k < 3 * NN; k += 64) s_out[k] = 0.0;
     for (int k = tid; k < nq * N; k += 64) s_P[k] = t.Pe[k];
     if (tid < nq) s_w[tid] = t.we[tid];
-    const double2* gp = reinterpret_cast<const double2*>(a.geom + 6 * (size_t)e);
-    const double2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
+    const double* g = a.geom + 6 * (size_t)e;
     bool any = false;
     for (int s = 0; s < 4; ++s) {
         const int nb = a.nbr[4 * (size_t)e + s], inf = a.info[4 * (size_t)e + s];
         const bool is_int = nb >= 0;
         if (is_int ? !interior : !boundary) continue;  // uniform over the workgroup
         any = true;
-        // own outward unit normal and ds = |side| / 2 (straight side of the bilinear map)
-        const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
-        const double dx = 2.0 * ((s & 1) ? g_c.x + sh * g_h.x : g_a.x + sh * g_h.x);
-        const double dy = 2.0 * ((s & 1) ? g_c.y + sh * g_h.y : g_a.y + sh * g_h.y);
-        const double l2 = dx * dx + dy * dy, il = rsqrt_nr(l2);
-        const double sil = (s >= 2) ? -il : il;
-        const double n0 = dy * sil, n1 = -dx * sil, ds = 0.5 * (l2 * il);
+        double n0, n1, ds;
+        side_normal(g, s, n0, n1, ds);
         __syncthreads();  // the previous side's traces and fluxes are consumed
         if (tid < N) {
             const int k = tid;
@@ -140,21 +171,7 @@ __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const Exa
                 uB = qb[1];
                 vB = qb[2];
             } else {
-                const int code = (inf >> 8) & 0xff;
-                if (code == 1) {  // reflecting (WaveEquation.hpp:64-69)
-                    pB = pA;
-                    uB = (n1 * n1 - n0 * n0) * uA - 2.0 * n0 * n1 * vA;
-                    vB = -2.0 * n0 * n1 * uA + (n0 * n0 - n1 * n1) * vA;
-                } else if (code == 0) {  // characteristic outflow, p_ext after u_ext
-                    const double o = pA + n0 * uA + n1 * vA, tang = n1 * uA - n0 * vA;
-                    uB = n1 * tang + 0.5 * n0 * o;
-                    vB = -n0 * tang + 0.5 * n1 * o;
-                    pB = n0 * uB + n1 * vB;
-                } else {  // Dirichlet p = 0 (mirror state)
-                    pB = -pA;
-                    uB = uA;
-                    vB = vA;
-                }
+                boundary_state((inf >> 8) & 0xff, n0, n1, pA, uA, vA, pB, uB, vB);
             }
             s_A[0 * N + k] = pA;
             s_A[1 * N + k] = uA;
@@ -198,113 +215,99 @@ __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const Exa
     for (int k = tid; k < 3 * NN; k += 64) y[k] += s_out[k];
 }
 
-// M(i, j) = sum over the mass rule of phi_i phi_j w det J (lower triangle) and
-// its Cholesky factor, written to L (c2d, c2d) column-major, upper part zero.
-template <int N>
-__global__ __launch_bounds__(256) void exact_mass_setup_kernel(const double* geom, const ExactMassTables m, double* L)
+// det J = d0 + d1 xi + d2 eta of the element's bilinear map (the xi eta terms
+// of x_xi y_eta - y_xi x_eta cancel).
+__device__ __forceinline__ void detj_affine(const double* g, double& d0, double& d1, double& d2)
 {
-    constexpr int C = N * N;
-    __shared__ double s_M[C * C];
-    __shared__ double s_B[kQMax * N], s_x[kQMax], s_w[kQMax], s_detJ[kQMax * kQMax];
-    const int e = blockIdx.x, tid = threadIdx.x, nq = m.nq;
-    for (int k = tid; k < nq * N; k += 256) s_B[k] = m.B[k];
-    if (tid < nq) {
-        s_x[tid] = m.x[tid];
-        s_w[tid] = m.w[tid];
-    }
-    __syncthreads();
-    const double* g = geom + 6 * (size_t)e;
-    for (int p = tid; p < nq * nq; p += 256) {
-        const double xi = s_x[p % nq], eta = s_x[p / nq];
-        const double J0 = g[0] + g[4] * eta, J1 = g[1] + g[5] * eta, J2 = g[2] + g[4] * xi, J3 = g[3] + g[5] * xi;
-        s_detJ[p] = J0 * J3 - J1 * J2;
-    }
-    __syncthreads();
-    for (int k = tid; k < C * C; k += 256) {
-        const int i = k % C, j = k / C;
-        double mij = 0.0;
-        if (i >= j)
-            for (int p = 0; p < nq * nq; ++p) {
-                const double bi = s_B[p % nq + nq * (i % N)] * s_B[p / nq + nq * (i / N)];
-                const double bj = s_B[p % nq + nq * (j % N)] * s_B[p / nq + nq * (j / N)];
-                mij += bi * bj * (s_detJ[p] * s_w[p % nq] * s_w[p / nq]);
-            }
-        s_M[k] = mij;
-    }
-    __syncthreads();
-    for (int k = 0; k < C; ++k) {
-        const double akk = s_M[k + C * k];
-        const int rem = C - k - 1;
-        for (int q = tid; q < rem * rem; q += 256) {
-            const int j = k + 1 + q / rem, i = k + 1 + q % rem;
-            if (i >= j) s_M[i + C * j] -= (s_M[j + C * k] / akk) * s_M[i + C * k];
-        }
-        __syncthreads();
-        const double sc = 1.0 / sqrt(akk);
-        for (int i = k + tid; i < C; i += 256) s_M[i + C * k] *= sc;
-        __syncthreads();
-    }
-    double* out = L + (size_t)e * C * C;
-    for (int k = tid; k < C * C; k += 256) out[k] = s_M[k];
+    d0 = g[0] * g[3] - g[1] * g[2];
+    d1 = g[0] * g[5] - g[1] * g[4];
+    d2 = g[3] * g[4] - g[2] * g[5];
 }
 
-// Mass-matrix application per element with the factor L:
+// Mass-matrix application per element, M = d0 A(x)A + d1 B(x)A + d2 A(x)B:
 //   kid < 0: -1 y = M x, -2 y = M^{-1} x (x = a.u_in, y = a.u_out; may alias);
 //   kid >= 0: the RHS / RK-stage epilogue of wave_kernel for kModeOf[kid],
 //             with the operator's output r (before M^{-1}) in `r`.
+// M^{-1} by fast diagonalisation: (V(x)V) diag(1 / (d0 + d1 lam_a + d2 lam_b)) (V(x)V)'.
 template <int N>
-__global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const double* r, const double* L, int kid)
+__global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const double* r, const ExactMassTables m,
+                                                          int kid)
 {
     constexpr int C = N * N;
-    __shared__ double s_x[3 * C], s_t[3 * C];
+    __shared__ double s_x[3 * C], s_t[3 * C], s_t2[3 * C], s_V[C], s_W[C], s_lam[N];
     const int e = blockIdx.x, tid = threadIdx.x;
-    const double* Le = L + (size_t)e * C * C;
     const size_t base = (size_t)e * 3 * C;
     const int mode = kid >= 0 ? kModeOf[kid] : 0;
     const bool force = kid >= 0 && (mode & M_FORCE);
+    const bool mult = kid == -1;
     for (int k = tid; k < 3 * C; k += kBlk) {
         double v = kid < 0 ? static_cast<const double*>(a.u_in)[base + k] : r[base + k];
         if (force) v += static_cast<const double*>(a.force)[base + k] * a.fs;
         s_x[k] = v;
     }
+    for (int k = tid; k < C; k += kBlk) {
+        s_V[k] = mult ? m.A[k] : m.V[k];
+        s_W[k] = m.B[k];
+    }
+    if (tid < N) s_lam[tid] = m.lam[tid];
+    double d0, d1, d2;
+    detj_affine(a.geom + 6 * (size_t)e, d0, d1, d2);
     __syncthreads();
-    if (kid == -1) {
-        // y = L (L' x)   (mult_chol, linalg.cpp:68-98)
+    if (mult) {
+        // (d0 A x + d1 B x, d2 A x) along xi, then A and B along eta
         for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = k / 3;
-            double y = 0.0;
-            for (int j = i; j < C; ++j) y += Le[j + C * i] * s_x[d + 3 * j];
-            s_t[k] = y;
+            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
+            double xa = 0.0, xb = 0.0;
+            for (int l = 0; l < N; ++l) {
+                const double x = s_x[d + 3 * (l + N * j)];
+                xa += s_V[i + N * l] * x;
+                xb += s_W[i + N * l] * x;
+            }
+            s_t[k] = d0 * xa + d1 * xb;
+            s_t2[k] = d2 * xa;
         }
         __syncthreads();
         for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = k / 3;
+            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
             double y = 0.0;
-            for (int j = 0; j <= i; ++j) y += Le[i + C * j] * s_t[d + 3 * j];
-            s_x[k] = y;
+            for (int l = 0; l < N; ++l)
+                y += s_V[j + N * l] * s_t[d + 3 * (i + N * l)] + s_W[j + N * l] * s_t2[d + 3 * (i + N * l)];
+            s_x[k] = y;  // s_x is not read in this pass
         }
-        __syncthreads();
     } else {
-        // x = (L L')^{-1} x   (solve_chol, linalg.cpp:32-66)
-        for (int j = 0; j < C; ++j) {
-            if (tid < 3) s_x[tid + 3 * j] /= Le[j + C * j];
-            __syncthreads();
-            for (int k = tid; k < 3 * (C - j - 1); k += kBlk) {
-                const int d = k % 3, i = j + 1 + k / 3;
-                s_x[d + 3 * i] -= Le[i + C * j] * s_x[d + 3 * j];
-            }
-            __syncthreads();
+        // t(d, a, j) = sum_i V(i, a) x(d, i, j)
+        for (int k = tid; k < 3 * C; k += kBlk) {
+            const int d = k % 3, aa = (k / 3) % N, j = k / (3 * N);
+            double t = 0.0;
+            for (int i = 0; i < N; ++i) t += s_V[i + N * aa] * s_x[d + 3 * (i + N * j)];
+            s_t[k] = t;
         }
-        for (int j = C - 1; j >= 0; --j) {
-            if (tid < 3) s_x[tid + 3 * j] /= Le[j + C * j];
-            __syncthreads();
-            for (int k = tid; k < 3 * j; k += kBlk) {
-                const int d = k % 3, i = k / 3;
-                s_x[d + 3 * i] -= Le[j + C * i] * s_x[d + 3 * j];
-            }
-            __syncthreads();
+        __syncthreads();
+        // x(d, a, b) = sum_j V(j, b) t(d, a, j) / (d0 + d1 lam_a + d2 lam_b)
+        for (int k = tid; k < 3 * C; k += kBlk) {
+            const int d = k % 3, aa = (k / 3) % N, b = k / (3 * N);
+            double t = 0.0;
+            for (int j = 0; j < N; ++j) t += s_V[j + N * b] * s_t[d + 3 * (aa + N * j)];
+            s_x[k] = t / (d0 + d1 * s_lam[aa] + d2 * s_lam[b]);
+        }
+        __syncthreads();
+        // t(d, i, b) = sum_a V(i, a) x(d, a, b)
+        for (int k = tid; k < 3 * C; k += kBlk) {
+            const int d = k % 3, i = (k / 3) % N, b = k / (3 * N);
+            double t = 0.0;
+            for (int aa = 0; aa < N; ++aa) t += s_V[i + N * aa] * s_x[d + 3 * (aa + N * b)];
+            s_t[k] = t;
+        }
+        __syncthreads();
+        // x(d, i, j) = sum_b V(j, b) t(d, i, b)
+        for (int k = tid; k < 3 * C; k += kBlk) {
+            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
+            double t = 0.0;
+            for (int b = 0; b < N; ++b) t += s_V[j + N * b] * s_t[d + 3 * (i + N * b)];
+            s_x[k] = t;
         }
     }
+    __syncthreads();
     const int out = mode & M_OUTMASK;
     for (int k = tid; k < 3 * C; k += kBlk) {
         const size_t g = base + k;
@@ -322,32 +325,357 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
     }
 }
 
+// ---------------------------------------------------------------- fused stage
+// The whole right-hand side / RK stage for the default rules (QV = N volume,
+// QE = N + 1 edge points) in one kernel.  E elements share a 64-lane
+// workgroup; every tensor pass gives one lane one (element, variable, line)
+// and keeps the line in registers, so the 1D tables are uniform operands read
+// through the scalar cache.  Per element three LDS regions are reused phase by
+// phase:
+//   RA: u -> u at quadrature points -> edge-point traces -> t1 -> t3
+//   RB: xi-interpolated u -> (Df, Pg) -> staged traces -> edge fluxes -> t2
+//   RC: operator output r -> M^{-1} (r + s F)
 template <int N>
-void launch_all(int what, const KArgs& a, const ExactTables& t, const ExactMassTables& m, const double* r, double* L,
-                int p0, int p1, hipStream_t s)
+struct StageShape {
+    static constexpr int cmax(int x, int y) { return x > y ? x : y; }
+    static constexpr int QV = N, QE = N + 1, NN = N * N;
+    static constexpr int E = cmax(64 / (3 * N), 1);
+    static constexpr int SA = cmax(cmax(3 * NN, 3 * QV * QV), 24 * QE);
+    static constexpr int SB = cmax(cmax(3 * N * QV, 6 * QV * N), cmax(cmax(24 * N, 12 * QE), 3 * NN));
+    static constexpr int SC = 3 * NN;
+    static constexpr int S = SA + SB + SC;
+};
+
+template <int N>
+__global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const double* __restrict__ tab, int kid)
+{
+    using Sh = StageShape<N>;
+    constexpr int QV = Sh::QV, QE = Sh::QE, NN = Sh::NN, E = Sh::E;
+    __shared__ double s_mem[E * Sh::S];
+    const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
+    const int mode = kModeOf[kid];
+    const double* __restrict__ u = static_cast<const double*>(a.u_in);
+    auto RA = [&](int e) { return s_mem + e * Sh::S; };
+    auto RB = [&](int e) { return s_mem + e * Sh::S + Sh::SA; };
+    auto RC = [&](int e) { return s_mem + e * Sh::S + Sh::SA + Sh::SB; };
+
+    for (int it = lane; it < E * 3 * NN; it += 64) {
+        const int e = it / (3 * NN), k = it % (3 * NN);
+        if (e0 + e < nel) RA(e)[k] = u[(size_t)(e0 + e) * 3 * NN + k];
+    }
+    __syncthreads();
+    // ---- volume: the passes of div_2d_q (Div.cpp:469-588)
+    // PU(l, d, i) = sum_k P(i, k) u(d, k, l)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, l = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* U = RA(e);
+        double x[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = U[d + 3 * (k + N * l)];
+        double* PU = RB(e);
+#pragma unroll
+        for (int i = 0; i < QV; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += tab[kTabPV + i + QV * k] * x[k];
+            PU[l + N * (d + 3 * i)] = acc;
+        }
+    }
+    __syncthreads();
+    // UQ(d, i, j) = sum_l P(j, l) PU(l, d, i)
+    for (int it = lane; it < E * 3 * QV; it += 64) {
+        const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, i = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* PU = RB(e);
+        double x[N];
+#pragma unroll
+        for (int l = 0; l < N; ++l) x[l] = PU[l + N * (d + 3 * i)];
+        double* UQ = RA(e);
+#pragma unroll
+        for (int jq = 0; jq < QV; ++jq) {
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < N; ++l) acc += tab[kTabPV + jq + QV * l] * x[l];
+            UQ[d + 3 * (i + QV * jq)] = acc;
+        }
+    }
+    __syncthreads();
+    // contravariant fluxes of f = (u, p, 0), g = (v, 0, p) at the points and
+    // Df(j, d, k) = sum_i D(i, k) F(d, i, j), Pg(j, d, k) = sum_i P(i, k) G(d, i, j)
+    for (int it = lane; it < E * 3 * QV; it += 64) {
+        const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, jq = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* g = a.geom + 6 * (size_t)(e0 + e);
+        const double ax = g[0], ay = g[1], cx = g[2], cy = g[3], hx = g[4], hy = g[5];
+        const double eta = tab[kTabXV + jq], wj = tab[kTabWV + jq];
+        const double* UQ = RA(e);
+        double df[N], pg[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) df[k] = pg[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < QV; ++i) {
+            const double xi = tab[kTabXV + i], W = tab[kTabWV + i] * wj;
+            const double X_xi = (ax + hx * eta) * W, Y_xi = (ay + hy * eta) * W;
+            const double X_eta = (cx + hx * xi) * W, Y_eta = (cy + hy * xi) * W;
+            const double* q = UQ + 3 * (i + QV * jq);
+            const double p = q[0], uu = q[1], vv = q[2];
+            const double s1 = d == 0 ? uu : (d == 1 ? p : 0.0);
+            const double s2 = d == 0 ? vv : (d == 2 ? p : 0.0);
+            const double F = Y_eta * s1 - X_eta * s2, G = -Y_xi * s1 + X_xi * s2;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                df[k] += tab[kTabDV + i + QV * k] * F;
+                pg[k] += tab[kTabPV + i + QV * k] * G;
+            }
+        }
+        double* DP = RB(e);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            DP[jq + QV * (d + 3 * k)] = df[k];
+            DP[3 * N * QV + jq + QV * (d + 3 * k)] = pg[k];
+        }
+    }
+    __syncthreads();
+    // r(d, k, l) = sum_j P(j, l) Df(j, d, k) + D(j, l) Pg(j, d, k)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, k = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* DP = RB(e);
+        double df[QV], pg[QV];
+#pragma unroll
+        for (int jq = 0; jq < QV; ++jq) {
+            df[jq] = DP[jq + QV * (d + 3 * k)];
+            pg[jq] = DP[3 * N * QV + jq + QV * (d + 3 * k)];
+        }
+        double* R = RC(e);
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+            double acc = 0.0;
+#pragma unroll
+            for (int jq = 0; jq < QV; ++jq) acc += tab[kTabPV + jq + QV * l] * df[jq] + tab[kTabDV + jq + QV * l] * pg[jq];
+            R[d + 3 * (k + N * l)] = acc;
+        }
+    }
+    __syncthreads();
+    // ---- faces (EdgeFlux<false>): own and exterior traces at the side's nodes
+    for (int it = lane; it < E * 4 * N; it += 64) {
+        const int e = it / (4 * N), j = it % (4 * N), s = j / N, k = j % N, ge = e0 + e;
+        if (ge >= nel) continue;
+        const int nb = a.nbr[4 * (size_t)ge + s], inf = a.info[4 * (size_t)ge + s];
+        const double* qa = u + ((size_t)ge * NN + side_node<N>(s, k)) * 3;
+        const double pA = qa[0], uA = qa[1], vA = qa[2];
+        double pB, uB, vB;
+        if (nb >= 0) {
+            const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+            const double* qb = u + ((size_t)nb * NN + side_node<N>(inf & 3, kk)) * 3;
+            pB = qb[0];
+            uB = qb[1];
+            vB = qb[2];
+        } else {
+            double n0, n1, ds;
+            side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
+            boundary_state((inf >> 8) & 0xff, n0, n1, pA, uA, vA, pB, uB, vB);
+        }
+        double* T = RB(e) + 6 * N * s;  // (own | exterior, d, k)
+        T[0 * N + k] = pA;
+        T[1 * N + k] = uA;
+        T[2 * N + k] = vA;
+        T[3 * N + k] = pB;
+        T[4 * N + k] = uB;
+        T[5 * N + k] = vB;
+    }
+    __syncthreads();
+    // traces at the edge points
+    for (int it = lane; it < E * 24; it += 64) {
+        const int e = it / 24, j = it % 24;  // j = 6 s + 3 (own | exterior) + d
+        if (e0 + e >= nel) continue;
+        const double* T = RB(e) + N * j;
+        double x[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = T[k];
+        double* Q = RA(e) + QE * j;
+#pragma unroll
+        for (int q = 0; q < QE; ++q) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += x[k] * tab[kTabPE + q + QE * k];
+            Q[q] = acc;
+        }
+    }
+    __syncthreads();
+    // closed-form flux at every edge point, weighted by w ds
+    for (int it = lane; it < E * 4 * QE; it += 64) {
+        const int e = it / (4 * QE), j = it % (4 * QE), s = j / QE, q = j % QE, ge = e0 + e;
+        if (ge >= nel) continue;
+        double n0, n1, ds;
+        side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
+        const double* Q = RA(e) + 6 * QE * s;
+        const double A0 = Q[0 * QE + q], A1 = Q[1 * QE + q], A2 = Q[2 * QE + q];
+        const double B0 = Q[3 * QE + q], B1 = Q[4 * QE + q], B2 = Q[5 * QE + q];
+        const double wds = tab[kTabWE + q] * ds;
+        double* Cf = RB(e) + 3 * QE * s;
+        Cf[0 * QE + q] = wds * (a.ha * (n0 * (A1 + B1) + n1 * (A2 + B2)) + a.hb * (A0 - B0));
+        const double cf = wds * (a.ha * (A0 + B0) + a.hb * (n0 * (A1 - B1) + n1 * (A2 - B2)));
+        Cf[1 * QE + q] = n0 * cf;
+        Cf[2 * QE + q] = n1 * cf;
+    }
+    __syncthreads();
+    // projection onto the side's nodes, added to r: sides 0 and 2 first, then
+    // 1 and 3, so each corner has one writer per pass
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int it = lane; it < E * 12; it += 64) {
+            const int e = it / 12, j = it % 12, s = j / 3, d = j % 3;
+            if ((s & 1) != pass || e0 + e >= nel) continue;
+            const double* Cf = RB(e) + QE * (3 * s + d);
+            double c[QE];
+#pragma unroll
+            for (int q = 0; q < QE; ++q) c[q] = Cf[q];
+            double* R = RC(e);
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < QE; ++q) v += c[q] * tab[kTabPE + q + QE * k];
+                R[d + 3 * side_node<N>(s, k)] += v;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- M^{-1} (r + s F) by fast diagonalisation
+    const bool force = mode & M_FORCE;
+    // t1(d, a, j) = sum_i V(i, a) x(d, i, j)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, jn = j / 3, ge = e0 + e;
+        if (ge >= nel) continue;
+        const double* R = RC(e);
+        double x[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double v = R[d + 3 * (i + N * jn)];
+            if (force) v += static_cast<const double*>(a.force)[(size_t)ge * 3 * NN + d + 3 * (i + N * jn)] * a.fs;
+            x[i] = v;
+        }
+        double* T1 = RA(e);
+#pragma unroll
+        for (int aa = 0; aa < N; ++aa) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) t += tab[kTabV + i + N * aa] * x[i];
+            T1[d + 3 * (aa + N * jn)] = t;
+        }
+    }
+    __syncthreads();
+    // t2(d, a, b) = sum_j V(j, b) t1(d, a, j) / (d0 + d1 lam_a + d2 lam_b)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, aa = j / 3, ge = e0 + e;
+        if (ge >= nel) continue;
+        double d0, d1, d2;
+        detj_affine(a.geom + 6 * (size_t)ge, d0, d1, d2);
+        const double da = d0 + d1 * tab[kTabLam + aa];
+        const double* T1 = RA(e);
+        double x[N];
+#pragma unroll
+        for (int jn = 0; jn < N; ++jn) x[jn] = T1[d + 3 * (aa + N * jn)];
+        double* T2 = RB(e);
+#pragma unroll
+        for (int b = 0; b < N; ++b) {
+            double t = 0.0;
+#pragma unroll
+            for (int jn = 0; jn < N; ++jn) t += tab[kTabV + jn + N * b] * x[jn];
+            T2[d + 3 * (aa + N * b)] = t * recip(da + d2 * tab[kTabLam + b]);
+        }
+    }
+    __syncthreads();
+    // t3(d, i, b) = sum_a V(i, a) t2(d, a, b)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, b = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* T2 = RB(e);
+        double x[N];
+#pragma unroll
+        for (int aa = 0; aa < N; ++aa) x[aa] = T2[d + 3 * (aa + N * b)];
+        double* T3 = RA(e);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int aa = 0; aa < N; ++aa) t += tab[kTabV + i + N * aa] * x[aa];
+            T3[d + 3 * (i + N * b)] = t;
+        }
+    }
+    __syncthreads();
+    // y(d, i, j) = sum_b V(j, b) t3(d, i, b)
+    for (int it = lane; it < E * 3 * N; it += 64) {
+        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, i = j / 3;
+        if (e0 + e >= nel) continue;
+        const double* T3 = RA(e);
+        double x[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) x[b] = T3[d + 3 * (i + N * b)];
+        double* Y = RC(e);
+#pragma unroll
+        for (int jn = 0; jn < N; ++jn) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < N; ++b) t += tab[kTabV + jn + N * b] * x[b];
+            Y[d + 3 * (i + N * jn)] = t;
+        }
+    }
+    __syncthreads();
+    // ---- epilogue: right-hand side, or the RK stage (and WaveHoltz accumulation)
+    const int out = mode & M_OUTMASK;
+    for (int it = lane; it < E * 3 * NN; it += 64) {
+        const int e = it / (3 * NN), k = it % (3 * NN);
+        if (e0 + e >= nel) continue;
+        const size_t g = (size_t)(e0 + e) * 3 * NN + k;
+        const double rhs = RC(e)[k];
+        if (out == M_OUT_RHS) {
+            static_cast<double*>(a.u_out)[g] = rhs;
+            continue;
+        }
+        double* du = static_cast<double*>(a.du);
+        const double dd = (out == M_OUT_STAGE0 ? 0.0 : a.rk_a * du[g]) + a.dt * rhs;
+        du[g] = dd;
+        const double w = u[g] + a.rk_b * dd;
+        static_cast<double*>(a.u_out)[g] = w;
+        if (mode & M_ACC) a.acc[g] += a.ac * w;
+    }
+}
+
+template <int N>
+void launch_stage(int kid, const KArgs& a, const double* tab, hipStream_t s)
+{
+    constexpr int E = StageShape<N>::E;
+    hipLaunchKernelGGL(exact_stage_kernel<N>, dim3((a.nel + E - 1) / E), dim3(64), 0, s, a, tab, kid);
+}
+
+template <int N>
+void launch_all(int what, const KArgs& a, const ExactTables& t, const ExactMassTables& m, const double* r, int p0,
+                int p1, hipStream_t s)
 {
     const dim3 grid(a.nel);
     switch (what) {
     case 0: hipLaunchKernelGGL(exact_volume_kernel<N>, grid, dim3(kBlk), 0, s, a, t, p0); break;
     case 1: hipLaunchKernelGGL(exact_face_kernel<N>, grid, dim3(64), 0, s, a, t, p0, p1); break;
-    case 2: hipLaunchKernelGGL(exact_mass_setup_kernel<N>, grid, dim3(256), 0, s, a.geom, m, L); break;
-    default: hipLaunchKernelGGL(exact_mass_kernel<N>, grid, dim3(kBlk), 0, s, a, r, L, p0); break;
+    default: hipLaunchKernelGGL(exact_mass_kernel<N>, grid, dim3(kBlk), 0, s, a, r, m, p0); break;
     }
 }
 
-void dispatch(int n, int what, const KArgs& a, const ExactTables& t, const ExactMassTables& m, const double* r,
-              double* L, int p0, int p1, hipStream_t s)
+void dispatch(int n, int what, const KArgs& a, const ExactTables& t, const ExactMassTables& m, const double* r, int p0,
+              int p1, hipStream_t s)
 {
     if (a.nel <= 0) return;
     switch (n) {
-    case 2: launch_all<2>(what, a, t, m, r, L, p0, p1, s); break;
-    case 3: launch_all<3>(what, a, t, m, r, L, p0, p1, s); break;
-    case 4: launch_all<4>(what, a, t, m, r, L, p0, p1, s); break;
-    case 5: launch_all<5>(what, a, t, m, r, L, p0, p1, s); break;
-    case 6: launch_all<6>(what, a, t, m, r, L, p0, p1, s); break;
-    case 7: launch_all<7>(what, a, t, m, r, L, p0, p1, s); break;
-    case 8: launch_all<8>(what, a, t, m, r, L, p0, p1, s); break;
-    case 9: launch_all<9>(what, a, t, m, r, L, p0, p1, s); break;
+    case 2: launch_all<2>(what, a, t, m, r, p0, p1, s); break;
+    case 3: launch_all<3>(what, a, t, m, r, p0, p1, s); break;
+    case 4: launch_all<4>(what, a, t, m, r, p0, p1, s); break;
+    case 5: launch_all<5>(what, a, t, m, r, p0, p1, s); break;
+    case 6: launch_all<6>(what, a, t, m, r, p0, p1, s); break;
+    case 7: launch_all<7>(what, a, t, m, r, p0, p1, s); break;
+    case 8: launch_all<8>(what, a, t, m, r, p0, p1, s); break;
+    case 9: launch_all<9>(what, a, t, m, r, p0, p1, s); break;
     default: break;
     }
 }
@@ -356,22 +684,33 @@ void dispatch(int n, int what, const KArgs& a, const ExactTables& t, const Exact
 
 void launch_exact_volume(int n, const KArgs& a, const ExactTables& t, int accumulate, hipStream_t s)
 {
-    dispatch(n, 0, a, t, ExactMassTables{}, nullptr, nullptr, accumulate, 0, s);
+    dispatch(n, 0, a, t, ExactMassTables{}, nullptr, accumulate, 0, s);
 }
 
 void launch_exact_face(int n, const KArgs& a, const ExactTables& t, int interior, int boundary, hipStream_t s)
 {
-    dispatch(n, 1, a, t, ExactMassTables{}, nullptr, nullptr, interior, boundary, s);
+    dispatch(n, 1, a, t, ExactMassTables{}, nullptr, interior, boundary, s);
 }
 
-void launch_exact_mass_setup(int n, const KArgs& a, const ExactMassTables& m, double* L, hipStream_t s)
+void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s)
 {
-    dispatch(n, 2, a, ExactTables{}, m, nullptr, L, 0, 0, s);
+    dispatch(n, 2, a, ExactTables{}, m, r, kid, 0, s);
 }
 
-void launch_exact_mass(int n, const KArgs& a, const double* r, const double* L, int kid, hipStream_t s)
+void launch_exact_stage(int n, int kid, const KArgs& a, const double* tab, hipStream_t s)
 {
-    dispatch(n, 3, a, ExactTables{}, ExactMassTables{}, r, const_cast<double*>(L), kid, 0, s);
+    if (a.nel <= 0) return;
+    switch (n) {
+    case 2: launch_stage<2>(kid, a, tab, s); break;
+    case 3: launch_stage<3>(kid, a, tab, s); break;
+    case 4: launch_stage<4>(kid, a, tab, s); break;
+    case 5: launch_stage<5>(kid, a, tab, s); break;
+    case 6: launch_stage<6>(kid, a, tab, s); break;
+    case 7: launch_stage<7>(kid, a, tab, s); break;
+    case 8: launch_stage<8>(kid, a, tab, s); break;
+    case 9: launch_stage<9>(kid, a, tab, s); break;
+    default: break;
+    }
 }
 
 }  // namespace wdg

@@ -153,4 +153,98 @@ void lagrange_interpolation_matrix(int ngrid, const double* xg, int neval, const
     }
 }
 
+bool mass_fast_diag(int n, const double* xb, int nq, const double* xq, const double* wq, double* A, double* B,
+                    double* V, double* lam)
+{
+    std::vector<double> P((size_t)nq * n);
+    lagrange_interpolation_matrix(n, xb, nq, xq, P.data());
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double a = 0.0, b = 0.0;
+            for (int q = 0; q < nq; ++q) {
+                const double pp = wq[q] * P[q + nq * i] * P[q + nq * j];
+                a += pp;
+                b += pp * xq[q];
+            }
+            A[i + n * j] = a;
+            B[i + n * j] = b;
+        }
+    // A = R R' (R lower triangular)
+    std::vector<double> R((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double d = A[j + n * j];
+        for (int k = 0; k < j; ++k) d -= R[j + n * k] * R[j + n * k];
+        if (!(d > 1e-14 * A[j + n * j])) return false;
+        const double rjj = std::sqrt(d);
+        R[j + n * j] = rjj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[i + n * j];
+            for (int k = 0; k < j; ++k) v -= R[i + n * k] * R[j + n * k];
+            R[i + n * j] = v / rjj;
+        }
+    }
+    // C = R^{-1} B R^{-T}: Y = R^{-1} B column by column, then C = R^{-1} Y'
+    auto lower_solve = [&](double* x) {  // x <- R^{-1} x
+        for (int i = 0; i < n; ++i) {
+            double v = x[i];
+            for (int k = 0; k < i; ++k) v -= R[i + n * k] * x[k];
+            x[i] = v / R[i + n * i];
+        }
+    };
+    std::vector<double> Y(B, B + (size_t)n * n), C((size_t)n * n), col(n);
+    for (int j = 0; j < n; ++j) lower_solve(&Y[(size_t)n * j]);
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) col[i] = Y[j + n * i];  // row j of Y
+        lower_solve(col.data());
+        for (int i = 0; i < n; ++i) C[i + n * j] = col[i];
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < j; ++i) C[i + n * j] = C[j + n * i] = 0.5 * (C[i + n * j] + C[j + n * i]);
+    // cyclic Jacobi: C = U diag(lam) U'
+    std::vector<double> U((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) U[i + n * i] = 1.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                tot += C[i + n * j] * C[i + n * j];
+                if (i != j) off += C[i + n * j] * C[i + n * j];
+            }
+        if (off <= 1e-34 * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = C[p + n * q];
+                if (apq == 0.0) continue;
+                const double theta = (C[q + n * q] - C[p + n * p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double ckp = C[k + n * p], ckq = C[k + n * q];
+                    C[k + n * p] = c * ckp - s * ckq;
+                    C[k + n * q] = s * ckp + c * ckq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double cpk = C[p + n * k], cqk = C[q + n * k];
+                    C[p + n * k] = c * cpk - s * cqk;
+                    C[q + n * k] = s * cpk + c * cqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double ukp = U[k + n * p], ukq = U[k + n * q];
+                    U[k + n * p] = c * ukp - s * ukq;
+                    U[k + n * q] = s * ukp + c * ukq;
+                }
+            }
+    }
+    // V = R^{-T} U
+    for (int j = 0; j < n; ++j) {
+        lam[j] = C[j + n * j];
+        for (int i = n - 1; i >= 0; --i) {
+            double v = U[i + n * j];
+            for (int k = i + 1; k < n; ++k) v -= R[k + n * i] * V[k + n * j];
+            V[i + n * j] = v / R[i + n * i];
+        }
+    }
+    return true;
+}
+
 }  // namespace wdg

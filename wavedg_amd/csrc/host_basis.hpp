@@ -24,4 +24,16 @@ void lagrange_derivative_matrix(int ngrid, const double* xg, int neval, const do
 // B(i, j) = phi_j(xe_i) (column-major, neval x ngrid) (lagrange_interpolation.cpp:14-31).
 void lagrange_interpolation_matrix(int ngrid, const double* xg, int neval, const double* xe, double* B);
 
+// The exact mass matrix of a bilinear element factorises because det J is
+// affine in (xi, eta): det J = d0 + d1 xi + d2 eta, so with the 1D matrices
+//   A(i, j) = sum_q w_q phi_i(x_q) phi_j(x_q),  B(i, j) = sum_q w_q x_q phi_i(x_q) phi_j(x_q)
+// (rule x_q, w_q; basis phi on xb) the element matrix on node (i + n j) is
+//   M = d0 A(x)A + d1 B(x)A + d2 A(x)B   (first factor on the xi index).
+// V and lam solve B V = A V diag(lam) with V' A V = I, hence
+//   M^{-1} = (V(x)V) diag(1 / (d0 + d1 lam_a + d2 lam_b)) (V(x)V)'.
+// All n x n column-major.  Returns false when A is not positive definite
+// (a rule too small for the basis).
+bool mass_fast_diag(int n, const double* xb, int nq, const double* xq, const double* wq, double* A, double* B,
+                    double* V, double* lam);
+
 }  // namespace wdg

@@ -143,13 +143,19 @@ struct wdg_wave2d_s {
     // approx_quad = false: exact-quadrature operators (FP64)
     bool exact = false;
     int nqv = 0, nqe = 0, nqm = 0;
-    double* d_extab = nullptr;  // xv, wv, Pv, Dv | xe, we, Pe | xm, wm, Bm (10 + 10 + 100 + 100 each block)
-    double* d_L = nullptr;      // Cholesky factors of the element mass matrices (n^4 per element)
-    void* d_exr = nullptr;      // operator output before M^{-1}
+    double* d_extab = nullptr;  // quadrature and mass tables, offsets wdg::kTab*
+    void* d_exr = nullptr;      // operator output before M^{-1} (custom rules only)
+    bool exact_fused() const { return nqv == n && nqe == n + 1; }
     wdg::ExactTables tables() const
     {
         const double* t = d_extab;
-        return wdg::ExactTables{t, t + 10, t + 20, t + 120, t + 220, t + 230, t + 240, nqv, nqe};
+        return wdg::ExactTables{t + wdg::kTabXV, t + wdg::kTabWV, t + wdg::kTabPV, t + wdg::kTabDV,
+                                t + wdg::kTabXE, t + wdg::kTabWE, t + wdg::kTabPE, nqv, nqe};
+    }
+    wdg::ExactMassTables mass_tables() const
+    {
+        const double* t = d_extab;
+        return wdg::ExactMassTables{t + wdg::kTabA, t + wdg::kTabB, t + wdg::kTabV, t + wdg::kTabLam};
     }
 
     ~wdg_wave2d_s()
@@ -165,7 +171,6 @@ struct wdg_wave2d_s {
         dfree(d_basis);
         dfree(d_lvtab);
         dfree(d_extab);
-        dfree(d_L);
         dfree(d_exr);
         dfree(d_tmp);
         dfree(d_du);
@@ -241,9 +246,10 @@ void launch_wave(int n, int kid, int f32, int sched, const KArgs& a, hipStream_t
     hip_check(hipGetLastError(), "wave kernel launch");
 }
 
-// One operator / RK-stage evaluation: the fused kernel, or for exact-
-// quadrature handles volume + face kernels into a scratch output and the
-// block mass-matrix kernel that finishes the right-hand side or stage.
+// One operator / RK-stage evaluation: the fused kernel; for exact-quadrature
+// handles the fused exact stage kernel (default rules), or volume + face
+// kernels into a scratch output and the mass-matrix kernel that finishes the
+// right-hand side or stage.
 void apply_op(wdg_wave2d op, int kid, const KArgs& a, hipStream_t s)
 {
     if (!op->exact) {
@@ -256,13 +262,15 @@ void apply_op(wdg_wave2d op, int kid, const KArgs& a, hipStream_t s)
         wdg::launch_exact_face(op->n, a, t, 1, 0, s);
     } else if (kid == wdg::K_BC) {
         wdg::launch_exact_face(op->n, a, t, 0, 1, s);
+    } else if (op->exact_fused()) {
+        wdg::launch_exact_stage(op->n, kid, a, op->d_extab, s);
     } else {
         op->ensure(op->d_exr, op->state_bytes());
         KArgs r = a;
         r.u_out = op->d_exr;
         wdg::launch_exact_volume(op->n, r, t, 0, s);
         wdg::launch_exact_face(op->n, r, t, 1, 1, s);
-        wdg::launch_exact_mass(op->n, a, static_cast<const double*>(op->d_exr), op->d_L, kid, s);
+        wdg::launch_exact_mass(op->n, a, static_cast<const double*>(op->d_exr), op->mass_tables(), kid, s);
     }
     hip_check(hipGetLastError(), "exact-quadrature kernel launch");
 }
@@ -846,19 +854,24 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, doub
     if (rc) return rc;
     const int rc2 = guarded([&] {
         const int n = n_colloc;
-        // volume rule @0 (x, w, P, D), edge rule @220 (x, w, P), mass rule @340 (x, w, B)
-        std::vector<double> tab(460, 0.0);
-        auto rule = [&](int nq, int off, bool deriv) {
-            std::vector<double> xq, wq;
+        std::vector<double> tab(wdg::kTabSize, 0.0);
+        auto rule = [&](int nq, std::vector<double>& xq, std::vector<double>& wq) {
             if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "no Gauss-Legendre rule of that size");
-            std::copy(xq.begin(), xq.end(), tab.begin() + off);
-            std::copy(wq.begin(), wq.end(), tab.begin() + off + 10);
-            wdg::lagrange_interpolation_matrix(n, op->x.data(), nq, xq.data(), tab.data() + off + 20);
-            if (deriv) wdg::lagrange_derivative_matrix(n, op->x.data(), nq, xq.data(), tab.data() + off + 120);
         };
-        rule(nq_volume, 0, true);
-        rule(nq_edge, 220, false);
-        rule(nq_mass, 340, false);
+        std::vector<double> xq, wq;
+        rule(nq_volume, xq, wq);
+        std::copy(xq.begin(), xq.end(), tab.begin() + wdg::kTabXV);
+        std::copy(wq.begin(), wq.end(), tab.begin() + wdg::kTabWV);
+        wdg::lagrange_interpolation_matrix(n, op->x.data(), nq_volume, xq.data(), tab.data() + wdg::kTabPV);
+        wdg::lagrange_derivative_matrix(n, op->x.data(), nq_volume, xq.data(), tab.data() + wdg::kTabDV);
+        rule(nq_edge, xq, wq);
+        std::copy(xq.begin(), xq.end(), tab.begin() + wdg::kTabXE);
+        std::copy(wq.begin(), wq.end(), tab.begin() + wdg::kTabWE);
+        wdg::lagrange_interpolation_matrix(n, op->x.data(), nq_edge, xq.data(), tab.data() + wdg::kTabPE);
+        rule(nq_mass, xq, wq);
+        if (!wdg::mass_fast_diag(n, op->x.data(), nq_mass, xq.data(), wq.data(), tab.data() + wdg::kTabA,
+                                 tab.data() + wdg::kTabB, tab.data() + wdg::kTabV, tab.data() + wdg::kTabLam))
+            fail(WDG_ERR_INVALID, "the mass rule is too small for the basis (singular mass matrix)");
         op->exact = true;
         op->nqv = nq_volume;
         op->nqe = nq_edge;
@@ -866,12 +879,6 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, doub
         op->d_extab = static_cast<double*>(dmalloc(sizeof(double) * tab.size()));
         hip_check(hipMemcpy(op->d_extab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice),
                   "upload quadrature tables");
-        const size_t nn2 = (size_t)n * n * n * n;
-        op->d_L = static_cast<double*>(dmalloc(sizeof(double) * nn2 * op->nel));
-        const wdg::ExactMassTables m{op->d_extab + 340, op->d_extab + 350, op->d_extab + 360, nq_mass};
-        wdg::launch_exact_mass_setup(n, op->base_args(), m, op->d_L, nullptr);
-        hip_check(hipGetLastError(), "mass-matrix setup launch");
-        hip_check(hipDeviceSynchronize(), "mass-matrix setup");
     });
     if (rc2) {
         wdg_hip_wave2d_destroy(op);
@@ -955,7 +962,7 @@ static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* str
             KArgs a = op->base_args();
             a.u_in = x;
             a.u_out = y;
-            wdg::launch_exact_mass(op->n, a, nullptr, op->d_L, inv ? -2 : -1, s);
+            wdg::launch_exact_mass(op->n, a, nullptr, op->mass_tables(), inv ? -2 : -1, s);
             hip_check(hipGetLastError(), "mass kernel launch");
             return;
         }

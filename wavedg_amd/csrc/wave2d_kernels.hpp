@@ -627,6 +627,22 @@ void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
+// One table block per handle, offsets in doubles: volume rule x, w, P, D;
+// edge rule x, w, P; mass matrices A, B, V, lam (ExactMassTables).
+enum : int {
+    kTabXV = 0,
+    kTabWV = 10,
+    kTabPV = 20,
+    kTabDV = 120,
+    kTabXE = 220,
+    kTabWE = 230,
+    kTabPE = 240,
+    kTabA = 340,
+    kTabB = 440,
+    kTabV = 540,
+    kTabLam = 640,
+    kTabSize = 650,
+};
 struct ExactTables {
     const double* xv;  // volume rule (Div<false>)
     const double* wv;
@@ -637,21 +653,22 @@ struct ExactTables {
     const double* Pe;
     int nqv, nqe;
 };
-struct ExactMassTables {  // mass-matrix rule (MassMatrix<false>)
-    const double* x;
-    const double* w;
-    const double* B;
-    int nq;
+struct ExactMassTables {  // MassMatrix<false> by fast diagonalisation (host mass_fast_diag)
+    const double* A;    // (n, n) sum_q w_q phi_i phi_j
+    const double* B;    // (n, n) sum_q w_q x_q phi_i phi_j
+    const double* V;    // (n, n) V' A V = I, V' B V = diag(lam)
+    const double* lam;  // (n)
 };
 // a.u_out (+)= Div<false>(a.u_in)
 void launch_exact_volume(int n, const KArgs& a, const ExactTables& t, int accumulate, hipStream_t s);
 // a.u_out += lifted EdgeFlux<false> of interior and/or boundary sides
 void launch_exact_face(int n, const KArgs& a, const ExactTables& t, int interior, int boundary, hipStream_t s);
-// Cholesky factors of the exact element mass matrices into L (n^4 per element)
-void launch_exact_mass_setup(int n, const KArgs& a, const ExactMassTables& m, double* L, hipStream_t s);
 // kid -1: y = M x, -2: y = M^{-1} x; kid >= 0: the RHS / RK-stage epilogue of
 // that kernel id applied to the operator output r
-void launch_exact_mass(int n, const KArgs& a, const double* r, const double* L, int kid, hipStream_t s);
+void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s);
+// The whole RHS / RK stage (kid >= K_RHS) for the default rules (n volume
+// points, n + 1 edge points) in one kernel; tab is the handle's table block.
+void launch_exact_stage(int n, int kid, const KArgs& a, const double* tab, hipStream_t s);
 
 // Host-side launchers, one translation unit per N (wave2d_kernels_n.hip).
 #define WDG_DECLARE_LAUNCH(NV)                                                                              \
