@@ -66,8 +66,9 @@ def build_workload(nx: int):
     return c, mesh, n, bc, u0, dt
 
 
-def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0, autotune: bool = False):
-    wave = capi.Wave2D(mesh, n, bc, precision=precision)
+def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0, autotune: bool = False,
+              approx_quad: bool = True):
+    wave = capi.Wave2D(mesh, n, bc, precision=precision, approx_quad=approx_quad)
     if autotune:
         best, ms = wave.autotune(reps=10)
         wave.tuned = {"schedule": best, "threads_per_workgroup": wave.schedule()[1],
@@ -285,11 +286,11 @@ def cnn_rates(torch, capi, M: int = 64, B: int = 16, steps: int = 50):
             "flops_per_train_step": f_train, "flops_per_forward": f_fwd}
 
 
-def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int):
+def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int, approx_quad: bool = True):
     from oracle import oracle as O
 
     om = O.Mesh(c["vertices"], c["elements"])
-    ow = O.Wave(om, n, bc)
+    ow = O.Wave(om, n, bc, approx_quad=approx_quad)
     O.set_threads(threads)
     t0 = time.perf_counter()
     ow.rk4(u0, dt, steps)
@@ -312,6 +313,9 @@ def main() -> int:
     ap.add_argument("--learned", type=int, default=16,
                     help="held-out problems for the plain vs batched+learned WaveHoltz line (0 = skip)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--exact-steps", type=int, default=400,
+                    help="RK4 steps of the approx_quad = false (exact quadrature) path on the same mesh (0 = skip)")
+    ap.add_argument("--exact-cpu-nx", type=int, default=128, help="mesh size of the exact path's CPU-oracle sample")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
@@ -449,6 +453,43 @@ def main() -> int:
         }
         print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
         del wave32, u32
+
+    if args.exact_steps > 0:
+        wx, ux, secx, ms_kx, nkx = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F64, args.exact_steps,
+                                             min(args.warmup, 4), approx_quad=False)
+        avgx = ms_kx / max(nkx, 1)
+        achx = bytes64 / (avgx * 1e-3) / 1e9
+        out["exact_quadrature"] = {
+            "workload": f"the same mesh, state and dt with approx_quad = false: Div<false> on {n} Gauss-Legendre "
+                        f"points, EdgeFlux<false> on {n + 1}, MassMatrix<false> (rule of {n + 1}); "
+                        f"{args.exact_steps} RK4 steps",
+            "value": ndof * 5 * args.exact_steps / secx,
+            "unit": "DoF*stage/s",
+            "ms_per_step": secx * 1e3 / args.exact_steps,
+            "kernel": "exact_stage_kernel<8> (one fused kernel per RK stage)",
+            "avg_launch_us": avgx * 1e3,
+            "bytes_per_launch": bytes64,
+            "achieved_GBs": achx,
+            "frac": achx / HBM_PEAK_GBS,
+        }
+        print(f"[bench] exact quadrature: {secx:.3f} s, {avgx*1e3:.1f} us/stage ({achx:.0f} GB/s)", file=sys.stderr,
+              flush=True)
+        del wx, ux
+        if args.cpu_steps > 0 and args.exact_cpu_nx > 0:
+            from wavedg_amd import meshgen as _mg
+
+            cs = _mg.config2(nx=args.exact_cpu_nx)
+            ms = capi.Mesh(cs["vertices"], cs["elements"])
+            nodes = ms.nodes(n)
+            us = _mg.nodal_state(nodes, _mg.gaussian_sum(nodes, cs["centers"], cs["sigmas"], cs["amps"]))
+            bcs = np.full(ms.n_boundary, cs["bc_code"], dtype=np.int32)
+            vx, sx = cpu_baseline(cs, n, bcs, us, 0.5 * ms.min_h() / n**2, 1, args.cpu_steps, approx_quad=False)
+            out["exact_quadrature"]["cpu_baseline"] = {
+                "value": vx, "unit": "DoF*stage/s", "cores": 1, "kind": "port",
+                "sample": f"CPU oracle (block Cholesky mass matrix as the reference), same workload type on "
+                          f"{args.exact_cpu_nx}^2 quads, {args.cpu_steps} RK4 step(s), {sx:.1f} s",
+            }
+            print(f"[bench] exact cpu: {vx:.3e} DoF*stage/s", file=sys.stderr, flush=True)
 
     if args.learned > 0:
         from wavedg_amd import warmstart

@@ -109,12 +109,13 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int *bc, double a, 
                           wdg_wave2d *out);
 /* The same handle with approx_quad = false: Div<false> (Div.cpp:312-360,
  * 469-588), EdgeFlux<false> (EdgeFlux.cpp:375-620) and MassMatrix<false>
- * (MassMatrix.cpp:85-227: block mass matrix, Cholesky per element), FP64,
- * with Gauss-Legendre rules of nq_volume, nq_edge and nq_mass points (<= 10;
+ * (MassMatrix.cpp:85-227, the exact block mass matrix), FP64, with
+ * Gauss-Legendre rules of nq_volume, nq_edge and nq_mass points (<= 10;
  * 0 = the reference's default for bilinear elements: n, n + 1, n + 1).  Every
  * operator, RK and WaveHoltz entry point then uses these operators (DivF and
- * the schedule autotuner are for approx_quad = true handles).  Memory: n^4
- * doubles per element for the mass-matrix factors. */
+ * the schedule autotuner are for approx_quad = true handles).  The mass matrix
+ * is applied and inverted from the element's affine det J (no per-element
+ * factors are stored); a mass rule too small for the basis is refused. */
 int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int *bc, double a, double b, int nq_volume,
                                 int nq_edge, int nq_mass, wdg_wave2d *out);
 int wdg_hip_wave2d_quadrature(wdg_wave2d op, int *exact, int *nq_volume, int *nq_edge, int *nq_mass);

@@ -118,3 +118,13 @@ def test_exact_rk4_and_waveholtz_run():
     u1, t = w.rk4(u, 1e-3, 3)
     assert t == pytest.approx(3e-3)
     assert np.all(np.isfinite(u1)) and np.linalg.norm(u1 - u) > 0
+
+
+@pytest.mark.parametrize("n", [3, 6])
+def test_exact_mass_inverse_inverts(n):
+    """solve_chol undoes mult_chol: M^{-1}(M u) = u on a skewed mesh."""
+    v, e = mesh_case("skewed6x5")
+    w = O.Wave(O.Mesh(v, e), n, approx_quad=False)
+    u = random_state(w.ndof, 7)
+    back = w.mass_inv(w.mass_action(u))
+    assert np.linalg.norm(back - u) <= 1e-13 * np.linalg.norm(u)

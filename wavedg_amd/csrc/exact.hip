@@ -1,8 +1,10 @@
 // Exact-quadrature operators (approx_quad = false) in FP64: Div<false>
 // (Div.cpp:312-360, 469-588), EdgeFlux<false> on Lobatto face traces
 // (EdgeFlux.cpp:375-420, 491-620) and MassMatrix<false> (MassMatrix.cpp:
-// 85-133, 193-227).  One element per workgroup throughout; tables are
-// column-major (n_quad, n) with P(q, k) = phi_k(x_q) and D(q, k) = phi_k'(x_q).
+// 85-133, 193-227).  Tables are column-major (n_quad, n) with
+// P(q, k) = phi_k(x_q) and D(q, k) = phi_k'(x_q).  The separate volume, face
+// and mass kernels take one element per workgroup and serve any rule sizes;
+// exact_stage_kernel fuses a whole RHS / RK stage for the default rules.
 //
 //  * volume: u -> quadrature points by the two tensor-product passes of
 //    div_2d_q, contravariant fluxes there, and back with D and P;
@@ -335,14 +337,23 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
 //   RA: u -> u at quadrature points -> edge-point traces -> t1 -> t3
 //   RB: xi-interpolated u -> (Df, Pg) -> staged traces -> edge fluxes -> t2
 //   RC: operator output r -> M^{-1} (r + s F)
+// Every array a pass reads along a line of lanes has an odd stride in
+// doubles (the L* pitches below), so a wave's reads spread over the LDS banks.
 template <int N>
 struct StageShape {
     static constexpr int cmax(int x, int y) { return x > y ? x : y; }
+    static constexpr int odd(int x) { return x | 1; }
     static constexpr int QV = N, QE = N + 1, NN = N * N;
     static constexpr int E = cmax(64 / (3 * N), 1);
-    static constexpr int SA = cmax(cmax(3 * NN, 3 * QV * QV), 24 * QE);
-    static constexpr int SB = cmax(cmax(3 * N * QV, 6 * QV * N), cmax(cmax(24 * N, 12 * QE), 3 * NN));
-    static constexpr int SC = 3 * NN;
+    static constexpr int LS = odd(3 * N);    // (3, N, N) arrays: d + 3 i + LS j
+    static constexpr int LQ = odd(3 * QV);   // u at the volume points: d + 3 i + LQ j
+    static constexpr int LP = odd(N);        // xi-interpolated u: (d + 3 i) LP + l
+    static constexpr int LD = odd(QV);       // Df, Pg: (d + 3 k) LD + j
+    static constexpr int LT = odd(N);        // staged side traces: row LT
+    static constexpr int LE = odd(QE);       // edge-point traces and fluxes: row LE
+    static constexpr int SA = cmax(cmax(LS * N, LQ * QV), 24 * LE);
+    static constexpr int SB = cmax(cmax(3 * QV * LP, 6 * N * LD), cmax(cmax(24 * LT, 12 * LE), LS * N));
+    static constexpr int SC = LS * N;
     static constexpr int S = SA + SB + SC;
 };
 
@@ -351,6 +362,7 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
 {
     using Sh = StageShape<N>;
     constexpr int QV = Sh::QV, QE = Sh::QE, NN = Sh::NN, E = Sh::E;
+    constexpr int LS = Sh::LS, LQ = Sh::LQ, LP = Sh::LP, LD = Sh::LD, LT = Sh::LT, LE = Sh::LE;
     __shared__ double s_mem[E * Sh::S];
     const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
     const int mode = kModeOf[kid];
@@ -358,10 +370,11 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     auto RA = [&](int e) { return s_mem + e * Sh::S; };
     auto RB = [&](int e) { return s_mem + e * Sh::S + Sh::SA; };
     auto RC = [&](int e) { return s_mem + e * Sh::S + Sh::SA + Sh::SB; };
+    auto sidx = [](int d, int i, int j) { return d + 3 * i + LS * j; };
 
     for (int it = lane; it < E * 3 * NN; it += 64) {
         const int e = it / (3 * NN), k = it % (3 * NN);
-        if (e0 + e < nel) RA(e)[k] = u[(size_t)(e0 + e) * 3 * NN + k];
+        if (e0 + e < nel) RA(e)[k + (LS - 3 * N) * (k / (3 * N))] = u[(size_t)(e0 + e) * 3 * NN + k];
     }
     __syncthreads();
     // ---- volume: the passes of div_2d_q (Div.cpp:469-588)
@@ -372,14 +385,14 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const double* U = RA(e);
         double x[N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) x[k] = U[d + 3 * (k + N * l)];
+        for (int k = 0; k < N; ++k) x[k] = U[sidx(d, k, l)];
         double* PU = RB(e);
-#pragma unroll
+#pragma unroll 2
         for (int i = 0; i < QV; ++i) {
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < N; ++k) acc += tab[kTabPV + i + QV * k] * x[k];
-            PU[l + N * (d + 3 * i)] = acc;
+            PU[(d + 3 * i) * LP + l] = acc;
         }
     }
     __syncthreads();
@@ -387,17 +400,17 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     for (int it = lane; it < E * 3 * QV; it += 64) {
         const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, i = j / 3;
         if (e0 + e >= nel) continue;
-        const double* PU = RB(e);
+        const double* PU = RB(e) + (d + 3 * i) * LP;
         double x[N];
 #pragma unroll
-        for (int l = 0; l < N; ++l) x[l] = PU[l + N * (d + 3 * i)];
+        for (int l = 0; l < N; ++l) x[l] = PU[l];
         double* UQ = RA(e);
-#pragma unroll
+#pragma unroll 2
         for (int jq = 0; jq < QV; ++jq) {
             double acc = 0.0;
 #pragma unroll
             for (int l = 0; l < N; ++l) acc += tab[kTabPV + jq + QV * l] * x[l];
-            UQ[d + 3 * (i + QV * jq)] = acc;
+            UQ[d + 3 * i + LQ * jq] = acc;
         }
     }
     __syncthreads();
@@ -409,17 +422,16 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const double* g = a.geom + 6 * (size_t)(e0 + e);
         const double ax = g[0], ay = g[1], cx = g[2], cy = g[3], hx = g[4], hy = g[5];
         const double eta = tab[kTabXV + jq], wj = tab[kTabWV + jq];
-        const double* UQ = RA(e);
+        const double* UQ = RA(e) + LQ * jq;
         double df[N], pg[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) df[k] = pg[k] = 0.0;
-#pragma unroll
+#pragma unroll 2
         for (int i = 0; i < QV; ++i) {
             const double xi = tab[kTabXV + i], W = tab[kTabWV + i] * wj;
             const double X_xi = (ax + hx * eta) * W, Y_xi = (ay + hy * eta) * W;
             const double X_eta = (cx + hx * xi) * W, Y_eta = (cy + hy * xi) * W;
-            const double* q = UQ + 3 * (i + QV * jq);
-            const double p = q[0], uu = q[1], vv = q[2];
+            const double p = UQ[3 * i], uu = UQ[3 * i + 1], vv = UQ[3 * i + 2];
             const double s1 = d == 0 ? uu : (d == 1 ? p : 0.0);
             const double s2 = d == 0 ? vv : (d == 2 ? p : 0.0);
             const double F = Y_eta * s1 - X_eta * s2, G = -Y_xi * s1 + X_xi * s2;
@@ -432,8 +444,8 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         double* DP = RB(e);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            DP[jq + QV * (d + 3 * k)] = df[k];
-            DP[3 * N * QV + jq + QV * (d + 3 * k)] = pg[k];
+            DP[(d + 3 * k) * LD + jq] = df[k];
+            DP[3 * N * LD + (d + 3 * k) * LD + jq] = pg[k];
         }
     }
     __syncthreads();
@@ -441,20 +453,20 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     for (int it = lane; it < E * 3 * N; it += 64) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, k = j / 3;
         if (e0 + e >= nel) continue;
-        const double* DP = RB(e);
+        const double* DP = RB(e) + (d + 3 * k) * LD;
         double df[QV], pg[QV];
 #pragma unroll
         for (int jq = 0; jq < QV; ++jq) {
-            df[jq] = DP[jq + QV * (d + 3 * k)];
-            pg[jq] = DP[3 * N * QV + jq + QV * (d + 3 * k)];
+            df[jq] = DP[jq];
+            pg[jq] = DP[3 * N * LD + jq];
         }
         double* R = RC(e);
-#pragma unroll
+#pragma unroll 2
         for (int l = 0; l < N; ++l) {
             double acc = 0.0;
 #pragma unroll
             for (int jq = 0; jq < QV; ++jq) acc += tab[kTabPV + jq + QV * l] * df[jq] + tab[kTabDV + jq + QV * l] * pg[jq];
-            R[d + 3 * (k + N * l)] = acc;
+            R[sidx(d, k, l)] = acc;
         }
     }
     __syncthreads();
@@ -477,25 +489,25 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
             side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
             boundary_state((inf >> 8) & 0xff, n0, n1, pA, uA, vA, pB, uB, vB);
         }
-        double* T = RB(e) + 6 * N * s;  // (own | exterior, d, k)
-        T[0 * N + k] = pA;
-        T[1 * N + k] = uA;
-        T[2 * N + k] = vA;
-        T[3 * N + k] = pB;
-        T[4 * N + k] = uB;
-        T[5 * N + k] = vB;
+        double* T = RB(e) + 6 * LT * s + k;  // rows: own p, u, v, exterior p, u, v
+        T[0 * LT] = pA;
+        T[1 * LT] = uA;
+        T[2 * LT] = vA;
+        T[3 * LT] = pB;
+        T[4 * LT] = uB;
+        T[5 * LT] = vB;
     }
     __syncthreads();
     // traces at the edge points
     for (int it = lane; it < E * 24; it += 64) {
-        const int e = it / 24, j = it % 24;  // j = 6 s + 3 (own | exterior) + d
+        const int e = it / 24, j = it % 24;  // row j = 6 s + 3 (own | exterior) + d
         if (e0 + e >= nel) continue;
-        const double* T = RB(e) + N * j;
+        const double* T = RB(e) + LT * j;
         double x[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) x[k] = T[k];
-        double* Q = RA(e) + QE * j;
-#pragma unroll
+        double* Q = RA(e) + LE * j;
+#pragma unroll 2
         for (int q = 0; q < QE; ++q) {
             double acc = 0.0;
 #pragma unroll
@@ -510,15 +522,15 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         if (ge >= nel) continue;
         double n0, n1, ds;
         side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
-        const double* Q = RA(e) + 6 * QE * s;
-        const double A0 = Q[0 * QE + q], A1 = Q[1 * QE + q], A2 = Q[2 * QE + q];
-        const double B0 = Q[3 * QE + q], B1 = Q[4 * QE + q], B2 = Q[5 * QE + q];
+        const double* Q = RA(e) + 6 * LE * s + q;
+        const double A0 = Q[0 * LE], A1 = Q[1 * LE], A2 = Q[2 * LE];
+        const double B0 = Q[3 * LE], B1 = Q[4 * LE], B2 = Q[5 * LE];
         const double wds = tab[kTabWE + q] * ds;
-        double* Cf = RB(e) + 3 * QE * s;
-        Cf[0 * QE + q] = wds * (a.ha * (n0 * (A1 + B1) + n1 * (A2 + B2)) + a.hb * (A0 - B0));
+        double* Cf = RB(e) + 3 * LE * s + q;
+        Cf[0 * LE] = wds * (a.ha * (n0 * (A1 + B1) + n1 * (A2 + B2)) + a.hb * (A0 - B0));
         const double cf = wds * (a.ha * (A0 + B0) + a.hb * (n0 * (A1 - B1) + n1 * (A2 - B2)));
-        Cf[1 * QE + q] = n0 * cf;
-        Cf[2 * QE + q] = n1 * cf;
+        Cf[1 * LE] = n0 * cf;
+        Cf[2 * LE] = n1 * cf;
     }
     __syncthreads();
     // projection onto the side's nodes, added to r: sides 0 and 2 first, then
@@ -528,17 +540,19 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int it = lane; it < E * 12; it += 64) {
             const int e = it / 12, j = it % 12, s = j / 3, d = j % 3;
             if ((s & 1) != pass || e0 + e >= nel) continue;
-            const double* Cf = RB(e) + QE * (3 * s + d);
+            const double* Cf = RB(e) + LE * (3 * s + d);
             double c[QE];
 #pragma unroll
             for (int q = 0; q < QE; ++q) c[q] = Cf[q];
             double* R = RC(e);
-#pragma unroll
+            // node k of side s is (i0 + di k, j0 + dj k)
+            const int i0 = s == 1 ? N - 1 : 0, j0 = s == 2 ? N - 1 : 0, di = (s & 1) ? 0 : 1, dj = (s & 1) ? 1 : 0;
+#pragma unroll 2
             for (int k = 0; k < N; ++k) {
                 double v = 0.0;
 #pragma unroll
                 for (int q = 0; q < QE; ++q) v += c[q] * tab[kTabPE + q + QE * k];
-                R[d + 3 * side_node<N>(s, k)] += v;
+                R[sidx(d, i0 + di * k, j0 + dj * k)] += v;
             }
         }
         __syncthreads();
@@ -553,17 +567,17 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         double x[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            double v = R[d + 3 * (i + N * jn)];
+            double v = R[sidx(d, i, jn)];
             if (force) v += static_cast<const double*>(a.force)[(size_t)ge * 3 * NN + d + 3 * (i + N * jn)] * a.fs;
             x[i] = v;
         }
         double* T1 = RA(e);
-#pragma unroll
+#pragma unroll 2
         for (int aa = 0; aa < N; ++aa) {
             double t = 0.0;
 #pragma unroll
             for (int i = 0; i < N; ++i) t += tab[kTabV + i + N * aa] * x[i];
-            T1[d + 3 * (aa + N * jn)] = t;
+            T1[sidx(d, aa, jn)] = t;
         }
     }
     __syncthreads();
@@ -577,14 +591,14 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const double* T1 = RA(e);
         double x[N];
 #pragma unroll
-        for (int jn = 0; jn < N; ++jn) x[jn] = T1[d + 3 * (aa + N * jn)];
+        for (int jn = 0; jn < N; ++jn) x[jn] = T1[sidx(d, aa, jn)];
         double* T2 = RB(e);
-#pragma unroll
+#pragma unroll 2
         for (int b = 0; b < N; ++b) {
             double t = 0.0;
 #pragma unroll
             for (int jn = 0; jn < N; ++jn) t += tab[kTabV + jn + N * b] * x[jn];
-            T2[d + 3 * (aa + N * b)] = t * recip(da + d2 * tab[kTabLam + b]);
+            T2[sidx(d, aa, b)] = t * recip(da + d2 * tab[kTabLam + b]);
         }
     }
     __syncthreads();
@@ -595,14 +609,14 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const double* T2 = RB(e);
         double x[N];
 #pragma unroll
-        for (int aa = 0; aa < N; ++aa) x[aa] = T2[d + 3 * (aa + N * b)];
+        for (int aa = 0; aa < N; ++aa) x[aa] = T2[sidx(d, aa, b)];
         double* T3 = RA(e);
-#pragma unroll
+#pragma unroll 2
         for (int i = 0; i < N; ++i) {
             double t = 0.0;
 #pragma unroll
             for (int aa = 0; aa < N; ++aa) t += tab[kTabV + i + N * aa] * x[aa];
-            T3[d + 3 * (i + N * b)] = t;
+            T3[sidx(d, i, b)] = t;
         }
     }
     __syncthreads();
@@ -613,14 +627,14 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const double* T3 = RA(e);
         double x[N];
 #pragma unroll
-        for (int b = 0; b < N; ++b) x[b] = T3[d + 3 * (i + N * b)];
+        for (int b = 0; b < N; ++b) x[b] = T3[sidx(d, i, b)];
         double* Y = RC(e);
-#pragma unroll
+#pragma unroll 2
         for (int jn = 0; jn < N; ++jn) {
             double t = 0.0;
 #pragma unroll
             for (int b = 0; b < N; ++b) t += tab[kTabV + jn + N * b] * x[b];
-            Y[d + 3 * (i + N * jn)] = t;
+            Y[sidx(d, i, jn)] = t;
         }
     }
     __syncthreads();
@@ -630,7 +644,7 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         const int e = it / (3 * NN), k = it % (3 * NN);
         if (e0 + e >= nel) continue;
         const size_t g = (size_t)(e0 + e) * 3 * NN + k;
-        const double rhs = RC(e)[k];
+        const double rhs = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
         if (out == M_OUT_RHS) {
             static_cast<double*>(a.u_out)[g] = rhs;
             continue;

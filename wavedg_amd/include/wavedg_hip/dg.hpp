@@ -296,8 +296,8 @@ private:
 
 // MassMatrix<true> (MassMatrix.hpp:77, MassMatrix.cpp:6-82), the diagonal
 // w_i w_j det J, and MassMatrix<false> (MassMatrix.cpp:85-227), the exact
-// block matrix held as per-element Cholesky factors; n_var is 3.  quad is
-// ignored for <true>, as in the reference.
+// block matrix (applied and inverted on the GPU without stored factors);
+// n_var is 3.  quad is ignored for <true>, as in the reference.
 template <bool ApproxQuadrature>
 class MassMatrix : public InvertibleOperator {
 public:
