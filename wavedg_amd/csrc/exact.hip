@@ -329,13 +329,13 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
 
 // ---------------------------------------------------------------- fused stage
 // The whole right-hand side / RK stage for the default rules (QV = N volume,
-// QE = N + 1 edge points) in one kernel.  E elements share a 64-lane
+// QE = N + 1 edge points) in one kernel.  E elements share an NT-lane
 // workgroup; every tensor pass gives one lane one (element, variable, line)
 // and keeps the line in registers, so the 1D tables are uniform operands read
-// through the scalar cache.  Per element three LDS regions are reused phase by
-// phase:
+// through the scalar cache, from row-contiguous copies.  Per element three
+// LDS regions are reused phase by phase:
 //   RA: u -> u at quadrature points -> edge-point traces -> t1 -> t3
-//   RB: xi-interpolated u -> (Df, Pg) -> staged traces -> edge fluxes -> t2
+//   RB: xi-interpolated u -> Df, then Pg -> staged traces -> edge fluxes -> t2
 //   RC: operator output r -> M^{-1} (r + s F)
 // Every array a pass reads along a line of lanes has an odd stride in
 // doubles (the L* pitches below), so a wave's reads spread over the LDS banks.
@@ -344,26 +344,30 @@ struct StageShape {
     static constexpr int cmax(int x, int y) { return x > y ? x : y; }
     static constexpr int odd(int x) { return x | 1; }
     static constexpr int QV = N, QE = N + 1, NN = N * N;
-    static constexpr int E = cmax(64 / (3 * N), 1);
+    static constexpr int NT = (N == 6 || N == 8) ? 128 : 64;  // fills the lanes best
+    static constexpr int E = cmax(NT / (3 * N), 1);
     static constexpr int LS = odd(3 * N);    // (3, N, N) arrays: d + 3 i + LS j
     static constexpr int LQ = odd(3 * QV);   // u at the volume points: d + 3 i + LQ j
     static constexpr int LP = odd(N);        // xi-interpolated u: (d + 3 i) LP + l
-    static constexpr int LD = odd(QV);       // Df, Pg: (d + 3 k) LD + j
+    static constexpr int LD = odd(QV);       // Df or Pg: (d + 3 k) LD + j
     static constexpr int LT = odd(N);        // staged side traces: row LT
     static constexpr int LE = odd(QE);       // edge-point traces and fluxes: row LE
     static constexpr int SA = cmax(cmax(LS * N, LQ * QV), 24 * LE);
-    static constexpr int SB = cmax(cmax(3 * QV * LP, 6 * N * LD), cmax(cmax(24 * LT, 12 * LE), LS * N));
+    static constexpr int SB = cmax(cmax(3 * QV * LP, 3 * N * LD), cmax(cmax(24 * LT, 12 * LE), LS * N));
     static constexpr int SC = LS * N;
     static constexpr int S = SA + SB + SC;
 };
 
 template <int N>
-__global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const double* __restrict__ tab, int kid)
+__global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KArgs a, const double* __restrict__ tab,
+                                                                         int kid)
 {
     using Sh = StageShape<N>;
-    constexpr int QV = Sh::QV, QE = Sh::QE, NN = Sh::NN, E = Sh::E;
+    constexpr int QV = Sh::QV, QE = Sh::QE, NN = Sh::NN, E = Sh::E, NT = Sh::NT;
     constexpr int LS = Sh::LS, LQ = Sh::LQ, LP = Sh::LP, LD = Sh::LD, LT = Sh::LT, LE = Sh::LE;
+    constexpr int NS = (E * 3 * NN + NT - 1) / NT;  // state values per lane
     __shared__ double s_mem[E * Sh::S];
+    __shared__ double s_geo[E * 6];
     const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
     const int mode = kModeOf[kid];
     const double* __restrict__ u = static_cast<const double*>(a.u_in);
@@ -371,15 +375,27 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     auto RB = [&](int e) { return s_mem + e * Sh::S + Sh::SA; };
     auto RC = [&](int e) { return s_mem + e * Sh::S + Sh::SA + Sh::SB; };
     auto sidx = [](int d, int i, int j) { return d + 3 * i + LS * j; };
-
-    for (int it = lane; it < E * 3 * NN; it += 64) {
-        const int e = it / (3 * NN), k = it % (3 * NN);
-        if (e0 + e < nel) RA(e)[k + (LS - 3 * N) * (k / (3 * N))] = u[(size_t)(e0 + e) * 3 * NN + k];
+    // the block's elements are contiguous in HBM: every load is issued before
+    // the first wait
+    const int nvalid = nel - e0 < E ? nel - e0 : E;
+    const double* ublk = u + (size_t)e0 * 3 * NN;
+    double v[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int it = lane + r * NT;
+        v[r] = it < nvalid * 3 * NN ? ublk[it] : 0.0;
     }
+    const double gv = lane < nvalid * 6 ? a.geom[6 * (size_t)e0 + lane] : 0.0;
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
+        if (it < nvalid * 3 * NN) RA(e)[k + (LS - 3 * N) * (k / (3 * N))] = v[r];
+    }
+    if (lane < nvalid * 6) s_geo[lane] = gv;
     __syncthreads();
     // ---- volume: the passes of div_2d_q (Div.cpp:469-588)
     // PU(l, d, i) = sum_k P(i, k) u(d, k, l)
-    for (int it = lane; it < E * 3 * N; it += 64) {
+    for (int it = lane; it < E * 3 * N; it += NT) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, l = j / 3;
         if (e0 + e >= nel) continue;
         const double* U = RA(e);
@@ -391,13 +407,13 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int i = 0; i < QV; ++i) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) acc += tab[kTabPV + i + QV * k] * x[k];
+            for (int k = 0; k < N; ++k) acc += tab[kTabPVT + k + N * i] * x[k];
             PU[(d + 3 * i) * LP + l] = acc;
         }
     }
     __syncthreads();
     // UQ(d, i, j) = sum_l P(j, l) PU(l, d, i)
-    for (int it = lane; it < E * 3 * QV; it += 64) {
+    for (int it = lane; it < E * 3 * QV; it += NT) {
         const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, i = j / 3;
         if (e0 + e >= nel) continue;
         const double* PU = RB(e) + (d + 3 * i) * LP;
@@ -409,97 +425,116 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int jq = 0; jq < QV; ++jq) {
             double acc = 0.0;
 #pragma unroll
-            for (int l = 0; l < N; ++l) acc += tab[kTabPV + jq + QV * l] * x[l];
+            for (int l = 0; l < N; ++l) acc += tab[kTabPVT + l + N * jq] * x[l];
             UQ[d + 3 * i + LQ * jq] = acc;
         }
     }
     __syncthreads();
-    // contravariant fluxes of f = (u, p, 0), g = (v, 0, p) at the points and
-    // Df(j, d, k) = sum_i D(i, k) F(d, i, j), Pg(j, d, k) = sum_i P(i, k) G(d, i, j)
-    for (int it = lane; it < E * 3 * QV; it += 64) {
-        const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, jq = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* g = a.geom + 6 * (size_t)(e0 + e);
-        const double ax = g[0], ay = g[1], cx = g[2], cy = g[3], hx = g[4], hy = g[5];
-        const double eta = tab[kTabXV + jq], wj = tab[kTabWV + jq];
-        const double* UQ = RA(e) + LQ * jq;
-        double df[N], pg[N];
+    // contravariant fluxes of f = (u, p, 0), g = (v, 0, p) at the points, then
+    // Df(j, d, k) = sum_i D(i, k) F(d, i, j)   -> r  = sum_j P(j, l) Df(j, d, k)
+    // Pg(j, d, k) = sum_i P(i, k) G(d, i, j)   -> r += sum_j D(j, l) Pg(j, d, k)
+    // in two rounds, so one buffer holds Df, then Pg.
 #pragma unroll
-        for (int k = 0; k < N; ++k) df[k] = pg[k] = 0.0;
+    for (int half = 0; half < 2; ++half) {
+        for (int it = lane; it < E * 3 * QV; it += NT) {
+            const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, jq = j / 3;
+            if (e0 + e >= nel) continue;
+            const double* g = s_geo + 6 * e;
+            const double ax = g[0], ay = g[1], cx = g[2], cy = g[3], hx = g[4], hy = g[5];
+            const double eta = tab[kTabXV + jq], wj = tab[kTabWV + jq];
+            const double* UQ = RA(e) + LQ * jq;
+            double acc[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc[k] = 0.0;
 #pragma unroll 2
-        for (int i = 0; i < QV; ++i) {
-            const double xi = tab[kTabXV + i], W = tab[kTabWV + i] * wj;
-            const double X_xi = (ax + hx * eta) * W, Y_xi = (ay + hy * eta) * W;
-            const double X_eta = (cx + hx * xi) * W, Y_eta = (cy + hy * xi) * W;
-            const double p = UQ[3 * i], uu = UQ[3 * i + 1], vv = UQ[3 * i + 2];
-            const double s1 = d == 0 ? uu : (d == 1 ? p : 0.0);
-            const double s2 = d == 0 ? vv : (d == 2 ? p : 0.0);
-            const double F = Y_eta * s1 - X_eta * s2, G = -Y_xi * s1 + X_xi * s2;
+            for (int i = 0; i < QV; ++i) {
+                const double xi = tab[kTabXV + i], W = tab[kTabWV + i] * wj;
+                const double p = UQ[3 * i], uu = UQ[3 * i + 1], vv = UQ[3 * i + 2];
+                const double s1 = d == 0 ? uu : (d == 1 ? p : 0.0);
+                const double s2 = d == 0 ? vv : (d == 2 ? p : 0.0);
+                double f;
+                if (half == 0) {
+                    const double X_eta = (cx + hx * xi) * W, Y_eta = (cy + hy * xi) * W;
+                    f = Y_eta * s1 - X_eta * s2;
+                } else {
+                    const double X_xi = (ax + hx * eta) * W, Y_xi = (ay + hy * eta) * W;
+                    f = -Y_xi * s1 + X_xi * s2;
+                }
 #pragma unroll
-            for (int k = 0; k < N; ++k) {
-                df[k] += tab[kTabDV + i + QV * k] * F;
-                pg[k] += tab[kTabPV + i + QV * k] * G;
+                for (int k = 0; k < N; ++k) acc[k] += tab[(half == 0 ? kTabDVT : kTabPVT) + k + N * i] * f;
+            }
+            double* DP = RB(e);
+#pragma unroll
+            for (int k = 0; k < N; ++k) DP[(d + 3 * k) * LD + jq] = acc[k];
+        }
+        __syncthreads();
+        for (int it = lane; it < E * 3 * N; it += NT) {
+            const int e = it / (3 * N), j = it % (3 * N), d = j % 3, k = j / 3;
+            if (e0 + e >= nel) continue;
+            const double* DP = RB(e) + (d + 3 * k) * LD;
+            double x[QV];
+#pragma unroll
+            for (int jq = 0; jq < QV; ++jq) x[jq] = DP[jq];
+            double* R = RC(e);
+#pragma unroll 2
+            for (int l = 0; l < N; ++l) {
+                double acc = 0.0;
+#pragma unroll
+                for (int jq = 0; jq < QV; ++jq) acc += tab[(half == 0 ? kTabPV : kTabDV) + jq + QV * l] * x[jq];
+                if (half == 0)
+                    R[sidx(d, k, l)] = acc;
+                else
+                    R[sidx(d, k, l)] += acc;
             }
         }
-        double* DP = RB(e);
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            DP[(d + 3 * k) * LD + jq] = df[k];
-            DP[3 * N * LD + (d + 3 * k) * LD + jq] = pg[k];
-        }
+        __syncthreads();
     }
-    __syncthreads();
-    // r(d, k, l) = sum_j P(j, l) Df(j, d, k) + D(j, l) Pg(j, d, k)
-    for (int it = lane; it < E * 3 * N; it += 64) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, k = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* DP = RB(e) + (d + 3 * k) * LD;
-        double df[QV], pg[QV];
-#pragma unroll
-        for (int jq = 0; jq < QV; ++jq) {
-            df[jq] = DP[jq];
-            pg[jq] = DP[3 * N * LD + jq];
-        }
-        double* R = RC(e);
-#pragma unroll 2
-        for (int l = 0; l < N; ++l) {
-            double acc = 0.0;
-#pragma unroll
-            for (int jq = 0; jq < QV; ++jq) acc += tab[kTabPV + jq + QV * l] * df[jq] + tab[kTabDV + jq + QV * l] * pg[jq];
-            R[sidx(d, k, l)] = acc;
-        }
-    }
-    __syncthreads();
     // ---- faces (EdgeFlux<false>): own and exterior traces at the side's nodes
-    for (int it = lane; it < E * 4 * N; it += 64) {
-        const int e = it / (4 * N), j = it % (4 * N), s = j / N, k = j % N, ge = e0 + e;
-        if (ge >= nel) continue;
-        const int nb = a.nbr[4 * (size_t)ge + s], inf = a.info[4 * (size_t)ge + s];
-        const double* qa = u + ((size_t)ge * NN + side_node<N>(s, k)) * 3;
-        const double pA = qa[0], uA = qa[1], vA = qa[2];
-        double pB, uB, vB;
-        if (nb >= 0) {
-            const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-            const double* qb = u + ((size_t)nb * NN + side_node<N>(inf & 3, kk)) * 3;
-            pB = qb[0];
-            uB = qb[1];
-            vB = qb[2];
-        } else {
-            double n0, n1, ds;
-            side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
-            boundary_state((inf >> 8) & 0xff, n0, n1, pA, uA, vA, pB, uB, vB);
+    {
+        constexpr int NF = (E * 4 * N + NT - 1) / NT;
+        int nb[NF], inf[NF];
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int it = lane + r * NT, e = it / (4 * N), s = (it % (4 * N)) / N;
+            const bool ok = it < nvalid * 4 * N;
+            nb[r] = ok ? a.nbr[4 * (size_t)(e0 + e) + s] : -1;
+            inf[r] = ok ? a.info[4 * (size_t)(e0 + e) + s] : 0;
         }
-        double* T = RB(e) + 6 * LT * s + k;  // rows: own p, u, v, exterior p, u, v
-        T[0 * LT] = pA;
-        T[1 * LT] = uA;
-        T[2 * LT] = vA;
-        T[3 * LT] = pB;
-        T[4 * LT] = uB;
-        T[5 * LT] = vB;
+        double tr[NF][6];
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int it = lane + r * NT, e = it / (4 * N), j = it % (4 * N), s = j / N, k = j % N;
+            if (it >= nvalid * 4 * N) continue;
+            const double* qa = u + ((size_t)(e0 + e) * NN + side_node<N>(s, k)) * 3;
+            tr[r][0] = qa[0];
+            tr[r][1] = qa[1];
+            tr[r][2] = qa[2];
+            if (nb[r] >= 0) {
+                const int kk = ((inf[r] >> 2) & 1) ? (N - 1 - k) : k;
+                const double* qb = u + ((size_t)nb[r] * NN + side_node<N>(inf[r] & 3, kk)) * 3;
+                tr[r][3] = qb[0];
+                tr[r][4] = qb[1];
+                tr[r][5] = qb[2];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NF; ++r) {
+            const int it = lane + r * NT, e = it / (4 * N), j = it % (4 * N), s = j / N, k = j % N;
+            if (it >= nvalid * 4 * N) continue;
+            if (nb[r] < 0) {
+                double n0, n1, ds;
+                side_normal(s_geo + 6 * e, s, n0, n1, ds);
+                boundary_state((inf[r] >> 8) & 0xff, n0, n1, tr[r][0], tr[r][1], tr[r][2], tr[r][3], tr[r][4],
+                               tr[r][5]);
+            }
+            double* T = RB(e) + 6 * LT * s + k;  // rows: own p, u, v, exterior p, u, v
+#pragma unroll
+            for (int c = 0; c < 6; ++c) T[c * LT] = tr[r][c];
+        }
     }
     __syncthreads();
     // traces at the edge points
-    for (int it = lane; it < E * 24; it += 64) {
+    for (int it = lane; it < E * 24; it += NT) {
         const int e = it / 24, j = it % 24;  // row j = 6 s + 3 (own | exterior) + d
         if (e0 + e >= nel) continue;
         const double* T = RB(e) + LT * j;
@@ -511,17 +546,17 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int q = 0; q < QE; ++q) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) acc += x[k] * tab[kTabPE + q + QE * k];
+            for (int k = 0; k < N; ++k) acc += x[k] * tab[kTabPET + k + N * q];
             Q[q] = acc;
         }
     }
     __syncthreads();
     // closed-form flux at every edge point, weighted by w ds
-    for (int it = lane; it < E * 4 * QE; it += 64) {
+    for (int it = lane; it < E * 4 * QE; it += NT) {
         const int e = it / (4 * QE), j = it % (4 * QE), s = j / QE, q = j % QE, ge = e0 + e;
         if (ge >= nel) continue;
         double n0, n1, ds;
-        side_normal(a.geom + 6 * (size_t)ge, s, n0, n1, ds);
+        side_normal(s_geo + 6 * e, s, n0, n1, ds);
         const double* Q = RA(e) + 6 * LE * s + q;
         const double A0 = Q[0 * LE], A1 = Q[1 * LE], A2 = Q[2 * LE];
         const double B0 = Q[3 * LE], B1 = Q[4 * LE], B2 = Q[5 * LE];
@@ -537,7 +572,7 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     // 1 and 3, so each corner has one writer per pass
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-        for (int it = lane; it < E * 12; it += 64) {
+        for (int it = lane; it < E * 12; it += NT) {
             const int e = it / 12, j = it % 12, s = j / 3, d = j % 3;
             if ((s & 1) != pass || e0 + e >= nel) continue;
             const double* Cf = RB(e) + LE * (3 * s + d);
@@ -560,7 +595,7 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     // ---- M^{-1} (r + s F) by fast diagonalisation
     const bool force = mode & M_FORCE;
     // t1(d, a, j) = sum_i V(i, a) x(d, i, j)
-    for (int it = lane; it < E * 3 * N; it += 64) {
+    for (int it = lane; it < E * 3 * N; it += NT) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, jn = j / 3, ge = e0 + e;
         if (ge >= nel) continue;
         const double* R = RC(e);
@@ -582,11 +617,11 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     }
     __syncthreads();
     // t2(d, a, b) = sum_j V(j, b) t1(d, a, j) / (d0 + d1 lam_a + d2 lam_b)
-    for (int it = lane; it < E * 3 * N; it += 64) {
+    for (int it = lane; it < E * 3 * N; it += NT) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, aa = j / 3, ge = e0 + e;
         if (ge >= nel) continue;
         double d0, d1, d2;
-        detj_affine(a.geom + 6 * (size_t)ge, d0, d1, d2);
+        detj_affine(s_geo + 6 * e, d0, d1, d2);
         const double da = d0 + d1 * tab[kTabLam + aa];
         const double* T1 = RA(e);
         double x[N];
@@ -603,7 +638,7 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
     }
     __syncthreads();
     // t3(d, i, b) = sum_a V(i, a) t2(d, a, b)
-    for (int it = lane; it < E * 3 * N; it += 64) {
+    for (int it = lane; it < E * 3 * N; it += NT) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, b = j / 3;
         if (e0 + e >= nel) continue;
         const double* T2 = RB(e);
@@ -615,13 +650,13 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int i = 0; i < N; ++i) {
             double t = 0.0;
 #pragma unroll
-            for (int aa = 0; aa < N; ++aa) t += tab[kTabV + i + N * aa] * x[aa];
+            for (int aa = 0; aa < N; ++aa) t += tab[kTabVT + aa + N * i] * x[aa];
             T3[sidx(d, i, b)] = t;
         }
     }
     __syncthreads();
     // y(d, i, j) = sum_b V(j, b) t3(d, i, b)
-    for (int it = lane; it < E * 3 * N; it += 64) {
+    for (int it = lane; it < E * 3 * N; it += NT) {
         const int e = it / (3 * N), j = it % (3 * N), d = j % 3, i = j / 3;
         if (e0 + e >= nel) continue;
         const double* T3 = RA(e);
@@ -633,28 +668,41 @@ __global__ __launch_bounds__(64) void exact_stage_kernel(const KArgs a, const do
         for (int jn = 0; jn < N; ++jn) {
             double t = 0.0;
 #pragma unroll
-            for (int b = 0; b < N; ++b) t += tab[kTabV + jn + N * b] * x[b];
+            for (int b = 0; b < N; ++b) t += tab[kTabVT + b + N * jn] * x[b];
             Y[sidx(d, i, jn)] = t;
         }
     }
     __syncthreads();
     // ---- epilogue: right-hand side, or the RK stage (and WaveHoltz accumulation)
     const int out = mode & M_OUTMASK;
-    for (int it = lane; it < E * 3 * NN; it += 64) {
-        const int e = it / (3 * NN), k = it % (3 * NN);
-        if (e0 + e >= nel) continue;
-        const size_t g = (size_t)(e0 + e) * 3 * NN + k;
-        const double rhs = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
-        if (out == M_OUT_RHS) {
-            static_cast<double*>(a.u_out)[g] = rhs;
-            continue;
+    const size_t gb = (size_t)e0 * 3 * NN;
+    if (out == M_OUT_RHS) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
+            if (it < nvalid * 3 * NN) static_cast<double*>(a.u_out)[gb + it] = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
         }
-        double* du = static_cast<double*>(a.du);
-        const double dd = (out == M_OUT_STAGE0 ? 0.0 : a.rk_a * du[g]) + a.dt * rhs;
-        du[g] = dd;
-        const double w = u[g] + a.rk_b * dd;
-        static_cast<double*>(a.u_out)[g] = w;
-        if (mode & M_ACC) a.acc[g] += a.ac * w;
+        return;
+    }
+    double* du = static_cast<double*>(a.du);
+    double dv[NS], av[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int it = lane + r * NT;
+        const bool ok = it < nvalid * 3 * NN;
+        dv[r] = ok && out != M_OUT_STAGE0 ? du[gb + it] : 0.0;
+        av[r] = ok && (mode & M_ACC) ? a.acc[gb + it] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
+        if (it >= nvalid * 3 * NN) continue;
+        const double rhs = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
+        const double dd = (out == M_OUT_STAGE0 ? 0.0 : a.rk_a * dv[r]) + a.dt * rhs;
+        du[gb + it] = dd;
+        const double w = v[r] + a.rk_b * dd;
+        static_cast<double*>(a.u_out)[gb + it] = w;
+        if (mode & M_ACC) a.acc[gb + it] = av[r] + a.ac * w;
     }
 }
 
@@ -662,7 +710,8 @@ template <int N>
 void launch_stage(int kid, const KArgs& a, const double* tab, hipStream_t s)
 {
     constexpr int E = StageShape<N>::E;
-    hipLaunchKernelGGL(exact_stage_kernel<N>, dim3((a.nel + E - 1) / E), dim3(64), 0, s, a, tab, kid);
+    hipLaunchKernelGGL(exact_stage_kernel<N>, dim3((a.nel + E - 1) / E), dim3(StageShape<N>::NT), 0, s, a, tab,
+                       kid);
 }
 
 template <int N>

@@ -872,6 +872,14 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, doub
         if (!wdg::mass_fast_diag(n, op->x.data(), nq_mass, xq.data(), wq.data(), tab.data() + wdg::kTabA,
                                  tab.data() + wdg::kTabB, tab.data() + wdg::kTabV, tab.data() + wdg::kTabLam))
             fail(WDG_ERR_INVALID, "the mass rule is too small for the basis (singular mass matrix)");
+        auto transpose = [&](int src, int rows, int cols, int dst) {  // (rows, cols) column-major
+            for (int c = 0; c < cols; ++c)
+                for (int r = 0; r < rows; ++r) tab[dst + c + cols * r] = tab[src + r + rows * c];
+        };
+        transpose(wdg::kTabPV, nq_volume, n, wdg::kTabPVT);
+        transpose(wdg::kTabDV, nq_volume, n, wdg::kTabDVT);
+        transpose(wdg::kTabPE, nq_edge, n, wdg::kTabPET);
+        transpose(wdg::kTabV, n, n, wdg::kTabVT);
         op->exact = true;
         op->nqv = nq_volume;
         op->nqe = nq_edge;

@@ -628,7 +628,9 @@ void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
 // One table block per handle, offsets in doubles: volume rule x, w, P, D;
-// edge rule x, w, P; mass matrices A, B, V, lam (ExactMassTables).
+// edge rule x, w, P; mass matrices A, B, V, lam (ExactMassTables); and the
+// transposes of P, D (volume), P (edge) and V, row-major copies whose rows
+// are contiguous for the fused stage kernel's scalar loads.
 enum : int {
     kTabXV = 0,
     kTabWV = 10,
@@ -641,7 +643,11 @@ enum : int {
     kTabB = 440,
     kTabV = 540,
     kTabLam = 640,
-    kTabSize = 650,
+    kTabPVT = 650,
+    kTabDVT = 750,
+    kTabPET = 850,
+    kTabVT = 950,
+    kTabSize = 1050,
 };
 struct ExactTables {
     const double* xv;  // volume rule (Div<false>)
