@@ -330,22 +330,24 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
 // ---------------------------------------------------------------- fused stage
 // The whole right-hand side / RK stage for the default rules (QV = N volume,
 // QE = N + 1 edge points) in one kernel.  E elements share an NT-lane
-// workgroup; every tensor pass gives one lane one (element, variable, line)
-// and keeps the line in registers, so the 1D tables are uniform operands read
-// through the scalar cache, from row-contiguous copies.  Per element three
-// LDS regions are reused phase by phase:
-//   RA: u -> u at quadrature points -> edge-point traces -> t1 -> t3
-//   RB: xi-interpolated u -> Df, then Pg -> staged traces -> edge fluxes -> t2
-//   RC: operator output r -> M^{-1} (r + s F)
-// Every array a pass reads along a line of lanes has an odd stride in
-// doubles (the L* pitches below), so a wave's reads spread over the LDS banks.
+// workgroup.  In every tensor pass a lane owns one (element, variable, line),
+// keeps the line in registers and reads the 1D tables as uniform operands
+// through the scalar cache (row-contiguous copies).  Two LDS regions per
+// element carry the transposes between passes:
+//   RA: u -> u at the volume points -> edge-point traces -> t1 -> M^{-1} r
+//   RB: xi-interpolated u -> Df, then Pg -> staged traces -> edge fluxes and
+//       their projections -> t3
+// The operator output r stays in the registers of the lane that owns its
+// line from the volume pass to the mass matrix; the face projections are
+// added there.  Arrays read along a line of lanes have odd strides in
+// doubles (the L* pitches), so a wave's reads spread over the LDS banks.
 template <int N>
 struct StageShape {
     static constexpr int cmax(int x, int y) { return x > y ? x : y; }
     static constexpr int odd(int x) { return x | 1; }
     static constexpr int QV = N, QE = N + 1, NN = N * N;
     static constexpr int NT = (N == 6 || N == 8) ? 128 : 64;  // fills the lanes best
-    static constexpr int E = cmax(NT / (3 * N), 1);
+    static constexpr int E = NT / (3 * N);                     // 3N line lanes per element
     static constexpr int LS = odd(3 * N);    // (3, N, N) arrays: d + 3 i + LS j
     static constexpr int LQ = odd(3 * QV);   // u at the volume points: d + 3 i + LQ j
     static constexpr int LP = odd(N);        // xi-interpolated u: (d + 3 i) LP + l
@@ -353,10 +355,35 @@ struct StageShape {
     static constexpr int LT = odd(N);        // staged side traces: row LT
     static constexpr int LE = odd(QE);       // edge-point traces and fluxes: row LE
     static constexpr int SA = cmax(cmax(LS * N, LQ * QV), 24 * LE);
-    static constexpr int SB = cmax(cmax(3 * QV * LP, 3 * N * LD), cmax(cmax(24 * LT, 12 * LE), LS * N));
-    static constexpr int SC = LS * N;
-    static constexpr int S = SA + SB + SC;
+    static constexpr int SB = cmax(cmax(3 * QV * LP, 3 * N * LD), cmax(cmax(24 * LT, 12 * LE + 12 * N), LS * N));
+    static constexpr int S = SA + SB;
 };
+
+// out_o = sum_k T[o NK + k] x[k] for o = 0 .. NO - 1, with T a uniform
+// table read through the scalar cache.  The row of output o + 2 is requested
+// while output o is computed; the compiler barrier keeps later rows from all
+// being hoisted to the top, which would exhaust the scalar registers.
+template <int NO, int NK, typename Store>
+__device__ __forceinline__ void contract_rows(const double* __restrict__ T, const double (&x)[NK], Store store)
+{
+    double r[3][NK];
+#pragma unroll
+    for (int o = 0; o < 2 && o < NO; ++o)
+#pragma unroll
+        for (int k = 0; k < NK; ++k) r[o][k] = T[o * NK + k];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        if (o + 2 < NO) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k) r[(o + 2) % 3][k] = T[(o + 2) * NK + k];
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) acc += r[o % 3][k] * x[k];
+        store(o, acc);
+        asm volatile("" ::: "memory");
+    }
+}
 
 template <int N>
 __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KArgs a, const double* __restrict__ tab,
@@ -366,6 +393,7 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     constexpr int QV = Sh::QV, QE = Sh::QE, NN = Sh::NN, E = Sh::E, NT = Sh::NT;
     constexpr int LS = Sh::LS, LQ = Sh::LQ, LP = Sh::LP, LD = Sh::LD, LT = Sh::LT, LE = Sh::LE;
     constexpr int NS = (E * 3 * NN + NT - 1) / NT;  // state values per lane
+    static_assert(QV == N, "the line lanes of the volume passes assume QV == N");
     __shared__ double s_mem[E * Sh::S];
     __shared__ double s_geo[E * 6];
     const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
@@ -373,11 +401,14 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     const double* __restrict__ u = static_cast<const double*>(a.u_in);
     auto RA = [&](int e) { return s_mem + e * Sh::S; };
     auto RB = [&](int e) { return s_mem + e * Sh::S + Sh::SA; };
-    auto RC = [&](int e) { return s_mem + e * Sh::S + Sh::SA + Sh::SB; };
     auto sidx = [](int d, int i, int j) { return d + 3 * i + LS * j; };
+    const int nvalid = nel - e0 < E ? nel - e0 : E;
+    // line lanes: element le, variable ld, line lk
+    const int le = lane / (3 * N), lj = lane % (3 * N), ld = lj % 3, lk = lj / 3;
+    const bool lon = le < nvalid;
+
     // the block's elements are contiguous in HBM: every load is issued before
     // the first wait
-    const int nvalid = nel - e0 < E ? nel - e0 : E;
     const double* ublk = u + (size_t)e0 * 3 * NN;
     double v[NS];
 #pragma unroll
@@ -394,55 +425,38 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     if (lane < nvalid * 6) s_geo[lane] = gv;
     __syncthreads();
     // ---- volume: the passes of div_2d_q (Div.cpp:469-588)
-    // PU(l, d, i) = sum_k P(i, k) u(d, k, l)
-    for (int it = lane; it < E * 3 * N; it += NT) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, l = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* U = RA(e);
+    // PU(l, d, i) = sum_k P(i, k) u(d, k, l)          lane (d, l)
+    if (lon) {
         double x[N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) x[k] = U[sidx(d, k, l)];
-        double* PU = RB(e);
-#pragma unroll 2
-        for (int i = 0; i < QV; ++i) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < N; ++k) acc += tab[kTabPVT + k + N * i] * x[k];
-            PU[(d + 3 * i) * LP + l] = acc;
-        }
+        for (int k = 0; k < N; ++k) x[k] = RA(le)[sidx(ld, k, lk)];
+        double* PU = RB(le);
+        contract_rows<QV, N>(tab + kTabPVT, x, [&](int i, double acc) { PU[(ld + 3 * i) * LP + lk] = acc; });
     }
     __syncthreads();
-    // UQ(d, i, j) = sum_l P(j, l) PU(l, d, i)
-    for (int it = lane; it < E * 3 * QV; it += NT) {
-        const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, i = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* PU = RB(e) + (d + 3 * i) * LP;
+    // UQ(d, i, j) = sum_l P(j, l) PU(l, d, i)          lane (d, i)
+    if (lon) {
+        const double* PU = RB(le) + (ld + 3 * lk) * LP;
         double x[N];
 #pragma unroll
         for (int l = 0; l < N; ++l) x[l] = PU[l];
-        double* UQ = RA(e);
-#pragma unroll 2
-        for (int jq = 0; jq < QV; ++jq) {
-            double acc = 0.0;
-#pragma unroll
-            for (int l = 0; l < N; ++l) acc += tab[kTabPVT + l + N * jq] * x[l];
-            UQ[d + 3 * i + LQ * jq] = acc;
-        }
+        double* UQ = RA(le);
+        contract_rows<QV, N>(tab + kTabPVT, x, [&](int jq, double acc) { UQ[ld + 3 * lk + LQ * jq] = acc; });
     }
     __syncthreads();
     // contravariant fluxes of f = (u, p, 0), g = (v, 0, p) at the points, then
-    // Df(j, d, k) = sum_i D(i, k) F(d, i, j)   -> r  = sum_j P(j, l) Df(j, d, k)
-    // Pg(j, d, k) = sum_i P(i, k) G(d, i, j)   -> r += sum_j D(j, l) Pg(j, d, k)
-    // in two rounds, so one buffer holds Df, then Pg.
+    //   Df(j, d, k) = sum_i D(i, k) F(d, i, j)   (lane (d, j)),  r  = sum_j P(j, l) Df(j, d, k)
+    //   Pg(j, d, k) = sum_i P(i, k) G(d, i, j),                  r += sum_j D(j, l) Pg(j, d, k)
+    // with r(d, k, :) in the registers of lane (d, k)
+    double R[N];
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
-        for (int it = lane; it < E * 3 * QV; it += NT) {
-            const int e = it / (3 * QV), j = it % (3 * QV), d = j % 3, jq = j / 3;
-            if (e0 + e >= nel) continue;
-            const double* g = s_geo + 6 * e;
+        if (lon) {
+            const int jq = lk;
+            const double* g = s_geo + 6 * le;
             const double ax = g[0], ay = g[1], cx = g[2], cy = g[3], hx = g[4], hy = g[5];
             const double eta = tab[kTabXV + jq], wj = tab[kTabWV + jq];
-            const double* UQ = RA(e) + LQ * jq;
+            const double* UQ = RA(le) + LQ * jq;
             double acc[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) acc[k] = 0.0;
@@ -450,8 +464,8 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
             for (int i = 0; i < QV; ++i) {
                 const double xi = tab[kTabXV + i], W = tab[kTabWV + i] * wj;
                 const double p = UQ[3 * i], uu = UQ[3 * i + 1], vv = UQ[3 * i + 2];
-                const double s1 = d == 0 ? uu : (d == 1 ? p : 0.0);
-                const double s2 = d == 0 ? vv : (d == 2 ? p : 0.0);
+                const double s1 = ld == 0 ? uu : (ld == 1 ? p : 0.0);
+                const double s2 = ld == 0 ? vv : (ld == 2 ? p : 0.0);
                 double f;
                 if (half == 0) {
                     const double X_eta = (cx + hx * xi) * W, Y_eta = (cy + hy * xi) * W;
@@ -463,29 +477,22 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
 #pragma unroll
                 for (int k = 0; k < N; ++k) acc[k] += tab[(half == 0 ? kTabDVT : kTabPVT) + k + N * i] * f;
             }
-            double* DP = RB(e);
+            double* DP = RB(le);
 #pragma unroll
-            for (int k = 0; k < N; ++k) DP[(d + 3 * k) * LD + jq] = acc[k];
+            for (int k = 0; k < N; ++k) DP[(ld + 3 * k) * LD + jq] = acc[k];
         }
         __syncthreads();
-        for (int it = lane; it < E * 3 * N; it += NT) {
-            const int e = it / (3 * N), j = it % (3 * N), d = j % 3, k = j / 3;
-            if (e0 + e >= nel) continue;
-            const double* DP = RB(e) + (d + 3 * k) * LD;
+        if (lon) {
+            const double* DP = RB(le) + (ld + 3 * lk) * LD;
             double x[QV];
 #pragma unroll
             for (int jq = 0; jq < QV; ++jq) x[jq] = DP[jq];
-            double* R = RC(e);
-#pragma unroll 2
-            for (int l = 0; l < N; ++l) {
-                double acc = 0.0;
-#pragma unroll
-                for (int jq = 0; jq < QV; ++jq) acc += tab[(half == 0 ? kTabPV : kTabDV) + jq + QV * l] * x[jq];
+            contract_rows<N, QV>(tab + (half == 0 ? kTabPV : kTabDV), x, [&](int l, double acc) {
                 if (half == 0)
-                    R[sidx(d, k, l)] = acc;
+                    R[l] = acc;
                 else
-                    R[sidx(d, k, l)] += acc;
-            }
+                    R[l] += acc;
+            });
         }
         __syncthreads();
     }
@@ -536,25 +543,19 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     // traces at the edge points
     for (int it = lane; it < E * 24; it += NT) {
         const int e = it / 24, j = it % 24;  // row j = 6 s + 3 (own | exterior) + d
-        if (e0 + e >= nel) continue;
+        if (e >= nvalid) continue;
         const double* T = RB(e) + LT * j;
         double x[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) x[k] = T[k];
         double* Q = RA(e) + LE * j;
-#pragma unroll 2
-        for (int q = 0; q < QE; ++q) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < N; ++k) acc += x[k] * tab[kTabPET + k + N * q];
-            Q[q] = acc;
-        }
+        contract_rows<QE, N>(tab + kTabPET, x, [&](int q, double acc) { Q[q] = acc; });
     }
     __syncthreads();
     // closed-form flux at every edge point, weighted by w ds
     for (int it = lane; it < E * 4 * QE; it += NT) {
-        const int e = it / (4 * QE), j = it % (4 * QE), s = j / QE, q = j % QE, ge = e0 + e;
-        if (ge >= nel) continue;
+        const int e = it / (4 * QE), j = it % (4 * QE), s = j / QE, q = j % QE;
+        if (e >= nvalid) continue;
         double n0, n1, ds;
         side_normal(s_geo + 6 * e, s, n0, n1, ds);
         const double* Q = RA(e) + 6 * LE * s + q;
@@ -568,109 +569,63 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
         Cf[2 * LE] = n1 * cf;
     }
     __syncthreads();
-    // projection onto the side's nodes, added to r: sides 0 and 2 first, then
-    // 1 and 3, so each corner has one writer per pass
+    // projection onto each side's nodes: FP(s, d, k)
+    for (int it = lane; it < E * 12; it += NT) {
+        const int e = it / 12, j = it % 12;  // j = 3 s + d
+        if (e >= nvalid) continue;
+        const double* Cf = RB(e) + LE * j;
+        double c[QE];
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int it = lane; it < E * 12; it += NT) {
-            const int e = it / 12, j = it % 12, s = j / 3, d = j % 3;
-            if ((s & 1) != pass || e0 + e >= nel) continue;
-            const double* Cf = RB(e) + LE * (3 * s + d);
-            double c[QE];
-#pragma unroll
-            for (int q = 0; q < QE; ++q) c[q] = Cf[q];
-            double* R = RC(e);
-            // node k of side s is (i0 + di k, j0 + dj k)
-            const int i0 = s == 1 ? N - 1 : 0, j0 = s == 2 ? N - 1 : 0, di = (s & 1) ? 0 : 1, dj = (s & 1) ? 1 : 0;
-#pragma unroll 2
-            for (int k = 0; k < N; ++k) {
-                double v = 0.0;
-#pragma unroll
-                for (int q = 0; q < QE; ++q) v += c[q] * tab[kTabPE + q + QE * k];
-                R[sidx(d, i0 + di * k, j0 + dj * k)] += v;
-            }
-        }
-        __syncthreads();
+        for (int q = 0; q < QE; ++q) c[q] = Cf[q];
+        double* FP = RB(e) + 12 * LE + N * j;
+        contract_rows<N, QE>(tab + kTabPE, c, [&](int k, double val) { FP[k] = val; });
     }
+    __syncthreads();
     // ---- M^{-1} (r + s F) by fast diagonalisation
-    const bool force = mode & M_FORCE;
-    // t1(d, a, j) = sum_i V(i, a) x(d, i, j)
-    for (int it = lane; it < E * 3 * N; it += NT) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, jn = j / 3, ge = e0 + e;
-        if (ge >= nel) continue;
-        const double* R = RC(e);
-        double x[N];
+    // lane (d, i): r(d, i, :) += faces and forcing; t1(d, i, b) = sum_j V(j, b) r(d, i, j)
+    if (lon) {
+        if (mode & M_FORCE) {
+            const double* F = static_cast<const double*>(a.force) + (size_t)(e0 + le) * 3 * NN + ld + 3 * lk;
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            double v = R[sidx(d, i, jn)];
-            if (force) v += static_cast<const double*>(a.force)[(size_t)ge * 3 * NN + d + 3 * (i + N * jn)] * a.fs;
-            x[i] = v;
+            for (int l = 0; l < N; ++l) R[l] += F[3 * N * l] * a.fs;
         }
-        double* T1 = RA(e);
-#pragma unroll 2
-        for (int aa = 0; aa < N; ++aa) {
-            double t = 0.0;
+        const double* FP = RB(le) + 12 * LE + N * ld;  // side s at + 3 N s
+        R[0] += FP[0 * 3 * N + lk];
+        R[N - 1] += FP[2 * 3 * N + lk];
+        if (lk == N - 1) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) t += tab[kTabV + i + N * aa] * x[i];
-            T1[sidx(d, aa, jn)] = t;
+            for (int l = 0; l < N; ++l) R[l] += FP[1 * 3 * N + l];
         }
+        if (lk == 0) {
+#pragma unroll
+            for (int l = 0; l < N; ++l) R[l] += FP[3 * 3 * N + l];
+        }
+        double* T1 = RA(le);
+        contract_rows<N, N>(tab + kTabV, R, [&](int b, double t) { T1[sidx(ld, lk, b)] = t; });
     }
     __syncthreads();
-    // t2(d, a, b) = sum_j V(j, b) t1(d, a, j) / (d0 + d1 lam_a + d2 lam_b)
-    for (int it = lane; it < E * 3 * N; it += NT) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, aa = j / 3, ge = e0 + e;
-        if (ge >= nel) continue;
+    // lane (d, b): t2(d, a, b) = sum_i V(i, a) t1(d, i, b) / (d0 + d1 lam_a + d2 lam_b),
+    //              t3(d, i, b) = sum_a V(i, a) t2(d, a, b)
+    if (lon) {
+        double x[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) x[i] = RA(le)[sidx(ld, i, lk)];
         double d0, d1, d2;
-        detj_affine(s_geo + 6 * e, d0, d1, d2);
-        const double da = d0 + d1 * tab[kTabLam + aa];
-        const double* T1 = RA(e);
-        double x[N];
-#pragma unroll
-        for (int jn = 0; jn < N; ++jn) x[jn] = T1[sidx(d, aa, jn)];
-        double* T2 = RB(e);
-#pragma unroll 2
-        for (int b = 0; b < N; ++b) {
-            double t = 0.0;
-#pragma unroll
-            for (int jn = 0; jn < N; ++jn) t += tab[kTabV + jn + N * b] * x[jn];
-            T2[sidx(d, aa, b)] = t * recip(da + d2 * tab[kTabLam + b]);
-        }
+        detj_affine(s_geo + 6 * le, d0, d1, d2);
+        const double db = d0 + d2 * tab[kTabLam + lk];
+        double t2[N];
+        contract_rows<N, N>(tab + kTabV, x, [&](int aa, double t) { t2[aa] = t * recip(db + d1 * tab[kTabLam + aa]); });
+        double* T3 = RB(le);
+        contract_rows<N, N>(tab + kTabVT, t2, [&](int i, double t) { T3[sidx(ld, i, lk)] = t; });
     }
     __syncthreads();
-    // t3(d, i, b) = sum_a V(i, a) t2(d, a, b)
-    for (int it = lane; it < E * 3 * N; it += NT) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, b = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* T2 = RB(e);
+    // lane (d, i): y(d, i, j) = sum_b V(j, b) t3(d, i, b)
+    if (lon) {
         double x[N];
 #pragma unroll
-        for (int aa = 0; aa < N; ++aa) x[aa] = T2[sidx(d, aa, b)];
-        double* T3 = RA(e);
-#pragma unroll 2
-        for (int i = 0; i < N; ++i) {
-            double t = 0.0;
-#pragma unroll
-            for (int aa = 0; aa < N; ++aa) t += tab[kTabVT + aa + N * i] * x[aa];
-            T3[sidx(d, i, b)] = t;
-        }
-    }
-    __syncthreads();
-    // y(d, i, j) = sum_b V(j, b) t3(d, i, b)
-    for (int it = lane; it < E * 3 * N; it += NT) {
-        const int e = it / (3 * N), j = it % (3 * N), d = j % 3, i = j / 3;
-        if (e0 + e >= nel) continue;
-        const double* T3 = RA(e);
-        double x[N];
-#pragma unroll
-        for (int b = 0; b < N; ++b) x[b] = T3[sidx(d, i, b)];
-        double* Y = RC(e);
-#pragma unroll 2
-        for (int jn = 0; jn < N; ++jn) {
-            double t = 0.0;
-#pragma unroll
-            for (int b = 0; b < N; ++b) t += tab[kTabVT + b + N * jn] * x[b];
-            Y[sidx(d, i, jn)] = t;
-        }
+        for (int b = 0; b < N; ++b) x[b] = RB(le)[sidx(ld, lk, b)];
+        double* Y = RA(le);
+        contract_rows<N, N>(tab + kTabVT, x, [&](int j, double t) { Y[sidx(ld, lk, j)] = t; });
     }
     __syncthreads();
     // ---- epilogue: right-hand side, or the RK stage (and WaveHoltz accumulation)
@@ -680,7 +635,7 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
 #pragma unroll
         for (int r = 0; r < NS; ++r) {
             const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
-            if (it < nvalid * 3 * NN) static_cast<double*>(a.u_out)[gb + it] = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
+            if (it < nvalid * 3 * NN) static_cast<double*>(a.u_out)[gb + it] = RA(e)[k + (LS - 3 * N) * (k / (3 * N))];
         }
         return;
     }
@@ -697,7 +652,7 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     for (int r = 0; r < NS; ++r) {
         const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
         if (it >= nvalid * 3 * NN) continue;
-        const double rhs = RC(e)[k + (LS - 3 * N) * (k / (3 * N))];
+        const double rhs = RA(e)[k + (LS - 3 * N) * (k / (3 * N))];
         const double dd = (out == M_OUT_STAGE0 ? 0.0 : a.rk_a * dv[r]) + a.dt * rhs;
         du[gb + it] = dd;
         const double w = v[r] + a.rk_b * dd;
