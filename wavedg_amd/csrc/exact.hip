@@ -360,26 +360,24 @@ struct StageShape {
 };
 
 // out_o = sum_k T[o NK + k] x[k] for o = 0 .. NO - 1, with T a uniform
-// table read through the scalar cache.  The row of output o + 2 is requested
+// table read through the scalar cache.  The row of output o + 1 is requested
 // while output o is computed; the compiler barrier keeps later rows from all
 // being hoisted to the top, which would exhaust the scalar registers.
 template <int NO, int NK, typename Store>
 __device__ __forceinline__ void contract_rows(const double* __restrict__ T, const double (&x)[NK], Store store)
 {
-    double r[3][NK];
+    double r[2][NK];
 #pragma unroll
-    for (int o = 0; o < 2 && o < NO; ++o)
-#pragma unroll
-        for (int k = 0; k < NK; ++k) r[o][k] = T[o * NK + k];
+    for (int k = 0; k < NK; ++k) r[0][k] = T[k];
 #pragma unroll
     for (int o = 0; o < NO; ++o) {
-        if (o + 2 < NO) {
+        if (o + 1 < NO) {
 #pragma unroll
-            for (int k = 0; k < NK; ++k) r[(o + 2) % 3][k] = T[(o + 2) * NK + k];
+            for (int k = 0; k < NK; ++k) r[(o + 1) % 2][k] = T[(o + 1) * NK + k];
         }
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < NK; ++k) acc += r[o % 3][k] * x[k];
+        for (int k = 0; k < NK; ++k) acc += r[o % 2][k] * x[k];
         store(o, acc);
         asm volatile("" ::: "memory");
     }
@@ -396,8 +394,21 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     static_assert(QV == N, "the line lanes of the volume passes assume QV == N");
     __shared__ double s_mem[E * Sh::S];
     __shared__ double s_geo[E * 6];
+    // the epilogue's scalars, parked in LDS so that they hold no scalar
+    // registers through the passes (which would spill them)
+    __shared__ double s_par[4];
+    __shared__ double* s_ptr[3];
     const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
     const int mode = kModeOf[kid];
+    if (lane == 0) {
+        s_par[0] = a.rk_a;
+        s_par[1] = a.rk_b;
+        s_par[2] = a.dt;
+        s_par[3] = a.ac;
+        s_ptr[0] = static_cast<double*>(a.u_out);
+        s_ptr[1] = static_cast<double*>(a.du);
+        s_ptr[2] = a.acc;
+    }
     const double* __restrict__ u = static_cast<const double*>(a.u_in);
     auto RA = [&](int e) { return s_mem + e * Sh::S; };
     auto RB = [&](int e) { return s_mem + e * Sh::S + Sh::SA; };
@@ -631,33 +642,38 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     // ---- epilogue: right-hand side, or the RK stage (and WaveHoltz accumulation)
     const int out = mode & M_OUTMASK;
     const size_t gb = (size_t)e0 * 3 * NN;
+    const volatile double* vpar = s_par;
+    double* const volatile* vptr = s_ptr;
+    double* const uo = vptr[0] + gb;
     if (out == M_OUT_RHS) {
 #pragma unroll
         for (int r = 0; r < NS; ++r) {
             const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
-            if (it < nvalid * 3 * NN) static_cast<double*>(a.u_out)[gb + it] = RA(e)[k + (LS - 3 * N) * (k / (3 * N))];
+            if (it < nvalid * 3 * NN) uo[it] = RA(e)[k + (LS - 3 * N) * (k / (3 * N))];
         }
         return;
     }
-    double* du = static_cast<double*>(a.du);
+    const double rk_a = vpar[0], rk_b = vpar[1], dt = vpar[2], ac = vpar[3];
+    double* const du = vptr[1] + gb;
+    double* const acc = vptr[2] + gb;
     double dv[NS], av[NS];
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
         const int it = lane + r * NT;
         const bool ok = it < nvalid * 3 * NN;
-        dv[r] = ok && out != M_OUT_STAGE0 ? du[gb + it] : 0.0;
-        av[r] = ok && (mode & M_ACC) ? a.acc[gb + it] : 0.0;
+        dv[r] = ok && out != M_OUT_STAGE0 ? du[it] : 0.0;
+        av[r] = ok && (mode & M_ACC) ? acc[it] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
         const int it = lane + r * NT, e = it / (3 * NN), k = it % (3 * NN);
         if (it >= nvalid * 3 * NN) continue;
         const double rhs = RA(e)[k + (LS - 3 * N) * (k / (3 * N))];
-        const double dd = (out == M_OUT_STAGE0 ? 0.0 : a.rk_a * dv[r]) + a.dt * rhs;
-        du[gb + it] = dd;
-        const double w = v[r] + a.rk_b * dd;
-        static_cast<double*>(a.u_out)[gb + it] = w;
-        if (mode & M_ACC) a.acc[gb + it] = av[r] + a.ac * w;
+        const double dd = (out == M_OUT_STAGE0 ? 0.0 : rk_a * dv[r]) + dt * rhs;
+        du[it] = dd;
+        const double w = v[r] + rk_b * dd;
+        uo[it] = w;
+        if (mode & M_ACC) acc[it] = av[r] + ac * w;
     }
 }
 
