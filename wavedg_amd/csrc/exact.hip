@@ -398,7 +398,9 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
     // registers through the passes (which would spill them)
     __shared__ double s_par[4];
     __shared__ double* s_ptr[3];
-    const int lane = threadIdx.x, e0 = blockIdx.x * E, nel = a.nel;
+    // each XCD walks one contiguous run of elements, so neighbour traces and
+    // the block's own side traces are read through the L2 that loaded them
+    const int lane = threadIdx.x, e0 = xcd_block(blockIdx.x, gridDim.x) * E, nel = a.nel;
     const int mode = kModeOf[kid];
     if (lane == 0) {
         s_par[0] = a.rk_a;
