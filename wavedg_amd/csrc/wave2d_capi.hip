@@ -122,6 +122,8 @@ struct wdg_wave2d_s {
     std::vector<double> x, w;
     double* d_corners = nullptr;
     double* d_geom = nullptr;
+    float* d_geom32 = nullptr;   // FP32 handles: geometry and basis rounded to float
+    float* d_basis32 = nullptr;
     int32_t* d_nbr = nullptr;
     int32_t* d_info = nullptr;
     double* d_basis = nullptr;
@@ -166,6 +168,8 @@ struct wdg_wave2d_s {
         if (ev1) (void)hipEventDestroy(ev1);
         dfree(d_corners);
         dfree(d_geom);
+        dfree(d_geom32);
+        dfree(d_basis32);
         dfree(d_nbr);
         dfree(d_info);
         dfree(d_basis);
@@ -199,6 +203,8 @@ struct wdg_wave2d_s {
         a.nbr = d_nbr;
         a.info = d_info;
         a.basis = d_basis;
+        a.geom32 = d_geom32;
+        a.basis32 = d_basis32;
         a.ha = ha;
         a.hb = hb;
         a.nel = nel;
@@ -828,6 +834,15 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
                   "upload basis");
         hip_check(hipMemcpy(op->d_lvtab, lv.data(), sizeof(double) * lv.size(), hipMemcpyHostToDevice),
                   "upload load-vector table");
+        if (precision == WDG_F32) {
+            const std::vector<float> g32(geom.begin(), geom.end()), b32(basis.begin(), basis.end());
+            op->d_geom32 = static_cast<float*>(dmalloc(sizeof(float) * g32.size()));
+            op->d_basis32 = static_cast<float*>(dmalloc(sizeof(float) * b32.size()));
+            hip_check(hipMemcpy(op->d_geom32, g32.data(), sizeof(float) * g32.size(), hipMemcpyHostToDevice),
+                      "upload float geometry");
+            hip_check(hipMemcpy(op->d_basis32, b32.data(), sizeof(float) * b32.size(), hipMemcpyHostToDevice),
+                      "upload float basis");
+        }
         *out = op.release();
     });
 }

@@ -42,6 +42,8 @@ struct KArgs {
     double rk_a, rk_b, dt;  // stage coefficients
     double ha, hb;          // flux: f = w ds [ha nA (u0+u1) + hb |nA| (u0-u1)]
     int nel;
+    const float* geom32;    // FP32 handles: geom and basis rounded to float once
+    const float* basis32;
 };
 
 enum : int {
@@ -200,11 +202,15 @@ struct NodeResult {
 // join the barriers).  Element indices in nbr are local to u_in / geom.
 // `prefetch(active, g)` is called after the state loads are issued and
 // before the first barrier, so the caller's extra loads overlap them.
-template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, typename Prefetch>
-__device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const double* __restrict__ geom,
+// GT and BT, the types of the geometry and basis tables, are double, or
+// float for FP32 handles that keep float copies (the rounding is the same as
+// converting in the kernel, so results do not change).
+template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, typename GT = double, typename BT = double,
+          typename Prefetch>
+__device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const GT* __restrict__ geom,
                                                         const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ info,
-                                                        const double* __restrict__ basis, double ha_, double hb_,
+                                                        const BT* __restrict__ basis, double ha_, double hb_,
                                                         int e0, int nloc, unsigned char* lds, int& el_out,
                                                         size_t& g_out, bool& active_out, Prefetch&& prefetch)
 {
@@ -215,14 +221,14 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     constexpr int FP = (NF + NN - 1) / NN;  // face nodes per thread (1 for N >= 4)
     constexpr bool FACE = IFACE || BFACE;
     using L = OpLds<N, R, VOL, FACE, TGT>;
-    double* s_xw = reinterpret_cast<double*>(lds + L::off_xw);
+    BT* s_xw = reinterpret_cast<BT*>(lds + L::off_xw);
     R* s_u = reinterpret_cast<R*>(lds + L::off_u);
     R* s_F = reinterpret_cast<R*>(lds + L::off_F);
     R* s_G = reinterpret_cast<R*>(lds + L::off_G);
     R* s_S = reinterpret_cast<R*>(lds + L::off_S);
     R* s_D = reinterpret_cast<R*>(lds + L::off_D);
-    const double* s_x = s_xw;
-    const double* s_w = s_xw + N;
+    const BT* s_x = s_xw;
+    const BT* s_w = s_xw + N;
 
     const int tid = threadIdx.x;
     const bool worker = tid < NT;
@@ -238,7 +244,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
         for (int r = 0; r < 3; ++r) ld[r] = ub[min(tid + r * NT, cnt - 1)];
     }
     const R dval = (R)basis[2 * N + min(tid, NN - 1)];
-    const double xwv = basis[min(tid, 2 * N - 1)];
+    const BT xwv = basis[min(tid, 2 * N - 1)];
     const int el = worker ? tid / NN : 0;
     const int node = worker ? tid - el * NN : 0;
     const int i = node % N, j = node / N;
@@ -256,8 +262,9 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     // Volume metrics are evaluated in the state precision R (double for FP64
     // handles); face normals and weights in FG.
     using FG = typename std::conditional<WDG_FACE_GEOM_DOUBLE, double, R>::type;
-    const double2* gp = reinterpret_cast<const double2*>(geom + 6 * (size_t)e);
-    const double2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
+    using GT2 = typename std::conditional<sizeof(GT) == 8, double2, float2>::type;
+    const GT2* gp = reinterpret_cast<const GT2*>(geom + 6 * (size_t)e);
+    const GT2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
     const R ax = (R)g_a.x, ay = (R)g_a.y, cx = (R)g_c.x, cy = (R)g_c.y, hx = (R)g_h.x, hy = (R)g_h.y;
     const FG fax = (FG)g_a.x, fay = (FG)g_a.y, fcx = (FG)g_c.x, fcy = (FG)g_c.y, fhx = (FG)g_h.x, fhy = (FG)g_h.y;
 
@@ -494,9 +501,13 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), WDG_MIN_WAVES) void wave_kerne
     int el;
     size_t g;
     bool active;
-    NodeResult<R> q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info,
-                                                              a.basis, a.ha, a.hb, e0, nloc, lds, el, g, active,
-                                                              prefetch);
+    NodeResult<R> q;
+    if constexpr (std::is_same<R, float>::value && !WDG_FACE_GEOM_DOUBLE)
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom32, a.nbr, a.info,
+                                                         a.basis32, a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
+    else
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info, a.basis,
+                                                         a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
     if (!active) return;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
