@@ -55,6 +55,7 @@ enum { WDG_BC_ABSORB = 0, WDG_BC_REFLECT = 1, WDG_BC_DIRICHLET = 2 };
 typedef struct wdg_mesh_s *wdg_mesh;
 typedef struct wdg_wave2d_s *wdg_wave2d;
 typedef struct wdg_waveholtz_s *wdg_waveholtz;
+typedef struct wdg_advection_s *wdg_advection;
 
 const char *wdg_hip_last_error(void);
 int wdg_hip_version(int *major, int *minor);
@@ -136,6 +137,22 @@ int wdg_hip_wave2d_action(wdg_wave2d op, const void *d_x, void *d_y, void *strea
  * volume term of WaveEquation::action. */
 int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double *A, const double *B, const void *d_u, void *d_divF,
                           void *stream);
+/* Advection<true> and AdvectionHomogeneousBC<true> (wavedg/Advection.hpp:
+ * 15-273) on the mesh and basis of an FP64 approx_quad = true wave handle
+ * (which must outlive it): n_var <= 8 variables in the layout
+ * (n_var, n, n, n_elem), coefficient A a host array of shape (n_var, n_var, 2)
+ * (constant_coefficient != 0) or (n_var, n_var, 2, n, n, n_elem), column-major,
+ * A(k, l, d) the (k, l) entry of the d-direction flux matrix; upwind flux
+ * (a = -1, b = -0.5 of EdgeFlux).  n.A must be symmetric at every face node
+ * (the reference's general real eigen-decomposition is restated for
+ * symmetric matrices only); otherwise create fails. */
+int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double *A, int constant_coefficient, wdg_advection *out);
+/* Advection::action (Advection.hpp:127-140): d_divF += Div(u) + interior edge fluxes. */
+int wdg_hip_advection_action(wdg_advection adv, const void *d_u, void *d_divF, void *stream);
+/* AdvectionHomogeneousBC::action (Advection.hpp:252-270): d_divF += boundary
+ * edge fluxes with u = 0 outside the domain. */
+int wdg_hip_advection_bc_action(wdg_advection adv, const void *d_u, void *d_divF, void *stream);
+int wdg_hip_advection_destroy(wdg_advection adv);
 /* WaveBC::action (WaveEquation.cpp:190-215): d_y += B d_x (boundary flux and lift). */
 int wdg_hip_wavebc_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
 /* MassMatrix<true>::inv (MassMatrix.cpp:69-82): d_x <- M^{-1} d_x, in place. */

@@ -73,6 +73,10 @@ def lib():
             "orc_waveholtz_postprocess": (None, [_vp, _dp, _dp]),
             "orc_load_vector_gauss": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]),
             "orc_edge_flux": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
+            "orc_advection_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
+            "orc_advection_free": (None, [_vp]),
+            "orc_advection_action": (None, [_vp, _dp, _dp]),
+            "orc_advection_bc_action": (None, [_vp, _dp, _dp]),
             "orc_norm": (ctypes.c_double, [ctypes.c_longlong, _dp]),
             "orc_error": (ctypes.c_double, [ctypes.c_longlong, _dp, _dp]),
         }
@@ -259,6 +263,38 @@ class Wave:
         Fp = _d(np.ascontiguousarray(F)) if F is not None else None
         lib().orc_rk4_steps(self.h, dt, ctypes.byref(tt), _d(u), n_steps, Fp, omega)
         return u, tt.value
+
+
+class Advection:
+    """Oracle Advection<true> + AdvectionHomogeneousBC<true> on a Wave's mesh
+    and basis.  A is (nvar, nvar, 2) with A[k, l, dim] (constant) or
+    (nvar, nvar, 2, n, n, nel) in the reference's column-major index order,
+    i.e. a numpy array indexed [el, j, i, dim, l, k] flattened; pass
+    `A_colmajor` already flattened in that order."""
+
+    def __init__(self, wave: Wave, nvar: int, A_colmajor, constant_coefficient: bool = True):
+        self.wave = wave
+        self.nvar = nvar
+        self._A = np.ascontiguousarray(np.asarray(A_colmajor, dtype=np.float64).reshape(-1))
+        self.h = lib().orc_advection_create(wave.h, nvar, _d(self._A), 1 if constant_coefficient else 0)
+        if not self.h:
+            raise ValueError("orc_advection_create failed (n.A must be symmetric, nvar <= 8)")
+        self.size = nvar * wave.ndof // 3
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_advection_free(self.h)
+            self.h = None
+
+    def action(self, u, y=None):
+        y = np.zeros(self.size) if y is None else y
+        lib().orc_advection_action(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(y))
+        return y
+
+    def bc_action(self, u, y=None):
+        y = np.zeros(self.size) if y is None else y
+        lib().orc_advection_bc_action(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(y))
+        return y
 
 
 class WaveHoltz:

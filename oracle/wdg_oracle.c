@@ -1316,3 +1316,249 @@ double orc_error(long long n, const double *x, const double *y)
     }
     return sqrt(r);
 }
+
+/* ===================================================================== */
+/* Advection<true>, AdvectionHomogeneousBC<true> (Advection.hpp:15-273)   */
+/* ===================================================================== */
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric m x m matrix, m <= 8
+ * (column-major): A = R diag(e) R^T.  Stands in for real_eig (eig.cpp:29-74)
+ * on the symmetric matrices this restatement accepts. */
+static void sym_eig(int m, const double *A0, double *R, double *e)
+{
+    double A[64];
+    memcpy(A, A0, sizeof(double) * m * m);
+    for (int i = 0; i < m * m; ++i) R[i] = (i % (m + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int q = 0; q < m; ++q)
+            for (int p = 0; p < q; ++p) off += A[p + m * q] * A[p + m * q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = A[p + m * q];
+                if (fabs(apq) < 1e-300) continue;
+                const double app = A[p + m * p], aqq = A[q + m * q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double akp = A[k + m * p], akq = A[k + m * q];
+                    A[k + m * p] = c * akp - s * akq;
+                    A[k + m * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double apk = A[p + m * k], aqk = A[q + m * k];
+                    A[p + m * k] = c * apk - s * aqk;
+                    A[q + m * k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double rkp = R[k + m * p], rkq = R[k + m * q];
+                    R[k + m * p] = c * rkp - s * rkq;
+                    R[k + m * q] = s * rkp + c * rkq;
+                }
+            }
+    }
+    for (int i = 0; i < m; ++i) e[i] = A[i + m * i];
+}
+
+/* _ComputeFlux::flux (EdgeFlux.cpp:31-89) for m x m symmetric n.A:
+ * Fs(c, d) = (a/2 nA + sgn b |nA|)(d, c).  Returns -1 if n.A is not symmetric. */
+static int compute_flux_m(int m, const double *nA, double a, double b, int side, double *Fs)
+{
+    const double sgn = (side == 0) ? 1.0 : -1.0;
+    if (b == 0.0) {
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) Fs[i + m * j] = 0.5 * a * nA[j + m * i];
+        return 0;
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            if (nA[i + m * j] != nA[j + m * i]) return -1;
+    double R[64], e[8], coef[8];
+    sym_eig(m, nA, R, e);
+    for (int k = 0; k < m; ++k) coef[k] = 0.5 * a * e[k] + sgn * b * fabs(e[k]);
+    for (int d = 0; d < m; ++d)
+        for (int c = 0; c < m; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < m; ++k) s += R[c + m * k] * coef[k] * R[d + m * k];
+            Fs[c + m * d] = s;
+        }
+    return 0;
+}
+
+struct orc_adv {
+    const orc_wave *W; /* mesh, basis and face maps */
+    int nvar, cc;
+    double *A;         /* (nvar, nvar, 2) or (nvar, nvar, 2, n, n, nel) */
+    double *FI, *FB;   /* F(i, c, d, s, e) = W(i) ds Fs(c, d), (n, nvar, nvar, 2, ne) */
+    double *uf;        /* face vector (n, nvar, 2, ne) */
+};
+
+/* A(k, l, dim) at global node j (Div.cpp:21-37 index _i, _el) */
+static const double *coef_at(const struct orc_adv *V, int j)
+{
+    const int v2 = V->nvar * V->nvar;
+    return V->cc ? V->A : V->A + (size_t)2 * v2 * j;
+}
+
+/* setup_flux_2d_cc_c / setup_flux_2d_c (EdgeFlux.cpp:91-186): the
+ * coefficient on side s is the face-prolonged A of that side; the exterior
+ * side of a boundary edge has A = 0. */
+static double *adv_flux_table(const struct orc_adv *V, int type, double a, double b, int *err)
+{
+    const orc_wave *W = V->W;
+    const int n = W->n, m = V->nvar, v2 = m * m, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    double *F = (double *)malloc(sizeof(double) * ((size_t)2 * n * v2 * ne + 1));
+    double nA[64], Fs[64];
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(W->m, e, type);
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (s + 2 * (size_t)e)];
+                if (j < 0 && !V->cc) {
+                    for (int d = 0; d < v2; ++d) nA[d] = 0.0;
+                } else {
+                    const double *A = coef_at(V, j < 0 ? 0 : j);
+                    for (int d = 0; d < v2; ++d) nA[d] = E->n[0] * A[d] + E->n[1] * A[v2 + d];
+                }
+                if (compute_flux_m(m, nA, a, b, s, Fs) != 0) *err = 1;
+                const double w = W->w[i] * E->meas;
+                for (int d = 0; d < m; ++d)
+                    for (int c = 0; c < m; ++c) F[i + n * (c + m * (d + m * (s + 2 * (size_t)e)))] = w * Fs[c + m * d];
+            }
+    }
+    return F;
+}
+
+orc_adv *orc_advection_create(const orc_wave *W, int nvar, const double *A, int constant_coefficient)
+{
+    if (nvar < 1 || nvar > 8) return NULL;
+    orc_adv *V = (orc_adv *)calloc(1, sizeof(orc_adv));
+    V->W = W;
+    V->nvar = nvar;
+    V->cc = constant_coefficient;
+    const size_t nA = (size_t)2 * nvar * nvar * (constant_coefficient ? 1 : (size_t)W->n * W->n * W->m->nel);
+    V->A = (double *)malloc(sizeof(double) * nA);
+    memcpy(V->A, A, sizeof(double) * nA);
+    int err = 0;
+    /* take negative a, b since -div is computed (Advection.hpp:86-87, 212) */
+    V->FI = adv_flux_table(V, 0, -1.0, -0.5, &err);
+    V->FB = adv_flux_table(V, 1, -1.0, -0.5, &err);
+    const int ne = W->m->nI > W->m->nB ? W->m->nI : W->m->nB;
+    V->uf = (double *)malloc(sizeof(double) * ((size_t)2 * W->n * nvar * ne + 1));
+    if (err) {
+        orc_advection_free(V);
+        return NULL;
+    }
+    return V;
+}
+
+void orc_advection_free(orc_adv *V)
+{
+    if (!V) return;
+    free(V->A);
+    free(V->FI);
+    free(V->FB);
+    free(V->uf);
+    free(V);
+}
+
+/* Div<true>::action with the (nvar, nvar, 2[, n, n, nel]) coefficient
+ * (Div.cpp:5-53, 131-179): f_d = sum_c A(d, c, 0) u_c, g_d = sum_c A(d, c, 1) u_c. */
+static void adv_div(const orc_adv *V, const double *u, double *divF)
+{
+    const orc_wave *W = V->W;
+    const int n = W->n, nn = n * n, nel = W->m->nel, m = V->nvar, v2 = m * m;
+    const double *D = W->D, *w = W->w, *x = W->x;
+#pragma omp parallel
+    {
+        double *F = (double *)malloc(sizeof(double) * 2 * (size_t)nn * m);
+#pragma omp for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const double *c = W->m->corners + 8 * (size_t)el;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double J[4];
+                    quad_jacobian(c, x[i], x[j], J);
+                    const double Wt = w[j] * w[i];
+                    const double Y_eta = J[3] * Wt, X_eta = J[2] * Wt;
+                    const double Y_xi = J[1] * Wt, X_xi = J[0] * Wt;
+                    const int node = i + n * (j + n * el);
+                    const double *A = coef_at(V, node);
+                    const double *uq = u + (size_t)m * node;
+                    for (int d = 0; d < m; ++d) {
+                        double Fd = 0.0, Gd = 0.0;
+                        for (int cc = 0; cc < m; ++cc) {
+                            const double a0 = A[d + m * cc], a1 = A[v2 + d + m * cc];
+                            Fd += (Y_eta * a0 - X_eta * a1) * uq[cc];
+                            Gd += (-Y_xi * a0 + X_xi * a1) * uq[cc];
+                        }
+                        F[0 + 2 * (i + n * (d + m * j))] = Fd;
+                        F[1 + 2 * (j + n * (d + m * i))] = Gd;
+                    }
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int d = 0; d < m; ++d) {
+                        double dF = 0.0;
+                        for (int l = 0; l < n; ++l)
+                            dF += D[l + n * i] * F[0 + 2 * (l + n * (d + m * j))] +
+                                  D[l + n * j] * F[1 + 2 * (l + n * (d + m * i))];
+                        divF[d + m * (i + n * (j + (size_t)n * el))] += dF;
+                    }
+        }
+        free(F);
+    }
+}
+
+/* face_prol->action, Flx->action, face_prol->t (Advection.hpp:127-140, 252-270):
+ * prolongation (FaceProlongator.cpp:565-590), flux_action_2d_c
+ * (EdgeFlux.cpp:289-325) and the transpose (FaceProlongator.cpp:650-672). */
+static void adv_faces(const orc_adv *V, int type, const double *u, double *divF)
+{
+    const orc_wave *W = V->W;
+    const int n = W->n, m = V->nvar, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    const double *F = type == 0 ? V->FI : V->FB;
+    double *uf = V->uf;
+    memset(uf, 0, sizeof(double) * (size_t)2 * n * m * ne);
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < m; ++d) uf[i + n * (d + m * (side + 2 * (size_t)e))] = u[d + m * (size_t)j];
+            }
+    double acc[16 * 8];
+    for (int e = 0; e < ne; ++e) {
+        for (int k = 0; k < n * m; ++k) acc[k] = 0.0;
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < m; ++d)
+                for (int c = 0; c < m; ++c)
+                    for (int i = 0; i < n; ++i)
+                        acc[i + n * d] += F[i + n * (c + m * (d + m * (s + 2 * (size_t)e)))] *
+                                          uf[i + n * (c + m * (s + 2 * (size_t)e))];
+        for (int d = 0; d < m; ++d)
+            for (int i = 0; i < n; ++i) {
+                uf[i + n * (d + m * (0 + 2 * (size_t)e))] = acc[i + n * d];
+                uf[i + n * (d + m * (1 + 2 * (size_t)e))] = -acc[i + n * d];
+            }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < m; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    divF[d + m * (size_t)j] += uf[i + n * (d + m * (side + 2 * (size_t)e))];
+                }
+}
+
+void orc_advection_action(const orc_adv *V, const double *u, double *divF)
+{
+    adv_div(V, u, divF);
+    adv_faces(V, 0, u, divF);
+}
+
+void orc_advection_bc_action(const orc_adv *V, const double *u, double *divF) { adv_faces(V, 1, u, divF); }

@@ -91,6 +91,19 @@ void orc_waveholtz_postprocess(const orc_wh *H, double *Hout, const double *u);
 void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src,
                            int var, double *F);
 
+/* ---- Advection<true>, AdvectionHomogeneousBC<true> (Advection.hpp:15-273) -
+ * On the mesh and basis of W, nvar <= 8 variables, coefficient A of shape
+ * (nvar, nvar, 2) (constant_coefficient) or (nvar, nvar, 2, n, n, nel), flux
+ * constants a = -1, b = -0.5.  n.A must be symmetric (the |n.A| of the
+ * upwind flux comes from a symmetric eigen-decomposition); create returns
+ * NULL otherwise.  action: divF += Div(u) + interior fluxes; bc_action:
+ * divF += boundary fluxes with u = 0 outside. */
+typedef struct orc_adv orc_adv;
+orc_adv *orc_advection_create(const orc_wave *W, int nvar, const double *A, int constant_coefficient);
+void orc_advection_free(orc_adv *V);
+void orc_advection_action(const orc_adv *V, const double *u, double *divF);
+void orc_advection_bc_action(const orc_adv *V, const double *u, double *divF);
+
 /* ---- reductions ------------------------------------------------------ */
 double orc_norm(long long n, const double *x);
 double orc_error(long long n, const double *x, const double *y);

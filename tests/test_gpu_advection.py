@@ -1,0 +1,92 @@
+"""Advection<true> and AdvectionHomogeneousBC<true> on the GPU against the
+oracle's restatement (1e-12 relative L2): constant and per-node
+coefficients, scalar and symmetric systems, and the wave equation's
+matrices, for which the operator is WaveEquation::action."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import mesh_case, random_state, rel_l2
+from tests.test_oracle_advection import WAVE_A, sym_coeff
+from wavedg_amd import capi
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _dev(torch, a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("name", ["skewed6x5", "skew", "L"])
+@pytest.mark.parametrize("n", [2, 4, 7, 9])
+@pytest.mark.parametrize("nvar", [1, 3])
+def test_advection_constant_coefficient(cuda, name, n, nvar):
+    torch = cuda
+    print(f"advection {name} n={n} nvar={nvar}", flush=True)
+    v, e = mesh_case(name)
+    A = sym_coeff(nvar, 100 + n + nvar)
+    ow = O.Wave(O.Mesh(v, e), n)
+    ov = O.Advection(ow, nvar, A, True)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gv = capi.Advection(gw, nvar, A, True)
+    size = nvar * gw.ndof // 3
+    u, y0 = random_state(size, 7 + n), random_state(size, 9)
+    ya, yb = _dev(torch, y0), _dev(torch, y0)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy(), ov.action(u, y0.copy())) < TOL
+    assert rel_l2(yb.cpu().numpy() - y0, ov.bc_action(u, y0.copy()) - y0) < TOL
+
+
+@pytest.mark.parametrize("n,nvar", [(3, 1), (5, 2), (8, 1)])
+def test_advection_variable_coefficient(cuda, n, nvar):
+    torch = cuda
+    v, e = mesh_case("skewed6x5")
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    rng = np.random.default_rng(n)
+    # a symmetric coefficient that varies from node to node
+    Anode = np.concatenate([sym_coeff(nvar, int(s)) * (1.0 + 0.3 * rng.standard_normal())
+                            for s in rng.integers(0, 1 << 30, n * n * nel)])
+    ow = O.Wave(O.Mesh(v, e), n)
+    ov = O.Advection(ow, nvar, Anode, False)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gv = capi.Advection(gw, nvar, Anode, False)
+    u = random_state(nvar * n * n * nel, 31)
+    ya = torch.zeros(u.size, dtype=torch.float64, device="cuda")
+    yb = torch.zeros_like(ya)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy(), ov.action(u)) < TOL
+    assert rel_l2(yb.cpu().numpy(), ov.bc_action(u)) < TOL
+
+
+def test_advection_wave_matrices_equal_wave_action(cuda):
+    torch = cuda
+    v, e = mesh_case("lshape8")
+    n = 6
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gv = capi.Advection(gw, 3, WAVE_A, True)
+    u = _dev(torch, random_state(gw.ndof, 5))
+    y1 = torch.zeros_like(u)
+    y2 = torch.zeros_like(u)
+    gv.action(u, y1)
+    gw.action(u, y2)
+    torch.cuda.synchronize()
+    assert float(torch.linalg.norm(y1 - y2) / torch.linalg.norm(y2)) < TOL
+
+
+def test_advection_rejects_bad_input(cuda):
+    v, e = mesh_case("rect5x4")
+    gw = capi.Wave2D(capi.Mesh(v, e), 3, None)
+    with pytest.raises(capi.WdgError):  # n.A not symmetric
+        capi.Advection(gw, 2, [0.0, 1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 0.0], True)
+    with pytest.raises(capi.WdgError):
+        capi.Advection(gw, 9, np.zeros(162), True)
+    g32 = capi.Wave2D(capi.Mesh(v, e), 3, None, precision=capi.F32)
+    with pytest.raises(capi.WdgError):
+        capi.Advection(g32, 1, [1.0, 0.5], True)

@@ -1,0 +1,111 @@
+"""Pins the oracle's Advection<true> and AdvectionHomogeneousBC<true>
+(Advection.hpp:15-273) by properties that fix them independently:
+
+* with the wave equation's flux matrices the operator is WaveEquation's
+  (Div<true> + EdgeFlux<true> with a = -1, b = -0.5), already pinned;
+* a variable coefficient that is the same matrix at every node gives the
+  constant-coefficient operator;
+* exactness: on an affine mesh with c(x) and u(x) linear, M^{-1} A u equals
+  -div(c u) at the nodes of every element away from the boundary (both the
+  volume term and the upwind flux are integrated exactly there);
+* conservation: the interior operator sums to zero over all nodes for every
+  variable;
+* energy: for symmetric systems the upwind operator with u = 0 outside is
+  dissipative, u^T (A + B) u < 0."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import mesh_case, random_state
+from wavedg_amd import meshgen
+
+WAVE_A = np.array([0.0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0])
+
+
+def sym_coeff(nvar: int, seed: int) -> np.ndarray:
+    """(nvar, nvar, 2) symmetric A_x, A_y flattened column-major."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((2, nvar, nvar))
+    for d in range(2):
+        M = rng.standard_normal((nvar, nvar))
+        out[d] = M + M.T
+    return np.ascontiguousarray(out.reshape(-1))  # [dim, l, k] order = A(k, l, dim) column-major
+
+
+@pytest.mark.parametrize("name,n", [("skewed6x5", 3), ("skew", 5), ("L", 4)])
+def test_advection_with_wave_matrices_is_wave_equation(name, n):
+    v, e = mesh_case(name)
+    W = O.Wave(O.Mesh(v, e), n)
+    V = O.Advection(W, 3, WAVE_A, True)
+    u = random_state(W.ndof, 2)
+    assert np.max(np.abs(V.action(u) - W.action(u))) <= 1e-14 * np.max(np.abs(W.action(u)))
+
+
+@pytest.mark.parametrize("nvar", [1, 2, 4])
+def test_variable_coefficient_reduces_to_constant(nvar):
+    v, e = mesh_case("skewed6x5")
+    n = 4
+    W = O.Wave(O.Mesh(v, e), n)
+    A = sym_coeff(nvar, 10 + nvar)
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    Avar = np.tile(A, n * n * nel)
+    Vc = O.Advection(W, nvar, A, True)
+    Vv = O.Advection(W, nvar, Avar, False)
+    u = random_state(nvar * n * n * nel, 4)
+    np.testing.assert_allclose(Vv.action(u), Vc.action(u), rtol=0, atol=1e-13 * np.abs(Vc.action(u)).max())
+    np.testing.assert_allclose(Vv.bc_action(u), Vc.bc_action(u), rtol=0, atol=1e-13 * np.abs(Vc.bc_action(u)).max())
+
+
+def test_scalar_variable_advection_exact_for_linear_fields():
+    nx = 6
+    v, e = meshgen.uniform_rect(nx, 0.0, 1.0, nx, 0.0, 1.0)
+    mesh = O.Mesh(v, e)
+    n = 4
+    W = O.Wave(mesh, n)
+    xg, _ = O.gauss_lobatto(n)
+    X = mesh.nodes(xg)  # [el, j, i, xy]
+    x, y = X[..., 0], X[..., 1]
+    cx = 0.3 + 0.5 * x - 0.2 * y
+    cy = -0.4 + 0.1 * x + 0.6 * y
+    u = 1.5 - 0.7 * x + 0.9 * y
+    nel = X.shape[0]
+    A = np.zeros((nel, n, n, 2))  # [el, j, i, dim] -> A(0, 0, dim, i, j, el)
+    A[..., 0], A[..., 1] = cx, cy
+    V = O.Advection(W, 1, A.reshape(-1), False)
+    r = W.mass_inv(V.action(u.reshape(-1)), nvar=1).reshape(nel, n, n)
+    want = -((0.5 + 0.6) * u + cx * (-0.7) + cy * 0.9)
+    interior = [el for el in range(nel) if 0 < el % nx < nx - 1 and 0 < el // nx < nx - 1]
+    assert np.max(np.abs(r[interior] - want[interior])) < 1e-12
+
+
+@pytest.mark.parametrize("nvar", [1, 3])
+def test_interior_operator_conserves(nvar):
+    v, e = mesh_case("skewed6x5")
+    n = 5
+    W = O.Wave(O.Mesh(v, e), n)
+    V = O.Advection(W, nvar, sym_coeff(nvar, 3), True)
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    u = random_state(nvar * n * n * nel, 8)
+    tot = V.action(u).reshape(-1, nvar).sum(axis=0)
+    scale = np.abs(V.action(u)).sum()
+    assert np.all(np.abs(tot) < 1e-13 * scale)
+
+
+@pytest.mark.parametrize("nvar", [1, 2, 3])
+def test_upwind_with_zero_inflow_dissipates(nvar):
+    v, e = mesh_case("skew")
+    n = 4
+    W = O.Wave(O.Mesh(v, e), n)
+    V = O.Advection(W, nvar, sym_coeff(nvar, 21 + nvar), True)
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    for seed in range(3):
+        u = random_state(nvar * n * n * nel, seed)
+        assert u @ (V.action(u) + V.bc_action(u)) < 0.0
+
+
+def test_rejects_non_symmetric_normal_matrix():
+    v, e = mesh_case("rect5x4")
+    W = O.Wave(O.Mesh(v, e), 3)
+    A = np.array([0.0, 1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 0.0])  # A_x = [[0, 2], [1, 0]]
+    with pytest.raises(ValueError):
+        O.Advection(W, 2, A, True)

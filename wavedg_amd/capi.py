@@ -87,6 +87,10 @@ _SIGS = {
     "wdg_hip_waveholtz_checkpoint": [_vp, ctypes.c_char_p, _vp, _i, _d],
     "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
+    "wdg_hip_advection_create": [_vp, _i, _dp, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_advection_action": [_vp, _vp, _vp, _vp],
+    "wdg_hip_advection_bc_action": [_vp, _vp, _vp, _vp],
+    "wdg_hip_advection_destroy": [_vp],
     "wdg_hip_schedule_count": [_ip, _ip],
     "wdg_hip_wave2d_set_schedule": [_vp, _i],
     "wdg_hip_wave2d_get_schedule": [_vp, _ip, _ip],
@@ -354,6 +358,35 @@ class Wave2D:
     def load_vector_gauss(self, sources, F, var: int = 0, stream=None):
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
         _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, dev_ptr(F),
+              stream if stream is not None else cur_stream())
+
+
+class Advection:
+    """Advection<true> + AdvectionHomogeneousBC<true> on an FP64 wave handle's
+    mesh and basis.  A_colmajor is the coefficient flattened in the
+    reference's column-major order: (n_var, n_var, 2) when constant, else
+    (n_var, n_var, 2, n, n, n_elem); n.A must be symmetric."""
+
+    def __init__(self, wave: Wave2D, n_var: int, A_colmajor, constant_coefficient: bool = True):
+        self.wave = wave  # the handle must outlive this object
+        self.n_var = n_var
+        A = np.ascontiguousarray(np.asarray(A_colmajor, dtype=np.float64).reshape(-1))
+        h = _vp()
+        _call("wdg_hip_advection_create", wave.h, n_var, _dptr(A), 1 if constant_coefficient else 0, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wdg_hip_advection_destroy(self.h)
+            self.h = None
+
+    def action(self, u, y, stream=None) -> None:
+        """y += Div(u) + interior edge fluxes."""
+        _call("wdg_hip_advection_action", self.h, dev_ptr(u), dev_ptr(y), stream if stream is not None else cur_stream())
+
+    def bc_action(self, u, y, stream=None) -> None:
+        """y += boundary edge fluxes with u = 0 outside."""
+        _call("wdg_hip_advection_bc_action", self.h, dev_ptr(u), dev_ptr(y),
               stream if stream is not None else cur_stream())
 
 

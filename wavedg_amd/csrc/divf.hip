@@ -1,5 +1,7 @@
 // DivF2D<true>::action (DivF.hpp:207-274) on the GPU for the linear flux
-// functor f(x, u) = (A u, B u) with constant (n_var, n_var) matrices: the
+// functor f(x, u) = (A u, B u) with constant (n_var, n_var) matrices, or
+// matrices given per node (the volume term of Advection<true>,
+// Div.cpp:5-53, 131-179, with a variable coefficient): the
 // wave equation's volume term is the case A, B = its flux matrices, linear
 // advection the case A = a_x I, B = a_y I.  One thread per node; a
 // workgroup holds Shape<N>::EPB whole elements.  Each thread evaluates the
@@ -42,12 +44,20 @@ __global__ __launch_bounds__(Shape<N>::NT) void divf_linear_kernel(const DivFArg
         R uv[kMaxVar];
 #pragma unroll
         for (int d = 0; d < kMaxVar; ++d) uv[d] = d < nv ? uq[d] : (R)0;
+        const double* An = a.Anode ? a.Anode + (size_t)2 * nv * nv * (node + (size_t)NN * e) : nullptr;
         for (int d = 0; d < nv; ++d) {
             R fx = 0, fy = 0;
+            if (An) {
+                for (int k = 0; k < nv; ++k) {
+                    fx = fma((R)An[d + nv * k], uv[k], fx);
+                    fy = fma((R)An[nv * nv + d + nv * k], uv[k], fy);
+                }
+            } else {
 #pragma unroll
-            for (int k = 0; k < kMaxVar; ++k) {
-                fx = fma((R)a.A[d + kMaxVar * k], uv[k], fx);
-                fy = fma((R)a.B[d + kMaxVar * k], uv[k], fy);
+                for (int k = 0; k < kMaxVar; ++k) {
+                    fx = fma((R)a.A[d + kMaxVar * k], uv[k], fx);
+                    fy = fma((R)a.B[d + kMaxVar * k], uv[k], fy);
+                }
             }
             s_F[(el * kMaxVar + d) * NN + j * N + i] = Ye * fx - Xe * fy;
             s_G[(el * kMaxVar + d) * NN + i * N + j] = Xx * fy - Yx * fx;

@@ -247,4 +247,52 @@ bool mass_fast_diag(int n, const double* xb, int nq, const double* xq, const dou
     return true;
 }
 
+bool symmetric_abs(int m, const double* C, double* absC)
+{
+    if (m < 1 || m > 8) return false;
+    double A[64], R[64];
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            if (C[i + m * j] != C[j + m * i]) return false;
+            A[i + m * j] = C[i + m * j];
+            R[i + m * j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int q = 0; q < m; ++q)
+            for (int p = 0; p < q; ++p) off += A[p + m * q] * A[p + m * q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = A[p + m * q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (A[q + m * q] - A[p + m * p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double akp = A[k + m * p], akq = A[k + m * q];
+                    A[k + m * p] = c * akp - s * akq;
+                    A[k + m * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double apk = A[p + m * k], aqk = A[q + m * k];
+                    A[p + m * k] = c * apk - s * aqk;
+                    A[q + m * k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double rkp = R[k + m * p], rkq = R[k + m * q];
+                    R[k + m * p] = c * rkp - s * rkq;
+                    R[k + m * q] = s * rkp + c * rkq;
+                }
+            }
+    }
+    for (int d = 0; d < m; ++d)
+        for (int c = 0; c < m; ++c) {
+            double v = 0.0;
+            for (int k = 0; k < m; ++k) v += R[c + m * k] * std::fabs(A[k + m * k]) * R[d + m * k];
+            absC[c + m * d] = v;
+        }
+    return true;
+}
+
 }  // namespace wdg

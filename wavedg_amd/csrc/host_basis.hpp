@@ -36,4 +36,8 @@ void lagrange_interpolation_matrix(int ngrid, const double* xg, int neval, const
 bool mass_fast_diag(int n, const double* xb, int nq, const double* xq, const double* wq, double* A, double* B,
                     double* V, double* lam);
 
+// |C| = R |diag(e)| R' for a symmetric m x m matrix C = R diag(e) R'
+// (column-major, m <= 8) by cyclic Jacobi; false if C is not symmetric.
+bool symmetric_abs(int m, const double* C, double* absC);
+
 }  // namespace wdg

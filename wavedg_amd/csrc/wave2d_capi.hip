@@ -1372,6 +1372,136 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const doubl
     });
 }
 
+// ------------------------------------------------------------ advection
+struct wdg_advection_s {
+    wdg_wave2d op = nullptr;
+    int nv = 0, cc = 1;
+    double A0[64] = {}, A1[64] = {};  // constant coefficient, padded to DivFArgs::kMaxVar
+    double* d_Anode = nullptr;        // variable coefficient (nv, nv, 2, n, n, nel)
+    double* d_H = nullptr;            // face matrices (2, nv, nv) per element, side, node
+    ~wdg_advection_s()
+    {
+        dfree(d_Anode);
+        dfree(d_H);
+    }
+};
+
+int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int constant_coefficient,
+                             wdg_advection* out)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!A || !out) fail(WDG_ERR_INVALID, "null argument");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (op->exact || op->prec != WDG_F64)
+            fail(WDG_ERR_INVALID, "advection needs an FP64 approx_quad = true wave handle");
+        const int N = op->n, NN = N * N, nel = op->nel, m = n_var, v2 = m * m;
+        auto adv = std::make_unique<wdg_advection_s>();
+        adv->op = op;
+        adv->nv = m;
+        adv->cc = constant_coefficient ? 1 : 0;
+        std::vector<double> geom(6 * (size_t)nel);
+        std::vector<int32_t> nbr(4 * (size_t)nel), info(4 * (size_t)nel);
+        hip_check(hipMemcpy(geom.data(), op->d_geom, sizeof(double) * geom.size(), hipMemcpyDeviceToHost), "geometry");
+        hip_check(hipMemcpy(nbr.data(), op->d_nbr, sizeof(int32_t) * nbr.size(), hipMemcpyDeviceToHost), "nbr");
+        hip_check(hipMemcpy(info.data(), op->d_info, sizeof(int32_t) * info.size(), hipMemcpyDeviceToHost), "info");
+        auto coef = [&](int e, int node) { return adv->cc ? A : A + (size_t)2 * v2 * (node + (size_t)NN * e); };
+        auto side_node = [&](int s, int k) {
+            const int i = (s == 0 || s == 2) ? k : (s == 1 ? N - 1 : 0);
+            const int j = (s == 1 || s == 3) ? k : (s == 2 ? N - 1 : 0);
+            return i + N * j;
+        };
+        // H_own = w ds (a/2 C + b |C|), H_other = w ds (a/2 C' - b |C'|), a = -1, b = -0.5,
+        // C = n.A at this node, C' = n.A at the other side's node, n this element's outward normal
+        std::vector<double> H((size_t)nel * 4 * N * 2 * v2, 0.0);
+        double C[64], absC[64];
+        auto normal_matrix = [&](const double* Ak, double n0, double n1) {
+            for (int q = 0; q < v2; ++q) C[q] = n0 * Ak[q] + n1 * Ak[v2 + q];
+            if (!wdg::symmetric_abs(m, C, absC)) fail(WDG_ERR_INVALID, "n.A is not symmetric at a face node");
+        };
+        for (int e = 0; e < nel; ++e) {
+            const double* g = &geom[6 * (size_t)e];
+            for (int s = 0; s < 4; ++s) {
+                const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
+                const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
+                const double dy = 2.0 * ((s & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
+                const double len = std::sqrt(dx * dx + dy * dy), sg = (s >= 2) ? -1.0 : 1.0;
+                const double n0 = sg * dy / len, n1 = -sg * dx / len, ds = 0.5 * len;
+                const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+                for (int k = 0; k < N; ++k) {
+                    double* Hk = &H[(((size_t)e * 4 + s) * N + k) * 2 * v2];
+                    const double wds = op->w[k] * ds;
+                    normal_matrix(coef(e, side_node(s, k)), n0, n1);
+                    for (int q = 0; q < v2; ++q) Hk[q] = wds * (-0.5 * C[q] - 0.5 * absC[q]);
+                    if (nb >= 0) {
+                        const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                        normal_matrix(coef(nb, side_node(inf & 3, kk)), n0, n1);
+                        for (int q = 0; q < v2; ++q) Hk[v2 + q] = wds * (-0.5 * C[q] + 0.5 * absC[q]);
+                    }
+                }
+            }
+        }
+        adv->d_H = static_cast<double*>(dmalloc(sizeof(double) * H.size()));
+        hip_check(hipMemcpy(adv->d_H, H.data(), sizeof(double) * H.size(), hipMemcpyHostToDevice), "upload H");
+        if (adv->cc) {
+            for (int c = 0; c < m; ++c)
+                for (int r = 0; r < m; ++r) {
+                    adv->A0[r + wdg::DivFArgs::kMaxVar * c] = A[r + m * c];
+                    adv->A1[r + wdg::DivFArgs::kMaxVar * c] = A[v2 + r + m * c];
+                }
+        } else {
+            const size_t na = (size_t)2 * v2 * NN * nel;
+            adv->d_Anode = static_cast<double*>(dmalloc(sizeof(double) * na));
+            hip_check(hipMemcpy(adv->d_Anode, A, sizeof(double) * na, hipMemcpyHostToDevice), "upload A");
+        }
+        *out = adv.release();
+    });
+}
+
+static int advection_launch(wdg_advection adv, const void* d_u, void* d_divF, bool volume, bool interior,
+                            bool boundary, void* stream)
+{
+    return guarded([&] {
+        if (!adv) fail(WDG_ERR_INVALID, "null advection handle");
+        if (!d_u || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
+        wdg_wave2d op = adv->op;
+        check_op(op);
+        const hipStream_t s = as_stream(stream);
+        if (volume) {
+            wdg::DivFArgs a{};
+            a.geom = op->d_geom;
+            a.basis = op->d_basis;
+            a.u = d_u;
+            a.out = d_divF;
+            a.nel = op->nel;
+            a.nv = adv->nv;
+            std::memcpy(a.A, adv->A0, sizeof(a.A));
+            std::memcpy(a.B, adv->A1, sizeof(a.B));
+            a.Anode = adv->d_Anode;
+            wdg::launch_divf_linear(op->n, 0, a, s);
+        }
+        wdg::launch_advect_face(op->n, adv->nv, static_cast<const double*>(d_u), static_cast<double*>(d_divF),
+                                op->d_nbr, op->d_info, adv->d_H, op->nel, interior ? 1 : 0, boundary ? 1 : 0, s);
+        hip_check(hipGetLastError(), "advection kernel launch");
+    });
+}
+
+int wdg_hip_advection_action(wdg_advection adv, const void* d_u, void* d_divF, void* stream)
+{
+    return advection_launch(adv, d_u, d_divF, true, true, false, stream);
+}
+
+int wdg_hip_advection_bc_action(wdg_advection adv, const void* d_u, void* d_divF, void* stream)
+{
+    return advection_launch(adv, d_u, d_divF, false, false, true, stream);
+}
+
+int wdg_hip_advection_destroy(wdg_advection adv)
+{
+    delete adv;
+    return WDG_OK;
+}
+
 // ------------------------------------------------------------ file formats
 int wdg_hip_to_file(const char* path, const void* d_x, long long n, int precision)
 {

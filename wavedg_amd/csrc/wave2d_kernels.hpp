@@ -633,8 +633,17 @@ struct DivFArgs {
     void* out;            // divF += ...
     int nel, nv;
     double A[kMaxVar * kMaxVar], B[kMaxVar * kMaxVar];
+    // if set, the coefficient varies by node: (nv, nv, 2, n, n, nel) with
+    // f_d = sum_c Anode(d, c, 0) u_c, g_d = sum_c Anode(d, c, 1) u_c (FP64)
+    const double* Anode;
 };
 void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
+
+// Advection<true> / AdvectionHomogeneousBC<true> face terms (advect.hip):
+// y(node) += H_own u(node) + H_other u(neighbour node) for every node of the
+// selected sides, H per (element, side, node): (2, nv, nv) column-major.
+void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
+                        const double* H, int nel, int interior, int boundary, hipStream_t s);
 
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
