@@ -159,6 +159,11 @@ int wdg_hip_wavebc_action(wdg_wave2d op, const void *d_x, void *d_y, void *strea
 int wdg_hip_mass_inv(wdg_wave2d op, void *d_x, void *stream);
 /* MassMatrix<true>::action (MassMatrix.cpp:52-67): d_y <- M d_x. */
 int wdg_hip_mass_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
+/* MassMatrix<true>::action / inv with n_var (MassMatrix.cpp:52-82): d_y <- M d_x
+ * (inverse = 0) or M^{-1} d_x (inverse != 0) for arrays of n_var variables
+ * per node, (n_var, n, n, n_elem); d_x == d_y is allowed.  Exact-quadrature
+ * handles take n_var = 3 only. */
+int wdg_hip_mass_apply(wdg_wave2d op, int n_var, const void *d_x, void *d_y, int inverse, void *stream);
 /* The time derivative of the examples and WaveHoltz (WaveHoltz.cpp:57-70):
  * d_y <- M^{-1} (A d_x + B d_x + s d_F), fused.  d_F may be NULL; it has the
  * handle's precision. */

@@ -133,6 +133,32 @@ int main(int argc, char** argv)
         }
         to_file(dir + "/pi0_exact.bin", ndof, pi0x.data());
         to_file(dir + "/W_exact.bin", ndof, Wx.data());
+        // --- scalar advection as examples/advec2d.cpp builds it (GLL basis,
+        // approx_quad = true, variable coefficient c = r (y, -x)), one time
+        // derivative m \ (a u + bc u)
+        {
+            const int n3 = 5;
+            auto basis3 = QuadratureRule::quadrature_rule(n3, QuadratureRule::GaussLobatto);
+            Mesh2D mesh3 = Mesh2D::uniform_rect(6, -1.0, 1.0, 6, -1.0, 1.0);
+            const std::vector<double> x3 = mesh3.physical_coordinates(basis3);  // (2, n, n, nel)
+            const size_t np = (size_t)n3 * n3 * mesh3.n_elem();
+            std::vector<double> c(2 * np), u3(np), du3(np, 0.0);
+            for (size_t q = 0; q < np; ++q) {
+                const double x = x3[2 * q], y = x3[2 * q + 1], r = std::hypot(x, y);
+                c[2 * q] = r * y;
+                c[2 * q + 1] = -r * x;
+                u3[q] = r * std::exp(-10.0 * r * r) * std::cos(3.0 * std::atan2(y, x));
+            }
+            Advection<true> adv(1, mesh3, basis3, c.data(), false);
+            AdvectionHomogeneousBC<true> adv_bc(1, mesh3, basis3, c.data(), false);
+            MassMatrix<true> m3(mesh3, basis3);
+            adv.action(u3.data(), du3.data());
+            adv_bc.action(u3.data(), du3.data());
+            m3.inv(1, du3.data());
+            to_file(dir + "/adv_c.bin", c.size(), c.data());
+            to_file(dir + "/adv_u.bin", np, u3.data());
+            to_file(dir + "/adv_dudt.bin", np, du3.data());
+        }
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {
         std::fprintf(stderr, "dg::hip::Error %d: %s\n", e.code, e.what());

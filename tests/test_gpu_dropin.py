@@ -72,3 +72,11 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     for _ in range(2):
         Wk = Hx.S(Wk) + pi0x
     assert rel_l2(_read(d, "W_exact.bin"), Wk) < 1e-12
+    # scalar advection with a variable coefficient, as examples/advec2d.cpp
+    v3, e3 = meshgen.uniform_rect(6, -1.0, 1.0, 6, -1.0, 1.0)
+    W3 = O.Wave(O.Mesh(v3, e3), 5)
+    c = _read(d, "adv_c.bin")
+    A3 = O.Advection(W3, 1, c, False)
+    u3 = _read(d, "adv_u.bin")
+    want = W3.mass_inv(A3.action(u3) + A3.bc_action(u3), nvar=1)
+    assert rel_l2(_read(d, "adv_dudt.bin"), want) < 1e-12

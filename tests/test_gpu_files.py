@@ -74,3 +74,22 @@ def test_waveholtz_checkpoint_resume_bitwise(cuda, tmp_path):
     w3, wh3, _ = _setup(torch, v, e, n, 6.0 * np.pi)
     with pytest.raises(capi.WdgError):
         wh3.resume(p, torch.empty_like(W2))
+
+
+def test_mass_apply_n_var(cuda):
+    """MassMatrix<true>::action / inv with n_var != 3 against the oracle."""
+    from oracle import oracle as O
+    torch = cuda
+    v, e = mesh_case("skewed6x5")
+    n = 4
+    ow = O.Wave(O.Mesh(v, e), n)
+    w = capi.Wave2D(capi.Mesh(v, e), n, None)
+    for nv in (1, 2, 5):
+        x = np.random.default_rng(nv).standard_normal(nv * w.ndof // 3)
+        dx = torch.tensor(x, device="cuda")
+        y = torch.empty_like(dx)
+        w.mass_apply(nv, dx, y, False)
+        w.mass_apply(nv, dx, dx, True)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(y.cpu().numpy(), ow.mass_action(x, nvar=nv), rtol=1e-13, atol=0)
+        np.testing.assert_allclose(dx.cpu().numpy(), ow.mass_inv(x, nvar=nv), rtol=1e-13, atol=0)

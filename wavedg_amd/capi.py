@@ -87,6 +87,7 @@ _SIGS = {
     "wdg_hip_waveholtz_checkpoint": [_vp, ctypes.c_char_p, _vp, _i, _d],
     "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
+    "wdg_hip_mass_apply": [_vp, _i, _vp, _vp, _i, _vp],
     "wdg_hip_advection_create": [_vp, _i, _dp, _i, ctypes.POINTER(_vp)],
     "wdg_hip_advection_action": [_vp, _vp, _vp, _vp],
     "wdg_hip_advection_bc_action": [_vp, _vp, _vp, _vp],
@@ -324,6 +325,11 @@ class Wave2D:
         ms, nk = _d(), _i()
         _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
         return ms.value, nk.value
+
+    def mass_apply(self, n_var: int, x, y, inverse: bool, stream=None) -> None:
+        """y <- M x or M^{-1} x for n_var variables per node (x may be y)."""
+        _call("wdg_hip_mass_apply", self.h, n_var, dev_ptr(x), dev_ptr(y), 1 if inverse else 0,
+              stream if stream is not None else cur_stream())
 
     def divf_linear(self, n_var: int, A, B, u, out, stream=None) -> None:
         """out += DivF2D(u) for the flux (A u, B u); A, B as (n_var, n_var) arrays."""

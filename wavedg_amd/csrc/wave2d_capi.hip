@@ -283,8 +283,8 @@ void apply_op(wdg_wave2d op, int kid, const KArgs& a, hipStream_t s)
 
 // ------------------------------------------------------------ small kernels
 template <typename R>
-__global__ void k_mass(const R* __restrict__ x, R* __restrict__ y, const double* __restrict__ corners,
-                       const double* __restrict__ basis, int N, long long nnodes, int inv)
+__global__ void k_mass(const R* x, R* y, const double* __restrict__ corners, const double* __restrict__ basis, int N,
+                       long long nnodes, int nv, int inv)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nnodes) return;
@@ -299,11 +299,11 @@ __global__ void k_mass(const R* __restrict__ x, R* __restrict__ y, const double*
     const double J2 = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
     const double J3 = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
     const double m = basis[N + i] * basis[N + j] * (J0 * J3 - J1 * J2);
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < nv; ++d) {  // x and y may alias (in-place inverse)
         if (inv)
-            y[3 * q + d] = (R)((double)x[3 * q + d] / m);
+            y[nv * q + d] = (R)((double)x[nv * q + d] / m);
         else
-            y[3 * q + d] = (R)(m * (double)x[3 * q + d]);
+            y[nv * q + d] = (R)(m * (double)x[nv * q + d]);
     }
 }
 
@@ -974,14 +974,16 @@ int wdg_hip_wave2d_rhs(wdg_wave2d op, const void* d_x, void* d_y, const void* d_
     return op_launch(op, d_F ? wdg::K_RHS_F : wdg::K_RHS, d_x, d_y, d_F, s, stream);
 }
 
-static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* stream)
+static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* stream, int nv = 3)
 {
     return guarded([&] {
         check_op(op);
         if (!x || !y) fail(WDG_ERR_INVALID, "null device pointer");
+        if (nv < 1 || nv > 64) fail(WDG_ERR_INVALID, "n_var out of range 1..64");
         const long long nn = op->ndof / 3;
         const hipStream_t s = as_stream(stream);
         if (op->exact) {
+            if (nv != 3) fail(WDG_ERR_INVALID, "the exact mass matrix is implemented for 3 variables");
             KArgs a = op->base_args();
             a.u_in = x;
             a.u_out = y;
@@ -991,10 +993,10 @@ static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* str
         }
         if (op->prec == WDG_F32)
             hipLaunchKernelGGL(k_mass<float>, dim3(grid_for(nn)), dim3(256), 0, s, static_cast<const float*>(x),
-                               static_cast<float*>(y), op->d_corners, op->d_basis, op->n, nn, inv);
+                               static_cast<float*>(y), op->d_corners, op->d_basis, op->n, nn, nv, inv);
         else
             hipLaunchKernelGGL(k_mass<double>, dim3(grid_for(nn)), dim3(256), 0, s, static_cast<const double*>(x),
-                               static_cast<double*>(y), op->d_corners, op->d_basis, op->n, nn, inv);
+                               static_cast<double*>(y), op->d_corners, op->d_basis, op->n, nn, nv, inv);
         hip_check(hipGetLastError(), "mass kernel launch");
     });
 }
@@ -1004,6 +1006,11 @@ int wdg_hip_mass_inv(wdg_wave2d op, void* d_x, void* stream) { return mass_launc
 int wdg_hip_mass_action(wdg_wave2d op, const void* d_x, void* d_y, void* stream)
 {
     return mass_launch(op, d_x, d_y, 0, stream);
+}
+
+int wdg_hip_mass_apply(wdg_wave2d op, int n_var, const void* d_x, void* d_y, int inverse, void* stream)
+{
+    return mass_launch(op, d_x, d_y, inverse ? 1 : 0, stream, n_var);
 }
 
 int wdg_hip_rk4_steps(wdg_wave2d op, double dt, double* t, void* d_u, int n_steps, const void* d_F, double omega,

@@ -296,8 +296,9 @@ private:
 
 // MassMatrix<true> (MassMatrix.hpp:77, MassMatrix.cpp:6-82), the diagonal
 // w_i w_j det J, and MassMatrix<false> (MassMatrix.cpp:85-227), the exact
-// block matrix (applied and inverted on the GPU without stored factors);
-// n_var is 3.  quad is ignored for <true>, as in the reference.
+// block matrix (applied and inverted on the GPU without stored factors).
+// The n_var overloads take any number of variables per node (3 only for
+// <false>); quad is ignored for <true>, as in the reference.
 template <bool ApproxQuadrature>
 class MassMatrix : public InvertibleOperator {
 public:
@@ -314,7 +315,15 @@ public:
         check(wdg_hip_mass_action(op_.get(), op_.x.get(), op_.y.get(), nullptr));
         op_.y.download(y, bytes);
     }
-    void action(int, const double* x, double* y) const override { action(x, y); }
+    void action(int n_var, const double* x, double* y) const override
+    {
+        const size_t bytes = op_.state_bytes() / 3 * n_var;
+        op_.x.resize(bytes);
+        op_.y.resize(bytes);
+        op_.x.upload(x, bytes);
+        check(wdg_hip_mass_apply(op_.get(), n_var, op_.x.get(), op_.y.get(), 0, nullptr));
+        op_.y.download(y, bytes);
+    }
     void inv(double* x) const override
     {
         const size_t bytes = op_.state_bytes();
@@ -323,11 +332,110 @@ public:
         inv_device(op_.x.get());
         op_.x.download(x, bytes);
     }
-    void inv(int, double* x) const override { inv(x); }
+    void inv(int n_var, double* x) const override
+    {
+        const size_t bytes = op_.state_bytes() / 3 * n_var;
+        op_.x.resize(bytes);
+        op_.x.upload(x, bytes);
+        check(wdg_hip_mass_apply(op_.get(), n_var, op_.x.get(), op_.x.get(), 1, nullptr));
+        op_.x.download(x, bytes);
+    }
     void inv_device(void* d_x, void* stream = nullptr) const { check(wdg_hip_mass_inv(op_.get(), d_x, stream)); }
 
 private:
     detail::Wave2D op_;
+};
+
+// Advection<true> (Advection.hpp:15-140) and AdvectionHomogeneousBC<true>
+// (Advection.hpp:142-270): n_var <= 8 variables, coefficient `a` of shape
+// (n_var, n_var, 2) if constant_coefficient else (n_var, n_var, 2, n, n,
+// n_elem), column-major as in the reference; upwind flux.  n.A must be
+// symmetric at every face node.  Only the approx_quad = true operators exist
+// on the GPU.
+namespace detail {
+class AdvectionHandle {
+public:
+    AdvectionHandle(const Mesh2D& mesh, const QuadratureRule* basis, int n_var, const double* a, bool cc)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64), n_var_(n_var)
+    {
+        wdg_advection h = nullptr;
+        check(wdg_hip_advection_create(op_.get(), n_var, a, cc ? 1 : 0, &h));
+        h_.reset(h);
+    }
+    wdg_advection get() const { return h_.get(); }
+    size_t bytes() const { return (size_t)op_.ndof() / 3 * n_var_ * sizeof(double); }
+    mutable DeviceBuffer x, y;
+
+private:
+    struct Del {
+        void operator()(wdg_advection p) const { wdg_hip_advection_destroy(p); }
+    };
+    Wave2D op_;  // declared before h_: destroyed after it
+    int n_var_;
+    std::unique_ptr<wdg_advection_s, Del> h_;
+};
+}  // namespace detail
+
+template <bool ApproxQuadrature>
+class Advection : public Operator {
+    static_assert(ApproxQuadrature, "dg::hip: only Advection<true> is implemented on the GPU");
+
+public:
+    Advection(int n_var, const Mesh2D& mesh, const QuadratureRule* basis, const double* a, bool constant_coefficient,
+              const QuadratureRule* quad = nullptr)
+        : h_(mesh, basis, n_var, a, constant_coefficient)
+    {
+        (void)quad;
+    }
+    void action(const double* u, double* divF) const override
+    {
+        const size_t bytes = h_.bytes();
+        h_.x.resize(bytes);
+        h_.y.resize(bytes);
+        h_.x.upload(u, bytes);
+        h_.y.upload(divF, bytes);
+        action_device(h_.x.get(), h_.y.get());
+        h_.y.download(divF, bytes);
+    }
+    void action(int, const double* u, double* divF) const override { action(u, divF); }
+    void action_device(const void* d_u, void* d_divF, void* stream = nullptr) const
+    {
+        check(wdg_hip_advection_action(h_.get(), d_u, d_divF, stream));
+    }
+
+private:
+    detail::AdvectionHandle h_;
+};
+
+template <bool ApproxQuadrature>
+class AdvectionHomogeneousBC : public Operator {
+    static_assert(ApproxQuadrature, "dg::hip: only AdvectionHomogeneousBC<true> is implemented on the GPU");
+
+public:
+    AdvectionHomogeneousBC(int n_var, const Mesh2D& mesh, const QuadratureRule* basis, const double* a,
+                           bool constant_coefficient, const QuadratureRule* quad = nullptr)
+        : h_(mesh, basis, n_var, a, constant_coefficient)
+    {
+        (void)quad;
+    }
+    void action(const double* u, double* divF) const override
+    {
+        const size_t bytes = h_.bytes();
+        h_.x.resize(bytes);
+        h_.y.resize(bytes);
+        h_.x.upload(u, bytes);
+        h_.y.upload(divF, bytes);
+        action_device(h_.x.get(), h_.y.get());
+        h_.y.download(divF, bytes);
+    }
+    void action(int, const double* u, double* divF) const override { action(u, divF); }
+    void action_device(const void* d_u, void* d_divF, void* stream = nullptr) const
+    {
+        check(wdg_hip_advection_bc_action(h_.get(), d_u, d_divF, stream));
+    }
+
+private:
+    detail::AdvectionHandle h_;
 };
 
 // The semi-discrete wave system u' = M^{-1}(A + B) u advanced by the
