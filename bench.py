@@ -286,6 +286,42 @@ def cnn_rates(torch, capi, M: int = 64, B: int = 16, steps: int = 50):
             "flops_per_train_step": f_train, "flops_per_forward": f_fwd}
 
 
+def advection_rate(torch, capi, mesh, n, reps: int):
+    """Scalar advection with the rotating coefficient c = r (y, -x) of the
+    reference's advec2d example, per-node coefficient, on the benchmark mesh:
+    time of one Advection::action + AdvectionHomogeneousBC::action."""
+    wave = capi.Wave2D(mesh, n, None)
+    X = mesh.nodes(n)  # [el, j, i, xy]
+    x, y = X[..., 0] - 0.5, X[..., 1] - 0.5
+    r = np.hypot(x, y)
+    A = np.stack([r * y, -r * x], axis=-1)  # (el, j, i, dim) = A(0, 0, dim, i, j, el)
+    adv = capi.Advection(wave, 1, A.reshape(-1), False)
+    u = torch.tensor((r * np.exp(-10 * r * r)).reshape(-1), dtype=torch.float64, device="cuda")
+    out = torch.zeros_like(u)
+    for _ in range(3):
+        adv.action(u, out)
+        adv.bc_action(u, out)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        adv.action(u, out)
+        adv.bc_action(u, out)
+    ev1.record()
+    torch.cuda.synchronize()
+    us = ev0.elapsed_time(ev1) * 1e3 / reps
+    npts, nel = u.numel(), wave.n_elem
+    # volume pass: u read, y read + write, 2 coefficient values per node and
+    # 48 B of map coefficients per element; face pass: connectivity (32 B per
+    # element), own and neighbour trace and 2 face matrices per side node, the
+    # element's y read + write; the boundary pass adds little
+    algo = npts * 8 * (1 + 2 + 2) + nel * 80 + nel * 4 * n * 8 * (2 + 2) + npts * 8 * 2
+    return {"workload": f"scalar advection, c = r (y, -x), per-node coefficient, same {int(round(nel ** 0.5))}^2 mesh, "
+                        f"P={n - 1}", "avg_action_us": us, "dof_per_s": npts / (us * 1e-6),
+            "bytes_per_action": algo, "achieved_GBs": algo / (us * 1e-6) / 1e9,
+            "frac": algo / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int, approx_quad: bool = True):
     from oracle import oracle as O
 
@@ -316,6 +352,8 @@ def main() -> int:
     ap.add_argument("--exact-steps", type=int, default=400,
                     help="RK4 steps of the approx_quad = false (exact quadrature) path on the same mesh (0 = skip)")
     ap.add_argument("--exact-cpu-nx", type=int, default=128, help="mesh size of the exact path's CPU-oracle sample")
+    ap.add_argument("--advection-reps", type=int, default=200,
+                    help="timed Advection + AdvectionHomogeneousBC actions on the same mesh (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
@@ -490,6 +528,10 @@ def main() -> int:
                           f"{args.exact_cpu_nx}^2 quads, {args.cpu_steps} RK4 step(s), {sx:.1f} s",
             }
             print(f"[bench] exact cpu: {vx:.3e} DoF*stage/s", file=sys.stderr, flush=True)
+
+    if args.advection_reps > 0:
+        out["advection"] = advection_rate(torch, capi, mesh, n, args.advection_reps)
+        print(f"[bench] advection: {out['advection']['avg_action_us']:.1f} us per action", file=sys.stderr, flush=True)
 
     if args.learned > 0:
         from wavedg_amd import warmstart
