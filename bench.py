@@ -354,7 +354,7 @@ def main() -> int:
     ap.add_argument("--exact-cpu-nx", type=int, default=128, help="mesh size of the exact path's CPU-oracle sample")
     ap.add_argument("--advection-reps", type=int, default=200,
                     help="timed Advection + AdvectionHomogeneousBC actions on the same mesh (0 = skip)")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="RK4 steps of the CPU-oracle sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
     ap.add_argument("--only-sweep", action="store_true", help="skip the headline workload")

@@ -433,7 +433,6 @@ void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const
 {
     op->ensure(op->d_tmp, op->state_bytes());
     op->ensure(op->d_du, op->state_bytes());
-    const int f32 = op->prec == WDG_F32;
     KArgs a = op->base_args();
     a.du = op->d_du;
     a.dt = dt;

@@ -215,7 +215,6 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                                                         size_t& g_out, bool& active_out, Prefetch&& prefetch)
 {
     constexpr int NN = N * N;
-    constexpr int EPB = Shape<N, TGT>::EPB;
     constexpr int NT = Shape<N, TGT>::NT;
     constexpr int NF = 4 * N;               // face nodes per element
     constexpr int FP = (NF + NN - 1) / NN;  // face nodes per thread (1 for N >= 4)
