@@ -90,6 +90,9 @@ constexpr int kModeOf[K_COUNT] = {
 #ifndef WDG_MIN_WAVES
 #define WDG_MIN_WAVES 1  // occupancy hint: minimum waves per SIMD
 #endif
+#ifndef WDG_MIN_WAVES_F32
+#define WDG_MIN_WAVES_F32 WDG_MIN_WAVES  // the same for the FP32 stage kernels
+#endif
 #ifndef WDG_FACE_GEOM_DOUBLE
 #define WDG_FACE_GEOM_DOUBLE 0  // 1: face normals and weights in double for every precision
 #endif
@@ -453,7 +456,8 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
 }
 
 template <int N, typename R, int MODE, int TGT = 256>
-__global__ __launch_bounds__((Shape<N, TGT>::NT), WDG_MIN_WAVES) void wave_kernel(const KArgs a)
+__global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVES_F32 : WDG_MIN_WAVES)) void wave_kernel(
+    const KArgs a)
 {
     constexpr int EPB = Shape<N, TGT>::EPB;
     constexpr bool VOL = (MODE & M_VOL) != 0;
