@@ -137,6 +137,19 @@ int wdg_hip_wave2d_action(wdg_wave2d op, const void *d_x, void *d_y, void *strea
  * volume term of WaveEquation::action. */
 int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double *A, const double *B, const void *d_u, void *d_divF,
                           void *stream);
+/* DivF2D<true>::action (wavedg/DivF.hpp:207-274) with the compressible
+ * Euler flux of examples/cl2d.cpp:57-81 for q = (rho, rho u, rho v, E) per
+ * node, (4, n, n, n_elem), ratio of specific heats gamma:
+ * d_divF += (F(q), grad v).  FP64 approx_quad = true handles. */
+int wdg_hip_divf2d_euler(wdg_wave2d op, double gamma, const void *d_q, void *d_divF, void *stream);
+/* The face part of examples/cl2d.cpp's time derivative (cl2d.cpp:180-215):
+ * EdgeFluxF2D<true>::action (wavedg/EdgeFluxF.hpp:136-171) with the local
+ * Lax-Friedrichs flux (cl2d.cpp:98-111) on the edges of face_type
+ * (WDG_INTERIOR, or WDG_BOUNDARY with the exterior state a copy of the
+ * interior one), negated and lifted: d_divF -= sum over those edges of
+ * (fh, v).  FP64 approx_quad = true handles. */
+int wdg_hip_edgefluxf2d_euler(wdg_wave2d op, double gamma, int face_type, const void *d_q, void *d_divF,
+                              void *stream);
 /* Advection<true> and AdvectionHomogeneousBC<true> (wavedg/Advection.hpp:
  * 15-273) on the mesh and basis of an FP64 approx_quad = true wave handle
  * (which must outlive it): n_var <= 8 variables in the layout

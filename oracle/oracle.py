@@ -73,6 +73,8 @@ def lib():
             "orc_waveholtz_postprocess": (None, [_vp, _dp, _dp]),
             "orc_load_vector_gauss": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]),
             "orc_edge_flux": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
+            "orc_divf_euler": (None, [_vp, ctypes.c_double, _dp, _dp]),
+            "orc_edgefluxf_euler": (None, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp]),
             "orc_advection_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
             "orc_advection_free": (None, [_vp]),
             "orc_advection_action": (None, [_vp, _dp, _dp]),
@@ -240,6 +242,20 @@ class Wave:
         du = np.zeros(self.ndof) if du is None else du
         lib().orc_wavebc_action(self.h, _d(np.ascontiguousarray(u)), _d(du))
         return du
+
+    def divf_euler(self, gamma: float, q, divF=None):
+        """DivF2D<true> with the compressible Euler flux; q is (4, n, n, nel)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        divF = np.zeros(q.size) if divF is None else divF
+        lib().orc_divf_euler(self.h, gamma, _d(q), _d(divF))
+        return divF
+
+    def edgefluxf_euler(self, gamma: float, face_type: int, q, divF=None):
+        """divF += negated, lifted LLF edge fluxes (cl2d.cpp's time derivative)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        divF = np.zeros(q.size) if divF is None else divF
+        lib().orc_edgefluxf_euler(self.h, gamma, face_type, _d(q), _d(divF))
+        return divF
 
     def mass_action(self, x, nvar: int = 3):
         y = np.zeros_like(x)

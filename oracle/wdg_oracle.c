@@ -1562,3 +1562,131 @@ void orc_advection_action(const orc_adv *V, const double *u, double *divF)
 }
 
 void orc_advection_bc_action(const orc_adv *V, const double *u, double *divF) { adv_faces(V, 1, u, divF); }
+
+/* ===================================================================== */
+/* DivF2D<true> / EdgeFluxF2D<true> with the compressible Euler flux and    */
+/* the local Lax-Friedrichs numerical flux of examples/cl2d.cpp:57-111     */
+/* ===================================================================== */
+
+/* cl2d.cpp:57-81 (F): q = (rho, rho u, rho v, E), f and g the x and y fluxes */
+static void euler_flux(double gamma, const double *q, double *f, double *g)
+{
+    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
+    const double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+    f[0] = rho * u;
+    f[1] = rho * u * u + p;
+    f[2] = rho * u * v;
+    f[3] = u * (E + p);
+    g[0] = rho * v;
+    g[1] = rho * u * v;
+    g[2] = rho * v * v + p;
+    g[3] = v * (E + p);
+}
+
+/* cl2d.cpp:84-96 (speed) */
+static double euler_speed(double gamma, const double *q)
+{
+    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
+    const double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+    return hypot(u, v) + sqrt(gamma * p / rho);
+}
+
+/* cl2d.cpp:98-111 (numerical_flux) */
+static void euler_llf(double gamma, const double *n, const double *qL, const double *qR, double *fh)
+{
+    double fL[4], gL[4], fR[4], gR[4];
+    euler_flux(gamma, qL, fL, gL);
+    euler_flux(gamma, qR, fR, gR);
+    const double sL = euler_speed(gamma, qL), sR = euler_speed(gamma, qR);
+    const double c = sL > sR ? sL : sR;
+    for (int d = 0; d < 4; ++d)
+        fh[d] = 0.5 * n[0] * (fL[d] + fR[d]) + 0.5 * n[1] * (gL[d] + gR[d]) + 0.5 * c * (qL[d] - qR[d]);
+}
+
+/* DivF2D<true>::action (DivF.hpp:207-274) with the Euler flux: divF += (F(q), grad v) */
+void orc_divf_euler(const orc_wave *W, double gamma, const double *u, double *divF)
+{
+    const int n = W->n, nn = n * n, nel = W->m->nel;
+    const double *D = W->D, *w = W->w, *x = W->x;
+#pragma omp parallel
+    {
+        double *F = (double *)malloc(sizeof(double) * 2 * (size_t)nn * 4);
+#pragma omp for schedule(static)
+        for (int el = 0; el < nel; ++el) {
+            const double *c = W->m->corners + 8 * (size_t)el;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double J[4];
+                    quad_jacobian(c, x[i], x[j], J);
+                    const double w_ij = w[i] * w[j];
+                    const double Y_eta = J[3] * w_ij, X_eta = J[2] * w_ij;
+                    const double Y_xi = J[1] * w_ij, X_xi = J[0] * w_ij;
+                    double f[4], g[4];
+                    euler_flux(gamma, u + 4 * (i + n * (j + (size_t)n * el)), f, g);
+                    for (int d = 0; d < 4; ++d) {
+                        F[0 + 2 * (i + n * (d + 4 * j))] = Y_eta * f[d] - X_eta * g[d];
+                        F[1 + 2 * (j + n * (d + 4 * i))] = -Y_xi * f[d] + X_xi * g[d];
+                    }
+                }
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i)
+                    for (int d = 0; d < 4; ++d) {
+                        double dF = 0.0;
+                        for (int l = 0; l < n; ++l)
+                            dF += D[l + n * i] * F[0 + 2 * (l + n * (d + 4 * j))] +
+                                  D[l + n * j] * F[1 + 2 * (l + n * (d + 4 * i))];
+                        divF[d + 4 * (i + n * (j + (size_t)n * el))] += dF;
+                    }
+        }
+        free(F);
+    }
+}
+
+/* The face part of cl2d.cpp's time derivative (cl2d.cpp:180-215):
+ * prolongation (FaceProlongator.cpp:565-590), EdgeFluxF2D<true>::action with
+ * the LLF flux (EdgeFluxF.hpp:136-171: fB = +-fh w ds), the sign flip, and
+ * the transpose (FaceProlongator.cpp:650-672).  type 0: interior edges;
+ * type 1: boundary edges with the exterior state a copy of the interior one. */
+void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double *u, double *divF)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    double *uf = (double *)calloc((size_t)8 * n * ne + 1, sizeof(double)); /* (n, 4, 2, ne) */
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < 4; ++d) uf[i + n * (d + 4 * (side + 2 * (size_t)e))] = u[d + 4 * (size_t)j];
+            }
+    if (type == 1)
+        for (int e = 0; e < ne; ++e)
+            for (int d = 0; d < 4; ++d)
+                for (int i = 0; i < n; ++i)
+                    uf[i + n * (d + 4 * (1 + 2 * (size_t)e))] = uf[i + n * (d + 4 * (0 + 2 * (size_t)e))];
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(W->m, e, type);
+        for (int i = 0; i < n; ++i) {
+            double qL[4], qR[4], fh[4];
+            for (int d = 0; d < 4; ++d) {
+                qL[d] = uf[i + n * (d + 4 * (0 + 2 * (size_t)e))];
+                qR[d] = uf[i + n * (d + 4 * (1 + 2 * (size_t)e))];
+            }
+            euler_llf(gamma, E->n, qL, qR, fh);
+            for (int d = 0; d < 4; ++d) {
+                const double fl = fh[d] * W->w[i] * E->meas;
+                uf[i + n * (d + 4 * (0 + 2 * (size_t)e))] = -fl;
+                uf[i + n * (d + 4 * (1 + 2 * (size_t)e))] = fl;
+            }
+        }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 4; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    divF[d + 4 * (size_t)j] += uf[i + n * (d + 4 * (side + 2 * (size_t)e))];
+                }
+    free(uf);
+}
+

@@ -104,6 +104,15 @@ void orc_advection_free(orc_adv *V);
 void orc_advection_action(const orc_adv *V, const double *u, double *divF);
 void orc_advection_bc_action(const orc_adv *V, const double *u, double *divF);
 
+/* ---- compressible Euler flux of examples/cl2d.cpp ----------------------
+ * q = (rho, rho u, rho v, E) per node, arrays (4, n, n, nel).
+ * orc_divf_euler: DivF2D<true>::action with F, divF += (F(q), grad v).
+ * orc_edgefluxf_euler: divF += the negated, lifted EdgeFluxF2D<true> flux with
+ * the local Lax-Friedrichs numerical flux on interior (type 0) or boundary
+ * (type 1, exterior = interior copy) edges, as cl2d.cpp's time derivative. */
+void orc_divf_euler(const orc_wave *W, double gamma, const double *u, double *divF);
+void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double *u, double *divF);
+
 /* ---- reductions ------------------------------------------------------ */
 double orc_norm(long long n, const double *x);
 double orc_error(long long n, const double *x, const double *y);

@@ -88,6 +88,8 @@ _SIGS = {
     "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
     "wdg_hip_mass_apply": [_vp, _i, _vp, _vp, _i, _vp],
+    "wdg_hip_divf2d_euler": [_vp, _d, _vp, _vp, _vp],
+    "wdg_hip_edgefluxf2d_euler": [_vp, _d, _i, _vp, _vp, _vp],
     "wdg_hip_advection_create": [_vp, _i, _dp, _i, ctypes.POINTER(_vp)],
     "wdg_hip_advection_action": [_vp, _vp, _vp, _vp],
     "wdg_hip_advection_bc_action": [_vp, _vp, _vp, _vp],
@@ -325,6 +327,16 @@ class Wave2D:
         ms, nk = _d(), _i()
         _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
         return ms.value, nk.value
+
+    def divf_euler(self, gamma: float, q, out, stream=None) -> None:
+        """out += DivF2D(q) for the compressible Euler flux; q is (4, n, n, n_elem)."""
+        _call("wdg_hip_divf2d_euler", self.h, gamma, dev_ptr(q), dev_ptr(out),
+              stream if stream is not None else cur_stream())
+
+    def edgefluxf_euler(self, gamma: float, face_type: int, q, out, stream=None) -> None:
+        """out -= lifted local Lax-Friedrichs edge fluxes (interior 0, boundary 1)."""
+        _call("wdg_hip_edgefluxf2d_euler", self.h, gamma, face_type, dev_ptr(q), dev_ptr(out),
+              stream if stream is not None else cur_stream())
 
     def mass_apply(self, n_var: int, x, y, inverse: bool, stream=None) -> None:
         """y <- M x or M^{-1} x for n_var variables per node (x may be y)."""

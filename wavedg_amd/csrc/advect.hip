@@ -61,7 +61,112 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
     for (int k = tid; k < nloc * vn; k += 64) yb[k] += s_out[k];
 }
 
+// Compressible Euler flux and local Lax-Friedrichs flux of examples/cl2d.cpp
+// (F :57-81, speed :84-96, numerical_flux :98-111).
+__device__ __forceinline__ void euler_fg(double gamma, const double* q, double* f, double* g, double& speed)
+{
+    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
+    const double p = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+    f[0] = rho * u;
+    f[1] = rho * u * u + p;
+    f[2] = rho * u * v;
+    f[3] = u * (E + p);
+    g[0] = rho * v;
+    g[1] = rho * u * v;
+    g[2] = rho * v * v + p;
+    g[3] = v * (E + p);
+    speed = hypot(u, v) + sqrt(gamma * p / rho);
+}
+
+// EdgeFluxF2D<true> with the LLF flux, negated and lifted as in cl2d.cpp's
+// time derivative (cl2d.cpp:180-215), element by element: at node k of side
+// s, with this element's outward normal n, y -= w_k ds LLF(n, q_own, q_other)
+// (the flux is odd in (n, qL, qR), so el0 and el1 of an edge agree with the
+// reference's +-fh); on boundary sides q_other = q_own.  E = 64 / N elements
+// per workgroup, one lane per (element, side node), sides in order.
+template <int N>
+__global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const double* __restrict__ q,
+                                                        double* __restrict__ y, const double* __restrict__ geom,
+                                                        const int32_t* __restrict__ nbr,
+                                                        const int32_t* __restrict__ info,
+                                                        const double* __restrict__ basis, int nel, int interior,
+                                                        int boundary)
+{
+    constexpr int NN = N * N, E = 64 / N;
+    __shared__ double s_out[E * 4 * NN];
+    const int tid = threadIdx.x, e0 = blockIdx.x * E;
+    const int nloc = nel - e0 < E ? nel - e0 : E;
+    for (int k = tid; k < E * 4 * NN; k += 64) s_out[k] = 0.0;
+    __syncthreads();
+    const int el = tid / N, k = tid - el * N, e = e0 + el;
+    const bool on = tid < E * N && el < nloc;
+    for (int s = 0; s < 4; ++s) {
+        if (on) {
+            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+            const bool is_int = nb >= 0;
+            if (is_int ? interior : boundary) {
+                const double* g = geom + 6 * (size_t)e;
+                const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
+                const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
+                const double dy = 2.0 * ((s & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
+                const double len = sqrt(dx * dx + dy * dy), sg = (s >= 2) ? -1.0 : 1.0;
+                const double n0 = sg * dy / len, n1 = -sg * dx / len, wds = basis[N + k] * (0.5 * len);
+                const int node = side_node<N>(s, k);
+                double qa[4], qb[4];
+                const double* pa = q + 4 * ((size_t)e * NN + node);
+#pragma unroll
+                for (int d = 0; d < 4; ++d) qa[d] = pa[d];
+                if (is_int) {
+                    const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                    const double* pb = q + 4 * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) qb[d] = pb[d];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) qb[d] = qa[d];
+                }
+                double fa[4], ga[4], fb[4], gb[4], sa, sb;
+                euler_fg(gamma, qa, fa, ga, sa);
+                euler_fg(gamma, qb, fb, gb, sb);
+                const double c = sa > sb ? sa : sb;
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const double fh = 0.5 * n0 * (fa[d] + fb[d]) + 0.5 * n1 * (ga[d] + gb[d]) + 0.5 * c * (qa[d] - qb[d]);
+                    s_out[el * 4 * NN + 4 * node + d] -= fh * wds;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* yb = y + (size_t)e0 * 4 * NN;
+    for (int t = tid; t < nloc * 4 * NN; t += 64) yb[t] += s_out[t];
+}
+
 }  // namespace
+
+void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
+                       const int32_t* info, const double* basis, int nel, int interior, int boundary, hipStream_t s)
+{
+    if (nel <= 0) return;
+    const dim3 block(64);
+#define WDG_EULER_CASE(NV_)                                                                                   \
+    case NV_:                                                                                                 \
+        hipLaunchKernelGGL(euler_face_kernel<NV_>, dim3((nel + 64 / NV_ - 1) / (64 / NV_)), block, 0, s, gamma, q, \
+                           y, geom, nbr, info, basis, nel, interior, boundary);                               \
+        break;
+    switch (n) {
+        WDG_EULER_CASE(2)
+        WDG_EULER_CASE(3)
+        WDG_EULER_CASE(4)
+        WDG_EULER_CASE(5)
+        WDG_EULER_CASE(6)
+        WDG_EULER_CASE(7)
+        WDG_EULER_CASE(8)
+        WDG_EULER_CASE(9)
+    default: break;
+    }
+#undef WDG_EULER_CASE
+}
 
 void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
                         const double* H, int nel, int interior, int boundary, hipStream_t s)

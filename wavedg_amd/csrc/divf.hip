@@ -18,6 +18,23 @@ namespace {
 
 constexpr int kMaxVar = DivFArgs::kMaxVar;
 
+// The compressible Euler flux of examples/cl2d.cpp:57-81 (F): q = (rho, rho u,
+// rho v, E), f and g the x- and y-fluxes.
+template <typename R>
+__device__ __forceinline__ void euler_flux(R gamma, const R* q, R* f, R* g)
+{
+    const R rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3];
+    const R p = (gamma - (R)1) * (E - (R)0.5 * rho * (u * u + v * v));
+    f[0] = rho * u;
+    f[1] = rho * u * u + p;
+    f[2] = rho * u * v;
+    f[3] = u * (E + p);
+    g[0] = rho * v;
+    g[1] = rho * u * v;
+    g[2] = rho * v * v + p;
+    g[3] = v * (E + p);
+}
+
 template <int N, typename R>
 __global__ __launch_bounds__(Shape<N>::NT) void divf_linear_kernel(const DivFArgs a)
 {
@@ -45,9 +62,14 @@ __global__ __launch_bounds__(Shape<N>::NT) void divf_linear_kernel(const DivFArg
 #pragma unroll
         for (int d = 0; d < kMaxVar; ++d) uv[d] = d < nv ? uq[d] : (R)0;
         const double* An = a.Anode ? a.Anode + (size_t)2 * nv * nv * (node + (size_t)NN * e) : nullptr;
+        R ef[4], eg[4];
+        if (a.kind == 1) euler_flux((R)a.gamma, uv, ef, eg);
         for (int d = 0; d < nv; ++d) {
             R fx = 0, fy = 0;
-            if (An) {
+            if (a.kind == 1) {
+                fx = ef[d & 3];
+                fy = eg[d & 3];
+            } else if (An) {
                 for (int k = 0; k < nv; ++k) {
                     fx = fma((R)An[d + nv * k], uv[k], fx);
                     fy = fma((R)An[nv * nv + d + nv * k], uv[k], fy);

@@ -1378,6 +1378,45 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const doubl
     });
 }
 
+// ------------------------------------------------------------ Euler flux
+int wdg_hip_divf2d_euler(wdg_wave2d op, double gamma, const void* d_q, void* d_divF, void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (op->exact || op->prec != WDG_F64) fail(WDG_ERR_INVALID, "needs an FP64 approx_quad = true wave handle");
+        if (!d_q || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
+        if (!(gamma > 1.0)) fail(WDG_ERR_INVALID, "gamma must exceed 1");
+        wdg::DivFArgs a{};
+        a.geom = op->d_geom;
+        a.basis = op->d_basis;
+        a.u = d_q;
+        a.out = d_divF;
+        a.nel = op->nel;
+        a.nv = 4;
+        a.kind = 1;
+        a.gamma = gamma;
+        wdg::launch_divf_linear(op->n, 0, a, as_stream(stream));
+        hip_check(hipGetLastError(), "divf kernel launch");
+    });
+}
+
+int wdg_hip_edgefluxf2d_euler(wdg_wave2d op, double gamma, int face_type, const void* d_q, void* d_divF,
+                              void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (op->exact || op->prec != WDG_F64) fail(WDG_ERR_INVALID, "needs an FP64 approx_quad = true wave handle");
+        if (!d_q || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
+        if (!(gamma > 1.0)) fail(WDG_ERR_INVALID, "gamma must exceed 1");
+        if (face_type != WDG_INTERIOR && face_type != WDG_BOUNDARY) fail(WDG_ERR_INVALID, "bad face type");
+        wdg::launch_euler_face(op->n, gamma, static_cast<const double*>(d_q), static_cast<double*>(d_divF),
+                               op->d_geom, op->d_nbr, op->d_info, op->d_basis, op->nel,
+                               face_type == WDG_INTERIOR ? 1 : 0, face_type == WDG_BOUNDARY ? 1 : 0,
+                               as_stream(stream));
+        hip_check(hipGetLastError(), "edge flux kernel launch");
+    });
+}
+
 // ------------------------------------------------------------ advection
 struct wdg_advection_s {
     wdg_wave2d op = nullptr;

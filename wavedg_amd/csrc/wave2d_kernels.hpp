@@ -639,6 +639,10 @@ struct DivFArgs {
     // if set, the coefficient varies by node: (nv, nv, 2, n, n, nel) with
     // f_d = sum_c Anode(d, c, 0) u_c, g_d = sum_c Anode(d, c, 1) u_c (FP64)
     const double* Anode;
+    // kind 1: the compressible Euler flux of q = (rho, rho u, rho v, E) with
+    // ratio of specific heats gamma (nv = 4), instead of the linear flux
+    int kind;
+    double gamma;
 };
 void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 
@@ -647,6 +651,10 @@ void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 // selected sides, H per (element, side, node): (2, nv, nv) column-major.
 void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
                         const double* H, int nel, int interior, int boundary, hipStream_t s);
+// y -= w ds LLF(n, q_own, q_other) at every node of the selected sides for the
+// compressible Euler flux (advect.hip); boundary sides take q_other = q_own.
+void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
+                       const int32_t* info, const double* basis, int nel, int interior, int boundary, hipStream_t s);
 
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
