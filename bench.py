@@ -322,6 +322,49 @@ def advection_rate(torch, capi, mesh, n, reps: int):
             "frac": algo / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
 
 
+def euler_rate(torch, capi, mesh, n, reps: int):
+    """The flux part of examples/cl2d.cpp's time derivative (DivF2D +
+    interior and boundary EdgeFluxF2D with the local Lax-Friedrichs flux,
+    gamma = 1.4) on the benchmark mesh with the cl2d vortex mapped onto the
+    unit square: time of one evaluation."""
+    gamma = 1.4
+    wave = capi.Wave2D(mesh, n, None)
+    X = mesh.nodes(n)
+    x, y = 2.0 + 6.0 * X[..., 0], -3.0 + 6.0 * X[..., 1]
+    r = (x - 5.0) ** 2 + y**2
+    b = 2.5 * np.exp(1.0 - r) / np.pi
+    uu, vv = 1.0 - b * y, b * (x - 5.0)
+    rho = (1.0 - (gamma - 1.0) / (4.0 * gamma) * b * b) ** (1.0 / (gamma - 1.0))
+    E = rho**gamma / (gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv)
+    q = torch.tensor(np.stack([rho, rho * uu, rho * vv, E], axis=-1).reshape(-1), device="cuda")
+    out = torch.zeros_like(q)
+
+    def rhs():
+        wave.divf_euler(gamma, q, out)
+        wave.edgefluxf_euler(gamma, capi.INTERIOR, q, out)
+        wave.edgefluxf_euler(gamma, capi.BOUNDARY, q, out)
+
+    for _ in range(3):
+        rhs()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        rhs()
+    ev1.record()
+    torch.cuda.synchronize()
+    us = ev0.elapsed_time(ev1) * 1e3 / reps
+    npts, nel = q.numel() // 4, wave.n_elem
+    # volume pass: q read, out read + write (4 words per node each), 48 B of map
+    # coefficients; face pass: 32 B of connectivity, own and neighbour trace
+    # (4 words each) per side node, the element's out read + write
+    algo = npts * 32 * 3 + nel * 80 + nel * 4 * n * 64 + npts * 32 * 2
+    return {"workload": f"compressible Euler (cl2d flux, gamma 1.4), 4 variables, same "
+                        f"{int(round(nel ** 0.5))}^2 mesh, P={n - 1}", "avg_rhs_us": us,
+            "dof_per_s": 4 * npts / (us * 1e-6), "bytes_per_rhs": algo, "achieved_GBs": algo / (us * 1e-6) / 1e9,
+            "frac": algo / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int, approx_quad: bool = True):
     from oracle import oracle as O
 
@@ -352,6 +395,8 @@ def main() -> int:
     ap.add_argument("--exact-steps", type=int, default=400,
                     help="RK4 steps of the approx_quad = false (exact quadrature) path on the same mesh (0 = skip)")
     ap.add_argument("--exact-cpu-nx", type=int, default=128, help="mesh size of the exact path's CPU-oracle sample")
+    ap.add_argument("--euler-reps", type=int, default=100,
+                    help="timed compressible-Euler right-hand sides (DivF2D + EdgeFluxF2D, cl2d) on the same mesh")
     ap.add_argument("--advection-reps", type=int, default=200,
                     help="timed Advection + AdvectionHomogeneousBC actions on the same mesh (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU-oracle sample (0 = skip)")
@@ -528,6 +573,10 @@ def main() -> int:
                           f"{args.exact_cpu_nx}^2 quads, {args.cpu_steps} RK4 step(s), {sx:.1f} s",
             }
             print(f"[bench] exact cpu: {vx:.3e} DoF*stage/s", file=sys.stderr, flush=True)
+
+    if args.euler_reps > 0:
+        out["euler"] = euler_rate(torch, capi, mesh, n, args.euler_reps)
+        print(f"[bench] euler: {out['euler']['avg_rhs_us']:.1f} us per right-hand side", file=sys.stderr, flush=True)
 
     if args.advection_reps > 0:
         out["advection"] = advection_rate(torch, capi, mesh, n, args.advection_reps)
