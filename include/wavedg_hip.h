@@ -312,6 +312,17 @@ int wdg_hip_waveholtz_checkpoint(wdg_waveholtz wh, const char *path, const doubl
                                  double residual);
 int wdg_hip_waveholtz_resume(wdg_waveholtz wh, const char *path, double *d_W, int *iterations, double *residual);
 
+/* ---- time integrators (ode.hpp:12-186) --------------------------------
+ * ForwardEuler (method 0), RungeKutta4 (1), RungeKutta2 (2), SSPRK3 (3) for
+ * any right-hand side the caller evaluates on the device: a step is, for each
+ * stage k, f(p, t + c_k dt, stage input) then wdg_hip_ode_update(method, k,
+ * ...).  The stage input is u for stage 0 and for every RK4 stage, w (the
+ * second register) for the later RK2 and SSPRK3 stages.  ode_info gives the
+ * stage count and the c_k. */
+int wdg_hip_ode_info(int method, int *n_stages, double *c);
+int wdg_hip_ode_update(int method, int stage, long long n, double dt, double *d_u, double *d_w, const double *d_p,
+                       void *stream);
+
 /* ---- reductions (linalg.cpp:102-130), double, deterministic ---------- */
 int wdg_hip_norm(const double *d_x, long long n, double *out, void *stream);
 int wdg_hip_error(const double *d_x, const double *d_y, long long n, double *out, void *stream);

@@ -158,6 +158,25 @@ int main(int argc, char** argv)
             to_file(dir + "/adv_c.bin", c.size(), c.data());
             to_file(dir + "/adv_u.bin", np, u3.data());
             to_file(dir + "/adv_dudt.bin", np, du3.data());
+            // three SSPRK3 steps on the device, as advec2d's time loop
+            DeviceBuffer du(8 * np);
+            du.upload(u3.data(), 8 * np);
+            ode::SSPRK3 rk(np);
+            auto time_derivative = [&](double* dudt, double, const double* uu) {
+                check(wdg_hip_memset(dudt, 0, 8 * np));
+                adv.action_device(uu, dudt);
+                adv_bc.action_device(uu, dudt);
+                m3.inv_device(1, dudt);
+            };
+            double t = 0.0;
+            const double dt = 0.01;
+            for (int it = 0; it < 3; ++it) rk.step(dt, time_derivative, t, static_cast<double*>(du.get()));
+            du.download(du3.data(), 8 * np);
+            to_file(dir + "/adv_ssprk3.bin", np, du3.data());
+            if (std::abs(t - 0.03) > 1e-15) {
+                std::fprintf(stderr, "SSPRK3 time %.17g\n", t);
+                return 1;
+            }
         }
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {

@@ -80,3 +80,13 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     u3 = _read(d, "adv_u.bin")
     want = W3.mass_inv(A3.action(u3) + A3.bc_action(u3), nvar=1)
     assert rel_l2(_read(d, "adv_dudt.bin"), want) < 1e-12
+    # three SSPRK3 steps (ode.hpp:145-186) of the same operator on the device
+    def f(x):
+        return W3.mass_inv(A3.action(x) + A3.bc_action(x), nvar=1)
+
+    uu = u3.copy()
+    for _ in range(3):
+        y = uu + 0.01 * f(uu)
+        y = 0.25 * (y + 3.0 * uu + 0.01 * f(y))
+        uu = (uu + 2.0 * y + 2.0 * 0.01 * f(y)) / 3.0
+    assert rel_l2(_read(d, "adv_ssprk3.bin"), uu) < 1e-12

@@ -88,6 +88,8 @@ _SIGS = {
     "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
     "wdg_hip_mass_apply": [_vp, _i, _vp, _vp, _i, _vp],
+    "wdg_hip_ode_info": [_i, _ip, _dp],
+    "wdg_hip_ode_update": [_i, _i, _ll, _d, _vp, _vp, _vp, _vp],
     "wdg_hip_divf2d_euler": [_vp, _d, _vp, _vp, _vp],
     "wdg_hip_edgefluxf2d_euler": [_vp, _d, _i, _vp, _vp, _vp],
     "wdg_hip_advection_create": [_vp, _i, _dp, _i, ctypes.POINTER(_vp)],
@@ -184,6 +186,28 @@ def cur_stream() -> int | None:
     import torch
 
     return torch.cuda.current_stream().cuda_stream
+
+
+FORWARD_EULER, RUNGE_KUTTA4, RUNGE_KUTTA2, SSPRK3 = 0, 1, 2, 3
+
+
+def ode_step(method: int, f, dt: float, t: float, u, w=None, p=None, stream=None) -> float:
+    """One step of ode::ForwardEuler / RungeKutta4 / RungeKutta2 / SSPRK3
+    (ode.hpp) on device arrays: f(p, t, x) writes the right-hand side at x
+    into p.  Returns t + dt."""
+    import torch
+
+    ns = _i()
+    c = np.zeros(5)
+    _call("wdg_hip_ode_info", method, ctypes.byref(ns), _dptr(c))
+    p = torch.empty_like(u) if p is None else p
+    w = torch.empty_like(u) if (w is None and method != FORWARD_EULER) else w
+    st = stream if stream is not None else cur_stream()
+    for k in range(ns.value):
+        x = u if (k == 0 or method in (FORWARD_EULER, RUNGE_KUTTA4)) else w
+        f(p, t + c[k] * dt, x)
+        _call("wdg_hip_ode_update", method, k, u.numel(), dt, dev_ptr(u), dev_ptr(w), dev_ptr(p), st)
+    return t + dt
 
 
 def gll(n: int):

@@ -1294,6 +1294,78 @@ int wdg_hip_load_vector_gauss(wdg_wave2d op, int n_sources, const double* source
     });
 }
 
+// ------------------------------------------------------------ ode updates
+// The vector updates of ode::ForwardEuler, RungeKutta2, RungeKutta4 and
+// SSPRK3 (ode.hpp:12-186) between right-hand-side evaluations; w is the
+// second register (RK4 du, RK2 / SSPRK3 y).
+__global__ void k_ode_update(int method, int stage, long long n, double dt, double a, double b, double* u, double* w,
+                             const double* p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    switch (method) {
+    case 0:  // ForwardEuler
+        u[i] += dt * p[i];
+        break;
+    case 1: {  // RungeKutta4, stage `stage` with a = rk4a[stage], b = rk4b[stage]
+        const double d = (stage == 0 ? 0.0 : a * w[i]) + dt * p[i];
+        w[i] = d;
+        u[i] += b * d;
+        break;
+    }
+    case 2:  // RungeKutta2 (midpoint)
+        if (stage == 0)
+            w[i] = u[i] + (0.5 * dt) * p[i];
+        else
+            u[i] += dt * p[i];
+        break;
+    default:  // SSPRK3
+        if (stage == 0)
+            w[i] = u[i] + dt * p[i];
+        else if (stage == 1)
+            w[i] = 0.25 * (w[i] + 3.0 * u[i] + dt * p[i]);
+        else
+            u[i] = (u[i] + 2.0 * w[i] + 2.0 * dt * p[i]) / 3.0;
+        break;
+    }
+}
+
+int wdg_hip_ode_info(int method, int* n_stages, double* c)
+{
+    return guarded([&] {
+        if (method < 0 || method > 3) fail(WDG_ERR_INVALID, "method: 0 ForwardEuler, 1 RK4, 2 RK2, 3 SSPRK3");
+        static const int ns[4] = {1, 5, 2, 3};
+        if (n_stages) *n_stages = ns[method];
+        if (!c) return;
+        for (int k = 0; k < ns[method]; ++k) {
+            if (method == 1)
+                c[k] = RK4C[k];
+            else if (method == 2)
+                c[k] = 0.5 * k;
+            else if (method == 3)
+                c[k] = k == 0 ? 0.0 : (k == 1 ? 1.0 : 0.5);
+            else
+                c[k] = 0.0;
+        }
+    });
+}
+
+int wdg_hip_ode_update(int method, int stage, long long n, double dt, double* d_u, double* d_w, const double* d_p,
+                       void* stream)
+{
+    return guarded([&] {
+        static const int n_stages[4] = {1, 5, 2, 3};
+        if (method < 0 || method > 3) fail(WDG_ERR_INVALID, "method: 0 ForwardEuler, 1 RK4, 2 RK2, 3 SSPRK3");
+        if (stage < 0 || stage >= n_stages[method]) fail(WDG_ERR_INVALID, "stage out of range for the method");
+        if (n < 0 || !d_u || !d_p || (method != 0 && !d_w)) fail(WDG_ERR_INVALID, "bad argument");
+        if (n == 0) return;
+        const double a = method == 1 ? RK4A[stage] : 0.0, b = method == 1 ? RK4B[stage] : 0.0;
+        hipLaunchKernelGGL(k_ode_update, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), method, stage, n, dt, a,
+                           b, d_u, d_w, d_p);
+        hip_check(hipGetLastError(), "ode update launch");
+    });
+}
+
 // ---------------------------------------------------------------- reductions
 int wdg_hip_norm(const double* d_x, long long n, double* out, void* stream)
 {

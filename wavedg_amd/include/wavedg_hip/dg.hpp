@@ -341,6 +341,10 @@ public:
         op_.x.download(x, bytes);
     }
     void inv_device(void* d_x, void* stream = nullptr) const { check(wdg_hip_mass_inv(op_.get(), d_x, stream)); }
+    void inv_device(int n_var, void* d_x, void* stream = nullptr) const
+    {
+        check(wdg_hip_mass_apply(op_.get(), n_var, d_x, d_x, 1, stream));
+    }
 
 private:
     detail::Wave2D op_;
@@ -437,6 +441,57 @@ public:
 private:
     detail::AdvectionHandle h_;
 };
+
+// ode::ForwardEuler, RungeKutta2, RungeKutta4, SSPRK3 (ode.hpp:12-186) on
+// device arrays: the functor is called as f(d_F, t, d_u) with device
+// pointers and evaluates the right-hand side on the GPU (e.g. with the
+// *_device methods); the vector updates between stages run on the GPU.
+namespace ode {
+class Integrator {
+public:
+    template <typename Func>
+    void step(double dt, Func f, double& t, double* d_u, void* stream = nullptr) const
+    {
+        double* p = static_cast<double*>(p_.get());
+        double* w = static_cast<double*>(w_.get());
+        for (int k = 0; k < ns_; ++k) {
+            const double* x = (k == 0 || method_ <= 1) ? d_u : w;
+            f(p, t + c_[k] * dt, x);
+            check(wdg_hip_ode_update(method_, k, n_, dt, d_u, w, p, stream));
+        }
+        t += dt;
+    }
+
+protected:
+    Integrator(int method, long long n) : method_(method), n_(n), p_(8 * (size_t)n), w_(method ? 8 * (size_t)n : 0)
+    {
+        check(wdg_hip_ode_info(method, &ns_, c_));
+    }
+
+private:
+    int method_;
+    long long n_;
+    int ns_ = 0;
+    double c_[5] = {};
+    DeviceBuffer p_, w_;
+};
+class ForwardEuler : public Integrator {
+public:
+    explicit ForwardEuler(long long n) : Integrator(0, n) {}
+};
+class RungeKutta4 : public Integrator {
+public:
+    explicit RungeKutta4(long long n) : Integrator(1, n) {}
+};
+class RungeKutta2 : public Integrator {
+public:
+    explicit RungeKutta2(long long n) : Integrator(2, n) {}
+};
+class SSPRK3 : public Integrator {
+public:
+    explicit SSPRK3(long long n) : Integrator(3, n) {}
+};
+}  // namespace ode
 
 // The semi-discrete wave system u' = M^{-1}(A + B) u advanced by the
 // low-storage RK4 of ode::RungeKutta4 (ode.hpp:46-93), one fused kernel per
