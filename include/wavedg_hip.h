@@ -137,6 +137,15 @@ int wdg_hip_wave2d_action(wdg_wave2d op, const void *d_x, void *d_y, void *strea
  * volume term of WaveEquation::action. */
 int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double *A, const double *B, const void *d_u, void *d_divF,
                           void *stream);
+/* LinearFunctional2D::action(n_var, f, F) (wavedg/LinearFunctional.hpp:
+ * 136-202) for a function the caller has evaluated at the quadrature points:
+ * d_fq (n_var, nq, nq, n_elem) holds f at the nq-point Gauss-Legendre rule
+ * mapped into every element (wdg_hip_mesh_gl_points; nq = 0 means n_colloc,
+ * the reference's default); d_F (n_var, n, n, n_elem) <- (f, phi_ij). */
+int wdg_hip_linear_functional(wdg_wave2d op, int n_var, int nq, const double *d_fq, double *d_F, void *stream);
+/* element_metrics(quad).physical_coordinates() (Mesh2D.cpp:687-697) for the
+ * nq-point Gauss-Legendre rule: out is (2, nq, nq, n_elem), host memory. */
+int wdg_hip_mesh_gl_points(wdg_mesh mesh, int nq, double *out);
 /* DivF2D<true>::action (wavedg/DivF.hpp:207-274) with the compressible
  * Euler flux of examples/cl2d.cpp:57-81 for q = (rho, rho u, rho v, E) per
  * node, (4, n, n, n_elem), ratio of specific heats gamma:

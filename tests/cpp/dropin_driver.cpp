@@ -149,6 +149,14 @@ int main(int argc, char** argv)
                 c[2 * q + 1] = -r * x;
                 u3[q] = r * std::exp(-10.0 * r * r) * std::cos(3.0 * std::atan2(y, x));
             }
+            // the initial condition projected as advec2d does: LF.action then m.inv
+            LinearFunctional2D LF3(mesh3, basis3);
+            std::vector<double> u_lf(np);
+            LF3.action(1, [](const double x[2], double* F) {
+                const double r = std::hypot(x[0], x[1]);
+                *F = r * std::exp(-10.0 * r * r) * std::cos(3.0 * std::atan2(x[1], x[0]));
+            }, u_lf.data());
+            to_file(dir + "/adv_lf.bin", np, u_lf.data());
             Advection<true> adv(1, mesh3, basis3, c.data(), false);
             AdvectionHomogeneousBC<true> adv_bc(1, mesh3, basis3, c.data(), false);
             MassMatrix<true> m3(mesh3, basis3);

@@ -80,6 +80,26 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     u3 = _read(d, "adv_u.bin")
     want = W3.mass_inv(A3.action(u3) + A3.bc_action(u3), nvar=1)
     assert rel_l2(_read(d, "adv_dudt.bin"), want) < 1e-12
+    # LinearFunctional2D::action with a functor: (f, phi) at 5 Gauss-Legendre
+    # points, against the oracle's quadrature of the same f (numpy at the points)
+    xq, wq = np.polynomial.legendre.leggauss(5)
+    xg, _ = O.gauss_lobatto(5)
+    B = np.array([[np.prod([(t - xg[m]) / (xg[i] - xg[m]) for m in range(5) if m != i]) for i in range(5)]
+                  for t in xq])  # B[k, i] = phi_i(x_k)
+    corners = np.asarray(v3)[np.asarray(e3)]  # (nel, 4, 2)
+    want = np.zeros((len(corners), 5, 5))
+    for el, c4 in enumerate(corners):
+        for l, eta in enumerate(xq):
+            for k, xi in enumerate(xq):
+                N4 = 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
+                xy = N4 @ c4
+                dN = 0.25 * np.array([[-(1 - eta), (1 - eta), (1 + eta), -(1 + eta)], [-(1 - xi), -(1 + xi), (1 + xi), (1 - xi)]])
+                J = dN @ c4
+                r = np.hypot(*xy)
+                f = r * np.exp(-10 * r * r) * np.cos(3 * np.arctan2(xy[1], xy[0]))
+                want[el] += f * np.linalg.det(J) * wq[k] * wq[l] * np.outer(B[l], B[k])
+    assert rel_l2(_read(d, "adv_lf.bin"), want.reshape(-1)) < 1e-12
+
     # three SSPRK3 steps (ode.hpp:145-186) of the same operator on the device
     def f(x):
         return W3.mass_inv(A3.action(x) + A3.bc_action(x), nvar=1)

@@ -88,6 +88,8 @@ _SIGS = {
     "wdg_hip_waveholtz_resume": [_vp, ctypes.c_char_p, _vp, _ip, _dp],
     "wdg_hip_divf2d_linear": [_vp, _i, _dp, _dp, _vp, _vp, _vp],
     "wdg_hip_mass_apply": [_vp, _i, _vp, _vp, _i, _vp],
+    "wdg_hip_linear_functional": [_vp, _i, _i, _vp, _vp, _vp],
+    "wdg_hip_mesh_gl_points": [_vp, _i, _dp],
     "wdg_hip_ode_info": [_i, _ip, _dp],
     "wdg_hip_ode_update": [_i, _i, _ll, _d, _vp, _vp, _vp, _vp],
     "wdg_hip_divf2d_euler": [_vp, _d, _vp, _vp, _vp],
@@ -278,6 +280,12 @@ class Mesh:
         _call("wdg_hip_mesh_nodes", self.h, n, _dptr(out))
         return out.reshape(self.n_elem, n, n, 2)
 
+    def gl_points(self, nq: int) -> np.ndarray:
+        """Physical coordinates of the nq-point Gauss-Legendre rule per element: [el, l, k, xy]."""
+        out = np.zeros(2 * nq * nq * self.n_elem)
+        _call("wdg_hip_mesh_gl_points", self.h, nq, _dptr(out))
+        return out.reshape(self.n_elem, nq, nq, 2)
+
 
 class Wave2D:
     """WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on the GPU."""
@@ -351,6 +359,11 @@ class Wave2D:
         ms, nk = _d(), _i()
         _call("wdg_hip_wave2d_last_timing", self.h, ctypes.byref(ms), ctypes.byref(nk))
         return ms.value, nk.value
+
+    def linear_functional(self, n_var: int, fq, F, nq: int = 0, stream=None) -> None:
+        """F <- (f, phi) from f at the Gauss-Legendre points (Mesh.gl_points(nq)), (n_var, nq, nq, n_elem)."""
+        _call("wdg_hip_linear_functional", self.h, n_var, nq, dev_ptr(fq), dev_ptr(F),
+              stream if stream is not None else cur_stream())
 
     def divf_euler(self, gamma: float, q, out, stream=None) -> None:
         """out += DivF2D(q) for the compressible Euler flux; q is (4, n, n, n_elem)."""

@@ -142,7 +142,69 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
     for (int t = tid; t < nloc * 4 * NN; t += 64) yb[t] += s_out[t];
 }
 
+template <int N>
+__global__ __launch_bounds__(64) void linear_functional_kernel(const double* __restrict__ corners,
+                                                               const double* __restrict__ tab, int nq, int nv,
+                                                               const double* __restrict__ fq, double* __restrict__ F)
+{
+    constexpr int kQ = 10;
+    __shared__ double s_g[kMaxVar * kQ * kQ], s_p[kMaxVar * kQ * N];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const double* xq = tab;
+    const double* wq = tab + nq;
+    const double* B = tab + 2 * nq;  // B(k, i) at k + nq i
+    const double* c = corners + 8 * (size_t)e;
+    // f w_k w_l det J at the points (the reference scales Fq by detJ W(k) W(l))
+    for (int t = tid; t < nv * nq * nq; t += 64) {
+        const int d = t % nv, k = (t / nv) % nq, l = t / (nv * nq);
+        const double xi = xq[k], eta = xq[l];
+        const double J0 = 0.25 * ((1.0 - eta) * (c[2] - c[0]) + (1.0 + eta) * (c[4] - c[6]));
+        const double J1 = 0.25 * ((1.0 - eta) * (c[3] - c[1]) + (1.0 + eta) * (c[5] - c[7]));
+        const double J2 = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
+        const double J3 = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
+        s_g[t] = fq[(size_t)nv * nq * nq * e + t] * ((J0 * J3 - J1 * J2) * wq[k] * wq[l]);
+    }
+    __syncthreads();
+    // project along y: P(d, k, j) = sum_l g(d, k, l) B(l, j)
+    for (int t = tid; t < nv * nq * N; t += 64) {
+        const int d = t % nv, k = (t / nv) % nq, j = t / (nv * nq);
+        double s = 0.0;
+        for (int l = 0; l < nq; ++l) s += s_g[d + nv * (k + nq * l)] * B[l + nq * j];
+        s_p[t] = s;
+    }
+    __syncthreads();
+    // project along x: F(d, i, j) = sum_k P(d, k, j) B(k, i)
+    for (int t = tid; t < nv * N * N; t += 64) {
+        const int d = t % nv, i = (t / nv) % N, j = t / (nv * N);
+        double s = 0.0;
+        for (int k = 0; k < nq; ++k) s += s_p[d + nv * (k + nq * j)] * B[k + nq * i];
+        F[(size_t)nv * N * N * e + t] = s;
+    }
+}
+
 }  // namespace
+
+void launch_linear_functional(int n, int nq, int nv, const double* corners, const double* tab, const double* fq,
+                              double* F, int nel, hipStream_t s)
+{
+    if (nel <= 0) return;
+#define WDG_LF_CASE(NV_)                                                                                      \
+    case NV_:                                                                                                 \
+        hipLaunchKernelGGL(linear_functional_kernel<NV_>, dim3(nel), dim3(64), 0, s, corners, tab, nq, nv, fq, F); \
+        break;
+    switch (n) {
+        WDG_LF_CASE(2)
+        WDG_LF_CASE(3)
+        WDG_LF_CASE(4)
+        WDG_LF_CASE(5)
+        WDG_LF_CASE(6)
+        WDG_LF_CASE(7)
+        WDG_LF_CASE(8)
+        WDG_LF_CASE(9)
+    default: break;
+    }
+#undef WDG_LF_CASE
+}
 
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
                        const int32_t* info, const double* basis, int nel, int interior, int boundary, hipStream_t s)

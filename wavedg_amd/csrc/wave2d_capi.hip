@@ -1450,6 +1450,41 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double* A, const doubl
     });
 }
 
+// ------------------------------------------------------------ linear functional
+int wdg_hip_mesh_gl_points(wdg_mesh mesh, int nq, double* out)
+{
+    return guarded([&] {
+        if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
+        std::vector<double> x, w;
+        if (!wdg::gauss_legendre_rule(nq, x, w)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
+        mesh->mesh.nodes(nq, x.data(), out);
+    });
+}
+
+int wdg_hip_linear_functional(wdg_wave2d op, int n_var, int nq, const double* d_fq, double* d_F, void* stream)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!d_fq || !d_F) fail(WDG_ERR_INVALID, "null device pointer");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (nq <= 0) nq = op->n;  // the reference's default rule: Gauss-Legendre of n_colloc points
+        std::vector<double> xq, wq;
+        if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
+        std::vector<double> tab(2 * (size_t)nq + (size_t)nq * op->n);
+        std::copy(xq.begin(), xq.end(), tab.begin());
+        std::copy(wq.begin(), wq.end(), tab.begin() + nq);
+        wdg::lagrange_interpolation_matrix(op->n, op->x.data(), nq, xq.data(), tab.data() + 2 * nq);
+        const hipStream_t s = as_stream(stream);
+        DevBuf d_tab(sizeof(double) * tab.size());
+        hip_check(hipMemcpyAsync(d_tab.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s),
+                  "upload quadrature table");
+        wdg::launch_linear_functional(op->n, nq, n_var, op->d_corners, static_cast<const double*>(d_tab.p), d_fq, d_F,
+                                      op->nel, s);
+        hip_check(hipGetLastError(), "linear functional launch");
+        hip_check(hipStreamSynchronize(s), "linear functional");  // the table is freed on return
+    });
+}
+
 // ------------------------------------------------------------ Euler flux
 int wdg_hip_divf2d_euler(wdg_wave2d op, double gamma, const void* d_q, void* d_divF, void* stream)
 {

@@ -626,6 +626,14 @@ __global__ __launch_bounds__(Shape<N>::NT) void load_vector_kernel(const double*
     }
 }
 
+// LinearFunctional2D::action (LinearFunctional.hpp:136-202) for values f at
+// the nq x nq Gauss-Legendre points of every element, fq (nv, nq, nq, nel):
+// F(d, i, j) = sum_{k,l} B(k, i) B(l, j) w_k w_l det J(k, l) fq(d, k, l), with
+// tab = xq[nq], wq[nq], B[nq * n] (B(k, i) = phi_i(xq_k) at k + nq i).
+// One element per 64-lane workgroup (advect.hip).
+void launch_linear_functional(int n, int nq, int nv, const double* corners, const double* tab, const double* fq,
+                              double* F, int nel, hipStream_t s);
+
 // DivF2D<true> with the linear flux f(x, u) = (A u, B u) (divf.hip): A, B
 // are (kMaxVar, kMaxVar) column-major with the leading (nv, nv) block used.
 struct DivFArgs {

@@ -613,9 +613,31 @@ private:
 // (4, n_sources): cx, cy, sigma, amp.
 class LinearFunctional2D {
 public:
-    LinearFunctional2D(const Mesh2D& mesh, const QuadratureRule* basis)
-        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+    // quad: a Gauss-Legendre rule for the integrals, or null for the
+    // reference's default (n_colloc points, LinearFunctional.hpp:42-53)
+    LinearFunctional2D(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64), mesh_(mesh), n_(basis->n),
+          nq_(detail::exact_rule_size(quad))
     {
+    }
+    // LinearFunctional2D::action(n_var, f, F) (LinearFunctional.hpp:136-202):
+    // f(const double x[2], double fx[n_var]) is evaluated on the host at every
+    // quadrature point; the projection F = (f, phi) runs on the GPU.
+    template <typename Func>
+    void action(int n_var, Func f, double* F) const
+    {
+        const int nq = nq_ > 0 ? nq_ : n_;
+        const int nel = mesh_.n_elem();
+        std::vector<double> x(2 * (size_t)nq * nq * nel), fq((size_t)n_var * nq * nq * nel);
+        check(wdg_hip_mesh_gl_points(mesh_.handle(), nq, x.data()));
+        for (size_t p = 0; p < (size_t)nq * nq * nel; ++p) f(&x[2 * p], &fq[(size_t)n_var * p]);
+        DeviceBuffer d_fq(8 * fq.size());
+        const size_t bytes = 8 * (size_t)n_var * n_ * n_ * nel;
+        op_.x.resize(bytes);
+        d_fq.upload(fq.data(), 8 * fq.size());
+        check(wdg_hip_linear_functional(op_.get(), n_var, nq, static_cast<const double*>(d_fq.get()),
+                                        static_cast<double*>(op_.x.get()), nullptr));
+        op_.x.download(F, bytes);
     }
     void gaussians(int n_sources, const double* sources, int var, double* F) const
     {
@@ -631,6 +653,8 @@ public:
 
 private:
     detail::Wave2D op_;
+    Mesh2D mesh_;
+    int n_, nq_;
 };
 
 // norm / error (linalg.cpp:102-130) of device arrays, in double.
