@@ -49,6 +49,9 @@ enum {
 };
 
 enum { WDG_F64 = 0, WDG_F32 = 1 };
+/* basis point families, in the order of QuadratureRule::QuadratureType
+ * (QuadratureRule.hpp:26-30) */
+enum { WDG_GAUSS_LEGENDRE = 0, WDG_GAUSS_LOBATTO = 1 };
 enum { WDG_INTERIOR = 0, WDG_BOUNDARY = 1 };  /* FaceType, Mesh2D.hpp */
 enum { WDG_BC_ABSORB = 0, WDG_BC_REFLECT = 1, WDG_BC_DIRICHLET = 2 };
 
@@ -119,6 +122,20 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int *bc, double a, 
  * factors are stored); a mass rule too small for the basis is refused. */
 int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int *bc, double a, double b, int nq_volume,
                                 int nq_edge, int nq_mass, wdg_wave2d *out);
+/* The general constructor, WaveEquation(mesh, basis, approx_quad, quad)
+ * (WaveEquation.hpp:28; WaveEquation.cpp:5-30): a basis of n_colloc
+ * Gauss-Lobatto (WDG_GAUSS_LOBATTO) or Gauss-Legendre (WDG_GAUSS_LEGENDRE)
+ * points and either quadrature mode.  A Gauss-Legendre basis takes side
+ * traces by interpolation along the element and lifts face terms back with
+ * the same weights, LegendreFaceProlongator (FaceProlongator.cpp:717-769,
+ * 809-835, 908-934).  approx_quad = 0 needs precision WDG_F64; nq_* as in
+ * wdg_hip_wave2d_create_exact (ignored for approx_quad = 1).  Batched
+ * WaveHoltz, DivF2D, Advection and EdgeFluxF2D take Gauss-Lobatto handles. */
+int wdg_hip_wave2d_create_basis(wdg_mesh mesh, int n_colloc, int basis_type, const int *bc, double a, double b,
+                                int precision, int approx_quad, int nq_volume, int nq_edge, int nq_mass,
+                                wdg_wave2d *out);
+/* the handle's basis: type, nodes x[n_colloc] and weights w[n_colloc] (any may be NULL) */
+int wdg_hip_wave2d_basis(wdg_wave2d op, int *basis_type, double *x, double *w);
 int wdg_hip_wave2d_quadrature(wdg_wave2d op, int *exact, int *nq_volume, int *nq_edge, int *nq_mass);
 int wdg_hip_wave2d_destroy(wdg_wave2d op);
 /* ndof = 3 n^2 n_elem */

@@ -51,6 +51,9 @@ def lib():
             "orc_mesh_nodes": (None, [_vp, ctypes.c_int, _dp, _dp]),
             "orc_mesh_detj": (None, [_vp, ctypes.c_int, _dp, _dp]),
             "orc_wave_create": (_vp, [_vp, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double]),
+            "orc_wave_create_basis": (_vp, [_vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double]),
+            "orc_wave_create_exact_basis": (_vp, [_vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "orc_wave_free": (None, [_vp]),
             "orc_wave_ndof": (ctypes.c_longlong, [_vp]),
             "orc_wave_action": (None, [_vp, _dp, _dp]),
@@ -197,18 +200,20 @@ class Wave:
     """Oracle WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on a mesh."""
 
     def __init__(self, mesh: Mesh, n: int, bc=None, a: float = -1.0, b: float = -0.5, approx_quad: bool = True,
-                 quad=(0, 0, 0)):
+                 quad=(0, 0, 0), legendre: bool = False):
         """approx_quad=False: exact-quadrature operators with Gauss-Legendre rules
-        quad = (volume, edge, mass) points, 0 = the reference default."""
+        quad = (volume, edge, mass) points, 0 = the reference default.
+        legendre=True: a basis of n Gauss-Legendre points instead of Gauss-Lobatto."""
         self.mesh = mesh
         self.n = n
         nB = mesh.n_edges(1)
         self._bc = np.ascontiguousarray(np.ones(nB, dtype=np.int32) if bc is None else np.asarray(bc, dtype=np.int32))
         assert self._bc.shape == (nB,)
+        self.legendre = bool(legendre)
         if approx_quad:
-            self.h = lib().orc_wave_create(mesh.h, n, _i(self._bc), a, b)
+            self.h = lib().orc_wave_create_basis(mesh.h, n, int(legendre), _i(self._bc), a, b)
         else:
-            self.h = lib().orc_wave_create_exact(mesh.h, n, _i(self._bc), a, b, *quad)
+            self.h = lib().orc_wave_create_exact_basis(mesh.h, n, int(legendre), _i(self._bc), a, b, *quad)
         if not self.h:
             raise ValueError("orc_wave_create failed")
         self.ndof = lib().orc_wave_ndof(self.h)

@@ -421,6 +421,8 @@ struct orc_wave {
     const orc_mesh *m;
     int n;
     double x[16], w[16], D[256];
+    int leg;       /* 1: Gauss-Legendre basis, 0: Gauss-Lobatto */
+    double P[16];  /* phi_k(-1), the Legendre face interpolation */
     int *bc;
     double a, b;
     int *v2eI, *v2eB; /* (n, 2, ne) */
@@ -542,16 +544,25 @@ static double *make_fs(const orc_mesh *m, int type, double a, double b)
     return Fs;
 }
 
-orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, double b)
+/* WaveEquation.cpp:5-30 with a basis of n Gauss-Lobatto (leg = 0) or
+ * Gauss-Legendre (leg = 1) points; the Legendre face interpolation weights
+ * P(k) = phi_k(-1) are LegendreFaceProlongator's (FaceProlongator.cpp:717-725). */
+orc_wave *orc_wave_create_basis(const orc_mesh *m, int n, int leg, const int *bc, double a, double b)
 {
+    if (n < 2 || n > 16) return NULL;
     orc_wave *W = (orc_wave *)calloc(1, sizeof(orc_wave));
     W->m = m;
     W->n = n;
     W->a = a;
     W->b = b;
-    if (orc_gauss_lobatto(n, W->x, W->w) != 0) {
+    W->leg = leg ? 1 : 0;
+    if ((leg ? orc_gauss_legendre(n, W->x, W->w) : orc_gauss_lobatto(n, W->x, W->w)) != 0) {
         free(W);
         return NULL;
+    }
+    {
+        const double xm1 = -1.0;
+        orc_lagrange_basis(W->P, n, W->x, 1, &xm1);
     }
     orc_lagrange_basis_deriv(W->D, n, W->x, n, W->x);
     W->bc = (int *)malloc(sizeof(int) * (m->nB + 1));
@@ -563,6 +574,11 @@ orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, dou
     W->uI = (double *)malloc(sizeof(double) * ((size_t)6 * n * m->nI + 1));
     W->uB = (double *)malloc(sizeof(double) * ((size_t)6 * n * m->nB + 1));
     return W;
+}
+
+orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, double b)
+{
+    return orc_wave_create_basis(m, n, 0, bc, a, b);
 }
 
 void orc_wave_free(orc_wave *W)
@@ -695,7 +711,36 @@ void orc_divf_linear(const orc_wave *W, int nvar, const double *A, const double 
     }
 }
 
-/* FaceProlongator.cpp:565-590 (lobatto_action_2d) */
+/* Gauss-Legendre bases (FaceProlongator.cpp:732-738, mapV2E of the Legendre
+ * prolongator): the node k rows into the element from the side node j of
+ * element side f is j + k * inward_step(f). */
+static int inward_step(int n, int f) { return f == 0 ? n : f == 1 ? -1 : f == 2 ? -n : 1; }
+
+/* Value of variable d (of nvar) on element side f at the side node j:
+ * u(d, j) for Gauss-Lobatto (lobatto_action_2d, FaceProlongator.cpp:565-590),
+ * sum_k u(d, j_k) P(k) for Gauss-Legendre (legendre_action_2d, :809-835). */
+static double face_value(const orc_wave *W, const double *u, int nvar, int j, int f, int d)
+{
+    if (!W->leg) return u[d + nvar * (size_t)j];
+    const int st = inward_step(W->n, f);
+    double v = 0.0;
+    for (int k = 0; k < W->n; ++k) v += u[d + nvar * (size_t)(j + k * st)] * W->P[k];
+    return v;
+}
+
+/* The transpose: u(d, j) += Y (lobatto_transpose_2d, :650-672) or
+ * u(d, j_k) += Y P(k) (legendre_transpose_2d, :908-934). */
+static void face_add(const orc_wave *W, double *u, int nvar, int j, int f, int d, double Y)
+{
+    if (!W->leg) {
+        u[d + nvar * (size_t)j] += Y;
+        return;
+    }
+    const int st = inward_step(W->n, f);
+    for (int k = 0; k < W->n; ++k) u[d + nvar * (size_t)(j + k * st)] += Y * W->P[k];
+}
+
+/* FaceProlongator.cpp:565-590 (lobatto_action_2d) / :809-835 (legendre_action_2d) */
 static void prolong(const orc_wave *W, int type, const double *u, double *uf)
 {
     const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
@@ -704,11 +749,13 @@ static void prolong(const orc_wave *W, int type, const double *u, double *uf)
     memset(uf, 0, sizeof(double) * (size_t)6 * n * ne);
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i) {
                 const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                for (int d = 0; d < 3; ++d) uf[i + n * (d + 3 * (side + 2 * (size_t)e))] = u[d + 3 * (size_t)j];
+                for (int d = 0; d < 3; ++d) uf[i + n * (d + 3 * (side + 2 * (size_t)e))] = face_value(W, u, 3, j, f, d);
             }
+        }
 }
 
 /* EdgeFlux.cpp:289-325 (flux_action_2d_c), in place. */
@@ -736,20 +783,23 @@ static void flux_apply(const orc_wave *W, int type, double *uf)
     }
 }
 
-/* FaceProlongator.cpp:650-672 (lobatto_transpose_2d); serial: corner nodes
- * receive contributions from two edges, in edge order. */
+/* FaceProlongator.cpp:650-672 (lobatto_transpose_2d) / :908-934
+ * (legendre_transpose_2d); serial: nodes receive contributions from several
+ * edges, in edge order. */
 static void lift(const orc_wave *W, int type, const double *uf, double *du)
 {
     const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
     const int *v2e = type == 0 ? W->v2eI : W->v2eB;
     const int n_sides = type == 0 ? 2 : 1;
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i)
                 for (int d = 0; d < 3; ++d) {
                     const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                    du[d + 3 * (size_t)j] += uf[i + n * (d + 3 * (side + 2 * (size_t)e))];
+                    face_add(W, du, 3, j, f, d, uf[i + n * (d + 3 * (side + 2 * (size_t)e))]);
                 }
+        }
 }
 
 /* ===================================================================== */
@@ -915,8 +965,8 @@ static void flux_apply_q(const orc_wave *W, int type, double *uf)
     }
 }
 
-orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double a, double b, int nqv, int nqe,
-                                int nqm)
+orc_wave *orc_wave_create_exact_basis(const orc_mesh *m, int n, int leg, const int *bc, double a, double b, int nqv,
+                                      int nqe, int nqm)
 {
     /* default rules of Div<false>, EdgeFlux<false>, MassMatrix<false> for
      * bilinear elements (Div.cpp:320-327, EdgeFlux.cpp:503-510, MassMatrix.cpp:92-97) */
@@ -924,7 +974,7 @@ orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double 
     if (nqe <= 0) nqe = 1 + (2 * (n - 1) + 2) / 2;
     if (nqm <= 0) nqm = 1 + (2 * (n - 1) + 2) / 2;
     if (nqv > 10 || nqe > 10 || nqm > 10) return NULL;
-    orc_wave *W = orc_wave_create(m, n, bc, a, b);
+    orc_wave *W = orc_wave_create_basis(m, n, leg, bc, a, b);
     if (!W) return NULL;
     struct orc_exact *X = (struct orc_exact *)calloc(1, sizeof(struct orc_exact));
     X->nqv = nqv;
@@ -971,6 +1021,12 @@ orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double 
         return NULL;
     }
     return W;
+}
+
+orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double a, double b, int nqv, int nqe,
+                                int nqm)
+{
+    return orc_wave_create_exact_basis(m, n, 0, bc, a, b, nqv, nqe, nqm);
 }
 
 /* WaveEquation.cpp:54-68 */
@@ -1419,9 +1475,15 @@ static double *adv_flux_table(const struct orc_adv *V, int type, double a, doubl
                 const int j = v2e[i + n * (s + 2 * (size_t)e)];
                 if (j < 0 && !V->cc) {
                     for (int d = 0; d < v2; ++d) nA[d] = 0.0;
-                } else {
+                } else if (V->cc || !W->leg) {
                     const double *A = coef_at(V, j < 0 ? 0 : j);
                     for (int d = 0; d < v2; ++d) nA[d] = E->n[0] * A[d] + E->n[1] * A[v2 + d];
+                } else {
+                    /* the coefficient's face values come from the same
+                     * prolongator (Advection.hpp:77-78) */
+                    for (int d = 0; d < v2; ++d)
+                        nA[d] = E->n[0] * face_value(W, V->A, 2 * v2, j, E->side[s], d) +
+                                E->n[1] * face_value(W, V->A, 2 * v2, j, E->side[s], v2 + d);
                 }
                 if (compute_flux_m(m, nA, a, b, s, Fs) != 0) *err = 1;
                 const double w = W->w[i] * E->meas;
@@ -1526,11 +1588,14 @@ static void adv_faces(const orc_adv *V, int type, const double *u, double *divF)
     double *uf = V->uf;
     memset(uf, 0, sizeof(double) * (size_t)2 * n * m * ne);
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i) {
                 const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                for (int d = 0; d < m; ++d) uf[i + n * (d + m * (side + 2 * (size_t)e))] = u[d + m * (size_t)j];
+                for (int d = 0; d < m; ++d)
+                    uf[i + n * (d + m * (side + 2 * (size_t)e))] = face_value(W, u, m, j, f, d);
             }
+        }
     double acc[16 * 8];
     for (int e = 0; e < ne; ++e) {
         for (int k = 0; k < n * m; ++k) acc[k] = 0.0;
@@ -1547,12 +1612,14 @@ static void adv_faces(const orc_adv *V, int type, const double *u, double *divF)
             }
     }
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i)
                 for (int d = 0; d < m; ++d) {
                     const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                    divF[d + m * (size_t)j] += uf[i + n * (d + m * (side + 2 * (size_t)e))];
+                    face_add(W, divF, m, j, f, d, uf[i + n * (d + m * (side + 2 * (size_t)e))]);
                 }
+        }
 }
 
 void orc_advection_action(const orc_adv *V, const double *u, double *divF)
@@ -1654,11 +1721,14 @@ void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double
     const int n_sides = type == 0 ? 2 : 1;
     double *uf = (double *)calloc((size_t)8 * n * ne + 1, sizeof(double)); /* (n, 4, 2, ne) */
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i) {
                 const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                for (int d = 0; d < 4; ++d) uf[i + n * (d + 4 * (side + 2 * (size_t)e))] = u[d + 4 * (size_t)j];
+                for (int d = 0; d < 4; ++d)
+                    uf[i + n * (d + 4 * (side + 2 * (size_t)e))] = face_value(W, u, 4, j, f, d);
             }
+        }
     if (type == 1)
         for (int e = 0; e < ne; ++e)
             for (int d = 0; d < 4; ++d)
@@ -1681,12 +1751,14 @@ void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double
         }
     }
     for (int e = 0; e < ne; ++e)
-        for (int side = 0; side < n_sides; ++side)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
             for (int i = 0; i < n; ++i)
                 for (int d = 0; d < 4; ++d) {
                     const int j = v2e[i + n * (side + 2 * (size_t)e)];
-                    divF[d + 4 * (size_t)j] += uf[i + n * (d + 4 * (side + 2 * (size_t)e))];
+                    face_add(W, divF, 4, j, f, d, uf[i + n * (d + 4 * (side + 2 * (size_t)e))]);
                 }
+        }
     free(uf);
 }
 

@@ -50,12 +50,16 @@ void orc_edge_flux(double n0, double n1, double a, double b, int side, double *F
 typedef struct orc_wave orc_wave;
 /* a, b follow the reference EdgeFlux sign convention (upwind: a=-1, b=-0.5). */
 orc_wave *orc_wave_create(const orc_mesh *m, int n, const int *bc, double a, double b);
+/* the same on a basis of n Gauss-Lobatto (leg = 0) or Gauss-Legendre (leg = 1) points */
+orc_wave *orc_wave_create_basis(const orc_mesh *m, int n, int leg, const int *bc, double a, double b);
 /* approx_quad = false (Div<false>, EdgeFlux<false>, MassMatrix<false>) with
  * Gauss-Legendre rules of nqv (volume), nqe (edges) and nqm (mass matrix)
  * points; 0 selects the reference's default for bilinear elements (n, n+1,
  * n+1).  Every other function then uses the exact-quadrature operators. */
 orc_wave *orc_wave_create_exact(const orc_mesh *m, int n, const int *bc, double a, double b, int nqv, int nqe,
                                 int nqm);
+orc_wave *orc_wave_create_exact_basis(const orc_mesh *m, int n, int leg, const int *bc, double a, double b, int nqv,
+                                      int nqe, int nqm);
 void orc_wave_free(orc_wave *W);
 long long orc_wave_ndof(const orc_wave *W);
 void orc_wave_action(const orc_wave *W, const double *u, double *du);   /* du += A u   */

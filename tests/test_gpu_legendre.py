@@ -1,0 +1,187 @@
+"""Gauss-Legendre bases (LegendreFaceProlongator, FaceProlongator.cpp:717-769,
+809-835, 908-934) on the GPU against the oracle on the same basis, FP64 to
+1e-12 relative L2: WaveEquation / WaveBC actions, the RK right-hand side with
+forcing, low-storage RK4 steps, WaveHoltz pi0 / S / postprocess, the mass
+matrix, the Gaussian load vector and the FP32 state.  Each case prints its
+mesh, order and boundary codes before touching the GPU."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import MESHES, bc_codes, mesh_case, random_state, rel_l2
+from wavedg_amd import capi, meshgen
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _setup(name, n, bckind, precision=capi.F64, a=1.0, b=0.5):
+    v, e = mesh_case(name)
+    om = O.Mesh(v, e)
+    gm = capi.Mesh(v, e)
+    bc = bc_codes(gm.n_boundary, bckind)
+    print(f"legendre case mesh={name} n={n} bc={bckind} prec={precision} nel={gm.n_elem}", flush=True)
+    ow = O.Wave(om, n, bc, a=-a, b=-b, legendre=True)
+    gw = capi.Wave2D(gm, n, bc, a=a, b=b, precision=precision, basis=capi.GAUSS_LEGENDRE)
+    return om, ow, gm, gw
+
+
+def _dev(torch, x, dtype=None):
+    return torch.tensor(np.asarray(x), dtype=dtype or torch.float64, device="cuda")
+
+
+def test_handle_reports_gauss_legendre_basis(cuda):
+    _, _, _, gw = _setup("rect5x4", 5, "reflect")
+    x, w = O.gauss_legendre(5)
+    assert gw.basis == capi.GAUSS_LEGENDRE
+    np.testing.assert_array_equal(gw.basis_x, x)
+    np.testing.assert_allclose(gw.basis_w, w, rtol=1e-15)
+
+
+@pytest.mark.parametrize("name", MESHES)
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 9])
+def test_action_and_bc(cuda, name, n):
+    torch = cuda
+    _, ow, _, gw = _setup(name, n, "mixed")
+    u = random_state(gw.ndof, 11 + n)
+    y0 = random_state(gw.ndof, 99)
+    ref_a = ow.action(u, y0.copy())
+    ref_b = ow.bc_action(u, y0.copy())
+    du = _dev(torch, u)
+    ya, yb = _dev(torch, y0), _dev(torch, y0)
+    gw.action(du, ya)
+    gw.bc_action(du, yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy() - y0, ref_a - y0) < TOL64
+    assert rel_l2(yb.cpu().numpy() - y0, ref_b - y0) < TOL64
+
+
+@pytest.mark.parametrize("name,n", [("skewed6x5", 4), ("skew", 7), ("L", 6)])
+@pytest.mark.parametrize("bckind", ["reflect", "absorb", "dirichlet"])
+def test_rhs_and_mass(cuda, name, n, bckind):
+    torch = cuda
+    _, ow, _, gw = _setup(name, n, bckind, a=1.0, b=0.0 if bckind == "reflect" else 0.5)
+    u = random_state(gw.ndof, 7 * n)
+    F = random_state(gw.ndof, 8 * n)
+    ref = ow.rhs(u, F, -0.3)
+    du, dF = _dev(torch, u), _dev(torch, F)
+    out = torch.zeros_like(du)
+    gw.rhs(du, out, dF, -0.3)
+    x = du.clone()
+    gw.mass_inv(x)
+    mx = torch.zeros_like(du)
+    gw.mass_action(du, mx)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < TOL64
+    assert rel_l2(x.cpu().numpy(), ow.mass_inv(u)) < 1e-15
+    assert rel_l2(mx.cpu().numpy(), ow.mass_action(u)) < 1e-15
+
+
+@pytest.mark.parametrize("name,n,bckind", [("skewed6x5", 4, "absorb"), ("skewed16", 8, "mixed"), ("lshape8", 3, "dirichlet")])
+def test_rk4_steps(cuda, name, n, bckind):
+    torch = cuda
+    _, ow, gm, gw = _setup(name, n, bckind)
+    u = random_state(gw.ndof, 3 + n)
+    dt = 0.5 * gm.min_h() / n**2
+    ref, tref = ow.rk4(u, dt, 4, t=0.25)
+    du = _dev(torch, u)
+    t = gw.rk4(du, dt, 4, t=0.25)
+    torch.cuda.synchronize()
+    assert t == tref
+    assert rel_l2(du.cpu().numpy(), ref) < TOL64
+
+
+def test_rk4_forced_and_graph(cuda):
+    torch = cuda
+    _, ow, gm, gw = _setup("skewed6x5", 5, "absorb")
+    u = random_state(gw.ndof, 17)
+    F = random_state(gw.ndof, 18)
+    dt = 0.5 * gm.min_h() / 25
+    ref, _ = ow.rk4(u, dt, 3, t=0.1, F=F, omega=7.0)
+    du, dF = _dev(torch, u), _dev(torch, F)
+    gw.rk4(du, dt, 3, t=0.1, F=dF, omega=7.0)
+    torch.cuda.synchronize()
+    assert rel_l2(du.cpu().numpy(), ref) < TOL64
+    a, b = _dev(torch, u), _dev(torch, u)
+    gw.rk4(a, dt, 8)
+    gw.rk4_graph(b, dt, 8, 4)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_load_vector_is_collocated_quadrature(cuda):
+    """With a Gauss-Legendre basis and the n-point rule, B = I: the load vector
+    is w_k w_l det J f at the nodes."""
+    torch = cuda
+    om, _, gm, gw = _setup("skewed6x5", 5, "reflect")
+    src = [[0.1, -0.2, 0.15, 1.5], [0.4, 0.3, 0.05, -0.7]]
+    F = torch.empty(gw.ndof, dtype=torch.float64, device="cuda")
+    gw.load_vector_gauss(src, F)
+    torch.cuda.synchronize()
+    x, w = O.gauss_legendre(5)
+    X = om.nodes(x)  # [el, j, i, xy]
+    detj = om.detj(x)  # [el, j, i]
+    f = np.zeros(X.shape[:-1])
+    for cx, cy, sg, amp in src:
+        f += amp * np.exp(-((X[..., 0] - cx) ** 2 + (X[..., 1] - cy) ** 2) / (2 * sg * sg))
+    want = np.zeros(X.shape[:-1] + (3,))
+    want[..., 0] = f * detj * np.outer(w, w)[None]
+    assert rel_l2(F.cpu().numpy(), want.reshape(-1)) < 1e-13
+
+
+def test_waveholtz(cuda):
+    torch = cuda
+    v, e = meshgen.uniform_rect(4, 0.0, 1.0, 4, 0.0, 1.0)
+    om, gm = O.Mesh(v, e), capi.Mesh(v, e)
+    bc = bc_codes(gm.n_boundary, "mixed", seed=3)
+    n, omega = 4, 2.0 * np.pi
+    print(f"legendre waveholtz n={n}", flush=True)
+    ow = O.Wave(om, n, bc, legendre=True)
+    gw = capi.Wave2D(gm, n, bc, basis=capi.GAUSS_LEGENDRE)
+    owh, gwh = O.WaveHoltz(ow, omega), capi.WaveHoltz(gw, omega)
+    F = random_state(gw.ndof, 41)
+    ref_pi0 = owh.pi0(F)
+    pi0 = torch.zeros(gw.ndof, dtype=torch.float64, device="cuda")
+    gwh.pi0(pi0, _dev(torch, F))
+    torch.cuda.synchronize()
+    assert rel_l2(pi0.cpu().numpy(), ref_pi0) < TOL64
+    w0 = random_state(gw.ndof, 31)
+    dw = _dev(torch, w0)
+    gwh.S(dw)
+    torch.cuda.synchronize()
+    W = owh.S(w0)
+    assert rel_l2(dw.cpu().numpy(), W) < TOL64
+    H = torch.zeros(2 * gw.ndof // 3, dtype=torch.float64, device="cuda")
+    gwh.postprocess(H, dw)
+    torch.cuda.synchronize()
+    assert rel_l2(H.cpu().numpy(), owh.postprocess(W)) < TOL64
+
+
+def test_fp32_state_close_to_fp64(cuda):
+    torch = cuda
+    _, _, gm, gw = _setup("skewed16", 8, "absorb")
+    gw32 = capi.Wave2D(gm, 8, bc_codes(gm.n_boundary, "absorb"), precision=capi.F32, basis=capi.GAUSS_LEGENDRE)
+    u = random_state(gw.ndof, 5)
+    dt = 0.5 * gm.min_h() / 64
+    a, b = _dev(torch, u), _dev(torch, u, torch.float32)
+    gw.rk4(a, dt, 20)
+    gw32.rk4(b, dt, 20)
+    torch.cuda.synchronize()
+    err = rel_l2(b.double().cpu().numpy(), a.cpu().numpy())
+    print(f"legendre fp32 vs fp64 after 20 steps: {err:.3e}")
+    assert err < 1e-5
+
+
+def test_refusals(cuda):
+    v, e = mesh_case("skewed6x5")
+    gm = capi.Mesh(v, e)
+    gw = capi.Wave2D(gm, 4, None, basis=capi.GAUSS_LEGENDRE)
+    with pytest.raises(capi.WdgError):
+        gw.autotune(2)
+    with pytest.raises(capi.WdgError):
+        capi.Advection(gw, 1, np.array([1.0, 0.5]), True)
+    with pytest.raises(capi.WdgError):
+        capi.Wave2D(gm, 4, None, basis=7)

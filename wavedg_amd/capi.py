@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WDG_LIB", os.path.join(_HERE, "libwavedg_hip.so"))
 
 F64, F32 = 0, 1
+GAUSS_LEGENDRE, GAUSS_LOBATTO = 0, 1  # basis point families (QuadratureRule::QuadratureType order)
 INTERIOR, BOUNDARY = 0, 1
 BC_ABSORB, BC_REFLECT, BC_DIRICHLET = 0, 1, 2
 
@@ -50,6 +51,8 @@ _SIGS = {
     "wdg_hip_gauss_legendre": [_i, _dp, _dp],
     "wdg_hip_wave2d_create": [_vp, _i, _ip, _d, _d, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_create_exact": [_vp, _i, _ip, _d, _d, _i, _i, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_wave2d_create_basis": [_vp, _i, _i, _ip, _d, _d, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "wdg_hip_wave2d_basis": [_vp, _ip, _dp, _dp],
     "wdg_hip_wave2d_quadrature": [_vp, _ip, _ip, _ip, _ip],
     "wdg_hip_wave2d_destroy": [_vp],
     "wdg_hip_wave2d_info": [_vp, ctypes.POINTER(_ll), _ip, _ip, _ip],
@@ -291,9 +294,10 @@ class Wave2D:
     """WaveEquation + WaveBC + MassMatrix<true> + RungeKutta4 on the GPU."""
 
     def __init__(self, mesh: Mesh, n: int, bc=None, a: float = 1.0, b: float = 0.5, precision: int = F64,
-                 approx_quad: bool = True, quad=(0, 0, 0)):
+                 approx_quad: bool = True, quad=(0, 0, 0), basis: int | None = None):
         """approx_quad=False: exact-quadrature operators (FP64) with Gauss-Legendre
-        rules quad = (volume, edge, mass) points, 0 = the reference default."""
+        rules quad = (volume, edge, mass) points, 0 = the reference default.
+        basis: GAUSS_LOBATTO (default) or GAUSS_LEGENDRE basis points."""
         bcp = None
         if bc is not None:
             self._bc = np.ascontiguousarray(np.asarray(bc, dtype=np.int32))
@@ -301,7 +305,10 @@ class Wave2D:
                 raise ValueError("bc must have one code per boundary edge")
             bcp = _iptr(self._bc)
         h = _vp()
-        if approx_quad:
+        if basis is not None:
+            _call("wdg_hip_wave2d_create_basis", mesh.h, n, basis, bcp, a, b, precision, int(approx_quad), *quad,
+                  ctypes.byref(h))
+        elif approx_quad:
             _call("wdg_hip_wave2d_create", mesh.h, n, bcp, a, b, precision, ctypes.byref(h))
         else:
             if precision != F64:
@@ -312,6 +319,10 @@ class Wave2D:
         _call("wdg_hip_wave2d_info", self.h, ctypes.byref(nd), ctypes.byref(nc), ctypes.byref(ne), ctypes.byref(pr))
         self.ndof, self.n, self.n_elem, self.precision = nd.value, nc.value, ne.value, pr.value
         self.min_h = mesh.min_h()
+        bt = _i()
+        self.basis_x, self.basis_w = np.zeros(self.n), np.zeros(self.n)
+        _call("wdg_hip_wave2d_basis", self.h, ctypes.byref(bt), _dptr(self.basis_x), _dptr(self.basis_w))
+        self.basis = bt.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

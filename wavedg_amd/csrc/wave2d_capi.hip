@@ -119,7 +119,8 @@ struct wdg_wave2d_s {
     size_t rsz = 8;
     double ha = -0.5, hb = -0.5;
     double min_h = 0.0;
-    std::vector<double> x, w;
+    std::vector<double> x, w;  // basis nodes and weights
+    int leg = 0;               // 1: Gauss-Legendre basis, 0: Gauss-Lobatto
     double* d_corners = nullptr;
     double* d_geom = nullptr;
     float* d_geom32 = nullptr;   // FP32 handles: geometry and basis rounded to float
@@ -205,6 +206,7 @@ struct wdg_wave2d_s {
         a.basis = d_basis;
         a.geom32 = d_geom32;
         a.basis32 = d_basis32;
+        a.leg = leg;
         a.ha = ha;
         a.hb = hb;
         a.nel = nel;
@@ -762,17 +764,25 @@ int wdg_hip_gauss_legendre(int n, double* x, double* w)
 }
 
 // ---------------------------------------------------------------- operator
-int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int precision,
-                          wdg_wave2d* out)
+// approx_quad = true handle on a Gauss-Lobatto (basis_type WDG_GAUSS_LOBATTO)
+// or Gauss-Legendre (WDG_GAUSS_LEGENDRE) basis of n_colloc points.
+static int create_collocated(wdg_mesh mesh, int n_colloc, int basis_type, const int* bc, double a, double b,
+                             int precision, wdg_wave2d* out)
 {
     return guarded([&] {
         if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
         if (precision != WDG_F64 && precision != WDG_F32) fail(WDG_ERR_INVALID, "precision must be WDG_F64 or WDG_F32");
+        if (basis_type != WDG_GAUSS_LOBATTO && basis_type != WDG_GAUSS_LEGENDRE)
+            fail(WDG_ERR_INVALID, "basis_type must be WDG_GAUSS_LOBATTO or WDG_GAUSS_LEGENDRE");
         if (!std::isfinite(a) || !std::isfinite(b)) fail(WDG_ERR_INVALID, "flux coefficients must be finite");
         const wdg::HostMesh& m = mesh->mesh;
         auto op = std::make_unique<wdg_wave2d_s>();
         // validate everything on the host before touching the device
-        if (!wdg::gll_rule(n_colloc, op->x, op->w)) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+        op->leg = basis_type == WDG_GAUSS_LEGENDRE ? 1 : 0;
+        if (n_colloc < 2 || n_colloc > 9) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+        const bool rule_ok = op->leg ? wdg::gauss_legendre_rule(n_colloc, op->x, op->w)
+                                     : wdg::gll_rule(n_colloc, op->x, op->w);
+        if (!rule_ok) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
         if (m.min_corner_detj() <= 0.0) fail(WDG_ERR_MESH, "mesh has an inverted or degenerate element (det J <= 0)");
         wdg::FaceTables ft;
         std::string err;
@@ -790,10 +800,14 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
         op->sched = tuned_schedule(op->arch, n_colloc, precision);
 
         const int n = n_colloc;
-        std::vector<double> basis(2 * n + n * n);
+        // x[n], w[n], D[n*n], P[n] = phi_k(-1): the face interpolation of a
+        // Gauss-Legendre basis (LegendreFaceProlongator, FaceProlongator.cpp:717-725)
+        std::vector<double> basis(3 * n + n * n);
         std::memcpy(basis.data(), op->x.data(), sizeof(double) * n);
         std::memcpy(basis.data() + n, op->w.data(), sizeof(double) * n);
         wdg::lagrange_derivative_matrix(n, op->x.data(), n, op->x.data(), basis.data() + 2 * n);
+        const double xm1 = -1.0;
+        wdg::lagrange_interpolation_matrix(n, op->x.data(), 1, &xm1, basis.data() + 2 * n + n * n);
         std::vector<double> xq, wq;
         wdg::gauss_legendre_rule(n, xq, wq);
         std::vector<double> lv(2 * n + n * n);
@@ -846,8 +860,14 @@ int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, 
     });
 }
 
-int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int nq_volume,
-                                int nq_edge, int nq_mass, wdg_wave2d* out)
+int wdg_hip_wave2d_create(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int precision,
+                          wdg_wave2d* out)
+{
+    return create_collocated(mesh, n_colloc, WDG_GAUSS_LOBATTO, bc, a, b, precision, out);
+}
+
+static int create_exact(wdg_mesh mesh, int n_colloc, int basis_type, const int* bc, double a, double b,
+                        int nq_volume, int nq_edge, int nq_mass, wdg_wave2d* out)
 {
     if (!out) {
         g_last_error = "null argument";
@@ -864,7 +884,11 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, doub
         return WDG_ERR_INVALID;
     }
     wdg_wave2d op = nullptr;
-    const int rc = wdg_hip_wave2d_create(mesh, n_colloc, bc, a, b, WDG_F64, &op);
+    if (basis_type == WDG_GAUSS_LEGENDRE) {
+        g_last_error = "approx_quad = false takes a Gauss-Lobatto basis";
+        return WDG_ERR_INVALID;
+    }
+    const int rc = create_collocated(mesh, n_colloc, basis_type, bc, a, b, WDG_F64, &op);
     if (rc) return rc;
     const int rc2 = guarded([&] {
         const int n = n_colloc;
@@ -908,6 +932,40 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, doub
     }
     *out = op;
     return WDG_OK;
+}
+
+int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int* bc, double a, double b, int nq_volume,
+                                int nq_edge, int nq_mass, wdg_wave2d* out)
+{
+    return create_exact(mesh, n_colloc, WDG_GAUSS_LOBATTO, bc, a, b, nq_volume, nq_edge, nq_mass, out);
+}
+
+int wdg_hip_wave2d_create_basis(wdg_mesh mesh, int n_colloc, int basis_type, const int* bc, double a, double b,
+                                int precision, int approx_quad, int nq_volume, int nq_edge, int nq_mass,
+                                wdg_wave2d* out)
+{
+    if (approx_quad) return create_collocated(mesh, n_colloc, basis_type, bc, a, b, precision, out);
+    if (precision != WDG_F64) {
+        if (out) *out = nullptr;
+        g_last_error = "approx_quad = false handles are FP64";
+        return WDG_ERR_INVALID;
+    }
+    if (basis_type != WDG_GAUSS_LOBATTO && basis_type != WDG_GAUSS_LEGENDRE) {
+        if (out) *out = nullptr;
+        g_last_error = "basis_type must be WDG_GAUSS_LOBATTO or WDG_GAUSS_LEGENDRE";
+        return WDG_ERR_INVALID;
+    }
+    return create_exact(mesh, n_colloc, basis_type, bc, a, b, nq_volume, nq_edge, nq_mass, out);
+}
+
+int wdg_hip_wave2d_basis(wdg_wave2d op, int* basis_type, double* x, double* w)
+{
+    return guarded([&] {
+        check_op(op);
+        if (basis_type) *basis_type = op->leg ? WDG_GAUSS_LEGENDRE : WDG_GAUSS_LOBATTO;
+        if (x) std::memcpy(x, op->x.data(), sizeof(double) * op->n);
+        if (w) std::memcpy(w, op->w.data(), sizeof(double) * op->n);
+    });
 }
 
 int wdg_hip_wave2d_quadrature(wdg_wave2d op, int* exact, int* nq_volume, int* nq_edge, int* nq_mass)
@@ -1513,6 +1571,7 @@ int wdg_hip_edgefluxf2d_euler(wdg_wave2d op, double gamma, int face_type, const 
     return guarded([&] {
         check_op(op);
         if (op->exact || op->prec != WDG_F64) fail(WDG_ERR_INVALID, "needs an FP64 approx_quad = true wave handle");
+        if (op->leg) fail(WDG_ERR_INVALID, "EdgeFluxF2D takes Gauss-Lobatto handles");
         if (!d_q || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
         if (!(gamma > 1.0)) fail(WDG_ERR_INVALID, "gamma must exceed 1");
         if (face_type != WDG_INTERIOR && face_type != WDG_BOUNDARY) fail(WDG_ERR_INVALID, "bad face type");
@@ -1547,6 +1606,7 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
         if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
         if (op->exact || op->prec != WDG_F64)
             fail(WDG_ERR_INVALID, "advection needs an FP64 approx_quad = true wave handle");
+        if (op->leg) fail(WDG_ERR_INVALID, "advection takes Gauss-Lobatto handles");
         const int N = op->n, NN = N * N, nel = op->nel, m = n_var, v2 = m * m;
         auto adv = std::make_unique<wdg_advection_s>();
         adv->op = op;
@@ -1811,6 +1871,7 @@ int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int* schedule, double* ms)
     return guarded([&] {
         check_op(op);
         if (op->exact) fail(WDG_ERR_INVALID, "schedules apply to the approx_quad = true kernels only");
+        if (op->leg) fail(WDG_ERR_INVALID, "Gauss-Legendre handles run the default schedule");
         if (reps < 1) fail(WDG_ERR_INVALID, "reps must be positive");
         const hipStream_t s = nullptr;
         const long long n = op->ndof;

@@ -34,7 +34,7 @@ struct KArgs {
     const double* geom;     // (6, nel): ax, ay, cx, cy, hx, hy of the bilinear map
     const int32_t* nbr;     // (4, nel)
     const int32_t* info;    // (4, nel), see host_mesh.hpp
-    const double* basis;    // x[N], w[N], D[N*N] (column-major)
+    const double* basis;    // x[N], w[N], D[N*N] (column-major), P[N] = phi_k(-1)
     const void* force;      // load vector (rhs/stage with M_FORCE)
     double fs;              // forcing scale
     double* acc;            // WaveHoltz accumulator (M_ACC)
@@ -44,6 +44,7 @@ struct KArgs {
     int nel;
     const float* geom32;    // FP32 handles: geom and basis rounded to float once
     const float* basis32;
+    int leg;                // 1: Gauss-Legendre basis (faces interpolate with P)
 };
 
 enum : int {
@@ -57,6 +58,7 @@ enum : int {
     M_OUTMASK = 3 << 4,
     M_FORCE = 1 << 8,
     M_ACC = 1 << 9,
+    M_LEG = 1 << 10,  // Gauss-Legendre basis
 };
 
 // Entry points (dispatch ids) and their mode bits.
@@ -175,15 +177,27 @@ __device__ __forceinline__ int side_node(int s, int k)
     return i + N * j;
 }
 
+// Step from a node of side s one row into the element: node m of the line
+// through side position k is side_node(s, k) + m * inward_stride(s).  With a
+// Gauss-Legendre basis the trace at side position k is
+// sum_m P(m) u(side_node(s, k) + m * inward_stride(s)), P(m) = phi_m(-1)
+// (mapV2E and legendre_action_2d, FaceProlongator.cpp:717-769, 809-835).
+template <int N>
+__device__ __forceinline__ int inward_stride(int s)
+{
+    return s == 0 ? N : s == 1 ? -1 : s == 2 ? -N : 1;
+}
+
 // LDS carve-up of apply_operator for one workgroup of Shape<N, TGT> elements.
-template <int N, typename R, bool VOL, bool FACE, int TGT = 256>
+template <int N, typename R, bool VOL, bool FACE, int TGT = 256, bool LEG = false>
 struct OpLds {
     static constexpr int NN = N * N, EPB = Shape<N, TGT>::EPB, NF = 4 * N;
+    static constexpr int NXW = LEG ? 3 * N : 2 * N;  // x[N], w[N] (, P[N])
     static constexpr size_t n_u = (size_t)EPB * 3 * NN;
     static constexpr size_t n_F = VOL ? n_u : 0, n_G = VOL ? n_u : 0;
     static constexpr size_t n_S = FACE ? (size_t)EPB * NF * 3 : 0;
-    static constexpr size_t off_xw = 0;                                  // double x[N], w[N]
-    static constexpr size_t off_u = ((2 * N * 8 + 15) / 16) * 16;
+    static constexpr size_t off_xw = 0;                                  // double x[N], w[N] (, P[N])
+    static constexpr size_t off_u = ((NXW * 8 + 15) / 16) * 16;
     static constexpr size_t off_F = off_u + n_u * sizeof(R);
     static constexpr size_t off_G = off_F + n_F * sizeof(R);
     static constexpr size_t off_S = off_G + n_G * sizeof(R);
@@ -207,9 +221,11 @@ struct NodeResult {
 // before the first barrier, so the caller's extra loads overlap them.
 // GT and BT, the types of the geometry and basis tables, are double, or
 // float for FP32 handles that keep float copies (the rounding is the same as
-// converting in the kernel, so results do not change).
-template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, typename GT = double, typename BT = double,
-          typename Prefetch>
+// converting in the kernel, so results do not change).  LEG: Gauss-Legendre
+// basis, whose side traces interpolate along the element (inward_stride) and
+// whose face contributions lift back with the same weights P.
+template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, bool LEG = false, typename GT = double,
+          typename BT = double, typename Prefetch>
 __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const GT* __restrict__ geom,
                                                         const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ info,
@@ -222,7 +238,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     constexpr int NF = 4 * N;               // face nodes per element
     constexpr int FP = (NF + NN - 1) / NN;  // face nodes per thread (1 for N >= 4)
     constexpr bool FACE = IFACE || BFACE;
-    using L = OpLds<N, R, VOL, FACE, TGT>;
+    using L = OpLds<N, R, VOL, FACE, TGT, LEG>;
     BT* s_xw = reinterpret_cast<BT*>(lds + L::off_xw);
     R* s_u = reinterpret_cast<R*>(lds + L::off_u);
     R* s_F = reinterpret_cast<R*>(lds + L::off_F);
@@ -231,6 +247,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     R* s_D = reinterpret_cast<R*>(lds + L::off_D);
     const BT* s_x = s_xw;
     const BT* s_w = s_xw + N;
+    const BT* s_P = s_xw + 2 * N;  // LEG only
 
     const int tid = threadIdx.x;
     const bool worker = tid < NT;
@@ -246,7 +263,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
         for (int r = 0; r < 3; ++r) ld[r] = ub[min(tid + r * NT, cnt - 1)];
     }
     const R dval = (R)basis[2 * N + min(tid, NN - 1)];
-    const BT xwv = basis[min(tid, 2 * N - 1)];
+    const BT xwv = (LEG && tid >= 2 * N) ? basis[NN + min(tid, 3 * N - 1)] : basis[min(tid, 2 * N - 1)];
     const int el = worker ? tid / NN : 0;
     const int node = worker ? tid - el * NN : 0;
     const int i = node % N, j = node / N;
@@ -296,9 +313,20 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                 const int inf = finfo[p];
                 const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
                 const R* q = u_in + ((size_t)fnb[p] * NN + side_node<N>(inf & 3, kk)) * 3;
-                nbv[p][0] = q[0];
-                nbv[p][1] = q[1];
-                nbv[p][2] = q[2];
+                if constexpr (LEG) {
+                    const int st = 3 * inward_stride<N>(inf & 3);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) {
+                        const R Pm = (R)basis[2 * N + NN + m];
+                        nbv[p][0] = fma(Pm, q[m * st + 0], nbv[p][0]);
+                        nbv[p][1] = fma(Pm, q[m * st + 1], nbv[p][1]);
+                        nbv[p][2] = fma(Pm, q[m * st + 2], nbv[p][2]);
+                    }
+                } else {
+                    nbv[p][0] = q[0];
+                    nbv[p][1] = q[1];
+                    nbv[p][2] = q[2];
+                }
             }
         }
     }
@@ -308,7 +336,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
 #pragma unroll
         for (int r = 0; r < 3; ++r) s_u[tid + r * NT] = ld[r];
         if (tid < NN) s_D[tid] = dval;
-        if (tid < 2 * N) s_xw[tid] = xwv;
+        if (tid < L::NXW) s_xw[tid] = xwv;
     }
     __syncthreads();
 
@@ -370,9 +398,23 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                 const int ie = (!first && flip) ? (N - 1 - k) : k;  // node index in el0's order
                 const R wds = (R)((FG)s_w[ie] * ((FG)0.5 * (l2 * il)));
                 const int ln = side_node<N>(s, k);
-                const R pA = s_u[el * 3 * NN + 3 * ln + 0];
-                const R uA = s_u[el * 3 * NN + 3 * ln + 1];
-                const R vA = s_u[el * 3 * NN + 3 * ln + 2];
+                R pA, uA, vA;
+                if constexpr (LEG) {
+                    pA = uA = vA = (R)0;
+                    const int st = 3 * inward_stride<N>(s);
+                    const R* q = s_u + el * 3 * NN + 3 * ln;
+#pragma unroll
+                    for (int m = 0; m < N; ++m) {
+                        const R Pm = (R)s_P[m];
+                        pA = fma(Pm, q[m * st + 0], pA);
+                        uA = fma(Pm, q[m * st + 1], uA);
+                        vA = fma(Pm, q[m * st + 2], vA);
+                    }
+                } else {
+                    pA = s_u[el * 3 * NN + 3 * ln + 0];
+                    uA = s_u[el * 3 * NN + 3 * ln + 1];
+                    vA = s_u[el * 3 * NN + 3 * ln + 2];
+                }
                 R pB, uB, vB;
                 if (interior) {
                     pB = nbv[pp][0];
@@ -428,6 +470,19 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     if constexpr (FACE) {
         // add this node's face contributions in side order 0..3
         const R* S = s_S + el * NF * 3;
+        if constexpr (LEG) {
+            // node (i, j) is row j of side 0, row N-1-i of side 1, row N-1-j
+            // of side 2 and row i of side 3 (legendre_transpose_2d,
+            // FaceProlongator.cpp:908-934)
+            const R q0 = (R)s_P[j], q1 = (R)s_P[N - 1 - i], q2 = (R)s_P[N - 1 - j], q3 = (R)s_P[i];
+            const R* S0 = S + (0 * N + i) * 3;
+            const R* S1 = S + (1 * N + j) * 3;
+            const R* S2 = S + (2 * N + i) * 3;
+            const R* S3 = S + (3 * N + j) * 3;
+            r0 += q0 * S0[0] + q1 * S1[0] + q2 * S2[0] + q3 * S3[0];
+            r1 += q0 * S0[1] + q1 * S1[1] + q2 * S2[1] + q3 * S3[1];
+            r2 += q0 * S0[2] + q1 * S1[2] + q2 * S2[2] + q3 * S3[2];
+        } else {
         if (j == 0) {
             r0 += S[(0 * N + i) * 3 + 0];
             r1 += S[(0 * N + i) * 3 + 1];
@@ -448,6 +503,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
             r1 += S[(3 * N + j) * 3 + 1];
             r2 += S[(3 * N + j) * 3 + 2];
         }
+        }
     }
     out.r0 = r0;
     out.r1 = r1;
@@ -467,7 +523,8 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     constexpr bool FORCE = (MODE & M_FORCE) != 0;
     constexpr bool ACC = (MODE & M_ACC) != 0;
     constexpr bool READ_DU = OUT == M_OUT_STAGE;
-    using L = OpLds<N, R, VOL, IFACE || BFACE, TGT>;
+    constexpr bool LEG = (MODE & M_LEG) != 0;
+    using L = OpLds<N, R, VOL, IFACE || BFACE, TGT, LEG>;
     __shared__ __align__(16) unsigned char lds[L::bytes];
 
     const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
@@ -506,10 +563,10 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     bool active;
     NodeResult<R> q;
     if constexpr (std::is_same<R, float>::value && !WDG_FACE_GEOM_DOUBLE)
-        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom32, a.nbr, a.info,
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom32, a.nbr, a.info,
                                                          a.basis32, a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
     else
-        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info, a.basis,
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info, a.basis,
                                                          a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
     if (!active) return;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;

@@ -32,10 +32,41 @@ void launch_sched(int kid, const KArgs& a, hipStream_t s)
     }
 }
 
+// Gauss-Legendre bases run every entry point with the default schedule.
+template <typename R>
+void launch_legendre(int kid, const KArgs& a, hipStream_t s)
+{
+    constexpr int N = WDG_N;
+    const int nblk = (a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
+    const dim3 grid(nblk), block(Shape<N>::NT);
+    switch (kid) {
+#define WDG_CASE(K)                                                                      \
+    case K:                                                                              \
+        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K] | M_LEG>), grid, block, 0, s, a); \
+        break;
+        WDG_CASE(K_ACTION)
+        WDG_CASE(K_BC)
+        WDG_CASE(K_RHS)
+        WDG_CASE(K_RHS_F)
+        WDG_CASE(K_STAGE0)
+        WDG_CASE(K_STAGE0_F)
+        WDG_CASE(K_STAGE)
+        WDG_CASE(K_STAGE_F)
+        WDG_CASE(K_STAGE_ACC)
+        WDG_CASE(K_STAGE_F_ACC)
+#undef WDG_CASE
+    default: break;
+    }
+}
+
 template <typename R>
 void launch_typed(int kid, int sched, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
+    if (a.leg) {
+        launch_legendre<R>(kid, a, s);
+        return;
+    }
     switch (kid) {
     case K_STAGE0:
     case K_STAGE0_F:
