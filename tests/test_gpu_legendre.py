@@ -185,3 +185,68 @@ def test_refusals(cuda):
         capi.Advection(gw, 1, np.array([1.0, 0.5]), True)
     with pytest.raises(capi.WdgError):
         capi.Wave2D(gm, 4, None, basis=7)
+    with pytest.raises(capi.WdgError):  # exact quadrature runs in FP64
+        capi.Wave2D(gm, 4, None, precision=capi.F32, approx_quad=False, basis=capi.GAUSS_LEGENDRE)
+
+
+def _setup_exact(name, n, bckind, quad=(0, 0, 0)):
+    v, e = mesh_case(name)
+    om, gm = O.Mesh(v, e), capi.Mesh(v, e)
+    bc = bc_codes(gm.n_boundary, bckind)
+    print(f"legendre exact case mesh={name} n={n} bc={bckind} quad={quad} nel={gm.n_elem}", flush=True)
+    ow = O.Wave(om, n, bc, approx_quad=False, quad=quad, legendre=True)
+    gw = capi.Wave2D(gm, n, bc, approx_quad=False, quad=quad, basis=capi.GAUSS_LEGENDRE)
+    return ow, gm, gw
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("quad", [(0, 0, 0), None], ids=["fused", "separate"])
+def test_exact_quadrature_rhs_and_rk4(cuda, n, quad):
+    """The fused stage kernel (default rules) and the separate volume / face /
+    mass kernels (a larger rule set), both on a Gauss-Legendre basis."""
+    torch = cuda
+    q = (0, 0, 0) if quad is not None else (n + 1, n + 2, n + 2)
+    if n > 8 and quad is None:
+        q = (10, 9, 10)  # rules have at most 10 points
+    ow, gm, gw = _setup_exact("skewed6x5", n, "mixed", q)
+    u = random_state(gw.ndof, 50 + n)
+    F = random_state(gw.ndof, 60 + n)
+    y0 = random_state(gw.ndof, 70 + n)
+    ya, yb = _dev(torch, y0), _dev(torch, y0)
+    du = _dev(torch, u)
+    gw.action(du, ya)
+    gw.bc_action(du, yb)
+    out = torch.zeros_like(du)
+    gw.rhs(du, out, _dev(torch, F), 0.7)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy() - y0, ow.action(u, y0.copy()) - y0) < TOL64
+    assert rel_l2(yb.cpu().numpy() - y0, ow.bc_action(u, y0.copy()) - y0) < TOL64
+    assert rel_l2(out.cpu().numpy(), ow.rhs(u, F, 0.7)) < TOL64
+    dt = 0.25 * gm.min_h() / n**2
+    ref, _ = ow.rk4(u, dt, 3, t=0.0)
+    gw.rk4(du, dt, 3)
+    torch.cuda.synchronize()
+    assert rel_l2(du.cpu().numpy(), ref) < TOL64
+
+
+@pytest.mark.parametrize("name,n,bckind", [("skew", 5, "absorb"), ("L", 4, "reflect"), ("lshape8", 7, "dirichlet")])
+def test_exact_quadrature_meshes(cuda, name, n, bckind):
+    torch = cuda
+    ow, gm, gw = _setup_exact(name, n, bckind)
+    u = random_state(gw.ndof, 80 + n)
+    out = torch.zeros(gw.ndof, dtype=torch.float64, device="cuda")
+    gw.rhs(_dev(torch, u), out)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ow.rhs(u)) < TOL64
+
+
+def test_exact_quadrature_waveholtz(cuda):
+    torch = cuda
+    ow, gm, gw = _setup_exact("lshape8", 4, "absorb")
+    omega = 4.0 * np.pi
+    owh, gwh = O.WaveHoltz(ow, omega), capi.WaveHoltz(gw, omega)
+    w0 = random_state(gw.ndof, 8)
+    dw = _dev(torch, w0)
+    gwh.S(dw)
+    torch.cuda.synchronize()
+    assert rel_l2(dw.cpu().numpy(), owh.S(w0)) < TOL64

@@ -1,6 +1,6 @@
 // Exact-quadrature operators (approx_quad = false) in FP64: Div<false>
-// (Div.cpp:312-360, 469-588), EdgeFlux<false> on Lobatto face traces
-// (EdgeFlux.cpp:375-420, 491-620) and MassMatrix<false> (MassMatrix.cpp:
+// (Div.cpp:312-360, 469-588), EdgeFlux<false> on Lobatto or Legendre face
+// traces (EdgeFlux.cpp:375-420, 491-620) and MassMatrix<false> (MassMatrix.cpp:
 // 85-133, 193-227).  Tables are column-major (n_quad, n) with
 // P(q, k) = phi_k(x_q) and D(q, k) = phi_k'(x_q).  The separate volume, face
 // and mass kernels take one element per workgroup and serve any rule sizes;
@@ -9,12 +9,14 @@
 //  * volume: u -> quadrature points by the two tensor-product passes of
 //    div_2d_q, contravariant fluxes there, and back with D and P;
 //  * faces: for each side the element's own trace and the other trace
-//    (neighbour nodes, or the boundary state built at the Lobatto nodes) are
+//    (neighbour nodes, or the boundary state built at the side's nodes) are
 //    interpolated to the edge rule, the closed-form flux
 //      c_p = w ds [ha n.(uA+uB) + hb (pA-pB)],  c_u = n w ds [ha (pA+pB) + hb n.(uA-uB)]
 //    is evaluated there (own outward normal: +f on el0, -f on el1 of the
 //    reference's edge flux) and projected onto the side's nodes; corner
-//    nodes add their two sides in side order, so every output has one writer;
+//    nodes add their two sides in side order, so every output has one writer.
+//    A Gauss-Legendre basis (KArgs::leg) takes each side node's trace as
+//    sum_m P(m) u(m rows in) and lifts back along the same line with P;
 //  * mass matrix: det J of a bilinear element is affine in (xi, eta), so the
 //    exact element matrix is d0 A(x)A + d1 B(x)A + d2 A(x)B with 1D matrices
 //    A, B of the mass rule; it is applied and inverted (by fast
@@ -141,6 +143,30 @@ __global__ __launch_bounds__(kBlk) void exact_volume_kernel(const KArgs a, const
     }
 }
 
+// Trace (p, u, v) on element side s at the side node whose state starts at q:
+// the node's values for a Gauss-Lobatto basis, sum_m P(m) u(m rows in) for a
+// Gauss-Legendre basis (legendre_action_2d, FaceProlongator.cpp:809-835).
+template <int N>
+__device__ __forceinline__ void side_trace(const double* q, int s, int leg, const double* P, double& p, double& uu,
+                                           double& vv)
+{
+    if (!leg) {
+        p = q[0];
+        uu = q[1];
+        vv = q[2];
+        return;
+    }
+    const int st = 3 * inward_stride<N>(s);
+    p = uu = vv = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        const double w = P[m];
+        p = fma(w, q[m * st + 0], p);
+        uu = fma(w, q[m * st + 1], uu);
+        vv = fma(w, q[m * st + 2], vv);
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const ExactTables t, int interior, int boundary)
 {
@@ -163,15 +189,13 @@ __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const Exa
         __syncthreads();  // the previous side's traces and fluxes are consumed
         if (tid < N) {
             const int k = tid;
-            const double* qa = u + ((size_t)e * NN + side_node<N>(s, k)) * 3;
-            const double pA = qa[0], uA = qa[1], vA = qa[2];
+            double pA, uA, vA;
+            side_trace<N>(u + ((size_t)e * NN + side_node<N>(s, k)) * 3, s, a.leg, t.Pf, pA, uA, vA);
             double pB, uB, vB;
             if (is_int) {
                 const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                const double* qb = u + ((size_t)nb * NN + side_node<N>(inf & 3, kk)) * 3;
-                pB = qb[0];
-                uB = qb[1];
-                vB = qb[2];
+                side_trace<N>(u + ((size_t)nb * NN + side_node<N>(inf & 3, kk)) * 3, inf & 3, a.leg, t.Pf, pB, uB,
+                              vB);
             } else {
                 boundary_state((inf >> 8) & 0xff, n0, n1, pA, uA, vA, pB, uB, vB);
             }
@@ -203,11 +227,16 @@ __global__ __launch_bounds__(64) void exact_face_kernel(const KArgs a, const Exa
         }
         __syncthreads();
         if (tid < N) {
-            const int k = tid, node = side_node<N>(s, k);
+            // lane k owns the line of nodes through side node k
+            const int k = tid, node = side_node<N>(s, k), st = inward_stride<N>(s);
             for (int d = 0; d < 3; ++d) {
                 double v = 0.0;
                 for (int q = 0; q < nq; ++q) v += s_c[d * kQMax + q] * s_P[q + nq * k];
-                s_out[d + 3 * node] += v;
+                if (a.leg) {
+                    for (int m = 0; m < N; ++m) s_out[d + 3 * (node + m * st)] += v * t.Pf[m];
+                } else {
+                    s_out[d + 3 * node] += v;
+                }
             }
         }
     }
@@ -383,7 +412,7 @@ __device__ __forceinline__ void contract_rows(const double* __restrict__ T, cons
     }
 }
 
-template <int N>
+template <int N, bool LEG>
 __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KArgs a, const double* __restrict__ tab,
                                                                          int kid)
 {
@@ -525,16 +554,12 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
         for (int r = 0; r < NF; ++r) {
             const int it = lane + r * NT, e = it / (4 * N), j = it % (4 * N), s = j / N, k = j % N;
             if (it >= nvalid * 4 * N) continue;
-            const double* qa = u + ((size_t)(e0 + e) * NN + side_node<N>(s, k)) * 3;
-            tr[r][0] = qa[0];
-            tr[r][1] = qa[1];
-            tr[r][2] = qa[2];
+            side_trace<N>(u + ((size_t)(e0 + e) * NN + side_node<N>(s, k)) * 3, s, LEG, tab + kTabPF, tr[r][0],
+                          tr[r][1], tr[r][2]);
             if (nb[r] >= 0) {
                 const int kk = ((inf[r] >> 2) & 1) ? (N - 1 - k) : k;
-                const double* qb = u + ((size_t)nb[r] * NN + side_node<N>(inf[r] & 3, kk)) * 3;
-                tr[r][3] = qb[0];
-                tr[r][4] = qb[1];
-                tr[r][5] = qb[2];
+                side_trace<N>(u + ((size_t)nb[r] * NN + side_node<N>(inf[r] & 3, kk)) * 3, inf[r] & 3, LEG,
+                              tab + kTabPF, tr[r][3], tr[r][4], tr[r][5]);
             }
         }
 #pragma unroll
@@ -603,15 +628,27 @@ __global__ __launch_bounds__(StageShape<N>::NT) void exact_stage_kernel(const KA
             for (int l = 0; l < N; ++l) R[l] += F[3 * N * l] * a.fs;
         }
         const double* FP = RB(le) + 12 * LE + N * ld;  // side s at + 3 N s
-        R[0] += FP[0 * 3 * N + lk];
-        R[N - 1] += FP[2 * 3 * N + lk];
-        if (lk == N - 1) {
+        if constexpr (LEG) {
+            // node (i, l) is row l of side 0, row N-1-i of side 1, row N-1-l
+            // of side 2 and row i of side 3 (legendre_transpose_2d,
+            // FaceProlongator.cpp:908-934)
+            const double* Pf = tab + kTabPF;
+            const double f0 = FP[0 * 3 * N + lk], f2 = FP[2 * 3 * N + lk];
+            const double p1 = Pf[N - 1 - lk], p3 = Pf[lk];
 #pragma unroll
-            for (int l = 0; l < N; ++l) R[l] += FP[1 * 3 * N + l];
-        }
-        if (lk == 0) {
+            for (int l = 0; l < N; ++l)
+                R[l] += Pf[l] * f0 + Pf[N - 1 - l] * f2 + p1 * FP[1 * 3 * N + l] + p3 * FP[3 * 3 * N + l];
+        } else {
+            R[0] += FP[0 * 3 * N + lk];
+            R[N - 1] += FP[2 * 3 * N + lk];
+            if (lk == N - 1) {
 #pragma unroll
-            for (int l = 0; l < N; ++l) R[l] += FP[3 * 3 * N + l];
+                for (int l = 0; l < N; ++l) R[l] += FP[1 * 3 * N + l];
+            }
+            if (lk == 0) {
+#pragma unroll
+                for (int l = 0; l < N; ++l) R[l] += FP[3 * 3 * N + l];
+            }
         }
         double* T1 = RA(le);
         contract_rows<N, N>(tab + kTabV, R, [&](int b, double t) { T1[sidx(ld, lk, b)] = t; });
@@ -683,8 +720,12 @@ template <int N>
 void launch_stage(int kid, const KArgs& a, const double* tab, hipStream_t s)
 {
     constexpr int E = StageShape<N>::E;
-    hipLaunchKernelGGL(exact_stage_kernel<N>, dim3((a.nel + E - 1) / E), dim3(StageShape<N>::NT), 0, s, a, tab,
-                       kid);
+    if (a.leg)
+        hipLaunchKernelGGL((exact_stage_kernel<N, true>), dim3((a.nel + E - 1) / E), dim3(StageShape<N>::NT), 0, s, a,
+                           tab, kid);
+    else
+        hipLaunchKernelGGL((exact_stage_kernel<N, false>), dim3((a.nel + E - 1) / E), dim3(StageShape<N>::NT), 0, s,
+                           a, tab, kid);
 }
 
 template <int N>

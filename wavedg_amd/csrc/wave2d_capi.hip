@@ -153,7 +153,7 @@ struct wdg_wave2d_s {
     {
         const double* t = d_extab;
         return wdg::ExactTables{t + wdg::kTabXV, t + wdg::kTabWV, t + wdg::kTabPV, t + wdg::kTabDV,
-                                t + wdg::kTabXE, t + wdg::kTabWE, t + wdg::kTabPE, nqv, nqe};
+                                t + wdg::kTabXE, t + wdg::kTabWE, t + wdg::kTabPE, nqv, nqe, t + wdg::kTabPF};
     }
     wdg::ExactMassTables mass_tables() const
     {
@@ -884,10 +884,6 @@ static int create_exact(wdg_mesh mesh, int n_colloc, int basis_type, const int* 
         return WDG_ERR_INVALID;
     }
     wdg_wave2d op = nullptr;
-    if (basis_type == WDG_GAUSS_LEGENDRE) {
-        g_last_error = "approx_quad = false takes a Gauss-Lobatto basis";
-        return WDG_ERR_INVALID;
-    }
     const int rc = create_collocated(mesh, n_colloc, basis_type, bc, a, b, WDG_F64, &op);
     if (rc) return rc;
     const int rc2 = guarded([&] {
@@ -918,6 +914,8 @@ static int create_exact(wdg_mesh mesh, int n_colloc, int basis_type, const int* 
         transpose(wdg::kTabDV, nq_volume, n, wdg::kTabDVT);
         transpose(wdg::kTabPE, nq_edge, n, wdg::kTabPET);
         transpose(wdg::kTabV, n, n, wdg::kTabVT);
+        const double xm1 = -1.0;
+        wdg::lagrange_interpolation_matrix(n, op->x.data(), 1, &xm1, tab.data() + wdg::kTabPF);
         op->exact = true;
         op->nqv = nq_volume;
         op->nqe = nq_edge;

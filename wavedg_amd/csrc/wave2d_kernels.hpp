@@ -726,7 +726,8 @@ void launch_euler_face(int n, double gamma, const double* q, double* y, const do
 // One table block per handle, offsets in doubles: volume rule x, w, P, D;
 // edge rule x, w, P; mass matrices A, B, V, lam (ExactMassTables); and the
 // transposes of P, D (volume), P (edge) and V, row-major copies whose rows
-// are contiguous for the fused stage kernel's scalar loads.
+// are contiguous for the fused stage kernel's scalar loads; and the face
+// interpolation weights of a Gauss-Legendre basis.
 enum : int {
     kTabXV = 0,
     kTabWV = 10,
@@ -743,7 +744,8 @@ enum : int {
     kTabDVT = 750,
     kTabPET = 850,
     kTabVT = 950,
-    kTabSize = 1050,
+    kTabPF = 1050,  // phi_k(-1): face interpolation of a Gauss-Legendre basis
+    kTabSize = 1060,
 };
 struct ExactTables {
     const double* xv;  // volume rule (Div<false>)
@@ -754,6 +756,7 @@ struct ExactTables {
     const double* we;
     const double* Pe;
     int nqv, nqe;
+    const double* Pf;  // (n) phi_k(-1), read for Gauss-Legendre bases (KArgs::leg)
 };
 struct ExactMassTables {  // MassMatrix<false> by fast diagonalisation (host mass_fast_diag)
     const double* A;    // (n, n) sum_q w_q phi_i phi_j
