@@ -1,7 +1,8 @@
 """Advection<true> and AdvectionHomogeneousBC<true> on the GPU against the
 oracle's restatement (1e-12 relative L2): constant and per-node
 coefficients, scalar and symmetric systems, and the wave equation's
-matrices, for which the operator is WaveEquation::action."""
+matrices, for which the operator is WaveEquation::action; on Gauss-Lobatto
+and Gauss-Legendre bases."""
 from __future__ import annotations
 
 import numpy as np
@@ -14,23 +15,29 @@ from wavedg_amd import capi
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
+BASES = pytest.mark.parametrize("leg", [False, True], ids=["lobatto", "legendre"])
+
+
+def _wave(v, e, n, leg):
+    return capi.Wave2D(capi.Mesh(v, e), n, None, basis=capi.GAUSS_LEGENDRE if leg else capi.GAUSS_LOBATTO)
 
 
 def _dev(torch, a):
     return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
 
 
+@BASES
 @pytest.mark.parametrize("name", ["skewed6x5", "skew", "L"])
 @pytest.mark.parametrize("n", [2, 4, 7, 9])
 @pytest.mark.parametrize("nvar", [1, 3])
-def test_advection_constant_coefficient(cuda, name, n, nvar):
+def test_advection_constant_coefficient(cuda, name, n, nvar, leg):
     torch = cuda
-    print(f"advection {name} n={n} nvar={nvar}", flush=True)
+    print(f"advection {name} n={n} nvar={nvar} legendre={leg}", flush=True)
     v, e = mesh_case(name)
     A = sym_coeff(nvar, 100 + n + nvar)
-    ow = O.Wave(O.Mesh(v, e), n)
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
     ov = O.Advection(ow, nvar, A, True)
-    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gw = _wave(v, e, n, leg)
     gv = capi.Advection(gw, nvar, A, True)
     size = nvar * gw.ndof // 3
     u, y0 = random_state(size, 7 + n), random_state(size, 9)
@@ -42,18 +49,20 @@ def test_advection_constant_coefficient(cuda, name, n, nvar):
     assert rel_l2(yb.cpu().numpy() - y0, ov.bc_action(u, y0.copy()) - y0) < TOL
 
 
+@BASES
 @pytest.mark.parametrize("n,nvar", [(3, 1), (5, 2), (8, 1)])
-def test_advection_variable_coefficient(cuda, n, nvar):
+def test_advection_variable_coefficient(cuda, n, nvar, leg):
     torch = cuda
+    print(f"advection variable n={n} nvar={nvar} legendre={leg}", flush=True)
     v, e = mesh_case("skewed6x5")
     nel = np.asarray(e).reshape(-1, 4).shape[0]
     rng = np.random.default_rng(n)
     # a symmetric coefficient that varies from node to node
     Anode = np.concatenate([sym_coeff(nvar, int(s)) * (1.0 + 0.3 * rng.standard_normal())
                             for s in rng.integers(0, 1 << 30, n * n * nel)])
-    ow = O.Wave(O.Mesh(v, e), n)
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
     ov = O.Advection(ow, nvar, Anode, False)
-    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gw = _wave(v, e, n, leg)
     gv = capi.Advection(gw, nvar, Anode, False)
     u = random_state(nvar * n * n * nel, 31)
     ya = torch.zeros(u.size, dtype=torch.float64, device="cuda")
@@ -65,11 +74,12 @@ def test_advection_variable_coefficient(cuda, n, nvar):
     assert rel_l2(yb.cpu().numpy(), ov.bc_action(u)) < TOL
 
 
-def test_advection_wave_matrices_equal_wave_action(cuda):
+@BASES
+def test_advection_wave_matrices_equal_wave_action(cuda, leg):
     torch = cuda
     v, e = mesh_case("lshape8")
     n = 6
-    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
+    gw = _wave(v, e, n, leg)
     gv = capi.Advection(gw, 3, WAVE_A, True)
     u = _dev(torch, random_state(gw.ndof, 5))
     y1 = torch.zeros_like(u)

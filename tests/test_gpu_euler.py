@@ -1,6 +1,7 @@
 """DivF2D<true> and EdgeFluxF2D<true> with the compressible Euler flux and
 the local Lax-Friedrichs flux of examples/cl2d.cpp on the GPU against the
-oracle's restatement (1e-12 relative L2), interior and boundary edges."""
+oracle's restatement (1e-12 relative L2), interior and boundary edges, on
+Gauss-Lobatto and Gauss-Legendre bases."""
 from __future__ import annotations
 
 import numpy as np
@@ -15,9 +16,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _state(mesh_o, n, seed):
+def _state(mesh_o, n, seed, leg=False):
     """The cl2d vortex mapped onto the mesh, plus a small random perturbation."""
-    xg, _ = O.gauss_lobatto(n)
+    xg, _ = O.gauss_legendre(n) if leg else O.gauss_lobatto(n)
     X = mesh_o.nodes(xg)
     lo, hi = X.reshape(-1, 2).min(axis=0), X.reshape(-1, 2).max(axis=0)
     s = (X - lo) / (hi - lo)
@@ -26,16 +27,17 @@ def _state(mesh_o, n, seed):
     return q.reshape(-1)
 
 
+@pytest.mark.parametrize("leg", [False, True], ids=["lobatto", "legendre"])
 @pytest.mark.parametrize("name", ["skewed6x5", "skew", "L"])
 @pytest.mark.parametrize("n", [2, 4, 7, 9])
-def test_euler_volume_and_faces(cuda, name, n):
+def test_euler_volume_and_faces(cuda, name, n, leg):
     torch = cuda
-    print(f"euler {name} n={n}", flush=True)
+    print(f"euler {name} n={n} legendre={leg}", flush=True)
     v, e = mesh_case(name)
     om = O.Mesh(v, e)
-    ow = O.Wave(om, n)
-    gw = capi.Wave2D(capi.Mesh(v, e), n, None)
-    q = _state(om, n, n)
+    ow = O.Wave(om, n, legendre=leg)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None, basis=capi.GAUSS_LEGENDRE if leg else capi.GAUSS_LOBATTO)
+    q = _state(om, n, n, leg)
     dq = torch.tensor(q, device="cuda")
     outs = []
     for which in range(3):

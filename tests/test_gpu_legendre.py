@@ -182,8 +182,6 @@ def test_refusals(cuda):
     with pytest.raises(capi.WdgError):
         gw.autotune(2)
     with pytest.raises(capi.WdgError):
-        capi.Advection(gw, 1, np.array([1.0, 0.5]), True)
-    with pytest.raises(capi.WdgError):
         capi.Wave2D(gm, 4, None, basis=7)
     with pytest.raises(capi.WdgError):  # exact quadrature runs in FP64
         capi.Wave2D(gm, 4, None, precision=capi.F32, approx_quad=False, basis=capi.GAUSS_LEGENDRE)

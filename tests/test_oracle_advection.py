@@ -11,13 +11,19 @@
 * conservation: the interior operator sums to zero over all nodes for every
   variable;
 * energy: for symmetric systems the upwind operator with u = 0 outside is
-  dissipative, u^T (A + B) u < 0."""
+  dissipative, u^T (A + B) u < 0.
+
+Each check runs on Gauss-Lobatto and Gauss-Legendre bases; on the latter the
+face values, and a variable coefficient's face values, are interpolated along
+the element (Advection.hpp:77-78, FaceProlongator.cpp:809-835)."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from tests.cases import mesh_case, random_state
 from wavedg_amd import meshgen
+
+BASES = pytest.mark.parametrize("leg", [False, True], ids=["lobatto", "legendre"])
 
 WAVE_A = np.array([0.0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0])
 
@@ -32,20 +38,22 @@ def sym_coeff(nvar: int, seed: int) -> np.ndarray:
     return np.ascontiguousarray(out.reshape(-1))  # [dim, l, k] order = A(k, l, dim) column-major
 
 
+@BASES
 @pytest.mark.parametrize("name,n", [("skewed6x5", 3), ("skew", 5), ("L", 4)])
-def test_advection_with_wave_matrices_is_wave_equation(name, n):
+def test_advection_with_wave_matrices_is_wave_equation(name, n, leg):
     v, e = mesh_case(name)
-    W = O.Wave(O.Mesh(v, e), n)
+    W = O.Wave(O.Mesh(v, e), n, legendre=leg)
     V = O.Advection(W, 3, WAVE_A, True)
     u = random_state(W.ndof, 2)
     assert np.max(np.abs(V.action(u) - W.action(u))) <= 1e-14 * np.max(np.abs(W.action(u)))
 
 
+@BASES
 @pytest.mark.parametrize("nvar", [1, 2, 4])
-def test_variable_coefficient_reduces_to_constant(nvar):
+def test_variable_coefficient_reduces_to_constant(nvar, leg):
     v, e = mesh_case("skewed6x5")
     n = 4
-    W = O.Wave(O.Mesh(v, e), n)
+    W = O.Wave(O.Mesh(v, e), n, legendre=leg)
     A = sym_coeff(nvar, 10 + nvar)
     nel = np.asarray(e).reshape(-1, 4).shape[0]
     Avar = np.tile(A, n * n * nel)
@@ -56,13 +64,14 @@ def test_variable_coefficient_reduces_to_constant(nvar):
     np.testing.assert_allclose(Vv.bc_action(u), Vc.bc_action(u), rtol=0, atol=1e-13 * np.abs(Vc.bc_action(u)).max())
 
 
-def test_scalar_variable_advection_exact_for_linear_fields():
+@BASES
+def test_scalar_variable_advection_exact_for_linear_fields(leg):
     nx = 6
     v, e = meshgen.uniform_rect(nx, 0.0, 1.0, nx, 0.0, 1.0)
     mesh = O.Mesh(v, e)
     n = 4
-    W = O.Wave(mesh, n)
-    xg, _ = O.gauss_lobatto(n)
+    W = O.Wave(mesh, n, legendre=leg)
+    xg, _ = O.gauss_legendre(n) if leg else O.gauss_lobatto(n)
     X = mesh.nodes(xg)  # [el, j, i, xy]
     x, y = X[..., 0], X[..., 1]
     cx = 0.3 + 0.5 * x - 0.2 * y
@@ -78,11 +87,12 @@ def test_scalar_variable_advection_exact_for_linear_fields():
     assert np.max(np.abs(r[interior] - want[interior])) < 1e-12
 
 
+@BASES
 @pytest.mark.parametrize("nvar", [1, 3])
-def test_interior_operator_conserves(nvar):
+def test_interior_operator_conserves(nvar, leg):
     v, e = mesh_case("skewed6x5")
     n = 5
-    W = O.Wave(O.Mesh(v, e), n)
+    W = O.Wave(O.Mesh(v, e), n, legendre=leg)
     V = O.Advection(W, nvar, sym_coeff(nvar, 3), True)
     nel = np.asarray(e).reshape(-1, 4).shape[0]
     u = random_state(nvar * n * n * nel, 8)
@@ -91,11 +101,12 @@ def test_interior_operator_conserves(nvar):
     assert np.all(np.abs(tot) < 1e-13 * scale)
 
 
+@BASES
 @pytest.mark.parametrize("nvar", [1, 2, 3])
-def test_upwind_with_zero_inflow_dissipates(nvar):
+def test_upwind_with_zero_inflow_dissipates(nvar, leg):
     v, e = mesh_case("skew")
     n = 4
-    W = O.Wave(O.Mesh(v, e), n)
+    W = O.Wave(O.Mesh(v, e), n, legendre=leg)
     V = O.Advection(W, nvar, sym_coeff(nvar, 21 + nvar), True)
     nel = np.asarray(e).reshape(-1, 4).shape[0]
     for seed in range(3):

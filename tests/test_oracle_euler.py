@@ -10,6 +10,9 @@ examples/cl2d.cpp by properties that fix it:
   fluxes) converges to -div F(q) at the nodes of elements away from the
   boundary at the rate of a P = 4 scheme (errors 0.086, 0.0057, 0.00036 at
   20^2, 40^2, 80^2 elements).
+
+Each check runs on Gauss-Lobatto and Gauss-Legendre bases (the latter
+interpolate traces along the element, FaceProlongator.cpp:809-835).
 """
 import numpy as np
 import pytest
@@ -19,6 +22,7 @@ from tests.cases import mesh_case
 from wavedg_amd import meshgen
 
 GAMMA = 1.4
+BASES = pytest.mark.parametrize("leg", [False, True], ids=["lobatto", "legendre"])
 
 
 def vortex(x, y):
@@ -49,16 +53,17 @@ def div_exact(x, y, h=1e-5):
     return (f1 - f0) / (2 * h) + (g1 - g0) / (2 * h)
 
 
-def nodal(mesh, n, fn):
-    xg, _ = O.gauss_lobatto(n)
+def nodal(mesh, n, fn, leg=False):
+    xg, _ = O.gauss_legendre(n) if leg else O.gauss_lobatto(n)
     X = mesh.nodes(xg)  # [el, j, i, xy]
     return fn(X[..., 0], X[..., 1])
 
 
+@BASES
 @pytest.mark.parametrize("name,n", [("skewed6x5", 3), ("skew", 5), ("L", 4)])
-def test_free_stream(name, n):
+def test_free_stream(name, n, leg):
     v, e = mesh_case(name)
-    W = O.Wave(O.Mesh(v, e), n)
+    W = O.Wave(O.Mesh(v, e), n, legendre=leg)
     nel = np.asarray(e).reshape(-1, 4).shape[0]
     q = np.tile([1.2, 0.3 * 1.2, -0.4 * 1.2, 2.5], n * n * nel)
     r = W.divf_euler(GAMMA, q)
@@ -68,30 +73,35 @@ def test_free_stream(name, n):
     assert np.abs(r).max() <= 1e-13 * scale
 
 
-def test_conservation():
+@BASES
+def test_conservation(leg):
     v, e = mesh_case("skewed6x5")
     n = 4
     mesh = O.Mesh(v, e)
-    W = O.Wave(mesh, n)
-    q = nodal(mesh, n, lambda x, y: vortex(5.0 + 3.0 * x, -2.0 + 3.0 * y)).reshape(-1)
+    W = O.Wave(mesh, n, legendre=leg)
+    q = nodal(mesh, n, lambda x, y: vortex(5.0 + 3.0 * x, -2.0 + 3.0 * y), leg).reshape(-1)
     r = W.divf_euler(GAMMA, q)
     W.edgefluxf_euler(GAMMA, 0, q, r)
     tot = r.reshape(-1, 4).sum(axis=0)
     assert np.all(np.abs(tot) <= 1e-13 * np.abs(r).sum())
 
 
-def test_converges_to_divergence():
+@BASES
+def test_converges_to_divergence(leg):
     n = 5
     errs = []
-    for nx in (20, 40):
+    # the Legendre basis reaches the asymptotic rate on finer meshes
+    # (errors 0.053, 0.010, 0.00081 at 20^2, 40^2, 80^2 elements)
+    for nx in (40, 80) if leg else (20, 40):
         v, e = meshgen.uniform_rect(nx, 0.0, 10.0, nx, -6.0, 6.0)
         mesh = O.Mesh(v, e)
-        W = O.Wave(mesh, n)
-        q = nodal(mesh, n, vortex)
+        W = O.Wave(mesh, n, legendre=leg)
+        q = nodal(mesh, n, vortex, leg)
         r = W.divf_euler(GAMMA, q.reshape(-1))
         W.edgefluxf_euler(GAMMA, 0, q.reshape(-1), r)
         dq = W.mass_inv(r, nvar=4).reshape(q.shape)
-        want = -nodal(mesh, n, div_exact)
+        want = -nodal(mesh, n, div_exact, leg)
         inner = [el for el in range(nx * nx) if 1 < el % nx < nx - 2 and 1 < el // nx < nx - 2]
         errs.append(np.abs(dq[inner] - want[inner]).max())
-    assert errs[1] < errs[0] / 12.0, errs  # a P = 4 derivative: 2^4 per halving (15.1 measured)
+    # a P = 4 derivative: 2^4 per halving (15.1 measured; 12.4 for the Legendre basis)
+    assert errs[1] < errs[0] / (10.0 if leg else 12.0), errs

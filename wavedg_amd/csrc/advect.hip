@@ -9,7 +9,11 @@
 // so y(node) += H_own u(node) + H_other u(other node), and on boundary sides
 // (u = 0 outside) y(node) += H_own u(node).  Sides are taken one round at a
 // time and corner nodes get their two sides in side order: every output has
-// one writer.
+// one writer.  With a Gauss-Legendre basis (P = phi_m(-1) non-null) "the
+// node" of a side is the line of nodes through it: u(node) stands for
+// sum_m P(m) u(m rows in) and the result is lifted back along that line with
+// the same weights (legendre_action_2d / legendre_transpose_2d,
+// FaceProlongator.cpp:809-835, 908-934).
 #include <hip/hip_runtime.h>
 
 #include "wave2d_kernels.hpp"
@@ -25,8 +29,8 @@ template <int N>
 __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restrict__ u, double* __restrict__ y,
                                                          const int32_t* __restrict__ nbr,
                                                          const int32_t* __restrict__ info,
-                                                         const double* __restrict__ H, int nv, int nel, int interior,
-                                                         int boundary)
+                                                         const double* __restrict__ H, const double* __restrict__ P,
+                                                         int nv, int nel, int interior, int boundary)
 {
     constexpr int NN = N * N;
     constexpr int kOut = kMaxVar * NN > 64 * N ? kMaxVar * NN : 64 * N;
@@ -47,13 +51,37 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
             const double* uA = u + (size_t)nv * ((size_t)e * NN + node);
             const double* Hk = H + (((size_t)e * 4 + s) * N + k) * 2 * v2;
             double acc = 0.0;
-            for (int c = 0; c < nv; ++c) acc += Hk[d + nv * c] * uA[c];
+            if (P) {
+                const int st = nv * inward_stride<N>(s);
+                for (int c = 0; c < nv; ++c) {
+                    double t = 0.0;
+                    for (int m = 0; m < N; ++m) t += uA[c + m * st] * P[m];
+                    acc += Hk[d + nv * c] * t;
+                }
+            } else {
+                for (int c = 0; c < nv; ++c) acc += Hk[d + nv * c] * uA[c];
+            }
             if (is_int) {
                 const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
                 const double* uB = u + (size_t)nv * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
-                for (int c = 0; c < nv; ++c) acc += Hk[v2 + d + nv * c] * uB[c];
+                if (P) {
+                    const int st = nv * inward_stride<N>(inf & 3);
+                    for (int c = 0; c < nv; ++c) {
+                        double t = 0.0;
+                        for (int m = 0; m < N; ++m) t += uB[c + m * st] * P[m];
+                        acc += Hk[v2 + d + nv * c] * t;
+                    }
+                } else {
+                    for (int c = 0; c < nv; ++c) acc += Hk[v2 + d + nv * c] * uB[c];
+                }
             }
-            s_out[el * vn + d + nv * node] += acc;
+            if (P) {
+                // lane (k, d) owns variable d of the line through side node k
+                const int st = inward_stride<N>(s);
+                for (int m = 0; m < N; ++m) s_out[el * vn + d + nv * (node + m * st)] += acc * P[m];
+            } else {
+                s_out[el * vn + d + nv * node] += acc;
+            }
         }
         __syncthreads();
     }
@@ -78,18 +106,40 @@ __device__ __forceinline__ void euler_fg(double gamma, const double* q, double* 
     speed = hypot(u, v) + sqrt(gamma * p / rho);
 }
 
+// The four conserved variables on element side s at the side node whose
+// state starts at p: its values (Gauss-Lobatto), or sum_m P(m) q(m rows in).
+template <int N>
+__device__ __forceinline__ void trace4(const double* p, int s, const double* P, double* out)
+{
+    if (!P) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) out[d] = p[d];
+        return;
+    }
+    const int st = 4 * inward_stride<N>(s);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        double t = 0.0;
+        for (int m = 0; m < N; ++m) t += p[d + m * st] * P[m];
+        out[d] = t;
+    }
+}
+
 // EdgeFluxF2D<true> with the LLF flux, negated and lifted as in cl2d.cpp's
 // time derivative (cl2d.cpp:180-215), element by element: at node k of side
 // s, with this element's outward normal n, y -= w_k ds LLF(n, q_own, q_other)
 // (the flux is odd in (n, qL, qR), so el0 and el1 of an edge agree with the
 // reference's +-fh); on boundary sides q_other = q_own.  E = 64 / N elements
-// per workgroup, one lane per (element, side node), sides in order.
+// per workgroup, one lane per (element, side node), sides in order.  A
+// Gauss-Legendre basis (P non-null) interpolates the traces along the
+// element and lifts back along the same line.
 template <int N>
 __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const double* __restrict__ q,
                                                         double* __restrict__ y, const double* __restrict__ geom,
                                                         const int32_t* __restrict__ nbr,
                                                         const int32_t* __restrict__ info,
-                                                        const double* __restrict__ basis, int nel, int interior,
+                                                        const double* __restrict__ basis,
+                                                        const double* __restrict__ P, int nel, int interior,
                                                         int boundary)
 {
     constexpr int NN = N * N, E = 64 / N;
@@ -114,13 +164,11 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
                 const int node = side_node<N>(s, k);
                 double qa[4], qb[4];
                 const double* pa = q + 4 * ((size_t)e * NN + node);
-#pragma unroll
-                for (int d = 0; d < 4; ++d) qa[d] = pa[d];
+                trace4<N>(pa, s, P, qa);
                 if (is_int) {
                     const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
                     const double* pb = q + 4 * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) qb[d] = pb[d];
+                    trace4<N>(pb, inf & 3, P, qb);
                 } else {
 #pragma unroll
                     for (int d = 0; d < 4; ++d) qb[d] = qa[d];
@@ -132,7 +180,12 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const double fh = 0.5 * n0 * (fa[d] + fb[d]) + 0.5 * n1 * (ga[d] + gb[d]) + 0.5 * c * (qa[d] - qb[d]);
-                    s_out[el * 4 * NN + 4 * node + d] -= fh * wds;
+                    if (P) {
+                        const int st = inward_stride<N>(s);
+                        for (int m = 0; m < N; ++m) s_out[el * 4 * NN + 4 * (node + m * st) + d] -= (fh * wds) * P[m];
+                    } else {
+                        s_out[el * 4 * NN + 4 * node + d] -= fh * wds;
+                    }
                 }
             }
         }
@@ -207,14 +260,15 @@ void launch_linear_functional(int n, int nq, int nv, const double* corners, cons
 }
 
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
-                       const int32_t* info, const double* basis, int nel, int interior, int boundary, hipStream_t s)
+                       const int32_t* info, const double* basis, const double* P, int nel, int interior, int boundary,
+                       hipStream_t s)
 {
     if (nel <= 0) return;
     const dim3 block(64);
 #define WDG_EULER_CASE(NV_)                                                                                   \
     case NV_:                                                                                                 \
         hipLaunchKernelGGL(euler_face_kernel<NV_>, dim3((nel + 64 / NV_ - 1) / (64 / NV_)), block, 0, s, gamma, q, \
-                           y, geom, nbr, info, basis, nel, interior, boundary);                               \
+                           y, geom, nbr, info, basis, P, nel, interior, boundary);                            \
         break;
     switch (n) {
         WDG_EULER_CASE(2)
@@ -231,14 +285,14 @@ void launch_euler_face(int n, double gamma, const double* q, double* y, const do
 }
 
 void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
-                        const double* H, int nel, int interior, int boundary, hipStream_t s)
+                        const double* H, const double* P, int nel, int interior, int boundary, hipStream_t s)
 {
     if (nel <= 0 || nv < 1 || nv > kMaxVar) return;
     const int per = n * nv, E = per < 64 ? 64 / per : 1;
     const dim3 grid((nel + E - 1) / E), block(64);
 #define WDG_ADVECT_CASE(NV_)                                                                                  \
     case NV_:                                                                                                 \
-        hipLaunchKernelGGL(advect_face_kernel<NV_>, grid, block, 0, s, u, y, nbr, info, H, nv, nel, interior, \
+        hipLaunchKernelGGL(advect_face_kernel<NV_>, grid, block, 0, s, u, y, nbr, info, H, P, nv, nel, interior, \
                            boundary);                                                                         \
         break;
     switch (n) {

@@ -192,6 +192,8 @@ struct wdg_wave2d_s {
         if (cur != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
     }
     size_t state_bytes() const { return (size_t)ndof * rsz; }
+    // device phi_m(-1) of a Gauss-Legendre basis, null for Gauss-Lobatto
+    const double* face_interp() const { return leg ? d_basis + 2 * n + n * n : nullptr; }
     void* ensure(void*& p, size_t bytes)
     {
         if (!p) p = dmalloc(bytes);
@@ -1569,12 +1571,11 @@ int wdg_hip_edgefluxf2d_euler(wdg_wave2d op, double gamma, int face_type, const 
     return guarded([&] {
         check_op(op);
         if (op->exact || op->prec != WDG_F64) fail(WDG_ERR_INVALID, "needs an FP64 approx_quad = true wave handle");
-        if (op->leg) fail(WDG_ERR_INVALID, "EdgeFluxF2D takes Gauss-Lobatto handles");
         if (!d_q || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
         if (!(gamma > 1.0)) fail(WDG_ERR_INVALID, "gamma must exceed 1");
         if (face_type != WDG_INTERIOR && face_type != WDG_BOUNDARY) fail(WDG_ERR_INVALID, "bad face type");
         wdg::launch_euler_face(op->n, gamma, static_cast<const double*>(d_q), static_cast<double*>(d_divF),
-                               op->d_geom, op->d_nbr, op->d_info, op->d_basis, op->nel,
+                               op->d_geom, op->d_nbr, op->d_info, op->d_basis, op->face_interp(), op->nel,
                                face_type == WDG_INTERIOR ? 1 : 0, face_type == WDG_BOUNDARY ? 1 : 0,
                                as_stream(stream));
         hip_check(hipGetLastError(), "edge flux kernel launch");
@@ -1604,7 +1605,6 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
         if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
         if (op->exact || op->prec != WDG_F64)
             fail(WDG_ERR_INVALID, "advection needs an FP64 approx_quad = true wave handle");
-        if (op->leg) fail(WDG_ERR_INVALID, "advection takes Gauss-Lobatto handles");
         const int N = op->n, NN = N * N, nel = op->nel, m = n_var, v2 = m * m;
         auto adv = std::make_unique<wdg_advection_s>();
         adv->op = op;
@@ -1615,11 +1615,30 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
         hip_check(hipMemcpy(geom.data(), op->d_geom, sizeof(double) * geom.size(), hipMemcpyDeviceToHost), "geometry");
         hip_check(hipMemcpy(nbr.data(), op->d_nbr, sizeof(int32_t) * nbr.size(), hipMemcpyDeviceToHost), "nbr");
         hip_check(hipMemcpy(info.data(), op->d_info, sizeof(int32_t) * info.size(), hipMemcpyDeviceToHost), "info");
-        auto coef = [&](int e, int node) { return adv->cc ? A : A + (size_t)2 * v2 * (node + (size_t)NN * e); };
         auto side_node = [&](int s, int k) {
             const int i = (s == 0 || s == 2) ? k : (s == 1 ? N - 1 : 0);
             const int j = (s == 1 || s == 3) ? k : (s == 2 ? N - 1 : 0);
             return i + N * j;
+        };
+        // the coefficient on side s of element e at side node k: the node's
+        // value, or for a Gauss-Legendre basis the interpolation along the
+        // element, as the reference prolongs it (Advection.hpp:77-78)
+        std::vector<double> Pf(N, 0.0), Aface(2 * v2);
+        if (op->leg) {
+            const double xm1 = -1.0;
+            wdg::lagrange_interpolation_matrix(N, op->x.data(), 1, &xm1, Pf.data());
+        }
+        auto coef = [&](int e, int s, int k) -> const double* {
+            if (adv->cc) return A;
+            const int node = side_node(s, k);
+            if (!op->leg) return A + (size_t)2 * v2 * (node + (size_t)NN * e);
+            const int st = s == 0 ? N : s == 1 ? -1 : s == 2 ? -N : 1;
+            for (int q = 0; q < 2 * v2; ++q) {
+                double t = 0.0;
+                for (int mm = 0; mm < N; ++mm) t += A[(size_t)2 * v2 * (node + mm * st + (size_t)NN * e) + q] * Pf[mm];
+                Aface[q] = t;
+            }
+            return Aface.data();
         };
         // H_own = w ds (a/2 C + b |C|), H_other = w ds (a/2 C' - b |C'|), a = -1, b = -0.5,
         // C = n.A at this node, C' = n.A at the other side's node, n this element's outward normal
@@ -1641,11 +1660,11 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
                 for (int k = 0; k < N; ++k) {
                     double* Hk = &H[(((size_t)e * 4 + s) * N + k) * 2 * v2];
                     const double wds = op->w[k] * ds;
-                    normal_matrix(coef(e, side_node(s, k)), n0, n1);
+                    normal_matrix(coef(e, s, k), n0, n1);
                     for (int q = 0; q < v2; ++q) Hk[q] = wds * (-0.5 * C[q] - 0.5 * absC[q]);
                     if (nb >= 0) {
                         const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                        normal_matrix(coef(nb, side_node(inf & 3, kk)), n0, n1);
+                        normal_matrix(coef(nb, inf & 3, kk), n0, n1);
                         for (int q = 0; q < v2; ++q) Hk[v2 + q] = wds * (-0.5 * C[q] + 0.5 * absC[q]);
                     }
                 }
@@ -1691,7 +1710,8 @@ static int advection_launch(wdg_advection adv, const void* d_u, void* d_divF, bo
             wdg::launch_divf_linear(op->n, 0, a, s);
         }
         wdg::launch_advect_face(op->n, adv->nv, static_cast<const double*>(d_u), static_cast<double*>(d_divF),
-                                op->d_nbr, op->d_info, adv->d_H, op->nel, interior ? 1 : 0, boundary ? 1 : 0, s);
+                                op->d_nbr, op->d_info, adv->d_H, op->face_interp(), op->nel, interior ? 1 : 0,
+                                boundary ? 1 : 0, s);
         hip_check(hipGetLastError(), "advection kernel launch");
     });
 }

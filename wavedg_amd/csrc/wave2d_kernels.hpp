@@ -714,12 +714,14 @@ void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 // Advection<true> / AdvectionHomogeneousBC<true> face terms (advect.hip):
 // y(node) += H_own u(node) + H_other u(neighbour node) for every node of the
 // selected sides, H per (element, side, node): (2, nv, nv) column-major.
+// P: face interpolation phi_m(-1) of a Gauss-Legendre basis, or null.
 void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
-                        const double* H, int nel, int interior, int boundary, hipStream_t s);
+                        const double* H, const double* P, int nel, int interior, int boundary, hipStream_t s);
 // y -= w ds LLF(n, q_own, q_other) at every node of the selected sides for the
 // compressible Euler flux (advect.hip); boundary sides take q_other = q_own.
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
-                       const int32_t* info, const double* basis, int nel, int interior, int boundary, hipStream_t s);
+                       const int32_t* info, const double* basis, const double* P, int nel, int interior, int boundary,
+                       hipStream_t s);
 
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
