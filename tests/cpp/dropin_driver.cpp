@@ -186,6 +186,38 @@ int main(int argc, char** argv)
                 return 1;
             }
         }
+        // --- examples/advec2d.cpp as written: Gauss-Legendre basis, the
+        // coefficient and the initial condition projected by
+        // LinearFunctional2D then MassMatrix<true>::inv, one time derivative
+        {
+            const int n4 = 5;
+            auto basis4 = QuadratureRule::quadrature_rule(n4, QuadratureRule::GaussLegendre);
+            Mesh2D mesh4 = Mesh2D::uniform_rect(6, -1.0, 1.0, 6, -1.0, 1.0);
+            const size_t np = (size_t)n4 * n4 * mesh4.n_elem();
+            MassMatrix<true> m4(mesh4, basis4);
+            LinearFunctional2D LF4(mesh4, basis4);
+            std::vector<double> c(2 * np), u4(np), du4(np, 0.0);
+            LF4.action(2, [](const double x[2], double* cc) {
+                const double r = std::hypot(x[0], x[1]);
+                cc[0] = r * x[1];
+                cc[1] = -r * x[0];
+            }, c.data());
+            m4.inv(2, c.data());
+            LF4.action(1, [](const double x[2], double* F) {
+                const double r = std::hypot(x[0], x[1]);
+                *F = r * std::exp(-10.0 * r * r) * std::cos(3.0 * std::atan2(x[1], x[0]));
+            }, u4.data());
+            m4.inv(1, u4.data());
+            Advection<true> adv(1, mesh4, basis4, c.data(), false);
+            AdvectionHomogeneousBC<true> adv_bc(1, mesh4, basis4, c.data(), false);
+            adv.action(u4.data(), du4.data());
+            adv_bc.action(u4.data(), du4.data());
+            m4.inv(1, du4.data());
+            to_file(dir + "/advgl_x.bin", 2 * np, mesh4.physical_coordinates(basis4).data());
+            to_file(dir + "/advgl_c.bin", c.size(), c.data());
+            to_file(dir + "/advgl_u.bin", np, u4.data());
+            to_file(dir + "/advgl_dudt.bin", np, du4.data());
+        }
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {
         std::fprintf(stderr, "dg::hip::Error %d: %s\n", e.code, e.what());

@@ -110,3 +110,20 @@ def test_dropin_waveholtz_and_operators(tmp_path):
         y = 0.25 * (y + 3.0 * uu + 0.01 * f(y))
         uu = (uu + 2.0 * y + 2.0 * 0.01 * f(y)) / 3.0
     assert rel_l2(_read(d, "adv_ssprk3.bin"), uu) < 1e-12
+
+    # examples/advec2d.cpp as written (Gauss-Legendre basis): with the n-point
+    # rule on the basis nodes, LinearFunctional2D then MassMatrix<true>::inv
+    # returns the functor's values at the nodes
+    W4 = O.Wave(O.Mesh(v3, e3), 5, legendre=True)
+    xg, _ = O.gauss_legendre(5)
+    X4 = O.Mesh(v3, e3).nodes(xg)  # [el, j, i, xy]
+    np.testing.assert_allclose(_read(d, "advgl_x.bin"), X4.reshape(-1), rtol=0, atol=1e-15)
+    r4 = np.hypot(X4[..., 0], X4[..., 1])
+    c4 = np.stack([r4 * X4[..., 1], -r4 * X4[..., 0]], axis=-1).reshape(-1)
+    u4 = (r4 * np.exp(-10 * r4 * r4) * np.cos(3 * np.arctan2(X4[..., 1], X4[..., 0]))).reshape(-1)
+    assert rel_l2(_read(d, "advgl_c.bin"), c4) < 1e-13
+    assert rel_l2(_read(d, "advgl_u.bin"), u4) < 1e-13
+    A4 = O.Advection(W4, 1, _read(d, "advgl_c.bin"), False)
+    u4 = _read(d, "advgl_u.bin")
+    want = W4.mass_inv(A4.action(u4) + A4.bc_action(u4), nvar=1)
+    assert rel_l2(_read(d, "advgl_dudt.bin"), want) < 1e-12

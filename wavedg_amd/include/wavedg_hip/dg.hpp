@@ -13,10 +13,10 @@
 // pointers (hipMalloc'd) and an optional hipStream_t and do no copies; they
 // are the ones to use in time loops.
 //
-// Bases are Gauss-Lobatto.  approx_quad = true collocates the integrals on the
-// basis nodes (FP64 or FP32); approx_quad = false integrates them exactly with
-// Gauss-Legendre rules, chosen as the reference chooses them when quad is null
-// (FP64 only).
+// Bases are Gauss-Lobatto or Gauss-Legendre rules, as in the reference.
+// approx_quad = true collocates the integrals on the basis nodes (FP64 or
+// FP32); approx_quad = false integrates them exactly with Gauss-Legendre
+// rules, chosen as the reference chooses them when quad is null (FP64 only).
 #pragma once
 
 #include <cmath>
@@ -75,8 +75,8 @@ private:
 
 // QuadratureRule::quadrature_rule(n, type) (QuadratureRule.hpp:77,
 // QuadratureRule.cpp:164-184): cached rules, Gauss-Legendre by default as in
-// the reference.  Gauss-Lobatto rules serve as bases; Gauss-Legendre rules as
-// the quad argument of the approx_quad = false operators.
+// the reference.  Either family serves as a basis; Gauss-Legendre rules also
+// as the quad argument of the approx_quad = false operators.
 class QuadratureRule {
 public:
     enum QuadratureType { GaussLegendre, GaussLobatto };
@@ -139,10 +139,11 @@ public:
     // element_metrics(basis).physical_coordinates(): (2, n, n, n_elem)
     std::vector<double> physical_coordinates(const QuadratureRule* basis) const
     {
-        if (basis->type != QuadratureRule::GaussLobatto)
-            throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
         std::vector<double> x(2 * (size_t)basis->n * basis->n * n_elem());
-        check(wdg_hip_mesh_nodes(h_.get(), basis->n, x.data()));
+        if (basis->type == QuadratureRule::GaussLobatto)
+            check(wdg_hip_mesh_nodes(h_.get(), basis->n, x.data()));
+        else
+            check(wdg_hip_mesh_gl_points(h_.get(), basis->n, x.data()));
         return x;
     }
     // (5, n_edges): el0, side0, el1, side1, delta in Mesh2D::edge(i, type) order
@@ -208,15 +209,15 @@ public:
            bool approx_quad = true, const QuadratureRule* quad = nullptr, const QuadratureRule* mass_quad = nullptr)
     {
         if (!basis) throw std::invalid_argument("dg::hip: null basis");
-        if (basis->type != QuadratureRule::GaussLobatto)
-            throw std::invalid_argument("dg::hip: only Gauss-Lobatto bases are supported");
+        const int type = basis->type == QuadratureRule::GaussLobatto ? WDG_GAUSS_LOBATTO : WDG_GAUSS_LEGENDRE;
         wdg_wave2d h = nullptr;
         if (approx_quad) {
-            check(wdg_hip_wave2d_create(mesh.handle(), basis->n, bc, a, b, (int)p, &h));
+            check(wdg_hip_wave2d_create_basis(mesh.handle(), basis->n, type, bc, a, b, (int)p, 1, 0, 0, 0, &h));
         } else {
             if (p != Precision::F64) throw std::invalid_argument("dg::hip: approx_quad = false runs in FP64 only");
             const int nq = exact_rule_size(quad);
-            check(wdg_hip_wave2d_create_exact(mesh.handle(), basis->n, bc, a, b, nq, nq, exact_rule_size(mass_quad), &h));
+            check(wdg_hip_wave2d_create_basis(mesh.handle(), basis->n, type, bc, a, b, (int)p, 0, nq, nq,
+                                              exact_rule_size(mass_quad), &h));
         }
         h_.reset(h);
         check(wdg_hip_wave2d_info(h, &ndof_, nullptr, nullptr, nullptr));
