@@ -67,8 +67,8 @@ def build_workload(nx: int):
 
 
 def time_path(torch, capi, mesh, n, bc, u0, dt, precision, steps, warmup, graph: int = 0, autotune: bool = False,
-              approx_quad: bool = True):
-    wave = capi.Wave2D(mesh, n, bc, precision=precision, approx_quad=approx_quad)
+              approx_quad: bool = True, basis=None):
+    wave = capi.Wave2D(mesh, n, bc, precision=precision, approx_quad=approx_quad, basis=basis)
     if autotune:
         best, ms = wave.autotune(reps=10)
         wave.tuned = {"schedule": best, "threads_per_workgroup": wave.schedule()[1],
@@ -392,6 +392,8 @@ def main() -> int:
     ap.add_argument("--learned", type=int, default=16,
                     help="held-out problems for the plain vs batched+learned WaveHoltz line (0 = skip)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--legendre-steps", type=int, default=400,
+                    help="RK4 steps of the same workload on a Gauss-Legendre basis (0 = skip)")
     ap.add_argument("--exact-steps", type=int, default=400,
                     help="RK4 steps of the approx_quad = false (exact quadrature) path on the same mesh (0 = skip)")
     ap.add_argument("--exact-cpu-nx", type=int, default=128, help="mesh size of the exact path's CPU-oracle sample")
@@ -573,6 +575,27 @@ def main() -> int:
                           f"{args.exact_cpu_nx}^2 quads, {args.cpu_steps} RK4 step(s), {sx:.1f} s",
             }
             print(f"[bench] exact cpu: {vx:.3e} DoF*stage/s", file=sys.stderr, flush=True)
+
+    if args.legendre_steps > 0:
+        wl, ul, secl, ms_kl, nkl = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F64, args.legendre_steps,
+                                             min(args.warmup, 4), basis=capi.GAUSS_LEGENDRE)
+        avgl = ms_kl / max(nkl, 1)
+        achl = bytes64 / (avgl * 1e-3) / 1e9
+        out["legendre_basis"] = {
+            "workload": f"the same mesh, state and dt on a Gauss-Legendre basis of {n} points "
+                        f"(approx_quad = true, FP64); {args.legendre_steps} RK4 steps",
+            "value": ndof * 5 * args.legendre_steps / secl,
+            "unit": "DoF*stage/s",
+            "ms_per_step": secl * 1e3 / args.legendre_steps,
+            "kernel": "wave_kernel<8, double, stage | M_LEG>",
+            "avg_launch_us": avgl * 1e3,
+            "bytes_per_launch": bytes64,
+            "achieved_GBs": achl,
+            "frac": achl / HBM_PEAK_GBS,
+        }
+        print(f"[bench] legendre basis: {secl:.3f} s, {avgl*1e3:.1f} us/stage ({achl:.0f} GB/s)", file=sys.stderr,
+              flush=True)
+        del wl, ul
 
     if args.euler_reps > 0:
         out["euler"] = euler_rate(torch, capi, mesh, n, args.euler_reps)
