@@ -59,6 +59,7 @@ typedef struct wdg_mesh_s *wdg_mesh;
 typedef struct wdg_wave2d_s *wdg_wave2d;
 typedef struct wdg_waveholtz_s *wdg_waveholtz;
 typedef struct wdg_advection_s *wdg_advection;
+typedef struct wdg_flux_s *wdg_flux;
 
 const char *wdg_hip_last_error(void);
 int wdg_hip_version(int *major, int *minor);
@@ -192,6 +193,40 @@ int wdg_hip_advection_action(wdg_advection adv, const void *d_u, void *d_divF, v
  * edge fluxes with u = 0 outside the domain. */
 int wdg_hip_advection_bc_action(wdg_advection adv, const void *d_u, void *d_divF, void *stream);
 int wdg_hip_advection_destroy(wdg_advection adv);
+
+/* ---- DivF2D / EdgeFluxF2D with flux functors --------------------------
+ * The reference's DivF2D<true>::action(n_var, f, u, divF) and
+ * EdgeFluxF2D<true>::action(n_var, flux, uB, fB) (DivF.hpp:207-274,
+ * EdgeFluxF.hpp:136-171) take C++ functors.  Here the functors are HIP
+ * source text in the reference's signatures, compiled for the GPU at run
+ * time (hipRTC) with the handle's order, n_var and basis, on first use:
+ *   __device__ void flux(const double *x, const double *u, double *F);
+ *       F (n_var, 2) column-major: F[d] = f_d, F[n_var + d] = g_d
+ *   __device__ void numerical_flux(const double *x, const double *n,
+ *                                  const double *uL, const double *uR, double *fh);
+ *   and optionally, after #define WDG_EXTERIOR_STATE,
+ *   __device__ void exterior_state(const double *x, const double *n,
+ *                                  const double *u, double *u_ext);
+ * A source may define only the functors it uses.  n_var <= 8, FP64
+ * approx_quad = true handles of either basis; the wave handle must outlive
+ * the flux handle. */
+int wdg_hip_flux_create(wdg_wave2d op, int n_var, const char *source, wdg_flux *out);
+/* compile now rather than on first use: which 0 = flux, 1 = numerical_flux;
+ * a source that does not compile returns WDG_ERR_INVALID with the compiler log */
+int wdg_hip_flux_compile(wdg_flux fx, int which);
+/* compile a source for (n_colloc, n_var, basis_type) without a device or a
+ * handle, to check it: WDG_ERR_INVALID with the compiler log if it fails */
+int wdg_hip_flux_check(const char *source, int n_colloc, int n_var, int basis_type, int which);
+/* d_divF += (F(x, u), grad v) at every node, u and divF (n_var, n, n, n_elem) */
+int wdg_hip_divf2d(wdg_flux fx, const void *d_u, void *d_divF, void *stream);
+/* Face prolongation, EdgeFluxF2D<true>::action and the transpose, for
+ * interior or boundary (face_type) edges: with el0's outward normal n and
+ * uL = el0's trace, fh = numerical_flux(x, n, uL, uR); el0's nodes receive
+ * scale * fh w ds and el1's -scale * fh w ds (cl2d.cpp:186-216 uses
+ * scale = -1).  Boundary edges take uR = exterior_state(x, n, uL), or uL
+ * itself when the source does not define WDG_EXTERIOR_STATE. */
+int wdg_hip_edgefluxf2d(wdg_flux fx, int face_type, double scale, const void *d_u, void *d_divF, void *stream);
+int wdg_hip_flux_destroy(wdg_flux fx);
 /* WaveBC::action (WaveEquation.cpp:190-215): d_y += B d_x (boundary flux and lift). */
 int wdg_hip_wavebc_action(wdg_wave2d op, const void *d_x, void *d_y, void *stream);
 /* MassMatrix<true>::inv (MassMatrix.cpp:69-82): d_x <- M^{-1} d_x, in place. */

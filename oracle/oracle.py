@@ -21,6 +21,13 @@ _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
+FLUX_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+NFLUX_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+EXT_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 def build() -> None:
@@ -78,6 +85,9 @@ def lib():
             "orc_edge_flux": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
             "orc_divf_euler": (None, [_vp, ctypes.c_double, _dp, _dp]),
             "orc_edgefluxf_euler": (None, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp]),
+            "orc_divf_functor": (None, [_vp, ctypes.c_int, FLUX_FN, _vp, _dp, _dp]),
+            "orc_edgefluxf_functor": (None, [_vp, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int, ctypes.c_double,
+                                             _dp, _dp]),
             "orc_advection_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
             "orc_advection_free": (None, [_vp]),
             "orc_advection_action": (None, [_vp, _dp, _dp]),
@@ -260,6 +270,38 @@ class Wave:
         q = np.ascontiguousarray(q, dtype=np.float64)
         divF = np.zeros(q.size) if divF is None else divF
         lib().orc_edgefluxf_euler(self.h, gamma, face_type, _d(q), _d(divF))
+        return divF
+
+    def divf_functor(self, nvar: int, f, u, divF=None):
+        """DivF2D<true> with a Python functor f(x[2], u[nvar]) -> F (nvar, 2)
+        as a (2 * nvar) array [f_0..f_{nvar-1}, g_0..g_{nvar-1}]."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        divF = np.zeros(u.size) if divF is None else divF
+
+        def cb(xp, up, Fp, _ctx):
+            x = np.ctypeslib.as_array(xp, (2,))
+            uu = np.ctypeslib.as_array(up, (nvar,))
+            np.ctypeslib.as_array(Fp, (2 * nvar,))[:] = f(x, uu)
+
+        lib().orc_divf_functor(self.h, nvar, FLUX_FN(cb), None, _d(u), _d(divF))
+        return divF
+
+    def edgefluxf_functor(self, nvar: int, fh, face_type: int, u, scale: float = 1.0, ext=None, divF=None):
+        """divF += scale * lifted EdgeFluxF2D<true> with a Python numerical
+        flux fh(x, n, uL, uR) -> (nvar,); ext(x, n, u) -> exterior state."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        divF = np.zeros(u.size) if divF is None else divF
+
+        def cb(xp, np_, lp, rp, fp, _ctx):
+            a = [np.ctypeslib.as_array(p, (k,)) for p, k in ((xp, 2), (np_, 2), (lp, nvar), (rp, nvar))]
+            np.ctypeslib.as_array(fp, (nvar,))[:] = fh(*a)
+
+        def cb_ext(xp, np_, up, ep, _ctx):
+            a = [np.ctypeslib.as_array(p, (k,)) for p, k in ((xp, 2), (np_, 2), (up, nvar))]
+            np.ctypeslib.as_array(ep, (nvar,))[:] = ext(*a)
+
+        ext_c = EXT_FN(cb_ext) if ext is not None else EXT_FN()
+        lib().orc_edgefluxf_functor(self.h, nvar, NFLUX_FN(cb), ext_c, None, face_type, scale, _d(u), _d(divF))
         return divF
 
     def mass_action(self, x, nvar: int = 3):

@@ -1309,6 +1309,110 @@ void orc_waveholtz_postprocess(const orc_wh *H, double *Hout, const double *u)
 /* load vector                                                           */
 /* ===================================================================== */
 
+
+/* ===================================================================== */
+/* DivF2D / EdgeFluxF2D with caller functors (C callbacks)                */
+/* ===================================================================== */
+
+/* DivF2D<true>::action (DivF.hpp:207-274) with the functor f(x, u, F),
+ * F (nvar, 2) column-major, at every node: divF += (F(x, u), grad v). */
+void orc_divf_functor(const orc_wave *W, int nvar, orc_flux_fn f, void *ctx, const double *u, double *divF)
+{
+    const int n = W->n, nel = W->m->nel;
+    const double *D = W->D, *w = W->w, *x = W->x;
+    double *F = (double *)malloc(sizeof(double) * 2 * (size_t)n * n * nvar);
+    double Fq[16], X[2];
+    for (int el = 0; el < nel; ++el) {
+        const double *c = W->m->corners + 8 * (size_t)el;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double J[4];
+                quad_jacobian(c, x[i], x[j], J);
+                quad_coords(c, x[i], x[j], X);
+                const double w_ij = w[i] * w[j];
+                const double Y_eta = J[3] * w_ij, X_eta = J[2] * w_ij;
+                const double Y_xi = J[1] * w_ij, X_xi = J[0] * w_ij;
+                for (int d = 0; d < 2 * nvar; ++d) Fq[d] = 0.0;
+                f(X, u + (size_t)nvar * (i + n * (j + (size_t)n * el)), Fq, ctx);
+                for (int d = 0; d < nvar; ++d) {
+                    F[0 + 2 * (i + n * (d + nvar * j))] = Y_eta * Fq[d] - X_eta * Fq[nvar + d];
+                    F[1 + 2 * (j + n * (d + nvar * i))] = -Y_xi * Fq[d] + X_xi * Fq[nvar + d];
+                }
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < nvar; ++d) {
+                    double dF = 0.0;
+                    for (int l = 0; l < n; ++l)
+                        dF += D[l + n * i] * F[0 + 2 * (l + n * (d + nvar * j))] +
+                              D[l + n * j] * F[1 + 2 * (l + n * (d + nvar * i))];
+                    divF[d + nvar * (i + n * (j + (size_t)n * el))] += dF;
+                }
+    }
+    free(F);
+}
+
+/* prol->action, EdgeFluxF2D<true>::action with the functor fh(x, n, uL, uR)
+ * (EdgeFluxF.hpp:136-171: fB(side 0) = +fh w ds, fB(side 1) = -fh w ds),
+ * a scale (cl2d.cpp:190-194 uses -1) and prol->t (FaceProlongator.cpp:565-672,
+ * 809-934).  x is the edge node on el0's side, n el0's outward normal.  On
+ * boundary edges (type 1) side 1 is ext(x, n, u) if given, else a copy of
+ * side 0 (cl2d.cpp:198-209). */
+void orc_edgefluxf_functor(const orc_wave *W, int nvar, orc_nflux_fn fh_fn, orc_ext_fn ext, void *ctx, int type,
+                           double scale, const double *u, double *divF)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    double *uf = (double *)calloc((size_t)2 * n * nvar * ne + 1, sizeof(double)); /* (n, nvar, 2, ne) */
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < nvar; ++d)
+                    uf[i + n * (d + nvar * (side + 2 * (size_t)e))] = face_value(W, u, nvar, j, f, d);
+            }
+        }
+    double uL[8], uR[8], fh[8], X[2];
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(W->m, e, type);
+        const int el0 = E->el[0], s0 = E->side[0];
+        const double *c = W->m->corners + 8 * (size_t)el0;
+        for (int i = 0; i < n; ++i) {
+            /* edge node i in el0's order, at the basis abscissa along side s0 */
+            const double xi = (s0 == 0 || s0 == 2) ? W->x[i] : (s0 == 1 ? 1.0 : -1.0);
+            const double eta = (s0 == 1 || s0 == 3) ? W->x[i] : (s0 == 2 ? 1.0 : -1.0);
+            quad_coords(c, xi, eta, X);
+            for (int d = 0; d < nvar; ++d) uL[d] = uf[i + n * (d + nvar * (0 + 2 * (size_t)e))];
+            if (type == 0) {
+                for (int d = 0; d < nvar; ++d) uR[d] = uf[i + n * (d + nvar * (1 + 2 * (size_t)e))];
+            } else if (ext) {
+                ext(X, E->n, uL, uR, ctx);
+            } else {
+                for (int d = 0; d < nvar; ++d) uR[d] = uL[d];
+            }
+            for (int d = 0; d < nvar; ++d) fh[d] = 0.0;
+            fh_fn(X, E->n, uL, uR, fh, ctx);
+            for (int d = 0; d < nvar; ++d) {
+                const double fl = fh[d] * W->w[i] * E->meas;
+                uf[i + n * (d + nvar * (0 + 2 * (size_t)e))] = scale * fl;
+                uf[i + n * (d + nvar * (1 + 2 * (size_t)e))] = -scale * fl;
+            }
+        }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < nvar; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    face_add(W, divF, nvar, j, f, d, uf[i + n * (d + nvar * (side + 2 * (size_t)e))]);
+                }
+        }
+    free(uf);
+}
+
 /* LinearFunctional.hpp:62-74 (constructor) and :136-202 (2D action). */
 void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src, int var, double *F)
 {

@@ -117,6 +117,20 @@ void orc_advection_bc_action(const orc_adv *V, const double *u, double *divF);
 void orc_divf_euler(const orc_wave *W, double gamma, const double *u, double *divF);
 void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double *u, double *divF);
 
+/* DivF2D<true> / EdgeFluxF2D<true> with caller functors in the reference's
+ * signatures (plus a context pointer), nvar <= 8.  orc_divf_functor:
+ * divF += (F(x, u), grad v), F (nvar, 2) column-major.
+ * orc_edgefluxf_functor: divF += scale * lifted (+-fh w ds) on interior
+ * (type 0) or boundary (type 1) edges; ext (may be NULL: copy) gives the
+ * exterior state of boundary edges. */
+typedef void (*orc_flux_fn)(const double *x, const double *u, double *F, void *ctx);
+typedef void (*orc_nflux_fn)(const double *x, const double *n, const double *uL, const double *uR, double *fh,
+                             void *ctx);
+typedef void (*orc_ext_fn)(const double *x, const double *n, const double *u, double *u_ext, void *ctx);
+void orc_divf_functor(const orc_wave *W, int nvar, orc_flux_fn f, void *ctx, const double *u, double *divF);
+void orc_edgefluxf_functor(const orc_wave *W, int nvar, orc_nflux_fn fh, orc_ext_fn ext, void *ctx, int type,
+                           double scale, const double *u, double *divF);
+
 /* ---- reductions ------------------------------------------------------ */
 double orc_norm(long long n, const double *x);
 double orc_error(long long n, const double *x, const double *y);

@@ -1,0 +1,135 @@
+"""DivF2D<true> / EdgeFluxF2D<true> with flux functors compiled at run time
+(hipRTC) on the GPU, 1e-12 relative L2:
+
+* cl2d's Euler flux and LLF numerical flux as source against the built-in
+  Euler kernels and the oracle's functor path (scale -1 as cl2d);
+* the wave equation's fluxes as source reproduce WaveEquation::action
+  (volume + interior faces) and WaveBC::action (boundary faces with the
+  reflecting exterior state as exterior_state);
+* a numerical flux that is not odd in (n, uL, uR), so that el1 must call it
+  with el0's normal and state, against the oracle's functor path;
+on Gauss-Lobatto and Gauss-Legendre bases."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import mesh_case, random_state, rel_l2
+from tests.test_gpu_euler import _state
+from tests.test_oracle_euler import GAMMA
+from tests.test_oracle_functor import euler_F, euler_llf
+from wavedg_amd import capi, fluxes
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+BASES = pytest.mark.parametrize("leg", [False, True], ids=["lobatto", "legendre"])
+
+# left-biased and x-dependent: fh = (n . c(x)) uL with c = (1 + x, 0.5 - y)
+SKEW = r"""
+__device__ void numerical_flux(const double* x, const double* n, const double* uL, const double* uR, double* fh)
+{
+    const double cn = n[0] * (1.0 + x[0]) + n[1] * (0.5 - x[1]);
+    fh[0] = cn * uL[0] + 0.1 * uR[1];
+    fh[1] = cn * uL[1] - 0.2 * x[1] * uR[0];
+}
+"""
+
+
+def skew_py(x, n, uL, uR):
+    cn = n[0] * (1.0 + x[0]) + n[1] * (0.5 - x[1])
+    return np.array([cn * uL[0] + 0.1 * uR[1], cn * uL[1] - 0.2 * x[1] * uR[0]])
+
+
+def _basis(leg):
+    return capi.GAUSS_LEGENDRE if leg else capi.GAUSS_LOBATTO
+
+
+@BASES
+@pytest.mark.parametrize("name", ["skewed6x5", "skew"])
+@pytest.mark.parametrize("n", [2, 4, 7, 9])
+def test_euler_source_matches_builtin_and_oracle(cuda, name, n, leg):
+    torch = cuda
+    print(f"flux euler {name} n={n} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    om = O.Mesh(v, e)
+    ow = O.Wave(om, n, legendre=leg)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None, basis=_basis(leg))
+    fx = capi.Flux(gw, 4, fluxes.EULER_LLF)
+    q = _state(om, n, n, leg)
+    dq = torch.tensor(q, device="cuda")
+    outs = [torch.zeros_like(dq) for _ in range(6)]
+    fx.divf(dq, outs[0])
+    fx.edgeflux(capi.INTERIOR, dq, outs[1], scale=-1.0)
+    fx.edgeflux(capi.BOUNDARY, dq, outs[2], scale=-1.0)
+    gw.divf_euler(GAMMA, dq, outs[3])
+    gw.edgefluxf_euler(GAMMA, 0, dq, outs[4])
+    gw.edgefluxf_euler(GAMMA, 1, dq, outs[5])
+    torch.cuda.synchronize()
+    got = [o.cpu().numpy() for o in outs]
+    for a, b in zip(got[:3], got[3:]):
+        assert rel_l2(a, b) < TOL
+    assert rel_l2(got[0], ow.divf_functor(4, euler_F, q)) < TOL
+    assert rel_l2(got[1], ow.edgefluxf_functor(4, euler_llf, 0, q, scale=-1.0)) < TOL
+    assert rel_l2(got[2], ow.edgefluxf_functor(4, euler_llf, 1, q, scale=-1.0)) < TOL
+
+
+@BASES
+@pytest.mark.parametrize("name,n", [("skewed6x5", 4), ("L", 6), ("lshape8", 3)])
+def test_wave_source_reproduces_wave_operator(cuda, name, n, leg):
+    torch = cuda
+    print(f"flux wave {name} n={n} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None, basis=_basis(leg))  # reflecting walls, upwind
+    fx = capi.Flux(gw, 3, fluxes.WAVE_UPWIND)
+    u = torch.tensor(random_state(gw.ndof, 3 + n), device="cuda")
+    a, b = torch.zeros_like(u), torch.zeros_like(u)
+    fx.divf(u, a)
+    fx.edgeflux(capi.INTERIOR, u, a)
+    fx.edgeflux(capi.BOUNDARY, u, b)
+    wa, wb = torch.zeros_like(u), torch.zeros_like(u)
+    gw.action(u, wa)
+    gw.bc_action(u, wb)
+    torch.cuda.synchronize()
+    assert rel_l2(a.cpu().numpy(), wa.cpu().numpy()) < TOL
+    assert rel_l2(b.cpu().numpy(), wb.cpu().numpy()) < TOL
+
+
+@BASES
+@pytest.mark.parametrize("name,n", [("skewed6x5", 3), ("skew", 5), ("L", 8)])
+def test_non_odd_numerical_flux(cuda, name, n, leg):
+    torch = cuda
+    print(f"flux skew {name} n={n} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None, basis=_basis(leg))
+    fx = capi.Flux(gw, 2, SKEW)
+    u = random_state(2 * gw.ndof // 3, 40 + n)
+    y0 = random_state(u.size, 41)
+    du = torch.tensor(u, device="cuda")
+    yi, yb = torch.tensor(y0, device="cuda"), torch.tensor(y0, device="cuda")
+    fx.edgeflux(capi.INTERIOR, du, yi, scale=0.75)
+    fx.edgeflux(capi.BOUNDARY, du, yb, scale=0.75)
+    torch.cuda.synchronize()
+    want_i = ow.edgefluxf_functor(2, skew_py, 0, u, scale=0.75)
+    want_b = ow.edgefluxf_functor(2, skew_py, 1, u, scale=0.75)
+    assert rel_l2(yi.cpu().numpy() - y0, want_i) < TOL
+    assert rel_l2(yb.cpu().numpy() - y0, want_b) < TOL
+
+
+def test_errors(cuda):
+    torch = cuda
+    v, e = mesh_case("rect5x4")
+    gw = capi.Wave2D(capi.Mesh(v, e), 3, None)
+    fx = capi.Flux(gw, 2, SKEW)  # no flux() defined
+    u = torch.zeros(2 * gw.ndof // 3, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(u)
+    with pytest.raises(capi.WdgError, match="does not compile"):
+        fx.divf(u, y)
+    fx.compile(1)
+    with pytest.raises(capi.WdgError):
+        fx.edgeflux(2, u, y)
+    with pytest.raises(capi.WdgError):
+        capi.Flux(capi.Wave2D(capi.Mesh(v, e), 3, None, approx_quad=False), 2, SKEW)
+    with pytest.raises(capi.WdgError):
+        capi.Flux(gw, 9, SKEW)

@@ -45,7 +45,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     for n in ORDERS:
         obj = os.path.join(OUT_DIR, f"wave2d_kernels_n{n}.o")
         steps.append(([HIPCC, *HIP_FLAGS, f"-DWDG_N={n}", "-c", os.path.join(CSRC, "wave2d_kernels_n.hip"), "-o", obj], obj))
-    for src in ("wave2d_capi.hip", "batch.hip", "cnn.hip", "divf.hip", "exact.hip", "advect.hip"):
+    for src in ("wave2d_capi.hip", "batch.hip", "cnn.hip", "divf.hip", "exact.hip", "advect.hip", "flux_rtc.hip"):
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
         steps.append(([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], obj))
     for src in ("host_basis.cpp", "host_mesh.cpp"):
@@ -58,7 +58,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
             f.result()
     objs = [o for _, o in steps]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-lhiprtc", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -53,6 +53,12 @@ _SIGS = {
     "wdg_hip_wave2d_create_exact": [_vp, _i, _ip, _d, _d, _i, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_create_basis": [_vp, _i, _i, _ip, _d, _d, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_basis": [_vp, _ip, _dp, _dp],
+    "wdg_hip_flux_create": [_vp, _i, ctypes.c_char_p, ctypes.POINTER(_vp)],
+    "wdg_hip_flux_compile": [_vp, _i],
+    "wdg_hip_flux_check": [ctypes.c_char_p, _i, _i, _i, _i],
+    "wdg_hip_divf2d": [_vp, _vp, _vp, _vp],
+    "wdg_hip_edgefluxf2d": [_vp, _i, _d, _vp, _vp, _vp],
+    "wdg_hip_flux_destroy": [_vp],
     "wdg_hip_wave2d_quadrature": [_vp, _ip, _ip, _ip, _ip],
     "wdg_hip_wave2d_destroy": [_vp],
     "wdg_hip_wave2d_info": [_vp, ctypes.POINTER(_ll), _ip, _ip, _ip],
@@ -424,6 +430,40 @@ class Wave2D:
     def load_vector_gauss(self, sources, F, var: int = 0, stream=None):
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
         _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, dev_ptr(F),
+              stream if stream is not None else cur_stream())
+
+
+def flux_check(source: str, n: int, n_var: int, basis: int = GAUSS_LOBATTO, which: int = 0) -> None:
+    """Compile a flux source for (n, n_var, basis) without a GPU; raises
+    WdgError with the compiler log if it does not compile."""
+    _call("wdg_hip_flux_check", source.encode(), n, n_var, basis, which)
+
+
+class Flux:
+    """DivF2D<true> / EdgeFluxF2D<true> with functors given as HIP source
+    (flux, numerical_flux and optionally exterior_state in the reference's
+    signatures), compiled for the GPU at run time."""
+
+    def __init__(self, wave: Wave2D, n_var: int, source: str):
+        self.wave = wave  # the handle must outlive this object
+        self.n_var = n_var
+        h = _vp()
+        _call("wdg_hip_flux_create", wave.h, n_var, source.encode(), ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.wdg_hip_flux_destroy(self.h)
+            self.h = None
+
+    def compile(self, which: int):
+        _call("wdg_hip_flux_compile", self.h, which)
+
+    def divf(self, u, divF, stream=None):
+        _call("wdg_hip_divf2d", self.h, dev_ptr(u), dev_ptr(divF), stream if stream is not None else cur_stream())
+
+    def edgeflux(self, face_type: int, u, divF, scale: float = 1.0, stream=None):
+        _call("wdg_hip_edgefluxf2d", self.h, face_type, scale, dev_ptr(u), dev_ptr(divF),
               stream if stream is not None else cur_stream())
 
 

@@ -1,0 +1,336 @@
+// DivF2D<true> and EdgeFluxF2D<true> with flux functors given as source
+// text and compiled for gfx950 at run time with hipRTC (DivF.hpp:207-274,
+// EdgeFluxF.hpp:136-171).  The reference takes C++ functors; a C ABI cannot
+// carry a functor to the GPU, so the caller passes the functor's body in the
+// reference's signature as HIP source:
+//
+//   __device__ void flux(const double* x, const double* u, double* F);
+//       F (n_var, 2) column-major: F[d] = f_d(x, u), F[n_var + d] = g_d(x, u)
+//   __device__ void numerical_flux(const double* x, const double* n,
+//                                  const double* uL, const double* uR, double* fh);
+//   optionally, with #define WDG_EXTERIOR_STATE,
+//   __device__ void exterior_state(const double* x, const double* n,
+//                                  const double* u, double* u_ext);
+//
+// The library appends its two kernels (below) to that text and compiles each
+// one on first use with the handle's order, variable count and basis as
+// macros, so the functor inlines into the kernel.  The kernels follow the
+// element-local layout of the other operators: one lane per node (volume) or
+// per side node (faces, sides in order, every output with one writer).
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "wave2d_kernels.hpp"
+
+namespace wdg {
+namespace {
+
+const char* const kRtcKernels = R"WDGRTC(
+namespace wdg_rtc {
+constexpr int N = WDG_N, NN = N * N, NV = WDG_NVAR;
+
+// Bilinear map of corners c (2, 4) at (xi, eta): X and
+// J = (x_xi, y_xi, x_eta, y_eta) (Element.cpp:5-27).
+__device__ inline void corner_map(const double* c, double xi, double eta, double* X, double* J)
+{
+    const double b0 = 0.25 * (1.0 - xi) * (1.0 - eta), b1 = 0.25 * (1.0 + xi) * (1.0 - eta);
+    const double b2 = 0.25 * (1.0 + xi) * (1.0 + eta), b3 = 0.25 * (1.0 - xi) * (1.0 + eta);
+    X[0] = c[0] * b0 + c[2] * b1 + c[4] * b2 + c[6] * b3;
+    X[1] = c[1] * b0 + c[3] * b1 + c[5] * b2 + c[7] * b3;
+    J[0] = 0.25 * ((1.0 - eta) * (c[2] - c[0]) + (1.0 + eta) * (c[4] - c[6]));
+    J[1] = 0.25 * ((1.0 - eta) * (c[3] - c[1]) + (1.0 + eta) * (c[5] - c[7]));
+    J[2] = 0.25 * ((1.0 - xi) * (c[6] - c[0]) + (1.0 + xi) * (c[4] - c[2]));
+    J[3] = 0.25 * ((1.0 - xi) * (c[7] - c[1]) + (1.0 + xi) * (c[5] - c[3]));
+}
+
+__device__ inline int side_node(int s, int k)
+{
+    const int i = (s == 0 || s == 2) ? k : (s == 1 ? N - 1 : 0);
+    const int j = (s == 1 || s == 3) ? k : (s == 2 ? N - 1 : 0);
+    return i + N * j;
+}
+
+__device__ inline int inward_stride(int s) { return s == 0 ? N : s == 1 ? -1 : s == 2 ? -N : 1; }
+
+// Values of the NV variables on element side s at side node k of element e:
+// the node's values (Gauss-Lobatto), or sum_m P(m) u(m rows in) (Gauss-Legendre).
+__device__ inline void side_trace(const double* u, int e, int s, int k, const double* P, double* out)
+{
+    const int node = side_node(s, k);
+    const double* q = u + (size_t)NV * ((size_t)e * NN + node);
+    if (WDG_LEG) {
+        const int st = NV * inward_stride(s);
+        for (int d = 0; d < NV; ++d) {
+            double t = 0.0;
+            for (int m = 0; m < N; ++m) t += q[d + m * st] * P[m];
+            out[d] = t;
+        }
+    } else {
+        for (int d = 0; d < NV; ++d) out[d] = q[d];
+    }
+}
+}  // namespace wdg_rtc
+
+#ifdef WDG_WANT_DIVF
+// DivF2D<true>::action (DivF.hpp:207-274): divF += sum_l D(l,i) F~(l,j) + D(l,j) G~(i,l)
+// with the contravariant fluxes F~ = w_i w_j (y_eta f - x_eta g),
+// G~ = w_i w_j (-y_xi f + x_xi g) of flux(x, u) at every node.  E elements
+// of N^2 lanes per workgroup.
+extern "C" __global__ void __launch_bounds__(256) wdg_divf_rtc(const double* __restrict__ u, double* __restrict__ out,
+                                                                const double* __restrict__ corners,
+                                                                const double* __restrict__ basis, int nel)
+{
+    using namespace wdg_rtc;
+    constexpr int E = 256 / NN;
+    __shared__ double sF[E * NV * NN], sG[E * NV * NN];
+    const int tid = threadIdx.x, el = tid / NN, node = tid - el * NN, i = node % N, j = node / N;
+    const int e = blockIdx.x * E + el;
+    const bool active = el < E && e < nel;
+    const double* xb = basis;
+    const double* wb = basis + N;
+    const double* D = basis + 2 * N;
+    if (active) {
+        double X[2], J[4];
+        corner_map(corners + 8 * (size_t)e, xb[i], xb[j], X, J);
+        const double wij = wb[i] * wb[j];
+        const double Ye = J[3] * wij, Xe = J[2] * wij, Yx = J[1] * wij, Xx = J[0] * wij;
+        double uq[NV], Fq[2 * NV];
+        for (int d = 0; d < NV; ++d) uq[d] = u[(size_t)NV * ((size_t)e * NN + node) + d];
+        for (int d = 0; d < 2 * NV; ++d) Fq[d] = 0.0;
+        flux(X, uq, Fq);
+        for (int d = 0; d < NV; ++d) {
+            sF[(el * NV + d) * NN + j * N + i] = Ye * Fq[d] - Xe * Fq[NV + d];
+            sG[(el * NV + d) * NN + i * N + j] = -Yx * Fq[d] + Xx * Fq[NV + d];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    for (int d = 0; d < NV; ++d) {
+        double r = 0.0;
+        for (int l = 0; l < N; ++l)
+            r += D[l + N * i] * sF[(el * NV + d) * NN + j * N + l] + D[l + N * j] * sG[(el * NV + d) * NN + i * N + l];
+        out[(size_t)NV * ((size_t)e * NN + node) + d] += r;
+    }
+}
+#endif
+
+#ifdef WDG_WANT_EDGE
+// EdgeFluxF2D<true>::action (EdgeFluxF.hpp:136-171) between the face
+// prolongation and its transpose, element by element: at side node k of
+// side s, with el0's normal n and uL = el0's trace, fh = numerical_flux(x, n,
+// uL, uR), and this element receives scale * (+fh w ds) if it is el0 and
+// scale * (-fh w ds) if el1, lifted into the side's node (or, for a
+// Gauss-Legendre basis, along its line with weights P).  Boundary sides take
+// the exterior state from exterior_state(), or a copy of the interior trace.
+// E = 64 / N elements per workgroup, one lane per (element, side node).
+extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* __restrict__ u,
+                                                                   double* __restrict__ out,
+                                                                   const double* __restrict__ corners,
+                                                                   const int* __restrict__ nbr,
+                                                                   const int* __restrict__ info,
+                                                                   const double* __restrict__ basis, int nel,
+                                                                   int interior, int boundary, double scale)
+{
+    using namespace wdg_rtc;
+    constexpr int E = 64 / N;
+    __shared__ double s_out[E * NV * NN];
+    const int tid = threadIdx.x, e0 = blockIdx.x * E;
+    const int nloc = nel - e0 < E ? nel - e0 : E;
+    for (int t = tid; t < E * NV * NN; t += 64) s_out[t] = 0.0;
+    __syncthreads();
+    const int el = tid / N, k = tid - el * N, e = e0 + el;
+    const bool on = tid < E * N && el < nloc;
+    const double* xb = basis;
+    const double* wb = basis + N;
+    const double* P = basis + 2 * N + NN;
+    for (int s = 0; s < 4; ++s) {
+        if (on) {
+            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+            const bool is_int = nb >= 0;
+            if (is_int ? interior : boundary) {
+                const double* c = corners + 8 * (size_t)e;
+                // side vector in traversal order (sides 2, 3 run 3->2, 0->3)
+                const int a0 = s == 0 ? 0 : s == 1 ? 1 : s == 2 ? 3 : 0;
+                const int a1 = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? 2 : 3;
+                const double dx = c[2 * a1] - c[2 * a0], dy = c[2 * a1 + 1] - c[2 * a0 + 1];
+                const double len = sqrt(dx * dx + dy * dy), sg = s >= 2 ? -1.0 : 1.0;
+                double nrm[2] = {sg * dy / len, -sg * dx / len};  // this element's outward normal
+                const double ds = 0.5 * len;
+                const double xi = s == 0 || s == 2 ? xb[k] : (s == 1 ? 1.0 : -1.0);
+                const double eta = s == 1 || s == 3 ? xb[k] : (s == 2 ? 1.0 : -1.0);
+                double X[2], J[4];
+                corner_map(c, xi, eta, X, J);
+                double uA[NV], uB[NV], fh[NV];
+                side_trace(u, e, s, k, P, uA);
+                if (is_int) {
+                    const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                    side_trace(u, nb, inf & 3, kk, P, uB);
+                } else {
+#ifdef WDG_EXTERIOR_STATE
+                    exterior_state(X, nrm, uA, uB);
+#else
+                    for (int d = 0; d < NV; ++d) uB[d] = uA[d];
+#endif
+                }
+                const bool first = (inf >> 3) & 1;
+                const int ie = (!first && ((inf >> 2) & 1)) ? (N - 1 - k) : k;  // el0's node order
+                for (int d = 0; d < NV; ++d) fh[d] = 0.0;
+                if (first) {
+                    numerical_flux(X, nrm, uA, uB, fh);
+                } else {
+                    nrm[0] = -nrm[0];
+                    nrm[1] = -nrm[1];
+                    numerical_flux(X, nrm, uB, uA, fh);
+                }
+                const double sgn = first ? scale : -scale;
+                const int node = side_node(s, k), st = inward_stride(s);
+                for (int d = 0; d < NV; ++d) {
+                    const double fl = sgn * (fh[d] * wb[ie] * ds);
+                    if (WDG_LEG) {
+                        for (int m = 0; m < N; ++m) s_out[(el * NN + node + m * st) * NV + d] += fl * P[m];
+                    } else {
+                        s_out[(el * NN + node) * NV + d] += fl;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* yb = out + (size_t)e0 * NV * NN;
+    for (int t = tid; t < nloc * NV * NN; t += 64) yb[t] += s_out[t];
+}
+#endif
+)WDGRTC";
+
+struct Compiled {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+};
+
+}  // namespace
+
+// Compiles the caller's source with the library's kernel for one order,
+// variable count and basis into a gfx950 code object (no device needed).
+bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::vector<char>& code, std::string& err)
+{
+    const std::string src = user + "\n" + kRtcKernels;
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "wdg_flux.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string dn = "-DWDG_N=" + std::to_string(n), dv = "-DWDG_NVAR=" + std::to_string(nv),
+                      dl = "-DWDG_LEG=" + std::to_string(leg ? 1 : 0);
+    // the same code-generation options as the library's own kernels; only
+    // the kernel asked for is compiled, so a source may define one functor
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops", dn.c_str(), dv.c_str(), dl.c_str(),
+                          divf ? "-DWDG_WANT_DIVF" : "-DWDG_WANT_EDGE"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        err = "flux source does not compile: " + log;
+        return false;
+    }
+    size_t sz = 0;
+    hiprtcGetCodeSize(prog, &sz);
+    code.assign(sz, 0);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+namespace {
+
+// rtc_build, then load the kernel on the current device.
+bool rtc_compile(const std::string& user, int n, int nv, int leg, bool divf, Compiled& out, std::string& err)
+{
+    const char* kernel = divf ? "wdg_divf_rtc" : "wdg_edgeflux_rtc";
+    std::vector<char> code;
+    if (!rtc_build(user, n, nv, leg, divf, code, err)) return false;
+    if (hipModuleLoadData(&out.mod, code.data()) != hipSuccess) {
+        err = "hipModuleLoadData failed for the compiled flux kernels";
+        return false;
+    }
+    if (hipModuleGetFunction(&out.fn, out.mod, kernel) != hipSuccess) {
+        (void)hipModuleUnload(out.mod);
+        out.mod = nullptr;
+        err = std::string("compiled flux module lacks ") + kernel;
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+struct RtcFlux {
+    std::string source;
+    int n = 0, nv = 0, leg = 0;
+    Compiled divf, edge;
+    std::mutex mu;
+    ~RtcFlux()
+    {
+        if (divf.mod) (void)hipModuleUnload(divf.mod);
+        if (edge.mod) (void)hipModuleUnload(edge.mod);
+    }
+};
+
+RtcFlux* rtc_flux_create(const char* source, int n, int nv, int leg)
+{
+    auto* f = new RtcFlux();
+    f->source = source;
+    f->n = n;
+    f->nv = nv;
+    f->leg = leg;
+    return f;
+}
+
+void rtc_flux_destroy(RtcFlux* f) { delete f; }
+
+bool rtc_flux_prepare(RtcFlux* f, int which, std::string& err)
+{
+    std::lock_guard<std::mutex> lk(f->mu);
+    Compiled& c = which == 0 ? f->divf : f->edge;
+    if (c.fn) return true;
+    return rtc_compile(f->source, f->n, f->nv, f->leg, which == 0, c, err);
+}
+
+bool rtc_divf(RtcFlux* f, const double* u, double* out, const double* corners, const double* basis, int nel,
+              hipStream_t s, std::string& err)
+{
+    if (!rtc_flux_prepare(f, 0, err)) return false;
+    if (nel <= 0) return true;
+    const int NN = f->n * f->n, E = 256 / NN;
+    void* params[] = {&u, &out, &corners, &basis, &nel};
+    if (hipModuleLaunchKernel(f->divf.fn, (nel + E - 1) / E, 1, 1, E * NN, 1, 1, 0, s, params, nullptr) != hipSuccess) {
+        err = "flux divergence kernel launch failed";
+        return false;
+    }
+    return true;
+}
+
+bool rtc_edgeflux(RtcFlux* f, const double* u, double* out, const double* corners, const int32_t* nbr,
+                  const int32_t* info, const double* basis, int nel, int interior, int boundary, double scale,
+                  hipStream_t s, std::string& err)
+{
+    if (!rtc_flux_prepare(f, 1, err)) return false;
+    if (nel <= 0) return true;
+    const int E = 64 / f->n;
+    void* params[] = {&u, &out, &corners, &nbr, &info, &basis, &nel, &interior, &boundary, &scale};
+    if (hipModuleLaunchKernel(f->edge.fn, (nel + E - 1) / E, 1, 1, 64, 1, 1, 0, s, params, nullptr) != hipSuccess) {
+        err = "edge flux kernel launch failed";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace wdg

@@ -1732,6 +1732,96 @@ int wdg_hip_advection_destroy(wdg_advection adv)
     return WDG_OK;
 }
 
+// ------------------------------------------------ flux functors (hipRTC)
+struct wdg_flux_s {
+    wdg_wave2d op = nullptr;  // borrowed: the wave handle outlives the flux
+    int nv = 0;
+    wdg::RtcFlux* f = nullptr;
+    ~wdg_flux_s() { wdg::rtc_flux_destroy(f); }
+};
+
+int wdg_hip_flux_create(wdg_wave2d op, int n_var, const char* source, wdg_flux* out)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!source || !out) fail(WDG_ERR_INVALID, "null argument");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (op->exact || op->prec != WDG_F64)
+            fail(WDG_ERR_INVALID, "flux functors need an FP64 approx_quad = true wave handle");
+        auto fx = std::make_unique<wdg_flux_s>();
+        fx->op = op;
+        fx->nv = n_var;
+        fx->f = wdg::rtc_flux_create(source, op->n, n_var, op->leg);
+        *out = fx.release();
+    });
+}
+
+int wdg_hip_flux_check(const char* source, int n_colloc, int n_var, int basis_type, int which)
+{
+    return guarded([&] {
+        if (!source) fail(WDG_ERR_INVALID, "null source");
+        if (n_colloc < 2 || n_colloc > 9) fail(WDG_ERR_INVALID, "n_colloc out of range 2..9");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (basis_type != WDG_GAUSS_LOBATTO && basis_type != WDG_GAUSS_LEGENDRE) fail(WDG_ERR_INVALID, "bad basis");
+        if (which != 0 && which != 1) fail(WDG_ERR_INVALID, "which must be 0 (flux) or 1 (numerical_flux)");
+        std::vector<char> code;
+        std::string err;
+        if (!wdg::rtc_build(source, n_colloc, n_var, basis_type == WDG_GAUSS_LEGENDRE, which == 0, code, err))
+            fail(WDG_ERR_INVALID, err);
+    });
+}
+
+int wdg_hip_flux_compile(wdg_flux fx, int which)
+{
+    return guarded([&] {
+        if (!fx) fail(WDG_ERR_INVALID, "null flux handle");
+        if (which != 0 && which != 1) fail(WDG_ERR_INVALID, "which must be 0 (flux) or 1 (numerical_flux)");
+        check_op(fx->op);
+        std::string err;
+        if (!wdg::rtc_flux_prepare(fx->f, which, err)) fail(WDG_ERR_INVALID, err);
+    });
+}
+
+int wdg_hip_divf2d(wdg_flux fx, const void* d_u, void* d_divF, void* stream)
+{
+    return guarded([&] {
+        if (!fx) fail(WDG_ERR_INVALID, "null flux handle");
+        if (!d_u || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
+        if (d_u == d_divF) fail(WDG_ERR_INVALID, "input and output must not alias");
+        wdg_wave2d op = fx->op;
+        check_op(op);
+        std::string err;
+        if (!wdg::rtc_divf(fx->f, static_cast<const double*>(d_u), static_cast<double*>(d_divF), op->d_corners,
+                           op->d_basis, op->nel, as_stream(stream), err))
+            fail(WDG_ERR_INVALID, err);
+        hip_check(hipGetLastError(), "flux divergence kernel");
+    });
+}
+
+int wdg_hip_edgefluxf2d(wdg_flux fx, int face_type, double scale, const void* d_u, void* d_divF, void* stream)
+{
+    return guarded([&] {
+        if (!fx) fail(WDG_ERR_INVALID, "null flux handle");
+        if (!d_u || !d_divF) fail(WDG_ERR_INVALID, "null device pointer");
+        if (d_u == d_divF) fail(WDG_ERR_INVALID, "input and output must not alias");
+        if (face_type != WDG_INTERIOR && face_type != WDG_BOUNDARY) fail(WDG_ERR_INVALID, "bad face type");
+        wdg_wave2d op = fx->op;
+        check_op(op);
+        std::string err;
+        if (!wdg::rtc_edgeflux(fx->f, static_cast<const double*>(d_u), static_cast<double*>(d_divF), op->d_corners,
+                               op->d_nbr, op->d_info, op->d_basis, op->nel, face_type == WDG_INTERIOR ? 1 : 0,
+                               face_type == WDG_BOUNDARY ? 1 : 0, scale, as_stream(stream), err))
+            fail(WDG_ERR_INVALID, err);
+        hip_check(hipGetLastError(), "edge flux kernel");
+    });
+}
+
+int wdg_hip_flux_destroy(wdg_flux fx)
+{
+    delete fx;
+    return WDG_OK;
+}
+
 // ------------------------------------------------------------ file formats
 int wdg_hip_to_file(const char* path, const void* d_x, long long n, int precision)
 {

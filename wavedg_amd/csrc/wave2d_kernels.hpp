@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+#include <vector>
 #include <type_traits>
 
 namespace wdg {
@@ -722,6 +724,25 @@ void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
                        const int32_t* info, const double* basis, const double* P, int nel, int interior, int boundary,
                        hipStream_t s);
+
+// DivF2D<true> / EdgeFluxF2D<true> with flux functors compiled from source
+// at run time (flux_rtc.hip).  Kernels compile on first use; the functions
+// return false with a message on a compile or launch failure.
+struct RtcFlux;
+// compile only (no device): the code object of one kernel for (n, nv, leg)
+bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::vector<char>& code, std::string& err);
+RtcFlux* rtc_flux_create(const char* source, int n, int nv, int leg);
+void rtc_flux_destroy(RtcFlux* f);
+// which: 0 the divergence kernel, 1 the edge-flux kernel
+bool rtc_flux_prepare(RtcFlux* f, int which, std::string& err);
+// out += DivF2D<true>(flux, u)
+bool rtc_divf(RtcFlux* f, const double* u, double* out, const double* corners, const double* basis, int nel,
+              hipStream_t s, std::string& err);
+// out += scale * lifted EdgeFluxF2D<true>(numerical_flux, traces of u) on
+// interior and / or boundary sides
+bool rtc_edgeflux(RtcFlux* f, const double* u, double* out, const double* corners, const int32_t* nbr,
+                  const int32_t* info, const double* basis, int nel, int interior, int boundary, double scale,
+                  hipStream_t s, std::string& err);
 
 // Exact-quadrature operators, FP64 (exact.hip).  Tables are device arrays,
 // column-major (n_quad, n): P(q, k) = phi_k(x_q), D(q, k) = phi_k'(x_q).
