@@ -218,6 +218,66 @@ int main(int argc, char** argv)
             to_file(dir + "/advgl_u.bin", np, u4.data());
             to_file(dir + "/advgl_dudt.bin", np, du4.data());
         }
+        // --- examples/cl2d.cpp's time derivative with the flux functors as
+        // source: DivF2D, EdgeFluxF2D on interior and boundary faces (scale
+        // -1, exterior = interior), MassMatrix<true>::inv, on the isentropic
+        // vortex (cl2d.cpp:37-55)
+        {
+            const FluxSource F{R"(
+#define GAMMA 1.4
+__device__ inline void euler_fg(const double* q, double* f, double* g)
+{
+    const double rho = q[0], u = q[1] / rho, v = q[2] / rho, E = q[3];
+    const double p = (GAMMA - 1.0) * (E - 0.5 * rho * (u * u + v * v));
+    f[0] = rho * u; f[1] = rho * u * u + p; f[2] = rho * u * v; f[3] = u * (E + p);
+    g[0] = rho * v; g[1] = rho * u * v; g[2] = rho * v * v + p; g[3] = v * (E + p);
+}
+__device__ inline double euler_speed(const double* q)
+{
+    const double rho = q[0], u = q[1] / rho, v = q[2] / rho;
+    const double p = (GAMMA - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v));
+    return hypot(u, v) + sqrt(GAMMA * p / rho);
+}
+__device__ void flux(const double* x, const double* q, double* F) { euler_fg(q, F, F + 4); }
+__device__ void numerical_flux(const double* x, const double* n, const double* qL, const double* qR, double* fh)
+{
+    double fL[4], gL[4], fR[4], gR[4];
+    euler_fg(qL, fL, gL);
+    euler_fg(qR, fR, gR);
+    const double cL = euler_speed(qL), cR = euler_speed(qR), c = cL > cR ? cL : cR;
+    for (int d = 0; d < 4; ++d)
+        fh[d] = 0.5 * n[0] * (fL[d] + fR[d]) + 0.5 * n[1] * (gL[d] + gR[d]) + 0.5 * c * (qL[d] - qR[d]);
+}
+)"};
+            const int n5 = 4;
+            auto basis5 = QuadratureRule::quadrature_rule(n5, QuadratureRule::GaussLobatto);
+            Mesh2D mesh5 = Mesh2D::uniform_rect(10, 0.0, 10.0, 10, -6.0, 6.0);
+            const size_t np = (size_t)n5 * n5 * mesh5.n_elem();
+            const std::vector<double> x5 = mesh5.physical_coordinates(basis5);
+            std::vector<double> q(4 * np), dqdt(4 * np, 0.0);
+            const double g = 1.4;
+            for (size_t k = 0; k < np; ++k) {
+                const double x = x5[2 * k], y = x5[2 * k + 1];
+                const double r = (x - 5.0) * (x - 5.0) + y * y, b = 2.5 * std::exp(1.0 - r) / M_PI;
+                const double u = 1.0 - b * y, v = b * (x - 5.0);
+                const double rho = std::pow(1.0 - (g - 1.0) / (4.0 * g) * b * b, 1.0 / (g - 1.0));
+                const double p = std::pow(rho, g);
+                q[4 * k] = rho;
+                q[4 * k + 1] = rho * u;
+                q[4 * k + 2] = rho * v;
+                q[4 * k + 3] = p / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+            }
+            DivF2D<true> div(mesh5, basis5);
+            EdgeFluxF2D<true> interior_flux(mesh5, FaceType::INTERIOR, basis5);
+            EdgeFluxF2D<true> boundary_flux(mesh5, FaceType::BOUNDARY, basis5);
+            MassMatrix<true> m5(mesh5, basis5);
+            div.action(4, F, q.data(), dqdt.data());
+            interior_flux.action(4, F, q.data(), dqdt.data(), -1.0);
+            boundary_flux.action(4, F, q.data(), dqdt.data(), -1.0);
+            m5.inv(4, dqdt.data());
+            to_file(dir + "/cl_q.bin", q.size(), q.data());
+            to_file(dir + "/cl_dqdt.bin", dqdt.size(), dqdt.data());
+        }
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {
         std::fprintf(stderr, "dg::hip::Error %d: %s\n", e.code, e.what());

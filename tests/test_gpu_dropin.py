@@ -127,3 +127,12 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     u4 = _read(d, "advgl_u.bin")
     want = W4.mass_inv(A4.action(u4) + A4.bc_action(u4), nvar=1)
     assert rel_l2(_read(d, "advgl_dudt.bin"), want) < 1e-12
+
+    # examples/cl2d.cpp's time derivative with the flux functors as source
+    v5, e5 = meshgen.uniform_rect(10, 0.0, 10.0, 10, -6.0, 6.0)
+    W5 = O.Wave(O.Mesh(v5, e5), 4)
+    q5 = _read(d, "cl_q.bin")
+    r5 = W5.divf_euler(1.4, q5)
+    W5.edgefluxf_euler(1.4, 0, q5, r5)
+    W5.edgefluxf_euler(1.4, 1, q5, r5)
+    assert rel_l2(_read(d, "cl_dqdt.bin"), W5.mass_inv(r5, nvar=4)) < 1e-12

@@ -443,6 +443,112 @@ private:
     detail::AdvectionHandle h_;
 };
 
+// DivF2D<true> and EdgeFluxF2D<true> (DivF.hpp:207-274, EdgeFluxF.hpp:
+// 136-171) with the functors as HIP source text: the reference's lambdas
+// become FluxSource strings holding
+//   __device__ void flux(const double* x, const double* u, double* F);
+//   __device__ void numerical_flux(const double* x, const double* n,
+//                                  const double* uL, const double* uR, double* fh);
+// (and optionally exterior_state, see wdg_hip_flux_create), compiled for
+// the GPU on first use and cached per (source, n_var).  EdgeFluxF2D::action
+// covers what the reference does with a face prolongator around the flux:
+// prolongation, the numerical flux times w ds (+ on el0, - on el1), a scale
+// (cl2d.cpp uses -1) and the transpose into divF.
+struct FluxSource {
+    std::string source;
+};
+
+namespace detail {
+class FluxCache {
+public:
+    FluxCache(const Mesh2D& mesh, const QuadratureRule* basis) : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+    {
+    }
+    wdg_flux get(int n_var, const FluxSource& f) const
+    {
+        const std::string key = std::to_string(n_var) + ":" + f.source;
+        auto it = cache_.find(key);
+        if (it != cache_.end()) return it->second.get();
+        wdg_flux h = nullptr;
+        check(wdg_hip_flux_create(op_.get(), n_var, f.source.c_str(), &h));
+        std::unique_ptr<wdg_flux_s, Del> p(h);
+        return cache_.emplace(key, std::move(p)).first->second.get();
+    }
+    size_t bytes(int n_var) const { return (size_t)op_.ndof() / 3 * n_var * sizeof(double); }
+    mutable DeviceBuffer x, y;
+
+private:
+    struct Del {
+        void operator()(wdg_flux p) const { wdg_hip_flux_destroy(p); }
+    };
+    Wave2D op_;  // declared before cache_: destroyed after it
+    mutable std::unordered_map<std::string, std::unique_ptr<wdg_flux_s, Del>> cache_;
+};
+}  // namespace detail
+
+template <bool ApproxQuadrature>
+class DivF2D {
+    static_assert(ApproxQuadrature, "dg::hip: only DivF2D<true> is implemented on the GPU");
+
+public:
+    DivF2D(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr) : c_(mesh, basis)
+    {
+        (void)quad;
+    }
+    // divF += (F(x, u), grad v); host arrays (n_var, n, n, n_elem)
+    void action(int n_var, const FluxSource& f, const double* u, double* divF) const
+    {
+        const size_t bytes = c_.bytes(n_var);
+        c_.x.resize(bytes);
+        c_.y.resize(bytes);
+        c_.x.upload(u, bytes);
+        c_.y.upload(divF, bytes);
+        action_device(n_var, f, c_.x.get(), c_.y.get());
+        c_.y.download(divF, bytes);
+    }
+    void action_device(int n_var, const FluxSource& f, const void* d_u, void* d_divF, void* stream = nullptr) const
+    {
+        check(wdg_hip_divf2d(c_.get(n_var, f), d_u, d_divF, stream));
+    }
+
+private:
+    detail::FluxCache c_;
+};
+
+template <bool ApproxQuadrature>
+class EdgeFluxF2D {
+    static_assert(ApproxQuadrature, "dg::hip: only EdgeFluxF2D<true> is implemented on the GPU");
+
+public:
+    EdgeFluxF2D(const Mesh2D& mesh, FaceType face_type, const QuadratureRule* basis,
+                const QuadratureRule* quad = nullptr)
+        : c_(mesh, basis), type_(face_type)
+    {
+        (void)quad;
+    }
+    // divF += scale * lifted (+-fh w ds) on this object's faces; host arrays
+    void action(int n_var, const FluxSource& f, const double* u, double* divF, double scale = 1.0) const
+    {
+        const size_t bytes = c_.bytes(n_var);
+        c_.x.resize(bytes);
+        c_.y.resize(bytes);
+        c_.x.upload(u, bytes);
+        c_.y.upload(divF, bytes);
+        action_device(n_var, f, c_.x.get(), c_.y.get(), scale);
+        c_.y.download(divF, bytes);
+    }
+    void action_device(int n_var, const FluxSource& f, const void* d_u, void* d_divF, double scale = 1.0,
+                       void* stream = nullptr) const
+    {
+        check(wdg_hip_edgefluxf2d(c_.get(n_var, f), type_ == FaceType::INTERIOR ? WDG_INTERIOR : WDG_BOUNDARY, scale,
+                                  d_u, d_divF, stream));
+    }
+
+private:
+    detail::FluxCache c_;
+    FaceType type_;
+};
+
 // ode::ForwardEuler, RungeKutta2, RungeKutta4, SSPRK3 (ode.hpp:12-186) on
 // device arrays: the functor is called as f(d_F, t, d_u) with device
 // pointers and evaluates the right-hand side on the GPU (e.g. with the
