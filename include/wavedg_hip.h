@@ -99,7 +99,8 @@ int wdg_hip_mesh_nodes(wdg_mesh mesh, int n_colloc, double *out);
  * and lagrange_basis_deriv (lagrange_interpolation.cpp:33-50): x, w (n), D (n, n). */
 int wdg_hip_gll(int n, double *x, double *w, double *D);
 /* QuadratureRule::quadrature_rule(n, GaussLegendre) (QuadratureRule.cpp:99-148,164-184):
- * the rule the approx_quad = false operators integrate with; x, w (n), n in 1..10. */
+ * the rule the approx_quad = false operators integrate with; x, w (n), n in 1..20
+ * (tabulated up to 10 as the reference; above, Newton-refined roots of P_n). */
 int wdg_hip_gauss_legendre(int n, double *x, double *w);
 
 /* ---- wave operator ---------------------------------------------------
@@ -211,6 +212,13 @@ int wdg_hip_advection_destroy(wdg_advection adv);
  * approx_quad = true handles of either basis; the wave handle must outlive
  * the flux handle. */
 int wdg_hip_flux_create(wdg_wave2d op, int n_var, const char *source, wdg_flux *out);
+/* The same functors with approx_quad = false: DivF2D<false> (DivF.hpp:
+ * 275-392) and EdgeFluxF2D<false> (EdgeFluxF.hpp:182-250) interpolate u to
+ * Gauss-Legendre rules of nq_volume x nq_volume and nq_edge points (0 = the
+ * reference's default of 2 n_colloc; at most 20) and evaluate the functors
+ * there.  Any FP64 wave handle of the mesh and basis serves. */
+int wdg_hip_flux_create_exact(wdg_wave2d op, int n_var, const char *source, int nq_volume, int nq_edge,
+                              wdg_flux *out);
 /* compile now rather than on first use: which 0 = flux, 1 = numerical_flux;
  * a source that does not compile returns WDG_ERR_INVALID with the compiler log */
 int wdg_hip_flux_compile(wdg_flux fx, int which);

@@ -86,6 +86,9 @@ def lib():
             "orc_divf_euler": (None, [_vp, ctypes.c_double, _dp, _dp]),
             "orc_edgefluxf_euler": (None, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp]),
             "orc_divf_functor": (None, [_vp, ctypes.c_int, FLUX_FN, _vp, _dp, _dp]),
+            "orc_divf_functor_q": (None, [_vp, ctypes.c_int, ctypes.c_int, FLUX_FN, _vp, _dp, _dp]),
+            "orc_edgefluxf_functor_q": (None, [_vp, ctypes.c_int, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int,
+                                               ctypes.c_double, _dp, _dp]),
             "orc_edgefluxf_functor": (None, [_vp, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int, ctypes.c_double,
                                              _dp, _dp]),
             "orc_advection_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
@@ -130,7 +133,7 @@ def gauss_legendre(n: int):
     x = np.zeros(n)
     w = np.zeros(n)
     if lib().orc_gauss_legendre(n, _d(x), _d(w)) != 0:
-        raise ValueError(f"gauss_legendre: n={n} outside the tabulated range 1..10")
+        raise ValueError(f"gauss_legendre: n={n} outside the range 1..32")
     return x, w
 
 
@@ -272,9 +275,10 @@ class Wave:
         lib().orc_edgefluxf_euler(self.h, gamma, face_type, _d(q), _d(divF))
         return divF
 
-    def divf_functor(self, nvar: int, f, u, divF=None):
-        """DivF2D<true> with a Python functor f(x[2], u[nvar]) -> F (nvar, 2)
-        as a (2 * nvar) array [f_0..f_{nvar-1}, g_0..g_{nvar-1}]."""
+    def divf_functor(self, nvar: int, f, u, divF=None, nq: int = 0):
+        """DivF2D<true> (nq = 0) or DivF2D<false> with an nq-point rule, with a
+        Python functor f(x[2], u[nvar]) -> F (nvar, 2) as a (2 * nvar) array
+        [f_0..f_{nvar-1}, g_0..g_{nvar-1}]."""
         u = np.ascontiguousarray(u, dtype=np.float64)
         divF = np.zeros(u.size) if divF is None else divF
 
@@ -283,10 +287,14 @@ class Wave:
             uu = np.ctypeslib.as_array(up, (nvar,))
             np.ctypeslib.as_array(Fp, (2 * nvar,))[:] = f(x, uu)
 
-        lib().orc_divf_functor(self.h, nvar, FLUX_FN(cb), None, _d(u), _d(divF))
+        if nq:
+            lib().orc_divf_functor_q(self.h, nvar, nq, FLUX_FN(cb), None, _d(u), _d(divF))
+        else:
+            lib().orc_divf_functor(self.h, nvar, FLUX_FN(cb), None, _d(u), _d(divF))
         return divF
 
-    def edgefluxf_functor(self, nvar: int, fh, face_type: int, u, scale: float = 1.0, ext=None, divF=None):
+    def edgefluxf_functor(self, nvar: int, fh, face_type: int, u, scale: float = 1.0, ext=None, divF=None,
+                          nq: int = 0):
         """divF += scale * lifted EdgeFluxF2D<true> with a Python numerical
         flux fh(x, n, uL, uR) -> (nvar,); ext(x, n, u) -> exterior state."""
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -301,7 +309,11 @@ class Wave:
             np.ctypeslib.as_array(ep, (nvar,))[:] = ext(*a)
 
         ext_c = EXT_FN(cb_ext) if ext is not None else EXT_FN()
-        lib().orc_edgefluxf_functor(self.h, nvar, NFLUX_FN(cb), ext_c, None, face_type, scale, _d(u), _d(divF))
+        if nq:
+            lib().orc_edgefluxf_functor_q(self.h, nvar, nq, NFLUX_FN(cb), ext_c, None, face_type, scale, _d(u),
+                                          _d(divF))
+        else:
+            lib().orc_edgefluxf_functor(self.h, nvar, NFLUX_FN(cb), ext_c, None, face_type, scale, _d(u), _d(divF))
         return divF
 
     def mass_action(self, x, nvar: int = 3):

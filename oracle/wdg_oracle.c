@@ -107,11 +107,26 @@ static const double GL_NODES[11][10] = {
     {-0.973906528517171720077964, -0.865063366688984510732097, -0.679409568299024406234327, -0.433395394129247190799266, -0.148874338981631210884826, 0.148874338981631210884826, 0.433395394129247190799266, 0.679409568299024406234327, 0.865063366688984510732097, 0.973906528517171720077964}};
 
 /* QuadratureRule.cpp:99-162: w_i = 2 / ((1 - x_i^2) P_n'(x_i)^2). */
+/* Gauss-Legendre rules: the reference's table for n <= 10 (QuadratureRule.cpp:
+ * 103-112); above, the roots of P_n by Newton's method from the Chebyshev-like
+ * guesses cos(pi (k + 3/4) / (n + 1/2)), ascending (the reference computes
+ * them by Golub-Welsch, QuadratureRule.cpp:114-162; both converge to the
+ * roots to rounding). */
 int orc_gauss_legendre(int n, double *x, double *w)
 {
-    if (n < 1 || n > 10) return -1;
+    if (n < 1 || n > 32) return -1;
     for (int i = 0; i < n; ++i) {
-        x[i] = GL_NODES[n][i];
+        if (n <= 10) {
+            x[i] = GL_NODES[n][i];
+        } else {
+            double t = -cos(M_PI * (i + 0.75) / (n + 0.5));
+            for (int it = 0; it < 100; ++it) {
+                const double dt = legendre_p(n, t) / legendre_dp(n, t);
+                t -= dt;
+                if (fabs(dt) < 1e-16) break;
+            }
+            x[i] = t;
+        }
         const double dP = legendre_dp(n, x[i]);
         w[i] = 2.0 / (1.0 - x[i] * x[i]) / (dP * dP);
     }
@@ -1400,6 +1415,147 @@ void orc_edgefluxf_functor(const orc_wave *W, int nvar, orc_nflux_fn fh_fn, orc_
                 uf[i + n * (d + nvar * (1 + 2 * (size_t)e))] = -scale * fl;
             }
         }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < nvar; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    face_add(W, divF, nvar, j, f, d, uf[i + n * (d + nvar * (side + 2 * (size_t)e))]);
+                }
+        }
+    free(uf);
+}
+
+
+/* DivF2D<false>::action (DivF.hpp:275-392) with the functor f(x, u, F): u
+ * interpolated to the nq x nq Gauss-Legendre points, F~ there with the
+ * metric of the bilinear map, projected back with D and P. */
+void orc_divf_functor_q(const orc_wave *W, int nvar, int nq, orc_flux_fn f, void *ctx, const double *u, double *divF)
+{
+    const int n = W->n, nel = W->m->nel;
+    double xq[32], wq[32], P[32 * 16], D[32 * 16]; /* (nq, n) column-major */
+    orc_gauss_legendre(nq, xq, wq);
+    orc_lagrange_basis(P, n, W->x, nq, xq);
+    orc_lagrange_basis_deriv(D, n, W->x, nq, xq);
+    double *Pu = (double *)malloc(sizeof(double) * (size_t)n * nvar * nq);
+    double *F = (double *)malloc(sizeof(double) * 2 * (size_t)nq * nvar * nq);
+    double *Df = (double *)malloc(sizeof(double) * (size_t)nq * nvar * n);
+    double *Pg = (double *)malloc(sizeof(double) * (size_t)nq * nvar * n);
+    double Uq[8], Fq[16], X[2], J[4];
+    for (int el = 0; el < nel; ++el) {
+        const double *c = W->m->corners + 8 * (size_t)el;
+        const double *ue = u + (size_t)nvar * n * n * el;
+        for (int i = 0; i < nq; ++i)
+            for (int l = 0; l < n; ++l)
+                for (int d = 0; d < nvar; ++d) {
+                    double pu = 0.0;
+                    for (int k = 0; k < n; ++k) pu += P[i + nq * k] * ue[d + nvar * (k + n * l)];
+                    Pu[l + n * (d + nvar * i)] = pu;
+                }
+        for (int j = 0; j < nq; ++j)
+            for (int i = 0; i < nq; ++i) {
+                quad_jacobian(c, xq[i], xq[j], J);
+                quad_coords(c, xq[i], xq[j], X);
+                const double w_ij = wq[i] * wq[j];
+                const double Y_eta = J[3] * w_ij, X_eta = J[2] * w_ij;
+                const double Y_xi = J[1] * w_ij, X_xi = J[0] * w_ij;
+                for (int d = 0; d < nvar; ++d) {
+                    double uq = 0.0;
+                    for (int l = 0; l < n; ++l) uq += P[j + nq * l] * Pu[l + n * (d + nvar * i)];
+                    Uq[d] = uq;
+                }
+                for (int d = 0; d < 2 * nvar; ++d) Fq[d] = 0.0;
+                f(X, Uq, Fq, ctx);
+                for (int d = 0; d < nvar; ++d) {
+                    F[0 + 2 * (i + nq * (d + nvar * j))] = Y_eta * Fq[d] - X_eta * Fq[nvar + d];
+                    F[1 + 2 * (i + nq * (d + nvar * j))] = -Y_xi * Fq[d] + X_xi * Fq[nvar + d];
+                }
+            }
+        for (int j = 0; j < nq; ++j)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < nvar; ++d) {
+                    double df = 0.0, pg = 0.0;
+                    for (int i = 0; i < nq; ++i) {
+                        df += D[i + nq * k] * F[0 + 2 * (i + nq * (d + nvar * j))];
+                        pg += P[i + nq * k] * F[1 + 2 * (i + nq * (d + nvar * j))];
+                    }
+                    Df[j + nq * (d + nvar * k)] = df;
+                    Pg[j + nq * (d + nvar * k)] = pg;
+                }
+        for (int l = 0; l < n; ++l)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < nvar; ++d) {
+                    double dF = 0.0;
+                    for (int j = 0; j < nq; ++j)
+                        dF += P[j + nq * l] * Df[j + nq * (d + nvar * k)] + D[j + nq * l] * Pg[j + nq * (d + nvar * k)];
+                    divF[d + nvar * (k + n * (l + (size_t)n * el))] += dF;
+                }
+    }
+    free(Pu);
+    free(F);
+    free(Df);
+    free(Pg);
+}
+
+/* prol->action, EdgeFluxF2D<false>::action (EdgeFluxF.hpp:182-250) with the
+ * functor fh(x, n, uL, uR) at nq Gauss-Legendre edge points, a scale and
+ * prol->t.  Edge points run in el0's side order. */
+void orc_edgefluxf_functor_q(const orc_wave *W, int nvar, int nq, orc_nflux_fn fh_fn, orc_ext_fn ext, void *ctx,
+                             int type, double scale, const double *u, double *divF)
+{
+    const int n = W->n, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    double xq[32], wq[32], P[32 * 16];
+    orc_gauss_legendre(nq, xq, wq);
+    orc_lagrange_basis(P, n, W->x, nq, xq);
+    double *uf = (double *)calloc((size_t)2 * n * nvar * ne + 1, sizeof(double)); /* (n, nvar, 2, ne) */
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < nvar; ++d)
+                    uf[i + n * (d + nvar * (side + 2 * (size_t)e))] = face_value(W, u, nvar, j, f, d);
+            }
+        }
+    double uL[8], uR[8], fh[8], X[2], Fq[32 * 8];
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(W->m, e, type);
+        const int el0 = E->el[0], s0 = E->side[0];
+        const double *c = W->m->corners + 8 * (size_t)el0;
+        for (int i = 0; i < nq; ++i) {
+            const double xi = (s0 == 0 || s0 == 2) ? xq[i] : (s0 == 1 ? 1.0 : -1.0);
+            const double eta = (s0 == 1 || s0 == 3) ? xq[i] : (s0 == 2 ? 1.0 : -1.0);
+            quad_coords(c, xi, eta, X);
+            for (int d = 0; d < nvar; ++d) {
+                double a = 0.0, b = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    a += P[i + nq * k] * uf[k + n * (d + nvar * (0 + 2 * (size_t)e))];
+                    b += P[i + nq * k] * uf[k + n * (d + nvar * (1 + 2 * (size_t)e))];
+                }
+                uL[d] = a;
+                uR[d] = b;
+            }
+            if (type == 1) {
+                if (ext)
+                    ext(X, E->n, uL, uR, ctx);
+                else
+                    for (int d = 0; d < nvar; ++d) uR[d] = uL[d];
+            }
+            for (int d = 0; d < nvar; ++d) fh[d] = 0.0;
+            fh_fn(X, E->n, uL, uR, fh, ctx);
+            for (int d = 0; d < nvar; ++d) Fq[i + nq * d] = fh[d];
+        }
+        for (int d = 0; d < nvar; ++d)
+            for (int k = 0; k < n; ++k) {
+                double fl = 0.0;
+                for (int i = 0; i < nq; ++i) fl += Fq[i + nq * d] * P[i + nq * k] * (wq[i] * E->meas);
+                uf[k + n * (d + nvar * (0 + 2 * (size_t)e))] = scale * fl;
+                uf[k + n * (d + nvar * (1 + 2 * (size_t)e))] = -scale * fl;
+            }
     }
     for (int e = 0; e < ne; ++e)
         for (int side = 0; side < n_sides; ++side) {

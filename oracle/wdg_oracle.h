@@ -130,6 +130,10 @@ typedef void (*orc_ext_fn)(const double *x, const double *n, const double *u, do
 void orc_divf_functor(const orc_wave *W, int nvar, orc_flux_fn f, void *ctx, const double *u, double *divF);
 void orc_edgefluxf_functor(const orc_wave *W, int nvar, orc_nflux_fn fh, orc_ext_fn ext, void *ctx, int type,
                            double scale, const double *u, double *divF);
+/* the approx_quad = false forms with Gauss-Legendre rules of nq points */
+void orc_divf_functor_q(const orc_wave *W, int nvar, int nq, orc_flux_fn f, void *ctx, const double *u, double *divF);
+void orc_edgefluxf_functor_q(const orc_wave *W, int nvar, int nq, orc_nflux_fn fh, orc_ext_fn ext, void *ctx,
+                             int type, double scale, const double *u, double *divF);
 
 /* ---- reductions ------------------------------------------------------ */
 double orc_norm(long long n, const double *x);

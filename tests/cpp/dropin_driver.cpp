@@ -277,6 +277,17 @@ __device__ void numerical_flux(const double* x, const double* n, const double* q
             m5.inv(4, dqdt.data());
             to_file(dir + "/cl_q.bin", q.size(), q.data());
             to_file(dir + "/cl_dqdt.bin", dqdt.size(), dqdt.data());
+            // the same with approx_quad = false, as cl2d builds it when asked
+            DivF2D<false> div_x(mesh5, basis5);
+            EdgeFluxF2D<false> interior_x(mesh5, FaceType::INTERIOR, basis5);
+            EdgeFluxF2D<false> boundary_x(mesh5, FaceType::BOUNDARY, basis5);
+            MassMatrix<false> m5x(mesh5, basis5);
+            std::fill(dqdt.begin(), dqdt.end(), 0.0);
+            div_x.action(4, F, q.data(), dqdt.data());
+            interior_x.action(4, F, q.data(), dqdt.data(), -1.0);
+            boundary_x.action(4, F, q.data(), dqdt.data(), -1.0);
+            m5x.inv(4, dqdt.data());
+            to_file(dir + "/cl_dqdt_exact.bin", dqdt.size(), dqdt.data());
         }
         std::printf("dropin_driver ok: ndof=%lld nt=%d\n", ndof, WH.nt());
     } catch (const Error& e) {

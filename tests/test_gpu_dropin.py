@@ -136,3 +136,11 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     W5.edgefluxf_euler(1.4, 0, q5, r5)
     W5.edgefluxf_euler(1.4, 1, q5, r5)
     assert rel_l2(_read(d, "cl_dqdt.bin"), W5.mass_inv(r5, nvar=4)) < 1e-12
+    # ... and with approx_quad = false (2n-point rules, the exact mass matrix)
+    from tests.test_oracle_functor import euler_F, euler_llf
+
+    W5x = O.Wave(O.Mesh(v5, e5), 4, approx_quad=False)
+    r5x = W5x.divf_functor(4, euler_F, q5, nq=8)
+    W5x.edgefluxf_functor(4, euler_llf, 0, q5, scale=-1.0, divF=r5x, nq=8)
+    W5x.edgefluxf_functor(4, euler_llf, 1, q5, scale=-1.0, divF=r5x, nq=8)
+    assert rel_l2(_read(d, "cl_dqdt_exact.bin"), W5x.mass_inv(r5x, nvar=4)) < 1e-12

@@ -133,3 +133,66 @@ def test_errors(cuda):
         capi.Flux(capi.Wave2D(capi.Mesh(v, e), 3, None, approx_quad=False), 2, SKEW)
     with pytest.raises(capi.WdgError):
         capi.Flux(gw, 9, SKEW)
+
+
+# ---- approx_quad = false: DivF2D<false> / EdgeFluxF2D<false>
+
+
+@BASES
+@pytest.mark.parametrize("name,n", [("skewed6x5", 4), ("skew", 3), ("L", 7), ("skewed6x5", 9)])
+def test_quadrature_wave_source_reproduces_exact_wave_operator(cuda, name, n, leg):
+    torch = cuda
+    print(f"flux q wave {name} n={n} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    gx = capi.Wave2D(capi.Mesh(v, e), n, None, approx_quad=False, basis=_basis(leg))
+    fx = capi.Flux(gx, 3, fluxes.WAVE_UPWIND, approx_quad=False)
+    u = torch.tensor(random_state(gx.ndof, 5 + n), device="cuda")
+    a, b, wa, wb = (torch.zeros_like(u) for _ in range(4))
+    fx.divf(u, a)
+    fx.edgeflux(capi.INTERIOR, u, a)
+    fx.edgeflux(capi.BOUNDARY, u, b)
+    gx.action(u, wa)
+    gx.bc_action(u, wb)
+    torch.cuda.synchronize()
+    assert rel_l2(a.cpu().numpy(), wa.cpu().numpy()) < TOL
+    assert rel_l2(b.cpu().numpy(), wb.cpu().numpy()) < TOL
+
+
+@BASES
+@pytest.mark.parametrize("name,n,quad", [("skewed6x5", 3, (0, 0)), ("skew", 5, (7, 9)), ("L", 8, (0, 0))])
+def test_quadrature_euler_and_non_odd_against_oracle(cuda, name, n, quad, leg):
+    torch = cuda
+    print(f"flux q euler {name} n={n} quad={quad} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    om = O.Mesh(v, e)
+    ow = O.Wave(om, n, legendre=leg)
+    gw = capi.Wave2D(capi.Mesh(v, e), n, None, basis=_basis(leg))
+    nqv, nqe = quad[0] or 2 * n, quad[1] or 2 * n
+    fe = capi.Flux(gw, 4, fluxes.EULER_LLF, approx_quad=False, quad=quad)
+    fs = capi.Flux(gw, 2, SKEW, approx_quad=False, quad=quad)
+    q = _state(om, n, n, leg)
+    dq = torch.tensor(q, device="cuda")
+    o = [torch.zeros_like(dq) for _ in range(3)]
+    fe.divf(dq, o[0])
+    fe.edgeflux(capi.INTERIOR, dq, o[1], scale=-1.0)
+    fe.edgeflux(capi.BOUNDARY, dq, o[2], scale=-1.0)
+    u = random_state(2 * gw.ndof // 3, 60 + n)
+    du = torch.tensor(u, device="cuda")
+    si, sb = torch.zeros_like(du), torch.zeros_like(du)
+    fs.edgeflux(capi.INTERIOR, du, si, scale=0.5)
+    fs.edgeflux(capi.BOUNDARY, du, sb, scale=0.5)
+    torch.cuda.synchronize()
+    assert rel_l2(o[0].cpu().numpy(), ow.divf_functor(4, euler_F, q, nq=nqv)) < TOL
+    assert rel_l2(o[1].cpu().numpy(), ow.edgefluxf_functor(4, euler_llf, 0, q, scale=-1.0, nq=nqe)) < TOL
+    assert rel_l2(o[2].cpu().numpy(), ow.edgefluxf_functor(4, euler_llf, 1, q, scale=-1.0, nq=nqe)) < TOL
+    assert rel_l2(si.cpu().numpy(), ow.edgefluxf_functor(2, skew_py, 0, u, scale=0.5, nq=nqe)) < TOL
+    assert rel_l2(sb.cpu().numpy(), ow.edgefluxf_functor(2, skew_py, 1, u, scale=0.5, nq=nqe)) < TOL
+
+
+def test_quadrature_rule_limits(cuda):
+    v, e = mesh_case("rect5x4")
+    gw = capi.Wave2D(capi.Mesh(v, e), 9, None)
+    with pytest.raises(capi.WdgError):
+        capi.Flux(gw, 8, fluxes.EULER_LLF, approx_quad=False, quad=(20, 0))  # local memory
+    with pytest.raises(capi.WdgError):
+        capi.Flux(gw, 4, fluxes.EULER_LLF, approx_quad=False, quad=(0, 21))

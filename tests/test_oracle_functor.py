@@ -94,3 +94,38 @@ def test_wave_upwind_functor_is_wave_face_part(name, n, leg):
     assert rel_l2(got_int, want_int) < 1e-13
     got_bc = W.edgefluxf_functor(3, wave_upwind, 1, u, ext=reflect)
     assert rel_l2(got_bc, W.bc_action(u)) < 1e-13
+
+
+def wave_F(x, w):
+    return np.array([w[1], w[0], 0.0, w[2], 0.0, w[0]])
+
+
+@BASES
+@pytest.mark.parametrize("name,n", [("skewed6x5", 4), ("skew", 3), ("L", 5)])
+def test_quadrature_forms_reproduce_exact_wave_operator(name, n, leg):
+    """DivF2D<false> / EdgeFluxF2D<false> with the wave fluxes and 2n-point
+    rules equal the exact-quadrature wave operator (n volume and n + 1 edge
+    points): both rules integrate these polynomial integrands exactly on
+    bilinear elements (volume integrands of degree <= 2n - 1 per direction,
+    edge integrands of degree 2n - 2)."""
+    v, e = mesh_case(name)
+    mesh = O.Mesh(v, e)
+    Wx = O.Wave(mesh, n, None, approx_quad=False, legendre=leg)  # reflecting walls
+    u = random_state(Wx.ndof, 9)
+    vol = Wx.divf_functor(3, wave_F, u, nq=2 * n)
+    assert rel_l2(vol, Wx.div(u)) < 1e-12
+    face = Wx.edgefluxf_functor(3, wave_upwind, 0, u, nq=2 * n)
+    assert rel_l2(vol + face, Wx.action(u)) < 1e-12
+    bc = Wx.edgefluxf_functor(3, wave_upwind, 1, u, ext=reflect, nq=2 * n)
+    assert rel_l2(bc, Wx.bc_action(u)) < 1e-12
+
+
+def test_quadrature_euler_free_stream():
+    v, e = mesh_case("skewed6x5")
+    W = O.Wave(O.Mesh(v, e), 4)
+    q = np.tile([1.2, 0.36, -0.48, 2.5], W.ndof // 3)
+    r = W.divf_functor(4, euler_F, q, nq=8)
+    scale = np.abs(r).max()
+    W.edgefluxf_functor(4, euler_llf, 0, q, scale=-1.0, divF=r, nq=8)
+    W.edgefluxf_functor(4, euler_llf, 1, q, scale=-1.0, divF=r, nq=8)
+    assert np.abs(r).max() <= 1e-13 * scale

@@ -55,6 +55,7 @@ _SIGS = {
     "wdg_hip_wave2d_basis": [_vp, _ip, _dp, _dp],
     "wdg_hip_flux_create": [_vp, _i, ctypes.c_char_p, ctypes.POINTER(_vp)],
     "wdg_hip_flux_compile": [_vp, _i],
+    "wdg_hip_flux_create_exact": [_vp, _i, ctypes.c_char_p, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_flux_check": [ctypes.c_char_p, _i, _i, _i, _i],
     "wdg_hip_divf2d": [_vp, _vp, _vp, _vp],
     "wdg_hip_edgefluxf2d": [_vp, _i, _d, _vp, _vp, _vp],
@@ -444,11 +445,16 @@ class Flux:
     (flux, numerical_flux and optionally exterior_state in the reference's
     signatures), compiled for the GPU at run time."""
 
-    def __init__(self, wave: Wave2D, n_var: int, source: str):
+    def __init__(self, wave: Wave2D, n_var: int, source: str, approx_quad: bool = True, quad=(0, 0)):
+        """approx_quad=False: DivF2D<false> / EdgeFluxF2D<false> with
+        Gauss-Legendre rules quad = (volume, edge) points, 0 = 2 n."""
         self.wave = wave  # the handle must outlive this object
         self.n_var = n_var
         h = _vp()
-        _call("wdg_hip_flux_create", wave.h, n_var, source.encode(), ctypes.byref(h))
+        if approx_quad:
+            _call("wdg_hip_flux_create", wave.h, n_var, source.encode(), ctypes.byref(h))
+        else:
+            _call("wdg_hip_flux_create_exact", wave.h, n_var, source.encode(), *quad, ctypes.byref(h))
         self.h = h
 
     def __del__(self):

@@ -262,16 +262,17 @@ __device__ __forceinline__ void detj_affine(const double* g, double& d0, double&
 // M^{-1} by fast diagonalisation: (V(x)V) diag(1 / (d0 + d1 lam_a + d2 lam_b)) (V(x)V)'.
 template <int N>
 __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const double* r, const ExactMassTables m,
-                                                          int kid)
+                                                          int kid, int nv)
 {
     constexpr int C = N * N;
-    __shared__ double s_x[3 * C], s_t[3 * C], s_t2[3 * C], s_V[C], s_W[C], s_lam[N];
+    constexpr int MV = DivFArgs::kMaxVar;
+    __shared__ double s_x[MV * C], s_t[MV * C], s_t2[MV * C], s_V[C], s_W[C], s_lam[N];
     const int e = blockIdx.x, tid = threadIdx.x;
-    const size_t base = (size_t)e * 3 * C;
+    const size_t base = (size_t)e * nv * C;
     const int mode = kid >= 0 ? kModeOf[kid] : 0;
     const bool force = kid >= 0 && (mode & M_FORCE);
     const bool mult = kid == -1;
-    for (int k = tid; k < 3 * C; k += kBlk) {
+    for (int k = tid; k < nv * C; k += kBlk) {
         double v = kid < 0 ? static_cast<const double*>(a.u_in)[base + k] : r[base + k];
         if (force) v += static_cast<const double*>(a.force)[base + k] * a.fs;
         s_x[k] = v;
@@ -286,11 +287,11 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
     __syncthreads();
     if (mult) {
         // (d0 A x + d1 B x, d2 A x) along xi, then A and B along eta
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, i = (k / nv) % N, j = k / (nv * N);
             double xa = 0.0, xb = 0.0;
             for (int l = 0; l < N; ++l) {
-                const double x = s_x[d + 3 * (l + N * j)];
+                const double x = s_x[d + nv * (l + N * j)];
                 xa += s_V[i + N * l] * x;
                 xb += s_W[i + N * l] * x;
             }
@@ -298,49 +299,49 @@ __global__ __launch_bounds__(kBlk) void exact_mass_kernel(const KArgs a, const d
             s_t2[k] = d2 * xa;
         }
         __syncthreads();
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, i = (k / nv) % N, j = k / (nv * N);
             double y = 0.0;
             for (int l = 0; l < N; ++l)
-                y += s_V[j + N * l] * s_t[d + 3 * (i + N * l)] + s_W[j + N * l] * s_t2[d + 3 * (i + N * l)];
+                y += s_V[j + N * l] * s_t[d + nv * (i + N * l)] + s_W[j + N * l] * s_t2[d + nv * (i + N * l)];
             s_x[k] = y;  // s_x is not read in this pass
         }
     } else {
         // t(d, a, j) = sum_i V(i, a) x(d, i, j)
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, aa = (k / 3) % N, j = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, aa = (k / nv) % N, j = k / (nv * N);
             double t = 0.0;
-            for (int i = 0; i < N; ++i) t += s_V[i + N * aa] * s_x[d + 3 * (i + N * j)];
+            for (int i = 0; i < N; ++i) t += s_V[i + N * aa] * s_x[d + nv * (i + N * j)];
             s_t[k] = t;
         }
         __syncthreads();
         // x(d, a, b) = sum_j V(j, b) t(d, a, j) / (d0 + d1 lam_a + d2 lam_b)
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, aa = (k / 3) % N, b = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, aa = (k / nv) % N, b = k / (nv * N);
             double t = 0.0;
-            for (int j = 0; j < N; ++j) t += s_V[j + N * b] * s_t[d + 3 * (aa + N * j)];
+            for (int j = 0; j < N; ++j) t += s_V[j + N * b] * s_t[d + nv * (aa + N * j)];
             s_x[k] = t / (d0 + d1 * s_lam[aa] + d2 * s_lam[b]);
         }
         __syncthreads();
         // t(d, i, b) = sum_a V(i, a) x(d, a, b)
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = (k / 3) % N, b = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, i = (k / nv) % N, b = k / (nv * N);
             double t = 0.0;
-            for (int aa = 0; aa < N; ++aa) t += s_V[i + N * aa] * s_x[d + 3 * (aa + N * b)];
+            for (int aa = 0; aa < N; ++aa) t += s_V[i + N * aa] * s_x[d + nv * (aa + N * b)];
             s_t[k] = t;
         }
         __syncthreads();
         // x(d, i, j) = sum_b V(j, b) t(d, i, b)
-        for (int k = tid; k < 3 * C; k += kBlk) {
-            const int d = k % 3, i = (k / 3) % N, j = k / (3 * N);
+        for (int k = tid; k < nv * C; k += kBlk) {
+            const int d = k % nv, i = (k / nv) % N, j = k / (nv * N);
             double t = 0.0;
-            for (int b = 0; b < N; ++b) t += s_V[j + N * b] * s_t[d + 3 * (i + N * b)];
+            for (int b = 0; b < N; ++b) t += s_V[j + N * b] * s_t[d + nv * (i + N * b)];
             s_x[k] = t;
         }
     }
     __syncthreads();
     const int out = mode & M_OUTMASK;
-    for (int k = tid; k < 3 * C; k += kBlk) {
+    for (int k = tid; k < nv * C; k += kBlk) {
         const size_t g = base + k;
         const double rhs = s_x[k];
         if (kid < 0 || out == M_OUT_RHS) {
@@ -736,7 +737,7 @@ void launch_all(int what, const KArgs& a, const ExactTables& t, const ExactMassT
     switch (what) {
     case 0: hipLaunchKernelGGL(exact_volume_kernel<N>, grid, dim3(kBlk), 0, s, a, t, p0); break;
     case 1: hipLaunchKernelGGL(exact_face_kernel<N>, grid, dim3(64), 0, s, a, t, p0, p1); break;
-    default: hipLaunchKernelGGL(exact_mass_kernel<N>, grid, dim3(kBlk), 0, s, a, r, m, p0); break;
+    default: hipLaunchKernelGGL(exact_mass_kernel<N>, grid, dim3(kBlk), 0, s, a, r, m, p0, p1); break;
     }
 }
 
@@ -769,9 +770,10 @@ void launch_exact_face(int n, const KArgs& a, const ExactTables& t, int interior
     dispatch(n, 1, a, t, ExactMassTables{}, nullptr, interior, boundary, s);
 }
 
-void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s)
+void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s,
+                       int nv)
 {
-    dispatch(n, 2, a, ExactTables{}, m, r, kid, 0, s);
+    dispatch(n, 2, a, ExactTables{}, m, r, kid, kid < 0 ? nv : 3, s);
 }
 
 void launch_exact_stage(int n, int kid, const KArgs& a, const double* tab, hipStream_t s)

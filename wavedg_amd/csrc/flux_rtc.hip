@@ -205,6 +205,190 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
     for (int t = tid; t < nloc * NV * NN; t += 64) yb[t] += s_out[t];
 }
 #endif
+
+#ifdef WDG_WANT_DIVF_Q
+// DivF2D<false>::action (DivF.hpp:275-392): u interpolated to the NQ x NQ
+// Gauss-Legendre points, the contravariant fluxes of flux(x, u) there with
+// the metric of the bilinear map, then divF(d, k, l) += sum_j P(j, l) Df(j, d, k)
+// + D(j, l) Pg(j, d, k) with Df(j, d, k) = sum_i D(i, k) F~(i, d, j) and
+// Pg(j, d, k) = sum_i P(i, k) G~(i, d, j).  One element per 256-lane
+// workgroup; tab = xq[NQ], wq[NQ], P[NQ * N], D[NQ * N] (P(i, k) at i + NQ k).
+extern "C" __global__ void __launch_bounds__(256) wdg_divf_q_rtc(const double* __restrict__ u,
+                                                                  double* __restrict__ out,
+                                                                  const double* __restrict__ corners,
+                                                                  const double* __restrict__ tab, int nel)
+{
+    using namespace wdg_rtc;
+    constexpr int NQ = WDG_NQ;
+    constexpr int SPU = N * NV * NQ, SDP = 2 * NQ * NV * N;
+    __shared__ double sF[2 * NQ * NV * NQ];
+    __shared__ double sW[SPU > SDP ? SPU : SDP];  // Pu, then Df and Pg
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const double* xq = tab;
+    const double* wq = tab + NQ;
+    const double* P = tab + 2 * NQ;
+    const double* D = tab + 2 * NQ + NQ * N;
+    const double* ue = u + (size_t)NV * NN * e;
+    // Pu(l, d, i) = sum_k P(i, k) u(d, k, l)
+    for (int t = tid; t < SPU; t += 256) {
+        const int l = t % N, d = (t / N) % NV, i = t / (N * NV);
+        double pu = 0.0;
+        for (int k = 0; k < N; ++k) pu += P[i + NQ * k] * ue[d + NV * (k + N * l)];
+        sW[t] = pu;
+    }
+    __syncthreads();
+    for (int t = tid; t < NQ * NQ; t += 256) {
+        const int i = t % NQ, j = t / NQ;
+        double X[2], J[4];
+        corner_map(corners + 8 * (size_t)e, xq[i], xq[j], X, J);
+        const double wij = wq[i] * wq[j];
+        const double Ye = J[3] * wij, Xe = J[2] * wij, Yx = J[1] * wij, Xx = J[0] * wij;
+        double uq[NV], Fq[2 * NV];
+        for (int d = 0; d < NV; ++d) {
+            double s = 0.0;
+            for (int l = 0; l < N; ++l) s += P[j + NQ * l] * sW[l + N * (d + NV * i)];
+            uq[d] = s;
+        }
+        for (int d = 0; d < 2 * NV; ++d) Fq[d] = 0.0;
+        flux(X, uq, Fq);
+        for (int d = 0; d < NV; ++d) {
+            sF[0 + 2 * (i + NQ * (d + NV * j))] = Ye * Fq[d] - Xe * Fq[NV + d];
+            sF[1 + 2 * (i + NQ * (d + NV * j))] = -Yx * Fq[d] + Xx * Fq[NV + d];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < NQ * NV * N; t += 256) {
+        const int j = t % NQ, d = (t / NQ) % NV, k = t / (NQ * NV);
+        double df = 0.0, pg = 0.0;
+        for (int i = 0; i < NQ; ++i) {
+            df += D[i + NQ * k] * sF[0 + 2 * (i + NQ * (d + NV * j))];
+            pg += P[i + NQ * k] * sF[1 + 2 * (i + NQ * (d + NV * j))];
+        }
+        sW[2 * t] = df;
+        sW[2 * t + 1] = pg;
+    }
+    __syncthreads();
+    for (int t = tid; t < NN * NV; t += 256) {
+        const int d = t % NV, k = (t / NV) % N, l = t / (NV * N);
+        double dF = 0.0;
+        for (int j = 0; j < NQ; ++j) {
+            const int b = 2 * (j + NQ * (d + NV * k));
+            dF += P[j + NQ * l] * sW[b] + D[j + NQ * l] * sW[b + 1];
+        }
+        out[(size_t)NV * NN * e + t] += dF;
+    }
+}
+#endif
+
+#ifdef WDG_WANT_EDGE_Q
+// EdgeFluxF2D<false>::action (EdgeFluxF.hpp:182-250) between the face
+// prolongation and its transpose, element by element: each side's traces at
+// the side nodes (own order) are interpolated to the NQ Gauss-Legendre edge
+// points, fh = numerical_flux(x, n, uL, uR) with el0's normal and uL = el0's
+// trace, and sum_q (+-scale) fh_q w_q ds P(q, k) is lifted into side node k.
+// tab = xq[NQ], wq[NQ], P[NQ * N].  E elements per 64-lane workgroup.
+extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double* __restrict__ u,
+                                                                     double* __restrict__ out,
+                                                                     const double* __restrict__ corners,
+                                                                     const int* __restrict__ nbr,
+                                                                     const int* __restrict__ info,
+                                                                     const double* __restrict__ basis,
+                                                                     const double* __restrict__ tab, int nel,
+                                                                     int interior, int boundary, double scale)
+{
+    using namespace wdg_rtc;
+    constexpr int NQ = WDG_NQ, W = NQ > N ? NQ : N, E = 64 / W;
+    __shared__ double s_out[E * NV * NN];
+    __shared__ double sA[E * N * NV], sB[E * N * NV], sC[E * NQ * NV];
+    __shared__ int s_ok[E];
+    const int tid = threadIdx.x, e0 = blockIdx.x * E;
+    const int nloc = nel - e0 < E ? nel - e0 : E;
+    const double* xq = tab;
+    const double* wq = tab + NQ;
+    const double* Pq = tab + 2 * NQ;
+    const double* Pf = basis + 2 * N + NN;  // Gauss-Legendre face interpolation
+    for (int t = tid; t < E * NV * NN; t += 64) s_out[t] = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        __syncthreads();  // the previous side's sA, sB, sC are consumed
+        // traces at the side nodes in this element's order
+        for (int t = tid; t < E * N; t += 64) {
+            const int el = t / N, k = t - el * N, e = e0 + el;
+            if (el >= nloc) continue;
+            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+            const bool on = nb >= 0 ? interior : boundary;
+            if (k == 0) s_ok[el] = on;
+            if (!on) continue;
+            side_trace(u, e, s, k, Pf, sA + (el * N + k) * NV);
+            if (nb >= 0) side_trace(u, nb, inf & 3, ((inf >> 2) & 1) ? (N - 1 - k) : k, Pf, sB + (el * N + k) * NV);
+        }
+        __syncthreads();
+        // numerical flux at the edge points
+        for (int t = tid; t < E * NQ; t += 64) {
+            const int el = t / NQ, q = t - el * NQ, e = e0 + el;
+            if (el >= nloc || !s_ok[el]) continue;
+            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+            const double* c = corners + 8 * (size_t)e;
+            const int a0 = s == 0 ? 0 : s == 1 ? 1 : s == 2 ? 3 : 0;
+            const int a1 = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? 2 : 3;
+            const double dx = c[2 * a1] - c[2 * a0], dy = c[2 * a1 + 1] - c[2 * a0 + 1];
+            const double len = sqrt(dx * dx + dy * dy), sg = s >= 2 ? -1.0 : 1.0;
+            double nrm[2] = {sg * dy / len, -sg * dx / len};
+            const double ds = 0.5 * len;
+            const double xi = s == 0 || s == 2 ? xq[q] : (s == 1 ? 1.0 : -1.0);
+            const double eta = s == 1 || s == 3 ? xq[q] : (s == 2 ? 1.0 : -1.0);
+            double X[2], J[4];
+            corner_map(c, xi, eta, X, J);
+            double uA[NV], uB[NV], fh[NV];
+            for (int d = 0; d < NV; ++d) {
+                double a = 0.0, b = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    a += Pq[q + NQ * k] * sA[(el * N + k) * NV + d];
+                    b += Pq[q + NQ * k] * sB[(el * N + k) * NV + d];
+                }
+                uA[d] = a;
+                uB[d] = b;
+            }
+            if (nb < 0) {
+#ifdef WDG_EXTERIOR_STATE
+                exterior_state(X, nrm, uA, uB);
+#else
+                for (int d = 0; d < NV; ++d) uB[d] = uA[d];
+#endif
+            }
+            const bool first = (inf >> 3) & 1;
+            for (int d = 0; d < NV; ++d) fh[d] = 0.0;
+            if (first) {
+                numerical_flux(X, nrm, uA, uB, fh);
+            } else {
+                nrm[0] = -nrm[0];
+                nrm[1] = -nrm[1];
+                numerical_flux(X, nrm, uB, uA, fh);
+            }
+            const double sgn = first ? scale : -scale;
+            for (int d = 0; d < NV; ++d) sC[(el * NQ + q) * NV + d] = sgn * (fh[d] * (wq[q] * ds));
+        }
+        __syncthreads();
+        // projection onto the side nodes and lift
+        for (int t = tid; t < E * N; t += 64) {
+            const int el = t / N, k = t - el * N;
+            if (el >= nloc || !s_ok[el]) continue;
+            const int node = side_node(s, k), st = inward_stride(s);
+            for (int d = 0; d < NV; ++d) {
+                double fl = 0.0;
+                for (int q = 0; q < NQ; ++q) fl += sC[(el * NQ + q) * NV + d] * Pq[q + NQ * k];
+                if (WDG_LEG) {
+                    for (int m = 0; m < N; ++m) s_out[(el * NN + node + m * st) * NV + d] += fl * Pf[m];
+                } else {
+                    s_out[(el * NN + node) * NV + d] += fl;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double* yb = out + (size_t)e0 * NV * NN;
+    for (int t = tid; t < nloc * NV * NN; t += 64) yb[t] += s_out[t];
+}
+#endif
 )WDGRTC";
 
 struct Compiled {
@@ -216,8 +400,15 @@ struct Compiled {
 
 // Compiles the caller's source with the library's kernel for one order,
 // variable count and basis into a gfx950 code object (no device needed).
-bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::vector<char>& code, std::string& err)
+bool rtc_build(const std::string& user, int n, int nv, int leg, int kind, int nq, std::vector<char>& code,
+               std::string& err)
 {
+    static const char* const kWant[4] = {"-DWDG_WANT_DIVF", "-DWDG_WANT_EDGE", "-DWDG_WANT_DIVF_Q",
+                                         "-DWDG_WANT_EDGE_Q"};
+    if (kind < 0 || kind > 3) {
+        err = "bad kernel kind";
+        return false;
+    }
     const std::string src = user + "\n" + kRtcKernels;
     hiprtcProgram prog = nullptr;
     if (hiprtcCreateProgram(&prog, src.c_str(), "wdg_flux.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
@@ -225,12 +416,11 @@ bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::
         return false;
     }
     const std::string dn = "-DWDG_N=" + std::to_string(n), dv = "-DWDG_NVAR=" + std::to_string(nv),
-                      dl = "-DWDG_LEG=" + std::to_string(leg ? 1 : 0);
+                      dl = "-DWDG_LEG=" + std::to_string(leg ? 1 : 0), dq = "-DWDG_NQ=" + std::to_string(nq);
     // the same code-generation options as the library's own kernels; only
     // the kernel asked for is compiled, so a source may define one functor
     const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops", dn.c_str(), dv.c_str(), dl.c_str(),
-                          divf ? "-DWDG_WANT_DIVF" : "-DWDG_WANT_EDGE"};
+                          "-packed-fp32-ops", dn.c_str(), dv.c_str(), dl.c_str(), dq.c_str(), kWant[kind]};
     const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
     if (rc != HIPRTC_SUCCESS) {
         size_t ls = 0;
@@ -252,11 +442,12 @@ bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::
 namespace {
 
 // rtc_build, then load the kernel on the current device.
-bool rtc_compile(const std::string& user, int n, int nv, int leg, bool divf, Compiled& out, std::string& err)
+bool rtc_compile(const std::string& user, int n, int nv, int leg, int kind, int nq, Compiled& out, std::string& err)
 {
-    const char* kernel = divf ? "wdg_divf_rtc" : "wdg_edgeflux_rtc";
+    static const char* const kName[4] = {"wdg_divf_rtc", "wdg_edgeflux_rtc", "wdg_divf_q_rtc", "wdg_edgeflux_q_rtc"};
+    const char* kernel = kName[kind];
     std::vector<char> code;
-    if (!rtc_build(user, n, nv, leg, divf, code, err)) return false;
+    if (!rtc_build(user, n, nv, leg, kind, nq, code, err)) return false;
     if (hipModuleLoadData(&out.mod, code.data()) != hipSuccess) {
         err = "hipModuleLoadData failed for the compiled flux kernels";
         return false;
@@ -275,12 +466,17 @@ bool rtc_compile(const std::string& user, int n, int nv, int leg, bool divf, Com
 struct RtcFlux {
     std::string source;
     int n = 0, nv = 0, leg = 0;
+    int nqv = 0, nqe = 0;                 // > 0: the quadrature (approx_quad = false) forms
+    double* d_tabv = nullptr;             // xq, wq, P, D of the volume rule
+    double* d_tabe = nullptr;             // xq, wq, P of the edge rule
     Compiled divf, edge;
     std::mutex mu;
     ~RtcFlux()
     {
         if (divf.mod) (void)hipModuleUnload(divf.mod);
         if (edge.mod) (void)hipModuleUnload(edge.mod);
+        if (d_tabv) (void)hipFree(d_tabv);
+        if (d_tabe) (void)hipFree(d_tabe);
     }
 };
 
@@ -294,6 +490,22 @@ RtcFlux* rtc_flux_create(const char* source, int n, int nv, int leg)
     return f;
 }
 
+bool rtc_flux_set_rules(RtcFlux* f, int nqv, const std::vector<double>& tabv, int nqe, const std::vector<double>& tabe,
+                        std::string& err)
+{
+    auto upload = [&](double*& d, const std::vector<double>& h) {
+        if (hipMalloc(&d, sizeof(double) * h.size()) != hipSuccess) return false;
+        return hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!upload(f->d_tabv, tabv) || !upload(f->d_tabe, tabe)) {
+        err = "cannot upload the quadrature tables";
+        return false;
+    }
+    f->nqv = nqv;
+    f->nqe = nqe;
+    return true;
+}
+
 void rtc_flux_destroy(RtcFlux* f) { delete f; }
 
 bool rtc_flux_prepare(RtcFlux* f, int which, std::string& err)
@@ -301,7 +513,9 @@ bool rtc_flux_prepare(RtcFlux* f, int which, std::string& err)
     std::lock_guard<std::mutex> lk(f->mu);
     Compiled& c = which == 0 ? f->divf : f->edge;
     if (c.fn) return true;
-    return rtc_compile(f->source, f->n, f->nv, f->leg, which == 0, c, err);
+    const bool q = f->nqv > 0;
+    const int kind = (which == 0 ? 0 : 1) + (q ? 2 : 0);
+    return rtc_compile(f->source, f->n, f->nv, f->leg, kind, which == 0 ? f->nqv : f->nqe, c, err);
 }
 
 bool rtc_divf(RtcFlux* f, const double* u, double* out, const double* corners, const double* basis, int nel,
@@ -309,9 +523,12 @@ bool rtc_divf(RtcFlux* f, const double* u, double* out, const double* corners, c
 {
     if (!rtc_flux_prepare(f, 0, err)) return false;
     if (nel <= 0) return true;
-    const int NN = f->n * f->n, E = 256 / NN;
-    void* params[] = {&u, &out, &corners, &basis, &nel};
-    if (hipModuleLaunchKernel(f->divf.fn, (nel + E - 1) / E, 1, 1, E * NN, 1, 1, 0, s, params, nullptr) != hipSuccess) {
+    const int NN = f->n * f->n, E = f->nqv > 0 ? 1 : 256 / NN;
+    const double* tab = f->nqv > 0 ? f->d_tabv : basis;
+    void* params[] = {&u, &out, &corners, &tab, &nel};
+    const unsigned threads = f->nqv > 0 ? 256 : E * NN;
+    if (hipModuleLaunchKernel(f->divf.fn, (nel + E - 1) / E, 1, 1, threads, 1, 1, 0, s, params, nullptr) !=
+        hipSuccess) {
         err = "flux divergence kernel launch failed";
         return false;
     }
@@ -324,6 +541,17 @@ bool rtc_edgeflux(RtcFlux* f, const double* u, double* out, const double* corner
 {
     if (!rtc_flux_prepare(f, 1, err)) return false;
     if (nel <= 0) return true;
+    if (f->nqe > 0) {
+        const int E = 64 / (f->nqe > f->n ? f->nqe : f->n);
+        const double* tab = f->d_tabe;
+        void* params[] = {&u, &out, &corners, &nbr, &info, &basis, &tab, &nel, &interior, &boundary, &scale};
+        if (hipModuleLaunchKernel(f->edge.fn, (nel + E - 1) / E, 1, 1, 64, 1, 1, 0, s, params, nullptr) !=
+            hipSuccess) {
+            err = "edge flux kernel launch failed";
+            return false;
+        }
+        return true;
+    }
     const int E = 64 / f->n;
     void* params[] = {&u, &out, &corners, &nbr, &info, &basis, &nel, &interior, &boundary, &scale};
     if (hipModuleLaunchKernel(f->edge.fn, (nel + E - 1) / E, 1, 1, 64, 1, 1, 0, s, params, nullptr) != hipSuccess) {

@@ -89,8 +89,23 @@ bool gll_rule(int n, std::vector<double>& x, std::vector<double>& w)
 
 bool gauss_legendre_rule(int n, std::vector<double>& x, std::vector<double>& w)
 {
-    if (n < 1 || n > 10) return false;
-    x.assign(kGl[n], kGl[n] + n);
+    if (n < 1 || n > 20) return false;
+    if (n <= 10) {
+        x.assign(kGl[n], kGl[n] + n);
+    } else {
+        // roots of P_n by Newton's method, ascending
+        x.resize(n);
+        for (int i = 0; i < n; ++i) {
+            double t = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            for (int it = 0; it < 100; ++it) {
+                const double dP = n * (t * legendre(n, t) - legendre(n - 1, t)) / (t * t - 1.0);
+                const double dt = legendre(n, t) / dP;
+                t -= dt;
+                if (std::fabs(dt) < 1e-16) break;
+            }
+            x[i] = t;
+        }
+    }
     w.resize(n);
     for (int i = 0; i < n; ++i) {
         // P_n'(x) = n (x P_n - P_{n-1}) / (x^2 - 1)

@@ -759,7 +759,7 @@ int wdg_hip_gauss_legendre(int n, double* x, double* w)
 {
     return guarded([&] {
         std::vector<double> xv, wv;
-        if (!wdg::gauss_legendre_rule(n, xv, wv)) fail(WDG_ERR_INVALID, "n out of range 1..10");
+        if (!wdg::gauss_legendre_rule(n, xv, wv)) fail(WDG_ERR_INVALID, "n out of range 1..20");
         if (x) std::memcpy(x, xv.data(), sizeof(double) * n);
         if (w) std::memcpy(w, wv.data(), sizeof(double) * n);
     });
@@ -1040,11 +1040,11 @@ static int mass_launch(wdg_wave2d op, const void* x, void* y, int inv, void* str
         const long long nn = op->ndof / 3;
         const hipStream_t s = as_stream(stream);
         if (op->exact) {
-            if (nv != 3) fail(WDG_ERR_INVALID, "the exact mass matrix is implemented for 3 variables");
+            if (nv > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "the exact mass matrix takes n_var <= 8");
             KArgs a = op->base_args();
             a.u_in = x;
             a.u_out = y;
-            wdg::launch_exact_mass(op->n, a, nullptr, op->mass_tables(), inv ? -2 : -1, s);
+            wdg::launch_exact_mass(op->n, a, nullptr, op->mass_tables(), inv ? -2 : -1, s, nv);
             hip_check(hipGetLastError(), "mass kernel launch");
             return;
         }
@@ -1514,7 +1514,7 @@ int wdg_hip_mesh_gl_points(wdg_mesh mesh, int nq, double* out)
     return guarded([&] {
         if (!mesh || !out) fail(WDG_ERR_INVALID, "null argument");
         std::vector<double> x, w;
-        if (!wdg::gauss_legendre_rule(nq, x, w)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
+        if (!wdg::gauss_legendre_rule(nq, x, w)) fail(WDG_ERR_INVALID, "nq out of range 1..20");
         mesh->mesh.nodes(nq, x.data(), out);
     });
 }
@@ -1526,6 +1526,7 @@ int wdg_hip_linear_functional(wdg_wave2d op, int n_var, int nq, const double* d_
         if (!d_fq || !d_F) fail(WDG_ERR_INVALID, "null device pointer");
         if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
         if (nq <= 0) nq = op->n;  // the reference's default rule: Gauss-Legendre of n_colloc points
+        if (nq > 10) fail(WDG_ERR_INVALID, "nq out of range 1..10");  // the kernel's tables hold 10 points
         std::vector<double> xq, wq;
         if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
         std::vector<double> tab(2 * (size_t)nq + (size_t)nq * op->n);
@@ -1766,8 +1767,49 @@ int wdg_hip_flux_check(const char* source, int n_colloc, int n_var, int basis_ty
         if (which != 0 && which != 1) fail(WDG_ERR_INVALID, "which must be 0 (flux) or 1 (numerical_flux)");
         std::vector<char> code;
         std::string err;
-        if (!wdg::rtc_build(source, n_colloc, n_var, basis_type == WDG_GAUSS_LEGENDRE, which == 0, code, err))
-            fail(WDG_ERR_INVALID, err);
+        for (int nq : {0, 2 * n_colloc}) {  // collocated and quadrature forms
+            const int kind = (which == 0 ? 0 : 1) + (nq ? 2 : 0);
+            if (!wdg::rtc_build(source, n_colloc, n_var, basis_type == WDG_GAUSS_LEGENDRE, kind, nq ? nq : n_colloc,
+                                code, err))
+                fail(WDG_ERR_INVALID, err);
+        }
+    });
+}
+
+int wdg_hip_flux_create_exact(wdg_wave2d op, int n_var, const char* source, int nq_volume, int nq_edge,
+                              wdg_flux* out)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!source || !out) fail(WDG_ERR_INVALID, "null argument");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (op->prec != WDG_F64) fail(WDG_ERR_INVALID, "flux functors need an FP64 wave handle");
+        const int n = op->n;
+        // the reference's default rule, 2 n points (DivF.cpp:56-60, EdgeFluxF.cpp:51-56)
+        if (nq_volume <= 0) nq_volume = 2 * n;
+        if (nq_edge <= 0) nq_edge = 2 * n;
+        if (nq_volume > 20 || nq_edge > 20) fail(WDG_ERR_INVALID, "quadrature rules have at most 20 points");
+        const int sp = n * n_var * nq_volume, sdp = 2 * nq_volume * n_var * n;
+        if ((size_t)(2 * nq_volume * n_var * nq_volume + std::max(sp, sdp)) * sizeof(double) > 65536)
+            fail(WDG_ERR_INVALID, "volume rule too large for n_var (local memory of one workgroup)");
+        auto table = [&](int nq, bool with_d) {
+            std::vector<double> xq, wq;
+            if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "no Gauss-Legendre rule of that size");
+            std::vector<double> t(2 * nq + (with_d ? 2 : 1) * nq * n);
+            std::copy(xq.begin(), xq.end(), t.begin());
+            std::copy(wq.begin(), wq.end(), t.begin() + nq);
+            wdg::lagrange_interpolation_matrix(n, op->x.data(), nq, xq.data(), t.data() + 2 * nq);
+            if (with_d) wdg::lagrange_derivative_matrix(n, op->x.data(), nq, xq.data(), t.data() + 2 * nq + nq * n);
+            return t;
+        };
+        const std::vector<double> tv = table(nq_volume, true), te = table(nq_edge, false);
+        auto fx = std::make_unique<wdg_flux_s>();
+        fx->op = op;
+        fx->nv = n_var;
+        fx->f = wdg::rtc_flux_create(source, n, n_var, op->leg);
+        std::string err;
+        if (!wdg::rtc_flux_set_rules(fx->f, nq_volume, tv, nq_edge, te, err)) fail(WDG_ERR_HIP, err);
+        *out = fx.release();
     });
 }
 

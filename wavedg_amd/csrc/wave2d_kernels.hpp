@@ -729,9 +729,16 @@ void launch_euler_face(int n, double gamma, const double* q, double* y, const do
 // at run time (flux_rtc.hip).  Kernels compile on first use; the functions
 // return false with a message on a compile or launch failure.
 struct RtcFlux;
-// compile only (no device): the code object of one kernel for (n, nv, leg)
-bool rtc_build(const std::string& user, int n, int nv, int leg, bool divf, std::vector<char>& code, std::string& err);
+// compile only (no device): the code object of one kernel for (n, nv, leg);
+// kind 0 collocated divergence, 1 collocated edge flux, 2 and 3 the same with
+// quadrature rules of nq points (DivF2D<false>, EdgeFluxF2D<false>)
+bool rtc_build(const std::string& user, int n, int nv, int leg, int kind, int nq, std::vector<char>& code,
+               std::string& err);
 RtcFlux* rtc_flux_create(const char* source, int n, int nv, int leg);
+// switch to the quadrature forms: tabv = xq, wq, P, D (nqv points), tabe =
+// xq, wq, P (nqe points), P and D column-major (nq, n)
+bool rtc_flux_set_rules(RtcFlux* f, int nqv, const std::vector<double>& tabv, int nqe, const std::vector<double>& tabe,
+                        std::string& err);
 void rtc_flux_destroy(RtcFlux* f);
 // which: 0 the divergence kernel, 1 the edge-flux kernel
 bool rtc_flux_prepare(RtcFlux* f, int which, std::string& err);
@@ -791,9 +798,10 @@ struct ExactMassTables {  // MassMatrix<false> by fast diagonalisation (host mas
 void launch_exact_volume(int n, const KArgs& a, const ExactTables& t, int accumulate, hipStream_t s);
 // a.u_out += lifted EdgeFlux<false> of interior and/or boundary sides
 void launch_exact_face(int n, const KArgs& a, const ExactTables& t, int interior, int boundary, hipStream_t s);
-// kid -1: y = M x, -2: y = M^{-1} x; kid >= 0: the RHS / RK-stage epilogue of
-// that kernel id applied to the operator output r
-void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s);
+// kid -1: y = M x, -2: y = M^{-1} x for nv <= 8 variables; kid >= 0: the
+// RHS / RK-stage epilogue of that kernel id applied to the operator output r
+void launch_exact_mass(int n, const KArgs& a, const double* r, const ExactMassTables& m, int kid, hipStream_t s,
+                       int nv = 3);
 // The whole RHS / RK stage (kid >= K_RHS) for the default rules (n volume
 // points, n + 1 edge points) in one kernel; tab is the handle's table block.
 void launch_exact_stage(int n, int kid, const KArgs& a, const double* tab, hipStream_t s);

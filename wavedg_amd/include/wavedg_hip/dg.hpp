@@ -443,8 +443,9 @@ private:
     detail::AdvectionHandle h_;
 };
 
-// DivF2D<true> and EdgeFluxF2D<true> (DivF.hpp:207-274, EdgeFluxF.hpp:
-// 136-171) with the functors as HIP source text: the reference's lambdas
+// DivF2D and EdgeFluxF2D (DivF.hpp:207-392, EdgeFluxF.hpp:136-250; <true>
+// collocated, <false> on Gauss-Legendre rules of quad->n points, 2 n when
+// quad is null) with the functors as HIP source text: the reference's lambdas
 // become FluxSource strings holding
 //   __device__ void flux(const double* x, const double* u, double* F);
 //   __device__ void numerical_flux(const double* x, const double* n,
@@ -459,9 +460,13 @@ struct FluxSource {
 };
 
 namespace detail {
+// approx_quad = false: nq points of the Gauss-Legendre rule (0 = 2 n, the
+// reference's default, DivF.cpp:56-60, EdgeFluxF.cpp:51-56)
 class FluxCache {
 public:
-    FluxCache(const Mesh2D& mesh, const QuadratureRule* basis) : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64)
+    FluxCache(const Mesh2D& mesh, const QuadratureRule* basis, bool approx_quad, const QuadratureRule* quad)
+        : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64), approx_(approx_quad),
+          nq_(approx_quad ? 0 : exact_rule_size(quad))
     {
     }
     wdg_flux get(int n_var, const FluxSource& f) const
@@ -470,7 +475,10 @@ public:
         auto it = cache_.find(key);
         if (it != cache_.end()) return it->second.get();
         wdg_flux h = nullptr;
-        check(wdg_hip_flux_create(op_.get(), n_var, f.source.c_str(), &h));
+        if (approx_)
+            check(wdg_hip_flux_create(op_.get(), n_var, f.source.c_str(), &h));
+        else
+            check(wdg_hip_flux_create_exact(op_.get(), n_var, f.source.c_str(), nq_, nq_, &h));
         std::unique_ptr<wdg_flux_s, Del> p(h);
         return cache_.emplace(key, std::move(p)).first->second.get();
     }
@@ -482,18 +490,18 @@ private:
         void operator()(wdg_flux p) const { wdg_hip_flux_destroy(p); }
     };
     Wave2D op_;  // declared before cache_: destroyed after it
+    bool approx_;
+    int nq_;
     mutable std::unordered_map<std::string, std::unique_ptr<wdg_flux_s, Del>> cache_;
 };
 }  // namespace detail
 
 template <bool ApproxQuadrature>
 class DivF2D {
-    static_assert(ApproxQuadrature, "dg::hip: only DivF2D<true> is implemented on the GPU");
-
 public:
-    DivF2D(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr) : c_(mesh, basis)
+    DivF2D(const Mesh2D& mesh, const QuadratureRule* basis, const QuadratureRule* quad = nullptr)
+        : c_(mesh, basis, ApproxQuadrature, quad)
     {
-        (void)quad;
     }
     // divF += (F(x, u), grad v); host arrays (n_var, n, n, n_elem)
     void action(int n_var, const FluxSource& f, const double* u, double* divF) const
@@ -517,14 +525,11 @@ private:
 
 template <bool ApproxQuadrature>
 class EdgeFluxF2D {
-    static_assert(ApproxQuadrature, "dg::hip: only EdgeFluxF2D<true> is implemented on the GPU");
-
 public:
     EdgeFluxF2D(const Mesh2D& mesh, FaceType face_type, const QuadratureRule* basis,
                 const QuadratureRule* quad = nullptr)
-        : c_(mesh, basis), type_(face_type)
+        : c_(mesh, basis, ApproxQuadrature, quad), type_(face_type)
     {
-        (void)quad;
     }
     // divF += scale * lifted (+-fh w ds) on this object's faces; host arrays
     void action(int n_var, const FluxSource& f, const double* u, double* divF, double scale = 1.0) const
