@@ -193,6 +193,15 @@ int wdg_hip_advection_action(wdg_advection adv, const void *d_u, void *d_divF, v
 /* AdvectionHomogeneousBC::action (Advection.hpp:252-270): d_divF += boundary
  * edge fluxes with u = 0 outside the domain. */
 int wdg_hip_advection_bc_action(wdg_advection adv, const void *d_u, void *d_divF, void *stream);
+/* Advection<false> / AdvectionHomogeneousBC<false> (Advection.hpp:59-93,
+ * 190-215): Div<false> with the coefficient (Div.cpp:312-360, 200-310) and
+ * EdgeFlux<false> (EdgeFlux.cpp:375-640) on Gauss-Legendre rules of
+ * nq_volume^2 and nq_edge points (0 = the reference's defaults for bilinear
+ * elements; at most 20); the coefficient is interpolated to the points as
+ * the reference does.  Any FP64 wave handle of the mesh and basis; the
+ * action entry points above then run the quadrature kernels. */
+int wdg_hip_advection_create_exact(wdg_wave2d op, int n_var, const double *A, int constant_coefficient,
+                                   int nq_volume, int nq_edge, wdg_advection *out);
 int wdg_hip_advection_destroy(wdg_advection adv);
 
 /* ---- DivF2D / EdgeFluxF2D with flux functors --------------------------

@@ -87,6 +87,11 @@ def lib():
             "orc_edgefluxf_euler": (None, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp]),
             "orc_divf_functor": (None, [_vp, ctypes.c_int, FLUX_FN, _vp, _dp, _dp]),
             "orc_divf_functor_q": (None, [_vp, ctypes.c_int, ctypes.c_int, FLUX_FN, _vp, _dp, _dp]),
+            "orc_advection_q_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "orc_advection_q_free": (None, [_vp]),
+            "orc_advection_q_rules": (None, [_vp, _ip, _ip]),
+            "orc_advection_q_action": (ctypes.c_int, [_vp, _dp, _dp]),
+            "orc_advection_q_bc_action": (ctypes.c_int, [_vp, _dp, _dp]),
             "orc_edgefluxf_functor_q": (None, [_vp, ctypes.c_int, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int,
                                                ctypes.c_double, _dp, _dp]),
             "orc_edgefluxf_functor": (None, [_vp, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int, ctypes.c_double,
@@ -369,6 +374,41 @@ class Advection:
     def bc_action(self, u, y=None):
         y = np.zeros(self.size) if y is None else y
         lib().orc_advection_bc_action(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(y))
+        return y
+
+
+class AdvectionQ:
+    """Oracle Advection<false> + AdvectionHomogeneousBC<false>: the same
+    coefficient layouts as Advection, Gauss-Legendre rules quad = (volume,
+    edge) points (0 = the reference's defaults)."""
+
+    def __init__(self, wave: Wave, nvar: int, A_colmajor, constant_coefficient: bool = True, quad=(0, 0)):
+        self.wave = wave
+        self.nvar = nvar
+        self._A = np.ascontiguousarray(np.asarray(A_colmajor, dtype=np.float64).reshape(-1))
+        self.h = lib().orc_advection_q_create(wave.h, nvar, _d(self._A), 1 if constant_coefficient else 0, *quad)
+        if not self.h:
+            raise ValueError("orc_advection_q_create failed")
+        a, b = ctypes.c_int(), ctypes.c_int()
+        lib().orc_advection_q_rules(self.h, ctypes.byref(a), ctypes.byref(b))
+        self.quad = (a.value, b.value)
+        self.size = nvar * wave.ndof // 3
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_advection_q_free(self.h)
+            self.h = None
+
+    def action(self, u, y=None):
+        y = np.zeros(self.size) if y is None else y
+        if lib().orc_advection_q_action(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(y)):
+            raise ValueError("normal matrix without a real eigen-decomposition")
+        return y
+
+    def bc_action(self, u, y=None):
+        y = np.zeros(self.size) if y is None else y
+        if lib().orc_advection_q_bc_action(self.h, _d(np.ascontiguousarray(u, dtype=np.float64)), _d(y)):
+            raise ValueError("normal matrix without a real eigen-decomposition")
         return y
 
 

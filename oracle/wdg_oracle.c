@@ -2022,3 +2022,229 @@ void orc_edgefluxf_euler(const orc_wave *W, double gamma, int type, const double
     free(uf);
 }
 
+
+/* ===================================================================== */
+/* Advection<false> (Advection.hpp:59-93, 190-215): Div<false> with the   */
+/* coefficient (Div.cpp:312-360, op_2d_cc / op_2d :200-310) and           */
+/* EdgeFlux<false> (EdgeFlux.cpp:375-640) with a = -1, b = -0.5           */
+/* ===================================================================== */
+
+struct orc_advq {
+    const orc_wave *W;
+    int nvar, cc, nqv, nqe;
+    double *A;
+    double xv[32], wv[32], Pv[32 * 16], Dv[32 * 16]; /* (nq, n) column-major */
+    double xe[32], we[32], Pe[32 * 16];
+};
+
+orc_advq *orc_advection_q_create(const orc_wave *W, int nvar, const double *A, int constant_coefficient, int nqv,
+                                 int nqe)
+{
+    const int n = W->n;
+    if (nvar < 1 || nvar > 8) return NULL;
+    /* default rules (Div.cpp:319-327, EdgeFlux.cpp:503-511; bilinear elements) */
+    if (nqv <= 0) nqv = 1 + ((n - 2) + (n - 1) + 1 + (constant_coefficient ? 0 : n - 1)) / 2;
+    if (nqe <= 0) nqe = 1 + (2 * (n - 1) + 2 + (constant_coefficient ? 0 : n - 1)) / 2;
+    if (nqv > 32 || nqe > 32) return NULL;
+    orc_advq *V = (orc_advq *)calloc(1, sizeof(orc_advq));
+    V->W = W;
+    V->nvar = nvar;
+    V->cc = constant_coefficient ? 1 : 0;
+    V->nqv = nqv;
+    V->nqe = nqe;
+    const size_t na = (size_t)2 * nvar * nvar * (V->cc ? 1 : (size_t)n * n * W->m->nel);
+    V->A = (double *)malloc(sizeof(double) * na);
+    memcpy(V->A, A, sizeof(double) * na);
+    orc_gauss_legendre(nqv, V->xv, V->wv);
+    orc_lagrange_basis(V->Pv, n, W->x, nqv, V->xv);
+    orc_lagrange_basis_deriv(V->Dv, n, W->x, nqv, V->xv);
+    orc_gauss_legendre(nqe, V->xe, V->we);
+    orc_lagrange_basis(V->Pe, n, W->x, nqe, V->xe);
+    return V;
+}
+
+void orc_advection_q_free(orc_advq *V)
+{
+    if (!V) return;
+    free(V->A);
+    free(V);
+}
+
+void orc_advection_q_rules(const orc_advq *V, int *nqv, int *nqe)
+{
+    *nqv = V->nqv;
+    *nqe = V->nqe;
+}
+
+static void advq_div(const orc_advq *V, const double *u, double *divF)
+{
+    const orc_wave *W = V->W;
+    const int n = W->n, nn = n * n, nel = W->m->nel, m = V->nvar, v2 = m * m, nq = V->nqv;
+    const double *P = V->Pv, *D = V->Dv;
+    double *Pu = (double *)malloc(sizeof(double) * (size_t)n * m * nq);
+    double *F = (double *)malloc(sizeof(double) * 2 * (size_t)nq * m * nq);
+    double *Df = (double *)malloc(sizeof(double) * (size_t)nq * m * n);
+    double *Pg = (double *)malloc(sizeof(double) * (size_t)nq * m * n);
+    double Aq[128], J[4];
+    for (int el = 0; el < nel; ++el) {
+        const double *c = W->m->corners + 8 * (size_t)el;
+        const double *ue = u + (size_t)m * nn * el;
+        for (int i = 0; i < nq; ++i)
+            for (int l = 0; l < n; ++l)
+                for (int d = 0; d < m; ++d) {
+                    double pu = 0.0;
+                    for (int k = 0; k < n; ++k) pu += P[i + nq * k] * ue[d + m * (k + n * l)];
+                    Pu[l + n * (d + m * i)] = pu;
+                }
+        for (int j = 0; j < nq; ++j)
+            for (int i = 0; i < nq; ++i) {
+                /* the coefficient at the point: A or its interpolation (op_2d) */
+                for (int t = 0; t < 2 * v2; ++t) {
+                    if (V->cc) {
+                        Aq[t] = V->A[t];
+                        continue;
+                    }
+                    double a = 0.0;
+                    for (int l = 0; l < n; ++l)
+                        for (int k = 0; k < n; ++k)
+                            a += P[i + nq * k] * P[j + nq * l] * V->A[t + 2 * v2 * (k + n * (l + (size_t)n * el))];
+                    Aq[t] = a;
+                }
+                quad_jacobian(c, V->xv[i], V->xv[j], J);
+                const double w_ij = V->wv[i] * V->wv[j];
+                const double Y_eta = J[3] * w_ij, X_eta = J[2] * w_ij, Y_xi = J[1] * w_ij, X_xi = J[0] * w_ij;
+                double uq[8];
+                for (int d = 0; d < m; ++d) {
+                    double s = 0.0;
+                    for (int l = 0; l < n; ++l) s += P[j + nq * l] * Pu[l + n * (d + m * i)];
+                    uq[d] = s;
+                }
+                for (int d = 0; d < m; ++d) {
+                    double Fd = 0.0, Gd = 0.0;
+                    for (int cc = 0; cc < m; ++cc) {
+                        const double a0 = Aq[d + m * cc], a1 = Aq[v2 + d + m * cc];
+                        Fd += (Y_eta * a0 - X_eta * a1) * uq[cc];
+                        Gd += (-Y_xi * a0 + X_xi * a1) * uq[cc];
+                    }
+                    F[0 + 2 * (i + nq * (d + m * j))] = Fd;
+                    F[1 + 2 * (i + nq * (d + m * j))] = Gd;
+                }
+            }
+        for (int j = 0; j < nq; ++j)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < m; ++d) {
+                    double df = 0.0, pg = 0.0;
+                    for (int i = 0; i < nq; ++i) {
+                        df += D[i + nq * k] * F[0 + 2 * (i + nq * (d + m * j))];
+                        pg += P[i + nq * k] * F[1 + 2 * (i + nq * (d + m * j))];
+                    }
+                    Df[j + nq * (d + m * k)] = df;
+                    Pg[j + nq * (d + m * k)] = pg;
+                }
+        for (int l = 0; l < n; ++l)
+            for (int k = 0; k < n; ++k)
+                for (int d = 0; d < m; ++d) {
+                    double dF = 0.0;
+                    for (int j = 0; j < nq; ++j)
+                        dF += P[j + nq * l] * Df[j + nq * (d + m * k)] + D[j + nq * l] * Pg[j + nq * (d + m * k)];
+                    divF[d + m * (k + n * (l + (size_t)n * el))] += dF;
+                }
+    }
+    free(Pu);
+    free(F);
+    free(Df);
+    free(Pg);
+}
+
+/* prolongation of u and of the coefficient, setup_flux_2d_(cc_)q and
+ * flux_action_2d_q at the edge rule, the transpose.  Boundary edges: the
+ * exterior side has u = 0 and (variable coefficient) A = 0. */
+static void advq_faces(const orc_advq *V, int type, const double *u, double *divF, int *err)
+{
+    const orc_wave *W = V->W;
+    const int n = W->n, m = V->nvar, v2 = m * m, nq = V->nqe, ne = orc_mesh_n_edges(W->m, type);
+    const int *v2e = type == 0 ? W->v2eI : W->v2eB;
+    const int n_sides = type == 0 ? 2 : 1;
+    const double *P = V->Pe;
+    double *uf = (double *)calloc((size_t)2 * n * m * ne + 1, sizeof(double));       /* (n, m, 2, ne) */
+    double *af = (double *)calloc((size_t)2 * n * 2 * v2 * ne + 1, sizeof(double));  /* (n, 2 v2, 2, ne) */
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i) {
+                const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                for (int d = 0; d < m; ++d) uf[i + n * (d + m * (side + 2 * (size_t)e))] = face_value(W, u, m, j, f, d);
+                if (!V->cc)
+                    for (int t = 0; t < 2 * v2; ++t)
+                        af[i + n * (t + 2 * v2 * (side + 2 * (size_t)e))] = face_value(W, V->A, 2 * v2, j, f, t);
+            }
+        }
+    double nA[64], Fs[64], acc[32 * 8], uq[8];
+    for (int e = 0; e < ne; ++e) {
+        const orc_edge *E = edge_of(W->m, e, type);
+        for (int t = 0; t < nq * m; ++t) acc[t] = 0.0;
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < nq; ++i) {
+                const double w = V->we[i] * E->meas;
+                for (int t = 0; t < v2; ++t) {
+                    double a0, a1;
+                    if (V->cc) {
+                        a0 = V->A[t];
+                        a1 = V->A[v2 + t];
+                    } else {
+                        a0 = a1 = 0.0;
+                        for (int k = 0; k < n; ++k) {
+                            a0 += af[k + n * (t + 2 * v2 * (s + 2 * (size_t)e))] * P[i + nq * k];
+                            a1 += af[k + n * (v2 + t + 2 * v2 * (s + 2 * (size_t)e))] * P[i + nq * k];
+                        }
+                    }
+                    nA[t] = E->n[0] * a0 + E->n[1] * a1;
+                }
+                if (compute_flux_m(m, nA, -1.0, -0.5, s, Fs) != 0) *err = 1;
+                for (int d = 0; d < m; ++d) {
+                    double t = 0.0;
+                    for (int k = 0; k < n; ++k) t += uf[k + n * (d + m * (s + 2 * (size_t)e))] * P[i + nq * k];
+                    uq[d] = t;
+                }
+                for (int d = 0; d < m; ++d) {
+                    double flx = 0.0;
+                    for (int c2 = 0; c2 < m; ++c2) flx += (w * Fs[c2 + m * d]) * uq[c2];
+                    acc[i + nq * d] += flx;
+                }
+            }
+        for (int d = 0; d < m; ++d)
+            for (int k = 0; k < n; ++k) {
+                double flx = 0.0;
+                for (int i = 0; i < nq; ++i) flx += acc[i + nq * d] * P[i + nq * k];
+                uf[k + n * (d + m * (0 + 2 * (size_t)e))] = flx;
+                uf[k + n * (d + m * (1 + 2 * (size_t)e))] = -flx;
+            }
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int side = 0; side < n_sides; ++side) {
+            const int f = edge_of(W->m, e, type)->side[side];
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < m; ++d) {
+                    const int j = v2e[i + n * (side + 2 * (size_t)e)];
+                    face_add(W, divF, m, j, f, d, uf[i + n * (d + m * (side + 2 * (size_t)e))]);
+                }
+        }
+    free(uf);
+    free(af);
+}
+
+int orc_advection_q_action(const orc_advq *V, const double *u, double *divF)
+{
+    int err = 0;
+    advq_div(V, u, divF);
+    advq_faces(V, 0, u, divF, &err);
+    return err;
+}
+
+int orc_advection_q_bc_action(const orc_advq *V, const double *u, double *divF)
+{
+    int err = 0;
+    advq_faces(V, 1, u, divF, &err);
+    return err;
+}
+

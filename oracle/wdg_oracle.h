@@ -130,6 +130,17 @@ typedef void (*orc_ext_fn)(const double *x, const double *n, const double *u, do
 void orc_divf_functor(const orc_wave *W, int nvar, orc_flux_fn f, void *ctx, const double *u, double *divF);
 void orc_edgefluxf_functor(const orc_wave *W, int nvar, orc_nflux_fn fh, orc_ext_fn ext, void *ctx, int type,
                            double scale, const double *u, double *divF);
+/* Advection<false> / AdvectionHomogeneousBC<false>: coefficient as for
+ * orc_advection_create, Gauss-Legendre rules of nqv x nqv volume and nqe edge
+ * points (0 = the reference's defaults for bilinear elements).  The actions
+ * return nonzero if a normal matrix has no real eigen-decomposition. */
+typedef struct orc_advq orc_advq;
+orc_advq *orc_advection_q_create(const orc_wave *W, int nvar, const double *A, int constant_coefficient, int nqv,
+                                 int nqe);
+void orc_advection_q_free(orc_advq *V);
+void orc_advection_q_rules(const orc_advq *V, int *nqv, int *nqe);
+int orc_advection_q_action(const orc_advq *V, const double *u, double *divF);
+int orc_advection_q_bc_action(const orc_advq *V, const double *u, double *divF);
 /* the approx_quad = false forms with Gauss-Legendre rules of nq points */
 void orc_divf_functor_q(const orc_wave *W, int nvar, int nq, orc_flux_fn f, void *ctx, const double *u, double *divF);
 void orc_edgefluxf_functor_q(const orc_wave *W, int nvar, int nq, orc_nflux_fn fh, orc_ext_fn ext, void *ctx,

@@ -217,6 +217,15 @@ int main(int argc, char** argv)
             to_file(dir + "/advgl_c.bin", c.size(), c.data());
             to_file(dir + "/advgl_u.bin", np, u4.data());
             to_file(dir + "/advgl_dudt.bin", np, du4.data());
+            // ... and with approx_quad = false (advec2d's other setting)
+            Advection<false> adv_x(1, mesh4, basis4, c.data(), false);
+            AdvectionHomogeneousBC<false> adv_bc_x(1, mesh4, basis4, c.data(), false);
+            MassMatrix<false> m4x(mesh4, basis4);
+            std::fill(du4.begin(), du4.end(), 0.0);
+            adv_x.action(u4.data(), du4.data());
+            adv_bc_x.action(u4.data(), du4.data());
+            m4x.inv(1, du4.data());
+            to_file(dir + "/advgl_dudt_exact.bin", np, du4.data());
         }
         // --- examples/cl2d.cpp's time derivative with the flux functors as
         // source: DivF2D, EdgeFluxF2D on interior and boundary faces (scale

@@ -100,3 +100,64 @@ def test_advection_rejects_bad_input(cuda):
     g32 = capi.Wave2D(capi.Mesh(v, e), 3, None, precision=capi.F32)
     with pytest.raises(capi.WdgError):
         capi.Advection(g32, 1, [1.0, 0.5], True)
+
+
+# ---- Advection<false> / AdvectionHomogeneousBC<false>
+
+
+@BASES
+@pytest.mark.parametrize("name,n,nvar", [("skewed6x5", 2, 1), ("skew", 4, 3), ("L", 7, 2), ("skewed6x5", 9, 1)])
+def test_exact_advection_constant_coefficient(cuda, name, n, nvar, leg):
+    torch = cuda
+    print(f"advection<false> {name} n={n} nvar={nvar} legendre={leg}", flush=True)
+    v, e = mesh_case(name)
+    A = sym_coeff(nvar, 200 + n + nvar)
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
+    ov = O.AdvectionQ(ow, nvar, A, True)
+    gv = capi.Advection(_wave(v, e, n, leg), nvar, A, True, approx_quad=False)
+    size = nvar * gv.wave.ndof // 3
+    u, y0 = random_state(size, 17 + n), random_state(size, 19)
+    ya, yb = _dev(torch, y0), _dev(torch, y0)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy() - y0, ov.action(u)) < TOL
+    assert rel_l2(yb.cpu().numpy() - y0, ov.bc_action(u)) < TOL
+
+
+@BASES
+@pytest.mark.parametrize("n,nvar,quad", [(3, 1, (0, 0)), (5, 2, (0, 0)), (6, 1, (9, 10))])
+def test_exact_advection_variable_coefficient(cuda, n, nvar, quad, leg):
+    torch = cuda
+    print(f"advection<false> variable n={n} nvar={nvar} quad={quad} legendre={leg}", flush=True)
+    v, e = mesh_case("skewed6x5")
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    rng = np.random.default_rng(n)
+    Anode = np.concatenate([sym_coeff(nvar, int(s)) * (1.0 + 0.3 * rng.standard_normal())
+                            for s in rng.integers(0, 1 << 30, n * n * nel)])
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
+    ov = O.AdvectionQ(ow, nvar, Anode, False, quad=quad)
+    gv = capi.Advection(_wave(v, e, n, leg), nvar, Anode, False, approx_quad=False, quad=quad)
+    u = random_state(nvar * n * n * nel, 33)
+    ya = torch.zeros(u.size, dtype=torch.float64, device="cuda")
+    yb = torch.zeros_like(ya)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy(), ov.action(u)) < TOL
+    assert rel_l2(yb.cpu().numpy(), ov.bc_action(u)) < TOL
+
+
+@BASES
+def test_exact_advection_wave_matrices_equal_exact_wave_action(cuda, leg):
+    torch = cuda
+    v, e = mesh_case("lshape8")
+    n = 5
+    gx = capi.Wave2D(capi.Mesh(v, e), n, None, approx_quad=False, basis=capi.GAUSS_LEGENDRE if leg else capi.GAUSS_LOBATTO)
+    gv = capi.Advection(gx, 3, WAVE_A, True, approx_quad=False, quad=(n, n + 1))
+    u = _dev(torch, random_state(gx.ndof, 6))
+    y1, y2 = torch.zeros_like(u), torch.zeros_like(u)
+    gv.action(u, y1)
+    gx.action(u, y2)
+    torch.cuda.synchronize()
+    assert float(torch.linalg.norm(y1 - y2) / torch.linalg.norm(y2)) < TOL

@@ -127,6 +127,10 @@ def test_dropin_waveholtz_and_operators(tmp_path):
     u4 = _read(d, "advgl_u.bin")
     want = W4.mass_inv(A4.action(u4) + A4.bc_action(u4), nvar=1)
     assert rel_l2(_read(d, "advgl_dudt.bin"), want) < 1e-12
+    W4x = O.Wave(O.Mesh(v3, e3), 5, approx_quad=False, legendre=True)
+    A4x = O.AdvectionQ(W4x, 1, _read(d, "advgl_c.bin"), False)
+    want = W4x.mass_inv(A4x.action(u4) + A4x.bc_action(u4), nvar=1)
+    assert rel_l2(_read(d, "advgl_dudt_exact.bin"), want) < 1e-12
 
     # examples/cl2d.cpp's time derivative with the flux functors as source
     v5, e5 = meshgen.uniform_rect(10, 0.0, 10.0, 10, -6.0, 6.0)

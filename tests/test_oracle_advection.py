@@ -120,3 +120,74 @@ def test_rejects_non_symmetric_normal_matrix():
     A = np.array([0.0, 1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 0.0])  # A_x = [[0, 2], [1, 0]]
     with pytest.raises(ValueError):
         O.Advection(W, 2, A, True)
+
+
+# ---- Advection<false>: exact quadrature with Gauss-Legendre rules
+
+
+@BASES
+@pytest.mark.parametrize("name,n", [("skewed6x5", 3), ("skew", 5), ("L", 4)])
+def test_exact_advection_with_wave_matrices_is_exact_wave_equation(name, n, leg):
+    """With the wave equation's matrices and the wave operator's rules (n
+    volume, n + 1 edge points), Advection<false> is WaveEquation with
+    approx_quad = false (and its boundary part with u = 0 outside is
+    pinned below by dissipation)."""
+    v, e = mesh_case(name)
+    Wx = O.Wave(O.Mesh(v, e), n, approx_quad=False, legendre=leg)
+    V = O.AdvectionQ(Wx, 3, WAVE_A, True, quad=(n, n + 1))
+    u = random_state(Wx.ndof, 2)
+    assert np.max(np.abs(V.action(u) - Wx.action(u))) <= 1e-13 * np.max(np.abs(Wx.action(u)))
+
+
+@BASES
+@pytest.mark.parametrize("nvar", [1, 3])
+def test_exact_variable_coefficient_reduces_to_constant(nvar, leg):
+    v, e = mesh_case("skewed6x5")
+    n = 4
+    Wx = O.Wave(O.Mesh(v, e), n, approx_quad=False, legendre=leg)
+    A = sym_coeff(nvar, 30 + nvar)
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    quad = (5, 6)  # the variable coefficient's default rules, used for both
+    Vc = O.AdvectionQ(Wx, nvar, A, True, quad=quad)
+    Vv = O.AdvectionQ(Wx, nvar, np.tile(A, n * n * nel), False)
+    assert Vv.quad == quad
+    u = random_state(nvar * n * n * nel, 4)
+    for f in ("action", "bc_action"):
+        a, b = getattr(Vv, f)(u), getattr(Vc, f)(u)
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
+
+
+@BASES
+def test_exact_advection_exact_for_linear_fields_on_bilinear_mesh(leg):
+    """Exact quadrature: for c(x) and u(x) linear, M^{-1} A u = -div(c u) at the
+    nodes of interior elements of a non-affine mesh too (the collocated
+    operator is exact only on affine elements)."""
+    v, e = mesh_case("skewed16")
+    mesh = O.Mesh(v, e)
+    n = 4
+    Wx = O.Wave(mesh, n, approx_quad=False, legendre=leg)
+    xg, _ = O.gauss_legendre(n) if leg else O.gauss_lobatto(n)
+    X = mesh.nodes(xg)
+    x, y = X[..., 0], X[..., 1]
+    cx = 0.3 + 0.5 * x - 0.2 * y
+    cy = -0.4 + 0.1 * x + 0.6 * y
+    u = 1.5 - 0.7 * x + 0.9 * y
+    nel = X.shape[0]
+    A = np.zeros((nel, n, n, 2))
+    A[..., 0], A[..., 1] = cx, cy
+    V = O.AdvectionQ(Wx, 1, A.reshape(-1), False)
+    r = Wx.mass_inv(V.action(u.reshape(-1)), nvar=1).reshape(nel, n, n)
+    want = -((0.5 + 0.6) * u + cx * (-0.7) + cy * 0.9)
+    nx = 16
+    interior = [el for el in range(nel) if 0 < el % nx < nx - 1 and 0 < el // nx < nx - 1]
+    assert np.max(np.abs(r[interior] - want[interior])) < 1e-11
+
+
+@pytest.mark.parametrize("nvar", [1, 2])
+def test_exact_upwind_with_zero_inflow_dissipates(nvar):
+    v, e = mesh_case("skew")
+    Wx = O.Wave(O.Mesh(v, e), 4, approx_quad=False)
+    V = O.AdvectionQ(Wx, nvar, sym_coeff(nvar, 51 + nvar), True)
+    for seed in range(3):
+        u = random_state(nvar * Wx.ndof // 3, seed)
+        assert u @ (V.action(u) + V.bc_action(u)) < 0.0

@@ -55,6 +55,7 @@ _SIGS = {
     "wdg_hip_wave2d_basis": [_vp, _ip, _dp, _dp],
     "wdg_hip_flux_create": [_vp, _i, ctypes.c_char_p, ctypes.POINTER(_vp)],
     "wdg_hip_flux_compile": [_vp, _i],
+    "wdg_hip_advection_create_exact": [_vp, _i, _dp, _i, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_flux_create_exact": [_vp, _i, ctypes.c_char_p, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_flux_check": [ctypes.c_char_p, _i, _i, _i, _i],
     "wdg_hip_divf2d": [_vp, _vp, _vp, _vp],
@@ -479,12 +480,20 @@ class Advection:
     reference's column-major order: (n_var, n_var, 2) when constant, else
     (n_var, n_var, 2, n, n, n_elem); n.A must be symmetric."""
 
-    def __init__(self, wave: Wave2D, n_var: int, A_colmajor, constant_coefficient: bool = True):
+    def __init__(self, wave: Wave2D, n_var: int, A_colmajor, constant_coefficient: bool = True,
+                 approx_quad: bool = True, quad=(0, 0)):
+        """approx_quad=False: Advection<false> with Gauss-Legendre rules
+        quad = (volume, edge) points, 0 = the reference's defaults."""
         self.wave = wave  # the handle must outlive this object
         self.n_var = n_var
         A = np.ascontiguousarray(np.asarray(A_colmajor, dtype=np.float64).reshape(-1))
         h = _vp()
-        _call("wdg_hip_advection_create", wave.h, n_var, _dptr(A), 1 if constant_coefficient else 0, ctypes.byref(h))
+        if approx_quad:
+            _call("wdg_hip_advection_create", wave.h, n_var, _dptr(A), 1 if constant_coefficient else 0,
+                  ctypes.byref(h))
+        else:
+            _call("wdg_hip_advection_create_exact", wave.h, n_var, _dptr(A), 1 if constant_coefficient else 0,
+                  *quad, ctypes.byref(h))
         self.h = h
 
     def __del__(self):

@@ -195,6 +195,167 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
     for (int t = tid; t < nloc * 4 * NN; t += 64) yb[t] += s_out[t];
 }
 
+// Advection<false> volume term (Div<false>, Div.cpp:312-360, 469-588): u
+// interpolated to the nq x nq Gauss-Legendre points, the fluxes op(., 0) u and
+// op(., 1) u there, projected back with D and P.  op (2, nv, nv, nq, nq, nel)
+// holds the metric-weighted coefficient w_i w_j (y_eta A_x - x_eta A_y,
+// -y_xi A_x + x_xi A_y) of every point, as the reference's op_2d table.
+// tab = P (nq, N) then D (nq, N).  One element per 256-lane workgroup; the
+// dynamic LDS holds Pu or Df/Pg, and F.
+template <int N>
+__global__ __launch_bounds__(256) void advect_volume_q_kernel(const double* __restrict__ u, double* __restrict__ out,
+                                                              const double* __restrict__ op,
+                                                              const double* __restrict__ tab, int nq, int nv)
+{
+    constexpr int NN = N * N;
+    extern __shared__ double s_dyn[];
+    const int e = blockIdx.x, tid = threadIdx.x, v2 = nv * nv;
+    const int spu = N * nv * nq, sdp = 2 * nq * nv * N;
+    double* sW = s_dyn;                             // Pu, then Df and Pg
+    double* sF = s_dyn + (spu > sdp ? spu : sdp);  // (2, nq, nv, nq)
+    const double* P = tab;
+    const double* D = tab + nq * N;
+    const double* ue = u + (size_t)nv * NN * e;
+    for (int t = tid; t < spu; t += 256) {
+        const int l = t % N, d = (t / N) % nv, i = t / (N * nv);
+        double pu = 0.0;
+        for (int k = 0; k < N; ++k) pu += P[i + nq * k] * ue[d + nv * (k + N * l)];
+        sW[t] = pu;
+    }
+    __syncthreads();
+    for (int t = tid; t < nq * nq * nv; t += 256) {
+        const int d = t % nv, i = (t / nv) % nq, j = t / (nv * nq);
+        const double* o = op + (size_t)2 * v2 * (i + nq * (j + (size_t)nq * e));
+        double f = 0.0, g = 0.0;
+        for (int c = 0; c < nv; ++c) {
+            double uq = 0.0;
+            for (int l = 0; l < N; ++l) uq += P[j + nq * l] * sW[l + N * (c + nv * i)];
+            f += o[2 * (d + nv * c)] * uq;
+            g += o[2 * (d + nv * c) + 1] * uq;
+        }
+        sF[0 + 2 * (i + nq * (d + nv * j))] = f;
+        sF[1 + 2 * (i + nq * (d + nv * j))] = g;
+    }
+    __syncthreads();
+    for (int t = tid; t < nq * nv * N; t += 256) {
+        const int j = t % nq, d = (t / nq) % nv, k = t / (nq * nv);
+        double df = 0.0, pg = 0.0;
+        for (int i = 0; i < nq; ++i) {
+            df += D[i + nq * k] * sF[0 + 2 * (i + nq * (d + nv * j))];
+            pg += P[i + nq * k] * sF[1 + 2 * (i + nq * (d + nv * j))];
+        }
+        sW[2 * t] = df;
+        sW[2 * t + 1] = pg;
+    }
+    __syncthreads();
+    for (int t = tid; t < NN * nv; t += 256) {
+        const int d = t % nv, k = (t / nv) % N, l = t / (nv * N);
+        double dF = 0.0;
+        for (int j = 0; j < nq; ++j) {
+            const int b = 2 * (j + nq * (d + nv * k));
+            dF += P[j + nq * l] * sW[b] + D[j + nq * l] * sW[b + 1];
+        }
+        out[(size_t)nv * NN * e + t] += dF;
+    }
+}
+
+// Advection<false> face terms (EdgeFlux<false>, EdgeFlux.cpp:375-640): per
+// element side the traces at the side nodes (this element's order; lines of
+// nodes for a Gauss-Legendre basis, P non-null) are interpolated to the nq
+// edge points, c_q = H_own(q) u_own(q) + H_other(q) u_other(q) with H
+// (2, nv, nv) per (element, side, point) already weighted by w_q ds, and
+// sum_q Pe(q, k) c_q is lifted into side node k.  tab = Pe (nq, N).
+// E = 64 / max(N, nq) elements per 64-lane workgroup; dynamic LDS.
+template <int N>
+__global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restrict__ u, double* __restrict__ y,
+                                                            const int32_t* __restrict__ nbr,
+                                                            const int32_t* __restrict__ info,
+                                                            const double* __restrict__ Hq,
+                                                            const double* __restrict__ tab,
+                                                            const double* __restrict__ P, int nq, int nv, int nel,
+                                                            int interior, int boundary)
+{
+    constexpr int NN = N * N;
+    extern __shared__ double s_dyn[];
+    const int W = nq > N ? nq : N, E = 64 / W, v2 = nv * nv;
+    double* s_out = s_dyn;                 // E nv NN
+    double* sA = s_out + E * nv * NN;      // E N nv
+    double* sB = sA + E * N * nv;          // E N nv
+    double* sC = sB + E * N * nv;          // E nq nv
+    const int tid = threadIdx.x, e0 = blockIdx.x * E;
+    const int nloc = nel - e0 < E ? nel - e0 : E;
+    for (int t = tid; t < E * nv * NN; t += 64) s_out[t] = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        __syncthreads();
+        for (int t = tid; t < E * N; t += 64) {
+            const int el = t / N, k = t - el * N, e = e0 + el;
+            if (el >= nloc) continue;
+            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+            if (nb >= 0 ? !interior : !boundary) continue;
+            const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+            for (int d = 0; d < nv; ++d) {
+                const double* qa = u + (size_t)nv * ((size_t)e * NN + side_node<N>(s, k)) + d;
+                double a = 0.0, b = 0.0;
+                if (P) {
+                    const int st = nv * inward_stride<N>(s);
+                    for (int m = 0; m < N; ++m) a += qa[m * st] * P[m];
+                } else {
+                    a = qa[0];
+                }
+                if (nb >= 0) {
+                    const double* qb = u + (size_t)nv * ((size_t)nb * NN + side_node<N>(inf & 3, kk)) + d;
+                    if (P) {
+                        const int st = nv * inward_stride<N>(inf & 3);
+                        for (int m = 0; m < N; ++m) b += qb[m * st] * P[m];
+                    } else {
+                        b = qb[0];
+                    }
+                }
+                sA[(el * N + k) * nv + d] = a;
+                sB[(el * N + k) * nv + d] = b;  // u = 0 outside the domain
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < E * nq * nv; t += 64) {
+            const int el = t / (nq * nv), r = t - el * nq * nv, q = r / nv, d = r - q * nv, e = e0 + el;
+            if (el >= nloc) continue;
+            const int nb = nbr[4 * (size_t)e + s];
+            if (nb >= 0 ? !interior : !boundary) continue;
+            const double* H = Hq + (((size_t)e * 4 + s) * nq + q) * 2 * v2;
+            double c = 0.0;
+            for (int cc = 0; cc < nv; ++cc) {
+                double a = 0.0, b = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    a += tab[q + nq * k] * sA[(el * N + k) * nv + cc];
+                    b += tab[q + nq * k] * sB[(el * N + k) * nv + cc];
+                }
+                c += H[d + nv * cc] * a + H[v2 + d + nv * cc] * b;
+            }
+            sC[(el * nq + q) * nv + d] = c;
+        }
+        __syncthreads();
+        for (int t = tid; t < E * N * nv; t += 64) {
+            const int el = t / (N * nv), r = t - el * N * nv, k = r / nv, d = r - k * nv, e = e0 + el;
+            if (el >= nloc) continue;
+            const int nb = nbr[4 * (size_t)e + s];
+            if (nb >= 0 ? !interior : !boundary) continue;
+            double fl = 0.0;
+            for (int q = 0; q < nq; ++q) fl += sC[(el * nq + q) * nv + d] * tab[q + nq * k];
+            const int node = side_node<N>(s, k);
+            if (P) {
+                // lane (k, d) owns variable d of the line through side node k
+                const int st = inward_stride<N>(s);
+                for (int m = 0; m < N; ++m) s_out[el * nv * NN + d + nv * (node + m * st)] += fl * P[m];
+            } else {
+                s_out[el * nv * NN + d + nv * node] += fl;
+            }
+        }
+    }
+    __syncthreads();
+    double* yb = y + (size_t)e0 * nv * NN;
+    for (int t = tid; t < nloc * nv * NN; t += 64) yb[t] += s_out[t];
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void linear_functional_kernel(const double* __restrict__ corners,
                                                                const double* __restrict__ tab, int nq, int nv,
@@ -307,6 +468,49 @@ void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t
     default: break;
     }
 #undef WDG_ADVECT_CASE
+}
+
+size_t advect_volume_q_lds(int n, int nq, int nv)
+{
+    const int spu = n * nv * nq, sdp = 2 * nq * nv * n;
+    return sizeof(double) * ((size_t)(spu > sdp ? spu : sdp) + 2 * (size_t)nq * nv * nq);
+}
+
+size_t advect_face_q_lds(int n, int nq, int nv)
+{
+    const int W = nq > n ? nq : n, E = 64 / W;
+    return sizeof(double) * ((size_t)E * nv * n * n + 2 * (size_t)E * n * nv + (size_t)E * nq * nv);
+}
+
+void launch_advect_q(int n, int nv, int nq_volume, int nq_edge, const double* u, double* y, const double* op,
+                     const double* tabv, const int32_t* nbr, const int32_t* info, const double* Hq,
+                     const double* tabe, const double* P, int nel, int volume, int interior, int boundary,
+                     hipStream_t s)
+{
+    if (nel <= 0 || nv < 1 || nv > kMaxVar) return;
+    const size_t lv = advect_volume_q_lds(n, nq_volume, nv), lf = advect_face_q_lds(n, nq_edge, nv);
+    const int W = nq_edge > n ? nq_edge : n, E = 64 / W;
+#define WDG_ADVQ_CASE(NV_)                                                                                       \
+    case NV_:                                                                                                    \
+        if (volume)                                                                                              \
+            hipLaunchKernelGGL(advect_volume_q_kernel<NV_>, dim3(nel), dim3(256), lv, s, u, y, op, tabv,          \
+                               nq_volume, nv);                                                                   \
+        if (interior || boundary)                                                                                \
+            hipLaunchKernelGGL(advect_face_q_kernel<NV_>, dim3((nel + E - 1) / E), dim3(64), lf, s, u, y, nbr,     \
+                               info, Hq, tabe, P, nq_edge, nv, nel, interior, boundary);                          \
+        break;
+    switch (n) {
+        WDG_ADVQ_CASE(2)
+        WDG_ADVQ_CASE(3)
+        WDG_ADVQ_CASE(4)
+        WDG_ADVQ_CASE(5)
+        WDG_ADVQ_CASE(6)
+        WDG_ADVQ_CASE(7)
+        WDG_ADVQ_CASE(8)
+        WDG_ADVQ_CASE(9)
+    default: break;
+    }
+#undef WDG_ADVQ_CASE
 }
 
 }  // namespace wdg

@@ -1590,10 +1590,19 @@ struct wdg_advection_s {
     double A0[64] = {}, A1[64] = {};  // constant coefficient, padded to DivFArgs::kMaxVar
     double* d_Anode = nullptr;        // variable coefficient (nv, nv, 2, n, n, nel)
     double* d_H = nullptr;            // face matrices (2, nv, nv) per element, side, node
+    // approx_quad = false: Gauss-Legendre rules and their tables
+    bool exact = false;
+    int nqv = 0, nqe = 0;
+    double* d_op = nullptr;    // (2, nv, nv, nqv, nqv, nel) metric-weighted coefficient
+    double* d_tabv = nullptr;  // P, D (nqv, n)
+    double* d_tabe = nullptr;  // P (nqe, n)
     ~wdg_advection_s()
     {
         dfree(d_Anode);
         dfree(d_H);
+        dfree(d_op);
+        dfree(d_tabv);
+        dfree(d_tabe);
     }
 };
 
@@ -1688,6 +1697,155 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
     });
 }
 
+int wdg_hip_advection_create_exact(wdg_wave2d op, int n_var, const double* A, int constant_coefficient,
+                                   int nq_volume, int nq_edge, wdg_advection* out)
+{
+    return guarded([&] {
+        check_op(op);
+        if (!A || !out) fail(WDG_ERR_INVALID, "null argument");
+        if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
+        if (op->prec != WDG_F64) fail(WDG_ERR_INVALID, "advection needs an FP64 wave handle");
+        const int N = op->n, NN = N * N, nel = op->nel, m = n_var, v2 = m * m;
+        const bool cc = constant_coefficient != 0;
+        // the reference's default rules for bilinear elements (Div.cpp:319-327, EdgeFlux.cpp:503-511)
+        if (nq_volume <= 0) nq_volume = 1 + ((N - 2) + (N - 1) + 1 + (cc ? 0 : N - 1)) / 2;
+        if (nq_edge <= 0) nq_edge = 1 + (2 * (N - 1) + 2 + (cc ? 0 : N - 1)) / 2;
+        if (nq_volume > 20 || nq_edge > 20) fail(WDG_ERR_INVALID, "quadrature rules have at most 20 points");
+        if (wdg::advect_volume_q_lds(N, nq_volume, m) > 65536 || wdg::advect_face_q_lds(N, nq_edge, m) > 65536)
+            fail(WDG_ERR_INVALID, "rules too large for n_var (local memory of one workgroup)");
+        std::vector<double> xv, wv, xe, we;
+        wdg::gauss_legendre_rule(nq_volume, xv, wv);
+        wdg::gauss_legendre_rule(nq_edge, xe, we);
+        std::vector<double> tabv(2 * (size_t)nq_volume * N), tabe((size_t)nq_edge * N), Pf(N, 0.0);
+        wdg::lagrange_interpolation_matrix(N, op->x.data(), nq_volume, xv.data(), tabv.data());
+        wdg::lagrange_derivative_matrix(N, op->x.data(), nq_volume, xv.data(), tabv.data() + nq_volume * N);
+        wdg::lagrange_interpolation_matrix(N, op->x.data(), nq_edge, xe.data(), tabe.data());
+        if (op->leg) {
+            const double xm1 = -1.0;
+            wdg::lagrange_interpolation_matrix(N, op->x.data(), 1, &xm1, Pf.data());
+        }
+        std::vector<double> geom(6 * (size_t)nel);
+        std::vector<int32_t> nbr(4 * (size_t)nel), info(4 * (size_t)nel);
+        hip_check(hipMemcpy(geom.data(), op->d_geom, sizeof(double) * geom.size(), hipMemcpyDeviceToHost), "geometry");
+        hip_check(hipMemcpy(nbr.data(), op->d_nbr, sizeof(int32_t) * nbr.size(), hipMemcpyDeviceToHost), "nbr");
+        hip_check(hipMemcpy(info.data(), op->d_info, sizeof(int32_t) * info.size(), hipMemcpyDeviceToHost), "info");
+        auto node_coef = [&](int e, int node) { return cc ? A : A + (size_t)2 * v2 * (node + (size_t)NN * e); };
+        // volume: op(k, d, c, i, j, el) = w_i w_j (metric terms) A at the point (Div.cpp:200-310)
+        const int nqv = nq_volume, nqe = nq_edge;
+        std::vector<double> opt((size_t)2 * v2 * nqv * nqv * nel);
+        std::vector<double> Aq(2 * v2);
+        for (int e = 0; e < nel; ++e) {
+            const double* g = &geom[6 * (size_t)e];
+            for (int j = 0; j < nqv; ++j)
+                for (int i = 0; i < nqv; ++i) {
+                    for (int t = 0; t < 2 * v2; ++t) {
+                        if (cc) {
+                            Aq[t] = A[t];
+                            continue;
+                        }
+                        double a = 0.0;
+                        for (int l = 0; l < N; ++l)
+                            for (int k = 0; k < N; ++k)
+                                a += tabv[i + nqv * k] * tabv[j + nqv * l] * node_coef(e, k + N * l)[t];
+                        Aq[t] = a;
+                    }
+                    const double xi = xv[i], eta = xv[j], W = wv[i] * wv[j];
+                    const double Xx = (g[0] + g[4] * eta) * W, Yx = (g[1] + g[5] * eta) * W;
+                    const double Xe = (g[2] + g[4] * xi) * W, Ye = (g[3] + g[5] * xi) * W;
+                    double* o = &opt[(size_t)2 * v2 * (i + nqv * (j + (size_t)nqv * e))];
+                    for (int c = 0; c < m; ++c)
+                        for (int d = 0; d < m; ++d) {
+                            const double ax = Aq[d + m * c], ay = Aq[v2 + d + m * c];
+                            o[2 * (d + m * c)] = Ye * ax - Xe * ay;
+                            o[2 * (d + m * c) + 1] = -Yx * ax + Xx * ay;
+                        }
+                }
+        }
+        // faces: H_own = w ds (a/2 C + b|C|), H_other = w ds (a/2 C' - b|C'|),
+        // a = -1, b = -1/2, C = n.A on this side and C' on the other at the
+        // edge point, n this element's outward normal (EdgeFlux.cpp:375-490)
+        auto side_node = [&](int sd, int k) {
+            const int i = (sd == 0 || sd == 2) ? k : (sd == 1 ? N - 1 : 0);
+            const int j = (sd == 1 || sd == 3) ? k : (sd == 2 ? N - 1 : 0);
+            return i + N * j;
+        };
+        auto trace_coef = [&](int e, int sd, int k, double* out2) {  // 2 v2 values on side sd at side node k
+            if (cc) {
+                for (int t = 0; t < 2 * v2; ++t) out2[t] = A[t];
+                return;
+            }
+            const int node = side_node(sd, k);
+            if (!op->leg) {
+                for (int t = 0; t < 2 * v2; ++t) out2[t] = node_coef(e, node)[t];
+                return;
+            }
+            const int st = sd == 0 ? N : sd == 1 ? -1 : sd == 2 ? -N : 1;
+            for (int t = 0; t < 2 * v2; ++t) {
+                double a = 0.0;
+                for (int mm = 0; mm < N; ++mm) a += node_coef(e, node + mm * st)[t] * Pf[mm];
+                out2[t] = a;
+            }
+        };
+        std::vector<double> Hq((size_t)nel * 4 * nqe * 2 * v2, 0.0), tro((size_t)N * 2 * v2), trb((size_t)N * 2 * v2);
+        double C[64], absC[64], Af[128];
+        auto normal_matrix = [&](const double* Ak, double n0, double n1) {
+            for (int q = 0; q < v2; ++q) C[q] = n0 * Ak[q] + n1 * Ak[v2 + q];
+            if (!wdg::symmetric_abs(m, C, absC)) fail(WDG_ERR_INVALID, "n.A is not symmetric at an edge point");
+        };
+        for (int e = 0; e < nel; ++e) {
+            const double* g = &geom[6 * (size_t)e];
+            for (int sd = 0; sd < 4; ++sd) {
+                const double sh = (sd == 0 || sd == 3) ? -1.0 : 1.0;
+                const double dx = 2.0 * ((sd & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
+                const double dy = 2.0 * ((sd & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
+                const double len = std::sqrt(dx * dx + dy * dy), sg = (sd >= 2) ? -1.0 : 1.0;
+                const double n0 = sg * dy / len, n1 = -sg * dx / len, ds = 0.5 * len;
+                const int nb = nbr[4 * (size_t)e + sd], inf = info[4 * (size_t)e + sd];
+                for (int k = 0; k < N; ++k) {
+                    trace_coef(e, sd, k, &tro[(size_t)2 * v2 * k]);
+                    if (nb >= 0) trace_coef(nb, inf & 3, ((inf >> 2) & 1) ? (N - 1 - k) : k, &trb[(size_t)2 * v2 * k]);
+                }
+                for (int q = 0; q < nqe; ++q) {
+                    double* H = &Hq[(((size_t)e * 4 + sd) * nqe + q) * 2 * v2];
+                    const double wds = we[q] * ds;
+                    for (int t = 0; t < 2 * v2; ++t) {
+                        double a = 0.0;
+                        for (int k = 0; k < N; ++k) a += tabe[q + nqe * k] * tro[(size_t)2 * v2 * k + t];
+                        Af[t] = a;
+                    }
+                    normal_matrix(Af, n0, n1);
+                    for (int t = 0; t < v2; ++t) H[t] = wds * (-0.5 * C[t] - 0.5 * absC[t]);
+                    if (nb >= 0) {
+                        for (int t = 0; t < 2 * v2; ++t) {
+                            double a = 0.0;
+                            for (int k = 0; k < N; ++k) a += tabe[q + nqe * k] * trb[(size_t)2 * v2 * k + t];
+                            Af[t] = a;
+                        }
+                        normal_matrix(Af, n0, n1);
+                        for (int t = 0; t < v2; ++t) H[v2 + t] = wds * (-0.5 * C[t] + 0.5 * absC[t]);
+                    }
+                }
+            }
+        }
+        auto adv = std::make_unique<wdg_advection_s>();
+        adv->op = op;
+        adv->nv = m;
+        adv->cc = cc ? 1 : 0;
+        adv->exact = true;
+        adv->nqv = nqv;
+        adv->nqe = nqe;
+        auto upload = [&](double*& d, const std::vector<double>& h, const char* what) {
+            d = static_cast<double*>(dmalloc(sizeof(double) * h.size()));
+            hip_check(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice), what);
+        };
+        upload(adv->d_op, opt, "upload volume table");
+        upload(adv->d_H, Hq, "upload face table");
+        upload(adv->d_tabv, tabv, "upload volume rule");
+        upload(adv->d_tabe, tabe, "upload edge rule");
+        *out = adv.release();
+    });
+}
+
 static int advection_launch(wdg_advection adv, const void* d_u, void* d_divF, bool volume, bool interior,
                             bool boundary, void* stream)
 {
@@ -1697,6 +1855,14 @@ static int advection_launch(wdg_advection adv, const void* d_u, void* d_divF, bo
         wdg_wave2d op = adv->op;
         check_op(op);
         const hipStream_t s = as_stream(stream);
+        if (adv->exact) {
+            wdg::launch_advect_q(op->n, adv->nv, adv->nqv, adv->nqe, static_cast<const double*>(d_u),
+                                 static_cast<double*>(d_divF), adv->d_op, adv->d_tabv, op->d_nbr, op->d_info,
+                                 adv->d_H, adv->d_tabe, op->face_interp(), op->nel, volume ? 1 : 0, interior ? 1 : 0,
+                                 boundary ? 1 : 0, s);
+            hip_check(hipGetLastError(), "advection kernel launch");
+            return;
+        }
         if (volume) {
             wdg::DivFArgs a{};
             a.geom = op->d_geom;

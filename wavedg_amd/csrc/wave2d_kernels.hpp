@@ -719,6 +719,18 @@ void launch_divf_linear(int n, int f32, const DivFArgs& a, hipStream_t s);
 // P: face interpolation phi_m(-1) of a Gauss-Legendre basis, or null.
 void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t* nbr, const int32_t* info,
                         const double* H, const double* P, int nel, int interior, int boundary, hipStream_t s);
+// Advection<false> (advect.hip): volume term on nq_volume^2 points with the
+// metric-weighted coefficient table op (2, nv, nv, nq, nq, nel) and tabv = P,
+// D (nq, n); face terms on nq_edge points with Hq (2, nv, nv) per (element,
+// side, point) and tabe = P (nq, n); P = the Gauss-Legendre face
+// interpolation or null.  The launches need advect_*_q_lds bytes of dynamic
+// LDS (at most 64 KiB, checked by the caller).
+size_t advect_volume_q_lds(int n, int nq, int nv);
+size_t advect_face_q_lds(int n, int nq, int nv);
+void launch_advect_q(int n, int nv, int nq_volume, int nq_edge, const double* u, double* y, const double* op,
+                     const double* tabv, const int32_t* nbr, const int32_t* info, const double* Hq,
+                     const double* tabe, const double* P, int nel, int volume, int interior, int boundary,
+                     hipStream_t s);
 // y -= w ds LLF(n, q_own, q_other) at every node of the selected sides for the
 // compressible Euler flux (advect.hip); boundary sides take q_other = q_own.
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,

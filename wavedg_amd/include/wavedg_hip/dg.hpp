@@ -355,16 +355,22 @@ private:
 // (Advection.hpp:142-270): n_var <= 8 variables, coefficient `a` of shape
 // (n_var, n_var, 2) if constant_coefficient else (n_var, n_var, 2, n, n,
 // n_elem), column-major as in the reference; upwind flux.  n.A must be
-// symmetric at every face node.  Only the approx_quad = true operators exist
-// on the GPU.
+// symmetric at every face node.  <false> integrates on Gauss-Legendre rules of
+// quad->n points (the reference's automatic sizes when quad is null).
 namespace detail {
 class AdvectionHandle {
 public:
-    AdvectionHandle(const Mesh2D& mesh, const QuadratureRule* basis, int n_var, const double* a, bool cc)
+    AdvectionHandle(const Mesh2D& mesh, const QuadratureRule* basis, int n_var, const double* a, bool cc,
+                    bool approx_quad, const QuadratureRule* quad)
         : op_(mesh, basis, nullptr, 1.0, 0.5, Precision::F64), n_var_(n_var)
     {
         wdg_advection h = nullptr;
-        check(wdg_hip_advection_create(op_.get(), n_var, a, cc ? 1 : 0, &h));
+        if (approx_quad) {
+            check(wdg_hip_advection_create(op_.get(), n_var, a, cc ? 1 : 0, &h));
+        } else {
+            const int nq = exact_rule_size(quad);
+            check(wdg_hip_advection_create_exact(op_.get(), n_var, a, cc ? 1 : 0, nq, nq, &h));
+        }
         h_.reset(h);
     }
     wdg_advection get() const { return h_.get(); }
@@ -383,14 +389,11 @@ private:
 
 template <bool ApproxQuadrature>
 class Advection : public Operator {
-    static_assert(ApproxQuadrature, "dg::hip: only Advection<true> is implemented on the GPU");
-
 public:
     Advection(int n_var, const Mesh2D& mesh, const QuadratureRule* basis, const double* a, bool constant_coefficient,
               const QuadratureRule* quad = nullptr)
-        : h_(mesh, basis, n_var, a, constant_coefficient)
+        : h_(mesh, basis, n_var, a, constant_coefficient, ApproxQuadrature, quad)
     {
-        (void)quad;
     }
     void action(const double* u, double* divF) const override
     {
@@ -414,14 +417,11 @@ private:
 
 template <bool ApproxQuadrature>
 class AdvectionHomogeneousBC : public Operator {
-    static_assert(ApproxQuadrature, "dg::hip: only AdvectionHomogeneousBC<true> is implemented on the GPU");
-
 public:
     AdvectionHomogeneousBC(int n_var, const Mesh2D& mesh, const QuadratureRule* basis, const double* a,
                            bool constant_coefficient, const QuadratureRule* quad = nullptr)
-        : h_(mesh, basis, n_var, a, constant_coefficient)
+        : h_(mesh, basis, n_var, a, constant_coefficient, ApproxQuadrature, quad)
     {
-        (void)quad;
     }
     void action(const double* u, double* divF) const override
     {
