@@ -387,11 +387,11 @@ def euler_rate(torch, capi, mesh, n, reps: int):
     return res
 
 
-def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int, approx_quad: bool = True):
+def cpu_baseline(c, n, bc, u0, dt, threads: int, steps: int, approx_quad: bool = True, legendre: bool = False):
     from oracle import oracle as O
 
     om = O.Mesh(c["vertices"], c["elements"])
-    ow = O.Wave(om, n, bc, approx_quad=approx_quad)
+    ow = O.Wave(om, n, bc, approx_quad=approx_quad, legendre=legendre)
     O.set_threads(threads)
     t0 = time.perf_counter()
     ow.rk4(u0, dt, steps)
@@ -618,6 +618,21 @@ def main() -> int:
         print(f"[bench] legendre basis: {secl:.3f} s, {avgl*1e3:.1f} us/stage ({achl:.0f} GB/s)", file=sys.stderr,
               flush=True)
         del wl, ul
+        if args.cpu_steps > 0 and args.exact_cpu_nx > 0:
+            from wavedg_amd import meshgen as _mg
+
+            cs = _mg.config2(nx=args.exact_cpu_nx)
+            ms = capi.Mesh(cs["vertices"], cs["elements"])
+            nodes = ms.nodes(n)
+            us = _mg.nodal_state(nodes, _mg.gaussian_sum(nodes, cs["centers"], cs["sigmas"], cs["amps"]))
+            bcs = np.full(ms.n_boundary, cs["bc_code"], dtype=np.int32)
+            vl, sl = cpu_baseline(cs, n, bcs, us, 0.5 * ms.min_h() / n**2, 1, args.cpu_steps, legendre=True)
+            out["legendre_basis"]["cpu_baseline"] = {
+                "value": vl, "unit": "DoF*stage/s", "cores": 1, "kind": "port",
+                "sample": f"CPU oracle on the Gauss-Legendre basis, same workload type on {args.exact_cpu_nx}^2 "
+                          f"quads, {args.cpu_steps} RK4 step(s), {sl:.1f} s",
+            }
+            print(f"[bench] legendre cpu: {vl:.3e} DoF*stage/s", file=sys.stderr, flush=True)
 
     if args.euler_reps > 0:
         out["euler"] = euler_rate(torch, capi, mesh, n, args.euler_reps)

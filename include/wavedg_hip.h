@@ -229,7 +229,9 @@ int wdg_hip_flux_create(wdg_wave2d op, int n_var, const char *source, wdg_flux *
 int wdg_hip_flux_create_exact(wdg_wave2d op, int n_var, const char *source, int nq_volume, int nq_edge,
                               wdg_flux *out);
 /* compile now rather than on first use: which 0 = flux, 1 = numerical_flux;
- * a source that does not compile returns WDG_ERR_INVALID with the compiler log */
+ * a source that does not compile returns WDG_ERR_INVALID with the compiler log.
+ * Call it before capturing wdg_hip_divf2d / wdg_hip_edgefluxf2d into a HIP
+ * graph: the first use loads a code module, which a capture cannot record. */
 int wdg_hip_flux_compile(wdg_flux fx, int which);
 /* compile a source for (n_colloc, n_var, basis_type) without a device or a
  * handle, to check it: WDG_ERR_INVALID with the compiler log if it fails */
