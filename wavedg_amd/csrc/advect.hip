@@ -35,10 +35,12 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
     constexpr int NN = N * N;
     constexpr int kOut = kMaxVar * NN > 64 * N ? kMaxVar * NN : 64 * N;
     __shared__ double s_out[kOut];
+    __shared__ int s_any[64];  // the element has a side in this launch
     const int per = N * nv, E = per < 64 ? 64 / per : 1;
     const int tid = threadIdx.x, e0 = blockIdx.x * E, vn = nv * NN, v2 = nv * nv;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     for (int k = tid; k < E * vn; k += 64) s_out[k] = 0.0;
+    if (tid < E) s_any[tid] = 0;
     __syncthreads();
     for (int s = 0; s < 4; ++s) {
         for (int it = tid; it < E * per; it += 64) {
@@ -47,6 +49,7 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
             const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
             const bool is_int = nb >= 0;
             if (is_int ? !interior : !boundary) continue;
+            s_any[el] = 1;
             const int node = side_node<N>(s, k);
             const double* uA = u + (size_t)nv * ((size_t)e * NN + node);
             const double* Hk = H + (((size_t)e * 4 + s) * N + k) * 2 * v2;
@@ -85,8 +88,11 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
         }
         __syncthreads();
     }
+    // elements without a side in this launch (e.g. away from the boundary)
+    // keep y untouched
     double* yb = y + (size_t)e0 * vn;
-    for (int k = tid; k < nloc * vn; k += 64) yb[k] += s_out[k];
+    for (int k = tid; k < nloc * vn; k += 64)
+        if (s_any[k / vn]) yb[k] += s_out[k];
 }
 
 // Compressible Euler flux and local Lax-Friedrichs flux of examples/cl2d.cpp
@@ -144,9 +150,11 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
 {
     constexpr int NN = N * N, E = 64 / N;
     __shared__ double s_out[E * 4 * NN];
+    __shared__ int s_any[E];  // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     for (int k = tid; k < E * 4 * NN; k += 64) s_out[k] = 0.0;
+    if (tid < E) s_any[tid] = 0;
     __syncthreads();
     const int el = tid / N, k = tid - el * N, e = e0 + el;
     const bool on = tid < E * N && el < nloc;
@@ -155,6 +163,7 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
             const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
             const bool is_int = nb >= 0;
             if (is_int ? interior : boundary) {
+                s_any[el] = 1;
                 const double* g = geom + 6 * (size_t)e;
                 const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
                 const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
@@ -192,7 +201,8 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
         __syncthreads();
     }
     double* yb = y + (size_t)e0 * 4 * NN;
-    for (int t = tid; t < nloc * 4 * NN; t += 64) yb[t] += s_out[t];
+    for (int t = tid; t < nloc * 4 * NN; t += 64)
+        if (s_any[t / (4 * NN)]) yb[t] += s_out[t];
 }
 
 // Advection<false> volume term (Div<false>, Div.cpp:312-360, 469-588): u
@@ -282,9 +292,11 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
     double* sA = s_out + E * nv * NN;      // E N nv
     double* sB = sA + E * N * nv;          // E N nv
     double* sC = sB + E * N * nv;          // E nq nv
+    __shared__ int s_any[64];  // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     for (int t = tid; t < E * nv * NN; t += 64) s_out[t] = 0.0;
+    if (tid < E) s_any[tid] = 0;
     for (int s = 0; s < 4; ++s) {
         __syncthreads();
         for (int t = tid; t < E * N; t += 64) {
@@ -292,6 +304,7 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
             if (el >= nloc) continue;
             const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
             if (nb >= 0 ? !interior : !boundary) continue;
+            s_any[el] = 1;
             const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
             for (int d = 0; d < nv; ++d) {
                 const double* qa = u + (size_t)nv * ((size_t)e * NN + side_node<N>(s, k)) + d;
@@ -353,7 +366,8 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
     }
     __syncthreads();
     double* yb = y + (size_t)e0 * nv * NN;
-    for (int t = tid; t < nloc * nv * NN; t += 64) yb[t] += s_out[t];
+    for (int t = tid; t < nloc * nv * NN; t += 64)
+        if (s_any[t / (nv * NN)]) yb[t] += s_out[t];
 }
 
 template <int N>

@@ -139,9 +139,11 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
     using namespace wdg_rtc;
     constexpr int E = 64 / N;
     __shared__ double s_out[E * NV * NN];
+    __shared__ int s_any[E];  // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     for (int t = tid; t < E * NV * NN; t += 64) s_out[t] = 0.0;
+    if (tid < E) s_any[tid] = 0;
     __syncthreads();
     const int el = tid / N, k = tid - el * N, e = e0 + el;
     const bool on = tid < E * N && el < nloc;
@@ -153,6 +155,7 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
             const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
             const bool is_int = nb >= 0;
             if (is_int ? interior : boundary) {
+                s_any[el] = 1;
                 const double* c = corners + 8 * (size_t)e;
                 // side vector in traversal order (sides 2, 3 run 3->2, 0->3)
                 const int a0 = s == 0 ? 0 : s == 1 ? 1 : s == 2 ? 3 : 0;
@@ -202,7 +205,8 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
         __syncthreads();
     }
     double* yb = out + (size_t)e0 * NV * NN;
-    for (int t = tid; t < nloc * NV * NN; t += 64) yb[t] += s_out[t];
+    for (int t = tid; t < nloc * NV * NN; t += 64)
+        if (s_any[t / (NV * NN)]) yb[t] += s_out[t];
 }
 #endif
 
@@ -300,7 +304,7 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double
     constexpr int NQ = WDG_NQ, W = NQ > N ? NQ : N, E = 64 / W;
     __shared__ double s_out[E * NV * NN];
     __shared__ double sA[E * N * NV], sB[E * N * NV], sC[E * NQ * NV];
-    __shared__ int s_ok[E];
+    __shared__ int s_ok[E], s_any[E];
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     const double* xq = tab;
@@ -308,6 +312,7 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double
     const double* Pq = tab + 2 * NQ;
     const double* Pf = basis + 2 * N + NN;  // Gauss-Legendre face interpolation
     for (int t = tid; t < E * NV * NN; t += 64) s_out[t] = 0.0;
+    if (tid < E) s_any[tid] = 0;
     for (int s = 0; s < 4; ++s) {
         __syncthreads();  // the previous side's sA, sB, sC are consumed
         // traces at the side nodes in this element's order
@@ -318,6 +323,7 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double
             const bool on = nb >= 0 ? interior : boundary;
             if (k == 0) s_ok[el] = on;
             if (!on) continue;
+            s_any[el] = 1;
             side_trace(u, e, s, k, Pf, sA + (el * N + k) * NV);
             if (nb >= 0) side_trace(u, nb, inf & 3, ((inf >> 2) & 1) ? (N - 1 - k) : k, Pf, sB + (el * N + k) * NV);
         }
@@ -386,7 +392,8 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double
     }
     __syncthreads();
     double* yb = out + (size_t)e0 * NV * NN;
-    for (int t = tid; t < nloc * NV * NN; t += 64) yb[t] += s_out[t];
+    for (int t = tid; t < nloc * NV * NN; t += 64)
+        if (s_any[t / (NV * NN)]) yb[t] += s_out[t];
 }
 #endif
 )WDGRTC";
