@@ -210,32 +210,42 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
 // op(., 1) u there, projected back with D and P.  op (2, nv, nv, nq, nq, nel)
 // holds the metric-weighted coefficient w_i w_j (y_eta A_x - x_eta A_y,
 // -y_xi A_x + x_xi A_y) of every point, as the reference's op_2d table.
-// tab = P (nq, N) then D (nq, N).  One element per 256-lane workgroup; the
-// dynamic LDS holds Pu or Df/Pg, and F.
+// tab = P (nq, N) then D (nq, N).  E elements per 256-lane workgroup (as many
+// as the largest pass fills); the dynamic LDS holds, per element, Pu or
+// Df/Pg, and F.
 template <int N>
 __global__ __launch_bounds__(256) void advect_volume_q_kernel(const double* __restrict__ u, double* __restrict__ out,
                                                               const double* __restrict__ op,
-                                                              const double* __restrict__ tab, int nq, int nv)
+                                                              const double* __restrict__ tab, int nq, int nv, int E,
+                                                              int nel)
 {
     constexpr int NN = N * N;
     extern __shared__ double s_dyn[];
-    const int e = blockIdx.x, tid = threadIdx.x, v2 = nv * nv;
-    const int spu = N * nv * nq, sdp = 2 * nq * nv * N;
-    double* sW = s_dyn;                             // Pu, then Df and Pg
-    double* sF = s_dyn + (spu > sdp ? spu : sdp);  // (2, nq, nv, nq)
-    const double* P = tab;
-    const double* D = tab + nq * N;
-    const double* ue = u + (size_t)nv * NN * e;
-    for (int t = tid; t < spu; t += 256) {
-        const int l = t % N, d = (t / N) % nv, i = t / (N * nv);
+    const int tid = threadIdx.x, v2 = nv * nv, e0 = blockIdx.x * E;
+    const int nloc = nel - e0 < E ? nel - e0 : E;
+    const int spu = N * nv * nq, sdp = 2 * nq * nv * N, sw = spu > sdp ? spu : sdp, sf = 2 * nq * nv * nq;
+    // the rule's P and D in LDS after the per-element regions
+    double* sT = s_dyn + E * (sw + sf);
+    for (int t = tid; t < 2 * nq * N; t += 256) sT[t] = tab[t];
+    __syncthreads();
+    const double* P = sT;
+    const double* D = sT + nq * N;
+    // per element: sW (Pu, then Df and Pg) and sF
+    for (int t = tid; t < nloc * spu; t += 256) {
+        const int el = t / spu, r = t - el * spu;
+        const int l = r % N, d = (r / N) % nv, i = r / (N * nv);
+        const double* ue = u + (size_t)nv * NN * (e0 + el);
         double pu = 0.0;
         for (int k = 0; k < N; ++k) pu += P[i + nq * k] * ue[d + nv * (k + N * l)];
-        sW[t] = pu;
+        s_dyn[el * (sw + sf) + r] = pu;
     }
     __syncthreads();
-    for (int t = tid; t < nq * nq * nv; t += 256) {
-        const int d = t % nv, i = (t / nv) % nq, j = t / (nv * nq);
-        const double* o = op + (size_t)2 * v2 * (i + nq * (j + (size_t)nq * e));
+    for (int t = tid; t < nloc * nq * nq * nv; t += 256) {
+        const int el = t / (nq * nq * nv), r = t - el * nq * nq * nv;
+        const int d = r % nv, i = (r / nv) % nq, j = r / (nv * nq);
+        const double* sW = s_dyn + el * (sw + sf);
+        double* sF = s_dyn + el * (sw + sf) + sw;
+        const double* o = op + (size_t)2 * v2 * (i + nq * (j + (size_t)nq * (e0 + el)));
         double f = 0.0, g = 0.0;
         for (int c = 0; c < nv; ++c) {
             double uq = 0.0;
@@ -247,25 +257,30 @@ __global__ __launch_bounds__(256) void advect_volume_q_kernel(const double* __re
         sF[1 + 2 * (i + nq * (d + nv * j))] = g;
     }
     __syncthreads();
-    for (int t = tid; t < nq * nv * N; t += 256) {
-        const int j = t % nq, d = (t / nq) % nv, k = t / (nq * nv);
+    for (int t = tid; t < nloc * nq * nv * N; t += 256) {
+        const int el = t / (nq * nv * N), r = t - el * nq * nv * N;
+        const int j = r % nq, d = (r / nq) % nv, k = r / (nq * nv);
+        double* sW = s_dyn + el * (sw + sf);
+        const double* sF = sW + sw;
         double df = 0.0, pg = 0.0;
         for (int i = 0; i < nq; ++i) {
             df += D[i + nq * k] * sF[0 + 2 * (i + nq * (d + nv * j))];
             pg += P[i + nq * k] * sF[1 + 2 * (i + nq * (d + nv * j))];
         }
-        sW[2 * t] = df;
-        sW[2 * t + 1] = pg;
+        sW[2 * r] = df;
+        sW[2 * r + 1] = pg;
     }
     __syncthreads();
-    for (int t = tid; t < NN * nv; t += 256) {
-        const int d = t % nv, k = (t / nv) % N, l = t / (nv * N);
+    for (int t = tid; t < nloc * NN * nv; t += 256) {
+        const int el = t / (NN * nv), r = t - el * NN * nv;
+        const int d = r % nv, k = (r / nv) % N, l = r / (nv * N);
+        const double* sW = s_dyn + el * (sw + sf);
         double dF = 0.0;
         for (int j = 0; j < nq; ++j) {
             const int b = 2 * (j + nq * (d + nv * k));
             dF += P[j + nq * l] * sW[b] + D[j + nq * l] * sW[b + 1];
         }
-        out[(size_t)nv * NN * e + t] += dF;
+        out[(size_t)nv * NN * e0 + t] += dF;
     }
 }
 
@@ -292,10 +307,12 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
     double* sA = s_out + E * nv * NN;      // E N nv
     double* sB = sA + E * N * nv;          // E N nv
     double* sC = sB + E * N * nv;          // E nq nv
+    double* sP = sC + E * nq * nv;         // the edge rule's P (nq, N)
     __shared__ int s_any[64];  // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
     for (int t = tid; t < E * nv * NN; t += 64) s_out[t] = 0.0;
+    for (int t = tid; t < nq * N; t += 64) sP[t] = tab[t];
     if (tid < E) s_any[tid] = 0;
     for (int s = 0; s < 4; ++s) {
         __syncthreads();
@@ -339,8 +356,8 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
             for (int cc = 0; cc < nv; ++cc) {
                 double a = 0.0, b = 0.0;
                 for (int k = 0; k < N; ++k) {
-                    a += tab[q + nq * k] * sA[(el * N + k) * nv + cc];
-                    b += tab[q + nq * k] * sB[(el * N + k) * nv + cc];
+                    a += sP[q + nq * k] * sA[(el * N + k) * nv + cc];
+                    b += sP[q + nq * k] * sB[(el * N + k) * nv + cc];
                 }
                 c += H[d + nv * cc] * a + H[v2 + d + nv * cc] * b;
             }
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
             const int nb = nbr[4 * (size_t)e + s];
             if (nb >= 0 ? !interior : !boundary) continue;
             double fl = 0.0;
-            for (int q = 0; q < nq; ++q) fl += sC[(el * nq + q) * nv + d] * tab[q + nq * k];
+            for (int q = 0; q < nq; ++q) fl += sC[(el * nq + q) * nv + d] * sP[q + nq * k];
             const int node = side_node<N>(s, k);
             if (P) {
                 // lane (k, d) owns variable d of the line through side node k
@@ -484,16 +501,29 @@ void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t
 #undef WDG_ADVECT_CASE
 }
 
+// elements per workgroup of the volume kernel: the largest pass fills the
+// 256 lanes at least once
+static int advect_volume_q_elems(int n, int nq, int nv)
+{
+    int w = nq * nq * nv;
+    w = w > n * nv * nq ? w : n * nv * nq;
+    w = w > n * n * nv ? w : n * n * nv;
+    const int e = 256 / w;
+    return e < 1 ? 1 : e;
+}
+
 size_t advect_volume_q_lds(int n, int nq, int nv)
 {
     const int spu = n * nv * nq, sdp = 2 * nq * nv * n;
-    return sizeof(double) * ((size_t)(spu > sdp ? spu : sdp) + 2 * (size_t)nq * nv * nq);
+    return sizeof(double) * ((size_t)advect_volume_q_elems(n, nq, nv) *
+                                 ((size_t)(spu > sdp ? spu : sdp) + 2 * (size_t)nq * nv * nq) +
+                             2 * (size_t)nq * n);
 }
 
 size_t advect_face_q_lds(int n, int nq, int nv)
 {
     const int W = nq > n ? nq : n, E = 64 / W;
-    return sizeof(double) * ((size_t)E * nv * n * n + 2 * (size_t)E * n * nv + (size_t)E * nq * nv);
+    return sizeof(double) * ((size_t)E * nv * n * n + 2 * (size_t)E * n * nv + (size_t)E * nq * nv + (size_t)nq * n);
 }
 
 void launch_advect_q(int n, int nv, int nq_volume, int nq_edge, const double* u, double* y, const double* op,
@@ -503,12 +533,13 @@ void launch_advect_q(int n, int nv, int nq_volume, int nq_edge, const double* u,
 {
     if (nel <= 0 || nv < 1 || nv > kMaxVar) return;
     const size_t lv = advect_volume_q_lds(n, nq_volume, nv), lf = advect_face_q_lds(n, nq_edge, nv);
+    const int Ev = advect_volume_q_elems(n, nq_volume, nv);
     const int W = nq_edge > n ? nq_edge : n, E = 64 / W;
 #define WDG_ADVQ_CASE(NV_)                                                                                       \
     case NV_:                                                                                                    \
         if (volume)                                                                                              \
-            hipLaunchKernelGGL(advect_volume_q_kernel<NV_>, dim3(nel), dim3(256), lv, s, u, y, op, tabv,          \
-                               nq_volume, nv);                                                                   \
+            hipLaunchKernelGGL(advect_volume_q_kernel<NV_>, dim3((nel + Ev - 1) / Ev), dim3(256), lv, s, u, y, op, \
+                               tabv, nq_volume, nv, Ev, nel);                                                    \
         if (interior || boundary)                                                                                \
             hipLaunchKernelGGL(advect_face_q_kernel<NV_>, dim3((nel + E - 1) / E), dim3(64), lf, s, u, y, nbr,     \
                                info, Hq, tabe, P, nq_edge, nv, nel, interior, boundary);                          \
