@@ -215,36 +215,48 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
 // Gauss-Legendre points, the contravariant fluxes of flux(x, u) there with
 // the metric of the bilinear map, then divF(d, k, l) += sum_j P(j, l) Df(j, d, k)
 // + D(j, l) Pg(j, d, k) with Df(j, d, k) = sum_i D(i, k) F~(i, d, j) and
-// Pg(j, d, k) = sum_i P(i, k) G~(i, d, j).  One element per 256-lane
-// workgroup; tab = xq[NQ], wq[NQ], P[NQ * N], D[NQ * N] (P(i, k) at i + NQ k).
+// Pg(j, d, k) = sum_i P(i, k) G~(i, d, j).  EQ elements per 256-lane
+// workgroup (as many as the largest pass fills); tab = xq[NQ], wq[NQ],
+// P[NQ * N], D[NQ * N] (P(i, k) at i + NQ k), kept in LDS.
+namespace wdg_rtc {
+constexpr int NQ = WDG_NQ;
+constexpr int SPU = N * NV * NQ, SDP = 2 * NQ * NV * N, SW = SPU > SDP ? SPU : SDP, SF = 2 * NQ * NV * NQ;
+constexpr int WMAX = (NQ * NQ > N * NQ ? NQ * NQ : N * NQ) * NV > NN * NV ? (NQ * NQ > N * NQ ? NQ * NQ : N * NQ) * NV
+                                                                          : NN * NV;
+constexpr int EQ = 256 / WMAX > 0 ? 256 / WMAX : 1;
+}  // namespace wdg_rtc
 extern "C" __global__ void __launch_bounds__(256) wdg_divf_q_rtc(const double* __restrict__ u,
                                                                   double* __restrict__ out,
                                                                   const double* __restrict__ corners,
                                                                   const double* __restrict__ tab, int nel)
 {
     using namespace wdg_rtc;
-    constexpr int NQ = WDG_NQ;
-    constexpr int SPU = N * NV * NQ, SDP = 2 * NQ * NV * N;
-    __shared__ double sF[2 * NQ * NV * NQ];
-    __shared__ double sW[SPU > SDP ? SPU : SDP];  // Pu, then Df and Pg
-    const int e = blockIdx.x, tid = threadIdx.x;
-    const double* xq = tab;
-    const double* wq = tab + NQ;
-    const double* P = tab + 2 * NQ;
-    const double* D = tab + 2 * NQ + NQ * N;
-    const double* ue = u + (size_t)NV * NN * e;
+    __shared__ double sM[EQ * (SW + SF)];  // per element: Pu, then Df and Pg; F
+    __shared__ double sT[2 * NQ + 2 * NQ * N];
+    const int tid = threadIdx.x, e0 = blockIdx.x * EQ;
+    const int nloc = nel - e0 < EQ ? nel - e0 : EQ;
+    for (int t = tid; t < 2 * NQ + 2 * NQ * N; t += 256) sT[t] = tab[t];
+    __syncthreads();
+    const double* xq = sT;
+    const double* wq = sT + NQ;
+    const double* P = sT + 2 * NQ;
+    const double* D = sT + 2 * NQ + NQ * N;
     // Pu(l, d, i) = sum_k P(i, k) u(d, k, l)
-    for (int t = tid; t < SPU; t += 256) {
-        const int l = t % N, d = (t / N) % NV, i = t / (N * NV);
+    for (int t = tid; t < nloc * SPU; t += 256) {
+        const int el = t / SPU, r = t - el * SPU;
+        const int l = r % N, d = (r / N) % NV, i = r / (N * NV);
+        const double* ue = u + (size_t)NV * NN * (e0 + el);
         double pu = 0.0;
         for (int k = 0; k < N; ++k) pu += P[i + NQ * k] * ue[d + NV * (k + N * l)];
-        sW[t] = pu;
+        sM[el * (SW + SF) + r] = pu;
     }
     __syncthreads();
-    for (int t = tid; t < NQ * NQ; t += 256) {
-        const int i = t % NQ, j = t / NQ;
+    for (int t = tid; t < nloc * NQ * NQ; t += 256) {
+        const int el = t / (NQ * NQ), r = t - el * NQ * NQ, i = r % NQ, j = r / NQ;
+        const double* sW = sM + el * (SW + SF);
+        double* sF = sM + el * (SW + SF) + SW;
         double X[2], J[4];
-        corner_map(corners + 8 * (size_t)e, xq[i], xq[j], X, J);
+        corner_map(corners + 8 * (size_t)(e0 + el), xq[i], xq[j], X, J);
         const double wij = wq[i] * wq[j];
         const double Ye = J[3] * wij, Xe = J[2] * wij, Yx = J[1] * wij, Xx = J[0] * wij;
         double uq[NV], Fq[2 * NV];
@@ -261,25 +273,30 @@ extern "C" __global__ void __launch_bounds__(256) wdg_divf_q_rtc(const double* _
         }
     }
     __syncthreads();
-    for (int t = tid; t < NQ * NV * N; t += 256) {
-        const int j = t % NQ, d = (t / NQ) % NV, k = t / (NQ * NV);
+    for (int t = tid; t < nloc * NQ * NV * N; t += 256) {
+        const int el = t / (NQ * NV * N), r = t - el * NQ * NV * N;
+        const int j = r % NQ, d = (r / NQ) % NV, k = r / (NQ * NV);
+        double* sW = sM + el * (SW + SF);
+        const double* sF = sW + SW;
         double df = 0.0, pg = 0.0;
         for (int i = 0; i < NQ; ++i) {
             df += D[i + NQ * k] * sF[0 + 2 * (i + NQ * (d + NV * j))];
             pg += P[i + NQ * k] * sF[1 + 2 * (i + NQ * (d + NV * j))];
         }
-        sW[2 * t] = df;
-        sW[2 * t + 1] = pg;
+        sW[2 * r] = df;
+        sW[2 * r + 1] = pg;
     }
     __syncthreads();
-    for (int t = tid; t < NN * NV; t += 256) {
-        const int d = t % NV, k = (t / NV) % N, l = t / (NV * N);
+    for (int t = tid; t < nloc * NN * NV; t += 256) {
+        const int el = t / (NN * NV), r = t - el * NN * NV;
+        const int d = r % NV, k = (r / NV) % N, l = r / (NV * N);
+        const double* sW = sM + el * (SW + SF);
         double dF = 0.0;
         for (int j = 0; j < NQ; ++j) {
             const int b = 2 * (j + NQ * (d + NV * k));
             dF += P[j + NQ * l] * sW[b] + D[j + NQ * l] * sW[b + 1];
         }
-        out[(size_t)NV * NN * e + t] += dF;
+        out[(size_t)NV * NN * e0 + t] += dF;
     }
 }
 #endif
@@ -470,6 +487,14 @@ bool rtc_compile(const std::string& user, int n, int nv, int leg, int kind, int 
 
 }  // namespace
 
+// elements per workgroup of wdg_divf_q_rtc (EQ there)
+int rtc_divf_q_elems(int n, int nv, int nq)
+{
+    const int a = nq * nq > n * nq ? nq * nq : n * nq;
+    const int w = a * nv > n * n * nv ? a * nv : n * n * nv;
+    return 256 / w > 0 ? 256 / w : 1;
+}
+
 struct RtcFlux {
     std::string source;
     int n = 0, nv = 0, leg = 0;
@@ -530,7 +555,7 @@ bool rtc_divf(RtcFlux* f, const double* u, double* out, const double* corners, c
 {
     if (!rtc_flux_prepare(f, 0, err)) return false;
     if (nel <= 0) return true;
-    const int NN = f->n * f->n, E = f->nqv > 0 ? 1 : 256 / NN;
+    const int NN = f->n * f->n, E = f->nqv > 0 ? rtc_divf_q_elems(f->n, f->nv, f->nqv) : 256 / NN;
     const double* tab = f->nqv > 0 ? f->d_tabv : basis;
     void* params[] = {&u, &out, &corners, &tab, &nel};
     const unsigned threads = f->nqv > 0 ? 256 : E * NN;

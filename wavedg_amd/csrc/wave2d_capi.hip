@@ -1956,7 +1956,10 @@ int wdg_hip_flux_create_exact(wdg_wave2d op, int n_var, const char* source, int 
         if (nq_edge <= 0) nq_edge = 2 * n;
         if (nq_volume > 20 || nq_edge > 20) fail(WDG_ERR_INVALID, "quadrature rules have at most 20 points");
         const int sp = n * n_var * nq_volume, sdp = 2 * nq_volume * n_var * n;
-        if ((size_t)(2 * nq_volume * n_var * nq_volume + std::max(sp, sdp)) * sizeof(double) > 65536)
+        const size_t lds = (size_t)wdg::rtc_divf_q_elems(n, n_var, nq_volume) *
+                               (2 * nq_volume * n_var * nq_volume + std::max(sp, sdp)) +
+                           2 * nq_volume + 2 * (size_t)nq_volume * n;
+        if (lds * sizeof(double) > 65536)
             fail(WDG_ERR_INVALID, "volume rule too large for n_var (local memory of one workgroup)");
         auto table = [&](int nq, bool with_d) {
             std::vector<double> xq, wq;

@@ -747,6 +747,8 @@ struct RtcFlux;
 bool rtc_build(const std::string& user, int n, int nv, int leg, int kind, int nq, std::vector<char>& code,
                std::string& err);
 RtcFlux* rtc_flux_create(const char* source, int n, int nv, int leg);
+// elements per workgroup of the quadrature volume kernel
+int rtc_divf_q_elems(int n, int nv, int nq);
 // switch to the quadrature forms: tabv = xq, wq, P, D (nqv points), tabe =
 // xq, wq, P (nqe points), P and D column-major (nq, n)
 bool rtc_flux_set_rules(RtcFlux* f, int nqv, const std::vector<double>& tabv, int nqe, const std::vector<double>& tabe,
