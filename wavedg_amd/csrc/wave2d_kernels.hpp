@@ -531,6 +531,13 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
 
     const int e0 = xcd_block(blockIdx.x, gridDim.x) * EPB;
     const int nloc = min(EPB, a.nel - e0);
+    if constexpr (BFACE && !VOL && !IFACE) {
+        // boundary terms alone (WaveBC::action): a workgroup whose elements
+        // have no boundary side adds nothing, so it leaves y untouched
+        const int t = threadIdx.x;
+        const bool bnd = t < 4 * nloc && a.nbr[4 * (size_t)e0 + t] < 0;
+        if (!__syncthreads_or(bnd)) return;
+    }
 
     // RK register / output / forcing / accumulator values, loaded early
     R pre0 = 0, pre1 = 0, pre2 = 0, fv0 = 0, fv1 = 0, fv2 = 0;
