@@ -33,3 +33,12 @@ def test_bad_source_reports_compiler_log():
                         3, 1, capi.GAUSS_LOBATTO, 0)
     with pytest.raises(capi.WdgError):
         capi.flux_check(fluxes.EULER_LLF, 10, 4, capi.GAUSS_LOBATTO, 0)
+
+
+def test_largest_collocated_case_compiles():
+    """n = 9, 8 variables: the collocated kernel compiles; the quadrature
+    form's 2n-point rule does not fit one workgroup's local memory, so the
+    check skips it (and Flux(..., approx_quad=False) refuses it)."""
+    src = ("__device__ void flux(const double* x, const double* u, double* F)"
+           " { for (int d = 0; d < 8; ++d) { F[d] = u[d]; F[8 + d] = 0.5 * u[d]; } }")
+    capi.flux_check(src, 9, 8, capi.GAUSS_LOBATTO, 0)

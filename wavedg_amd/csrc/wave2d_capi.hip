@@ -1923,6 +1923,16 @@ int wdg_hip_flux_create(wdg_wave2d op, int n_var, const char* source, wdg_flux* 
     });
 }
 
+// Local memory of the run-time compiled quadrature volume kernel (per
+// workgroup) fits the 64 KiB a workgroup may hold.
+static bool flux_q_fits(int n, int n_var, int nq)
+{
+    const int sp = n * n_var * nq, sdp = 2 * nq * n_var * n;
+    const size_t lds = (size_t)wdg::rtc_divf_q_elems(n, n_var, nq) * (2 * nq * n_var * nq + std::max(sp, sdp)) +
+                       2 * nq + 2 * (size_t)nq * n;
+    return lds * sizeof(double) <= 65536;
+}
+
 int wdg_hip_flux_check(const char* source, int n_colloc, int n_var, int basis_type, int which)
 {
     return guarded([&] {
@@ -1934,6 +1944,7 @@ int wdg_hip_flux_check(const char* source, int n_colloc, int n_var, int basis_ty
         std::vector<char> code;
         std::string err;
         for (int nq : {0, 2 * n_colloc}) {  // collocated and quadrature forms
+            if (nq && which == 0 && !flux_q_fits(n_colloc, n_var, nq)) continue;  // refused at creation anyway
             const int kind = (which == 0 ? 0 : 1) + (nq ? 2 : 0);
             if (!wdg::rtc_build(source, n_colloc, n_var, basis_type == WDG_GAUSS_LEGENDRE, kind, nq ? nq : n_colloc,
                                 code, err))
@@ -1955,11 +1966,7 @@ int wdg_hip_flux_create_exact(wdg_wave2d op, int n_var, const char* source, int 
         if (nq_volume <= 0) nq_volume = 2 * n;
         if (nq_edge <= 0) nq_edge = 2 * n;
         if (nq_volume > 20 || nq_edge > 20) fail(WDG_ERR_INVALID, "quadrature rules have at most 20 points");
-        const int sp = n * n_var * nq_volume, sdp = 2 * nq_volume * n_var * n;
-        const size_t lds = (size_t)wdg::rtc_divf_q_elems(n, n_var, nq_volume) *
-                               (2 * nq_volume * n_var * nq_volume + std::max(sp, sdp)) +
-                           2 * nq_volume + 2 * (size_t)nq_volume * n;
-        if (lds * sizeof(double) > 65536)
+        if (!flux_q_fits(n, n_var, nq_volume))
             fail(WDG_ERR_INVALID, "volume rule too large for n_var (local memory of one workgroup)");
         auto table = [&](int nq, bool with_d) {
             std::vector<double> xq, wq;
