@@ -39,6 +39,16 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
     const int per = N * nv, E = per < 64 ? 64 / per : 1;
     const int tid = threadIdx.x, e0 = blockIdx.x * E, vn = nv * NN, v2 = nv * nv;
     const int nloc = nel - e0 < E ? nel - e0 : E;
+    {
+        // a workgroup whose elements have no side in this launch (e.g. away
+        // from the boundary) has nothing to do
+        bool act = false;
+        for (int t = threadIdx.x; t < 4 * (nel - (int)blockIdx.x * E < E ? nel - (int)blockIdx.x * E : E); t += 64) {
+            const int nb = nbr[4 * ((size_t)blockIdx.x * E + t / 4) + (t % 4)];
+            act = act || (nb >= 0 ? interior != 0 : boundary != 0);
+        }
+        if (!__syncthreads_or(act)) return;
+    }
     for (int k = tid; k < E * vn; k += 64) s_out[k] = 0.0;
     if (tid < E) s_any[tid] = 0;
     __syncthreads();
@@ -135,10 +145,12 @@ __device__ __forceinline__ void trace4(const double* p, int s, const double* P, 
 // time derivative (cl2d.cpp:180-215), element by element: at node k of side
 // s, with this element's outward normal n, y -= w_k ds LLF(n, q_own, q_other)
 // (the flux is odd in (n, qL, qR), so el0 and el1 of an edge agree with the
-// reference's +-fh); on boundary sides q_other = q_own.  E = 64 / N elements
-// per workgroup, one lane per (element, side node), sides in order.  A
-// Gauss-Legendre basis (P non-null) interpolates the traces along the
-// element and lifts back along the same line.
+// reference's +-fh); on boundary sides q_other = q_own.  One lane per
+// (element, side, side node), all four sides at once, E = 64 / 4N elements
+// per workgroup; each lane leaves its contribution in LDS and the nodes then
+// add their sides in side order 0..3 (corner nodes two), so every output has
+// one writer.  A Gauss-Legendre basis (P non-null) interpolates the traces
+// along the element and lifts every side back along its lines with P.
 template <int N>
 __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const double* __restrict__ q,
                                                         double* __restrict__ y, const double* __restrict__ geom,
@@ -148,61 +160,80 @@ __global__ __launch_bounds__(64) void euler_face_kernel(double gamma, const doub
                                                         const double* __restrict__ P, int nel, int interior,
                                                         int boundary)
 {
-    constexpr int NN = N * N, E = 64 / N;
-    __shared__ double s_out[E * 4 * NN];
-    __shared__ int s_any[E];  // the element has a side in this launch
+    constexpr int NN = N * N, NF = 4 * N, E = 64 / NF > 0 ? 64 / NF : 1;
+    __shared__ double s_f[E * NF * 4];  // each side node's contribution, this element's order
+    __shared__ int s_any[E];            // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
-    for (int k = tid; k < E * 4 * NN; k += 64) s_out[k] = 0.0;
     if (tid < E) s_any[tid] = 0;
-    __syncthreads();
-    const int el = tid / N, k = tid - el * N, e = e0 + el;
-    const bool on = tid < E * N && el < nloc;
-    for (int s = 0; s < 4; ++s) {
-        if (on) {
-            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
-            const bool is_int = nb >= 0;
-            if (is_int ? interior : boundary) {
-                s_any[el] = 1;
-                const double* g = geom + 6 * (size_t)e;
-                const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
-                const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
-                const double dy = 2.0 * ((s & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
-                const double len = sqrt(dx * dx + dy * dy), sg = (s >= 2) ? -1.0 : 1.0;
-                const double n0 = sg * dy / len, n1 = -sg * dx / len, wds = basis[N + k] * (0.5 * len);
-                const int node = side_node<N>(s, k);
-                double qa[4], qb[4];
-                const double* pa = q + 4 * ((size_t)e * NN + node);
-                trace4<N>(pa, s, P, qa);
-                if (is_int) {
-                    const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                    const double* pb = q + 4 * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
-                    trace4<N>(pb, inf & 3, P, qb);
-                } else {
+    {
+        // a workgroup whose elements have no side in this launch (e.g. away
+        // from the boundary) has nothing to do
+        bool act = false;
+        for (int t = tid; t < 4 * nloc; t += 64) {
+            const int nb = nbr[4 * ((size_t)e0 + t / 4) + (t % 4)];
+            act = act || (nb >= 0 ? interior != 0 : boundary != 0);
+        }
+        if (!__syncthreads_or(act)) return;
+    }
+    const int el = tid / NF, f = tid - el * NF, s = f / N, k = f - (f / N) * N, e = e0 + el;
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    if (tid < E * NF && el < nloc) {
+        const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+        const bool is_int = nb >= 0;
+        if (is_int ? interior : boundary) {
+            s_any[el] = 1;
+            const double* g = geom + 6 * (size_t)e;
+            const double sh = (s == 0 || s == 3) ? -1.0 : 1.0;
+            const double dx = 2.0 * ((s & 1) ? g[2] + sh * g[4] : g[0] + sh * g[4]);
+            const double dy = 2.0 * ((s & 1) ? g[3] + sh * g[5] : g[1] + sh * g[5]);
+            const double len = sqrt(dx * dx + dy * dy), sg = (s >= 2) ? -1.0 : 1.0;
+            const double n0 = sg * dy / len, n1 = -sg * dx / len, wds = basis[N + k] * (0.5 * len);
+            const int node = side_node<N>(s, k);
+            double qa[4], qb[4];
+            trace4<N>(q + 4 * ((size_t)e * NN + node), s, P, qa);
+            if (is_int) {
+                const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                trace4<N>(q + 4 * ((size_t)nb * NN + side_node<N>(inf & 3, kk)), inf & 3, P, qb);
+            } else {
 #pragma unroll
-                    for (int d = 0; d < 4; ++d) qb[d] = qa[d];
-                }
-                double fa[4], ga[4], fb[4], gb[4], sa, sb;
-                euler_fg(gamma, qa, fa, ga, sa);
-                euler_fg(gamma, qb, fb, gb, sb);
-                const double c = sa > sb ? sa : sb;
+                for (int d = 0; d < 4; ++d) qb[d] = qa[d];
+            }
+            double fa[4], ga[4], fb[4], gb[4], sa, sb;
+            euler_fg(gamma, qa, fa, ga, sa);
+            euler_fg(gamma, qb, fb, gb, sb);
+            const double cm = sa > sb ? sa : sb;
 #pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    const double fh = 0.5 * n0 * (fa[d] + fb[d]) + 0.5 * n1 * (ga[d] + gb[d]) + 0.5 * c * (qa[d] - qb[d]);
-                    if (P) {
-                        const int st = inward_stride<N>(s);
-                        for (int m = 0; m < N; ++m) s_out[el * 4 * NN + 4 * (node + m * st) + d] -= (fh * wds) * P[m];
-                    } else {
-                        s_out[el * 4 * NN + 4 * node + d] -= fh * wds;
-                    }
-                }
+            for (int d = 0; d < 4; ++d) {
+                const double fh = 0.5 * n0 * (fa[d] + fb[d]) + 0.5 * n1 * (ga[d] + gb[d]) + 0.5 * cm * (qa[d] - qb[d]);
+                c[d] = -(fh * wds);
             }
         }
-        __syncthreads();
     }
-    double* yb = y + (size_t)e0 * 4 * NN;
-    for (int t = tid; t < nloc * 4 * NN; t += 64)
-        if (s_any[t / (4 * NN)]) yb[t] += s_out[t];
+    if (tid < E * NF) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) s_f[(el * NF + f) * 4 + d] = c[d];
+    }
+    __syncthreads();
+    for (int t = tid; t < nloc * 4 * NN; t += 64) {
+        const int el2 = t / (4 * NN), r = t - el2 * 4 * NN, node = r / 4, d = r - node * 4;
+        if (!s_any[el2]) continue;
+        const int i = node % N, j = node / N;
+        const double* F = s_f + el2 * NF * 4 + d;
+        double add;
+        if (P) {
+            add = P[j] * F[(0 * N + i) * 4] + P[N - 1 - i] * F[(1 * N + j) * 4] + P[N - 1 - j] * F[(2 * N + i) * 4] +
+                  P[i] * F[(3 * N + j) * 4];
+        } else {
+            if (!(i == 0 || j == 0 || i == N - 1 || j == N - 1)) continue;  // no side reaches an inner node
+            add = 0.0;
+            if (j == 0) add += F[(0 * N + i) * 4];
+            if (i == N - 1) add += F[(1 * N + j) * 4];
+            if (j == N - 1) add += F[(2 * N + i) * 4];
+            if (i == 0) add += F[(3 * N + j) * 4];
+        }
+        y[(size_t)(e0 + el2) * 4 * NN + r] += add;
+    }
 }
 
 // Advection<false> volume term (Div<false>, Div.cpp:312-360, 469-588): u
@@ -303,6 +334,16 @@ __global__ __launch_bounds__(64) void advect_face_q_kernel(const double* __restr
     constexpr int NN = N * N;
     extern __shared__ double s_dyn[];
     const int W = nq > N ? nq : N, E = 64 / W, v2 = nv * nv;
+    {
+        // a workgroup whose elements have no side in this launch (e.g. away
+        // from the boundary) has nothing to do
+        bool act = false;
+        for (int t = threadIdx.x; t < 4 * (nel - (int)blockIdx.x * E < E ? nel - (int)blockIdx.x * E : E); t += 64) {
+            const int nb = nbr[4 * ((size_t)blockIdx.x * E + t / 4) + (t % 4)];
+            act = act || (nb >= 0 ? interior != 0 : boundary != 0);
+        }
+        if (!__syncthreads_or(act)) return;
+    }
     double* s_out = s_dyn;                 // E nv NN
     double* sA = s_out + E * nv * NN;      // E N nv
     double* sB = sA + E * N * nv;          // E N nv
@@ -451,6 +492,9 @@ void launch_linear_functional(int n, int nq, int nv, const double* corners, cons
 #undef WDG_LF_CASE
 }
 
+// elements per workgroup of euler_face_kernel: one lane per side node
+static int euler_elems(int n) { return 64 / (4 * n) > 0 ? 64 / (4 * n) : 1; }
+
 void launch_euler_face(int n, double gamma, const double* q, double* y, const double* geom, const int32_t* nbr,
                        const int32_t* info, const double* basis, const double* P, int nel, int interior, int boundary,
                        hipStream_t s)
@@ -459,8 +503,8 @@ void launch_euler_face(int n, double gamma, const double* q, double* y, const do
     const dim3 block(64);
 #define WDG_EULER_CASE(NV_)                                                                                   \
     case NV_:                                                                                                 \
-        hipLaunchKernelGGL(euler_face_kernel<NV_>, dim3((nel + 64 / NV_ - 1) / (64 / NV_)), block, 0, s, gamma, q, \
-                           y, geom, nbr, info, basis, P, nel, interior, boundary);                            \
+        hipLaunchKernelGGL(euler_face_kernel<NV_>, dim3((nel + euler_elems(NV_) - 1) / euler_elems(NV_)), block, 0, \
+                           s, gamma, q, y, geom, nbr, info, basis, P, nel, interior, boundary);               \
         break;
     switch (n) {
         WDG_EULER_CASE(2)

@@ -127,7 +127,8 @@ extern "C" __global__ void __launch_bounds__(256) wdg_divf_rtc(const double* __r
 // scale * (-fh w ds) if el1, lifted into the side's node (or, for a
 // Gauss-Legendre basis, along its line with weights P).  Boundary sides take
 // the exterior state from exterior_state(), or a copy of the interior trace.
-// E = 64 / N elements per workgroup, one lane per (element, side node).
+// One lane per (element, side, side node), all sides at once, E = 64 / 4N
+// elements per workgroup; the nodes then add their sides in side order.
 extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* __restrict__ u,
                                                                    double* __restrict__ out,
                                                                    const double* __restrict__ corners,
@@ -137,76 +138,93 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_rtc(const double* 
                                                                    int interior, int boundary, double scale)
 {
     using namespace wdg_rtc;
-    constexpr int E = 64 / N;
-    __shared__ double s_out[E * NV * NN];
-    __shared__ int s_any[E];  // the element has a side in this launch
+    constexpr int NF = 4 * N, E = 64 / NF > 0 ? 64 / NF : 1;
+    __shared__ double s_f[E * NF * NV];  // each side node's contribution, this element's order
+    __shared__ int s_any[E];             // the element has a side in this launch
     const int tid = threadIdx.x, e0 = blockIdx.x * E;
     const int nloc = nel - e0 < E ? nel - e0 : E;
-    for (int t = tid; t < E * NV * NN; t += 64) s_out[t] = 0.0;
     if (tid < E) s_any[tid] = 0;
-    __syncthreads();
-    const int el = tid / N, k = tid - el * N, e = e0 + el;
-    const bool on = tid < E * N && el < nloc;
+    {
+        bool act = false;
+        for (int t = tid; t < 4 * nloc; t += 64) {
+            const int nb = nbr[4 * ((size_t)e0 + t / 4) + (t % 4)];
+            act = act || (nb >= 0 ? interior != 0 : boundary != 0);
+        }
+        if (!__syncthreads_or(act)) return;
+    }
     const double* xb = basis;
     const double* wb = basis + N;
     const double* P = basis + 2 * N + NN;
-    for (int s = 0; s < 4; ++s) {
-        if (on) {
-            const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
-            const bool is_int = nb >= 0;
-            if (is_int ? interior : boundary) {
-                s_any[el] = 1;
-                const double* c = corners + 8 * (size_t)e;
-                // side vector in traversal order (sides 2, 3 run 3->2, 0->3)
-                const int a0 = s == 0 ? 0 : s == 1 ? 1 : s == 2 ? 3 : 0;
-                const int a1 = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? 2 : 3;
-                const double dx = c[2 * a1] - c[2 * a0], dy = c[2 * a1 + 1] - c[2 * a0 + 1];
-                const double len = sqrt(dx * dx + dy * dy), sg = s >= 2 ? -1.0 : 1.0;
-                double nrm[2] = {sg * dy / len, -sg * dx / len};  // this element's outward normal
-                const double ds = 0.5 * len;
-                const double xi = s == 0 || s == 2 ? xb[k] : (s == 1 ? 1.0 : -1.0);
-                const double eta = s == 1 || s == 3 ? xb[k] : (s == 2 ? 1.0 : -1.0);
-                double X[2], J[4];
-                corner_map(c, xi, eta, X, J);
-                double uA[NV], uB[NV], fh[NV];
-                side_trace(u, e, s, k, P, uA);
-                if (is_int) {
-                    const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                    side_trace(u, nb, inf & 3, kk, P, uB);
-                } else {
+    const int el = tid / NF, f = tid - el * NF, s = f / N, k = f - (f / N) * N, e = e0 + el;
+    double c[NV];
+    for (int d = 0; d < NV; ++d) c[d] = 0.0;
+    if (tid < E * NF && el < nloc) {
+        const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
+        const bool is_int = nb >= 0;
+        if (is_int ? interior : boundary) {
+            s_any[el] = 1;
+            const double* cr = corners + 8 * (size_t)e;
+            // side vector in traversal order (sides 2, 3 run 3->2, 0->3)
+            const int a0 = s == 0 ? 0 : s == 1 ? 1 : s == 2 ? 3 : 0;
+            const int a1 = s == 0 ? 1 : s == 1 ? 2 : s == 2 ? 2 : 3;
+            const double dx = cr[2 * a1] - cr[2 * a0], dy = cr[2 * a1 + 1] - cr[2 * a0 + 1];
+            const double len = sqrt(dx * dx + dy * dy), sg = s >= 2 ? -1.0 : 1.0;
+            double nrm[2] = {sg * dy / len, -sg * dx / len};  // this element's outward normal
+            const double ds = 0.5 * len;
+            const double xi = s == 0 || s == 2 ? xb[k] : (s == 1 ? 1.0 : -1.0);
+            const double eta = s == 1 || s == 3 ? xb[k] : (s == 2 ? 1.0 : -1.0);
+            double X[2], J[4];
+            corner_map(cr, xi, eta, X, J);
+            double uA[NV], uB[NV], fh[NV];
+            side_trace(u, e, s, k, P, uA);
+            if (is_int) {
+                const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                side_trace(u, nb, inf & 3, kk, P, uB);
+            } else {
 #ifdef WDG_EXTERIOR_STATE
-                    exterior_state(X, nrm, uA, uB);
+                exterior_state(X, nrm, uA, uB);
 #else
-                    for (int d = 0; d < NV; ++d) uB[d] = uA[d];
+                for (int d = 0; d < NV; ++d) uB[d] = uA[d];
 #endif
-                }
-                const bool first = (inf >> 3) & 1;
-                const int ie = (!first && ((inf >> 2) & 1)) ? (N - 1 - k) : k;  // el0's node order
-                for (int d = 0; d < NV; ++d) fh[d] = 0.0;
-                if (first) {
-                    numerical_flux(X, nrm, uA, uB, fh);
-                } else {
-                    nrm[0] = -nrm[0];
-                    nrm[1] = -nrm[1];
-                    numerical_flux(X, nrm, uB, uA, fh);
-                }
-                const double sgn = first ? scale : -scale;
-                const int node = side_node(s, k), st = inward_stride(s);
-                for (int d = 0; d < NV; ++d) {
-                    const double fl = sgn * (fh[d] * wb[ie] * ds);
-                    if (WDG_LEG) {
-                        for (int m = 0; m < N; ++m) s_out[(el * NN + node + m * st) * NV + d] += fl * P[m];
-                    } else {
-                        s_out[(el * NN + node) * NV + d] += fl;
-                    }
-                }
             }
+            const bool first = (inf >> 3) & 1;
+            const int ie = (!first && ((inf >> 2) & 1)) ? (N - 1 - k) : k;  // el0's node order
+            for (int d = 0; d < NV; ++d) fh[d] = 0.0;
+            if (first) {
+                numerical_flux(X, nrm, uA, uB, fh);
+            } else {
+                nrm[0] = -nrm[0];
+                nrm[1] = -nrm[1];
+                numerical_flux(X, nrm, uB, uA, fh);
+            }
+            const double sgn = first ? scale : -scale;
+            for (int d = 0; d < NV; ++d) c[d] = sgn * (fh[d] * wb[ie] * ds);
         }
-        __syncthreads();
     }
+    if (tid < E * NF) {
+        for (int d = 0; d < NV; ++d) s_f[(el * NF + f) * NV + d] = c[d];
+    }
+    __syncthreads();
     double* yb = out + (size_t)e0 * NV * NN;
-    for (int t = tid; t < nloc * NV * NN; t += 64)
-        if (s_any[t / (NV * NN)]) yb[t] += s_out[t];
+    for (int t = tid; t < nloc * NV * NN; t += 64) {
+        const int el2 = t / (NV * NN), r = t - el2 * NV * NN, node = r / NV, d = r - node * NV;
+        if (!s_any[el2]) continue;
+        const int i = node % N, j = node / N;
+        const double* F = s_f + el2 * NF * NV + d;
+        double add;
+        if (WDG_LEG) {
+            add = P[j] * F[(0 * N + i) * NV] + P[N - 1 - i] * F[(1 * N + j) * NV] +
+                  P[N - 1 - j] * F[(2 * N + i) * NV] + P[i] * F[(3 * N + j) * NV];
+        } else {
+            if (!(i == 0 || j == 0 || i == N - 1 || j == N - 1)) continue;  // no side reaches an inner node
+            add = 0.0;
+            if (j == 0) add += F[(0 * N + i) * NV];
+            if (i == N - 1) add += F[(1 * N + j) * NV];
+            if (j == N - 1) add += F[(2 * N + i) * NV];
+            if (i == 0) add += F[(3 * N + j) * NV];
+        }
+        yb[t] += add;
+    }
 }
 #endif
 
@@ -319,6 +337,16 @@ extern "C" __global__ void __launch_bounds__(64) wdg_edgeflux_q_rtc(const double
 {
     using namespace wdg_rtc;
     constexpr int NQ = WDG_NQ, W = NQ > N ? NQ : N, E = 64 / W;
+    {
+        // a workgroup whose elements have no side in this launch (e.g. away
+        // from the boundary) has nothing to do
+        bool act = false;
+        for (int t = threadIdx.x; t < 4 * (nel - (int)blockIdx.x * E < E ? nel - (int)blockIdx.x * E : E); t += 64) {
+            const int nb = nbr[4 * ((size_t)blockIdx.x * E + t / 4) + (t % 4)];
+            act = act || (nb >= 0 ? interior != 0 : boundary != 0);
+        }
+        if (!__syncthreads_or(act)) return;
+    }
     __shared__ double s_out[E * NV * NN];
     __shared__ double sA[E * N * NV], sB[E * N * NV], sC[E * NQ * NV];
     __shared__ int s_ok[E], s_any[E];
@@ -584,7 +612,7 @@ bool rtc_edgeflux(RtcFlux* f, const double* u, double* out, const double* corner
         }
         return true;
     }
-    const int E = 64 / f->n;
+    const int E = 64 / (4 * f->n) > 0 ? 64 / (4 * f->n) : 1;  // wdg_edgeflux_rtc's elements per workgroup
     void* params[] = {&u, &out, &corners, &nbr, &info, &basis, &nel, &interior, &boundary, &scale};
     if (hipModuleLaunchKernel(f->edge.fn, (nel + E - 1) / E, 1, 1, 64, 1, 1, 0, s, params, nullptr) != hipSuccess) {
         err = "edge flux kernel launch failed";
