@@ -35,12 +35,14 @@ __device__ __forceinline__ void euler_flux(R gamma, const R* q, R* f, R* g)
     g[3] = v * (E + p);
 }
 
-template <int N, typename R>
+// VS >= n_var: the LDS stride between variables, so that few variables leave
+// room for more resident workgroups
+template <int N, typename R, int VS>
 __global__ __launch_bounds__(Shape<N>::NT) void divf_linear_kernel(const DivFArgs a)
 {
     constexpr int NN = N * N, EPB = Shape<N>::EPB, NT = Shape<N>::NT;
-    __shared__ R s_F[EPB * kMaxVar * NN];
-    __shared__ R s_G[EPB * kMaxVar * NN];
+    __shared__ R s_F[EPB * VS * NN];
+    __shared__ R s_G[EPB * VS * NN];
     __shared__ R s_D[NN];
     __shared__ double s_xw[2 * N];
     const int tid = threadIdx.x;
@@ -81,16 +83,16 @@ __global__ __launch_bounds__(Shape<N>::NT) void divf_linear_kernel(const DivFArg
                     fy = fma((R)a.B[d + kMaxVar * k], uv[k], fy);
                 }
             }
-            s_F[(el * kMaxVar + d) * NN + j * N + i] = Ye * fx - Xe * fy;
-            s_G[(el * kMaxVar + d) * NN + i * N + j] = Xx * fy - Yx * fx;
+            s_F[(el * VS + d) * NN + j * N + i] = Ye * fx - Xe * fy;
+            s_G[(el * VS + d) * NN + i * N + j] = Xx * fy - Yx * fx;
         }
     }
     __syncthreads();
     if (!active) return;
     R* out = static_cast<R*>(a.out) + (size_t)nv * (node + (size_t)NN * e);
     for (int d = 0; d < nv; ++d) {
-        const R* Fr = s_F + (el * kMaxVar + d) * NN + j * N;
-        const R* Gc = s_G + (el * kMaxVar + d) * NN + i * N;
+        const R* Fr = s_F + (el * VS + d) * NN + j * N;
+        const R* Gc = s_G + (el * VS + d) * NN + i * N;
         R r = 0;
 #pragma unroll
         for (int l = 0; l < N; ++l) {
@@ -105,10 +107,21 @@ template <int N>
 void launch_n(int f32, const DivFArgs& a, hipStream_t s)
 {
     const dim3 grid((a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB), block(Shape<N>::NT);
-    if (f32)
-        hipLaunchKernelGGL((divf_linear_kernel<N, float>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((divf_linear_kernel<N, double>), grid, block, 0, s, a);
+#define WDG_DIVF_VS(VS_)                                                                   \
+    if (f32)                                                                               \
+        hipLaunchKernelGGL((divf_linear_kernel<N, float, VS_>), grid, block, 0, s, a);     \
+    else                                                                                   \
+        hipLaunchKernelGGL((divf_linear_kernel<N, double, VS_>), grid, block, 0, s, a);
+    if (a.nv <= 1) {
+        WDG_DIVF_VS(1)
+    } else if (a.nv <= 2) {
+        WDG_DIVF_VS(2)
+    } else if (a.nv <= 4) {
+        WDG_DIVF_VS(4)
+    } else {
+        WDG_DIVF_VS(8)
+    }
+#undef WDG_DIVF_VS
 }
 
 }  // namespace
