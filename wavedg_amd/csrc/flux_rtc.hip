@@ -88,12 +88,15 @@ extern "C" __global__ void __launch_bounds__(256) wdg_divf_rtc(const double* __r
     using namespace wdg_rtc;
     constexpr int E = 256 / NN;
     __shared__ double sF[E * NV * NN], sG[E * NV * NN];
+    __shared__ double sB[2 * N + NN];  // x, w, D of the basis
     const int tid = threadIdx.x, el = tid / NN, node = tid - el * NN, i = node % N, j = node / N;
     const int e = blockIdx.x * E + el;
     const bool active = el < E && e < nel;
-    const double* xb = basis;
-    const double* wb = basis + N;
-    const double* D = basis + 2 * N;
+    for (int t = tid; t < 2 * N + NN; t += E * NN) sB[t] = basis[t];
+    __syncthreads();
+    const double* xb = sB;
+    const double* wb = sB + N;
+    const double* D = sB + 2 * N;
     if (active) {
         double X[2], J[4];
         corner_map(corners + 8 * (size_t)e, xb[i], xb[j], X, J);
