@@ -23,8 +23,10 @@ namespace {
 
 constexpr int kMaxVar = DivFArgs::kMaxVar;
 
-// E = max(1, 64 / (N nv)) elements per 64-lane workgroup, one lane per
-// (element, node of the side, variable) in each side round.
+// One lane per (element, side node, variable) for all four sides at once,
+// E = max(1, 64 / (4 N nv)) elements per 64-lane workgroup (lanes loop when
+// an element needs more); each lane leaves its contribution in LDS and the
+// nodes then add their sides in side order (only the nodes a side reaches).
 template <int N>
 __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restrict__ u, double* __restrict__ y,
                                                          const int32_t* __restrict__ nbr,
@@ -32,77 +34,85 @@ __global__ __launch_bounds__(64) void advect_face_kernel(const double* __restric
                                                          const double* __restrict__ H, const double* __restrict__ P,
                                                          int nv, int nel, int interior, int boundary)
 {
-    constexpr int NN = N * N;
-    constexpr int kOut = kMaxVar * NN > 64 * N ? kMaxVar * NN : 64 * N;
-    __shared__ double s_out[kOut];
-    __shared__ int s_any[64];  // the element has a side in this launch
-    const int per = N * nv, E = per < 64 ? 64 / per : 1;
+    constexpr int NN = N * N, NF = 4 * N;
+    constexpr int kF = NF * kMaxVar > 64 ? NF * kMaxVar : 64;
+    __shared__ double s_f[kF];  // E NF nv contributions (E NF nv <= max(64, NF nv))
+    __shared__ int s_any[64];   // the element has a side in this launch
+    const int per = NF * nv, E = per < 64 ? 64 / per : 1;
     const int tid = threadIdx.x, e0 = blockIdx.x * E, vn = nv * NN, v2 = nv * nv;
     const int nloc = nel - e0 < E ? nel - e0 : E;
+    if (tid < E) s_any[tid] = 0;
     {
         // a workgroup whose elements have no side in this launch (e.g. away
         // from the boundary) has nothing to do
         bool act = false;
-        for (int t = threadIdx.x; t < 4 * (nel - (int)blockIdx.x * E < E ? nel - (int)blockIdx.x * E : E); t += 64) {
-            const int nb = nbr[4 * ((size_t)blockIdx.x * E + t / 4) + (t % 4)];
+        for (int t = tid; t < 4 * nloc; t += 64) {
+            const int nb = nbr[4 * ((size_t)e0 + t / 4) + (t % 4)];
             act = act || (nb >= 0 ? interior != 0 : boundary != 0);
         }
         if (!__syncthreads_or(act)) return;
     }
-    for (int k = tid; k < E * vn; k += 64) s_out[k] = 0.0;
-    if (tid < E) s_any[tid] = 0;
-    __syncthreads();
-    for (int s = 0; s < 4; ++s) {
-        for (int it = tid; it < E * per; it += 64) {
-            const int el = it / per, r = it - el * per, k = r / nv, d = r - k * nv, e = e0 + el;
-            if (el >= nloc) continue;
+    for (int it = tid; it < E * per; it += 64) {
+        const int el = it / per, r = it - el * per, f = r / nv, d = r - f * nv, s = f / N, k = f - s * N;
+        const int e = e0 + el;
+        double acc = 0.0;
+        if (el < nloc) {
             const int nb = nbr[4 * (size_t)e + s], inf = info[4 * (size_t)e + s];
             const bool is_int = nb >= 0;
-            if (is_int ? !interior : !boundary) continue;
-            s_any[el] = 1;
-            const int node = side_node<N>(s, k);
-            const double* uA = u + (size_t)nv * ((size_t)e * NN + node);
-            const double* Hk = H + (((size_t)e * 4 + s) * N + k) * 2 * v2;
-            double acc = 0.0;
-            if (P) {
-                const int st = nv * inward_stride<N>(s);
-                for (int c = 0; c < nv; ++c) {
-                    double t = 0.0;
-                    for (int m = 0; m < N; ++m) t += uA[c + m * st] * P[m];
-                    acc += Hk[d + nv * c] * t;
-                }
-            } else {
-                for (int c = 0; c < nv; ++c) acc += Hk[d + nv * c] * uA[c];
-            }
-            if (is_int) {
-                const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                const double* uB = u + (size_t)nv * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
+            if (is_int ? interior : boundary) {
+                s_any[el] = 1;
+                const int node = side_node<N>(s, k);
+                const double* uA = u + (size_t)nv * ((size_t)e * NN + node);
+                const double* Hk = H + (((size_t)e * 4 + s) * N + k) * 2 * v2;
                 if (P) {
-                    const int st = nv * inward_stride<N>(inf & 3);
+                    const int st = nv * inward_stride<N>(s);
                     for (int c = 0; c < nv; ++c) {
                         double t = 0.0;
-                        for (int m = 0; m < N; ++m) t += uB[c + m * st] * P[m];
-                        acc += Hk[v2 + d + nv * c] * t;
+                        for (int m = 0; m < N; ++m) t += uA[c + m * st] * P[m];
+                        acc += Hk[d + nv * c] * t;
                     }
                 } else {
-                    for (int c = 0; c < nv; ++c) acc += Hk[v2 + d + nv * c] * uB[c];
+                    for (int c = 0; c < nv; ++c) acc += Hk[d + nv * c] * uA[c];
+                }
+                if (is_int) {
+                    const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
+                    const double* uB = u + (size_t)nv * ((size_t)nb * NN + side_node<N>(inf & 3, kk));
+                    if (P) {
+                        const int st = nv * inward_stride<N>(inf & 3);
+                        for (int c = 0; c < nv; ++c) {
+                            double t = 0.0;
+                            for (int m = 0; m < N; ++m) t += uB[c + m * st] * P[m];
+                            acc += Hk[v2 + d + nv * c] * t;
+                        }
+                    } else {
+                        for (int c = 0; c < nv; ++c) acc += Hk[v2 + d + nv * c] * uB[c];
+                    }
                 }
             }
-            if (P) {
-                // lane (k, d) owns variable d of the line through side node k
-                const int st = inward_stride<N>(s);
-                for (int m = 0; m < N; ++m) s_out[el * vn + d + nv * (node + m * st)] += acc * P[m];
-            } else {
-                s_out[el * vn + d + nv * node] += acc;
-            }
         }
-        __syncthreads();
+        s_f[(el * NF + f) * nv + d] = acc;
     }
-    // elements without a side in this launch (e.g. away from the boundary)
-    // keep y untouched
+    __syncthreads();
     double* yb = y + (size_t)e0 * vn;
-    for (int k = tid; k < nloc * vn; k += 64)
-        if (s_any[k / vn]) yb[k] += s_out[k];
+    for (int t = tid; t < nloc * vn; t += 64) {
+        const int el = t / vn, r = t - el * vn, node = r / nv, d = r - node * nv;
+        if (!s_any[el]) continue;
+        const int i = node % N, j = node / N;
+        const double* F = s_f + el * NF * nv + d;
+        double add;
+        if (P) {
+            add = P[j] * F[(0 * N + i) * nv] + P[N - 1 - i] * F[(1 * N + j) * nv] +
+                  P[N - 1 - j] * F[(2 * N + i) * nv] + P[i] * F[(3 * N + j) * nv];
+        } else {
+            if (!(i == 0 || j == 0 || i == N - 1 || j == N - 1)) continue;  // no side reaches an inner node
+            add = 0.0;
+            if (j == 0) add += F[(0 * N + i) * nv];
+            if (i == N - 1) add += F[(1 * N + j) * nv];
+            if (j == N - 1) add += F[(2 * N + i) * nv];
+            if (i == 0) add += F[(3 * N + j) * nv];
+        }
+        yb[t] += add;
+    }
 }
 
 // Compressible Euler flux and local Lax-Friedrichs flux of examples/cl2d.cpp
@@ -524,7 +534,7 @@ void launch_advect_face(int n, int nv, const double* u, double* y, const int32_t
                         const double* H, const double* P, int nel, int interior, int boundary, hipStream_t s)
 {
     if (nel <= 0 || nv < 1 || nv > kMaxVar) return;
-    const int per = n * nv, E = per < 64 ? 64 / per : 1;
+    const int per = 4 * n * nv, E = per < 64 ? 64 / per : 1;  // advect_face_kernel's elements per workgroup
     const dim3 grid((nel + E - 1) / E), block(64);
 #define WDG_ADVECT_CASE(NV_)                                                                                  \
     case NV_:                                                                                                 \
