@@ -16,7 +16,8 @@
  *
  * Layouts (column-major, first index fastest, as in the reference,
  * docs/docs.md:55-61):
- *   state        (3, n, n, n_elem)  (p, u, v) per Gauss-Lobatto node
+ *   state        (3, n, n, n_elem)  (p, u, v) per basis node (Gauss-Lobatto unless
+ *                                     the handle was made with a Gauss-Legendre basis)
  *   vertices     (2, n_vertices);   elements (4, n_elem), corners counter-clockwise
  *   H            (2, n, n, n_elem)  real/imag part of the Helmholtz solution
  *
@@ -131,8 +132,10 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int *bc, doub
  * traces by interpolation along the element and lifts face terms back with
  * the same weights, LegendreFaceProlongator (FaceProlongator.cpp:717-769,
  * 809-835, 908-934).  approx_quad = 0 needs precision WDG_F64; nq_* as in
- * wdg_hip_wave2d_create_exact (ignored for approx_quad = 1).  Batched
- * WaveHoltz, DivF2D, Advection and EdgeFluxF2D take Gauss-Lobatto handles. */
+ * wdg_hip_wave2d_create_exact (ignored for approx_quad = 1).  Every operator
+ * built on a handle (DivF2D, EdgeFluxF2D, flux functors, Advection, the load
+ * vector) follows its basis; batched WaveHoltz and the schedule autotuner
+ * take Gauss-Lobatto bases. */
 int wdg_hip_wave2d_create_basis(wdg_mesh mesh, int n_colloc, int basis_type, const int *bc, double a, double b,
                                 int precision, int approx_quad, int nq_volume, int nq_edge, int nq_mass,
                                 wdg_wave2d *out);
