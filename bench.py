@@ -623,7 +623,8 @@ def main() -> int:
 
     if args.legendre_steps > 0:
         wl, ul, secl, ms_kl, nkl = time_path(torch, capi, mesh, n, bc, u0, dt, capi.F64, args.legendre_steps,
-                                             min(args.warmup, 4), basis=capi.GAUSS_LEGENDRE)
+                                             min(args.warmup, 4), autotune=bool(args.autotune),
+                                             basis=capi.GAUSS_LEGENDRE)
         avgl = ms_kl / max(nkl, 1)
         achl = bytes64 / (avgl * 1e-3) / 1e9
         out["legendre_basis"] = {
@@ -638,6 +639,8 @@ def main() -> int:
             "achieved_GBs": achl,
             "frac": achl / HBM_PEAK_GBS,
         }
+        if getattr(wl, "tuned", None):
+            out["legendre_basis"]["schedule"] = wl.tuned
         print(f"[bench] legendre basis: {secl:.3f} s, {avgl*1e3:.1f} us/stage ({achl:.0f} GB/s)", file=sys.stderr,
               flush=True)
         del wl, ul

@@ -134,8 +134,8 @@ int wdg_hip_wave2d_create_exact(wdg_mesh mesh, int n_colloc, const int *bc, doub
  * 809-835, 908-934).  approx_quad = 0 needs precision WDG_F64; nq_* as in
  * wdg_hip_wave2d_create_exact (ignored for approx_quad = 1).  Every operator
  * built on a handle (DivF2D, EdgeFluxF2D, flux functors, Advection, the load
- * vector) follows its basis; batched WaveHoltz and the schedule autotuner
- * take Gauss-Lobatto bases. */
+ * vector) follows its basis, and so do the RK-stage schedules and their
+ * autotuner; batched WaveHoltz takes Gauss-Lobatto bases. */
 int wdg_hip_wave2d_create_basis(wdg_mesh mesh, int n_colloc, int basis_type, const int *bc, double a, double b,
                                 int precision, int approx_quad, int nq_volume, int nq_edge, int nq_mass,
                                 wdg_wave2d *out);
@@ -286,8 +286,10 @@ int wdg_hip_wave2d_get_schedule(wdg_wave2d op, int *schedule, int *threads_per_w
 /* Times `reps` launches of the RK stage kernel under every schedule on
  * private buffers (the handle's state is untouched), admits only schedules
  * whose output equals schedule 0 bit for bit, selects the fastest for this
- * handle and records it for (architecture, n_colloc, precision): handles
- * created afterwards in this process start from it.  ms (may be NULL) gets
+ * handle and, for a Gauss-Lobatto basis, records it for (architecture,
+ * n_colloc, precision): Gauss-Lobatto handles created afterwards in this
+ * process start from it (Gauss-Legendre handles start from schedule 0 and
+ * keep their tuned schedule on the handle).  ms (may be NULL) gets
  * the average launch time per schedule, -1 for a schedule not admitted.
  * Synchronises the device. */
 int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int *schedule, double *ms);

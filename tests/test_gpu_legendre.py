@@ -178,9 +178,6 @@ def test_fp32_state_close_to_fp64(cuda):
 def test_refusals(cuda):
     v, e = mesh_case("skewed6x5")
     gm = capi.Mesh(v, e)
-    gw = capi.Wave2D(gm, 4, None, basis=capi.GAUSS_LEGENDRE)
-    with pytest.raises(capi.WdgError):
-        gw.autotune(2)
     with pytest.raises(capi.WdgError):
         capi.Wave2D(gm, 4, None, basis=7)
     with pytest.raises(capi.WdgError):  # exact quadrature runs in FP64

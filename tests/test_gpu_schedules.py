@@ -39,6 +39,48 @@ def test_schedules_bitwise_equal(cuda, n, prec):
         assert torch.equal(outs[k], outs[0]), k
 
 
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("prec", [capi.F64, capi.F32])
+def test_legendre_schedules_bitwise_equal(cuda, n, prec):
+    """The Gauss-Legendre stage kernels under every schedule, as above."""
+    torch = cuda
+    print(f"legendre schedules n={n} prec={prec}", flush=True)
+    v, e = mesh_case("skewed16")
+    m = capi.Mesh(v, e)
+    w = capi.Wave2D(m, n, bc_codes(m.n_boundary, "mixed"), precision=prec, basis=capi.GAUSS_LEGENDRE)
+    assert w.schedule()[0] == 0
+    dt = 0.2 * 2.0 / 16 / n ** 2
+    dtype = torch.float64 if prec == capi.F64 else torch.float32
+    u0 = torch.tensor(random_state(w.ndof, 57 + n), device="cuda").to(dtype)
+    F = torch.tensor(random_state(w.ndof, 91 + n), device="cuda").to(dtype)
+    outs = []
+    for k in range(capi.schedule_count()[0]):
+        w.set_schedule(k)
+        u = u0.clone()
+        w.rk4(u, dt, 3, F=F, omega=5.0)
+        torch.cuda.synchronize()
+        outs.append(u)
+    for k in range(1, len(outs)):
+        assert torch.equal(outs[k], outs[0]), k
+
+
+def test_legendre_autotune_stays_on_the_handle(cuda):
+    """A Gauss-Legendre handle's tuned schedule is not recorded for the
+    Gauss-Lobatto handles created afterwards."""
+    v, e = mesh_case("skewed16")
+    m = capi.Mesh(v, e)
+    lob = capi.Wave2D(m, 7, None)
+    lob.set_schedule(0)
+    before = capi.Wave2D(m, 7, None).schedule()[0]
+    w = capi.Wave2D(m, 7, None, basis=capi.GAUSS_LEGENDRE)
+    best, ms = w.autotune(reps=3)
+    print(f"legendre autotune n=7: best {best}, ms {ms}", flush=True)
+    assert w.schedule()[0] == best and ms[best] > 0
+    assert ms[0] > 0  # schedule 0 is the parity reference and always admitted
+    assert capi.Wave2D(m, 7, None).schedule()[0] == before
+    assert capi.Wave2D(m, 7, None, basis=capi.GAUSS_LEGENDRE).schedule()[0] == 0
+
+
 def test_waveholtz_iterate_schedules_equal(cuda):
     torch = cuda
     v, e = mesh_case("lshape8")

@@ -799,7 +799,8 @@ static int create_collocated(wdg_mesh mesh, int n_colloc, int basis_type, const 
         op->hb = -b;
         op->min_h = m.min_h();
         op->arch = device_arch(op->dev);
-        op->sched = tuned_schedule(op->arch, n_colloc, precision);
+        // the tuned table holds Gauss-Lobatto schedules; Gauss-Legendre handles start from 0
+        op->sched = op->leg ? 0 : tuned_schedule(op->arch, n_colloc, precision);
 
         const int n = n_colloc;
         // x[n], w[n], D[n*n], P[n] = phi_k(-1): the face interpolation of a
@@ -2197,7 +2198,6 @@ int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int* schedule, double* ms)
     return guarded([&] {
         check_op(op);
         if (op->exact) fail(WDG_ERR_INVALID, "schedules apply to the approx_quad = true kernels only");
-        if (op->leg) fail(WDG_ERR_INVALID, "Gauss-Legendre handles run the default schedule");
         if (reps < 1) fail(WDG_ERR_INVALID, "reps must be positive");
         const hipStream_t s = nullptr;
         const long long n = op->ndof;
@@ -2264,7 +2264,7 @@ int wdg_hip_wave2d_autotune(wdg_wave2d op, int reps, int* schedule, double* ms)
         op->sched = best;
         for (auto& kv : op->graphs) (void)hipGraphExecDestroy(kv.second);
         op->graphs.clear();
-        {
+        if (!op->leg) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{op->arch, op->n, op->prec}] = best;
         }

@@ -10,7 +10,8 @@ namespace wdg {
 
 namespace {
 
-template <typename R, int TGT>
+// XM: mode bits added to every kernel (M_LEG for Gauss-Legendre bases).
+template <typename R, int TGT, int XM = 0>
 void launch_sched(int kid, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
@@ -19,7 +20,7 @@ void launch_sched(int kid, const KArgs& a, hipStream_t s)
     switch (kid) {
 #define WDG_CASE(K)                                                                   \
     case K:                                                                           \
-        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K], TGT>), grid, block, 0, s, a); \
+        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K] | XM, TGT>), grid, block, 0, s, a); \
         break;
         WDG_CASE(K_STAGE0)
         WDG_CASE(K_STAGE0_F)
@@ -32,11 +33,27 @@ void launch_sched(int kid, const KArgs& a, hipStream_t s)
     }
 }
 
-// Gauss-Legendre bases run every entry point with the default schedule.
+// Gauss-Legendre bases: the RK stage kernels run with the schedule picked
+// for the handle, the other entry points with the default schedule.
 template <typename R>
-void launch_legendre(int kid, const KArgs& a, hipStream_t s)
+void launch_legendre(int kid, int sched, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
+    switch (kid) {
+    case K_STAGE0:
+    case K_STAGE0_F:
+    case K_STAGE:
+    case K_STAGE_F:
+    case K_STAGE_ACC:
+    case K_STAGE_F_ACC:
+        switch (sched) {
+        case 1: launch_sched<R, kSchedTgt[1], M_LEG>(kid, a, s); return;
+        case 2: launch_sched<R, kSchedTgt[2], M_LEG>(kid, a, s); return;
+        case 3: launch_sched<R, kSchedTgt[3], M_LEG>(kid, a, s); return;
+        default: launch_sched<R, kSchedTgt[0], M_LEG>(kid, a, s); return;
+        }
+    default: break;
+    }
     const int nblk = (a.nel + Shape<N>::EPB - 1) / Shape<N>::EPB;
     const dim3 grid(nblk), block(Shape<N>::NT);
     switch (kid) {
@@ -48,12 +65,6 @@ void launch_legendre(int kid, const KArgs& a, hipStream_t s)
         WDG_CASE(K_BC)
         WDG_CASE(K_RHS)
         WDG_CASE(K_RHS_F)
-        WDG_CASE(K_STAGE0)
-        WDG_CASE(K_STAGE0_F)
-        WDG_CASE(K_STAGE)
-        WDG_CASE(K_STAGE_F)
-        WDG_CASE(K_STAGE_ACC)
-        WDG_CASE(K_STAGE_F_ACC)
 #undef WDG_CASE
     default: break;
     }
@@ -64,7 +75,7 @@ void launch_typed(int kid, int sched, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
     if (a.leg) {
-        launch_legendre<R>(kid, a, s);
+        launch_legendre<R>(kid, sched, a, s);
         return;
     }
     switch (kid) {
