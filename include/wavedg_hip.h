@@ -279,7 +279,9 @@ int wdg_hip_rk4_steps_graph(wdg_wave2d op, double dt, double *t, void *d_u, int 
  * workgroup size of the RK stage kernels: about threads[k] threads, i.e.
  * threads[k] / n^2 whole elements per workgroup (one wavefront per element
  * for n = 8 and 64 threads).  Every schedule computes the same values in the
- * same order; the results are bitwise identical. */
+ * same order; the results are bitwise identical.  Setting a schedule and
+ * the autotuner refuse approx_quad = 0 handles, whose fused stage kernel
+ * has one workgroup shape. */
 int wdg_hip_schedule_count(int *count, int *threads);
 int wdg_hip_wave2d_set_schedule(wdg_wave2d op, int schedule);
 int wdg_hip_wave2d_get_schedule(wdg_wave2d op, int *schedule, int *threads_per_workgroup);

@@ -151,6 +151,8 @@ def test_exact_rejects_f32_and_schedules(cuda):
     gw = capi.Wave2D(gm, 4, None, approx_quad=False)
     with pytest.raises(capi.WdgError):
         gw.autotune(2)
+    with pytest.raises(capi.WdgError):
+        gw.set_schedule(1)
     with pytest.raises(capi.WdgError):  # a mass rule too small for the basis
         capi.Wave2D(gm, 4, None, approx_quad=False, quad=(0, 0, 3))
 

@@ -2174,6 +2174,7 @@ int wdg_hip_wave2d_set_schedule(wdg_wave2d op, int schedule)
 {
     return guarded([&] {
         check_op(op);
+        if (op->exact) fail(WDG_ERR_INVALID, "schedules apply to the approx_quad = true kernels only");
         if (schedule < 0 || schedule >= wdg::kSchedules) fail(WDG_ERR_INVALID, "schedule out of range");
         op->sched = schedule;
         for (auto& kv : op->graphs) (void)hipGraphExecDestroy(kv.second);
