@@ -81,6 +81,7 @@ def lib():
             "orc_waveholtz_pi0": (None, [_vp, _dp, _dp]),
             "orc_waveholtz_S": (None, [_vp, _dp]),
             "orc_waveholtz_postprocess": (None, [_vp, _dp, _dp]),
+            "orc_linear_functional": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
             "orc_load_vector_gauss": (None, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]),
             "orc_edge_flux": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
             "orc_divf_euler": (None, [_vp, ctypes.c_double, _dp, _dp]),
@@ -205,6 +206,17 @@ class Mesh:
         out = np.zeros(nq * nq * self.n_elem)
         lib().orc_mesh_detj(self.h, nq, _d(xq), _d(out))
         return out.reshape(self.n_elem, nq, nq)
+
+    def linear_functional(self, n: int, fq, nq: int | None = None) -> np.ndarray:
+        """(f, phi_ij) for f given at the mapped Gauss-Legendre(nq) points:
+        fq has shape (nel, nq, nq, nvar) ([el, l, k, d]); returns (nel, n, n, nvar)."""
+        nq = nq or n
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        nvar = fq.shape[-1]
+        assert fq.shape == (self.n_elem, nq, nq, nvar)
+        F = np.zeros(nvar * n * n * self.n_elem)
+        lib().orc_linear_functional(self.h, n, nq, nvar, _d(fq), _d(F))
+        return F.reshape(self.n_elem, n, n, nvar)
 
     def load_vector_gauss(self, n: int, sources, var: int = 0, nq: int | None = None) -> np.ndarray:
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))

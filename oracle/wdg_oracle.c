@@ -1570,9 +1570,20 @@ void orc_edgefluxf_functor_q(const orc_wave *W, int nvar, int nq, orc_nflux_fn f
 }
 
 /* LinearFunctional.hpp:62-74 (constructor) and :136-202 (2D action). */
-void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src, int var, double *F)
+/* LinearFunctional<2>::action(n_var, f, F) (LinearFunctional.hpp:136-202)
+ * with f already evaluated at the mapped Gauss-Legendre(nq) points:
+ * fq is (nvar, nq, nq, nel) column-major, fq(d, k, l, el) = f_d(x(xq_k, xq_l)).
+ * Each value is weighted by detJ w_k w_l, then projected onto the basis along
+ * y (index l -> j) and then along x (k -> i), B(l, j) = phi_j(xq_l).  F is
+ * (nvar, n, n, nel). */
+void orc_linear_functional(const orc_mesh *m, int n, int nq, int nvar, const double *fq, double *F)
 {
-    double xb[16], wb[16], xq[16], wq[16], B[256];
+    double xb[16], wb[16], xq[24], wq[24], B[16 * 24];
+    if (n < 2 || n > 16 || nq < 1 || nq > 24 || nvar < 1 || nvar > 8) {
+        if (nvar >= 1 && n >= 1)
+            for (size_t q = 0; q < (size_t)nvar * n * n * m->nel; ++q) F[q] = NAN;
+        return;
+    }
     orc_gauss_lobatto(n, xb, wb);
     orc_gauss_legendre(nq, xq, wq);
     orc_lagrange_basis(B, n, xb, nq, xq); /* B(l, j) = phi_j(xq_l), (nq, n) */
@@ -1580,36 +1591,60 @@ void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const dou
 #pragma omp parallel for schedule(static)
     for (int el = 0; el < nel; ++el) {
         const double *c = m->corners + 8 * (size_t)el;
-        double Fq[3 * 16 * 16], ProjF[3 * 16 * 16];
+        double Fq[8 * 24 * 24], ProjF[8 * 24 * 16];
         for (int l = 0; l < nq; ++l)
             for (int k = 0; k < nq; ++k) {
-                double X[2], J[4];
-                quad_coords(c, xq[k], xq[l], X);
+                double J[4];
                 quad_jacobian(c, xq[k], xq[l], J);
-                double fv[3] = {0.0, 0.0, 0.0};
-                for (int s = 0; s < nsrc; ++s) {
-                    const double dx = X[0] - src[4 * s], dy = X[1] - src[4 * s + 1], sg = src[4 * s + 2];
-                    fv[var] += src[4 * s + 3] * exp(-(dx * dx + dy * dy) / (2.0 * sg * sg));
-                }
                 const double detJ = J[0] * J[3] - J[1] * J[2];
                 const double wt = detJ * wq[k] * wq[l];
-                for (int d = 0; d < 3; ++d) Fq[d + 3 * (k + nq * l)] = fv[d] * wt;
+                for (int d = 0; d < nvar; ++d)
+                    Fq[d + nvar * (k + nq * l)] = fq[d + nvar * (k + nq * (l + (size_t)nq * el))] * wt;
             }
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < nq; ++k)
-                for (int d = 0; d < 3; ++d) {
+                for (int d = 0; d < nvar; ++d) {
                     double s = 0.0;
-                    for (int l = 0; l < nq; ++l) s += Fq[d + 3 * (k + nq * l)] * B[l + nq * j];
-                    ProjF[d + 3 * (k + nq * j)] = s;
+                    for (int l = 0; l < nq; ++l) s += Fq[d + nvar * (k + nq * l)] * B[l + nq * j];
+                    ProjF[d + nvar * (k + nq * j)] = s;
                 }
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i)
-                for (int d = 0; d < 3; ++d) {
+                for (int d = 0; d < nvar; ++d) {
                     double s = 0.0;
-                    for (int k = 0; k < nq; ++k) s += ProjF[d + 3 * (k + nq * j)] * B[k + nq * i];
-                    F[d + 3 * (i + n * (j + (size_t)n * el))] = s;
+                    for (int k = 0; k < nq; ++k) s += ProjF[d + nvar * (k + nq * j)] * B[k + nq * i];
+                    F[d + nvar * (i + n * (j + (size_t)n * el))] = s;
                 }
     }
+}
+
+/* The same functional for a Gaussian sum in variable `var` of 3: the sources
+ * are evaluated at the mapped Gauss-Legendre points (the functor the examples
+ * pass, waveholtz2d.cpp:137) and handed to orc_linear_functional. */
+void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src, int var, double *F)
+{
+    double xq[24], wq[24];
+    orc_gauss_legendre(nq, xq, wq);
+    const int nel = m->nel;
+    const size_t npt = (size_t)nq * nq;
+    double *fq = calloc(3 * npt * nel, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int el = 0; el < nel; ++el) {
+        const double *c = m->corners + 8 * (size_t)el;
+        for (int l = 0; l < nq; ++l)
+            for (int k = 0; k < nq; ++k) {
+                double X[2];
+                quad_coords(c, xq[k], xq[l], X);
+                double fv = 0.0;
+                for (int s = 0; s < nsrc; ++s) {
+                    const double dx = X[0] - src[4 * s], dy = X[1] - src[4 * s + 1], sg = src[4 * s + 2];
+                    fv += src[4 * s + 3] * exp(-(dx * dx + dy * dy) / (2.0 * sg * sg));
+                }
+                fq[var + 3 * (k + nq * (l + (size_t)nq * el))] = fv;
+            }
+    }
+    orc_linear_functional(m, n, nq, 3, fq, F);
+    free(fq);
 }
 
 /* ===================================================================== */

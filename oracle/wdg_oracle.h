@@ -89,7 +89,11 @@ void orc_waveholtz_S(const orc_wh *H, double *u);
 void orc_waveholtz_postprocess(const orc_wh *H, double *Hout, const double *u);
 
 /* ---- linear functional (load vector) ----------------------------------
- * (f, phi_ij) with Gauss-Legendre(nq) quadrature, f = sum of Gaussians in
+ * (f, phi_ij) with Gauss-Legendre(nq) quadrature (nq <= 24, nvar <= 8) for
+ * values of f given at the mapped points: fq (nvar, nq, nq, nel), F
+ * (nvar, n, n, nel). */
+void orc_linear_functional(const orc_mesh *m, int n, int nq, int nvar, const double *fq, double *F);
+/* The same for f = sum of Gaussians in
  * variable `var`: f(x) = sum_k amp_k exp(-|x - c_k|^2 / (2 sigma_k^2)).
  * src is (4, nsrc): cx, cy, sigma, amp. */
 void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const double *src,

@@ -53,3 +53,29 @@ def test_gl_points_are_mapped_gauss_legendre_nodes(cuda):
     # uniform 5 x 4 grid on [-1, 1]^2: element 0 spans [-1, -0.6] x [-1, -0.5]
     np.testing.assert_allclose(xy[0, 0, :, 0], -0.8 + 0.2 * x, atol=1e-15)
     np.testing.assert_allclose(xy[0, :, 0, 1], -0.75 + 0.25 * x, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["affine", "bilinear"])
+@pytest.mark.parametrize("n,extra", [(3, 0), (4, 0), (6, 0), (8, 0), (6, 3)])
+def test_linear_functional_per_node_exact(cuda, kind, n, extra):
+    """Every F(i, j) equals the exact integral of f phi_i(xi) phi_j(eta) detJ
+    for a polynomial f within the Gauss-Legendre rule's exactness
+    (LinearFunctional.hpp:136-202), on one parallelogram and one general
+    quadrilateral; the integrand is anisotropic, so a transposed projection
+    fails."""
+    from tests.test_oracle_pins import ELEMENTS, exact_projection, poly_f
+
+    torch = cuda
+    c = ELEMENTS[kind]
+    gm = capi.Mesh(c, np.array([[0, 1, 2, 3]], dtype=np.int32))
+    gw = capi.Wave2D(gm, n, None)
+    deg = 2 if n <= 4 else 3
+    nq = n + extra
+    xy = gm.gl_points(nq)  # [el, l, k, xy]
+    fq = poly_f(xy[..., 0], xy[..., 1], deg)
+    F = torch.zeros(2 * n * n, dtype=torch.float64, device="cuda")
+    gw.linear_functional(2, torch.tensor(fq.reshape(-1), device="cuda"), F, nq=extra and nq)
+    torch.cuda.synchronize()
+    got = F.cpu().numpy().reshape(n, n, 2)  # [j, i, d]
+    want = exact_projection(c, n, deg)
+    assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want))

@@ -81,13 +81,17 @@ def test_legendre_autotune_stays_on_the_handle(cuda):
     assert capi.Wave2D(m, 7, None, basis=capi.GAUSS_LEGENDRE).schedule()[0] == 0
 
 
-def test_waveholtz_iterate_schedules_equal(cuda):
+@pytest.mark.parametrize("basis", [capi.GAUSS_LOBATTO, capi.GAUSS_LEGENDRE])
+@pytest.mark.parametrize("precision", [capi.F64, capi.F32])
+def test_waveholtz_iterate_schedules_equal(cuda, basis, precision):
+    """The WaveHoltz filter-accumulating stage kernels give bitwise equal
+    iterates under every schedule, for both bases and precisions."""
     torch = cuda
     v, e = mesh_case("lshape8")
     m = capi.Mesh(v, e)
     res = []
     for k in range(capi.schedule_count()[0]):
-        w = capi.Wave2D(m, 5, bc_codes(m.n_boundary, "absorb"))
+        w = capi.Wave2D(m, 5, bc_codes(m.n_boundary, "absorb"), precision=precision, basis=basis)
         w.set_schedule(k)
         wh = capi.WaveHoltz(w, 6.0 * np.pi)
         F = torch.empty(w.ndof, dtype=torch.float64, device="cuda")

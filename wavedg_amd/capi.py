@@ -195,6 +195,28 @@ def dev_ptr(t) -> int | None:
     return t.data_ptr()
 
 
+_F64_T, _F32_T = "torch.float64", "torch.float32"
+
+
+def _arg(t, dtype: str, numel: int, name: str, optional: bool = False) -> int | None:
+    """Device address of tensor t after checking that it is what the kernels
+    will read or write: a contiguous device tensor of `dtype` with exactly
+    `numel` elements.  The C ABI takes no lengths, so a short or mistyped
+    array would otherwise be read and written out of bounds.  Raw int
+    addresses are passed through unchecked (the caller vouches for them)."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: a device tensor is required")
+    if isinstance(t, int):
+        return t
+    if str(t.dtype) != dtype:
+        raise ValueError(f"{name}: expected dtype {dtype[6:]}, got {str(t.dtype)[6:]}")
+    if t.numel() != numel:
+        raise ValueError(f"{name}: expected {numel} elements, got {t.numel()}")
+    return dev_ptr(t)
+
+
 def cur_stream() -> int | None:
     import torch
 
@@ -219,7 +241,8 @@ def ode_step(method: int, f, dt: float, t: float, u, w=None, p=None, stream=None
     for k in range(ns.value):
         x = u if (k == 0 or method in (FORWARD_EULER, RUNGE_KUTTA4)) else w
         f(p, t + c[k] * dt, x)
-        _call("wdg_hip_ode_update", method, k, u.numel(), dt, dev_ptr(u), dev_ptr(w), dev_ptr(p), st)
+        _call("wdg_hip_ode_update", method, k, u.numel(), dt, _arg(u, _F64_T, u.numel(), "u"),
+              _arg(w, _F64_T, u.numel(), "w", optional=True), _arg(p, _F64_T, u.numel(), "p"), st)
     return t + dt
 
 
@@ -343,31 +366,42 @@ class Wave2D:
 
         return torch.float32 if self.precision == F32 else torch.float64
 
+    @property
+    def _dt(self) -> str:
+        return _F32_T if self.precision == F32 else _F64_T
+
+    def _state(self, t, name: str, optional: bool = False, n_var: int = 3):
+        """A state-shaped argument, (n_var, n, n, n_elem) in the handle's precision."""
+        return _arg(t, self._dt, n_var * self.n * self.n * self.n_elem, name, optional)
+
     def action(self, x, y, stream=None):
-        _call("wdg_hip_wave2d_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+        _call("wdg_hip_wave2d_action", self.h, self._state(x, "x"), self._state(y, "y"),
+              stream if stream is not None else cur_stream())
 
     def bc_action(self, x, y, stream=None):
-        _call("wdg_hip_wavebc_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+        _call("wdg_hip_wavebc_action", self.h, self._state(x, "x"), self._state(y, "y"),
+              stream if stream is not None else cur_stream())
 
     def mass_inv(self, x, stream=None):
-        _call("wdg_hip_mass_inv", self.h, dev_ptr(x), stream if stream is not None else cur_stream())
+        _call("wdg_hip_mass_inv", self.h, self._state(x, "x"), stream if stream is not None else cur_stream())
 
     def mass_action(self, x, y, stream=None):
-        _call("wdg_hip_mass_action", self.h, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+        _call("wdg_hip_mass_action", self.h, self._state(x, "x"), self._state(y, "y"),
+              stream if stream is not None else cur_stream())
 
     def rhs(self, x, y, F=None, s: float = 0.0, stream=None):
-        _call("wdg_hip_wave2d_rhs", self.h, dev_ptr(x), dev_ptr(y), dev_ptr(F), s,
+        _call("wdg_hip_wave2d_rhs", self.h, self._state(x, "x"), self._state(y, "y"), self._state(F, "F", True), s,
               stream if stream is not None else cur_stream())
 
     def rk4(self, u, dt: float, n_steps: int, t: float = 0.0, F=None, omega: float = 0.0, stream=None) -> float:
         tt = _d(t)
-        _call("wdg_hip_rk4_steps", self.h, dt, ctypes.byref(tt), dev_ptr(u), n_steps, dev_ptr(F), omega,
-              stream if stream is not None else cur_stream())
+        _call("wdg_hip_rk4_steps", self.h, dt, ctypes.byref(tt), self._state(u, "u"), n_steps,
+              self._state(F, "F", True), omega, stream if stream is not None else cur_stream())
         return tt.value
 
     def rk4_graph(self, u, dt: float, n_steps: int, steps_per_graph: int, t: float = 0.0, stream=None) -> float:
         tt = _d(t)
-        _call("wdg_hip_rk4_steps_graph", self.h, dt, ctypes.byref(tt), dev_ptr(u), n_steps, steps_per_graph,
+        _call("wdg_hip_rk4_steps_graph", self.h, dt, ctypes.byref(tt), self._state(u, "u"), n_steps, steps_per_graph,
               stream if stream is not None else cur_stream())
         return tt.value
 
@@ -381,30 +415,32 @@ class Wave2D:
 
     def linear_functional(self, n_var: int, fq, F, nq: int = 0, stream=None) -> None:
         """F <- (f, phi) from f at the Gauss-Legendre points (Mesh.gl_points(nq)), (n_var, nq, nq, n_elem)."""
-        _call("wdg_hip_linear_functional", self.h, n_var, nq, dev_ptr(fq), dev_ptr(F),
+        q = nq or self.n
+        _call("wdg_hip_linear_functional", self.h, n_var, nq, _arg(fq, _F64_T, n_var * q * q * self.n_elem, "fq"),
+              _arg(F, _F64_T, n_var * self.n * self.n * self.n_elem, "F"),
               stream if stream is not None else cur_stream())
 
     def divf_euler(self, gamma: float, q, out, stream=None) -> None:
         """out += DivF2D(q) for the compressible Euler flux; q is (4, n, n, n_elem)."""
-        _call("wdg_hip_divf2d_euler", self.h, gamma, dev_ptr(q), dev_ptr(out),
+        _call("wdg_hip_divf2d_euler", self.h, gamma, self._state(q, "q", n_var=4), self._state(out, "out", n_var=4),
               stream if stream is not None else cur_stream())
 
     def edgefluxf_euler(self, gamma: float, face_type: int, q, out, stream=None) -> None:
         """out -= lifted local Lax-Friedrichs edge fluxes (interior 0, boundary 1)."""
-        _call("wdg_hip_edgefluxf2d_euler", self.h, gamma, face_type, dev_ptr(q), dev_ptr(out),
-              stream if stream is not None else cur_stream())
+        _call("wdg_hip_edgefluxf2d_euler", self.h, gamma, face_type, self._state(q, "q", n_var=4),
+              self._state(out, "out", n_var=4), stream if stream is not None else cur_stream())
 
     def mass_apply(self, n_var: int, x, y, inverse: bool, stream=None) -> None:
         """y <- M x or M^{-1} x for n_var variables per node (x may be y)."""
-        _call("wdg_hip_mass_apply", self.h, n_var, dev_ptr(x), dev_ptr(y), 1 if inverse else 0,
-              stream if stream is not None else cur_stream())
+        _call("wdg_hip_mass_apply", self.h, n_var, self._state(x, "x", n_var=n_var), self._state(y, "y", n_var=n_var),
+              1 if inverse else 0, stream if stream is not None else cur_stream())
 
     def divf_linear(self, n_var: int, A, B, u, out, stream=None) -> None:
         """out += DivF2D(u) for the flux (A u, B u); A, B as (n_var, n_var) arrays."""
         Ac = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(n_var, n_var).T.reshape(-1))
         Bc = np.ascontiguousarray(np.asarray(B, dtype=np.float64).reshape(n_var, n_var).T.reshape(-1))
-        _call("wdg_hip_divf2d_linear", self.h, n_var, _dptr(Ac), _dptr(Bc), dev_ptr(u), dev_ptr(out),
-              stream if stream is not None else cur_stream())
+        _call("wdg_hip_divf2d_linear", self.h, n_var, _dptr(Ac), _dptr(Bc), self._state(u, "u", n_var=n_var),
+              self._state(out, "out", n_var=n_var), stream if stream is not None else cur_stream())
 
     def quadrature(self):
         """(exact, volume, edge, mass rule sizes)."""
@@ -431,7 +467,7 @@ class Wave2D:
 
     def load_vector_gauss(self, sources, F, var: int = 0, stream=None):
         src = np.ascontiguousarray(np.asarray(sources, dtype=np.float64).reshape(-1, 4))
-        _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, dev_ptr(F),
+        _call("wdg_hip_load_vector_gauss", self.h, src.shape[0], _dptr(src), var, _arg(F, _F64_T, self.ndof, "F"),
               stream if stream is not None else cur_stream())
 
 
@@ -467,11 +503,14 @@ class Flux:
         _call("wdg_hip_flux_compile", self.h, which)
 
     def divf(self, u, divF, stream=None):
-        _call("wdg_hip_divf2d", self.h, dev_ptr(u), dev_ptr(divF), stream if stream is not None else cur_stream())
+        w, k = self.wave, self.n_var
+        _call("wdg_hip_divf2d", self.h, w._state(u, "u", n_var=k), w._state(divF, "divF", n_var=k),
+              stream if stream is not None else cur_stream())
 
     def edgeflux(self, face_type: int, u, divF, scale: float = 1.0, stream=None):
-        _call("wdg_hip_edgefluxf2d", self.h, face_type, scale, dev_ptr(u), dev_ptr(divF),
-              stream if stream is not None else cur_stream())
+        w, k = self.wave, self.n_var
+        _call("wdg_hip_edgefluxf2d", self.h, face_type, scale, w._state(u, "u", n_var=k),
+              w._state(divF, "divF", n_var=k), stream if stream is not None else cur_stream())
 
 
 class Advection:
@@ -503,11 +542,14 @@ class Advection:
 
     def action(self, u, y, stream=None) -> None:
         """y += Div(u) + interior edge fluxes."""
-        _call("wdg_hip_advection_action", self.h, dev_ptr(u), dev_ptr(y), stream if stream is not None else cur_stream())
+        w, k = self.wave, self.n_var
+        _call("wdg_hip_advection_action", self.h, w._state(u, "u", n_var=k), w._state(y, "y", n_var=k),
+              stream if stream is not None else cur_stream())
 
     def bc_action(self, u, y, stream=None) -> None:
         """y += boundary edge fluxes with u = 0 outside."""
-        _call("wdg_hip_advection_bc_action", self.h, dev_ptr(u), dev_ptr(y),
+        w, k = self.wave, self.n_var
+        _call("wdg_hip_advection_bc_action", self.h, w._state(u, "u", n_var=k), w._state(y, "y", n_var=k),
               stream if stream is not None else cur_stream())
 
 
@@ -527,15 +569,20 @@ class WaveHoltz:
             _lib.wdg_hip_waveholtz_destroy(self.h)
             self.h = None
 
+    def _w(self, t, name: str):
+        """WaveHoltz arrays are double at every precision, ndof long."""
+        return _arg(t, _F64_T, self.wave.ndof, name)
+
     def pi0(self, w, F, stream=None):
-        _call("wdg_hip_waveholtz_pi0", self.h, dev_ptr(w), dev_ptr(F), stream if stream is not None else cur_stream())
+        _call("wdg_hip_waveholtz_pi0", self.h, self._w(w, "w"), self._w(F, "F"),
+              stream if stream is not None else cur_stream())
 
     def S(self, w, stream=None):
-        _call("wdg_hip_waveholtz_S", self.h, dev_ptr(w), stream if stream is not None else cur_stream())
+        _call("wdg_hip_waveholtz_S", self.h, self._w(w, "w"), stream if stream is not None else cur_stream())
 
     def iterate(self, w, pi0, pi0_norm: float, stream=None) -> float:
         r = _d()
-        _call("wdg_hip_waveholtz_iterate", self.h, dev_ptr(w), dev_ptr(pi0), pi0_norm, ctypes.byref(r),
+        _call("wdg_hip_waveholtz_iterate", self.h, self._w(w, "w"), self._w(pi0, "pi0"), pi0_norm, ctypes.byref(r),
               stream if stream is not None else cur_stream())
         return r.value
 
@@ -546,35 +593,43 @@ class WaveHoltz:
         (iterations, refinement iterations, final residual)."""
         it = (ctypes.c_int * 2)()
         r = _d()
-        _call("wdg_hip_waveholtz_solve", self.h, refine.h if refine is not None else None, dev_ptr(w), dev_ptr(pi0),
+        _call("wdg_hip_waveholtz_solve", self.h, refine.h if refine is not None else None, self._w(w, "w"),
+              self._w(pi0, "pi0"),
               pi0_norm, tol, max_iter, it, ctypes.byref(r), stream if stream is not None else cur_stream())
         return it[0], it[1], r.value
 
     def checkpoint(self, path: str, W, iterations: int, residual: float) -> None:
-        _call("wdg_hip_waveholtz_checkpoint", self.h, path.encode(), dev_ptr(W), iterations, residual)
+        _call("wdg_hip_waveholtz_checkpoint", self.h, path.encode(), self._w(W, "W"), iterations, residual)
 
     def resume(self, path: str, W):
         """Loads a checkpoint into W; returns (iterations, residual) recorded with it."""
         it, r = _i(), _d()
-        _call("wdg_hip_waveholtz_resume", self.h, path.encode(), dev_ptr(W), ctypes.byref(it), ctypes.byref(r))
+        _call("wdg_hip_waveholtz_resume", self.h, path.encode(), self._w(W, "W"), ctypes.byref(it), ctypes.byref(r))
         return it.value, r.value
 
     def postprocess(self, H, w, stream=None):
-        _call("wdg_hip_waveholtz_postprocess", self.h, dev_ptr(H), dev_ptr(w),
+        nn = self.wave.n * self.wave.n * self.wave.n_elem
+        _call("wdg_hip_waveholtz_postprocess", self.h, _arg(H, _F64_T, 2 * nn, "H"), self._w(w, "w"),
               stream if stream is not None else cur_stream())
+
+
+def _file_arg(x) -> int | None:
+    if str(x.dtype) not in (_F64_T, _F32_T):
+        raise ValueError(f"x: expected float64 or float32, got {str(x.dtype)[6:]}")
+    return dev_ptr(x)
 
 
 def to_file(path: str, x) -> None:
     """Raw float64 file of a device tensor (float32 tensors converted)."""
     import torch
 
-    _call("wdg_hip_to_file", path.encode(), dev_ptr(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
+    _call("wdg_hip_to_file", path.encode(), _file_arg(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
 
 
 def from_file(path: str, x) -> None:
     import torch
 
-    _call("wdg_hip_from_file", path.encode(), dev_ptr(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
+    _call("wdg_hip_from_file", path.encode(), _file_arg(x), x.numel(), F32 if x.dtype == torch.float32 else F64)
 
 
 def schedule_count():
@@ -632,7 +687,7 @@ class Batch:
         _call("wdg_hip_batch_set_stall_window", self.h, window)
 
     def set_initial(self, k: int, W0=None):
-        _call("wdg_hip_batch_set_initial", self.h, k, dev_ptr(W0))
+        _call("wdg_hip_batch_set_initial", self.h, k, _arg(W0, _F64_T, self.info(k)[0], "W0", optional=True))
 
     def solve(self, tol: float, max_iter: int, steps_per_graph: int = 16, stream=None):
         _call("wdg_hip_batch_solve", self.h, tol, max_iter, steps_per_graph,
@@ -643,19 +698,21 @@ class Batch:
 
     def result(self, k: int, W=None):
         it, r = _i(), _d()
-        _call("wdg_hip_batch_result", self.h, k, dev_ptr(W), ctypes.byref(it), ctypes.byref(r))
+        _call("wdg_hip_batch_result", self.h, k, _arg(W, _F64_T, self.info(k)[0], "W", optional=True),
+              ctypes.byref(it), ctypes.byref(r))
         return it.value, r.value
 
 
 def norm(x, stream=None) -> float:
     r = _d()
-    _call("wdg_hip_norm", dev_ptr(x), x.numel(), ctypes.byref(r), stream if stream is not None else cur_stream())
+    _call("wdg_hip_norm", _arg(x, _F64_T, x.numel(), "x"), x.numel(), ctypes.byref(r),
+          stream if stream is not None else cur_stream())
     return r.value
 
 
 def error(x, y, stream=None) -> float:
     r = _d()
-    _call("wdg_hip_error", dev_ptr(x), dev_ptr(y), x.numel(), ctypes.byref(r),
+    _call("wdg_hip_error", _arg(x, _F64_T, x.numel(), "x"), _arg(y, _F64_T, x.numel(), "y"), x.numel(), ctypes.byref(r),
           stream if stream is not None else cur_stream())
     return r.value
 
@@ -678,6 +735,21 @@ class CNN:
             _lib.wdg_hip_cnn_destroy(self.h)
             self.h = None
 
+    def _io(self, B: int, x, y, yname: str):
+        """Checked addresses of `B` input images (bf16, [B, M, M, in_ld], the
+        tensor may hold more) and B output/target images (fp32, out_ld)."""
+        if not 1 <= B <= self.max_batch:
+            raise ValueError(f"batch {B} outside 1..{self.max_batch}")
+        px = self.M * self.M
+        for t, dt, ld, name in ((x, "torch.bfloat16", self.in_ld, "x"), (y, _F32_T, self.out_ld, yname)):
+            if isinstance(t, int):
+                continue
+            if str(t.dtype) != dt:
+                raise ValueError(f"{name}: expected dtype {dt[6:]}, got {str(t.dtype)[6:]}")
+            if t.numel() < B * px * ld:
+                raise ValueError(f"{name}: {B} images need {B * px * ld} elements, got {t.numel()}")
+        return dev_ptr(x), dev_ptr(y)
+
     def set_params(self, p: np.ndarray):
         p = np.ascontiguousarray(p, dtype=np.float32)
         assert p.size == self.n_params
@@ -691,40 +763,45 @@ class CNN:
     def forward(self, x, y, batch: int | None = None, stream=None):
         """x: uint16 (bf16 bits) [B, M, M, in_ld] device tensor; y: float32 [B, M, M, out_ld]."""
         B = batch if batch is not None else x.shape[0]
-        _call("wdg_hip_cnn_forward", self.h, B, dev_ptr(x), dev_ptr(y), stream if stream is not None else cur_stream())
+        xp, yp = self._io(B, x, y, "y")
+        _call("wdg_hip_cnn_forward", self.h, B, xp, yp, stream if stream is not None else cur_stream())
 
     def loss_grad(self, x, t, batch: int | None = None, want_grad: bool = True, stream=None):
         B = batch if batch is not None else x.shape[0]
+        xp, tp = self._io(B, x, t, "t")
         loss = ctypes.c_float()
         g = np.zeros(self.n_params, dtype=np.float32) if want_grad else None
-        _call("wdg_hip_cnn_loss_grad", self.h, B, dev_ptr(x), dev_ptr(t), ctypes.byref(loss),
+        _call("wdg_hip_cnn_loss_grad", self.h, B, xp, tp, ctypes.byref(loss),
               g.ctypes.data_as(_fp) if g is not None else None, stream if stream is not None else cur_stream())
         return loss.value, g
 
     def train_step(self, x, t, lr: float, batch: int | None = None, want_loss: bool = True, stream=None):
         B = batch if batch is not None else x.shape[0]
+        xp, tp = self._io(B, x, t, "t")
         loss = ctypes.c_float()
-        _call("wdg_hip_cnn_train_step", self.h, B, dev_ptr(x), dev_ptr(t), lr,
+        _call("wdg_hip_cnn_train_step", self.h, B, xp, tp, lr,
               ctypes.byref(loss) if want_loss else None, stream if stream is not None else cur_stream())
         return loss.value if want_loss else None
 
 
 def f32_to_bf16(src, dst, stream=None):
-    _call("wdg_hip_f32_to_bf16", dev_ptr(src), dev_ptr(dst), src.numel(), stream if stream is not None else cur_stream())
+    _call("wdg_hip_f32_to_bf16", _arg(src, _F32_T, src.numel(), "src"), _arg(dst, "torch.bfloat16", src.numel(), "dst"),
+          src.numel(), stream if stream is not None else cur_stream())
 
 
 def ws_features(d_nodes, n: int, nel: int, d_sources, n_sources: int, gain: float, omega: float, omega_feature: float,
                 green_gain: float, box, ld: int, d_x, stream=None):
     b = np.ascontiguousarray(np.asarray(box, dtype=np.float64))
-    _call("wdg_hip_ws_features", dev_ptr(d_nodes), n, nel, n_sources, dev_ptr(d_sources), gain, omega, omega_feature,
-          green_gain, _dptr(b), ld, dev_ptr(d_x), stream if stream is not None else cur_stream())
+    _call("wdg_hip_ws_features", _arg(d_nodes, _F64_T, 2 * n * n * nel, "d_nodes"), n, nel, n_sources,
+          _arg(d_sources, _F64_T, 4 * n_sources, "d_sources"), gain, omega, omega_feature, green_gain, _dptr(b), ld,
+          _arg(d_x, "torch.bfloat16", nel * ld, "d_x"), stream if stream is not None else cur_stream())
 
 
 def ws_dg_to_image(d_W, n: int, nel: int, scale: float, ld: int, d_t, stream=None):
-    _call("wdg_hip_ws_dg_to_image", dev_ptr(d_W), n, nel, scale, ld, dev_ptr(d_t),
-          stream if stream is not None else cur_stream())
+    _call("wdg_hip_ws_dg_to_image", _arg(d_W, _F64_T, 3 * n * n * nel, "d_W"), n, nel, scale, ld,
+          _arg(d_t, _F32_T, nel * ld, "d_t"), stream if stream is not None else cur_stream())
 
 
 def ws_image_to_dg(d_y, ld: int, n: int, nel: int, scale: float, d_W, stream=None):
-    _call("wdg_hip_ws_image_to_dg", dev_ptr(d_y), ld, n, nel, scale, dev_ptr(d_W),
-          stream if stream is not None else cur_stream())
+    _call("wdg_hip_ws_image_to_dg", _arg(d_y, _F32_T, nel * ld, "d_y"), ld, n, nel, scale,
+          _arg(d_W, _F64_T, 3 * n * n * nel, "d_W"), stream if stream is not None else cur_stream())
