@@ -71,6 +71,9 @@ def test_divf_fp32_close(cuda):
     torch.cuda.synchronize()
     assert float(torch.linalg.norm(y32.double() - y64) / torch.linalg.norm(y64)) < 1e-5
 
+    # n_var = 9 is refused by the library (raw addresses skip the binding's length check)
+    z = torch.zeros(9 * n * n * nel, dtype=torch.float64, device="cuda")
     with pytest.raises(capi.WdgError):
-        w64.divf_linear(9, np.eye(9), np.eye(9), torch.zeros(9, dtype=torch.float64, device="cuda"),
-                        torch.zeros(9, dtype=torch.float64, device="cuda"))
+        w64.divf_linear(9, np.eye(9), np.eye(9), z.data_ptr(), z.data_ptr())
+    with pytest.raises(ValueError):  # the binding refuses a state of the wrong length
+        w64.divf_linear(nvar, A, B, torch.zeros(9, dtype=torch.float64, device="cuda"), y64)
