@@ -413,6 +413,17 @@ int wdg_hip_ode_update(int method, int stage, long long n, double dt, double *d_
 /* ---- reductions (linalg.cpp:102-130), double, deterministic ---------- */
 int wdg_hip_norm(const double *d_x, long long n, double *out, void *stream);
 int wdg_hip_error(const double *d_x, const double *d_y, long long n, double *out, void *stream);
+/* wdg_hip_error without any device allocation.  It has no counterpart in
+ * the reference.  It is for callers that cannot allocate device memory:
+ * code under stream capture, tight solver loops, and harnesses that forbid
+ * hipMalloc.  Its scratch is a fixed array of 8,200 bytes per GPU, which
+ * holds 1024 partial sums and the total.  The reduction strides over any
+ * length with a fixed grid, so every n >= 0 fits; n < 0 is refused.  The
+ * result equals wdg_hip_error's bit for bit.  All calls on one GPU share
+ * that GPU's scratch, so calls that overlap on one GPU (two host threads,
+ * or two streams without ordering between them) race: order them.  The
+ * call blocks until the result is on the host, as wdg_hip_error does. */
+int wdg_hip_error_noalloc(const double *d_x, const double *d_y, long long n, double *out, void *stream);
 /* Converts between the handle's precision and double (n values). */
 int wdg_hip_convert(const void *d_src, int src_precision, void *d_dst, int dst_precision, long long n, void *stream);
 

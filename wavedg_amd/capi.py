@@ -92,6 +92,7 @@ _SIGS = {
     "wdg_hip_batch_timing": [_vp, _dp, ctypes.POINTER(_ll)],
     "wdg_hip_norm": [_vp, _ll, _dp, _vp],
     "wdg_hip_error": [_vp, _vp, _ll, _dp, _vp],
+    "wdg_hip_error_noalloc": [_vp, _vp, _ll, _dp, _vp],
     "wdg_hip_convert": [_vp, _i, _vp, _i, _ll, _vp],
     "wdg_hip_to_file": [ctypes.c_char_p, _vp, _ll, _i],
     "wdg_hip_from_file": [ctypes.c_char_p, _vp, _ll, _i],

@@ -1462,6 +1462,54 @@ int wdg_hip_error(const double* d_x, const double* d_y, long long n, double* out
     });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free residual
+// Device scratch of wdg_hip_error_noalloc: the kRedBlocks partial sums of
+// k_sumsq, then the total.  The reduction strides over any n with a fixed
+// grid, so this fixed size (8,200 bytes) serves every length.  Each GPU holds
+// its own copy until the process ends; in host code `scratch` names a host
+// stand-in, so kernels get the address from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[kRedBlocks + 1];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::once_flag g_scratch_once[kScratchDevices];
+double* g_scratch_ptr[kScratchDevices] = {};
+hipError_t g_scratch_err[kScratchDevices] = {};
+
+// This device's `scratch`, looked up once per device with that device current.
+double* device_scratch()
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kScratchDevices) fail(WDG_ERR_INVALID, "device index beyond the scratch table");
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g_scratch_err[dev] == hipSuccess) g_scratch_ptr[dev] = static_cast<double*>(p);
+    });
+    hip_check(g_scratch_err[dev], "hipGetSymbolAddress(scratch)");
+    if (!g_scratch_ptr[dev]) fail(WDG_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int wdg_hip_error_noalloc(const double* d_x, const double* d_y, long long n, double* out, void* stream)
+{
+    return guarded([&] {
+        if (!d_x || !d_y || !out || n < 0) fail(WDG_ERR_INVALID, "bad argument");
+        double* buf = device_scratch();
+        // k_sumsq writes every one of the kRedBlocks partials and k_final the
+        // total, so nothing left by an earlier call is read.
+        const double v = reduce_sumsq(d_x, d_y, n, buf, buf + kRedBlocks, as_stream(stream));
+        *out = std::sqrt(v);
+        if (!std::isfinite(*out)) fail(WDG_ERR_NONFINITE, "error norm is not finite");
+    });
+}
+
 int wdg_hip_convert(const void* d_src, int src_precision, void* d_dst, int dst_precision, long long n, void* stream)
 {
     return guarded([&] {
