@@ -320,6 +320,12 @@ int wdg_hip_waveholtz_S(wdg_waveholtz wh, double *d_w, void *stream);
  * (linalg.cpp:116-130), all in double on the device.  Synchronises `stream`. */
 int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double *d_w, const double *d_pi0, double pi0_norm,
                               double *rel_change, void *stream);
+/* No counterpart in the reference.  enable != 0 (the default): iterate and
+ * solve replay the step's launches (the nt RK4 steps of S, the update and the
+ * residual sum) from a HIP graph captured on the first call for the given
+ * d_w, d_pi0 and stage schedule; enable = 0: launch them one by one.  Both
+ * give identical results. */
+int wdg_hip_waveholtz_set_graph(wdg_waveholtz wh, int enable);
 /* The caller loop of examples/waveholtz2d.cpp:165-185 run to a tolerance:
  * iterate with `wh` until rel_change <= tol, or, when `wh_refine` is not NULL,
  * until the residual stops decreasing (it stalls at a precision floor when

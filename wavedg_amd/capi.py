@@ -81,6 +81,7 @@ _SIGS = {
     "wdg_hip_waveholtz_iterate": [_vp, _vp, _vp, _d, _dp, _vp],
     "wdg_hip_waveholtz_postprocess": [_vp, _vp, _vp, _vp],
     "wdg_hip_waveholtz_solve": [_vp, _vp, _vp, _vp, _d, _d, _i, _ip, _dp, _vp],
+    "wdg_hip_waveholtz_set_graph": [_vp, _i],
     "wdg_hip_load_vector_gauss": [_vp, _i, _dp, _i, _vp, _vp],
     "wdg_hip_batch_create": [_i, _vp, _i, ctypes.POINTER(_vp)],
     "wdg_hip_batch_destroy": [_vp],
@@ -580,6 +581,10 @@ class WaveHoltz:
 
     def S(self, w, stream=None):
         _call("wdg_hip_waveholtz_S", self.h, self._w(w, "w"), stream if stream is not None else cur_stream())
+
+    def set_graph(self, on: bool) -> None:
+        """Replay each fixed-point step from a captured HIP graph (default) or launch eagerly."""
+        _call("wdg_hip_waveholtz_set_graph", self.h, 1 if on else 0)
 
     def iterate(self, w, pi0, pi0_norm: float, stream=None) -> float:
         r = _d()

@@ -223,8 +223,14 @@ struct wdg_waveholtz_s {
     void* d_p = nullptr;       // RK state (handle precision)
     double* d_wprev = nullptr;
     void* d_F = nullptr;       // forcing in handle precision (F32 only)
+    // One fixed-point update (S, W += pi0, residual sum) captured as a HIP
+    // graph for the arrays and schedule in it_key, replayed by iterate.
+    bool use_graph = true;
+    hipGraphExec_t it_graph = nullptr;
+    std::tuple<double*, const double*, int> it_key{nullptr, nullptr, -1};
     ~wdg_waveholtz_s()
     {
+        if (it_graph) (void)hipGraphExecDestroy(it_graph);
         dfree(d_p);
         dfree(d_wprev);
         dfree(d_F);
@@ -1235,16 +1241,61 @@ int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double* d_w, const double* d_pi0
         wdg_wave2d op = wh->op;
         const hipStream_t s = as_stream(stream);
         if (!wh->d_wprev) wh->d_wprev = static_cast<double*>(dmalloc(sizeof(double) * op->ndof));
-        wh_apply_S(wh, d_w, wh->d_wprev, s);
-        hipLaunchKernelGGL(k_wh_finalize, dim3(kRedBlocks), dim3(kRedThreads), 0, s, d_w, d_pi0, wh->d_wprev,
-                           op->ndof, op->d_partial);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(kRedThreads), 0, s, op->d_partial, kRedBlocks, op->d_result);
-        hip_check(hipGetLastError(), "finalize launch");
+        op->ensure(op->d_tmp, op->state_bytes());
+        op->ensure(op->d_du, op->state_bytes());
+        op->ensure(wh->d_p, op->state_bytes());
+        // S(W) with W_prev = W, then W += pi0 and the residual's sum of
+        // squares (waveholtz2d.cpp:170-175); the same launches eager or as a
+        // replayed graph, so both give identical results
+        auto enqueue = [&](hipStream_t q) {
+            wh_apply_S(wh, d_w, wh->d_wprev, q);
+            hipLaunchKernelGGL(k_wh_finalize, dim3(kRedBlocks), dim3(kRedThreads), 0, q, d_w, d_pi0, wh->d_wprev,
+                               op->ndof, op->d_partial);
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(kRedThreads), 0, q, op->d_partial, kRedBlocks, op->d_result);
+            hip_check(hipGetLastError(), "finalize launch");
+        };
+        if (wh->use_graph) {
+            const auto key = std::make_tuple(d_w, d_pi0, op->sched);
+            if (!wh->it_graph || wh->it_key != key) {
+                if (wh->it_graph) (void)hipGraphExecDestroy(wh->it_graph);
+                wh->it_graph = nullptr;
+                if (!op->cap_stream)
+                    hip_check(hipStreamCreateWithFlags(&op->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+                hipGraph_t g = nullptr;
+                hip_check(hipStreamBeginCapture(op->cap_stream, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+                try {
+                    enqueue(op->cap_stream);
+                } catch (...) {
+                    (void)hipStreamEndCapture(op->cap_stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(op->cap_stream, &g), "hipStreamEndCapture");
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                hip_check(ie, "hipGraphInstantiate");
+                wh->it_graph = ex;
+                wh->it_key = key;
+            }
+            hip_check(hipGraphLaunch(wh->it_graph, s), "hipGraphLaunch");
+        } else {
+            enqueue(s);
+        }
         double ss = 0.0;
         hip_check(hipMemcpyAsync(&ss, op->d_result, sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         *rel_change = std::sqrt(ss) / pi0_norm;
         if (!std::isfinite(*rel_change)) fail(WDG_ERR_NONFINITE, "WaveHoltz residual is not finite (check dt / CFL)");
+    });
+}
+
+int wdg_hip_waveholtz_set_graph(wdg_waveholtz wh, int enable)
+{
+    return guarded([&] {
+        check_wh(wh);
+        wh->use_graph = enable != 0;
     });
 }
 
