@@ -185,17 +185,29 @@ def waveholtz_solve(torch, capi, mode: str, tol: float, max_iter: int, n_grid: i
     return W, stats
 
 
-def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_per_graph):
-    """config-4 style batch: every problem solved alone with the plain driver
-    (pi0, then W <- S(W) + pi0 to tol), then all of them in one batch."""
+def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_per_graph, max_work=None):
+    """Uneven batch from the seed-4000 generator (square meshes nmin^2..nmax^2,
+    P 3..7, omega in [5 pi, 25 pi], 1-4 sources): every problem solved alone
+    with the plain driver (pi0, then W <- S(W) + pi0 to tol, each step replayed
+    from a HIP graph), then all of them in one batch.  With max_work, only the
+    generator's problems whose DoF-stages per period, ndof * nt * 5, are at
+    most max_work are kept (in generator order)."""
+    import math
+
     from wavedg_amd import meshgen
 
     cases = meshgen.config4(count, nmin, nmax)
     probs = []
     for c in cases:
         m = capi.Mesh(c["vertices"], c["elements"])
-        probs.append(dict(mesh=m, n=c["n"], omega=c["omega"], sources=c["sources"],
-                          bc=np.zeros(m.n_boundary, dtype=np.int32)))
+        n = c["n"]
+        nt = math.ceil((2.0 * math.pi / c["omega"]) / (0.5 * m.min_h() / n**2))
+        work = 3 * n * n * m.n_elem * nt * 5
+        if max_work is not None and work > max_work:
+            continue
+        probs.append(dict(mesh=m, n=n, omega=c["omega"], sources=c["sources"],
+                          bc=np.zeros(m.n_boundary, dtype=np.int32), name=c["name"], work=work))
+    print(f"[batched] {len(probs)} of {count} generated problems (max work {max_work})", file=sys.stderr, flush=True)
     # sequential plain solves
     seq_W, seq_it = [], []
     torch.cuda.synchronize()
@@ -213,22 +225,28 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
         seq_it.append(it)
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
+    print(f"[batched] sequential plain: {t_seq:.1f} s", file=sys.stderr, flush=True)
     b = capi.Batch(probs)
     t0 = time.perf_counter()
     sec, steps = b.solve(tol, max_iter, steps_per_graph)
     t_bat = time.perf_counter() - t0
-    worst, its = 0.0, []
+    print(f"[batched] batched: {t_bat:.1f} s", file=sys.stderr, flush=True)
+    worst, its, worst_res = 0.0, [], 0.0
     for k in range(len(probs)):
         W = torch.empty_like(seq_W[k])
         it, r = b.result(k, W)
         its.append(it)
+        worst_res = max(worst_res, r)
         worst = max(worst, float(torch.linalg.norm(W - seq_W[k]) / torch.linalg.norm(seq_W[k])))
     ndofs = [b.info(k)[0] for k in range(len(probs))]
     nts = [b.info(k)[1] for k in range(len(probs))]
-    return {"problems": len(probs), "mesh_range": [nmin, nmax], "ndof_range": [min(ndofs), max(ndofs)],
+    return {"problems": len(probs), "generated": count, "max_work_per_period": max_work,
+            "mesh_range": [nmin, nmax], "names": [p["name"] for p in probs],
+            "ndof_range": [min(ndofs), max(ndofs)],
             "nt_range": [min(nts), max(nts)], "tol": tol, "sequential_seconds": t_seq, "batched_seconds": t_bat,
             "batched_gpu_seconds": sec, "speedup": t_seq / t_bat, "batched_steps": steps,
             "updates_total": int(sum(its)), "updates_equal_to_sequential": its == seq_it,
+            "max_final_residual": worst_res,
             "max_rel_l2_batched_vs_sequential": worst, "steps_per_graph": steps_per_graph}
 
 
@@ -448,10 +466,12 @@ def main() -> int:
                     help="timed Advection + AdvectionHomogeneousBC actions on the same mesh (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--sweep", type=int, default=0, help="also time this many steps of the per-order roofline set")
+    ap.add_argument("--sweep", type=int, default=10,
+                    help="also time this many steps of the per-order roofline set, P = 3..7 (0 = skip)")
     ap.add_argument("--only-sweep", action="store_true", help="skip the headline workload")
     ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the copy-bandwidth calibration (0 = skip)")
-    ap.add_argument("--waveholtz", action="store_true", help="also time WaveHoltz to --tol on the L-shaped domain")
+    ap.add_argument("--waveholtz", type=int, default=1,
+                    help="also time WaveHoltz to --tol on the 384^2 L-shape, FP64 and mixed (1) or not (0)")
     ap.add_argument("--only-waveholtz", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=400)
@@ -461,6 +481,13 @@ def main() -> int:
     ap.add_argument("--batch-nmin", type=int, default=32)
     ap.add_argument("--batch-nmax", type=int, default=256)
     ap.add_argument("--batch-graph", type=int, default=16)
+    ap.add_argument("--batched", type=int, default=1,
+                    help="also time the uneven batch (seed-4000 generator, problems under --batch-max-work) against "
+                         "sequential plain solves (1) or not (0)")
+    ap.add_argument("--batch-max-work", type=float, default=5e9,
+                    help="keep the generator's problems whose ndof * nt * 5 (DoF-stages per period) is at most this")
+    ap.add_argument("--fp32-long-steps", type=int, default=2000,
+                    help="RK4 steps of the FP32-vs-FP64 error check on the headline workload (0 = skip)")
     ap.add_argument("--only-learned", action="store_true",
                     help="WaveHoltz to --tol with and without the learned warm start (held-out seeds)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "wavedg_amd",
@@ -485,7 +512,8 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.only_batched:
         r = batched_vs_sequential(torch, capi, args.batch_count, args.batch_nmin, args.batch_nmax, args.tol,
-                                  args.max_iter, args.batch_graph)
+                                  args.max_iter, args.batch_graph,
+                                  args.batch_max_work if args.batch_max_work > 0 else None)
         print(f"[batched] {r}", file=sys.stderr, flush=True)
         print(json.dumps({"batched": r}))
         return 0
@@ -582,6 +610,19 @@ def main() -> int:
             "schedule": getattr(wave32, "tuned", None),
         }
         print(f"[bench] fp32: {sec32:.3f} s, rel err {err:.3e}", file=sys.stderr, flush=True)
+        if args.fp32_long_steps > 0:
+            # the workload's full run, outside the timed region: FP32 against FP64
+            # from the same initial state after every one of its steps
+            a64 = torch.tensor(u0, dtype=torch.float64, device="cuda")
+            a32 = a64.float()
+            wave64.rk4(a64, dt, args.fp32_long_steps)
+            wave32.rk4(a32, dt, args.fp32_long_steps)
+            torch.cuda.synchronize()
+            e_long = float(torch.linalg.norm(a32.double() - a64) / torch.linalg.norm(a64))
+            out["fp32"]["rel_l2_vs_fp64_long"] = {"steps": args.fp32_long_steps, "rel_l2": e_long}
+            print(f"[bench] fp32 after {args.fp32_long_steps} steps: rel err {e_long:.3e}", file=sys.stderr,
+                  flush=True)
+            del a64, a32
         del wave32, u32
 
     if args.exact_steps > 0:
@@ -683,13 +724,23 @@ def main() -> int:
         del ws
 
     if args.waveholtz:
-        wh = {}
+        wh = {"workload": f"WaveHoltz on the L-shape ([0,1]^2 minus [0.5,1]^2) from a {args.wh_grid}^2 grid, P=5, "
+                          "omega=20pi, 3 seeded Gaussian sources (sigma 0.01), outflow, to relative fixed-point "
+                          f"residual {args.tol}; each fixed-point step replayed from a HIP graph"}
         W64, wh["f64"] = waveholtz_solve(torch, capi, "f64", args.tol, args.max_iter, args.wh_grid)
+        print(f"[waveholtz] f64: {wh['f64']}", file=sys.stderr, flush=True)
         W32, wh["mixed"] = waveholtz_solve(torch, capi, "mixed", args.tol, args.max_iter, args.wh_grid)
         wh["mixed"]["rel_l2_vs_f64"] = float(torch.linalg.norm(W32 - W64) / torch.linalg.norm(W64))
         wh["mixed"]["speedup_vs_f64"] = wh["f64"]["seconds"] / wh["mixed"]["seconds"]
+        print(f"[waveholtz] mixed: {wh['mixed']}", file=sys.stderr, flush=True)
         out["waveholtz"] = wh
         del W64, W32
+
+    if args.batched:
+        out["waveholtz_batched"] = batched_vs_sequential(
+            torch, capi, args.batch_count, args.batch_nmin, args.batch_nmax, args.tol, args.max_iter,
+            args.batch_graph, args.batch_max_work if args.batch_max_work > 0 else None)
+        print(f"[batched] speedup {out['waveholtz_batched']['speedup']:.2f}", file=sys.stderr, flush=True)
 
     if args.sweep > 0:
         out["roofline_by_P"] = sweep(torch, capi, args.sweep, [capi.F32, capi.F64])
