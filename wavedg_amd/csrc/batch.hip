@@ -65,6 +65,8 @@ struct BatchArgs {
     const double* geom;
     const int32_t* nbr;
     const int32_t* info;
+    const void* fgeo;  // wdg::FaceGeom<R> per element side
+    const wdg::FaceNb* fnb;
     const double* ctab;
     const double* ftab;
     const int* active;
@@ -136,8 +138,8 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
     size_t g;
     bool active;
     const wdg::NodeResult<R> q = wdg::apply_operator<N, R, true, true, true>(
-        u_in, a.geom + 6 * (size_t)pr.eoff, a.nbr + 4 * (size_t)pr.eoff, a.info + 4 * (size_t)pr.eoff, pr.basis,
-        pr.ha, pr.hb, blk.e0, blk.nloc, lds, el, g, active, prefetch);
+        u_in, a.geom + 6 * (size_t)pr.eoff, static_cast<const wdg::FaceGeom<R>*>(a.fgeo) + 4 * (size_t)pr.eoff,
+        a.fnb + 4 * (size_t)pr.eoff, pr.basis, (R)pr.ha, (R)pr.hb, blk.e0, blk.nloc, lds, el, g, active, prefetch);
     (void)el;
 
     double part = 0.0;
@@ -410,6 +412,8 @@ struct wdg_batch_s {
     double* d_geom = nullptr;
     int32_t* d_nbr = nullptr;
     int32_t* d_info = nullptr;
+    void* d_fgeo = nullptr;
+    wdg::FaceNb* d_fnb = nullptr;
     double* d_ctab = nullptr;
     double* d_ftab = nullptr;
     void* d_buf0 = nullptr;
@@ -439,7 +443,7 @@ struct wdg_batch_s {
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
-        void* ptrs[] = {d_best, d_stall, d_prob_ids, d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
+        void* ptrs[] = {d_best, d_stall, d_prob_ids, d_blocks, d_probs, d_basis, d_geom, d_nbr, d_info, d_fgeo, d_fnb, d_ctab, d_ftab, d_buf0, d_buf1, d_du, d_F,
                         d_acc, d_wprev, d_pi0, d_winit, d_warm, d_active, d_iters, d_partial, d_pn, d_hist, d_G,
                         d_done};
         for (void* p : ptrs)
@@ -462,6 +466,8 @@ struct wdg_batch_s {
         a.geom = d_geom;
         a.nbr = d_nbr;
         a.info = d_info;
+        a.fgeo = d_fgeo;
+        a.fnb = d_fnb;
         a.ctab = d_ctab;
         a.ftab = d_ftab;
         a.active = d_active;
@@ -545,6 +551,9 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         std::vector<BatchBlock> blocks;
         std::vector<double> basis_all, geom, ctab, ftab;
         std::vector<int32_t> nbr, info;
+        std::vector<wdg::FaceGeom<double>> fg64;
+        std::vector<wdg::FaceGeom<float>> fg32;
+        std::vector<wdg::FaceNb> fnb;
         std::vector<long long> basis_off(10, -1);
         std::vector<std::vector<double>> src_list(n_problems);
         long long soff = 0, coff = 0, foff = 0;
@@ -619,6 +628,21 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
             }
             nbr.insert(nbr.end(), ft.nbr.begin(), ft.nbr.end());
             info.insert(info.end(), ft.info.begin(), ft.info.end());
+            {
+                // side normals, half lengths and neighbour trace bases, local to the problem's state
+                const double* gk = geom.data() + geom.size() - 6 * (size_t)nel;
+                std::vector<wdg::FaceNb> fk;
+                if (precision == WDG_F32) {
+                    std::vector<wdg::FaceGeom<float>> gq;
+                    wdg::build_face_tables(n, nel, gk, ft.nbr.data(), ft.info.data(), gq, fk);
+                    fg32.insert(fg32.end(), gq.begin(), gq.end());
+                } else {
+                    std::vector<wdg::FaceGeom<double>> gq;
+                    wdg::build_face_tables(n, nel, gk, ft.nbr.data(), ft.info.data(), gq, fk);
+                    fg64.insert(fg64.end(), gq.begin(), gq.end());
+                }
+                fnb.insert(fnb.end(), fk.begin(), fk.end());
+            }
             b->probs.push_back(pr);
             b->ndof.push_back(3LL * NN * nel);
             b->omega.push_back(P.omega);
@@ -647,6 +671,15 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         bcheck(hipMemcpy(b->d_geom, geom.data(), sizeof(double) * geom.size(), hipMemcpyHostToDevice), "upload");
         bcheck(hipMemcpy(b->d_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice), "upload");
         bcheck(hipMemcpy(b->d_info, info.data(), sizeof(int32_t) * info.size(), hipMemcpyHostToDevice), "upload");
+        if (precision == WDG_F32) {
+            b->d_fgeo = dalloc<wdg::FaceGeom<float>>(fg32.size());
+            bcheck(hipMemcpy(b->d_fgeo, fg32.data(), sizeof(fg32[0]) * fg32.size(), hipMemcpyHostToDevice), "upload");
+        } else {
+            b->d_fgeo = dalloc<wdg::FaceGeom<double>>(fg64.size());
+            bcheck(hipMemcpy(b->d_fgeo, fg64.data(), sizeof(fg64[0]) * fg64.size(), hipMemcpyHostToDevice), "upload");
+        }
+        b->d_fnb = dalloc<wdg::FaceNb>(fnb.size());
+        bcheck(hipMemcpy(b->d_fnb, fnb.data(), sizeof(wdg::FaceNb) * fnb.size(), hipMemcpyHostToDevice), "upload");
         bcheck(hipMemcpy(b->d_ctab, ctab.data(), sizeof(double) * ctab.size(), hipMemcpyHostToDevice), "upload");
         bcheck(hipMemcpy(b->d_ftab, ftab.data(), sizeof(double) * ftab.size(), hipMemcpyHostToDevice), "upload");
         for (int k = 0; k < n_problems; ++k) b->probs[k].basis = b->d_basis + basis_off[b->probs[k].N];

@@ -127,6 +127,8 @@ struct wdg_wave2d_s {
     float* d_basis32 = nullptr;
     int32_t* d_nbr = nullptr;
     int32_t* d_info = nullptr;
+    void* d_fgeo = nullptr;      // FaceGeom<R> per element side (handle precision)
+    wdg::FaceNb* d_fnb = nullptr;       // neighbour trace base and flags per element side
     double* d_basis = nullptr;
     double* d_lvtab = nullptr;
     void* d_tmp = nullptr;  // RK ping-pong partner of the caller's state
@@ -173,6 +175,8 @@ struct wdg_wave2d_s {
         dfree(d_basis32);
         dfree(d_nbr);
         dfree(d_info);
+        dfree(d_fgeo);
+        dfree(d_fnb);
         dfree(d_basis);
         dfree(d_lvtab);
         dfree(d_extab);
@@ -205,6 +209,8 @@ struct wdg_wave2d_s {
         a.geom = d_geom;
         a.nbr = d_nbr;
         a.info = d_info;
+        a.fgeo = d_fgeo;
+        a.fnb = d_fnb;
         a.basis = d_basis;
         a.geom32 = d_geom32;
         a.basis32 = d_basis32;
@@ -856,6 +862,25 @@ static int create_collocated(wdg_mesh mesh, int n_colloc, int basis_type, const 
                   "upload basis");
         hip_check(hipMemcpy(op->d_lvtab, lv.data(), sizeof(double) * lv.size(), hipMemcpyHostToDevice),
                   "upload load-vector table");
+        {
+            std::vector<wdg::FaceNb> fnb;
+            if (precision == WDG_F32) {
+                std::vector<wdg::FaceGeom<float>> fg;
+                wdg::build_face_tables(n, op->nel, geom.data(), ft.nbr.data(), ft.info.data(), fg, fnb);
+                op->d_fgeo = dmalloc(sizeof(fg[0]) * fg.size());
+                hip_check(hipMemcpy(op->d_fgeo, fg.data(), sizeof(fg[0]) * fg.size(), hipMemcpyHostToDevice),
+                          "upload face geometry");
+            } else {
+                std::vector<wdg::FaceGeom<double>> fg;
+                wdg::build_face_tables(n, op->nel, geom.data(), ft.nbr.data(), ft.info.data(), fg, fnb);
+                op->d_fgeo = dmalloc(sizeof(fg[0]) * fg.size());
+                hip_check(hipMemcpy(op->d_fgeo, fg.data(), sizeof(fg[0]) * fg.size(), hipMemcpyHostToDevice),
+                          "upload face geometry");
+            }
+            op->d_fnb = static_cast<wdg::FaceNb*>(dmalloc(sizeof(wdg::FaceNb) * fnb.size()));
+            hip_check(hipMemcpy(op->d_fnb, fnb.data(), sizeof(wdg::FaceNb) * fnb.size(), hipMemcpyHostToDevice),
+                      "upload face neighbours");
+        }
         if (precision == WDG_F32) {
             const std::vector<float> g32(geom.begin(), geom.end()), b32(basis.begin(), basis.end());
             op->d_geom32 = static_cast<float*>(dmalloc(sizeof(float) * g32.size()));

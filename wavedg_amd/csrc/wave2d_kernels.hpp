@@ -22,10 +22,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
 #include <type_traits>
+
+#include "host_faces.hpp"
 
 namespace wdg {
 
@@ -47,7 +50,11 @@ struct KArgs {
     const float* geom32;    // FP32 handles: geom and basis rounded to float once
     const float* basis32;
     int leg;                // 1: Gauss-Legendre basis (faces interpolate with P)
+    const void* fgeo;       // (4, nel) FaceGeom<R>: side normal and half length (build_face_tables)
+    const FaceNb* fnb;      // (4, nel) neighbour trace base and flags (build_face_tables)
+    float ha32, hb32, dt32, rk_a32, rk_b32, fs32;  // the scalars above rounded to float (FP32 handles)
 };
+
 
 enum : int {
     M_VOL = 1,
@@ -96,9 +103,6 @@ constexpr int kModeOf[K_COUNT] = {
 #endif
 #ifndef WDG_MIN_WAVES_F32
 #define WDG_MIN_WAVES_F32 WDG_MIN_WAVES  // the same for the FP32 stage kernels
-#endif
-#ifndef WDG_FACE_GEOM_DOUBLE
-#define WDG_FACE_GEOM_DOUBLE 0  // 1: face normals and weights in double for every precision
 #endif
 #ifndef WDG_NT_STORES
 #define WDG_NT_STORES 0  // 1: stream stage outputs with non-temporal stores
@@ -174,9 +178,10 @@ __device__ __forceinline__ float rsqrt_nr(float x)
 template <int N>
 __device__ __forceinline__ int side_node(int s, int k)
 {
-    const int i = (s == 0 || s == 2) ? k : (s == 1 ? N - 1 : 0);
-    const int j = (s == 1 || s == 3) ? k : (s == 2 ? N - 1 : 0);
-    return i + N * j;
+    // sides 0 and 2 run along i (step 1), sides 1 and 3 along j (step N),
+    // starting at node 0 (sides 0, 3), N-1 (side 1) or N(N-1) (side 2)
+    const int start = s == 1 ? N - 1 : (s == 2 ? N * (N - 1) : 0);
+    return start + k * ((s & 1) ? N : 1);
 }
 
 // Step from a node of side s one row into the element: node m of the line
@@ -197,13 +202,13 @@ struct OpLds {
     static constexpr int NXW = LEG ? 3 * N : 2 * N;  // x[N], w[N] (, P[N])
     static constexpr size_t n_u = (size_t)EPB * 3 * NN;
     static constexpr size_t n_F = VOL ? n_u : 0, n_G = VOL ? n_u : 0;
-    static constexpr size_t n_S = FACE ? (size_t)EPB * NF * 3 : 0;
+    static constexpr size_t n_S = FACE ? (size_t)EPB * (NF + 1) * 3 : 0;  // slot NF of an element: zeros
     static constexpr size_t off_xw = 0;                                  // double x[N], w[N] (, P[N])
     static constexpr size_t off_u = ((NXW * 8 + 15) / 16) * 16;
     static constexpr size_t off_F = off_u + n_u * sizeof(R);
     static constexpr size_t off_G = off_F + n_F * sizeof(R);
     static constexpr size_t off_S = off_G + n_G * sizeof(R);
-    static constexpr size_t off_D = off_S + n_S * sizeof(R);
+    static constexpr size_t off_D = ((off_S + n_S * sizeof(R) + 15) / 16) * 16;  // 16-byte rows for wide reads
     static constexpr size_t bytes = ((off_D + NN * sizeof(R) + 15) / 16) * 16;
 };
 
@@ -229,9 +234,9 @@ struct NodeResult {
 template <int N, typename R, bool VOL, bool IFACE, bool BFACE, int TGT = 256, bool LEG = false, typename GT = double,
           typename BT = double, typename Prefetch>
 __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_in, const GT* __restrict__ geom,
-                                                        const int32_t* __restrict__ nbr,
-                                                        const int32_t* __restrict__ info,
-                                                        const BT* __restrict__ basis, double ha_, double hb_,
+                                                        const FaceGeom<R>* __restrict__ fgeo,
+                                                        const FaceNb* __restrict__ fnb,
+                                                        const BT* __restrict__ basis, R ha, R hb,
                                                         int e0, int nloc, unsigned char* lds, int& el_out,
                                                         size_t& g_out, bool& active_out, Prefetch&& prefetch)
 {
@@ -278,45 +283,38 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     active_out = active;
 
     // Bilinear element map x(xi, eta) = x0 + a xi + c eta + h xi eta (per
-    // coordinate), so dx/dxi = a + h eta, dx/deta = c + h xi (Element.cpp:21-27)
-    // and side s runs along 2(a -+ h) (s = 0, 2) or 2(c +- h) (s = 1, 3).
+    // coordinate), so dx/dxi = a + h eta, dx/deta = c + h xi (Element.cpp:21-27).
     // Volume metrics are evaluated in the state precision R (double for FP64
-    // handles); face normals and weights in FG.
-    using FG = typename std::conditional<WDG_FACE_GEOM_DOUBLE, double, R>::type;
+    // handles); face normals and half lengths come from the fgeo table.
     using GT2 = typename std::conditional<sizeof(GT) == 8, double2, float2>::type;
     const GT2* gp = reinterpret_cast<const GT2*>(geom + 6 * (size_t)e);
     const GT2 g_a = gp[0], g_c = gp[1], g_h = gp[2];
     const R ax = (R)g_a.x, ay = (R)g_a.y, cx = (R)g_c.x, cy = (R)g_c.y, hx = (R)g_h.x, hy = (R)g_h.y;
-    const FG fax = (FG)g_a.x, fay = (FG)g_a.y, fcx = (FG)g_c.x, fcy = (FG)g_c.y, fhx = (FG)g_h.x, fhy = (FG)g_h.y;
 
-    // Face lanes: face node f = node + p NN (side f / N, position f % N).
+    // Face lanes: face node f = node + p NN (side f / N, position f % N),
+    // with the side's table entries and the neighbour's trace.
     R nbv[FP][3];
-    int finfo[FP], fnb[FP];
+    int finfo[FP], fnb3[FP];
+    // Orders N >= 6 read each side's normal and half length from the fgeo
+    // table; lower orders, with more elements per DoF, compute them from the
+    // map coefficients instead of spending HBM bytes on the table.
+    constexpr bool FTAB = N >= 6;
+    FaceGeom<R> fgv[FP];
     if constexpr (FACE) {
-        int4 nb4 = make_int4(0, 0, 0, 0), in4 = make_int4(0, 0, 0, 0);
-        if constexpr (NN % 64 == 0) {
-            nb4 = *reinterpret_cast<const int4*>(nbr + 4 * (size_t)e);
-            in4 = *reinterpret_cast<const int4*>(info + 4 * (size_t)e);
-        }
 #pragma unroll
         for (int p = 0; p < FP; ++p) {
             const int f = node + p * NN;
             const bool fon = active && f < NF;
             const int s = fon ? f / N : 0, k = fon ? f - (f / N) * N : 0;
-            if constexpr (NN % 64 == 0) {
-                fnb[p] = s == 0 ? nb4.x : s == 1 ? nb4.y : s == 2 ? nb4.z : nb4.w;
-                finfo[p] = s == 0 ? in4.x : s == 1 ? in4.y : s == 2 ? in4.z : in4.w;
-            } else {
-                fnb[p] = nbr[4 * (size_t)e + s];
-                finfo[p] = info[4 * (size_t)e + s];
-            }
+            const FaceNb fx = fnb[4 * (size_t)e + s];
+            if constexpr (FTAB) fgv[p] = fgeo[4 * (size_t)e + s];
+            fnb3[p] = fx.x;
+            finfo[p] = fx.y;
             nbv[p][0] = nbv[p][1] = nbv[p][2] = (R)0;
-            if (IFACE && fon && fnb[p] >= 0) {
-                const int inf = finfo[p];
-                const int kk = ((inf >> 2) & 1) ? (N - 1 - k) : k;
-                const R* q = u_in + ((size_t)fnb[p] * NN + side_node<N>(inf & 3, kk)) * 3;
+            if (IFACE && fon && fx.x >= 0) {
+                const R* q = u_in + (fx.x + k * (fx.y >> 16));
                 if constexpr (LEG) {
-                    const int st = 3 * inward_stride<N>(inf & 3);
+                    const int st = 3 * inward_stride<N>(fx.y & 3);
 #pragma unroll
                     for (int m = 0; m < N; ++m) {
                         const R Pm = (R)basis[2 * N + NN + m];
@@ -378,27 +376,35 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
         // A and the other trace B, the contribution is
         //   c_p = w ds [ha n.(uA+uB) + hb (pA-pB)],  c_u = n w ds [ha (pA+pB) + hb n.(uA-uB)]
         // which equals +f on el0 and -f on el1 for the reference's edge flux f.
-        const R ha = (R)ha_, hb = (R)hb_;
 #pragma unroll
         for (int pp = 0; pp < FP; ++pp) {
             const int f = node + pp * NN;
             if (!(active && f < NF)) continue;
             const int s = f / N, k = f - s * N;
             const int inf = finfo[pp];
-            const bool interior = fnb[pp] >= 0;
+            const bool interior = fnb3[pp] >= 0;
             R c0 = 0, c1 = 0, c2 = 0;
             if (interior ? IFACE : BFACE) {
-                const bool flip = (inf >> 2) & 1, first = (inf >> 3) & 1;
-                // own side vector in traversal order; sides 2, 3 run clockwise
-                const FG sh = (s == 0 || s == 3) ? (FG)-1 : (FG)1;
-                const FG dx = (FG)2 * ((s & 1) ? fcx + sh * fhx : fax + sh * fhx);
-                const FG dy = (FG)2 * ((s & 1) ? fcy + sh * fhy : fay + sh * fhy);
-                const FG l2 = dx * dx + dy * dy;
-                const FG il = rsqrt_nr(l2);  // 1/len
-                const FG sil = (s >= 2) ? -il : il;
-                const R n0 = (R)(dy * sil), n1 = (R)(-dx * sil);
-                const int ie = (!first && flip) ? (N - 1 - k) : k;  // node index in el0's order
-                const R wds = (R)((FG)s_w[ie] * ((FG)0.5 * (l2 * il)));
+                R n0, n1, hl;
+                if constexpr (FTAB) {
+                    n0 = fgv[pp].n0;
+                    n1 = fgv[pp].n1;
+                    hl = fgv[pp].hl;
+                } else {
+                    // side s runs along 2(a -+ h) (s = 0, 2) or 2(c +- h) (s = 1, 3),
+                    // sides 2 and 3 clockwise
+                    const R sh = (s == 0 || s == 3) ? (R)-1 : (R)1;
+                    const R dx = (R)2 * ((s & 1) ? cx + sh * hx : ax + sh * hx);
+                    const R dy = (R)2 * ((s & 1) ? cy + sh * hy : ay + sh * hy);
+                    const R l2 = dx * dx + dy * dy;
+                    const R il = rsqrt_nr(l2);  // 1/len
+                    const R sil = (s >= 2) ? -il : il;
+                    n0 = dy * sil;
+                    n1 = -dx * sil;
+                    hl = (R)0.5 * (l2 * il);
+                }
+                const int ie = ((inf >> 6) & 1) ? (N - 1 - k) : k;  // node index in el0's order
+                const R wds = (R)s_w[ie] * hl;
                 const int ln = side_node<N>(s, k);
                 R pA, uA, vA;
                 if constexpr (LEG) {
@@ -445,11 +451,12 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                 c1 = n0 * cf;
                 c2 = n1 * cf;
             }
-            R* S = s_S + (el * NF + f) * 3;
+            R* S = s_S + (el * (NF + 1) + f) * 3;
             S[0] = c0;
             S[1] = c1;
             S[2] = c2;
         }
+        if (active && node < 3) s_S[(el * (NF + 1) + NF) * 3 + node] = (R)0;
     }
     __syncthreads();
 
@@ -470,8 +477,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
         }
     }
     if constexpr (FACE) {
-        // add this node's face contributions in side order 0..3
-        const R* S = s_S + el * NF * 3;
+        const R* S = s_S + el * (NF + 1) * 3;
         if constexpr (LEG) {
             // node (i, j) is row j of side 0, row N-1-i of side 1, row N-1-j
             // of side 2 and row i of side 3 (legendre_transpose_2d,
@@ -485,26 +491,16 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
             r1 += q0 * S0[1] + q1 * S1[1] + q2 * S2[1] + q3 * S3[1];
             r2 += q0 * S0[2] + q1 * S1[2] + q2 * S2[2] + q3 * S3[2];
         } else {
-        if (j == 0) {
-            r0 += S[(0 * N + i) * 3 + 0];
-            r1 += S[(0 * N + i) * 3 + 1];
-            r2 += S[(0 * N + i) * 3 + 2];
-        }
-        if (i == N - 1) {
-            r0 += S[(1 * N + j) * 3 + 0];
-            r1 += S[(1 * N + j) * 3 + 1];
-            r2 += S[(1 * N + j) * 3 + 2];
-        }
-        if (j == N - 1) {
-            r0 += S[(2 * N + i) * 3 + 0];
-            r1 += S[(2 * N + i) * 3 + 1];
-            r2 += S[(2 * N + i) * 3 + 2];
-        }
-        if (i == 0) {
-            r0 += S[(3 * N + j) * 3 + 0];
-            r1 += S[(3 * N + j) * 3 + 1];
-            r2 += S[(3 * N + j) * 3 + 2];
-        }
+            // node (i, j) lies on at most one side of each direction: side 0
+            // (j = 0) or 2 (j = N-1), and side 3 (i = 0) or 1 (i = N-1); other
+            // nodes read the zero slot NF
+            const int fa = j == 0 ? i : (j == N - 1 ? 2 * N + i : NF);
+            const int fb = i == N - 1 ? N + j : (i == 0 ? 3 * N + j : NF);
+            const R* Sa = S + fa * 3;
+            const R* Sb = S + fb * 3;
+            r0 += Sa[0] + Sb[0];
+            r1 += Sa[1] + Sb[1];
+            r2 += Sa[2] + Sb[2];
         }
     }
     out.r0 = r0;
@@ -571,12 +567,16 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     size_t g;
     bool active;
     NodeResult<R> q;
-    if constexpr (std::is_same<R, float>::value && !WDG_FACE_GEOM_DOUBLE)
-        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom32, a.nbr, a.info,
-                                                         a.basis32, a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
+    constexpr bool F32 = std::is_same<R, float>::value;
+    const FaceGeom<R>* fgeo = static_cast<const FaceGeom<R>*>(a.fgeo);
+    if constexpr (F32)
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom32, fgeo, a.fnb,
+                                                         a.basis32, a.ha32, a.hb32, e0, nloc, lds, el, g, active,
+                                                         prefetch);
     else
-        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom, a.nbr, a.info, a.basis,
-                                                         a.ha, a.hb, e0, nloc, lds, el, g, active, prefetch);
+        q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom, fgeo, a.fnb,
+                                                         a.basis, (R)a.ha, (R)a.hb, e0, nloc, lds, el, g, active,
+                                                         prefetch);
     if (!active) return;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
@@ -587,7 +587,7 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     } else {
         const R im = recip(q.m);
         if constexpr (FORCE) {
-            const R fs = (R)a.fs;
+            const R fs = F32 ? (R)a.fs32 : (R)a.fs;
             r0 += fv0 * fs;
             r1 += fv1 * fs;
             r2 += fv2 * fs;
@@ -604,14 +604,14 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
             // low-storage RK stage (ode.hpp:74-90)
             R* du = static_cast<R*>(a.du);
             R* uo = static_cast<R*>(a.u_out);
-            const R dt = (R)a.dt, rb = (R)a.rk_b;
+            const R dt = F32 ? (R)a.dt32 : (R)a.dt, rb = F32 ? (R)a.rk_b32 : (R)a.rk_b;
             R d0, d1, d2;
             if constexpr (OUT == M_OUT_STAGE0) {
                 d0 = dt * r0;
                 d1 = dt * r1;
                 d2 = dt * r2;
             } else {
-                const R ra = (R)a.rk_a;
+                const R ra = F32 ? (R)a.rk_a32 : (R)a.rk_a;
                 d0 = ra * pre0 + dt * r0;
                 d1 = ra * pre1 + dt * r1;
                 d2 = ra * pre2 + dt * r2;

@@ -118,9 +118,17 @@ void launch_typed(int kid, int sched, const KArgs& a, hipStream_t s)
 void WDG_CAT(launch_wave_n, WDG_N)(int kid, int f32, int sched, const KArgs& a, hipStream_t s)
 {
     if (a.nel <= 0) return;
-    if (f32)
-        launch_typed<float>(kid, sched, a, s);
-    else
+    if (f32) {
+        // the FP32 kernels take their scalars as float kernel arguments
+        KArgs b = a;
+        b.ha32 = (float)a.ha;
+        b.hb32 = (float)a.hb;
+        b.dt32 = (float)a.dt;
+        b.rk_a32 = (float)a.rk_a;
+        b.rk_b32 = (float)a.rk_b;
+        b.fs32 = (float)a.fs;
+        launch_typed<float>(kid, sched, b, s);
+    } else
         launch_typed<double>(kid, sched, a, s);
 }
 
