@@ -107,39 +107,41 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
 
     R pre0 = 0, pre1 = 0, pre2 = 0, f0 = 0, f1 = 0, f2 = 0;
     double ac0 = 0, ac1 = 0, ac2 = 0, x0 = 0, x1 = 0, x2 = 0;
-    auto prefetch = [&](bool act, size_t g) {
+    auto prefetch = [&](bool act, size_t gb, uint32_t g) {
         if (!act) return;
         if constexpr (KIND != 0) {
-            pre0 = du[g + 0];
-            pre1 = du[g + 1];
-            pre2 = du[g + 2];
+            const R* dub = du + gb;
+            pre0 = dub[g + 0];
+            pre1 = dub[g + 1];
+            pre2 = dub[g + 2];
         }
         if (forced) {
-            const R* F = static_cast<const R*>(a.F) + pr.soff;
+            const R* F = static_cast<const R*>(a.F) + pr.soff + gb;
             f0 = F[g + 0];
             f1 = F[g + 1];
             f2 = F[g + 2];
         }
         if constexpr (KIND == 2) {
-            const double* ac = a.acc + pr.soff;
+            const double* ac = a.acc + pr.soff + gb;
             ac0 = ac[g + 0];
             ac1 = ac[g + 1];
             ac2 = ac[g + 2];
             if (period_end) {
                 // period 0 hands over to the initial iterate; later periods need pi0 and W_k
-                const double* x = it == 0 ? a.winit : (a.pi0);
-                x0 = x[pr.soff + g + 0];
-                x1 = x[pr.soff + g + 1];
-                x2 = x[pr.soff + g + 2];
+                const double* x = (it == 0 ? a.winit : a.pi0) + pr.soff + gb;
+                x0 = x[g + 0];
+                x1 = x[g + 1];
+                x2 = x[g + 2];
             }
         }
     };
     int el;
-    size_t g;
+    size_t gb;
+    uint32_t g;
     bool active;
     const wdg::NodeResult<R> q = wdg::apply_operator<N, R, true, true, true>(
         u_in, a.geom + 6 * (size_t)pr.eoff, static_cast<const wdg::FaceGeom<R>*>(a.fgeo) + 4 * (size_t)pr.eoff,
-        a.fnb + 4 * (size_t)pr.eoff, pr.basis, (R)pr.ha, (R)pr.hb, blk.e0, blk.nloc, lds, el, g, active, prefetch);
+        a.fnb + 4 * (size_t)pr.eoff, pr.basis, (R)pr.ha, (R)pr.hb, blk.e0, blk.nloc, lds, el, gb, g, active, prefetch);
     (void)el;
 
     double part = 0.0;
@@ -168,10 +170,14 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
             d1 = ra * pre1 + dt * r1;
             d2 = ra * pre2 + dt * r2;
         }
+        du += gb;
+        u_out += gb;
         du[g + 0] = d0;
         du[g + 1] = d1;
         du[g + 2] = d2;
-        const R w0 = q.p + rb * d0, w1 = q.uu + rb * d1, w2 = q.vv + rb * d2;
+        R w0, w1, w2;
+        wdg::stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, wdg::sr_seed((int)s, st), w0,
+                          w1, w2);
         if constexpr (KIND != 2) {
             u_out[g + 0] = w0;
             u_out[g + 1] = w1;
@@ -179,7 +185,7 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
         } else {
             const double c = a.ctab[pr.coff + s + 1];
             const double n0 = ac0 + c * (double)w0, n1 = ac1 + c * (double)w1, n2 = ac2 + c * (double)w2;
-            double* ac = a.acc + pr.soff;
+            double* ac = a.acc + pr.soff + gb;
             if (!period_end) {
                 u_out[g + 0] = w0;
                 u_out[g + 1] = w1;
@@ -189,9 +195,9 @@ __device__ __forceinline__ void batch_body(const BatchArgs& a, const BatchBlock&
                 ac[g + 2] = n2;
             } else {
                 double W0, W1, W2;
-                double* wp = a.wprev + pr.soff;
+                double* wp = a.wprev + pr.soff + gb;
                 if (it == 0) {
-                    double* pz = a.pi0 + pr.soff;
+                    double* pz = a.pi0 + pr.soff + gb;
                     pz[g + 0] = n0;
                     pz[g + 1] = n1;
                     pz[g + 2] = n2;

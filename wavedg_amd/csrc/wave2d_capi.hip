@@ -462,6 +462,7 @@ void* run_steps(wdg_wave2d op, double dt, double& t, void* u, int n_steps, const
             a.u_out = nxt;
             a.rk_a = RK4A[st];
             a.rk_b = RK4B[st];
+            a.sr_seed = wdg::sr_seed(it, st);
             int kid;
             if (st == 0) {
                 kid = force ? wdg::K_STAGE0_F : wdg::K_STAGE0;

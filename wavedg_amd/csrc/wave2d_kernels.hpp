@@ -53,7 +53,51 @@ struct KArgs {
     const void* fgeo;       // (4, nel) FaceGeom<R>: side normal and half length (build_face_tables)
     const FaceNb* fnb;      // (4, nel) neighbour trace base and flags (build_face_tables)
     float ha32, hb32, dt32, rk_a32, rk_b32, fs32;  // the scalars above rounded to float (FP32 handles)
+    uint32_t sr_seed;       // FP32 stages: seed of the state update's stochastic rounding (sr_seed())
 };
+
+// Stochastic rounding of the FP32 state update u + b du.  Rounding to nearest
+// loses the same fraction of an ulp stage after stage when b du is small and
+// slowly varying next to u, so the FP32 error grows with the number of
+// stages; rounding up with probability equal to the discarded fraction makes
+// each rounding error zero on average.  The sum of two floats is exact in
+// double (to within 2^-29 of the smaller one); adding a uniform 29-bit integer
+// to the 29 mantissa bits a float drops, then clearing them, rounds the
+// magnitude up with exactly that probability (for either sign).
+__host__ __device__ inline uint32_t sr_seed(int step, int stage)
+{
+    return (uint32_t)(step * 5 + stage + 1) * 0x9E3779B9u;
+}
+__device__ __forceinline__ uint32_t sr_hash(uint32_t idx, uint32_t seed)
+{
+    uint32_t h = (idx ^ seed) * 0x85EBCA6Bu;  // MurmurHash3 finaliser
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
+}
+__device__ __forceinline__ float add_sr(float u, float bd, uint32_t r29)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong((double)u + (double)bd);
+    b = (b + r29) & ~0x1FFFFFFFull;
+    return (float)__longlong_as_double((long long)b);
+}
+// u + bd for the three variables of a node: stochastic rounding in float
+// (random bits from the node index and seed), plain arithmetic in double.
+template <typename R>
+__device__ __forceinline__ void stage_update(R p, R uu, R vv, R bd0, R bd1, R bd2, uint32_t idx, uint32_t seed,
+                                             R& w0, R& w1, R& w2)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        const uint32_t h = sr_hash(idx, seed);
+        w0 = add_sr(p, bd0, h >> 3);
+        w1 = add_sr(uu, bd1, (h * 0x9E3779B1u) >> 3);
+        w2 = add_sr(vv, bd2, (h * 0x27D4EB2Fu) >> 3);
+    } else {
+        w0 = p + bd0;
+        w1 = uu + bd1;
+        w2 = vv + bd2;
+    }
+}
 
 
 enum : int {
@@ -224,7 +268,10 @@ struct NodeResult {
 // The operator for one workgroup of elements [e0, e0 + nloc) of one mesh,
 // run by threads 0 .. Shape<N, TGT>::NT-1 of the workgroup (threads beyond only
 // join the barriers).  Element indices in nbr are local to u_in / geom.
-// `prefetch(active, g)` is called after the state loads are issued and
+// The node's values are at [gb + g, gb + g + 3) of every state-shaped array:
+// gb is the workgroup's first element (uniform, 64-bit), g the node's 32-bit
+// offset from it.  `prefetch(active, gb, g)` is called after the state loads
+// are issued and
 // before the first barrier, so the caller's extra loads overlap them.
 // GT and BT, the types of the geometry and basis tables, are double, or
 // float for FP32 handles that keep float copies (the rounding is the same as
@@ -238,7 +285,8 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
                                                         const FaceNb* __restrict__ fnb,
                                                         const BT* __restrict__ basis, R ha, R hb,
                                                         int e0, int nloc, unsigned char* lds, int& el_out,
-                                                        size_t& g_out, bool& active_out, Prefetch&& prefetch)
+                                                        size_t& gb_out, uint32_t& g_out, bool& active_out,
+                                                        Prefetch&& prefetch)
 {
     constexpr int NN = N * N;
     constexpr int NT = Shape<N, TGT>::NT;
@@ -277,8 +325,10 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     const bool active = worker && el < nloc;
     int e = e0 + (active ? el : 0);
     if constexpr (NN % 64 == 0) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave
-    const size_t g = ((size_t)e * NN + (active ? node : 0)) * 3;
+    const size_t gb = (size_t)e0 * NN * 3;
+    const uint32_t g = (uint32_t)(((active ? el : 0) * NN + (active ? node : 0)) * 3);
     el_out = el;
+    gb_out = gb;
     g_out = g;
     active_out = active;
 
@@ -330,7 +380,7 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
             }
         }
     }
-    prefetch(active, g);
+    prefetch(active, gb, g);
 
     if (worker) {
 #pragma unroll
@@ -538,49 +588,51 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     // RK register / output / forcing / accumulator values, loaded early
     R pre0 = 0, pre1 = 0, pre2 = 0, fv0 = 0, fv1 = 0, fv2 = 0;
     double ac0 = 0, ac1 = 0, ac2 = 0;
-    auto prefetch = [&](bool act, size_t g) {
+    auto prefetch = [&](bool act, size_t gb, uint32_t g) {
         if (!act) return;
         if constexpr (READ_DU) {
-            const R* du = static_cast<const R*>(a.du);
+            const R* du = static_cast<const R*>(a.du) + gb;
             pre0 = du[g + 0];
             pre1 = du[g + 1];
             pre2 = du[g + 2];
         } else if constexpr (OUT == M_OUT_ACCUM) {
-            const R* y = static_cast<const R*>(a.u_out);
+            const R* y = static_cast<const R*>(a.u_out) + gb;
             pre0 = y[g + 0];
             pre1 = y[g + 1];
             pre2 = y[g + 2];
         }
         if constexpr (FORCE) {
-            const R* Fv = static_cast<const R*>(a.force);
+            const R* Fv = static_cast<const R*>(a.force) + gb;
             fv0 = Fv[g + 0];
             fv1 = Fv[g + 1];
             fv2 = Fv[g + 2];
         }
         if constexpr (ACC) {
-            ac0 = a.acc[g + 0];
-            ac1 = a.acc[g + 1];
-            ac2 = a.acc[g + 2];
+            const double* ac = a.acc + gb;
+            ac0 = ac[g + 0];
+            ac1 = ac[g + 1];
+            ac2 = ac[g + 2];
         }
     };
     int el;
-    size_t g;
+    size_t gb;
+    uint32_t g;
     bool active;
     NodeResult<R> q;
     constexpr bool F32 = std::is_same<R, float>::value;
     const FaceGeom<R>* fgeo = static_cast<const FaceGeom<R>*>(a.fgeo);
     if constexpr (F32)
         q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom32, fgeo, a.fnb,
-                                                         a.basis32, a.ha32, a.hb32, e0, nloc, lds, el, g, active,
+                                                         a.basis32, a.ha32, a.hb32, e0, nloc, lds, el, gb, g, active,
                                                          prefetch);
     else
         q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom, fgeo, a.fnb,
-                                                         a.basis, (R)a.ha, (R)a.hb, e0, nloc, lds, el, g, active,
+                                                         a.basis, (R)a.ha, (R)a.hb, e0, nloc, lds, el, gb, g, active,
                                                          prefetch);
     if (!active) return;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
-        R* y = static_cast<R*>(a.u_out);
+        R* y = static_cast<R*>(a.u_out) + gb;
         y[g + 0] = pre0 + r0;
         y[g + 1] = pre1 + r1;
         y[g + 2] = pre2 + r2;
@@ -596,14 +648,14 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
         r1 *= im;
         r2 *= im;
         if constexpr (OUT == M_OUT_RHS) {
-            R* y = static_cast<R*>(a.u_out);
+            R* y = static_cast<R*>(a.u_out) + gb;
             y[g + 0] = r0;
             y[g + 1] = r1;
             y[g + 2] = r2;
         } else {
             // low-storage RK stage (ode.hpp:74-90)
-            R* du = static_cast<R*>(a.du);
-            R* uo = static_cast<R*>(a.u_out);
+            R* du = static_cast<R*>(a.du) + gb;
+            R* uo = static_cast<R*>(a.u_out) + gb;
             const R dt = F32 ? (R)a.dt32 : (R)a.dt, rb = F32 ? (R)a.rk_b32 : (R)a.rk_b;
             R d0, d1, d2;
             if constexpr (OUT == M_OUT_STAGE0) {
@@ -619,16 +671,18 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
             st_stream(du + g + 0, d0);
             st_stream(du + g + 1, d1);
             st_stream(du + g + 2, d2);
-            const R w0 = q.p + rb * d0, w1 = q.uu + rb * d1, w2 = q.vv + rb * d2;
+            R w0, w1, w2;
+            stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
             st_stream(uo + g + 0, w0);
             st_stream(uo + g + 1, w1);
             st_stream(uo + g + 2, w2);
             if constexpr (ACC) {
                 // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
                 const double acw = a.ac;
-                a.acc[g + 0] = ac0 + acw * (double)w0;
-                a.acc[g + 1] = ac1 + acw * (double)w1;
-                a.acc[g + 2] = ac2 + acw * (double)w2;
+                double* ac = a.acc + gb;
+                ac[g + 0] = ac0 + acw * (double)w0;
+                ac[g + 1] = ac1 + acw * (double)w1;
+                ac[g + 2] = ac2 + acw * (double)w2;
             }
         }
     }
