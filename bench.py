@@ -161,10 +161,12 @@ def waveholtz_solve(torch, capi, mode: str, tol: float, max_iter: int, n_grid: i
     mesh = capi.Mesh(c["vertices"], c["elements"])
     bc = np.full(mesh.n_boundary, c["bc_code"], dtype=np.int32)
     wave64 = capi.Wave2D(mesh, c["n"], bc, precision=capi.F64)
+    wave64.autotune(reps=10)  # setup: the stage schedule for this order and precision
     wh64 = capi.WaveHoltz(wave64, c["omega"])
     wh32 = None
     if mode == "mixed":
         wave32 = capi.Wave2D(mesh, c["n"], bc, precision=capi.F32)
+        wave32.autotune(reps=10)
         wh32 = capi.WaveHoltz(wave32, c["omega"])
     F = torch.empty(wave64.ndof, dtype=torch.float64, device="cuda")
     pi0 = torch.empty_like(F)
@@ -181,7 +183,9 @@ def waveholtz_solve(torch, capi, mode: str, tol: float, max_iter: int, n_grid: i
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     stats = {"mode": mode, "iterations_f32": it32, "iterations_f64": it64, "seconds": sec, "final_residual": res,
-             "converged": res <= tol, "nt": wh64.nt, "ndof": wave64.ndof, "tol": tol}
+             "converged": res <= tol, "nt": wh64.nt, "ndof": wave64.ndof, "tol": tol,
+             "stage_schedule_threads": {"f64": wave64.schedule()[1],
+                                        **({"f32": wave32.schedule()[1]} if mode == "mixed" else {})}}
     return W, stats
 
 
@@ -452,7 +456,7 @@ def main() -> int:
     ap.add_argument("--cnn-batch", type=int, default=16)
     ap.add_argument("--cnn-steps", type=int, default=50)
     ap.add_argument("--autotune", type=int, default=1, help="pick the RK-stage schedule per order first (1) or not")
-    ap.add_argument("--learned", type=int, default=16,
+    ap.add_argument("--learned", type=int, default=8,
                     help="held-out problems for the plain vs batched+learned WaveHoltz line (0 = skip)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--legendre-steps", type=int, default=400,
@@ -491,7 +495,7 @@ def main() -> int:
     ap.add_argument("--only-learned", action="store_true",
                     help="WaveHoltz to --tol with and without the learned warm start (held-out seeds)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "wavedg_amd",
-                                                      "weights", "warmstart_m64_p4.npz"))
+                                                      "weights", "warmstart_m128_p4.npz"))
     ap.add_argument("--learned-count", type=int, default=64)
     ap.add_argument("--learned-sequential", type=int, default=1,
                     help="also time the plain per-problem GPU solves (1) or not (0)")
@@ -717,8 +721,10 @@ def main() -> int:
         r = warmstart.evaluate(ws, seeds, args.tol, args.max_iter, args.batch_graph, sequential=True,
                                log=lambda m: print(f"[learned] {m}", file=sys.stderr, flush=True))
         out["waveholtz_learned"] = {
-            "workload": f"{args.learned} held-out WaveHoltz problems: 64x64 quads on the unit square, P=4, outflow, "
-                        "omega in [5pi, 20pi], 1-4 Gaussian sources; tol 1e-6",
+            "workload": f"{args.learned} held-out WaveHoltz problems: {ws.M}x{ws.M} quads on the unit square, P=4, "
+                        "outflow, omega in [5pi, 20pi], 1-4 Gaussian sources (sigma 0.005-0.03); tol 1e-6; "
+                        "plain = FP64 per-problem solves with graph-replayed fixed-point steps",
+            "weights": os.path.basename(args.weights),
             **{k: v for k, v in r.items() if not isinstance(v, list)},
         }
         del ws

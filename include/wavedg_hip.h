@@ -163,7 +163,10 @@ int wdg_hip_divf2d_linear(wdg_wave2d op, int n_var, const double *A, const doubl
  * 136-202) for a function the caller has evaluated at the quadrature points:
  * d_fq (n_var, nq, nq, n_elem) holds f at the nq-point Gauss-Legendre rule
  * mapped into every element (wdg_hip_mesh_gl_points; nq = 0 means n_colloc,
- * the reference's default); d_F (n_var, n, n, n_elem) <- (f, phi_ij). */
+ * the reference's default); d_F (n_var, n, n, n_elem) <- (f, phi_ij).
+ * The first call with a given nq builds the rule's table on the handle with
+ * a blocking upload (not capturable in a graph); later calls with that nq
+ * only enqueue the kernel on `stream`. */
 int wdg_hip_linear_functional(wdg_wave2d op, int n_var, int nq, const double *d_fq, double *d_F, void *stream);
 /* element_metrics(quad).physical_coordinates() (Mesh2D.cpp:687-697) for the
  * nq-point Gauss-Legendre rule: out is (2, nq, nq, n_elem), host memory. */

@@ -1,8 +1,8 @@
-"""What the trained network gains: WaveHoltz to 1e-6 on held-out problems
-(64 x 64 quads, P = 4, seeds never used in training), batched, with and
-without the learned warm start.  The warm-started solve must converge to the
-same fixed point (the plain iteration judges the residual) in fewer
-updates."""
+"""What the trained networks gain: WaveHoltz to 1e-6 on held-out problems
+(64 x 64 and 128 x 128 quads, P = 4, seeds never used in training), batched,
+with and without the learned warm start.  The warm-started solve must
+converge to the same fixed point (the plain iteration judges the residual) in
+fewer updates."""
 from __future__ import annotations
 
 import os
@@ -14,14 +14,16 @@ from wavedg_amd import warmstart
 
 pytestmark = pytest.mark.gpu
 
-WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "wavedg_amd", "weights",
-                       "warmstart_m64_p4.npz")
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "wavedg_amd", "weights")
 
 
-def test_warm_start_reduces_iterations(cuda):
+@pytest.mark.parametrize("M,count", [(64, 8), (128, 4)])
+def test_warm_start_reduces_iterations(cuda, M, count):
     torch = cuda
-    ws = warmstart.WarmStart.load(WEIGHTS)
-    specs = warmstart.specs_for(range(warmstart.EVAL_SEED0 + 500, warmstart.EVAL_SEED0 + 508))
+    ws = warmstart.WarmStart.load(os.path.join(WEIGHTS, f"warmstart_m{M}_p4.npz"))
+    assert ws.M == M
+    print(f"warm start {M}^2: {count} held-out problems", flush=True)
+    specs = warmstart.specs_for(range(warmstart.EVAL_SEED0 + 500, warmstart.EVAL_SEED0 + 500 + count))
     tol = 1e-6
     print("warm start: plain batched solve", flush=True)
     Wp, itp, _, resp, _, tp = warmstart.solve_batched(ws, specs, tol, 300, warm=False)

@@ -127,6 +127,7 @@ struct wdg_wave2d_s {
     float* d_basis32 = nullptr;
     int32_t* d_nbr = nullptr;
     int32_t* d_info = nullptr;
+    std::map<int, double*> d_lftab;  // LinearFunctional tables per rule size (wdg_hip_linear_functional)
     void* d_fgeo = nullptr;      // FaceGeom<R> per element side (handle precision)
     wdg::FaceNb* d_fnb = nullptr;       // neighbour trace base and flags per element side
     double* d_basis = nullptr;
@@ -177,6 +178,7 @@ struct wdg_wave2d_s {
         dfree(d_info);
         dfree(d_fgeo);
         dfree(d_fnb);
+        for (auto& kv : d_lftab) dfree(kv.second);
         dfree(d_basis);
         dfree(d_lvtab);
         dfree(d_extab);
@@ -1653,20 +1655,27 @@ int wdg_hip_linear_functional(wdg_wave2d op, int n_var, int nq, const double* d_
         if (n_var < 1 || n_var > wdg::DivFArgs::kMaxVar) fail(WDG_ERR_INVALID, "n_var out of range 1..8");
         if (nq <= 0) nq = op->n;  // the reference's default rule: Gauss-Legendre of n_colloc points
         if (nq > 10) fail(WDG_ERR_INVALID, "nq out of range 1..10");  // the kernel's tables hold 10 points
-        std::vector<double> xq, wq;
-        if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
-        std::vector<double> tab(2 * (size_t)nq + (size_t)nq * op->n);
-        std::copy(xq.begin(), xq.end(), tab.begin());
-        std::copy(wq.begin(), wq.end(), tab.begin() + nq);
-        wdg::lagrange_interpolation_matrix(op->n, op->x.data(), nq, xq.data(), tab.data() + 2 * nq);
+        auto it = op->d_lftab.find(nq);
+        if (it == op->d_lftab.end()) {
+            // the rule's table, built and uploaded (synchronously) on the first call with this nq and
+            // kept on the handle, so later calls only enqueue the kernel
+            std::vector<double> xq, wq;
+            if (!wdg::gauss_legendre_rule(nq, xq, wq)) fail(WDG_ERR_INVALID, "nq out of range 1..10");
+            std::vector<double> tab(2 * (size_t)nq + (size_t)nq * op->n);
+            std::copy(xq.begin(), xq.end(), tab.begin());
+            std::copy(wq.begin(), wq.end(), tab.begin() + nq);
+            wdg::lagrange_interpolation_matrix(op->n, op->x.data(), nq, xq.data(), tab.data() + 2 * nq);
+            double* d = static_cast<double*>(dmalloc(sizeof(double) * tab.size()));
+            const hipError_t e = hipMemcpy(d, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                dfree(d);
+                hip_check(e, "upload quadrature table");
+            }
+            it = op->d_lftab.emplace(nq, d).first;
+        }
         const hipStream_t s = as_stream(stream);
-        DevBuf d_tab(sizeof(double) * tab.size());
-        hip_check(hipMemcpyAsync(d_tab.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s),
-                  "upload quadrature table");
-        wdg::launch_linear_functional(op->n, nq, n_var, op->d_corners, static_cast<const double*>(d_tab.p), d_fq, d_F,
-                                      op->nel, s);
+        wdg::launch_linear_functional(op->n, nq, n_var, op->d_corners, it->second, d_fq, d_F, op->nel, s);
         hip_check(hipGetLastError(), "linear functional launch");
-        hip_check(hipStreamSynchronize(s), "linear functional");  // the table is freed on return
     });
 }
 

@@ -588,30 +588,33 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     // RK register / output / forcing / accumulator values, loaded early
     R pre0 = 0, pre1 = 0, pre2 = 0, fv0 = 0, fv1 = 0, fv2 = 0;
     double ac0 = 0, ac1 = 0, ac2 = 0;
+    // per-lane pointers are formed once (uniform base + zero-extended 32-bit
+    // offset), so the three variables sit at immediate offsets
     auto prefetch = [&](bool act, size_t gb, uint32_t g) {
         if (!act) return;
+        const size_t o = gb + (size_t)g;
         if constexpr (READ_DU) {
-            const R* du = static_cast<const R*>(a.du) + gb;
-            pre0 = du[g + 0];
-            pre1 = du[g + 1];
-            pre2 = du[g + 2];
+            const R* du = static_cast<const R*>(a.du) + o;
+            pre0 = du[0];
+            pre1 = du[1];
+            pre2 = du[2];
         } else if constexpr (OUT == M_OUT_ACCUM) {
-            const R* y = static_cast<const R*>(a.u_out) + gb;
-            pre0 = y[g + 0];
-            pre1 = y[g + 1];
-            pre2 = y[g + 2];
+            const R* y = static_cast<const R*>(a.u_out) + o;
+            pre0 = y[0];
+            pre1 = y[1];
+            pre2 = y[2];
         }
         if constexpr (FORCE) {
-            const R* Fv = static_cast<const R*>(a.force) + gb;
-            fv0 = Fv[g + 0];
-            fv1 = Fv[g + 1];
-            fv2 = Fv[g + 2];
+            const R* Fv = static_cast<const R*>(a.force) + o;
+            fv0 = Fv[0];
+            fv1 = Fv[1];
+            fv2 = Fv[2];
         }
         if constexpr (ACC) {
-            const double* ac = a.acc + gb;
-            ac0 = ac[g + 0];
-            ac1 = ac[g + 1];
-            ac2 = ac[g + 2];
+            const double* ac = a.acc + o;
+            ac0 = ac[0];
+            ac1 = ac[1];
+            ac2 = ac[2];
         }
     };
     int el;
@@ -630,12 +633,13 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                                                          a.basis, (R)a.ha, (R)a.hb, e0, nloc, lds, el, gb, g, active,
                                                          prefetch);
     if (!active) return;
+    const size_t o = gb + (size_t)g;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
-        R* y = static_cast<R*>(a.u_out) + gb;
-        y[g + 0] = pre0 + r0;
-        y[g + 1] = pre1 + r1;
-        y[g + 2] = pre2 + r2;
+        R* y = static_cast<R*>(a.u_out) + o;
+        y[0] = pre0 + r0;
+        y[1] = pre1 + r1;
+        y[2] = pre2 + r2;
     } else {
         const R im = recip(q.m);
         if constexpr (FORCE) {
@@ -648,14 +652,14 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
         r1 *= im;
         r2 *= im;
         if constexpr (OUT == M_OUT_RHS) {
-            R* y = static_cast<R*>(a.u_out) + gb;
-            y[g + 0] = r0;
-            y[g + 1] = r1;
-            y[g + 2] = r2;
+            R* y = static_cast<R*>(a.u_out) + o;
+            y[0] = r0;
+            y[1] = r1;
+            y[2] = r2;
         } else {
             // low-storage RK stage (ode.hpp:74-90)
-            R* du = static_cast<R*>(a.du) + gb;
-            R* uo = static_cast<R*>(a.u_out) + gb;
+            R* du = static_cast<R*>(a.du) + o;
+            R* uo = static_cast<R*>(a.u_out) + o;
             const R dt = F32 ? (R)a.dt32 : (R)a.dt, rb = F32 ? (R)a.rk_b32 : (R)a.rk_b;
             R d0, d1, d2;
             if constexpr (OUT == M_OUT_STAGE0) {
@@ -668,21 +672,21 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 d1 = ra * pre1 + dt * r1;
                 d2 = ra * pre2 + dt * r2;
             }
-            st_stream(du + g + 0, d0);
-            st_stream(du + g + 1, d1);
-            st_stream(du + g + 2, d2);
+            st_stream(du + 0, d0);
+            st_stream(du + 1, d1);
+            st_stream(du + 2, d2);
             R w0, w1, w2;
             stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
-            st_stream(uo + g + 0, w0);
-            st_stream(uo + g + 1, w1);
-            st_stream(uo + g + 2, w2);
+            st_stream(uo + 0, w0);
+            st_stream(uo + 1, w1);
+            st_stream(uo + 2, w2);
             if constexpr (ACC) {
                 // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
                 const double acw = a.ac;
-                double* ac = a.acc + gb;
-                ac[g + 0] = ac0 + acw * (double)w0;
-                ac[g + 1] = ac1 + acw * (double)w1;
-                ac[g + 2] = ac2 + acw * (double)w2;
+                double* ac = a.acc + o;
+                ac[0] = ac0 + acw * (double)w0;
+                ac[1] = ac1 + acw * (double)w1;
+                ac[2] = ac2 + acw * (double)w2;
             }
         }
     }

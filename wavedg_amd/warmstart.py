@@ -30,6 +30,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
@@ -178,6 +179,79 @@ def generate(ws: WarmStart, seeds, tol: float, max_iter: int, precision: int = c
     return X, T, dict(seconds=secs, iterations=its, residuals=res)
 
 
+GROUP_SEED0 = 70000  # training groups GROUP_SEED0 + g
+GROUP_SOURCES = 5
+GROUP_MAX_SUBSET = 4  # config-5 problems have 1-4 sources
+
+
+def group_spec(seed: int):
+    """One training group: omega uniform in [5 pi, 20 pi] and GROUP_SOURCES
+    Gaussian sources (centres uniform in [0.1, 0.9]^2, sigma uniform in
+    [0.005, 0.03]) with unit amplitude, from SplitMix64(seed)."""
+    rng = meshgen.SplitMix64(seed)
+    omega = float(rng.uniform(1, OMEGA_LO, OMEGA_HI)[0])
+    c = rng.uniform(2 * GROUP_SOURCES, 0.1, 0.9).reshape(GROUP_SOURCES, 2)
+    sig = rng.uniform(GROUP_SOURCES, 0.005, 0.03)
+    return omega, np.column_stack([c, sig, np.ones(GROUP_SOURCES)])
+
+
+def group_examples() -> int:
+    """Training examples per group: source subsets of 1..GROUP_MAX_SUBSET of the GROUP_SOURCES."""
+    return sum(math.comb(GROUP_SOURCES, k) for k in range(1, GROUP_MAX_SUBSET + 1))
+
+
+def generate_groups(ws: WarmStart, groups, tol: float, max_iter: int, precision: int = capi.F32, chunk: int = 64,
+                    steps_per_graph: int = 16, seed: int = 0, log=print):
+    """Training data by superposition: the WaveHoltz solution is linear in the
+    forcing, so for every group (one omega) the GROUP_SOURCES single-source
+    problems are solved with the batched solver and each source subset of
+    1..GROUP_MAX_SUBSET sources, with N(0, 1) amplitudes, gives an exact
+    training example sum_k a_k W_k at no further solver cost (30 examples from
+    5 solves).  Returns input images, target images and solver statistics."""
+    torch = ws.torch
+    groups = list(groups)
+    subsets = [[k for k in range(GROUP_SOURCES) if (m >> k) & 1] for m in range(1, 1 << GROUP_SOURCES)]
+    subsets = [sub for sub in subsets if len(sub) <= GROUP_MAX_SUBSET]
+    N = len(groups) * len(subsets)
+    X = torch.empty((N, ws.M, ws.M, IN_LD), dtype=torch.bfloat16, device="cuda")
+    T = torch.empty((N, ws.M, ws.M, ws.out_ld), dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(seed)
+    its, res, secs, row = [], [], 0.0, 0
+    per = max(1, chunk // GROUP_SOURCES)
+    for g0 in range(0, len(groups), per):
+        gspec = [group_spec(int(g)) for g in groups[g0:g0 + per]]
+        probs = [ws.problem(dict(omega=om, sources=src[k:k + 1])) for om, src in gspec for k in range(GROUP_SOURCES)]
+        b = capi.Batch(probs, precision)
+        sec, _ = b.solve(tol, max_iter, steps_per_graph)
+        secs += sec
+        Wk = []
+        for k in range(len(probs)):
+            W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+            it, r = b.result(k, W)
+            its.append(it)
+            res.append(r)
+            Wk.append(W)
+        del b
+        specs, Ws = [], []
+        for gi, (om, src) in enumerate(gspec):
+            for sub in subsets:
+                amp = rng.standard_normal(len(sub))
+                W = torch.zeros(ws.ndof, dtype=torch.float64, device="cuda")
+                for a_k, k in zip(amp, sub):
+                    W.add_(Wk[gi * GROUP_SOURCES + k], alpha=float(a_k))
+                s = src[sub].copy()
+                s[:, 3] = amp
+                specs.append(dict(omega=om, sources=s))
+                Ws.append(W)
+        ws.features(specs, X[row:row + len(specs)])
+        ws.targets(Ws, specs, T[row:row + len(specs)])
+        row += len(specs)
+        del Wk, Ws
+        log(f"generated {min(g0 + per, len(groups))}/{len(groups)} groups ({len(its)} solves, {row} examples), "
+            f"{secs:.1f} s solver time, mean iterations {np.mean(its):.1f}")
+    return X, T, dict(seconds=secs, iterations=its, residuals=res, solves=len(its), examples=N)
+
+
 def set_target_gain(ws: WarmStart, T, n_real: int) -> float:
     """Rescales the target images to unit RMS over their real channels and
     records the factor in ws (it scales predictions back)."""
@@ -273,15 +347,11 @@ def solve_batched(ws: WarmStart, specs, tol: float, max_iter: int, warm: bool = 
     return Ws, it1, it2, res, t_inf, time.perf_counter() - t0
 
 
-def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: int = 16, sequential: bool = False,
-             log=print, plain_cache: dict | None = None) -> dict:
-    """Iterations and time to tol with and without the warm start (both
-    batched), plus optionally the plain per-problem GPU solves.  plain_cache
-    (a dict) keeps the plain batched results between calls on the same seeds."""
+def _evaluate_chunk(ws: WarmStart, specs, tol: float, max_iter: int, steps_per_graph: int, sequential: bool) -> dict:
+    """Raw per-problem figures of one chunk of evaluate()."""
     torch = ws.torch
-    specs = specs_for(seeds)
     probs = [ws.problem(s) for s in specs]
-    out = {"problems": len(specs), "M": ws.M, "P": ws.n - 1, "tol": tol}
+    r = {}
     if sequential:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -297,74 +367,98 @@ def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: i
             it, _, _ = wh.solve(W, pi0, capi.norm(pi0), tol, max_iter)
             seq_it.append(it)
         torch.cuda.synchronize()
-        out["sequential_plain_seconds"] = time.perf_counter() - t0
-        out["sequential_plain_iterations"] = seq_it
-    if plain_cache is not None and plain_cache.get("key") == (tuple(seeds), tol, max_iter):
-        plain_W, plain_it, plain_res = plain_cache["W"], plain_cache["it"], plain_cache["res"]
-        sec_plain, wall_plain = plain_cache["sec"], plain_cache["wall"]
-    else:
-        b = capi.Batch(probs)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sec_plain, _ = b.solve(tol, max_iter, steps_per_graph)
-        wall_plain = time.perf_counter() - t0
-        plain_W, plain_it, plain_res = [], [], []
-        for k in range(len(probs)):
-            W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
-            it, r = b.result(k, W)
-            plain_W.append(W)
-            plain_it.append(it)
-            plain_res.append(r)
-        del b
-        if plain_cache is not None:
-            plain_cache.update(key=(tuple(seeds), tol, max_iter), W=plain_W, it=plain_it, res=plain_res,
-                               sec=sec_plain, wall=wall_plain)
+        r["seq_seconds"] = time.perf_counter() - t0
+        r["seq_it"] = seq_it
+    b = capi.Batch(probs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r["plain_gpu"], _ = b.solve(tol, max_iter, steps_per_graph)
+    r["plain_wall"] = time.perf_counter() - t0
+    plain_W, r["plain_it"], r["plain_res"] = [], [], []
+    for k in range(len(probs)):
+        W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
+        it, res = b.result(k, W)
+        plain_W.append(W)
+        r["plain_it"].append(it)
+        r["plain_res"].append(res)
+    del b
     b = capi.Batch(probs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     Wp = ws.predict(specs)
     torch.cuda.synchronize()
-    t_inf = time.perf_counter() - t0
+    r["inf"] = time.perf_counter() - t0
     for k, W in enumerate(Wp):
         b.set_initial(k, W)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sec_warm, _ = b.solve(tol, max_iter, steps_per_graph)
-    wall_warm = time.perf_counter() - t0
-    warm_it, warm_res, pred_err, final_diff = [], [], [], []
+    r["warm_gpu"], _ = b.solve(tol, max_iter, steps_per_graph)
+    r["warm_wall"] = time.perf_counter() - t0
+    r["warm_it"], r["warm_res"], r["pred_err"], r["final_diff"] = [], [], [], []
     for k in range(len(probs)):
         W = torch.empty(ws.ndof, dtype=torch.float64, device="cuda")
-        it, r = b.result(k, W)
-        warm_it.append(it)
-        warm_res.append(r)
+        it, res = b.result(k, W)
+        r["warm_it"].append(it)
+        r["warm_res"].append(res)
         pn = float(torch.linalg.norm(plain_W[k]))
-        pred_err.append(float(torch.linalg.norm(Wp[k] - plain_W[k])) / pn)
-        final_diff.append(float(torch.linalg.norm(W - plain_W[k])) / pn)
-    out.update({
-        "plain_iterations_mean": float(np.mean(plain_it)), "warm_iterations_mean": float(np.mean(warm_it)),
-        "plain_iterations": plain_it, "warm_iterations": warm_it,
-        "plain_batched_seconds": wall_plain, "warm_batched_seconds": wall_warm, "inference_seconds": t_inf,
-        "plain_batched_gpu_seconds": sec_plain, "warm_batched_gpu_seconds": sec_warm,
-        "iteration_gain": float(np.mean(plain_it) / np.mean(warm_it)),
-        "time_gain_batched": wall_plain / (wall_warm + t_inf),
-        "prediction_rel_error_median": float(np.median(pred_err)),
-        "prediction_rel_error_mean": float(np.mean(pred_err)),
-        "warm_vs_plain_solution_max_rel_diff": float(np.max(final_diff)),
-        "plain_max_residual": float(np.max(plain_res)), "warm_max_residual": float(np.max(warm_res)),
-    })
+        r["pred_err"].append(float(torch.linalg.norm(Wp[k] - plain_W[k])) / pn)
+        r["final_diff"].append(float(torch.linalg.norm(W - plain_W[k])) / pn)
+    del b
     Wm, i32, i64, rm, t_inf_m, wall_m = solve_batched(ws, specs, tol, max_iter, warm=True, mixed=True,
                                                       steps_per_graph=steps_per_graph)
-    out.update({
-        "warm_mixed_iterations_f32_mean": float(np.mean(i32)), "warm_mixed_iterations_f64_mean": float(np.mean(i64)),
-        "warm_mixed_batched_seconds": wall_m + t_inf_m, "warm_mixed_max_residual": float(np.max(rm)),
-        "warm_mixed_vs_plain_solution_max_rel_diff": float(max(
-            float(torch.linalg.norm(Wm[k] - plain_W[k]) / torch.linalg.norm(plain_W[k])) for k in range(len(specs)))),
-    })
-    out["time_gain_batched_mixed"] = wall_plain / (wall_m + t_inf_m)
+    r.update(mixed_i32=i32, mixed_i64=i64, mixed_res=rm, mixed_wall=wall_m + t_inf_m,
+             mixed_diff=[float(torch.linalg.norm(Wm[k] - plain_W[k]) / torch.linalg.norm(plain_W[k]))
+                         for k in range(len(specs))])
+    return r
+
+
+def evaluate(ws: WarmStart, seeds, tol: float, max_iter: int, steps_per_graph: int = 16, sequential: bool = False,
+             log=print, chunk: int = 16) -> dict:
+    """Iterations and time to tol with and without the warm start (both
+    batched), the warm start with the FP32-then-FP64 batched solve, and
+    optionally the plain per-problem GPU solves (FP64, each fixed-point step
+    replayed from a HIP graph).  Runs in chunks of `chunk` problems (each
+    chunk's batches are timed separately and the times added) and logs after
+    every chunk."""
+    seeds = list(seeds)
+    acc: dict = {}
+    for c0 in range(0, len(seeds), chunk):
+        r = _evaluate_chunk(ws, specs_for(seeds[c0:c0 + chunk]), tol, max_iter, steps_per_graph, sequential)
+        for k, v in r.items():
+            if isinstance(v, list):
+                acc.setdefault(k, []).extend(v)
+            else:
+                acc[k] = acc.get(k, 0.0) + v
+        log(f"evaluated {min(c0 + chunk, len(seeds))}/{len(seeds)}: plain batched {acc['plain_wall']:.1f} s, "
+            f"warm {acc['warm_wall']:.1f} s, warm mixed {acc['mixed_wall']:.1f} s")
+    out = {"problems": len(seeds), "M": ws.M, "P": ws.n - 1, "tol": tol, "chunk": chunk}
     if sequential:
-        out["speedup_batched_warm_vs_sequential_plain"] = out["sequential_plain_seconds"] / (wall_warm + t_inf)
-        out["speedup_batched_plain_vs_sequential_plain"] = out["sequential_plain_seconds"] / wall_plain
-        out["speedup_batched_warm_mixed_vs_sequential_plain"] = out["sequential_plain_seconds"] / (wall_m + t_inf_m)
+        out["sequential_plain_seconds"] = acc["seq_seconds"]
+        out["sequential_plain_iterations"] = acc["seq_it"]
+    wall_plain, wall_warm, t_inf, wall_m = acc["plain_wall"], acc["warm_wall"], acc["inf"], acc["mixed_wall"]
+    out.update({
+        "plain_iterations_mean": float(np.mean(acc["plain_it"])), "warm_iterations_mean": float(np.mean(acc["warm_it"])),
+        "plain_iterations": acc["plain_it"], "warm_iterations": acc["warm_it"],
+        "plain_batched_seconds": wall_plain, "warm_batched_seconds": wall_warm, "inference_seconds": t_inf,
+        "plain_batched_gpu_seconds": acc["plain_gpu"], "warm_batched_gpu_seconds": acc["warm_gpu"],
+        "iteration_gain": float(np.mean(acc["plain_it"]) / np.mean(acc["warm_it"])),
+        "time_gain_batched": wall_plain / (wall_warm + t_inf),
+        "prediction_rel_error_median": float(np.median(acc["pred_err"])),
+        "prediction_rel_error_mean": float(np.mean(acc["pred_err"])),
+        "prediction_rel_error_p90": float(np.percentile(acc["pred_err"], 90)),
+        "warm_vs_plain_solution_max_rel_diff": float(np.max(acc["final_diff"])),
+        "plain_max_residual": float(np.max(acc["plain_res"])), "warm_max_residual": float(np.max(acc["warm_res"])),
+        "warm_mixed_iterations_f32_mean": float(np.mean(acc["mixed_i32"])),
+        "warm_mixed_iterations_f64_mean": float(np.mean(acc["mixed_i64"])),
+        "warm_mixed_batched_seconds": wall_m, "warm_mixed_max_residual": float(np.max(acc["mixed_res"])),
+        "warm_mixed_vs_plain_solution_max_rel_diff": float(np.max(acc["mixed_diff"])),
+        "time_gain_batched_mixed": wall_plain / wall_m,
+    })
+    if sequential:
+        seq = acc["seq_seconds"]
+        out["speedup_batched_plain_vs_sequential_plain"] = seq / wall_plain
+        out["speedup_batched_warm_vs_sequential_plain"] = seq / (wall_warm + t_inf)
+        out["speedup_batched_warm_mixed_vs_sequential_plain"] = seq / wall_m
     log(json.dumps({k: v for k, v in out.items() if not isinstance(v, list)}))
     return out
 
@@ -375,6 +469,11 @@ def main(argv=None) -> int:
     ap.add_argument("--M", type=int, default=64)
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--n-train", type=int, default=1024)
+    ap.add_argument("--groups", type=int, default=0,
+                    help="train on superposed single-source solves of this many omega groups "
+                         f"({GROUP_SOURCES} solves and {group_examples()} examples each) instead of --n-train")
+    ap.add_argument("--init-weights", default=None, help="start training from these weights (same architecture)")
+    ap.add_argument("--val-groups", type=int, default=8)
     ap.add_argument("--n-val", type=int, default=64)
     ap.add_argument("--n-eval", type=int, default=128)
     ap.add_argument("--gen-tol", type=float, default=1e-5)
@@ -389,6 +488,7 @@ def main(argv=None) -> int:
     ap.add_argument("--weights", default=None)
     ap.add_argument("--out", default=None)
     ap.add_argument("--sequential", action="store_true", help="also time plain per-problem solves")
+    ap.add_argument("--chunk", type=int, default=16, help="evaluation problems per timed batch")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -396,17 +496,31 @@ def main(argv=None) -> int:
 
     if a.mode == "train":
         ws = WarmStart(a.M, a.base, max_batch=max(a.batch, 16))
-        ws.net.set_params(init_params(a.base, seed=a.init_seed))
+        if a.init_weights:
+            z = np.load(a.init_weights, allow_pickle=False)
+            if int(z["base"]) != a.base or int(z["n"]) != N_COLLOC:
+                ap.error("--init-weights: a network of another width or order")
+            ws.net.set_params(z["params"])
+        else:
+            ws.net.set_params(init_params(a.base, seed=a.init_seed))
         prec = capi.F64 if a.gen_precision == "f64" else capi.F32
-        X, T, st = generate(ws, range(TRAIN_SEED0, TRAIN_SEED0 + a.n_train + a.n_val), a.gen_tol, a.gen_max_iter,
-                            prec, log=log)
+        if a.groups > 0:
+            X, T, st = generate_groups(ws, range(GROUP_SEED0, GROUP_SEED0 + a.groups + a.val_groups), a.gen_tol,
+                                       a.gen_max_iter, prec, log=log)
+            a.n_train = a.groups * group_examples()
+            a.n_val = a.val_groups * group_examples()
+        else:
+            X, T, st = generate(ws, range(TRAIN_SEED0, TRAIN_SEED0 + a.n_train + a.n_val), a.gen_tol, a.gen_max_iter,
+                                prec, log=log)
         set_target_gain(ws, T[:a.n_train], 3 * N_COLLOC ** 2)
         T[a.n_train:].mul_(ws.tgt_gain)
         Xt, Tt, Xv, Tv = X[:a.n_train], T[:a.n_train], X[a.n_train:], T[a.n_train:]
         v0 = validation_loss(ws, Xv, Tv)
         losses, t_train = train(ws, Xt, Tt, a.steps, a.batch, a.lr, log=log)
         v1 = validation_loss(ws, Xv, Tv)
-        meta = dict(M=a.M, base=a.base, n_train=a.n_train, n_val=a.n_val, gen_tol=a.gen_tol,
+        meta = dict(M=a.M, base=a.base, n_train=a.n_train, n_val=a.n_val, groups=a.groups,
+                    init_weights=os.path.basename(a.init_weights) if a.init_weights else None,
+                    solves=int(st.get("solves", len(st["iterations"]))), gen_tol=a.gen_tol,
                     gen_precision=a.gen_precision, tgt_gain=ws.tgt_gain,
                     gen_seconds=st["seconds"], gen_iterations_mean=float(np.mean(st["iterations"])),
                     steps=a.steps, batch=a.batch, lr=a.lr, train_seconds=t_train, val_loss_init=v0,
@@ -415,12 +529,14 @@ def main(argv=None) -> int:
         if a.out:
             ws.save(a.out, meta)
         if a.n_eval > 0:
-            evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log)
+            evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log,
+                     chunk=a.chunk)
         return 0
     if not a.weights:
         ap.error("eval needs --weights")
     ws = WarmStart.load(a.weights)
-    evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log)
+    evaluate(ws, range(EVAL_SEED0, EVAL_SEED0 + a.n_eval), a.tol, a.max_iter, sequential=a.sequential, log=log,
+             chunk=a.chunk)
     return 0
 
 
