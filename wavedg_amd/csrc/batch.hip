@@ -557,7 +557,6 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         std::vector<BatchBlock> blocks;
         std::vector<double> basis_all, geom, ctab, ftab;
         std::vector<int32_t> nbr, info;
-        std::vector<wdg::FaceGeom<double>> fg64;
         std::vector<wdg::FaceGeom<float>> fg32;
         std::vector<wdg::FaceNb> fnb;
         std::vector<long long> basis_off(10, -1);
@@ -643,9 +642,8 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
                     wdg::build_face_tables(n, nel, gk, ft.nbr.data(), ft.info.data(), gq, fk);
                     fg32.insert(fg32.end(), gq.begin(), gq.end());
                 } else {
-                    std::vector<wdg::FaceGeom<double>> gq;
+                    std::vector<wdg::FaceGeom<double>> gq;  // unused: FP64 kernels compute normals
                     wdg::build_face_tables(n, nel, gk, ft.nbr.data(), ft.info.data(), gq, fk);
-                    fg64.insert(fg64.end(), gq.begin(), gq.end());
                 }
                 fnb.insert(fnb.end(), fk.begin(), fk.end());
             }
@@ -680,10 +678,7 @@ int wdg_hip_batch_create(int n_problems, const wdg_batch_problem* problems, int 
         if (precision == WDG_F32) {
             b->d_fgeo = dalloc<wdg::FaceGeom<float>>(fg32.size());
             bcheck(hipMemcpy(b->d_fgeo, fg32.data(), sizeof(fg32[0]) * fg32.size(), hipMemcpyHostToDevice), "upload");
-        } else {
-            b->d_fgeo = dalloc<wdg::FaceGeom<double>>(fg64.size());
-            bcheck(hipMemcpy(b->d_fgeo, fg64.data(), sizeof(fg64[0]) * fg64.size(), hipMemcpyHostToDevice), "upload");
-        }
+        }  // FP64 kernels compute side normals from the map coefficients (no face-geometry table)
         b->d_fnb = dalloc<wdg::FaceNb>(fnb.size());
         bcheck(hipMemcpy(b->d_fnb, fnb.data(), sizeof(wdg::FaceNb) * fnb.size(), hipMemcpyHostToDevice), "upload");
         bcheck(hipMemcpy(b->d_ctab, ctab.data(), sizeof(double) * ctab.size(), hipMemcpyHostToDevice), "upload");

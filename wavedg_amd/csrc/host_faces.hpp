@@ -11,6 +11,7 @@ namespace wdg {
 
 // Per element side, in the handle's precision: outward unit normal of the
 // side and half its length (ds of the straight side, Edge.hpp:107-122).
+// The kernels read it for FP32 handles of n_colloc >= 6.
 template <typename R>
 struct FaceGeom {
     R n0, n1, hl;

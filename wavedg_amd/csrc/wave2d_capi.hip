@@ -217,6 +217,8 @@ struct wdg_wave2d_s {
         a.geom32 = d_geom32;
         a.basis32 = d_basis32;
         a.leg = leg;
+        // u, u' and du of a stage (3 state arrays) beyond the 256 MB Infinity Cache
+        a.nt_store = 3.0 * (double)ndof * (double)rsz > 256.0 * 1024 * 1024 ? 1 : 0;
         a.ha = ha;
         a.hb = hb;
         a.nel = nel;
@@ -874,11 +876,9 @@ static int create_collocated(wdg_mesh mesh, int n_colloc, int basis_type, const 
                 hip_check(hipMemcpy(op->d_fgeo, fg.data(), sizeof(fg[0]) * fg.size(), hipMemcpyHostToDevice),
                           "upload face geometry");
             } else {
+                // FP64 kernels compute side normals from the map coefficients: only the neighbour table
                 std::vector<wdg::FaceGeom<double>> fg;
                 wdg::build_face_tables(n, op->nel, geom.data(), ft.nbr.data(), ft.info.data(), fg, fnb);
-                op->d_fgeo = dmalloc(sizeof(fg[0]) * fg.size());
-                hip_check(hipMemcpy(op->d_fgeo, fg.data(), sizeof(fg[0]) * fg.size(), hipMemcpyHostToDevice),
-                          "upload face geometry");
             }
             op->d_fnb = static_cast<wdg::FaceNb*>(dmalloc(sizeof(wdg::FaceNb) * fnb.size()));
             hip_check(hipMemcpy(op->d_fnb, fnb.data(), sizeof(wdg::FaceNb) * fnb.size(), hipMemcpyHostToDevice),

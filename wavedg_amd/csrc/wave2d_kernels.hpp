@@ -54,6 +54,7 @@ struct KArgs {
     const FaceNb* fnb;      // (4, nel) neighbour trace base and flags (build_face_tables)
     float ha32, hb32, dt32, rk_a32, rk_b32, fs32;  // the scalars above rounded to float (FP32 handles)
     uint32_t sr_seed;       // FP32 stages: seed of the state update's stochastic rounding (sr_seed())
+    int nt_store;           // 1: stage outputs as non-temporal stores (arrays beyond the Infinity Cache)
 };
 
 // Stochastic rounding of the FP32 state update u + b du.  Rounding to nearest
@@ -112,6 +113,7 @@ enum : int {
     M_FORCE = 1 << 8,
     M_ACC = 1 << 9,
     M_LEG = 1 << 10,  // Gauss-Legendre basis
+    M_NT = 1 << 11,   // stage outputs as non-temporal stores (KArgs::nt_store)
 };
 
 // Entry points (dispatch ids) and their mode bits.
@@ -148,20 +150,19 @@ constexpr int kModeOf[K_COUNT] = {
 #ifndef WDG_MIN_WAVES_F32
 #define WDG_MIN_WAVES_F32 WDG_MIN_WAVES  // the same for the FP32 stage kernels
 #endif
-#ifndef WDG_NT_STORES
-#define WDG_NT_STORES 0  // 1: stream stage outputs with non-temporal stores
-#endif
-
-// Store that bypasses cache allocation for data no kernel reads before the
-// next launch.
-template <typename T>
+// Stage outputs u' and du, which no workgroup of the same launch reads.
+// When the stage's arrays are larger than the 256 MB Infinity Cache the
+// M_NT instantiations send them as non-temporal stores, streaming to HBM
+// without displacing the neighbour traces (2-5 % faster at 512^2, P = 7);
+// arrays that fit stay cached for the next stage (non-temporal stores there
+// cost up to 25 %).
+template <bool NT, typename T>
 __device__ __forceinline__ void st_stream(T* p, T v)
 {
-#if WDG_NT_STORES
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
 }
 
 // Workgroup schedule: about TGT threads, i.e. EPB = TGT / n^2 whole elements
@@ -345,10 +346,12 @@ __device__ __forceinline__ NodeResult<R> apply_operator(const R* __restrict__ u_
     // with the side's table entries and the neighbour's trace.
     R nbv[FP][3];
     int finfo[FP], fnb3[FP];
-    // Orders N >= 6 read each side's normal and half length from the fgeo
-    // table; lower orders, with more elements per DoF, compute them from the
-    // map coefficients instead of spending HBM bytes on the table.
-    constexpr bool FTAB = N >= 6;
+    // FP32 kernels of orders N >= 6 read each side's normal and half length
+    // from the fgeo table.  Lower orders, with more elements per DoF, and the
+    // FP64 kernels, which are bound by memory rather than by VALU issue,
+    // compute them from the map coefficients instead of spending HBM bytes
+    // on the table (3 % faster at P = 7 in FP64).
+    constexpr bool FTAB = N >= 6 && sizeof(R) == 4;
     FaceGeom<R> fgv[FP];
     if constexpr (FACE) {
 #pragma unroll
@@ -572,6 +575,7 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     constexpr bool ACC = (MODE & M_ACC) != 0;
     constexpr bool READ_DU = OUT == M_OUT_STAGE;
     constexpr bool LEG = (MODE & M_LEG) != 0;
+    constexpr bool NT = (MODE & M_NT) != 0;
     using L = OpLds<N, R, VOL, IFACE || BFACE, TGT, LEG>;
     __shared__ __align__(16) unsigned char lds[L::bytes];
 
@@ -672,14 +676,14 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 d1 = ra * pre1 + dt * r1;
                 d2 = ra * pre2 + dt * r2;
             }
-            st_stream(du + 0, d0);
-            st_stream(du + 1, d1);
-            st_stream(du + 2, d2);
+            st_stream<NT>(du + 0, d0);
+            st_stream<NT>(du + 1, d1);
+            st_stream<NT>(du + 2, d2);
             R w0, w1, w2;
             stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
-            st_stream(uo + 0, w0);
-            st_stream(uo + 1, w1);
-            st_stream(uo + 2, w2);
+            st_stream<NT>(uo + 0, w0);
+            st_stream<NT>(uo + 1, w1);
+            st_stream<NT>(uo + 2, w2);
             if constexpr (ACC) {
                 // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
                 const double acw = a.ac;
