@@ -103,6 +103,14 @@ int wdg_hip_gll(int n, double *x, double *w, double *D);
  * the rule the approx_quad = false operators integrate with; x, w (n), n in 1..20
  * (tabulated up to 10 as the reference; above, Newton-refined roots of P_n). */
 int wdg_hip_gauss_legendre(int n, double *x, double *w);
+/* _ComputeFlux::flux (EdgeFlux.cpp:31-89), host only: the m x m flux matrix
+ * F = (a/2 nA + sgn b |nA|)^T of one normal matrix nA (column-major, m <= 8),
+ * sgn = +1 for side 0 and -1 for side 1, |nA| = R diag(|e|) R^{-1} from the
+ * real eigen-decomposition nA = R diag(e) R^{-1} (eig.cpp:29-74).  Fails
+ * when nA has complex eigenvalues or linearly dependent eigenvectors (a
+ * repeated eigenvalue without a full set of eigenvectors).  The advection
+ * operators build their face tables with the same routine. */
+int wdg_hip_flux_matrix(int m, const double *nA, double a, double b, int side, double *F);
 
 /* ---- wave operator ---------------------------------------------------
  * One handle holds WaveEquation (WaveEquation.cpp:5-30), WaveBC
@@ -190,9 +198,9 @@ int wdg_hip_edgefluxf2d_euler(wdg_wave2d op, double gamma, int face_type, const 
  * (n_var, n, n, n_elem), coefficient A a host array of shape (n_var, n_var, 2)
  * (constant_coefficient != 0) or (n_var, n_var, 2, n, n, n_elem), column-major,
  * A(k, l, d) the (k, l) entry of the d-direction flux matrix; upwind flux
- * (a = -1, b = -0.5 of EdgeFlux).  n.A must be symmetric at every face node
- * (the reference's general real eigen-decomposition is restated for
- * symmetric matrices only); otherwise create fails. */
+ * (a = -1, b = -0.5 of EdgeFlux).  n.A must have real eigenvalues and a
+ * full set of eigenvectors at every face node, as wdg_hip_flux_matrix
+ * requires; otherwise create fails. */
 int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double *A, int constant_coefficient, wdg_advection *out);
 /* Advection::action (Advection.hpp:127-140): d_divF += Div(u) + interior edge fluxes. */
 int wdg_hip_advection_action(wdg_advection adv, const void *d_u, void *d_divF, void *stream);

@@ -98,6 +98,7 @@ def lib():
             "orc_edgefluxf_functor": (None, [_vp, ctypes.c_int, NFLUX_FN, EXT_FN, _vp, ctypes.c_int, ctypes.c_double,
                                              _dp, _dp]),
             "orc_advection_create": (_vp, [_vp, ctypes.c_int, _dp, ctypes.c_int]),
+            "orc_flux_matrix": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
             "orc_advection_free": (None, [_vp]),
             "orc_advection_action": (None, [_vp, _dp, _dp]),
             "orc_advection_bc_action": (None, [_vp, _dp, _dp]),
@@ -462,6 +463,18 @@ def edge_flux(n0: float, n1: float, a: float, b: float, side: int) -> np.ndarray
     Fs = np.zeros(9)
     lib().orc_edge_flux(n0, n1, a, b, side, _d(Fs))
     return Fs.reshape(3, 3).T
+
+
+def flux_matrix(nA: np.ndarray, a: float, b: float, side: int):
+    """_ComputeFlux::flux (EdgeFlux.cpp:31-89) for one normal matrix: returns
+    (status, Fs) with Fs = (a/2 nA + sgn b |nA|)^T, status 0, 1 for complex
+    eigenvalues, 2 for linearly dependent eigenvectors, 3 if it fails."""
+    nA = np.asarray(nA, dtype=np.float64)
+    m = nA.shape[0]
+    out = np.zeros(m * m)
+    colmajor = np.ascontiguousarray(nA.T).reshape(-1)
+    rc = lib().orc_flux_matrix(m, _d(colmajor), a, b, side, _d(out))
+    return rc, out.reshape(m, m).T
 
 
 def norm(x) -> float:

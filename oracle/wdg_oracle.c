@@ -24,6 +24,7 @@
  */
 #include "wdg_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -1713,8 +1714,192 @@ static void sym_eig(int m, const double *A0, double *R, double *e)
     for (int i = 0; i < m; ++i) e[i] = A[i + m * i];
 }
 
-/* _ComputeFlux::flux (EdgeFlux.cpp:31-89) for m x m symmetric n.A:
- * Fs(c, d) = (a/2 nA + sgn b |nA|)(d, c).  Returns -1 if n.A is not symmetric. */
+/* Eigenvalues of a real m x m matrix (m <= 8, column-major) by the shifted
+ * QR iteration on the full matrix, A <- R Q + mu I with A - mu I = Q R
+ * (Householder), mu the Wilkinson shift of the trailing 2 x 2 block, and
+ * deflation of the last row or the last two rows once their entries left of
+ * the diagonal block are negligible.  A 2 x 2 block whose eigenvalues are
+ * complex by more than tol ends with 1; no convergence ends with 3. */
+static int qr_eigenvalues(int m, const double *A0, double tol, double *e)
+{
+    double A[64];
+    memcpy(A, A0, sizeof(double) * m * m);
+    double nrm = 0.0;
+    for (int i = 0; i < m * m; ++i) nrm += A[i] * A[i];
+    nrm = sqrt(nrm);
+    const double small = 4.0 * DBL_EPSILON * nrm;
+    int k = m, its = 0;
+    while (k > 0) {
+        if (k == 1) {
+            e[0] = A[0];
+            break;
+        }
+        double off1 = 0.0, off2 = 0.0;
+        for (int j = 0; j < k - 1; ++j) off1 += A[(k - 1) + m * j] * A[(k - 1) + m * j];
+        for (int j = 0; j < k - 2; ++j)
+            off2 += A[(k - 1) + m * j] * A[(k - 1) + m * j] + A[(k - 2) + m * j] * A[(k - 2) + m * j];
+        const double p = A[(k - 2) + m * (k - 2)], q = A[(k - 2) + m * (k - 1)];
+        const double r = A[(k - 1) + m * (k - 2)], t = A[(k - 1) + m * (k - 1)];
+        if (sqrt(off1) <= small) {
+            e[k - 1] = t;
+            --k;
+            its = 0;
+            continue;
+        }
+        const double half = 0.5 * (p + t), disc = 0.25 * (p - t) * (p - t) + q * r;
+        if (sqrt(off2) <= small) {
+            if (disc < 0.0 && sqrt(-disc) > tol) return 1;
+            const double w = disc > 0.0 ? sqrt(disc) : 0.0;
+            e[k - 2] = half + w;
+            e[k - 1] = half - w;
+            k -= 2;
+            its = 0;
+            continue;
+        }
+        if (++its > 400) return 3;
+        double mu;
+        if (its % 11 == 0) {
+            mu = t + sqrt(off1); /* exceptional shift */
+        } else if (disc >= 0.0) {
+            const double w = sqrt(disc);
+            mu = fabs(half + w - t) < fabs(half - w - t) ? half + w : half - w;
+        } else {
+            mu = half;
+        }
+        /* Householder QR of the k x k block of A - mu I: the reflectors v_c are
+         * kept in V; R overwrites B */
+        double B[64], V[64], beta[8];
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < k; ++i) B[i + m * j] = A[i + m * j] - (i == j ? mu : 0.0);
+        for (int c = 0; c < k; ++c) {
+            double tail = 0.0;
+            for (int i = c + 1; i < k; ++i) tail += B[i + m * c] * B[i + m * c];
+            beta[c] = 0.0;
+            if (tail == 0.0) continue;
+            const double x0 = B[c + m * c], nx = sqrt(x0 * x0 + tail), alpha = x0 > 0.0 ? -nx : nx;
+            for (int i = 0; i < k; ++i) V[i + m * c] = i < c ? 0.0 : (i == c ? x0 - alpha : B[i + m * c]);
+            beta[c] = 2.0 / ((x0 - alpha) * (x0 - alpha) + tail);
+            for (int j = c; j < k; ++j) {
+                double d = 0.0;
+                for (int i = c; i < k; ++i) d += V[i + m * c] * B[i + m * j];
+                d *= beta[c];
+                for (int i = c; i < k; ++i) B[i + m * j] -= d * V[i + m * c];
+            }
+        }
+        /* A <- R Q + mu I, Q = H_0 H_1 ... H_{k-1}: apply the reflectors to R's columns */
+        for (int c = 0; c < k; ++c) {
+            if (beta[c] == 0.0) continue;
+            for (int i = 0; i < k; ++i) {
+                double d = 0.0;
+                for (int j = c; j < k; ++j) d += B[i + m * j] * V[j + m * c];
+                d *= beta[c];
+                for (int j = c; j < k; ++j) B[i + m * j] -= d * V[j + m * c];
+            }
+        }
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < k; ++i) A[i + m * j] = B[i + m * j] + (i == j ? mu : 0.0);
+    }
+    return 0;
+}
+
+/* Null space of the m x m matrix M by Gaussian elimination with complete
+ * pivoting, pivots of magnitude <= tol counting as zero: writes the basis
+ * (unit columns) into N and returns its dimension. */
+static int null_space(int m, const double *M0, double tol, double *N)
+{
+    double M[64];
+    int col[8];
+    memcpy(M, M0, sizeof(double) * m * m);
+    for (int j = 0; j < m; ++j) col[j] = j;
+    int rank = 0;
+    for (; rank < m; ++rank) {
+        int pi = rank, pj = rank;
+        for (int j = rank; j < m; ++j)
+            for (int i = rank; i < m; ++i)
+                if (fabs(M[i + m * j]) > fabs(M[pi + m * pj])) {
+                    pi = i;
+                    pj = j;
+                }
+        if (fabs(M[pi + m * pj]) <= tol) break;
+        for (int j = 0; j < m; ++j) {
+            const double x = M[rank + m * j];
+            M[rank + m * j] = M[pi + m * j];
+            M[pi + m * j] = x;
+        }
+        for (int i = 0; i < m; ++i) {
+            const double x = M[i + m * rank];
+            M[i + m * rank] = M[i + m * pj];
+            M[i + m * pj] = x;
+        }
+        const int c = col[rank];
+        col[rank] = col[pj];
+        col[pj] = c;
+        for (int i = rank + 1; i < m; ++i) {
+            const double f = M[i + m * rank] / M[rank + m * rank];
+            for (int j = rank; j < m; ++j) M[i + m * j] -= f * M[rank + m * j];
+        }
+    }
+    /* free variables rank..m-1: set one to 1, back-substitute the pivots */
+    for (int f = rank; f < m; ++f) {
+        double y[8] = {0.0};
+        y[f] = 1.0;
+        for (int i = rank - 1; i >= 0; --i) {
+            double v = 0.0;
+            for (int j = i + 1; j < m; ++j) v -= M[i + m * j] * y[j];
+            y[i] = v / M[i + m * i];
+        }
+        double nn = 0.0;
+        for (int j = 0; j < m; ++j) nn += y[j] * y[j];
+        nn = sqrt(nn);
+        for (int j = 0; j < m; ++j) N[col[j] + m * (f - rank)] = y[j] / nn;
+    }
+    return m - rank;
+}
+
+/* real_eig (eig.cpp:29-74) for a general real n.A: eigenvalues e and right
+ * eigenvectors R (columns).  Eigenvalues closer than tol form one repeated
+ * eigenvalue whose eigenvectors span the null space of nA - lambda I.
+ * Returns 0, 1 for complex eigenvalues, 2 if the eigenvectors of a repeated
+ * eigenvalue are fewer than its multiplicity, 3 if the iteration fails. */
+static int real_eig_general(int m, const double *nA, double *R, double *e)
+{
+    double nrm = 0.0;
+    for (int i = 0; i < m * m; ++i) nrm += nA[i] * nA[i];
+    const double tol = 1e-7 * sqrt(nrm);
+    double w[8];
+    const int rc = qr_eigenvalues(m, nA, tol, w);
+    if (rc) return rc;
+    for (int i = 1; i < m; ++i) /* insertion sort */
+        for (int j = i; j > 0 && w[j] < w[j - 1]; --j) {
+            const double x = w[j];
+            w[j] = w[j - 1];
+            w[j - 1] = x;
+        }
+    int nv = 0;
+    for (int i = 0; i < m;) {
+        int j = i + 1;
+        while (j < m && w[j] - w[j - 1] <= tol) ++j;
+        double lam = 0.0, M[64], N[64];
+        for (int q = i; q < j; ++q) lam += w[q];
+        lam /= (j - i);
+        for (int q = 0; q < m * m; ++q) M[q] = nA[q] - (q % (m + 1) == 0 ? lam : 0.0);
+        const int dim = null_space(m, M, tol, N);
+        if (dim < j - i) return 2;
+        for (int q = 0; q < j - i; ++q) {
+            memcpy(R + m * nv, N + m * q, sizeof(double) * m);
+            e[nv++] = lam;
+        }
+        i = j;
+    }
+    return 0;
+}
+
+/* _ComputeFlux::flux (EdgeFlux.cpp:31-89): Fs(c, d) = (a/2 nA + sgn b |nA|)(d, c).
+ * Symmetric n.A: Jacobi eigenvectors, R^{-1} = R^T.  General n.A: the
+ * reference's steps, F <- R', F <- diag(a/2 e + sgn b |e|) F, then
+ * F <- R^{-T} F (the dgels solve, here elimination with partial pivoting).
+ * Returns 0, 1 for complex eigenvalues of n.A, 2 for linearly dependent
+ * eigenvectors, 3 if the eigenvalue iteration fails. */
 static int compute_flux_m(int m, const double *nA, double a, double b, int side, double *Fs)
 {
     const double sgn = (side == 0) ? 1.0 : -1.0;
@@ -1723,9 +1908,48 @@ static int compute_flux_m(int m, const double *nA, double a, double b, int side,
             for (int i = 0; i < m; ++i) Fs[i + m * j] = 0.5 * a * nA[j + m * i];
         return 0;
     }
+    int sym = 1;
     for (int i = 0; i < m; ++i)
         for (int j = 0; j < m; ++j)
-            if (nA[i + m * j] != nA[j + m * i]) return -1;
+            if (nA[i + m * j] != nA[j + m * i]) sym = 0;
+    if (!sym) {
+        double R[64], e[8], T[64];
+        const int rc = real_eig_general(m, nA, R, e);
+        if (rc) return rc;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                Fs[i + m * j] = (0.5 * a * e[i] + sgn * b * fabs(e[i])) * R[j + m * i];
+                T[i + m * j] = R[j + m * i];
+            }
+        double tmax = 0.0;
+        for (int i = 0; i < m * m; ++i) tmax = fmax(tmax, fabs(T[i]));
+        for (int c = 0; c < m; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < m; ++r)
+                if (fabs(T[r + m * c]) > fabs(T[piv + m * c])) piv = r;
+            if (fabs(T[piv + m * c]) <= 1e-12 * tmax) return 2;
+            for (int j = 0; j < m; ++j) {
+                double x = T[c + m * j];
+                T[c + m * j] = T[piv + m * j];
+                T[piv + m * j] = x;
+                x = Fs[c + m * j];
+                Fs[c + m * j] = Fs[piv + m * j];
+                Fs[piv + m * j] = x;
+            }
+            for (int r = c + 1; r < m; ++r) {
+                const double f = T[r + m * c] / T[c + m * c];
+                for (int j = c; j < m; ++j) T[r + m * j] -= f * T[c + m * j];
+                for (int j = 0; j < m; ++j) Fs[r + m * j] -= f * Fs[c + m * j];
+            }
+        }
+        for (int c = m - 1; c >= 0; --c)
+            for (int j = 0; j < m; ++j) {
+                double v = Fs[c + m * j];
+                for (int q = c + 1; q < m; ++q) v -= T[c + m * q] * Fs[q + m * j];
+                Fs[c + m * j] = v / T[c + m * c];
+            }
+        return 0;
+    }
     double R[64], e[8], coef[8];
     sym_eig(m, nA, R, e);
     for (int k = 0; k < m; ++k) coef[k] = 0.5 * a * e[k] + sgn * b * fabs(e[k]);
@@ -1736,6 +1960,12 @@ static int compute_flux_m(int m, const double *nA, double a, double b, int side,
             Fs[c + m * d] = s;
         }
     return 0;
+}
+
+int orc_flux_matrix(int m, const double *nA, double a, double b, int side, double *Fs)
+{
+    if (m < 1 || m > 8) return 3;
+    return compute_flux_m(m, nA, a, b, side, Fs);
 }
 
 struct orc_adv {

@@ -102,11 +102,17 @@ void orc_load_vector_gauss(const orc_mesh *m, int n, int nq, int nsrc, const dou
 /* ---- Advection<true>, AdvectionHomogeneousBC<true> (Advection.hpp:15-273) -
  * On the mesh and basis of W, nvar <= 8 variables, coefficient A of shape
  * (nvar, nvar, 2) (constant_coefficient) or (nvar, nvar, 2, n, n, nel), flux
- * constants a = -1, b = -0.5.  n.A must be symmetric (the |n.A| of the
- * upwind flux comes from a symmetric eigen-decomposition); create returns
- * NULL otherwise.  action: divF += Div(u) + interior fluxes; bc_action:
+ * constants a = -1, b = -0.5.  n.A must have real eigenvalues and a full
+ * set of eigenvectors at every face node (real_eig, eig.cpp:29-74); create
+ * returns NULL otherwise.  action: divF += Div(u) + interior fluxes; bc_action:
  * divF += boundary fluxes with u = 0 outside. */
 typedef struct orc_adv orc_adv;
+/* _ComputeFlux::flux (EdgeFlux.cpp:31-89) for one m x m normal matrix n.A
+ * (column-major, m <= 8): Fs(c, d) = (a/2 nA + sgn b |nA|)(d, c), sgn = +1
+ * on side 0 and -1 on side 1.  Returns 0; 1 if n.A has complex eigenvalues;
+ * 2 if its eigenvectors are linearly dependent; 3 if m is out of range or
+ * the eigenvalue iteration fails. */
+int orc_flux_matrix(int m, const double *nA, double a, double b, int side, double *Fs);
 orc_adv *orc_advection_create(const orc_wave *W, int nvar, const double *A, int constant_coefficient);
 void orc_advection_free(orc_adv *V);
 void orc_advection_action(const orc_adv *V, const double *u, double *divF);

@@ -206,6 +206,52 @@ int main(int argc, char** argv)
         double absC[9];
         CHECK(wdg::symmetric_abs(3, C, absC));
     }
+    // |C| of non-symmetric matrices C = L diag(e) L^{-1} (L unit lower
+    // triangular, repeated and zero eigenvalues), host routine against the
+    // oracle's flux matrix; complex and defective matrices are refused
+    {
+        unsigned seed = 12345u;
+        auto rnd = [&seed] {
+            seed = seed * 1664525u + 1013904223u;
+            return (seed >> 8) / double(1u << 24) - 0.5;
+        };
+        for (int m = 1; m <= 8; ++m)
+            for (int trial = 0; trial < 20; ++trial) {
+                double L[64] = {0}, Li[64] = {0}, e[8], C[64], absC[64], F[64];
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < m; ++i) L[i + m * j] = i == j ? 1.0 : (i > j ? rnd() : 0.0);
+                for (int j = 0; j < m; ++j) {  // columns of L^{-1} by forward substitution
+                    for (int i = 0; i < m; ++i) {
+                        double v = i == j ? 1.0 : 0.0;
+                        for (int k = 0; k < i; ++k) v -= L[i + m * k] * Li[k + m * j];
+                        Li[i + m * j] = v;
+                    }
+                }
+                for (int i = 0; i < m; ++i) e[i] = trial % 2 ? std::floor(4.0 * rnd()) : rnd();
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < m; ++i) {
+                        double v = 0.0;
+                        for (int k = 0; k < m; ++k) v += L[i + m * k] * e[k] * Li[k + m * j];
+                        C[i + m * j] = v;
+                    }
+                CHECK(wdg::real_abs(m, C, absC) == 0);
+                CHECK(orc_flux_matrix(m, C, -1.0, -0.5, 0, F) == 0);
+                double worst = 0.0, scale = 1e-300;
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < m; ++i) {
+                        scale = std::fmax(scale, std::fabs(C[i + m * j]));
+                        worst = std::fmax(worst, std::fabs(F[j + m * i] - (-0.5 * C[i + m * j] - 0.5 * absC[i + m * j])));
+                    }
+                CHECK(worst <= 1e-9 * scale);
+            }
+        const double rot[4] = {0.0, 1.0, -1.0, 0.0}, jordan[4] = {1.0, 0.0, 1.0, 1.0};
+        double out[4];
+        CHECK(wdg::real_abs(2, rot, out) == 1);
+        CHECK(wdg::real_abs(2, jordan, out) == 2);
+        CHECK(orc_flux_matrix(2, rot, -1.0, -0.5, 0, out) == 1);
+        CHECK(orc_flux_matrix(2, jordan, -1.0, -0.5, 0, out) == 2);
+        CHECK(wdg::real_abs(9, rot, out) == 3);
+    }
     // meshes: structured, perturbed, L-shaped, and the reference's own
     std::vector<wdg::HostMesh> meshes;
     meshes.push_back(wdg::HostMesh::uniform_rect(5, -1.0, 1.0, 4, -1.0, 1.0));

@@ -84,3 +84,43 @@ def test_bc_codes_are_indexed_in_boundary_edge_order():
     om, gm = O.Mesh(v, e), capi.Mesh(v, e)
     assert np.array_equal(om.edges(1), gm.edges(1))
     assert bc_codes(gm.n_boundary, "mixed").shape == (gm.n_boundary,)
+
+
+@pytest.mark.parametrize("m", range(1, 9))
+def test_host_flux_matrix_is_the_eigen_decomposition(m):
+    """wdg_hip_flux_matrix (the routine the advection face tables use) against
+    R diag(|e|) R^{-1} of the chosen factors, LAPACK's eigen-decomposition
+    where the eigenvalues are distinct, and the oracle's restatement."""
+    from tests.test_oracle_advection import flux_cases
+
+    for C, absC, cond, distinct in flux_cases(m):
+        scale = max(np.abs(C).max(), 1e-300)
+        refs = [absC]
+        if distinct:
+            lam, V = np.linalg.eig(C)
+            refs.append((V @ np.diag(np.abs(lam)) @ np.linalg.inv(V)).real)
+        for side, sgn in ((0, 1.0), (1, -1.0)):
+            F = capi.flux_matrix(C, -1.0, -0.5, side)
+            for ref in refs:
+                assert np.max(np.abs(F - (-0.5 * C + sgn * -0.5 * ref).T)) <= 1e-12 * cond * scale
+            rc, Fo = O.flux_matrix(C, -1.0, -0.5, side)
+            assert rc == 0 and np.max(np.abs(F - Fo)) <= 1e-12 * cond * scale
+
+
+def test_host_flux_matrix_refusals():
+    with pytest.raises(capi.WdgError, match="complex eigenvalues"):
+        capi.flux_matrix(np.array([[0.0, -1.0], [1.0, 0.0]]), -1.0, -0.5, 0)
+    with pytest.raises(capi.WdgError, match="linearly dependent"):
+        capi.flux_matrix(np.array([[1.0, 1.0], [0.0, 1.0]]), -1.0, -0.5, 0)
+    rng = np.random.default_rng(5)
+    for m in range(2, 9):
+        J = np.diag(rng.choice([-1.0, 0.0, 2.0], m))
+        k = int(rng.integers(0, m - 1))
+        J[k, k + 1], J[k + 1, k + 1] = 1.0, J[k, k]
+        Q = rng.standard_normal((m, m)) + 2.0 * np.eye(m)
+        with pytest.raises(capi.WdgError, match="linearly dependent"):
+            capi.flux_matrix(Q @ J @ np.linalg.inv(Q), -1.0, -0.5, 1)
+    with pytest.raises(capi.WdgError):
+        capi.flux_matrix(np.eye(9), -1.0, -0.5, 0)
+    rot = np.array([[0.0, -1.0], [1.0, 0.0]])
+    assert np.array_equal(capi.flux_matrix(rot, 0.8, 0.0, 1), 0.4 * rot.T)

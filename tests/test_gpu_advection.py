@@ -1,8 +1,10 @@
 """Advection<true> and AdvectionHomogeneousBC<true> on the GPU against the
 oracle's restatement (1e-12 relative L2): constant and per-node
-coefficients, scalar and symmetric systems, and the wave equation's
-matrices, for which the operator is WaveEquation::action; on Gauss-Lobatto
-and Gauss-Legendre bases."""
+coefficients, scalar, symmetric and non-symmetric systems (the latter with
+real eigenvalues: T S T^{-1} for a symmetric S, and linearised shallow
+water), and the wave equation's matrices, for which the operator is
+WaveEquation::action; on Gauss-Lobatto and Gauss-Legendre bases; the same
+for Advection<false>."""
 from __future__ import annotations
 
 import numpy as np
@@ -10,7 +12,7 @@ import pytest
 
 from oracle import oracle as O
 from tests.cases import mesh_case, random_state, rel_l2
-from tests.test_oracle_advection import WAVE_A, sym_coeff
+from tests.test_oracle_advection import WAVE_A, shallow_water_coeff, similar_coeff, sym_coeff
 from wavedg_amd import capi
 
 pytestmark = pytest.mark.gpu
@@ -90,11 +92,65 @@ def test_advection_wave_matrices_equal_wave_action(cuda, leg):
     assert float(torch.linalg.norm(y1 - y2) / torch.linalg.norm(y2)) < TOL
 
 
+def _nonsymmetric(nvar, cc, n, nel, seed):
+    rng = np.random.default_rng(seed)
+    T = rng.standard_normal((nvar, nvar)) + 2.0 * np.eye(nvar)
+    if cc:
+        return similar_coeff(sym_coeff(nvar, seed), nvar, T)
+    S = np.concatenate([sym_coeff(nvar, int(s)) * (1.0 + 0.3 * rng.standard_normal())
+                        for s in rng.integers(0, 1 << 30, n * n * nel)])
+    return similar_coeff(S, nvar, T)
+
+
+@BASES
+@pytest.mark.parametrize("approx_quad", [True, False], ids=["collocated", "exact"])
+@pytest.mark.parametrize("n,nvar,cc", [(3, 2, True), (6, 3, True), (4, 2, False), (8, 3, False)])
+def test_advection_nonsymmetric_coefficient(cuda, n, nvar, cc, approx_quad, leg):
+    torch = cuda
+    print(f"advection non-symmetric n={n} nvar={nvar} constant={cc} approx_quad={approx_quad} legendre={leg}",
+          flush=True)
+    v, e = mesh_case("skewed6x5")
+    nel = np.asarray(e).reshape(-1, 4).shape[0]
+    A = _nonsymmetric(nvar, cc, n, nel, 300 + n)
+    ow = O.Wave(O.Mesh(v, e), n, legendre=leg)
+    ov = O.Advection(ow, nvar, A, cc) if approx_quad else O.AdvectionQ(ow, nvar, A, cc)
+    gv = capi.Advection(_wave(v, e, n, leg), nvar, A, cc, approx_quad=approx_quad)
+    u = random_state(nvar * n * n * nel, 41)
+    ya = torch.zeros(u.size, dtype=torch.float64, device="cuda")
+    yb = torch.zeros_like(ya)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy(), ov.action(u)) < TOL
+    assert rel_l2(yb.cpu().numpy(), ov.bc_action(u)) < TOL
+
+
+@BASES
+def test_advection_shallow_water(cuda, leg):
+    torch = cuda
+    v, e = mesh_case("L")
+    n = 5
+    A, _ = shallow_water_coeff(0.3, -0.2, 1.5, 9.81)
+    ov = O.Advection(O.Wave(O.Mesh(v, e), n, legendre=leg), 3, A, True)
+    gw = _wave(v, e, n, leg)
+    gv = capi.Advection(gw, 3, A, True)
+    u = random_state(gw.ndof, 43)
+    ya = torch.zeros(u.size, dtype=torch.float64, device="cuda")
+    yb = torch.zeros_like(ya)
+    gv.action(_dev(torch, u), ya)
+    gv.bc_action(_dev(torch, u), yb)
+    torch.cuda.synchronize()
+    assert rel_l2(ya.cpu().numpy(), ov.action(u)) < TOL
+    assert rel_l2(yb.cpu().numpy(), ov.bc_action(u)) < TOL
+
+
 def test_advection_rejects_bad_input(cuda):
     v, e = mesh_case("rect5x4")
     gw = capi.Wave2D(capi.Mesh(v, e), 3, None)
-    with pytest.raises(capi.WdgError):  # n.A not symmetric
-        capi.Advection(gw, 2, [0.0, 1.0, 2.0, 0.0, 0.0, 0.0, 0.0, 0.0], True)
+    with pytest.raises(capi.WdgError, match="complex eigenvalues"):  # A_x = [[0, -1], [1, 0]]
+        capi.Advection(gw, 2, [0.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0], True)
+    with pytest.raises(capi.WdgError, match="complex eigenvalues"):
+        capi.Advection(gw, 2, [0.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0], True, approx_quad=False)
     with pytest.raises(capi.WdgError):
         capi.Advection(gw, 9, np.zeros(162), True)
     g32 = capi.Wave2D(capi.Mesh(v, e), 3, None, precision=capi.F32)

@@ -49,6 +49,7 @@ _SIGS = {
     "wdg_hip_mesh_nodes": [_vp, _i, _dp],
     "wdg_hip_gll": [_i, _dp, _dp, _dp],
     "wdg_hip_gauss_legendre": [_i, _dp, _dp],
+    "wdg_hip_flux_matrix": [_i, _dp, ctypes.c_double, ctypes.c_double, _i, _dp],
     "wdg_hip_wave2d_create": [_vp, _i, _ip, _d, _d, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_create_exact": [_vp, _i, _ip, _d, _d, _i, _i, _i, ctypes.POINTER(_vp)],
     "wdg_hip_wave2d_create_basis": [_vp, _i, _i, _ip, _d, _d, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
@@ -252,6 +253,18 @@ def gll(n: int):
     x, w, D = np.zeros(n), np.zeros(n), np.zeros(n * n)
     _call("wdg_hip_gll", n, _dptr(x), _dptr(w), _dptr(D))
     return x, w, D.reshape(n, n).T  # D[i, j] = phi_j'(x_i)
+
+
+def flux_matrix(nA, a: float, b: float, side: int) -> np.ndarray:
+    """The flux matrix F = (a/2 nA + sgn b |nA|)^T of EdgeFlux.cpp:31-89 for
+    one m x m normal matrix (host only); raises WdgError when nA has complex
+    eigenvalues or linearly dependent eigenvectors."""
+    nA = np.asarray(nA, dtype=np.float64)
+    m = nA.shape[0]
+    F = np.zeros(m * m)
+    colmajor = np.ascontiguousarray(nA.T).reshape(-1)
+    _call("wdg_hip_flux_matrix", m, _dptr(colmajor), a, b, side, _dptr(F))
+    return F.reshape(m, m).T
 
 
 def gauss_legendre(n: int):
@@ -519,7 +532,8 @@ class Advection:
     """Advection<true> + AdvectionHomogeneousBC<true> on an FP64 wave handle's
     mesh and basis.  A_colmajor is the coefficient flattened in the
     reference's column-major order: (n_var, n_var, 2) when constant, else
-    (n_var, n_var, 2, n, n, n_elem); n.A must be symmetric."""
+    (n_var, n_var, 2, n, n, n_elem); n.A must have real eigenvalues and a full
+    set of eigenvectors at every face node."""
 
     def __init__(self, wave: Wave2D, n_var: int, A_colmajor, constant_coefficient: bool = True,
                  approx_quad: bool = True, quad=(0, 0)):

@@ -1,6 +1,8 @@
 #include "host_basis.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace wdg {
 
@@ -308,6 +310,338 @@ bool symmetric_abs(int m, const double* C, double* absC)
             absC[c + m * d] = v;
         }
     return true;
+}
+
+namespace {
+
+// General real m x m matrices, m <= 8, column-major: M(i, j) = M[i + m j].
+
+// Diagonal similarity by powers of two that evens out row and column norms
+// (Parlett and Reinsch, "Balancing a matrix for calculation of eigenvalues
+// and eigenvectors", Numer. Math. 13, 1969; LAPACK's scaling-only balance):
+// B <- D^{-1} B D with D = diag(d).  Exact in floating point.
+void balance(int m, double* B, double* d)
+{
+    for (int i = 0; i < m; ++i) d[i] = 1.0;
+    for (bool again = true; again;) {
+        again = false;
+        for (int i = 0; i < m; ++i) {
+            double c = 0.0, r = 0.0;
+            for (int j = 0; j < m; ++j)
+                if (j != i) {
+                    c += std::fabs(B[j + m * i]);
+                    r += std::fabs(B[i + m * j]);
+                }
+            if (c == 0.0 || r == 0.0) continue;
+            const double s = c + r;
+            double f = 1.0;
+            while (c < 0.5 * r) {
+                f *= 2.0;
+                c *= 4.0;
+            }
+            while (c >= 2.0 * r) {
+                f *= 0.5;
+                c *= 0.25;
+            }
+            if ((c + r) / f < 0.95 * s) {
+                again = true;
+                d[i] *= f;
+                for (int j = 0; j < m; ++j) {
+                    B[i + m * j] /= f;
+                    B[j + m * i] *= f;
+                }
+            }
+        }
+    }
+}
+
+// Householder reduction to upper Hessenberg form (similarity; only the
+// eigenvalues are used afterwards).
+void hessenberg(int m, double* H)
+{
+    for (int k = 0; k + 2 < m; ++k) {
+        double v[8] = {0.0}, nx = 0.0;
+        for (int i = k + 1; i < m; ++i) nx += H[i + m * k] * H[i + m * k];
+        nx = std::sqrt(nx);
+        if (nx == 0.0) continue;
+        const double alpha = H[k + 1 + m * k] > 0.0 ? -nx : nx;
+        double vv = 0.0;
+        for (int i = k + 1; i < m; ++i) {
+            v[i] = H[i + m * k] - (i == k + 1 ? alpha : 0.0);
+            vv += v[i] * v[i];
+        }
+        if (vv == 0.0) continue;
+        const double beta = 2.0 / vv;
+        for (int j = 0; j < m; ++j) {
+            double t = 0.0;
+            for (int i = k + 1; i < m; ++i) t += v[i] * H[i + m * j];
+            t *= beta;
+            for (int i = k + 1; i < m; ++i) H[i + m * j] -= t * v[i];
+        }
+        for (int i = 0; i < m; ++i) {
+            double t = 0.0;
+            for (int j = k + 1; j < m; ++j) t += H[i + m * j] * v[j];
+            t *= beta;
+            for (int j = k + 1; j < m; ++j) H[i + m * j] -= t * v[j];
+        }
+        for (int i = k + 2; i < m; ++i) H[i + m * k] = 0.0;
+    }
+}
+
+// Eigenvalues of the real 2 x 2 block [[a, b], [c, d]]: the pair mid +- r
+// written as d + z and d - bc / z (no cancellation), or a complex pair.
+void eig2(double a, double b, double c, double d, double* wr, double* wi)
+{
+    const double p = 0.5 * (a - d), bc = b * c, disc = p * p + bc;
+    if (disc >= 0.0) {
+        const double r = std::sqrt(disc), z = p >= 0.0 ? p + r : p - r;
+        if (z == 0.0) {
+            wr[0] = wr[1] = d;
+        } else {
+            wr[0] = d + z;
+            wr[1] = d - bc / z;
+        }
+        wi[0] = wi[1] = 0.0;
+    } else {
+        wr[0] = wr[1] = 0.5 * (a + d);
+        wi[0] = std::sqrt(-disc);
+        wi[1] = -wi[0];
+    }
+}
+
+// Reflector I - beta v v' with v = x - alpha e1 mapping x (n = 2 or 3) onto
+// alpha e1; beta = 0 when x is already a multiple of e1.
+double reflector(int n, const double* x, double* v)
+{
+    double tail = 0.0;
+    for (int i = 1; i < n; ++i) tail += x[i] * x[i];
+    if (tail == 0.0) return 0.0;
+    const double nx = std::sqrt(x[0] * x[0] + tail), alpha = x[0] > 0.0 ? -nx : nx;
+    v[0] = x[0] - alpha;
+    for (int i = 1; i < n; ++i) v[i] = x[i];
+    return 2.0 / (v[0] * v[0] + tail);
+}
+
+// Eigenvalues of an upper Hessenberg matrix by the Francis double-shift QR
+// iteration with deflation (Golub and Van Loan, Matrix Computations, 4th ed.,
+// Algorithms 7.5.1-7.5.2), transforms restricted to the active window.
+// False if the iteration does not converge.
+bool francis_eigenvalues(int m, double* H, double* wr, double* wi)
+{
+    const double eps = std::numeric_limits<double>::epsilon();
+    double norm = 0.0;
+    for (int i = 0; i < m * m; ++i) norm += H[i] * H[i];
+    norm = std::sqrt(norm);
+    auto h = [&](int i, int j) -> double& { return H[i + m * j]; };
+    int hi = m - 1, its = 0, total = 0;
+    while (hi >= 0) {
+        // a subdiagonal negligible next to its diagonal neighbours, or next
+        // to the whole matrix (a repeated eigenvalue leaves subdiagonals at
+        // rounding level that no shift reduces relative to the diagonal)
+        int l = hi;
+        for (; l > 0; --l) {
+            const double s = std::fabs(h(l - 1, l - 1)) + std::fabs(h(l, l));
+            if (std::fabs(h(l, l - 1)) <= eps * s || std::fabs(h(l, l - 1)) <= 4.0 * eps * norm) {
+                h(l, l - 1) = 0.0;
+                break;
+            }
+        }
+        if (l == hi) {
+            wr[hi] = h(hi, hi);
+            wi[hi] = 0.0;
+            --hi;
+            its = 0;
+            continue;
+        }
+        if (l == hi - 1) {
+            eig2(h(hi - 1, hi - 1), h(hi - 1, hi), h(hi, hi - 1), h(hi, hi), wr + hi - 1, wi + hi - 1);
+            hi -= 2;
+            its = 0;
+            continue;
+        }
+        if (++total > 100 * m) return false;
+        ++its;
+        double s, t;
+        if (its % 10 == 0) {  // exceptional shifts h(hi, hi) + 0.75 sig +- i 0.66 sig against cycling
+            const double sig = std::fabs(h(hi, hi - 1)) + std::fabs(h(hi - 1, hi - 2)), x = h(hi, hi) + 0.75 * sig;
+            s = 2.0 * x;
+            t = x * x + 0.4375 * sig * sig;
+        } else {
+            s = h(hi - 1, hi - 1) + h(hi, hi);
+            t = h(hi - 1, hi - 1) * h(hi, hi) - h(hi - 1, hi) * h(hi, hi - 1);
+        }
+        double x[3] = {h(l, l) * h(l, l) + h(l, l + 1) * h(l + 1, l) - s * h(l, l) + t,
+                       h(l + 1, l) * (h(l, l) + h(l + 1, l + 1) - s), h(l + 1, l) * h(l + 2, l + 1)};
+        for (int k = l; k + 2 <= hi; ++k) {
+            double v[3];
+            const double beta = reflector(3, x, v);
+            if (beta != 0.0) {
+                for (int j = std::max(l, k - 1); j <= hi; ++j) {
+                    const double d = beta * (v[0] * h(k, j) + v[1] * h(k + 1, j) + v[2] * h(k + 2, j));
+                    for (int q = 0; q < 3; ++q) h(k + q, j) -= d * v[q];
+                }
+                for (int i = l; i <= std::min(k + 3, hi); ++i) {
+                    const double d = beta * (h(i, k) * v[0] + h(i, k + 1) * v[1] + h(i, k + 2) * v[2]);
+                    for (int q = 0; q < 3; ++q) h(i, k + q) -= d * v[q];
+                }
+            }
+            x[0] = h(k + 1, k);
+            x[1] = h(k + 2, k);
+            if (k + 3 <= hi) x[2] = h(k + 3, k);
+        }
+        double v[2];
+        const double beta = reflector(2, x, v);
+        if (beta != 0.0) {
+            for (int j = std::max(l, hi - 2); j <= hi; ++j) {
+                const double d = beta * (v[0] * h(hi - 1, j) + v[1] * h(hi, j));
+                h(hi - 1, j) -= d * v[0];
+                h(hi, j) -= d * v[1];
+            }
+            for (int i = l; i <= hi; ++i) {
+                const double d = beta * (h(i, hi - 1) * v[0] + h(i, hi) * v[1]);
+                h(i, hi - 1) -= d * v[0];
+                h(i, hi) -= d * v[1];
+            }
+        }
+    }
+    return true;
+}
+
+// One-sided Jacobi SVD (Hestenes): M V = U diag(sig), V orthogonal.  M is
+// overwritten by U diag(sig).
+void jacobi_svd(int m, double* M, double* V, double* sig)
+{
+    for (int i = 0; i < m * m; ++i) V[i] = (i % (m + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int p = 0; p + 1 < m; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int k = 0; k < m; ++k) {
+                    al += M[k + m * p] * M[k + m * p];
+                    be += M[k + m * q] * M[k + m * q];
+                    ga += M[k + m * p] * M[k + m * q];
+                }
+                if (std::fabs(ga) <= 1e-15 * std::sqrt(al * be) || ga == 0.0) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int k = 0; k < m; ++k) {
+                    const double mp = M[k + m * p], mq = M[k + m * q];
+                    M[k + m * p] = c * mp - s * mq;
+                    M[k + m * q] = s * mp + c * mq;
+                    const double vp = V[k + m * p], vq = V[k + m * q];
+                    V[k + m * p] = c * vp - s * vq;
+                    V[k + m * q] = s * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
+    }
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < m; ++k) s += M[k + m * j] * M[k + m * j];
+        sig[j] = std::sqrt(s);
+    }
+}
+
+}  // namespace
+
+int real_abs(int m, const double* C, double* absC)
+{
+    if (m < 1 || m > 8) return 3;
+    bool sym = true;
+    for (int j = 0; j < m && sym; ++j)
+        for (int i = 0; i < j; ++i)
+            if (C[i + m * j] != C[j + m * i]) {
+                sym = false;
+                break;
+            }
+    if (sym) return symmetric_abs(m, C, absC) ? 0 : 3;
+
+    double B[64], d[8], H[64], wr[8], wi[8];
+    for (int i = 0; i < m * m; ++i) B[i] = C[i];
+    balance(m, B, d);
+    double nb = 0.0;
+    for (int i = 0; i < m * m; ++i) nb += B[i] * B[i];
+    nb = std::sqrt(nb);
+    for (int i = 0; i < m * m; ++i) H[i] = B[i];
+    hessenberg(m, H);
+    if (!francis_eigenvalues(m, H, wr, wi)) return 3;
+    // Real eigenvalues only (eig.cpp:63-72).  Eigenvalues within tol of each
+    // other are one repeated eigenvalue: rounding moves a repeated eigenvalue
+    // by O(eps |C|) if it has a full set of eigenvectors, and splits it into a
+    // real or complex pair O(sqrt(eps |C|)) apart if it has not, so the
+    // tolerance sits between the two; the eigenvector count below then
+    // refuses the second kind instead of building |C| from nearly parallel
+    // eigenvectors.
+    const double tol = 1e-7 * nb;
+    for (int i = 0; i < m; ++i)
+        if (std::fabs(wi[i]) > tol) return 1;
+    std::sort(wr, wr + m);
+
+    // eigenvectors: for each cluster of equal eigenvalues (multiplicity k), the
+    // k right singular vectors of B - lambda I with the smallest singular values
+    double R[64], lam[8];
+    int nv = 0;
+    for (int i = 0; i < m;) {
+        int j = i + 1;
+        while (j < m && wr[j] - wr[j - 1] <= tol) ++j;
+        const int k = j - i;
+        double mean = 0.0;
+        for (int q = i; q < j; ++q) mean += wr[q];
+        mean /= k;
+        double M[64], V[64], sig[8];
+        for (int q = 0; q < m * m; ++q) M[q] = B[q] - (q % (m + 1) == 0 ? mean : 0.0);
+        jacobi_svd(m, M, V, sig);
+        int idx[8];
+        for (int q = 0; q < m; ++q) idx[q] = q;
+        std::sort(idx, idx + m, [&](int a, int b) { return sig[a] < sig[b]; });
+        if (sig[idx[k - 1]] > tol) return 2;  // fewer than k independent eigenvectors
+        for (int q = 0; q < k; ++q) {
+            for (int r = 0; r < m; ++r) R[r + m * nv] = V[r + m * idx[q]];
+            lam[nv++] = std::fabs(mean);
+        }
+        i = j;
+    }
+
+    // |B| R = R diag|lam|: solve R' X = (R diag|lam|)' by elimination with
+    // partial pivoting, |B| = X'
+    double A[64], X[64];
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            A[i + m * j] = R[j + m * i];
+            X[i + m * j] = R[j + m * i] * lam[i];
+        }
+    double amax = 0.0;
+    for (int i = 0; i < m * m; ++i) amax = std::max(amax, std::fabs(A[i]));
+    for (int c = 0; c < m; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < m; ++r)
+            if (std::fabs(A[r + m * c]) > std::fabs(A[piv + m * c])) piv = r;
+        if (std::fabs(A[piv + m * c]) <= 1e-12 * amax) return 2;
+        if (piv != c)
+            for (int j = 0; j < m; ++j) {
+                std::swap(A[c + m * j], A[piv + m * j]);
+                std::swap(X[c + m * j], X[piv + m * j]);
+            }
+        for (int r = c + 1; r < m; ++r) {
+            const double f = A[r + m * c] / A[c + m * c];
+            if (f == 0.0) continue;
+            for (int j = c; j < m; ++j) A[r + m * j] -= f * A[c + m * j];
+            for (int j = 0; j < m; ++j) X[r + m * j] -= f * X[c + m * j];
+        }
+    }
+    for (int c = m - 1; c >= 0; --c)
+        for (int j = 0; j < m; ++j) {
+            double v = X[c + m * j];
+            for (int q = c + 1; q < m; ++q) v -= A[c + m * q] * X[q + m * j];
+            X[c + m * j] = v / A[c + m * c];
+        }
+    // |C| = D |B| D^{-1}
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) absC[i + m * j] = d[i] * X[j + m * i] / d[j];
+    return 0;
 }
 
 }  // namespace wdg

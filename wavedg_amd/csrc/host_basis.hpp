@@ -40,4 +40,16 @@ bool mass_fast_diag(int n, const double* xb, int nq, const double* xq, const dou
 // (column-major, m <= 8) by cyclic Jacobi; false if C is not symmetric.
 bool symmetric_abs(int m, const double* C, double* absC);
 
+// |C| = R diag(|e|) R^{-1} for a real m x m matrix C = R diag(e) R^{-1}
+// (column-major, m <= 8) with real eigenvalues and a full set of
+// eigenvectors: the factorisation _ComputeFlux::flux builds from real_eig
+// (EdgeFlux.cpp:31-89, eig.cpp:29-74).  Symmetric C takes the cyclic Jacobi
+// route of symmetric_abs; otherwise C is balanced, its eigenvalues come from
+// the Francis QR iteration, the eigenvectors of each group of equal
+// eigenvalues from the null space of C - lambda I (Jacobi SVD), and |C| from
+// one solve with the eigenvector matrix.  Returns 0; 1 if C has complex
+// eigenvalues; 2 if its eigenvectors are linearly dependent; 3 if m is out
+// of range or the iteration does not converge.
+int real_abs(int m, const double* C, double* absC);
+
 }  // namespace wdg

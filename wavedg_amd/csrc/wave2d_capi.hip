@@ -772,6 +772,31 @@ int wdg_hip_gll(int n, double* x, double* w, double* D)
     });
 }
 
+// |C| of a normal matrix C = n.A with real eigenvalues and a full set of
+// eigenvectors (real_eig, eig.cpp:29-74); refusals name the failure as the
+// reference's EdgeFlux errors do (EdgeFlux.cpp:59-60, 86-88).
+static void abs_normal_matrix(int m, const double* C, double* absC, const char* where)
+{
+    const int rc = wdg::real_abs(m, C, absC);
+    if (rc == 1) fail(WDG_ERR_INVALID, std::string("n.A has complex eigenvalues ") + where);
+    if (rc == 2) fail(WDG_ERR_INVALID, std::string("eigenvectors of n.A are linearly dependent ") + where);
+    if (rc != 0) fail(WDG_ERR_INVALID, std::string("eigen-decomposition of n.A failed ") + where);
+}
+
+int wdg_hip_flux_matrix(int m, const double* nA, double a, double b, int side, double* F)
+{
+    return guarded([&] {
+        if (m < 1 || m > 8) fail(WDG_ERR_INVALID, "m out of range 1..8");
+        if (!nA || !F) fail(WDG_ERR_INVALID, "null pointer");
+        if (side != 0 && side != 1) fail(WDG_ERR_INVALID, "side must be 0 or 1");
+        double absC[64] = {0.0};
+        if (b != 0.0) abs_normal_matrix(m, nA, absC, "");
+        const double sgn = side == 0 ? 1.0 : -1.0;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) F[i + m * j] = 0.5 * a * nA[j + m * i] + sgn * b * absC[j + m * i];
+    });
+}
+
 int wdg_hip_gauss_legendre(int n, double* x, double* w)
 {
     return guarded([&] {
@@ -1791,7 +1816,7 @@ int wdg_hip_advection_create(wdg_wave2d op, int n_var, const double* A, int cons
         double C[64], absC[64];
         auto normal_matrix = [&](const double* Ak, double n0, double n1) {
             for (int q = 0; q < v2; ++q) C[q] = n0 * Ak[q] + n1 * Ak[v2 + q];
-            if (!wdg::symmetric_abs(m, C, absC)) fail(WDG_ERR_INVALID, "n.A is not symmetric at a face node");
+            abs_normal_matrix(m, C, absC, "at a face node");
         };
         for (int e = 0; e < nel; ++e) {
             const double* g = &geom[6 * (size_t)e];
@@ -1925,7 +1950,7 @@ int wdg_hip_advection_create_exact(wdg_wave2d op, int n_var, const double* A, in
         double C[64], absC[64], Af[128];
         auto normal_matrix = [&](const double* Ak, double n0, double n1) {
             for (int q = 0; q < v2; ++q) C[q] = n0 * Ak[q] + n1 * Ak[v2 + q];
-            if (!wdg::symmetric_abs(m, C, absC)) fail(WDG_ERR_INVALID, "n.A is not symmetric at an edge point");
+            abs_normal_matrix(m, C, absC, "at an edge point");
         };
         for (int e = 0; e < nel; ++e) {
             const double* g = &geom[6 * (size_t)e];

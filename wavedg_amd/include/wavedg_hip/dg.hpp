@@ -354,8 +354,9 @@ private:
 // Advection<true> (Advection.hpp:15-140) and AdvectionHomogeneousBC<true>
 // (Advection.hpp:142-270): n_var <= 8 variables, coefficient `a` of shape
 // (n_var, n_var, 2) if constant_coefficient else (n_var, n_var, 2, n, n,
-// n_elem), column-major as in the reference; upwind flux.  n.A must be
-// symmetric at every face node.  <false> integrates on Gauss-Legendre rules of
+// n_elem), column-major as in the reference; upwind flux.  n.A must have
+// real eigenvalues and a full set of eigenvectors at every face node, as
+// real_eig requires (eig.cpp:29-74).  <false> integrates on Gauss-Legendre rules of
 // quad->n points (the reference's automatic sizes when quad is null).
 namespace detail {
 class AdvectionHandle {
