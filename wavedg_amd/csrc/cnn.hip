@@ -44,6 +44,8 @@
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;     // waves per workgroup of the conv kernels
@@ -277,6 +279,113 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(ConvArgs a)
     }
 }
 
+// The same convolution on gfx950's v_mfma_f32_16x16x32_bf16: slices of 32
+// input channels (a last slice of 16 is padded with zeros), so each operand
+// lane holds 8 consecutive channels (one 16-byte LDS read) and a slice costs
+// half the MFMA instructions, LDS reads and staging rounds of the 16-channel
+// kernel.  A lane l holds channel co + l % 16 and input channels
+// 8 (l / 16) .. + 7 of the weight row; B lane l pixel l % 16 and the same 8
+// input channels; the accumulator layout is that of conv3_tiled_kernel.
+template <int TA, bool F32OUT>
+__global__ __launch_bounds__(256) void conv3_k32_kernel(ConvArgs a)
+{
+    constexpr int CO_T = 32 * TA;
+    __shared__ __align__(16) uint16_t s_in[kHaloMax * 32];
+    __shared__ __align__(16) uint16_t s_w[9 * CO_T * 32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const int TW = min(a.W, 64), TH = 64 / TW;
+    const int tiles_x = a.W / TW, tiles_y = (a.H + TH - 1) / TH;
+    const int t = blockIdx.x;
+    const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int co0 = blockIdx.y * CO_T;
+    const int cw = (wave & 1) * (16 * TA);
+    const int pw = (wave >> 1) * 32;
+    const int HW2 = TW + 2, NH = (TH + 2) * HW2;
+    int py[2], px[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int p = pw + 16 * j + ln;
+        py[j] = p / TW;
+        px[j] = p - py[j] * TW;
+    }
+    f32x4 acc[TA][2];
+#pragma unroll
+    for (int i = 0; i < TA; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        __syncthreads();  // the previous slice is consumed
+        for (int q = tid; q < NH * 4; q += 256) {
+            const int hp = q >> 2, part = q & 3;
+            const int hy = hp / HW2, hx = hp - hy * HW2;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W && c0 + 8 * part < a.cin)
+                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
+                                                    c0 + 8 * part);
+            *reinterpret_cast<uint4*>(s_in + hp * 32 + 8 * part) = v;
+        }
+        for (int q = tid; q < 9 * CO_T * 4; q += 256) {
+            const int row = q >> 2, part = q & 3;
+            const int tap = row / CO_T, co = row - tap * CO_T;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (c0 + 8 * part < a.cin)
+                v = *reinterpret_cast<const uint4*>(a.w + ((size_t)tap * a.cout + co0 + co) * a.cin + c0 + 8 * part);
+            *reinterpret_cast<uint4*>(s_w + row * 32 + 8 * part) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int hy = tap / 3, hx = tap % 3;
+            bf16x8 av[TA], bv[2];
+#pragma unroll
+            for (int i = 0; i < TA; ++i)
+                av[i] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(s_w + (tap * CO_T + cw + 16 * i + ln) * 32 + 8 * kq));
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                bv[j] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(s_in + ((py[j] + hy) * HW2 + px[j] + hx) * 32 + 8 * kq));
+#pragma unroll
+            for (int i = 0; i < TA; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TA; ++i) {
+        const int co = co0 + cw + 16 * i + 4 * kq;
+        float bias[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.bias) {
+            const float4 b4 = *reinterpret_cast<const float4*>(a.bias + co);
+            bias[0] = b4.x, bias[1] = b4.y, bias[2] = b4.z, bias[3] = b4.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int y = y0 + py[j], x = x0 + px[j];
+            if (y >= a.H) continue;
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                v[q] = acc[i][j][q] + bias[q];
+                if (a.relu) v[q] = fmaxf(v[q], 0.f);
+            }
+            const size_t o = ((size_t)(b * a.H + y) * a.W + x) * a.out_ld + a.out_off + co;
+            if constexpr (F32OUT) {
+                *reinterpret_cast<float4*>(static_cast<float*>(a.out) + o) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out) + o) = make_uint2(lo, hi);
+            }
+        }
+    }
+}
+
 // dw[tap][co][ci] = sum_p dout[p, co] in[p + off(tap), ci]: a GEMM with
 // M = output channels, N = input channels, K = pixels.  One wave owns one
 // 16 x 16 tile of one tap over one chunk of pixels and writes its partial
@@ -288,7 +397,8 @@ struct WgradArgs {
     int in_ld, in_off, cin;
     int dout_ld, cout;
     int B, H, W, ks;
-    int tiles_per_chunk;  // wgrad3_tiled_kernel only
+    int tiles_per_chunk;  // wgrad3_tr_kernel
+    float* bpartial;      // wgrad3_tr_kernel: bias partial sums [chunk][co]
 };
 
 __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
@@ -348,119 +458,159 @@ __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
     for (int r = 0; r < 4; ++r) dst[(size_t)(cob * 16 + 4 * kq + r) * a.cin + cib * 16 + ln] = acc[r];
 }
 
-// Weight gradient with LDS tiles (3x3, W a power of two; same 64-pixel tiles
-// as conv3_tiled_kernel).  A workgroup owns 32 output x CI_T input channels
-// for all 9 taps and a chunk of kWgTiles tiles.  Per tile it stages the
-// output gradient transposed ([co][pixel], so an A fragment is one 8-byte
-// read) and the input with its halo transposed ([ci][halo pixel]); wave w
-// accumulates the (tap, co block, ci block) MFMA tiles w, w + 4, ... over the
-// chunk (a.tiles_per_chunk tiles) and writes its partial sums to
-// partial[chunk][tap][co][ci].
+// Weight and bias gradients of a 3x3 layer (W a power of two, 32k output and
+// 16k input channels; a last block of 16 input channels is padded with
+// zeros) on gfx950's transposing LDS load.  A workgroup of six
+// waves owns 32 output x 32 input channels for all 9 taps and a chunk of
+// a.tiles_per_chunk 64-pixel tiles (the tiles of conv3_tiled_kernel).  Per
+// tile it stages the output gradient as [pixel][32 co] and the input with its
+// halo as [halo pixel][32 ci], both in the NHWC order of global memory with
+// 16-byte stores.  The weight gradient is a GEMM over pixels, so an MFMA
+// operand lane needs one channel at 8 consecutive pixels: ds_read_b64_tr_b16
+// hands lane 4a + e of each 16-lane group element e of the 8-byte reads of
+// lanes a, a + 4, a + 8 and a + 12, so when lane a + 4j reads channels
+// 4a .. 4a + 3 of pixel p + j, lane i receives channel i at pixels p .. p + 3;
+// two such reads fill one v_mfma_f32_16x16x32_bf16 operand.  Wave w owns the
+// input-channel block w % 2 and the taps w / 2, w / 2 + 3, w / 2 + 6 for both
+// output-channel blocks; its partial sums go to partial[chunk][tap][co][ci].
+// Wave 0 of the first input-channel block also sums the output gradient over
+// the chunk's pixels (the bias gradient) into bpartial[chunk][co].
+constexpr int kWgWaves = 6;
 
-template <int CI_T>
-__global__ __launch_bounds__(256) void wgrad3_tiled_kernel(WgradArgs a)
+__device__ __forceinline__ bf16x8 lds_tr_pair(const uint16_t* p0, const uint16_t* p1)
 {
-    constexpr int NO = 9 * 2 * (CI_T / 16);  // output MFMA tiles
-    constexpr int NQ = (NO + 3) / 4;         // per wave
-    __shared__ __align__(16) uint16_t s_d[32 * 64];
-    __shared__ __align__(16) uint16_t s_x[CI_T * kHaloMax];
+    typedef short i16x4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
+    const i16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)p0);
+    const i16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)p1);
+    return __builtin_bit_cast(bf16x8, i16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]});
+}
+
+__global__ __launch_bounds__(64 * kWgWaves) void wgrad3_tr_kernel(WgradArgs a)
+{
+    __shared__ __align__(16) uint16_t s_d[64 * 32];
+    __shared__ __align__(16) uint16_t s_x[kHaloMax * 32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ln = lane & 15, kq = lane >> 4;
+    const int g = lane >> 4, ln = lane & 15, la = ln & 3, lj = ln >> 2;
     const int TW = min(a.W, 64), TH = 64 / TW;
     const int sh = __builtin_ctz((unsigned)TW);
     const int tiles_x = a.W / TW, tiles_y = (a.H + TH - 1) / TH;
     const int ntiles = a.B * tiles_y * tiles_x;
     const int HW2 = TW + 2, NH = (TH + 2) * HW2;
-    const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * CI_T;
+    const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * 32;
     const int t0 = blockIdx.x * a.tiles_per_chunk, t1 = min(ntiles, t0 + a.tiles_per_chunk);
-    f32x4 acc[NQ];
+    const int ib = wave & 1, tapb = wave >> 1;
+    const bool bias = wave == 0 && blockIdx.z == 0;
+    f32x4 acc[3][2];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < 3; ++u) acc[u][0] = acc[u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[2] = {0.f, 0.f};
     for (int t = t0; t < t1; ++t) {
         const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
         const int y0 = ty * TH, x0 = tx * TW;
         __syncthreads();  // the previous tile is consumed
-        {
-            // output gradient: 64 pixels x 32 channels, four 8-channel parts per pixel
+        if (tid < 256) {
             const int pp = tid >> 2, part = tid & 3;
             const int y = y0 + (pp >> sh), x = x0 + (pp & (TW - 1));
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (y < a.H)
                 v = *reinterpret_cast<const uint4*>(a.dout + ((size_t)(b * a.H + y) * a.W + x) * a.dout_ld + co0 +
                                                     8 * part);
-            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                s_d[(8 * part + k) * 64 + pp] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+            *reinterpret_cast<uint4*>(s_d + pp * 32 + 8 * part) = v;
         }
-        for (int q = tid; q < NH * (CI_T / 8); q += 256) {
-            const int hp = q / (CI_T / 8), part = q - hp * (CI_T / 8);
+        for (int q = tid; q < NH * 4; q += 64 * kWgWaves) {
+            const int hp = q >> 2, part = q & 3;
             const int hy = hp / HW2, hx = hp - hy * HW2;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W && ci0 + 8 * part < a.cin)
                 v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
                                                     ci0 + 8 * part);
-            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                s_x[(8 * part + k) * kHaloMax + hp] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+            *reinterpret_cast<uint4*>(s_x + hp * 32 + 8 * part) = v;
         }
         __syncthreads();
+        // output-gradient operands: channel co0 + 16 cb + ln at pixels 32 kh + 8 g + 0..7
+        bf16x8 A[2][2];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int ot = wave + 4 * q;
-            if (ot >= NO) break;  // uniform per wave
-            const int tap = ot / (2 * (CI_T / 16)), rem = ot - tap * (2 * (CI_T / 16));
-            const int cb = rem / (CI_T / 16), ib = rem - cb * (CI_T / 16);
-            const int hy = tap / 3, hx = tap % 3;
+        for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int pb = 16 * k + 4 * kq;  // this lane's pixels pb .. pb + 3
-                const bf16x4 av =
-                    __builtin_bit_cast(bf16x4, *reinterpret_cast<const uint2*>(s_d + (cb * 16 + ln) * 64 + pb));
-                bf16x4 bv;
+            for (int kh = 0; kh < 2; ++kh) {
+                const int p = 32 * kh + 8 * g + lj;
+                A[cb][kh] = lds_tr_pair(s_d + p * 32 + 16 * cb + 4 * la, s_d + (p + 4) * 32 + 16 * cb + 4 * la);
+            }
+        if (bias) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int pr = pb + r;
-                    bv[r] = (short)s_x[(ib * 16 + ln) * kHaloMax + ((pr >> sh) + hy) * HW2 + (pr & (TW - 1)) + hx];
-                }
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int kh = 0; kh < 2; ++kh)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) bsum[cb] += (float)A[cb][kh][q];
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int tap = tapb + 3 * u, hy = tap / 3, hx = tap - 3 * hy;
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+                const int p0 = 32 * kh + 8 * g + lj, p1 = p0 + 4;
+                const int h0 = ((p0 >> sh) + hy) * HW2 + (p0 & (TW - 1)) + hx;
+                const int h1 = ((p1 >> sh) + hy) * HW2 + (p1 & (TW - 1)) + hx;
+                const bf16x8 bv = lds_tr_pair(s_x + h0 * 32 + 16 * ib + 4 * la, s_x + h1 * 32 + 16 * ib + 4 * la);
+                acc[u][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][kh], bv, acc[u][0], 0, 0, 0);
+                acc[u][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][kh], bv, acc[u][1], 0, 0, 0);
             }
         }
     }
+    // accumulator lane l: co = co0 + 16 cb + 4 (l / 16) + r, ci = ci0 + 16 ib + l % 16
+    if (ci0 + 16 * ib < a.cin)  // a last block of 16 input channels is padded with zeros
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int ot = wave + 4 * q;
-        if (ot >= NO) break;
-        const int tap = ot / (2 * (CI_T / 16)), rem = ot - tap * (2 * (CI_T / 16));
-        const int cb = rem / (CI_T / 16), ib = rem - cb * (CI_T / 16);
-        float* dst = a.partial + ((size_t)blockIdx.x * 9 + tap) * a.cout * a.cin;
+    for (int u = 0; u < 3; ++u) {
+        float* dst = a.partial + ((size_t)blockIdx.x * 9 + tapb + 3 * u) * a.cout * a.cin;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            dst[(size_t)(co0 + cb * 16 + 4 * kq + r) * a.cin + ci0 + ib * 16 + ln] = acc[q][r];
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                dst[(size_t)(co0 + 16 * cb + 4 * g + r) * a.cin + ci0 + 16 * ib + ln] = acc[u][cb][r];
+    }
+    if (bias) {
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            float v = bsum[cb];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (g == 0) a.bpartial[(size_t)blockIdx.x * a.cout + co0 + 16 * cb + ln] = v;
+        }
     }
 }
 
-// out[i] = sum_{c < chunks} partial[c n + i]: four interleaved running sums
-// (chunks c = 0, 1, 2, 3 mod 4) combined in a fixed order, so the result does
-// not depend on the launch and several loads are in flight per thread.
-__global__ void reduce_chunks_kernel(const float* partial, long long n, int chunks, float* out)
+// out[i] = sum_{c < chunks} partial[c n + i] in a fixed order: a workgroup of
+// 16 waves owns 64 consecutive outputs; wave w sums the chunks c = w, w + 16,
+// ... with two interleaved running sums, and the 16 wave sums are then added
+// in wave order, so the result does not depend on the launch.
+constexpr int kRedWaves = 16;
+
+__global__ __launch_bounds__(64 * kRedWaves) void reduce_chunks_kernel(const float* partial, long long n, int chunks,
+                                                                        float* out)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int c = 0;
-    for (; c + 4 <= chunks; c += 4) {
-        s0 += partial[(size_t)c * n + i];
-        s1 += partial[(size_t)(c + 1) * n + i];
-        s2 += partial[(size_t)(c + 2) * n + i];
-        s3 += partial[(size_t)(c + 3) * n + i];
+    __shared__ float red[kRedWaves][64];
+    const int o = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long i = (long long)blockIdx.x * 64 + o;
+    float s0 = 0.f, s1 = 0.f;
+    if (i < n) {
+        int c = w;
+        for (; c + kRedWaves < chunks; c += 2 * kRedWaves) {
+            s0 += partial[(size_t)c * n + i];
+            s1 += partial[(size_t)(c + kRedWaves) * n + i];
+        }
+        if (c < chunks) s0 += partial[(size_t)c * n + i];
     }
-    if (c < chunks) s0 += partial[(size_t)c * n + i];
-    if (c + 1 < chunks) s1 += partial[(size_t)(c + 1) * n + i];
-    if (c + 2 < chunks) s2 += partial[(size_t)(c + 2) * n + i];
-    out[i] = (s0 + s1) + (s2 + s3);
+    red[w][o] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && i < n) {
+        float tot = 0.f;
+#pragma unroll
+        for (int k = 0; k < kRedWaves; ++k) tot += red[k][o];
+        out[i] = tot;
+    }
 }
 
 // partial[chunk][c] = sum over the chunk's kBiasChunk pixels of d[p, c].
@@ -828,21 +978,21 @@ struct wdg_cnn_s {
 
 namespace {
 
-// The tiled weight-gradient kernel covers 3x3 layers with 32k output and 16k
-// input channels on power-of-two images.
-bool wgrad_tiled_ok(const Layer& L, int H, int in_ld, int in_off)
+// The transposing-load weight gradient covers 3x3 layers with 32k output and
+// 16k input channels on power-of-two images.
+bool wgrad_tr_ok(const Layer& L, int H, int in_ld, int in_off)
 {
     return L.ks == 3 && H > 0 && (H & (H - 1)) == 0 && L.cout % 32 == 0 && L.cin % 16 == 0 && in_ld % 8 == 0 &&
            in_off % 8 == 0;
 }
 
-// Tiles per chunk of the tiled weight gradient: enough chunks for about 512
+// Tiles per chunk of the transposing-load weight gradient: enough chunks for about 512
 // workgroups, at most 16 tiles each.  Returns the chunk count.
 int wgrad_tiled_chunks(const Layer& L, int B, int H, int* tiles_per_chunk)
 {
     const int TW = std::min(H, 64), TH = 64 / TW;
     const long long tiles = (long long)B * ((H + TH - 1) / TH) * (H / TW);
-    const long long per = (long long)(L.cout / 32) * (L.cin / (L.cin % 32 == 0 ? 32 : 16));
+    const long long per = (long long)(L.cout / 32) * ((L.cin + 31) / 32);
     const int tpc = (int)std::max<long long>(1, std::min<long long>(16, tiles * per / 512));
     if (tiles_per_chunk) *tiles_per_chunk = tpc;
     return (int)((tiles + tpc - 1) / tpc);
@@ -861,6 +1011,20 @@ void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_l
         const long long tiles = (long long)B * ((H + TH - 1) / TH) * (W / TW);
         const int TA = cout % 64 == 0 ? 2 : 1;
         const dim3 grid((unsigned)tiles, (unsigned)(cout / (32 * TA)));
+        if (cin % 32 == 0 || cin > 32) {  // 32-channel slices; a last slice of 16 is zero-padded
+            if (TA == 2) {
+                if (f32out)
+                    hipLaunchKernelGGL((conv3_k32_kernel<2, true>), grid, dim3(256), 0, s, a);
+                else
+                    hipLaunchKernelGGL((conv3_k32_kernel<2, false>), grid, dim3(256), 0, s, a);
+            } else {
+                if (f32out)
+                    hipLaunchKernelGGL((conv3_k32_kernel<1, true>), grid, dim3(256), 0, s, a);
+                else
+                    hipLaunchKernelGGL((conv3_k32_kernel<1, false>), grid, dim3(256), 0, s, a);
+            }
+            return;
+        }
         if (TA == 2) {
             if (f32out)
                 hipLaunchKernelGGL((conv3_tiled_kernel<2, true>), grid, dim3(256), 0, s, a);
@@ -915,24 +1079,29 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int chunks = (int)((npix + kChunk - 1) / kChunk);
     const int ntap = L.ks * L.ks;
     const long long nw = (long long)ntap * L.cout * L.cin;
-    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1};
-    int wchunks = chunks;
-    if (wgrad_tiled_ok(L, H, in_ld, in_off)) {
-        wchunks = wgrad_tiled_chunks(L, B, H, &wa.tiles_per_chunk);
-        const int cit = L.cin % 32 == 0 ? 32 : 16;
-        const dim3 grid(wchunks, L.cout / 32, L.cin / cit);
-        if (cit == 32)
-            hipLaunchKernelGGL(wgrad3_tiled_kernel<32>, grid, dim3(256), 0, s, wa);
-        else
-            hipLaunchKernelGGL(wgrad3_tiled_kernel<16>, grid, dim3(256), 0, s, wa);
-    } else {
-        const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
-        hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
+    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1, nullptr};
+    const dim3 rblk(64 * kRedWaves);
+    if (wgrad_tr_ok(L, H, in_ld, in_off)) {
+        const int wchunks = wgrad_tiled_chunks(L, B, H, &wa.tiles_per_chunk);
+        wa.bpartial = n.partial + (size_t)wchunks * nw;
+        hipLaunchKernelGGL(wgrad3_tr_kernel, dim3(wchunks, L.cout / 32, (L.cin + 31) / 32), dim3(64 * kWgWaves), 0, s,
+                           wa);
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw, 64)), rblk, 0, s, n.partial, nw, wchunks,
+                           n.g + L.w);
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout, 64)), rblk, 0, s, wa.bpartial,
+                           (long long)L.cout, wchunks, n.g + L.b);
+        if (gin)
+            launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H,
+                        L.ks, false);
+        return;
     }
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.partial, nw, wchunks, n.g + L.w);
+    const int wchunks = chunks;
+    const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
+    hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw, 64)), rblk, 0, s, n.partial, nw, wchunks, n.g + L.w);
     const int bchunks = (int)((npix + kBiasChunk - 1) / kBiasChunk);
     hipLaunchKernelGGL(bias_partial_kernel, dim3(bchunks), dim3(256), 0, s, dpre, L.cout, (int)npix, n.partial);
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout)), dim3(256), 0, s, n.partial, (long long)L.cout,
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout, 64)), rblk, 0, s, n.partial, (long long)L.cout,
                        bchunks, n.g + L.b);
     if (gin)
         launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H, L.ks,
@@ -1128,7 +1297,7 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
             const long long chunks =
                 std::max<long long>(((long long)max_batch * H * H + kChunk - 1) / kChunk, tchunks);
             const long long bchunks = ((long long)max_batch * H * H + kBiasChunk - 1) / kBiasChunk;
-            maxpart = std::max({maxpart, chunks * nw, bchunks * L.cout});
+            maxpart = std::max({maxpart, chunks * nw, tchunks * (nw + L.cout), bchunks * L.cout});
         }
         // the loss partials (two doubles per sample and chunk) reuse the buffer
         const long long loss_chunks = ((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk;
