@@ -49,6 +49,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;     // waves per workgroup of the conv kernels
+constexpr int NLAYER_MAX = 15;  // layers of the network (the Layer enum below)
 constexpr int kChunk = 1024;  // pixels per weight-gradient partial
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -286,73 +287,106 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(ConvArgs a)
 // kernel.  A lane l holds channel co + l % 16 and input channels
 // 8 (l / 16) .. + 7 of the weight row; B lane l pixel l % 16 and the same 8
 // input channels; the accumulator layout is that of conv3_tiled_kernel.
-template <int TA, bool F32OUT>
+// RT > 1 (images 64 or more pixels wide) makes the tile RT rows of 64 pixels:
+// fewer halo rows per pixel and the slice's weights staged once for RT times
+// the pixels.
+template <int TA, int RT, bool F32OUT>
 __global__ __launch_bounds__(256) void conv3_k32_kernel(ConvArgs a)
 {
     constexpr int CO_T = 32 * TA;
-    __shared__ __align__(16) uint16_t s_in[kHaloMax * 32];
+    constexpr int HALO = RT == 1 ? kHaloMax : (RT + 2) * 66;  // halo pixels at most
+    constexpr int NHP = (HALO * 4 + 255) / 256;                // 16-byte halo parts per thread
+    constexpr int NWP = (9 * CO_T * 4 + 255) / 256;           // 16-byte weight parts per thread
+    __shared__ __align__(16) uint16_t s_in[HALO * 32];
     __shared__ __align__(16) uint16_t s_w[9 * CO_T * 32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ln = lane & 15, kq = lane >> 4;
-    const int TW = min(a.W, 64), TH = 64 / TW;
+    const int TW = min(a.W, 64), TH = 64 / TW * RT;
     const int tiles_x = a.W / TW, tiles_y = (a.H + TH - 1) / TH;
     const int t = blockIdx.x;
     const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
     const int y0 = ty * TH, x0 = tx * TW;
     const int co0 = blockIdx.y * CO_T;
     const int cw = (wave & 1) * (16 * TA);
-    const int pw = (wave >> 1) * 32;
+    const int pw = (wave >> 1) * 32 * RT;
     const int HW2 = TW + 2, NH = (TH + 2) * HW2;
-    int py[2], px[2];
+    int py[2 * RT], px[2 * RT];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < 2 * RT; ++j) {
         const int p = pw + 16 * j + ln;
         py[j] = p / TW;
         px[j] = p - py[j] * TW;
     }
-    f32x4 acc[TA][2];
+    // The 16-byte parts this thread stages: part q = tid + 256 k goes to LDS
+    // element 8 q (halo pixel q / 4 or weight row q / 4, channels 8 (q % 4) ..);
+    // its global offset without the slice's first channel, -1 for a zero
+    // (outside the image), -2 for no part.
+    const int part = tid & 3;
+    long long hoff[NHP], woff[NWP];
+#pragma unroll
+    for (int k = 0; k < NHP; ++k) {
+        const int q = tid + 256 * k, hp = q >> 2;
+        const int hy = hp / HW2, hx = hp - hy * HW2;
+        const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+        hoff[k] = q >= NH * 4 ? -2
+                  : (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                      ? ((long long)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off + 8 * part
+                      : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < NWP; ++k) {
+        const int q = tid + 256 * k, row = q >> 2;
+        const int tap = row / CO_T, co = row - tap * CO_T;
+        woff[k] = q >= 9 * CO_T * 4 ? -2 : ((long long)tap * a.cout + co0 + co) * a.cin + 8 * part;
+    }
+    uint4 hreg[NHP], wreg[NWP];
+    // global loads of the slice starting at channel c0 into registers
+    auto load = [&](int c0) {
+        const bool in_range = c0 + 8 * part < a.cin;  // a last slice of 16 channels is padded with zeros
+#pragma unroll
+        for (int k = 0; k < NHP; ++k) {
+            hreg[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (hoff[k] >= 0 && in_range) hreg[k] = *reinterpret_cast<const uint4*>(a.in + hoff[k] + c0);
+        }
+#pragma unroll
+        for (int k = 0; k < NWP; ++k) {
+            wreg[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (woff[k] >= 0 && in_range) wreg[k] = *reinterpret_cast<const uint4*>(a.w + woff[k] + c0);
+        }
+    };
+    f32x4 acc[TA][2 * RT];
 #pragma unroll
     for (int i = 0; i < TA; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < 2 * RT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    load(0);
     for (int c0 = 0; c0 < a.cin; c0 += 32) {
         __syncthreads();  // the previous slice is consumed
-        for (int q = tid; q < NH * 4; q += 256) {
-            const int hp = q >> 2, part = q & 3;
-            const int hy = hp / HW2, hx = hp - hy * HW2;
-            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W && c0 + 8 * part < a.cin)
-                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
-                                                    c0 + 8 * part);
-            *reinterpret_cast<uint4*>(s_in + hp * 32 + 8 * part) = v;
-        }
-        for (int q = tid; q < 9 * CO_T * 4; q += 256) {
-            const int row = q >> 2, part = q & 3;
-            const int tap = row / CO_T, co = row - tap * CO_T;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (c0 + 8 * part < a.cin)
-                v = *reinterpret_cast<const uint4*>(a.w + ((size_t)tap * a.cout + co0 + co) * a.cin + c0 + 8 * part);
-            *reinterpret_cast<uint4*>(s_w + row * 32 + 8 * part) = v;
-        }
+#pragma unroll
+        for (int k = 0; k < NHP; ++k)
+            if (hoff[k] != -2) *reinterpret_cast<uint4*>(s_in + 8 * (tid + 256 * k)) = hreg[k];
+#pragma unroll
+        for (int k = 0; k < NWP; ++k)
+            if (woff[k] != -2) *reinterpret_cast<uint4*>(s_w + 8 * (tid + 256 * k)) = wreg[k];
         __syncthreads();
+        if (c0 + 32 < a.cin) load(c0 + 32);  // in flight during this slice's MFMAs
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int hy = tap / 3, hx = tap % 3;
-            bf16x8 av[TA], bv[2];
+            bf16x8 av[TA], bv[2 * RT];
 #pragma unroll
             for (int i = 0; i < TA; ++i)
                 av[i] = __builtin_bit_cast(
                     bf16x8, *reinterpret_cast<const uint4*>(s_w + (tap * CO_T + cw + 16 * i + ln) * 32 + 8 * kq));
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 2 * RT; ++j)
                 bv[j] = __builtin_bit_cast(
                     bf16x8, *reinterpret_cast<const uint4*>(s_in + ((py[j] + hy) * HW2 + px[j] + hx) * 32 + 8 * kq));
 #pragma unroll
             for (int i = 0; i < TA; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < 2 * RT; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
     }
@@ -365,7 +399,7 @@ __global__ __launch_bounds__(256) void conv3_k32_kernel(ConvArgs a)
             bias[0] = b4.x, bias[1] = b4.y, bias[2] = b4.z, bias[3] = b4.w;
         }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 2 * RT; ++j) {
             const int y = y0 + py[j], x = x0 + px[j];
             if (y >= a.H) continue;
             float v[4];
@@ -398,7 +432,6 @@ struct WgradArgs {
     int dout_ld, cout;
     int B, H, W, ks;
     int tiles_per_chunk;  // wgrad3_tr_kernel
-    float* bpartial;      // wgrad3_tr_kernel: bias partial sums [chunk][co]
 };
 
 __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
@@ -472,9 +505,11 @@ __global__ __launch_bounds__(64 * kWaves) void wgrad_kernel(WgradArgs a)
 // 4a .. 4a + 3 of pixel p + j, lane i receives channel i at pixels p .. p + 3;
 // two such reads fill one v_mfma_f32_16x16x32_bf16 operand.  Wave w owns the
 // input-channel block w % 2 and the taps w / 2, w / 2 + 3, w / 2 + 6 for both
-// output-channel blocks; its partial sums go to partial[chunk][tap][co][ci].
-// Wave 0 of the first input-channel block also sums the output gradient over
-// the chunk's pixels (the bias gradient) into bpartial[chunk][co].
+// output-channel blocks.  Wave 0 of the first input-channel block also sums
+// the output gradient over the chunk's pixels (the bias gradient).  A chunk's
+// partial sums form one row of 9 cout cin + cout floats, [tap][co][ci] then
+// [co], the layout of the layer's weights and bias in the parameter vector,
+// so one reduction over the rows gives both gradients.
 constexpr int kWgWaves = 6;
 
 __device__ __forceinline__ bf16x8 lds_tr_pair(const uint16_t* p0, const uint16_t* p1)
@@ -505,30 +540,46 @@ __global__ __launch_bounds__(64 * kWgWaves) void wgrad3_tr_kernel(WgradArgs a)
 #pragma unroll
     for (int u = 0; u < 3; ++u) acc[u][0] = acc[u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum[2] = {0.f, 0.f};
-    for (int t = t0; t < t1; ++t) {
+    // the 16-byte parts this thread stages per tile: one of the output
+    // gradient (threads < 256: pixel tid / 4) and up to NXP of the halo (part
+    // q = tid + 384 k: halo pixel q / 4), all channels 8 (tid % 4) ..
+    constexpr int NXP = (kHaloMax * 4 + 64 * kWgWaves - 1) / (64 * kWgWaves);
+    const int part = tid & 3;
+    const bool ci_in = ci0 + 8 * part < a.cin;
+    uint4 dreg, xreg[NXP];
+    auto load = [&](int t) {
         const int b = t / (tiles_x * tiles_y), ty = (t / tiles_x) % tiles_y, tx = t % tiles_x;
         const int y0 = ty * TH, x0 = tx * TW;
-        __syncthreads();  // the previous tile is consumed
+        dreg = make_uint4(0u, 0u, 0u, 0u);
         if (tid < 256) {
-            const int pp = tid >> 2, part = tid & 3;
+            const int pp = tid >> 2;
             const int y = y0 + (pp >> sh), x = x0 + (pp & (TW - 1));
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (y < a.H)
-                v = *reinterpret_cast<const uint4*>(a.dout + ((size_t)(b * a.H + y) * a.W + x) * a.dout_ld + co0 +
-                                                    8 * part);
-            *reinterpret_cast<uint4*>(s_d + pp * 32 + 8 * part) = v;
+                dreg = *reinterpret_cast<const uint4*>(a.dout + ((size_t)(b * a.H + y) * a.W + x) * a.dout_ld + co0 +
+                                                       8 * part);
         }
-        for (int q = tid; q < NH * 4; q += 64 * kWgWaves) {
-            const int hp = q >> 2, part = q & 3;
+#pragma unroll
+        for (int k = 0; k < NXP; ++k) {
+            const int q = tid + 64 * kWgWaves * k, hp = q >> 2;
             const int hy = hp / HW2, hx = hp - hy * HW2;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W && ci0 + 8 * part < a.cin)
-                v = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld + a.in_off +
-                                                    ci0 + 8 * part);
-            *reinterpret_cast<uint4*>(s_x + hp * 32 + 8 * part) = v;
+            xreg[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (q < NH * 4 && y >= 0 && y < a.H && x >= 0 && x < a.W && ci_in)
+                xreg[k] = *reinterpret_cast<const uint4*>(a.in + ((size_t)(b * a.H + y) * a.W + x) * a.in_ld +
+                                                          a.in_off + ci0 + 8 * part);
+        }
+    };
+    if (t0 < t1) load(t0);
+    for (int t = t0; t < t1; ++t) {
+        __syncthreads();  // the previous tile is consumed
+        if (tid < 256) *reinterpret_cast<uint4*>(s_d + 8 * tid) = dreg;
+#pragma unroll
+        for (int k = 0; k < NXP; ++k) {
+            const int q = tid + 64 * kWgWaves * k;
+            if (q < NH * 4) *reinterpret_cast<uint4*>(s_x + 8 * q) = xreg[k];
         }
         __syncthreads();
+        if (t + 1 < t1) load(t + 1);  // in flight during this tile's MFMAs
         // output-gradient operands: channel co0 + 16 cb + ln at pixels 32 kh + 8 g + 0..7
         bf16x8 A[2][2];
 #pragma unroll
@@ -561,10 +612,12 @@ __global__ __launch_bounds__(64 * kWgWaves) void wgrad3_tr_kernel(WgradArgs a)
         }
     }
     // accumulator lane l: co = co0 + 16 cb + 4 (l / 16) + r, ci = ci0 + 16 ib + l % 16
+    const size_t row = (size_t)9 * a.cout * a.cin + a.cout;
+    float* const prow = a.partial + (size_t)blockIdx.x * row;
     if (ci0 + 16 * ib < a.cin)  // a last block of 16 input channels is padded with zeros
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
-        float* dst = a.partial + ((size_t)blockIdx.x * 9 + tapb + 3 * u) * a.cout * a.cin;
+        float* dst = prow + (size_t)(tapb + 3 * u) * a.cout * a.cin;
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
@@ -577,7 +630,7 @@ __global__ __launch_bounds__(64 * kWgWaves) void wgrad3_tr_kernel(WgradArgs a)
             float v = bsum[cb];
             v += __shfl_xor(v, 16);
             v += __shfl_xor(v, 32);
-            if (g == 0) a.bpartial[(size_t)blockIdx.x * a.cout + co0 + 16 * cb + ln] = v;
+            if (g == 0) prow[(size_t)9 * a.cout * a.cin + co0 + 16 * cb + ln] = v;
         }
     }
 }
@@ -670,6 +723,23 @@ __global__ void upsample_kernel(const uint16_t* in, uint16_t* out, int out_ld, i
     out[(size_t)p * out_ld + out_off + c] = in[q * C + c];
 }
 
+// The same with 8 channels per thread (C, out_ld and out_off multiples of 8):
+// one 16-byte load and store per thread.
+__global__ void upsample8_kernel(const uint16_t* in, uint16_t* out, int out_ld, int out_off, int B, int H, int W,
+                                 int C)
+{
+    const int C8 = C >> 3;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)B * H * W * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const long long p = i / C8;
+    const int x = (int)(p % W);
+    const long long t = p / W;
+    const int y = (int)(t % H), b = (int)(t / H);
+    const size_t q = ((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2;
+    *reinterpret_cast<uint4*>(out + (size_t)p * out_ld + out_off + c) = *reinterpret_cast<const uint4*>(in + q * C + c);
+}
+
 // ReLU backward with the incoming gradient assembled from its sources:
 //   g = g1[p, off1 + c]                        (up = 0), or
 //   g = sum of g1 over the 2 x 2 children of p (up = 1, g1 at 2H x 2W);
@@ -705,6 +775,56 @@ __global__ void relu_grad_kernel(PrepArgs a)
     }
     const bool on = bf2f(a.act[(size_t)p * a.act_ld + a.act_off + c]) > 0.f;
     a.out[i] = on ? f2bf(g) : (uint16_t)0;
+}
+
+// relu_grad_kernel with 8 channels per thread (C and every leading dimension
+// and offset multiples of 8): 16- and 32-byte accesses, the same arithmetic
+// in the same order per element.
+__global__ void relu_grad8_kernel(PrepArgs a)
+{
+    const int C8 = a.C >> 3;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)a.B * a.H * a.W * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const long long p = i / C8;
+    const int x = (int)(p % a.W);
+    const long long t = p / a.W;
+    const int y = (int)(t % a.H), b = (int)(t / a.H);
+    auto load8 = [](const float* src, float* d) {
+        const float4 u = *reinterpret_cast<const float4*>(src), v = *reinterpret_cast<const float4*>(src + 4);
+        d[0] = u.x, d[1] = u.y, d[2] = u.z, d[3] = u.w, d[4] = v.x, d[5] = v.y, d[6] = v.z, d[7] = v.w;
+    };
+    float g[8];
+    if (a.up) {
+        const size_t q = ((size_t)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x;
+        const size_t s0 = q * a.g1_ld + a.g1_off + c, s1 = (q + 2 * a.W) * a.g1_ld + a.g1_off + c;
+        float g00[8], g01[8], g10[8], g11[8];
+        load8(a.g1 + s0, g00);
+        load8(a.g1 + s0 + a.g1_ld, g01);
+        load8(a.g1 + s1, g10);
+        load8(a.g1 + s1 + a.g1_ld, g11);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] = (g00[k] + g01[k]) + (g10[k] + g11[k]);
+    } else {
+        load8(a.g1 + (size_t)p * a.g1_ld + a.g1_off + c, g);
+    }
+    if (a.g2) {
+        const size_t q = ((size_t)b * (a.H / 2) + y / 2) * (a.W / 2) + x / 2;
+        float h[8];
+        load8(a.g2 + q * a.g2_ld + c, h);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] += 0.25f * h[k];
+    }
+    const uint4 av = *reinterpret_cast<const uint4*>(a.act + (size_t)p * a.act_ld + a.act_off + c);
+    const uint32_t aw[4] = {av.x, av.y, av.z, av.w};
+    uint32_t ow[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint16_t lo = bf2f((uint16_t)(aw[k] & 0xffffu)) > 0.f ? f2bf(g[2 * k]) : (uint16_t)0;
+        const uint16_t hi = bf2f((uint16_t)(aw[k] >> 16)) > 0.f ? f2bf(g[2 * k + 1]) : (uint16_t)0;
+        ow[k] = (uint32_t)lo | ((uint32_t)hi << 16);
+    }
+    *reinterpret_cast<uint4*>(a.out + (size_t)p * a.C + c) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
 }
 
 // Per-sample relative squared error ||y - t||^2 / ||t||^2 over the first
@@ -743,18 +863,33 @@ __global__ __launch_bounds__(256) void loss_partial_kernel(const float* y, const
     }
 }
 
-__global__ void loss_final_kernel(const double* partial, int chunks, int B, float* loss, float* scale)
+// One workgroup per sample: thread k sums the chunks k, k + 256, ... in order,
+// then a fixed-shape tree adds the 256 thread sums.
+__global__ __launch_bounds__(256) void loss_final_kernel(const double* partial, int chunks, int B, float* loss,
+                                                         float* scale)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
+    __shared__ double red[2][256];
+    const int b = blockIdx.x, k = threadIdx.x;
     double tt = 0.0, ee = 0.0;
-    for (int c = 0; c < chunks; ++c) {
+    for (int c = k; c < chunks; c += 256) {
         tt += partial[2 * ((size_t)b * chunks + c)];
         ee += partial[2 * ((size_t)b * chunks + c) + 1];
     }
-    const double tn = tt > 1e-300 ? tt : 1e-300;
-    loss[b] = (float)(ee / tn);
-    scale[b] = (float)(2.0 / (B * tn));
+    red[0][k] = tt;
+    red[1][k] = ee;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (k < w) {
+            red[0][k] += red[0][k + w];
+            red[1][k] += red[1][k + w];
+        }
+        __syncthreads();
+    }
+    if (k == 0) {
+        const double tn = red[0][0] > 1e-300 ? red[0][0] : 1e-300;
+        loss[b] = (float)(red[1][0] / tn);
+        scale[b] = (float)(2.0 / (B * tn));
+    }
 }
 
 __global__ void loss_grad_kernel(const float* y, const float* t, int ld, int creal, int n, int B, const float* scale,
@@ -790,16 +925,28 @@ __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long long n)
 }
 
 // Backward-data weights: wb[t][ci][co] = bf16(w[ntap-1-t][co][ci]).
-__global__ void wflip_kernel(const float* w, int cout, int cin, int ntap, uint16_t* wb)
+// Every layer's flipped weights in one launch: element i of the flipped
+// array belongs to the first layer l with i < end[l].
+struct FlipTable {
+    long long w[NLAYER_MAX], wb[NLAYER_MAX], end[NLAYER_MAX];
+    int cout[NLAYER_MAX], cin[NLAYER_MAX], ntap[NLAYER_MAX];
+    int nl;
+};
+
+__global__ void wflip_all_kernel(const float* w, uint16_t* wb, FlipTable t)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long n = (long long)ntap * cout * cin;
-    if (i >= n) return;
-    const int co = (int)(i % cout);
-    const long long r = i / cout;
-    const int ci = (int)(r % cin), t = (int)(r / cin);
-    wb[i] = f2bf(w[((long long)(ntap - 1 - t) * cout + co) * cin + ci]);
+    if (i >= t.end[t.nl - 1]) return;
+    int l = 0;
+    while (i >= t.end[l]) ++l;
+    const long long j = i - t.wb[l];
+    const int cout = t.cout[l], cin = t.cin[l], ntap = t.ntap[l];
+    const int co = (int)(j % cout);
+    const long long r = j / cout;
+    const int ci = (int)(r % cin), tp = (int)(r / cin);
+    wb[i] = f2bf(w[t.w[l] + ((long long)(ntap - 1 - tp) * cout + co) * cin + ci]);
 }
+
 
 // ------------------------------------------------------- DG <-> image maps
 // Bessel functions J0 and Y0 of the first and second kind, order 0, from the
@@ -938,6 +1085,7 @@ struct Layer {
 
 // Layer indices in the parameter vector.
 enum { E1A, E1B, E2A, E2B, E3A, E3B, BA, BB, D3A, D3B, D2A, D2B, D1A, D1B, OUT, NLAYER };
+static_assert(NLAYER == NLAYER_MAX, "FlipTable holds every layer");
 
 }  // namespace
 
@@ -1012,18 +1160,21 @@ void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_l
         const int TA = cout % 64 == 0 ? 2 : 1;
         const dim3 grid((unsigned)tiles, (unsigned)(cout / (32 * TA)));
         if (cin % 32 == 0 || cin > 32) {  // 32-channel slices; a last slice of 16 is zero-padded
-            if (TA == 2) {
-                if (f32out)
-                    hipLaunchKernelGGL((conv3_k32_kernel<2, true>), grid, dim3(256), 0, s, a);
-                else
-                    hipLaunchKernelGGL((conv3_k32_kernel<2, false>), grid, dim3(256), 0, s, a);
-            } else {
-                if (f32out)
-                    hipLaunchKernelGGL((conv3_k32_kernel<1, true>), grid, dim3(256), 0, s, a);
-                else
-                    hipLaunchKernelGGL((conv3_k32_kernel<1, false>), grid, dim3(256), 0, s, a);
-            }
-            return;
+            // taller tiles on images 64 or more pixels wide while about 512 workgroups remain
+            int RT = 1;
+            while (RT < 4 && W >= 64 && H % (2 * RT) == 0 && tiles / (2 * RT) * (cout / (32 * TA)) >= 512) RT *= 2;
+            const dim3 g2((unsigned)(tiles / RT), (unsigned)(cout / (32 * TA)));
+#define WDG_K32_CASE(ta, rt)                                                                       \
+    if (TA == ta && RT == rt) {                                                                    \
+        if (f32out)                                                                                \
+            hipLaunchKernelGGL((conv3_k32_kernel<ta, rt, true>), g2, dim3(256), 0, s, a);          \
+        else                                                                                       \
+            hipLaunchKernelGGL((conv3_k32_kernel<ta, rt, false>), g2, dim3(256), 0, s, a);         \
+        return;                                                                                    \
+    }
+            WDG_K32_CASE(1, 1) WDG_K32_CASE(1, 2) WDG_K32_CASE(1, 4)
+            WDG_K32_CASE(2, 1) WDG_K32_CASE(2, 2) WDG_K32_CASE(2, 4)
+#undef WDG_K32_CASE
         }
         if (TA == 2) {
             if (f32out)
@@ -1079,17 +1230,15 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int chunks = (int)((npix + kChunk - 1) / kChunk);
     const int ntap = L.ks * L.ks;
     const long long nw = (long long)ntap * L.cout * L.cin;
-    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1, nullptr};
+    WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1};
     const dim3 rblk(64 * kRedWaves);
     if (wgrad_tr_ok(L, H, in_ld, in_off)) {
         const int wchunks = wgrad_tiled_chunks(L, B, H, &wa.tiles_per_chunk);
-        wa.bpartial = n.partial + (size_t)wchunks * nw;
         hipLaunchKernelGGL(wgrad3_tr_kernel, dim3(wchunks, L.cout / 32, (L.cin + 31) / 32), dim3(64 * kWgWaves), 0, s,
                            wa);
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw, 64)), rblk, 0, s, n.partial, nw, wchunks,
-                           n.g + L.w);
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout, 64)), rblk, 0, s, wa.bpartial,
-                           (long long)L.cout, wchunks, n.g + L.b);
+        // rows of [weights][bias]: the bias follows the weights in the parameter vector
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw + L.cout, 64)), rblk, 0, s, n.partial,
+                           nw + L.cout, wchunks, n.g + L.w);
         if (gin)
             launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H,
                         L.ks, false);
@@ -1113,7 +1262,10 @@ void relu_grad(wdg_cnn_s& n, hipStream_t s, int level, int C, const float* g1, i
 {
     const int H = n.res(level);
     PrepArgs a{g1, g2, act, n.dpre, g1_ld, g1_off, up ? 1 : 0, g2_ld, act_ld, act_off, B, H, H, C};
-    hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, a);
+    if ((C | g1_ld | g1_off | g2_ld | act_ld | act_off) % 8 == 0)
+        hipLaunchKernelGGL(relu_grad8_kernel, dim3(blocks_for((long long)B * H * H * C / 8)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, a);
 }
 
 void pool(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, int in_ld, int in_off, uint16_t* out, int C,
@@ -1127,8 +1279,12 @@ void pool(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, int in
 void upsample(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, uint16_t* out, int out_ld, int C, int B)
 {
     const int H = n.res(level_out);
-    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, in, out, out_ld, 0,
-                       B, H, H, C);
+    if ((C | out_ld) % 8 == 0)
+        hipLaunchKernelGGL(upsample8_kernel, dim3(blocks_for((long long)B * H * H * C / 8)), dim3(256), 0, s, in, out,
+                           out_ld, 0, B, H, H, C);
+    else
+        hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, in, out,
+                           out_ld, 0, B, H, H, C);
 }
 
 void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
@@ -1198,14 +1354,27 @@ void backward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
     bwd(n, s, E1A, x, n.in_ld, 0, n.dpre, nullptr, B);
 }
 
+// The flipped bf16 weights of every layer from the fp32 master weights.
+void flip_weights(wdg_cnn_s& n, hipStream_t s)
+{
+    FlipTable t{};
+    t.nl = NLAYER;
+    for (int l = 0; l < NLAYER; ++l) {
+        const Layer& L = n.L[l];
+        t.w[l] = L.w;
+        t.wb[l] = L.wb;
+        t.end[l] = L.wb + (long long)L.ks * L.ks * L.cout * L.cin;
+        t.cout[l] = L.cout;
+        t.cin[l] = L.cin;
+        t.ntap[l] = L.ks * L.ks;
+    }
+    hipLaunchKernelGGL(wflip_all_kernel, dim3(blocks_for(n.nflip)), dim3(256), 0, s, n.w, n.wflip, t);
+}
+
 void refresh_copies(wdg_cnn_s& n, hipStream_t s)
 {
     hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n.nparam)), dim3(256), 0, s, n.w, n.wbf, n.nparam);
-    for (const Layer& L : n.L) {
-        const long long nw = (long long)L.ks * L.ks * L.cout * L.cin;
-        hipLaunchKernelGGL(wflip_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n.w + L.w, L.cout, L.cin, L.ks * L.ks,
-                           n.wflip + L.wb);
-    }
+    flip_weights(n, s);
 }
 
 void loss_grad(wdg_cnn_s& n, hipStream_t s, int B, const uint16_t* x, const float* t)
@@ -1215,7 +1384,7 @@ void loss_grad(wdg_cnn_s& n, hipStream_t s, int B, const uint16_t* x, const floa
     const int chunks = (nper + kLossChunk - 1) / kLossChunk;
     double* part = reinterpret_cast<double*>(n.partial);  // free between the weight-gradient passes
     hipLaunchKernelGGL(loss_partial_kernel, dim3(chunks, B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, nper, part);
-    hipLaunchKernelGGL(loss_final_kernel, dim3((B + 63) / 64), dim3(64), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
+    hipLaunchKernelGGL(loss_final_kernel, dim3(B), dim3(256), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
     hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for((long long)B * nper)), dim3(256), 0, s, n.y, t, n.out_ld,
                        n.out_real, nper, B, n.loss + n.maxB, n.dy);
     backward(n, s, x, B);
@@ -1440,11 +1609,7 @@ int wdg_hip_cnn_train_step(wdg_cnn n, int batch, const uint16_t* d_x, const floa
         const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)n->step)));
         hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n->nparam)), dim3(256), 0, s, n->w, n->m, n->v, n->g, n->wbf,
                            n->nparam, lr, b1, b2, eps, c1, c2);
-        for (const Layer& L : n->L) {
-            const long long nw = (long long)L.ks * L.ks * L.cout * L.cin;
-            hipLaunchKernelGGL(wflip_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, n->w + L.w, L.cout, L.cin,
-                               L.ks * L.ks, n->wflip + L.wb);
-        }
+        flip_weights(*n, s);
         ck(hipGetLastError(), "train_step");
         if (loss) *loss = read_loss(*n, s, batch);
     });
