@@ -10,7 +10,8 @@
 // (3, n, n) column-major) is the pixel's channel vector.
 //
 // Numerics.  Activations and weights are bf16 (round to nearest even),
-// products run on v_mfma_f32_16x16x16bf16_1k with fp32 accumulation, the
+// products run on v_mfma_f32_16x16x32_bf16 (v_mfma_f32_16x16x16bf16_1k in
+// the kernels for 16-channel layers) with fp32 accumulation, the
 // master weights, Adam moments, gradients, bias terms, loss and the
 // output layer are fp32.  Every reduction runs in a fixed order (per-chunk
 // partials, then a sequential sum over chunks): results are bitwise
