@@ -676,14 +676,44 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 d1 = ra * pre1 + dt * r1;
                 d2 = ra * pre2 + dt * r2;
             }
-            st_stream<NT>(du + 0, d0);
-            st_stream<NT>(du + 1, d1);
-            st_stream<NT>(du + 2, d2);
             R w0, w1, w2;
-            stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
-            st_stream<NT>(uo + 0, w0);
-            st_stream<NT>(uo + 1, w1);
-            st_stream<NT>(uo + 2, w2);
+            if constexpr (NT && N == 8 && !F32) {
+                // One element per wavefront: its 64 nodes x 3 doubles are one
+                // contiguous 1536-byte block per array.  Three strided 8-byte
+                // non-temporal stores per lane leave partial lines that reach
+                // HBM as extra write traffic, so the wave stages each block in
+                // its element's state tile (free once the operator has
+                // returned: every read of it precedes the operator's last
+                // barrier) and stores it as 96 contiguous 16-byte pieces.  LDS
+                // accesses of one wavefront complete in order; the wave barrier
+                // keeps the compiler from reordering them.
+                typedef double dpair __attribute__((ext_vector_type(2)));
+                const int lane = threadIdx.x & 63;
+                R* tile = reinterpret_cast<R*>(lds + L::off_u) + (size_t)el * 3 * L::NN;
+                auto store_block = [&](R* dst, R v0, R v1, R v2) {
+                    tile[3 * lane + 0] = v0;
+                    tile[3 * lane + 1] = v1;
+                    tile[3 * lane + 2] = v2;
+                    __builtin_amdgcn_wave_barrier();
+                    const dpair* t2 = reinterpret_cast<const dpair*>(tile);
+                    dpair* d2 = reinterpret_cast<dpair*>(dst);
+                    __builtin_nontemporal_store(t2[lane], d2 + lane);
+                    if (lane < 32) __builtin_nontemporal_store(t2[64 + lane], d2 + 64 + lane);
+                    __builtin_amdgcn_wave_barrier();
+                };
+                const size_t eb = gb + (size_t)el * 3 * L::NN;  // the element's first value
+                store_block(static_cast<R*>(a.du) + eb, d0, d1, d2);
+                stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
+                store_block(static_cast<R*>(a.u_out) + eb, w0, w1, w2);
+            } else {
+                st_stream<NT>(du + 0, d0);
+                st_stream<NT>(du + 1, d1);
+                st_stream<NT>(du + 2, d2);
+                stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
+                st_stream<NT>(uo + 0, w0);
+                st_stream<NT>(uo + 1, w1);
+                st_stream<NT>(uo + 2, w2);
+            }
             if constexpr (ACC) {
                 // WaveHoltz filter accumulation in double (WaveHoltz.cpp:83-87)
                 const double acw = a.ac;
