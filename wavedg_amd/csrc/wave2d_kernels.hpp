@@ -636,7 +636,16 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
         q = apply_operator<N, R, VOL, IFACE, BFACE, TGT, LEG>(static_cast<const R*>(a.u_in), a.geom, fgeo, a.fnb,
                                                          a.basis, (R)a.ha, (R)a.hb, e0, nloc, lds, el, gb, g, active,
                                                          prefetch);
-    if (!active) return;
+    // FP64 stage outputs with streaming stores at orders whose elements do not
+    // fill whole wavefronts: the workgroup's elements are one contiguous block
+    // per array, staged in the workgroup's state tile and stored as 16-byte
+    // pieces (EPB 3 n^2 even keeps every workgroup block 16-byte aligned).
+    // Every thread of the workgroup takes part in its barriers.
+    constexpr bool WG_BLOCKS = NT && !F32 && N != 8 && (OUT == M_OUT_STAGE0 || OUT == M_OUT_STAGE) &&
+                               (Shape<N, TGT>::EPB * 3 * N * N) % 2 == 0;
+    if constexpr (!WG_BLOCKS) {
+        if (!active) return;
+    }
     const size_t o = gb + (size_t)g;
     R r0 = q.r0, r1 = q.r1, r2 = q.r2;
     if constexpr (OUT == M_OUT_ACCUM) {
@@ -705,6 +714,31 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 store_block(static_cast<R*>(a.du) + eb, d0, d1, d2);
                 stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
                 store_block(static_cast<R*>(a.u_out) + eb, w0, w1, w2);
+            } else if constexpr (WG_BLOCKS) {
+                typedef double dpair __attribute__((ext_vector_type(2)));
+                R* tile = reinterpret_cast<R*>(lds + L::off_u);  // the workgroup's block, node-major
+                const int nval = nloc * 3 * L::NN;               // values of the workgroup's elements
+                auto store_block = [&](R* dst, R v0, R v1, R v2) {
+                    if (active) {
+                        tile[g + 0] = v0;
+                        tile[g + 1] = v1;
+                        tile[g + 2] = v2;
+                    }
+                    __syncthreads();
+                    const dpair* t2 = reinterpret_cast<const dpair*>(tile);
+                    dpair* d2 = reinterpret_cast<dpair*>(dst);
+                    for (int k = threadIdx.x; 2 * k < nval; k += blockDim.x) {
+                        if (2 * k + 1 < nval)
+                            __builtin_nontemporal_store(t2[k], d2 + k);
+                        else
+                            __builtin_nontemporal_store(tile[2 * k], dst + 2 * k);
+                    }
+                    __syncthreads();
+                };
+                store_block(static_cast<R*>(a.du) + gb, d0, d1, d2);
+                stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
+                store_block(static_cast<R*>(a.u_out) + gb, w0, w1, w2);
+                if (!active) return;
             } else {
                 st_stream<NT>(du + 0, d0);
                 st_stream<NT>(du + 1, d1);
