@@ -4,9 +4,11 @@ stage outputs then leave through LDS: at n = 8 each wavefront stages its
 element's block, at other orders the workgroup stages its block when
 EPB * 3 n^2 is even (otherwise the lanes store their own values).
 
-* RK4 steps on meshes above the threshold, at odd and even orders, under
-  every workgroup schedule (so the staged and the per-lane store paths both
-  run), against the oracle at 1e-12 relative L2;
+* RK4 steps on meshes above the threshold, at odd and even orders, on
+  Gauss-Lobatto and Gauss-Legendre bases, under every workgroup schedule (so
+  the staged and the per-lane store paths both run), against the oracle at
+  1e-12 relative L2 (the forced stage kernels run in the 384^2 L-shape
+  configuration test, also above the threshold);
 * one WaveHoltz fixed-point update (the filter-accumulating stage kernels)
   on a mesh above the threshold: bitwise equal under every schedule."""
 from __future__ import annotations
@@ -38,20 +40,21 @@ def _state(gm, n):
     return meshgen.nodal_state(nodes, meshgen.gaussian_sum(nodes, [(0.4, 0.5), (0.6, 0.55)], [0.05, 0.08], [1.0, -0.5]))
 
 
-@pytest.mark.parametrize("n,nx", [(5, 420), (6, 340), (4, 512), (7, 300)])
-def test_streaming_rk4_matches_oracle_under_every_schedule(cuda, n, nx):
+@pytest.mark.parametrize("n,nx,leg", [(5, 420, False), (6, 340, False), (4, 512, False), (7, 300, False),
+                                      (8, 256, True), (5, 420, True)])
+def test_streaming_rk4_matches_oracle_under_every_schedule(cuda, n, nx, leg):
     torch = cuda
     v, e = meshgen.perturbed_rect(nx, nx, 0.2, 900 + n)
     gm, om = capi.Mesh(v, e), O.Mesh(v, e)
     bc = np.ones(gm.n_boundary, dtype=np.int32)  # outflow
     ndof = 3 * n * n * gm.n_elem
     assert 3 * ndof * 8 > CACHE, "the mesh must be above the streaming threshold"
-    print(f"streaming stores: {nx}^2 perturbed, n={n}, ndof {ndof}, 2 RK4 steps", flush=True)
+    print(f"streaming stores: {nx}^2 perturbed, n={n}, legendre={leg}, ndof {ndof}, 2 RK4 steps", flush=True)
     u0 = _state(gm, n)
     dt = 0.5 * gm.min_h() / n**2
-    ref, _ = O.Wave(om, n, bc).rk4(u0, dt, 2)
+    ref, _ = O.Wave(om, n, bc, legendre=leg).rk4(u0, dt, 2)
     for k in range(capi.schedule_count()[0]):
-        gw = capi.Wave2D(gm, n, bc)
+        gw = capi.Wave2D(gm, n, bc, basis=capi.GAUSS_LEGENDRE if leg else capi.GAUSS_LOBATTO)
         gw.set_schedule(k)
         du = torch.tensor(u0, device="cuda")
         gw.rk4(du, dt, 2)

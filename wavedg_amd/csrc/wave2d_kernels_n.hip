@@ -10,9 +10,10 @@ namespace wdg {
 
 namespace {
 
-// XM: mode bits added to every kernel (M_LEG for Gauss-Legendre bases).
-template <typename R, int TGT, int XM = 0>
-void launch_sched(int kid, const KArgs& a, hipStream_t s)
+// XM: mode bits added to every kernel (M_LEG for Gauss-Legendre bases,
+// M_NT for non-temporal stage output stores).
+template <typename R, int TGT, int XM>
+void launch_modes(int kid, const KArgs& a, hipStream_t s)
 {
     constexpr int N = WDG_N;
     using S = Shape<N, TGT>;
@@ -33,22 +34,15 @@ void launch_sched(int kid, const KArgs& a, hipStream_t s)
     }
 }
 
-// The unforced RK stage kernels with non-temporal output stores, for
-// handles whose stage arrays exceed the Infinity Cache (KArgs::nt_store).
-template <typename R, int TGT>
-bool launch_sched_nt(int kid, const KArgs& a, hipStream_t s)
+// The RK stage kernels of one schedule; handles whose stage arrays exceed the
+// Infinity Cache (KArgs::nt_store) take the non-temporal-store variants.
+template <typename R, int TGT, int XM = 0>
+void launch_sched(int kid, const KArgs& a, hipStream_t s)
 {
-    constexpr int N = WDG_N;
-    using S = Shape<N, TGT>;
-    const dim3 grid((a.nel + S::EPB - 1) / S::EPB), block(S::NT);
-    switch (kid) {
-    case K_STAGE0: hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K_STAGE0] | M_NT, TGT>), grid, block, 0, s, a); return true;
-    case K_STAGE: hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K_STAGE] | M_NT, TGT>), grid, block, 0, s, a); return true;
-    case K_STAGE_ACC:
-        hipLaunchKernelGGL((wave_kernel<N, R, kModeOf[K_STAGE_ACC] | M_NT, TGT>), grid, block, 0, s, a);
-        return true;
-    default: return false;
-    }
+    if (a.nt_store)
+        launch_modes<R, TGT, XM | M_NT>(kid, a, s);
+    else
+        launch_modes<R, TGT, XM>(kid, a, s);
 }
 
 // Gauss-Legendre bases: the RK stage kernels run with the schedule picked
@@ -104,16 +98,6 @@ void launch_typed(int kid, int sched, const KArgs& a, hipStream_t s)
     case K_STAGE_ACC:
     case K_STAGE_F_ACC:
         // the RK stage kernels run with the schedule picked for this order
-        if (a.nt_store) {
-            bool done = false;
-            switch (sched) {
-            case 1: done = launch_sched_nt<R, kSchedTgt[1]>(kid, a, s); break;
-            case 2: done = launch_sched_nt<R, kSchedTgt[2]>(kid, a, s); break;
-            case 3: done = launch_sched_nt<R, kSchedTgt[3]>(kid, a, s); break;
-            default: done = launch_sched_nt<R, kSchedTgt[0]>(kid, a, s); break;
-            }
-            if (done) return;
-        }
         switch (sched) {
         case 1: launch_sched<R, kSchedTgt[1]>(kid, a, s); return;
         case 2: launch_sched<R, kSchedTgt[2]>(kid, a, s); return;
