@@ -639,10 +639,12 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
     // FP64 stage outputs with streaming stores at orders whose elements do not
     // fill whole wavefronts: the workgroup's elements are one contiguous block
     // per array, staged in the workgroup's state tile and stored as 16-byte
-    // pieces (EPB 3 n^2 even keeps every workgroup block 16-byte aligned).
-    // Every thread of the workgroup takes part in its barriers.
+    // pieces (EPB 3 n^2 a multiple of the values per piece keeps every
+    // workgroup block 16-byte aligned).  Every thread of the workgroup takes
+    // part in its barriers.
+    constexpr int VPR = 16 / (int)sizeof(R);  // values per 16-byte piece
     constexpr bool WG_BLOCKS = NT && !F32 && N != 8 && (OUT == M_OUT_STAGE0 || OUT == M_OUT_STAGE) &&
-                               (Shape<N, TGT>::EPB * 3 * N * N) % 2 == 0;
+                               (Shape<N, TGT>::EPB * 3 * N * N) % VPR == 0;
     if constexpr (!WG_BLOCKS) {
         if (!active) return;
     }
@@ -686,17 +688,20 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 d2 = ra * pre2 + dt * r2;
             }
             R w0, w1, w2;
+            typedef R piece_t __attribute__((ext_vector_type(VPR)));
             if constexpr (NT && N == 8 && !F32) {
                 // One element per wavefront: its 64 nodes x 3 doubles are one
-                // contiguous 1536-byte block per array.  Three strided 8-byte
-                // non-temporal stores per lane leave partial lines that reach
-                // HBM as extra write traffic, so the wave stages each block in
-                // its element's state tile (free once the operator has
-                // returned: every read of it precedes the operator's last
-                // barrier) and stores it as 96 contiguous 16-byte pieces.  LDS
-                // accesses of one wavefront complete in order; the wave barrier
-                // keeps the compiler from reordering them.
-                typedef double dpair __attribute__((ext_vector_type(2)));
+                // contiguous 1536-byte block per array.  Strided per-lane
+                // non-temporal stores leave partial lines that reach HBM as
+                // extra write traffic, so the wave stages each block in its
+                // element's state tile (free once the operator has returned:
+                // every read of it precedes the operator's last barrier) and
+                // stores it as contiguous 16-byte pieces.  LDS accesses of one
+                // wavefront complete in order; the wave barrier keeps the
+                // compiler from reordering them.  (FP32 lanes already store
+                // their 12 contiguous bytes as one access; staging them
+                // measured slower.)
+                constexpr int PIECES = 3 * 64 / VPR;
                 const int lane = threadIdx.x & 63;
                 R* tile = reinterpret_cast<R*>(lds + L::off_u) + (size_t)el * 3 * L::NN;
                 auto store_block = [&](R* dst, R v0, R v1, R v2) {
@@ -704,10 +709,10 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                     tile[3 * lane + 1] = v1;
                     tile[3 * lane + 2] = v2;
                     __builtin_amdgcn_wave_barrier();
-                    const dpair* t2 = reinterpret_cast<const dpair*>(tile);
-                    dpair* d2 = reinterpret_cast<dpair*>(dst);
-                    __builtin_nontemporal_store(t2[lane], d2 + lane);
-                    if (lane < 32) __builtin_nontemporal_store(t2[64 + lane], d2 + 64 + lane);
+                    const piece_t* t2 = reinterpret_cast<const piece_t*>(tile);
+                    piece_t* d2 = reinterpret_cast<piece_t*>(dst);
+#pragma unroll
+                    for (int k = lane; k < PIECES; k += 64) __builtin_nontemporal_store(t2[k], d2 + k);
                     __builtin_amdgcn_wave_barrier();
                 };
                 const size_t eb = gb + (size_t)el * 3 * L::NN;  // the element's first value
@@ -715,7 +720,6 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                 stage_update(q.p, q.uu, q.vv, rb * d0, rb * d1, rb * d2, (uint32_t)gb + g, a.sr_seed, w0, w1, w2);
                 store_block(static_cast<R*>(a.u_out) + eb, w0, w1, w2);
             } else if constexpr (WG_BLOCKS) {
-                typedef double dpair __attribute__((ext_vector_type(2)));
                 R* tile = reinterpret_cast<R*>(lds + L::off_u);  // the workgroup's block, node-major
                 const int nval = nloc * 3 * L::NN;               // values of the workgroup's elements
                 auto store_block = [&](R* dst, R v0, R v1, R v2) {
@@ -725,14 +729,12 @@ __global__ __launch_bounds__((Shape<N, TGT>::NT), (sizeof(R) == 4 ? WDG_MIN_WAVE
                         tile[g + 2] = v2;
                     }
                     __syncthreads();
-                    const dpair* t2 = reinterpret_cast<const dpair*>(tile);
-                    dpair* d2 = reinterpret_cast<dpair*>(dst);
-                    for (int k = threadIdx.x; 2 * k < nval; k += blockDim.x) {
-                        if (2 * k + 1 < nval)
-                            __builtin_nontemporal_store(t2[k], d2 + k);
-                        else
-                            __builtin_nontemporal_store(tile[2 * k], dst + 2 * k);
-                    }
+                    const piece_t* t2 = reinterpret_cast<const piece_t*>(tile);
+                    piece_t* d2 = reinterpret_cast<piece_t*>(dst);
+                    const int npc = nval / VPR;
+                    for (int k = threadIdx.x; k < npc; k += blockDim.x) __builtin_nontemporal_store(t2[k], d2 + k);
+                    for (int k = npc * VPR + threadIdx.x; k < nval; k += blockDim.x)
+                        __builtin_nontemporal_store(tile[k], dst + k);
                     __syncthreads();
                 };
                 store_block(static_cast<R*>(a.du) + gb, d0, d1, d2);
