@@ -636,6 +636,130 @@ __global__ __launch_bounds__(64 * kWgWaves) void wgrad3_tr_kernel(WgradArgs a)
     }
 }
 
+// Weight and bias gradients of a 1x1 layer (16k output channels, at most 256;
+// 16k input channels, at most 64) on the transposing LDS load: a workgroup of
+// cout / 16 waves takes a chunk of a.tiles_per_chunk tiles of 64 consecutive
+// pixels, staged in LDS in their NHWC order ([64][cout] and [64][cin], 16-byte
+// stores); wave w owns output-channel block w for every input block and sums
+// its output-gradient fragments for the bias.  A chunk's partial row is
+// [co][ci] then [co], as in wgrad3_tr_kernel.
+constexpr int kW1MaxIb = 4;
+
+__global__ __launch_bounds__(1024) void wgrad1_tr_kernel(WgradArgs a)
+{
+    extern __shared__ __align__(16) uint16_t smem[];
+    uint16_t* const s_d = smem;               // [64][cout]
+    uint16_t* const s_x = smem + 64 * a.cout;  // [64][cin]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, ln = lane & 15, la = ln & 3, lj = ln >> 2;
+    const int nib = a.cin / 16, nthr = blockDim.x;
+    const long long npix = (long long)a.B * a.H * a.W;
+    const long long ntiles = (npix + 63) / 64;
+    const long long t0 = (long long)blockIdx.x * a.tiles_per_chunk;
+    const long long t1 = min(ntiles, t0 + a.tiles_per_chunk);
+    const int dpp = a.cout / 8, xpp = a.cin / 8;  // 16-byte parts per pixel
+    f32x4 acc[kW1MaxIb];
+#pragma unroll
+    for (int ib = 0; ib < kW1MaxIb; ++ib) acc[ib] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (long long t = t0; t < t1; ++t) {
+        const long long p0 = t * 64;
+        __syncthreads();  // the previous tile is consumed
+        for (int q = tid; q < 64 * dpp; q += nthr) {
+            const int pp = q / dpp, part = q - pp * dpp;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (p0 + pp < npix) v = *reinterpret_cast<const uint4*>(a.dout + (p0 + pp) * a.dout_ld + 8 * part);
+            *reinterpret_cast<uint4*>(s_d + 8 * q) = v;
+        }
+        for (int q = tid; q < 64 * xpp; q += nthr) {
+            const int pp = q / xpp, part = q - pp * xpp;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (p0 + pp < npix)
+                v = *reinterpret_cast<const uint4*>(a.in + (p0 + pp) * a.in_ld + a.in_off + 8 * part);
+            *reinterpret_cast<uint4*>(s_x + 8 * q) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kh = 0; kh < 2; ++kh) {
+            const int p = 32 * kh + 8 * g + lj;
+            const bf16x8 av = lds_tr_pair(s_d + p * a.cout + 16 * wave + 4 * la,
+                                          s_d + (p + 4) * a.cout + 16 * wave + 4 * la);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bsum += (float)av[q];
+#pragma unroll
+            for (int ib = 0; ib < kW1MaxIb; ++ib) {
+                if (ib >= nib) break;  // uniform
+                const bf16x8 bv = lds_tr_pair(s_x + p * a.cin + 16 * ib + 4 * la, s_x + (p + 4) * a.cin + 16 * ib + 4 * la);
+                acc[ib] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[ib], 0, 0, 0);
+            }
+        }
+    }
+    float* const prow = a.partial + (size_t)blockIdx.x * ((size_t)a.cout * a.cin + a.cout);
+#pragma unroll
+    for (int ib = 0; ib < kW1MaxIb; ++ib) {
+        if (ib >= nib) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) prow[(size_t)(16 * wave + 4 * g + r) * a.cin + 16 * ib + ln] = acc[ib][r];
+    }
+    float v = bsum;
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (g == 0) prow[(size_t)a.cout * a.cin + 16 * wave + ln] = v;
+}
+
+// 1x1 convolution with fp32 output (the output layer, forward with bias and
+// backward-data without): out[p, c] = bias[c] + sum_k in[p, k] w[c][k] on
+// 16x16x32 MFMAs with both operands loaded straight from global memory (the
+// weights are a few KB and stay in cache).  A wave computes 16 pixels x all
+// output channels (at most 16 kC1MaxOb); slices of 32 input channels, a last
+// slice of 16 padded with zeros.
+constexpr int kC1MaxOb = 8;
+
+__global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ln = lane & 15, kq = lane >> 4;
+    const long long npix = (long long)a.B * a.H * a.W;
+    const long long pb = ((long long)blockIdx.x * 4 + wave) * 16;
+    if (pb >= npix) return;  // no barriers below
+    const long long p = pb + ln;
+    const bool pok = p < npix;
+    const int nob = a.cout / 16;
+    f32x4 acc[kC1MaxOb];
+#pragma unroll
+    for (int ob = 0; ob < kC1MaxOb; ++ob) acc[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < a.cin; k0 += 32) {
+        const int kk = k0 + 8 * kq;
+        const bool kin = kk < a.cin;
+        uint4 b = make_uint4(0u, 0u, 0u, 0u);
+        if (pok && kin) b = *reinterpret_cast<const uint4*>(a.in + p * a.in_ld + a.in_off + kk);
+        const bf16x8 bv = __builtin_bit_cast(bf16x8, b);
+#pragma unroll
+        for (int ob = 0; ob < kC1MaxOb; ++ob) {
+            if (ob >= nob) break;  // uniform
+            uint4 w = make_uint4(0u, 0u, 0u, 0u);
+            if (kin) w = *reinterpret_cast<const uint4*>(a.w + (size_t)(16 * ob + ln) * a.cin + kk);
+            acc[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w), bv, acc[ob], 0, 0, 0);
+        }
+    }
+    if (!pok) return;
+#pragma unroll
+    for (int ob = 0; ob < kC1MaxOb; ++ob) {
+        if (ob >= nob) break;
+        const int c = 16 * ob + 4 * kq;
+        float v[4] = {acc[ob][0], acc[ob][1], acc[ob][2], acc[ob][3]};
+        if (a.bias) {
+            const float4 b4 = *reinterpret_cast<const float4*>(a.bias + c);
+            v[0] += b4.x, v[1] += b4.y, v[2] += b4.z, v[3] += b4.w;
+        }
+        if (a.relu)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q], 0.f);
+        *reinterpret_cast<float4*>(static_cast<float*>(a.out) + p * a.out_ld + a.out_off + c) =
+            make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
 // out[i] = sum_{c < chunks} partial[c n + i] in a fixed order: a workgroup of
 // 16 waves owns 64 consecutive outputs; wave w sums the chunks c = w, w + 16,
 // ... with two interleaved running sums, and the 16 wave sums are then added
@@ -1135,6 +1259,24 @@ bool wgrad_tr_ok(const Layer& L, int H, int in_ld, int in_off)
            in_off % 8 == 0;
 }
 
+// The 1x1 transposing-load weight gradient: 16k output channels up to 256, 16k
+// input channels up to 64.
+bool wgrad1_tr_ok(const Layer& L, int in_ld, int in_off)
+{
+    return L.ks == 1 && L.cout % 16 == 0 && L.cout <= 256 && L.cin % 16 == 0 && L.cin <= 16 * kW1MaxIb &&
+           in_ld % 8 == 0 && in_off % 8 == 0;
+}
+
+// Chunks of 64-pixel tiles for wgrad1_tr_kernel: about 512 workgroups, at
+// most 16 tiles each.  Returns the chunk count.
+int wgrad1_chunks(int B, int H, int* tiles_per_chunk)
+{
+    const long long tiles = ((long long)B * H * H + 63) / 64;
+    const int tpc = (int)std::max<long long>(1, std::min<long long>(16, tiles / 512));
+    if (tiles_per_chunk) *tiles_per_chunk = tpc;
+    return (int)((tiles + tpc - 1) / tpc);
+}
+
 // Tiles per chunk of the transposing-load weight gradient: enough chunks for about 512
 // workgroups, at most 16 tiles each.  Returns the chunk count.
 int wgrad_tiled_chunks(const Layer& L, int B, int H, int* tiles_per_chunk)
@@ -1190,6 +1332,11 @@ void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_l
         }
         return;
     }
+    if (ks == 1 && f32out && cout % 16 == 0 && cout <= 16 * kC1MaxOb && cin % 8 == 0 && in_ld % 8 == 0 &&
+        in_off % 8 == 0 && out_ld % 4 == 0 && out_off % 4 == 0) {
+        hipLaunchKernelGGL(conv1_k32_kernel, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, a);
+        return;
+    }
     const int nb = cout / 16;
     const int TA = nb % 4 == 0 ? 4 : (nb % 2 == 0 ? 2 : 1);
     int TB = 4;
@@ -1238,6 +1385,17 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
         hipLaunchKernelGGL(wgrad3_tr_kernel, dim3(wchunks, L.cout / 32, (L.cin + 31) / 32), dim3(64 * kWgWaves), 0, s,
                            wa);
         // rows of [weights][bias]: the bias follows the weights in the parameter vector
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw + L.cout, 64)), rblk, 0, s, n.partial,
+                           nw + L.cout, wchunks, n.g + L.w);
+        if (gin)
+            launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H,
+                        L.ks, false);
+        return;
+    }
+    if (wgrad1_tr_ok(L, in_ld, in_off)) {
+        const int wchunks = wgrad1_chunks(B, H, &wa.tiles_per_chunk);
+        const size_t lds = (size_t)64 * (L.cout + L.cin) * sizeof(uint16_t);
+        hipLaunchKernelGGL(wgrad1_tr_kernel, dim3(wchunks), dim3(64 * (L.cout / 16)), lds, s, wa);
         hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw + L.cout, 64)), rblk, 0, s, n.partial,
                            nw + L.cout, wchunks, n.g + L.w);
         if (gin)
@@ -1467,7 +1625,9 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
             const long long chunks =
                 std::max<long long>(((long long)max_batch * H * H + kChunk - 1) / kChunk, tchunks);
             const long long bchunks = ((long long)max_batch * H * H + kBiasChunk - 1) / kBiasChunk;
-            maxpart = std::max({maxpart, chunks * nw, tchunks * (nw + L.cout), bchunks * L.cout});
+            long long c1 = 0;  // 1x1 transposing-load chunks
+            for (int bb = 1; bb <= max_batch; ++bb) c1 = std::max<long long>(c1, wgrad1_chunks(bb, H, nullptr));
+            maxpart = std::max({maxpart, chunks * nw, tchunks * (nw + L.cout), c1 * (nw + L.cout), bchunks * L.cout});
         }
         // the loss partials (two doubles per sample and chunk) reuse the buffer
         const long long loss_chunks = ((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk;
