@@ -1446,7 +1446,9 @@ void upsample(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, ui
                            out_ld, 0, B, H, H, C);
 }
 
-void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
+// y_out: where the output layer writes ([B][M][M][out_ld] fp32); the
+// handle's own buffer when null.
+void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B, float* y_out = nullptr)
 {
     const int C1 = n.C[0], C2 = n.C[1], C3 = n.C[2], C4 = n.C[3];
     fwd(n, s, E1A, x, n.in_ld, 0, n.e1a, C1, 0, B);
@@ -1470,7 +1472,8 @@ void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
     fwd(n, s, D1A, n.cat1, C2 + C1, 0, n.d1a, C1, 0, B);
     fwd(n, s, D1B, n.d1a, C1, 0, n.d1, C1, 0, B);
     const Layer& L = n.L[OUT];
-    launch_conv(n, s, n.d1, C1, 0, C1, n.wbf + L.w, n.w + L.b, n.y, n.out_ld, 0, n.out_ld, true, B, n.M, n.M, 1, false);
+    launch_conv(n, s, n.d1, C1, 0, C1, n.wbf + L.w, n.w + L.b, y_out ? y_out : n.y, n.out_ld, 0, n.out_ld, true, B, n.M,
+                n.M, 1, false);
 }
 
 // Gradients of the mean per-sample loss w.r.t. every parameter (into n.g),
@@ -1729,10 +1732,7 @@ int wdg_hip_cnn_forward(wdg_cnn n, int batch, const uint16_t* d_x, float* d_y, v
         if (!d_x || !d_y) throw Fail{WDG_ERR_INVALID, "null device pointer"};
         set_device(*n);
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        forward(*n, s, d_x, batch);
-        ck(hipMemcpyAsync(d_y, n->y, sizeof(float) * batch * (long long)n->M * n->M * n->out_ld,
-                          hipMemcpyDeviceToDevice, s),
-           "copy output");
+        forward(*n, s, d_x, batch, d_y);
         ck(hipGetLastError(), "forward");
     });
 }
