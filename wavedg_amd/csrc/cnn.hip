@@ -80,6 +80,11 @@ struct ConvArgs {
     int in_ld, in_off, cin;
     int out_ld, out_off, cout;
     int B, H, W, ks, relu;
+    // conv3_k32_kernel with bf16 output: when set, outputs whose value at
+    // mask[p, mask_off + c] (leading dimension mask_ld) is not positive are
+    // stored as 0 (the ReLU backward of the layer below, fused)
+    const uint16_t* mask;
+    int mask_ld, mask_off;
 };
 
 template <int TA, int TB, bool F32OUT>
@@ -409,10 +414,18 @@ __global__ __launch_bounds__(256) void conv3_k32_kernel(ConvArgs a)
                 v[q] = acc[i][j][q] + bias[q];
                 if (a.relu) v[q] = fmaxf(v[q], 0.f);
             }
-            const size_t o = ((size_t)(b * a.H + y) * a.W + x) * a.out_ld + a.out_off + co;
+            const size_t pix = (size_t)(b * a.H + y) * a.W + x;
+            const size_t o = pix * a.out_ld + a.out_off + co;
             if constexpr (F32OUT) {
                 *reinterpret_cast<float4*>(static_cast<float*>(a.out) + o) = make_float4(v[0], v[1], v[2], v[3]);
             } else {
+                if (a.mask) {
+                    const uint2 mv = *reinterpret_cast<const uint2*>(a.mask + pix * a.mask_ld + a.mask_off + co);
+                    const uint32_t mw[2] = {mv.x, mv.y};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (!(bf2f((uint16_t)(mw[q >> 1] >> (16 * (q & 1)))) > 0.f)) v[q] = 0.f;
+                }
                 const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
                 const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
                 *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out) + o) = make_uint2(lo, hi);
@@ -1228,7 +1241,7 @@ struct wdg_cnn_s {
     uint16_t *p3 = nullptr, *ba = nullptr, *bb = nullptr;
     float* y = nullptr;
     // gradients
-    uint16_t *dy = nullptr, *dpre = nullptr;
+    uint16_t *dy = nullptr, *dpre = nullptr, *dpre2 = nullptr;  // dpre2: the other pre-activation gradient
     float *gA = nullptr, *gcat1 = nullptr, *gcat2 = nullptr, *gcat3 = nullptr, *gp1 = nullptr, *gp2 = nullptr,
           *gp3 = nullptr;
     float *partial = nullptr, *loss = nullptr;
@@ -1289,12 +1302,24 @@ int wgrad_tiled_chunks(const Layer& L, int B, int H, int* tiles_per_chunk)
     return (int)((tiles + tpc - 1) / tpc);
 }
 
+// The 3x3 convolutions that run on conv3_k32_kernel (the only one that
+// applies a ReLU-backward mask).
+bool conv_k32_ok(int cin, int cout, int ks, int W, int in_ld, int in_off)
+{
+    return ks == 3 && W > 0 && (W & (W - 1)) == 0 && cout % 32 == 0 && cin % 16 == 0 && in_ld % 8 == 0 &&
+           in_off % 8 == 0 && (cin % 32 == 0 || cin > 32);
+}
+
 void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_ld, int in_off, int cin,
                  const uint16_t* w, const float* bias, void* out, int out_ld, int out_off, int cout, bool f32out,
-                 int B, int H, int W, int ks, bool relu)
+                 int B, int H, int W, int ks, bool relu, const uint16_t* mask = nullptr, int mask_ld = 0,
+                 int mask_off = 0)
 {
     (void)n;
-    ConvArgs a{in, w, bias, out, in_ld, in_off, cin, out_ld, out_off, cout, B, H, W, ks, relu ? 1 : 0};
+    ConvArgs a{in,  w, bias, out, in_ld, in_off, cin, out_ld, out_off, cout, B, H, W, ks, relu ? 1 : 0,
+               mask, mask_ld, mask_off};
+    if (mask && (f32out || !conv_k32_ok(cin, cout, ks, W, in_ld, in_off)))
+        throw Fail{WDG_ERR_INVALID, "a ReLU-backward mask needs the 32-channel 3x3 convolution with bf16 output"};
     const long long npix = (long long)B * H * W;
     const bool pow2w = W > 0 && (W & (W - 1)) == 0;
     if (ks == 3 && pow2w && cout % 32 == 0 && cin % 16 == 0 && in_ld % 8 == 0 && in_off % 8 == 0) {
@@ -1427,6 +1452,15 @@ void relu_grad(wdg_cnn_s& n, hipStream_t s, int level, int C, const float* g1, i
         hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((long long)B * H * H * C)), dim3(256), 0, s, a);
 }
 
+// Parameter gradients of layer li, then the gradient w.r.t. its pre-ReLU
+// input where that input is act (leading dimension act_ld, offset act_off):
+// the backward-data convolution and the ReLU backward in one pass when
+// conv3_k32_kernel applies (bf16 output masked by act > 0), else the fp32
+// input gradient and relu_grad_kernel.  Returns the buffer that holds the
+// result (the other of n.dpre and n.dpre2 than dpre when fused, n.dpre else).
+const uint16_t* bwd_relu(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int in_off,
+                         const uint16_t* dpre, const uint16_t* act, int act_ld, int act_off, int B);
+
 void pool(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, int in_ld, int in_off, uint16_t* out, int C,
           int B)
 {
@@ -1448,6 +1482,23 @@ void upsample(wdg_cnn_s& n, hipStream_t s, int level_out, const uint16_t* in, ui
 
 // y_out: where the output layer writes ([B][M][M][out_ld] fp32); the
 // handle's own buffer when null.
+const uint16_t* bwd_relu(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int in_off,
+                         const uint16_t* dpre, const uint16_t* act, int act_ld, int act_off, int B)
+{
+    const Layer& L = n.L[li];
+    const int H = n.res(L.level);
+    if (conv_k32_ok(L.cout, L.cin, L.ks, H, L.cout, 0) && act_ld % 4 == 0 && act_off % 4 == 0) {
+        uint16_t* out = dpre == n.dpre ? n.dpre2 : n.dpre;
+        bwd(n, s, li, in, in_ld, in_off, dpre, nullptr, B);
+        launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, out, L.cin, 0, L.cin, false, B, H, H,
+                    L.ks, false, act, act_ld, act_off);
+        return out;
+    }
+    bwd(n, s, li, in, in_ld, in_off, dpre, n.gA, B);
+    relu_grad(n, s, L.level, L.cin, n.gA, L.cin, 0, false, nullptr, 0, act, act_ld, act_off, B);
+    return n.dpre;
+}
+
 void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B, float* y_out = nullptr)
 {
     const int C1 = n.C[0], C2 = n.C[1], C3 = n.C[2], C4 = n.C[3];
@@ -1484,36 +1535,29 @@ void backward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B)
     // output 1x1 layer: dpre is dy itself (no activation)
     bwd(n, s, OUT, n.d1, C1, 0, n.dy, n.gA, B);
     relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.d1, C1, 0, B);
-    bwd(n, s, D1B, n.d1a, C1, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.d1a, C1, 0, B);
-    bwd(n, s, D1A, n.cat1, C2 + C1, 0, n.dpre, n.gcat1, B);
+    const uint16_t* dp = bwd_relu(n, s, D1B, n.d1a, C1, 0, n.dpre, n.d1a, C1, 0, B);
+    bwd(n, s, D1A, n.cat1, C2 + C1, 0, dp, n.gcat1, B);
     // level 1 decoder: d2 feeds up() into cat1[:, 0:C2]
     relu_grad(n, s, 1, C2, n.gcat1, C2 + C1, 0, true, nullptr, 0, n.d2, C2, 0, B);
-    bwd(n, s, D2B, n.d2a, C2, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 1, C2, n.gA, C2, 0, false, nullptr, 0, n.d2a, C2, 0, B);
-    bwd(n, s, D2A, n.cat2, C3 + C2, 0, n.dpre, n.gcat2, B);
+    dp = bwd_relu(n, s, D2B, n.d2a, C2, 0, n.dpre, n.d2a, C2, 0, B);
+    bwd(n, s, D2A, n.cat2, C3 + C2, 0, dp, n.gcat2, B);
     relu_grad(n, s, 2, C3, n.gcat2, C3 + C2, 0, true, nullptr, 0, n.d3, C3, 0, B);
-    bwd(n, s, D3B, n.d3a, C3, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 2, C3, n.gA, C3, 0, false, nullptr, 0, n.d3a, C3, 0, B);
-    bwd(n, s, D3A, n.cat3, C4 + C3, 0, n.dpre, n.gcat3, B);
+    dp = bwd_relu(n, s, D3B, n.d3a, C3, 0, n.dpre, n.d3a, C3, 0, B);
+    bwd(n, s, D3A, n.cat3, C4 + C3, 0, dp, n.gcat3, B);
     // bottleneck
     relu_grad(n, s, 3, C4, n.gcat3, C4 + C3, 0, true, nullptr, 0, n.bb, C4, 0, B);
-    bwd(n, s, BB, n.ba, C4, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 3, C4, n.gA, C4, 0, false, nullptr, 0, n.ba, C4, 0, B);
-    bwd(n, s, BA, n.p3, C3, 0, n.dpre, n.gp3, B);
+    dp = bwd_relu(n, s, BB, n.ba, C4, 0, n.dpre, n.ba, C4, 0, B);
+    bwd(n, s, BA, n.p3, C3, 0, dp, n.gp3, B);
     // encoder, deepest first: e_l receives the skip gradient and the pool gradient
     relu_grad(n, s, 2, C3, n.gcat3, C4 + C3, C4, false, n.gp3, C3, n.cat3, C4 + C3, C4, B);
-    bwd(n, s, E3B, n.e3a, C3, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 2, C3, n.gA, C3, 0, false, nullptr, 0, n.e3a, C3, 0, B);
-    bwd(n, s, E3A, n.p2, C2, 0, n.dpre, n.gp2, B);
+    dp = bwd_relu(n, s, E3B, n.e3a, C3, 0, n.dpre, n.e3a, C3, 0, B);
+    bwd(n, s, E3A, n.p2, C2, 0, dp, n.gp2, B);
     relu_grad(n, s, 1, C2, n.gcat2, C3 + C2, C3, false, n.gp2, C2, n.cat2, C3 + C2, C3, B);
-    bwd(n, s, E2B, n.e2a, C2, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 1, C2, n.gA, C2, 0, false, nullptr, 0, n.e2a, C2, 0, B);
-    bwd(n, s, E2A, n.p1, C1, 0, n.dpre, n.gp1, B);
+    dp = bwd_relu(n, s, E2B, n.e2a, C2, 0, n.dpre, n.e2a, C2, 0, B);
+    bwd(n, s, E2A, n.p1, C1, 0, dp, n.gp1, B);
     relu_grad(n, s, 0, C1, n.gcat1, C2 + C1, C2, false, n.gp1, C1, n.cat1, C2 + C1, C2, B);
-    bwd(n, s, E1B, n.e1a, C1, 0, n.dpre, n.gA, B);
-    relu_grad(n, s, 0, C1, n.gA, C1, 0, false, nullptr, 0, n.e1a, C1, 0, B);
-    bwd(n, s, E1A, x, n.in_ld, 0, n.dpre, nullptr, B);
+    dp = bwd_relu(n, s, E1B, n.e1a, C1, 0, n.dpre, n.e1a, C1, 0, B);
+    bwd(n, s, E1A, x, n.in_ld, 0, dp, nullptr, B);
 }
 
 // The flipped bf16 weights of every layer from the fp32 master weights.
@@ -1670,6 +1714,7 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
         long long maxg = 0;
         for (int l = 0; l < 4; ++l) maxg = std::max(maxg, px(l) * (long long)n->C[l]);
         n->dpre = n->alloc<uint16_t>(std::max(maxg, px(0) * n->out_ld));
+        n->dpre2 = n->alloc<uint16_t>(std::max(maxg, px(0) * n->out_ld));
         n->gA = n->alloc<float>(maxg);
         n->gcat1 = n->alloc<float>(px(0) * (C2 + C1));
         n->gcat2 = n->alloc<float>(px(1) * (C3 + C2));
