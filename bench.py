@@ -115,7 +115,7 @@ def sweep(torch, capi, steps: int, precisions, autotune: bool = True):
         u0 = meshgen.nodal_state(nodes, p0)
         dt = 0.5 * mesh.min_h() / n**2
         for prec in precisions:
-            wave, u, sec, ms_k, nk = time_path(torch, capi, mesh, n, bc, u0, dt, prec, steps, 2, autotune=autotune)
+            wave, u, sec, ms_k, nk = time_path(torch, capi, mesh, n, bc, u0, dt, prec, steps, 10, autotune=autotune)
             s = 8 if prec == capi.F64 else 4
             avg = ms_k / max(nk, 1)
             gbs = stage_bytes(wave.ndof, wave.n_elem, s) / (avg * 1e-3) / 1e9
@@ -470,7 +470,7 @@ def main() -> int:
                     help="timed Advection + AdvectionHomogeneousBC actions on the same mesh (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU-oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--sweep", type=int, default=10,
+    ap.add_argument("--sweep", type=int, default=50,
                     help="also time this many steps of the per-order roofline set, P = 3..7 (0 = skip)")
     ap.add_argument("--only-sweep", action="store_true", help="skip the headline workload")
     ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the copy-bandwidth calibration (0 = skip)")
