@@ -189,13 +189,16 @@ def waveholtz_solve(torch, capi, mode: str, tol: float, max_iter: int, n_grid: i
     return W, stats
 
 
-def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_per_graph, max_work=None):
+def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_per_graph, max_work=None,
+                          hybrid_cut=None):
     """Uneven batch from the seed-4000 generator (square meshes nmin^2..nmax^2,
     P 3..7, omega in [5 pi, 25 pi], 1-4 sources): every problem solved alone
     with the plain driver (pi0, then W <- S(W) + pi0 to tol, each step replayed
     from a HIP graph), then all of them in one batch.  With max_work, only the
     generator's problems whose DoF-stages per period, ndof * nt * 5, are at
-    most max_work are kept (in generator order)."""
+    most max_work are kept (in generator order).  With hybrid_cut, the set is
+    also solved by capi.solve_many, which batches the problems at or below
+    that work per period and solves the others alone."""
     import math
 
     from wavedg_amd import meshgen
@@ -244,6 +247,21 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
         worst = max(worst, float(torch.linalg.norm(W - seq_W[k]) / torch.linalg.norm(seq_W[k])))
     ndofs = [b.info(k)[0] for k in range(len(probs))]
     nts = [b.info(k)[1] for k in range(len(probs))]
+    del b
+    hybrid = None
+    if hybrid_cut is not None:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, split = capi.solve_many(probs, tol, max_iter, steps_per_graph, max_batch_work=hybrid_cut)
+        torch.cuda.synchronize()
+        t_hyb = time.perf_counter() - t0
+        print(f"[batched] solve_many (cut {hybrid_cut:g}): {t_hyb:.1f} s", file=sys.stderr, flush=True)
+        hybrid = {"cut": hybrid_cut, "batched": len(split["batched"]), "sequential": len(split["sequential"]),
+                  "seconds": t_hyb, "speedup": t_seq / t_hyb,
+                  "updates_equal_to_sequential": [r[1] for r in res] == seq_it,
+                  "max_rel_l2_vs_sequential": max(float(torch.linalg.norm(r[0] - seq_W[k]) / torch.linalg.norm(seq_W[k]))
+                                                  for k, r in enumerate(res))}
+        del res
     return {"problems": len(probs), "generated": count, "max_work_per_period": max_work,
             "mesh_range": [nmin, nmax], "names": [p["name"] for p in probs],
             "ndof_range": [min(ndofs), max(ndofs)],
@@ -251,7 +269,8 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
             "batched_gpu_seconds": sec, "speedup": t_seq / t_bat, "batched_steps": steps,
             "updates_total": int(sum(its)), "updates_equal_to_sequential": its == seq_it,
             "max_final_residual": worst_res,
-            "max_rel_l2_batched_vs_sequential": worst, "steps_per_graph": steps_per_graph}
+            "max_rel_l2_batched_vs_sequential": worst, "steps_per_graph": steps_per_graph,
+            "solve_many": hybrid}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
@@ -490,6 +509,9 @@ def main() -> int:
                          "sequential plain solves (1) or not (0)")
     ap.add_argument("--batch-max-work", type=float, default=5e9,
                     help="keep the generator's problems whose ndof * nt * 5 (DoF-stages per period) is at most this")
+    ap.add_argument("--batch-hybrid-cut", type=float, default=5e9,
+                    help="with --only-batched, also time capi.solve_many, which batches the problems at or below "
+                         "this work per period and solves the others alone (0 = skip)")
     ap.add_argument("--fp32-long-steps", type=int, default=2000,
                     help="RK4 steps of the FP32-vs-FP64 error check on the headline workload (0 = skip)")
     ap.add_argument("--only-learned", action="store_true",
@@ -517,7 +539,8 @@ def main() -> int:
     if args.only_batched:
         r = batched_vs_sequential(torch, capi, args.batch_count, args.batch_nmin, args.batch_nmax, args.tol,
                                   args.max_iter, args.batch_graph,
-                                  args.batch_max_work if args.batch_max_work > 0 else None)
+                                  args.batch_max_work if args.batch_max_work > 0 else None,
+                                  args.batch_hybrid_cut if args.batch_hybrid_cut > 0 else None)
         print(f"[batched] {r}", file=sys.stderr, flush=True)
         print(json.dumps({"batched": r}))
         return 0

@@ -109,3 +109,27 @@ def test_stall_window_retires_fp32_at_its_floor(cuda):
         it, r = b.result(k)
         print(f"fp32 stall: problem {k} stopped after {it} updates at residual {r:.2e}", flush=True)
         assert it < 400 and r > 1e-14
+
+
+def test_solve_many_batches_small_problems_and_solves_large_ones_alone(cuda):
+    """solve_many splits the problems by their work per period: the ones at
+    or below the threshold go through one Batch, the others through the
+    standalone driver; every result equals its standalone solve either way,
+    and the work estimate uses the solver's own number of steps per period."""
+    torch = cuda
+    probs = _problems()
+    work = [capi.waveholtz_work(pb["mesh"], pb["n"], pb["omega"]) for pb in probs]
+    cut = sorted(work)[len(work) // 2]
+    tol, max_iter = 1e-8, 300
+    out, split = capi.solve_many(probs, tol, max_iter, steps_per_graph=8, max_batch_work=cut)
+    print(f"solve_many: batched {split['batched']}, sequential {split['sequential']}", flush=True)
+    assert split["batched"] and split["sequential"]
+    assert sorted(split["batched"] + split["sequential"]) == list(range(len(probs)))
+    for k, pb in enumerate(probs):
+        Ws, its, rs, nt = _solo(torch, pb, tol, max_iter)
+        assert work[k] == 3.0 * pb["n"] ** 2 * pb["mesh"].n_elem * nt * 5
+        W, it, r = out[k]
+        torch.cuda.synchronize()
+        assert it == its
+        assert abs(r - rs) <= 1e-9 * rs
+        assert rel_l2(W.cpu().numpy(), Ws.cpu().numpy()) < 1e-12
