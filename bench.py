@@ -198,7 +198,8 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
     generator's problems whose DoF-stages per period, ndof * nt * 5, are at
     most max_work are kept (in generator order).  With hybrid_cut, the set is
     also solved by capi.solve_many, which batches the problems at or below
-    that work per period and solves the others alone."""
+    that work per period and solves the others alone, once in turn and once
+    with the batch on a second stream and host thread."""
     import math
 
     from wavedg_amd import meshgen
@@ -250,18 +251,23 @@ def batched_vs_sequential(torch, capi, count, nmin, nmax, tol, max_iter, steps_p
     del b
     hybrid = None
     if hybrid_cut is not None:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res, split = capi.solve_many(probs, tol, max_iter, steps_per_graph, max_batch_work=hybrid_cut)
-        torch.cuda.synchronize()
-        t_hyb = time.perf_counter() - t0
-        print(f"[batched] solve_many (cut {hybrid_cut:g}): {t_hyb:.1f} s", file=sys.stderr, flush=True)
-        hybrid = {"cut": hybrid_cut, "batched": len(split["batched"]), "sequential": len(split["sequential"]),
-                  "seconds": t_hyb, "speedup": t_seq / t_hyb,
-                  "updates_equal_to_sequential": [r[1] for r in res] == seq_it,
-                  "max_rel_l2_vs_sequential": max(float(torch.linalg.norm(r[0] - seq_W[k]) / torch.linalg.norm(seq_W[k]))
-                                                  for k, r in enumerate(res))}
-        del res
+        hybrid = {}
+        for overlap in (False, True):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res, split = capi.solve_many(probs, tol, max_iter, steps_per_graph, max_batch_work=hybrid_cut,
+                                         overlap=overlap)
+            torch.cuda.synchronize()
+            t_hyb = time.perf_counter() - t0
+            tag = "overlapped" if overlap else "serial"
+            print(f"[batched] solve_many {tag} (cut {hybrid_cut:g}): {t_hyb:.1f} s", file=sys.stderr, flush=True)
+            hybrid[tag] = {"cut": hybrid_cut, "batched": len(split["batched"]),
+                           "sequential": len(split["sequential"]), "seconds": t_hyb, "speedup": t_seq / t_hyb,
+                           "updates_equal_to_sequential": [r[1] for r in res] == seq_it,
+                           "max_rel_l2_vs_sequential": max(
+                               float(torch.linalg.norm(r[0] - seq_W[k]) / torch.linalg.norm(seq_W[k]))
+                               for k, r in enumerate(res))}
+            del res
     return {"problems": len(probs), "generated": count, "max_work_per_period": max_work,
             "mesh_range": [nmin, nmax], "names": [p["name"] for p in probs],
             "ndof_range": [min(ndofs), max(ndofs)],

@@ -111,17 +111,22 @@ def test_stall_window_retires_fp32_at_its_floor(cuda):
         assert it < 400 and r > 1e-14
 
 
-def test_solve_many_batches_small_problems_and_solves_large_ones_alone(cuda):
+@pytest.mark.parametrize("overlap", [False, True], ids=["serial", "overlapped"])
+def test_solve_many_batches_small_problems_and_solves_large_ones_alone(cuda, overlap):
     """solve_many splits the problems by their work per period: the ones at
     or below the threshold go through one Batch, the others through the
-    standalone driver; every result equals its standalone solve either way,
-    and the work estimate uses the solver's own number of steps per period."""
+    standalone driver (with overlap, the batch runs from a second host
+    thread on its own stream meanwhile); every result equals its standalone
+    solve either way, and the work estimate uses the solver's own number of
+    steps per period."""
     torch = cuda
     probs = _problems()
     work = [capi.waveholtz_work(pb["mesh"], pb["n"], pb["omega"]) for pb in probs]
     cut = sorted(work)[len(work) // 2]
     tol, max_iter = 1e-8, 300
-    out, split = capi.solve_many(probs, tol, max_iter, steps_per_graph=8, max_batch_work=cut)
+    print(f"solve_many overlap={overlap}", flush=True)
+    out, split = capi.solve_many(probs, tol, max_iter, steps_per_graph=8, max_batch_work=cut, overlap=overlap)
+    torch.cuda.synchronize()
     print(f"solve_many: batched {split['batched']}, sequential {split['sequential']}", flush=True)
     assert split["batched"] and split["sequential"]
     assert sorted(split["batched"] + split["sequential"]) == list(range(len(probs)))

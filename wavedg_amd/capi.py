@@ -734,42 +734,89 @@ def waveholtz_work(mesh: Mesh, n: int, omega: float) -> float:
 
 
 def solve_many(problems, tol: float, max_iter: int, steps_per_graph: int = 16, max_batch_work: float = 5e9,
-               stream=None):
+               stream=None, overlap: bool = False):
     """Solve independent WaveHoltz problems (dicts as for Batch) to the
     relative fixed-point residual tol, each from W_0 = 0, in FP64.  Problems
     whose work per period (waveholtz_work) is at most max_batch_work run as
     one Batch, where lockstep per-stage launches amortise the launch and
     reduction overhead that dominates small solves; larger ones, which fill
     the GPU on their own, run one after another with graph-replayed
-    fixed-point steps.  Both paths give every problem the same iterates as a
-    standalone solve.  Returns ([(W, iterations, residual)] in problem order,
+    fixed-point steps.  With overlap, the batch runs on a stream of its own
+    from a second host thread while this thread solves the large problems on
+    another, so the batch's launch and polling gaps are filled by the large
+    solves' kernels.  Every handle and array is created before the two
+    threads start, and both use non-blocking streams: HIP refuses null-stream
+    work while another thread captures a graph (the library serialises its
+    own captures, allocations and copies on those paths).  Every problem gets
+    the same iterates as a standalone solve.  Returns
+    ([(W, iterations, residual)] in problem order,
     {"batched": indices, "sequential": indices})."""
+    import threading
+
     import torch
 
     work = [waveholtz_work(pb["mesh"], pb["n"], pb["omega"]) for pb in problems]
     small = [k for k, wk in enumerate(work) if wk <= max_batch_work]
     large = [k for k, wk in enumerate(work) if wk > max_batch_work]
     out = [None] * len(problems)
+    b, bW = None, []
     if small:
         b = Batch([problems[k] for k in small])
-        b.solve(tol, max_iter, steps_per_graph, stream=stream)
+        bW = [torch.empty(b.info(i)[0], dtype=torch.float64, device="cuda") for i in range(len(small))]
+
+    def run_batch(s):
+        b.solve(tol, max_iter, steps_per_graph, stream=s)
         for i, k in enumerate(small):
-            W = torch.empty(b.info(i)[0], dtype=torch.float64, device="cuda")
-            it, r = b.result(i, W)
-            out[k] = (W, it, r)
-        del b
-    for k in large:
+            it, r = b.result(i, bW[i])  # synchronous device-to-device copy
+            out[k] = (bW[i], it, r)
+
+    def setup(k, s):
         pb = problems[k]
         w = Wave2D(pb["mesh"], pb["n"], pb.get("bc"))
         wh = WaveHoltz(w, pb["omega"])
         F = torch.empty(w.ndof, dtype=torch.float64, device="cuda")
-        w.load_vector_gauss(pb["sources"], F, stream=stream)
+        w.load_vector_gauss(pb["sources"], F, stream=s)
         pi0 = torch.zeros_like(F)
-        wh.pi0(pi0, F, stream=stream)
-        W = torch.zeros_like(F)
-        it, _, r = wh.solve(W, pi0, norm(pi0, stream=stream), tol, max_iter, stream=stream)
-        out[k] = (W, it, r)
-        del wh, w, F, pi0
+        wh.pi0(pi0, F, stream=s)
+        return w, wh, pi0, torch.zeros_like(F), norm(pi0, stream=s)
+
+    if not (b is not None and large and overlap):
+        s = stream if stream is not None else cur_stream()
+        if b is not None:
+            run_batch(s)
+        for k in large:
+            w, wh, pi0, W, nrm = setup(k, s)
+            it, _, r = wh.solve(W, pi0, nrm, tol, max_iter, stream=s)
+            out[k] = (W, it, r)
+            del wh, w, pi0
+        del b
+        return out, {"batched": small, "sequential": large}
+
+    base = stream if stream is not None else cur_stream()
+    jobs = [setup(k, base) for k in large]
+    torch.cuda.synchronize()  # inputs complete before the two streams start
+    bs, ms = torch.cuda.Stream(), torch.cuda.Stream()
+    failure = []
+
+    def target():
+        try:
+            run_batch(bs.cuda_stream)
+        except BaseException as exc:  # re-raised on the calling thread
+            failure.append(exc)
+
+    worker = threading.Thread(target=target, name="wdg-batch")
+    worker.start()
+    try:
+        for k, (w, wh, pi0, W, nrm) in zip(large, jobs):
+            it, _, r = wh.solve(W, pi0, nrm, tol, max_iter, stream=ms.cuda_stream)
+            out[k] = (W, it, r)
+    finally:
+        worker.join()
+        ms.synchronize()
+        bs.synchronize()
+    if failure:
+        raise failure[0]
+    del jobs, b
     return out, {"batched": small, "sequential": large}
 
 

@@ -33,6 +33,7 @@
 
 #include "host_basis.hpp"
 #include "host_mesh.hpp"
+#include "host_sync.hpp"
 #include "wave2d_kernels.hpp"
 
 namespace {
@@ -528,6 +529,7 @@ void compact(wdg_batch_s& b, const std::vector<int>& run, hipStream_t s)
 template <typename Fn>
 void capture(wdg_batch_s& b, int steps, Fn&& step)
 {
+    std::lock_guard<std::mutex> lk(wdg::capture_mutex());
     if (b.graph) (void)hipGraphExecDestroy(b.graph);
     b.graph = nullptr;
     if (!b.cap) bcheck(hipStreamCreateWithFlags(&b.cap, hipStreamNonBlocking), "hipStreamCreate");
@@ -798,6 +800,7 @@ int wdg_hip_batch_solve(wdg_batch b, double tol, int max_iter, int steps_per_gra
         const hipStream_t s = static_cast<hipStream_t>(stream);
         // reset: zero state and accumulators, all problems active
         if (b->hist_len < max_iter + 1) {
+            std::lock_guard<std::mutex> lk(wdg::capture_mutex());
             if (b->d_hist) (void)hipFree(b->d_hist);
             b->hist_len = max_iter + 1;
             b->d_hist = dalloc<double>((size_t)b->hist_len * b->nprob);
@@ -866,6 +869,7 @@ int wdg_hip_batch_result(wdg_batch b, int k, double* d_W, int* iterations, doubl
 {
     return bguard([&] {
         if (!b || k < 0 || k >= b->nprob) throw Fail{WDG_ERR_INVALID, "bad batch handle or index"};
+        std::lock_guard<std::mutex> lk(wdg::capture_mutex());  // null-stream copies
         if (d_W)
             bcheck(hipMemcpy(d_W, b->d_wprev + b->probs[k].soff, sizeof(double) * b->ndof[k], hipMemcpyDeviceToDevice),
                    "copy result");

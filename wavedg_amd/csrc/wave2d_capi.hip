@@ -21,6 +21,7 @@
 
 #include "host_basis.hpp"
 #include "host_mesh.hpp"
+#include "host_sync.hpp"
 #include "wave2d_kernels.hpp"
 
 using wdg::KArgs;
@@ -28,7 +29,13 @@ using wdg::KArgs;
 // ------------------------------------------------------------------ errors
 namespace wdg {
 thread_local std::string g_last_error;
+
+std::mutex& capture_mutex()
+{
+    static std::mutex m;
+    return m;
 }
+}  // namespace wdg
 using wdg::g_last_error;
 
 namespace {
@@ -1165,6 +1172,7 @@ int wdg_hip_rk4_steps_graph(wdg_wave2d op, double dt, double* t, void* d_u, int 
         const auto key = std::make_tuple(d_u, steps_per_graph, dt);
         auto it = op->graphs.find(key);
         if (it == op->graphs.end()) {
+            std::lock_guard<std::mutex> lk(wdg::capture_mutex());
             if (!op->cap_stream) hip_check(hipStreamCreateWithFlags(&op->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
             op->ensure(op->d_tmp, op->state_bytes());
             op->ensure(op->d_du, op->state_bytes());
@@ -1293,10 +1301,13 @@ int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double* d_w, const double* d_pi0
         if (!(pi0_norm > 0.0)) fail(WDG_ERR_INVALID, "pi0_norm must be positive");
         wdg_wave2d op = wh->op;
         const hipStream_t s = as_stream(stream);
-        if (!wh->d_wprev) wh->d_wprev = static_cast<double*>(dmalloc(sizeof(double) * op->ndof));
-        op->ensure(op->d_tmp, op->state_bytes());
-        op->ensure(op->d_du, op->state_bytes());
-        op->ensure(wh->d_p, op->state_bytes());
+        {
+            std::lock_guard<std::mutex> lk(wdg::capture_mutex());
+            if (!wh->d_wprev) wh->d_wprev = static_cast<double*>(dmalloc(sizeof(double) * op->ndof));
+            op->ensure(op->d_tmp, op->state_bytes());
+            op->ensure(op->d_du, op->state_bytes());
+            op->ensure(wh->d_p, op->state_bytes());
+        }
         // S(W) with W_prev = W, then W += pi0 and the residual's sum of
         // squares (waveholtz2d.cpp:170-175); the same launches eager or as a
         // replayed graph, so both give identical results
@@ -1310,6 +1321,7 @@ int wdg_hip_waveholtz_iterate(wdg_waveholtz wh, double* d_w, const double* d_pi0
         if (wh->use_graph) {
             const auto key = std::make_tuple(d_w, d_pi0, op->sched);
             if (!wh->it_graph || wh->it_key != key) {
+                std::lock_guard<std::mutex> lk(wdg::capture_mutex());
                 if (wh->it_graph) (void)hipGraphExecDestroy(wh->it_graph);
                 wh->it_graph = nullptr;
                 if (!op->cap_stream)
