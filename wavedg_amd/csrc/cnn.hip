@@ -743,7 +743,20 @@ __global__ __launch_bounds__(1024) void wgrad1_tr_kernel(WgradArgs a)
 // slice of 16 padded with zeros.
 constexpr int kC1MaxOb = 8;
 
-__global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a)
+// Training-time epilogue of the output layer: the target t (same layout as
+// the output) is read alongside, the output buffer receives e = y - t on the
+// first creal channels and 0 on the padding ones, and each wave adds
+// sum t^2 and sum (y - t)^2 over its 16 pixels in double (fixed xor-shuffle
+// order) into part[2 * (pixel / 16)], part[2 * (pixel / 16) + 1]: a wave's
+// pixels lie in one sample when H * W is a multiple of 16.
+struct LossArgs {
+    const float* t;
+    double* part;
+    int creal;
+};
+
+template <bool LOSS>
+__global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a, LossArgs l)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ln = lane & 15, kq = lane >> 4;
@@ -770,7 +783,10 @@ __global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a)
             acc[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w), bv, acc[ob], 0, 0, 0);
         }
     }
-    if (!pok) return;
+    if constexpr (!LOSS) {
+        if (!pok) return;
+    }
+    double tt = 0.0, ee = 0.0;
 #pragma unroll
     for (int ob = 0; ob < kC1MaxOb; ++ob) {
         if (ob >= nob) break;
@@ -783,8 +799,37 @@ __global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a)
         if (a.relu)
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q], 0.f);
-        *reinterpret_cast<float4*>(static_cast<float*>(a.out) + p * a.out_ld + a.out_off + c) =
-            make_float4(v[0], v[1], v[2], v[3]);
+        if constexpr (LOSS) {
+            if (pok) {
+                const float4 t4 = *reinterpret_cast<const float4*>(l.t + p * a.out_ld + a.out_off + c);
+                const float tq[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (c + q < l.creal) {
+                        const double tv = tq[q], dv = (double)v[q] - tv;
+                        tt += tv * tv;
+                        ee += dv * dv;
+                        v[q] = v[q] - tq[q];
+                    } else {
+                        v[q] = 0.f;
+                    }
+                }
+            }
+        }
+        if (pok)
+            *reinterpret_cast<float4*>(static_cast<float*>(a.out) + p * a.out_ld + a.out_off + c) =
+                make_float4(v[0], v[1], v[2], v[3]);
+    }
+    if constexpr (LOSS) {
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            tt += __shfl_xor(tt, m);
+            ee += __shfl_xor(ee, m);
+        }
+        if (lane == 0) {
+            l.part[2 * (pb / 16)] = tt;
+            l.part[2 * (pb / 16) + 1] = ee;
+        }
     }
 }
 
@@ -983,7 +1028,9 @@ __global__ void relu_grad8_kernel(PrepArgs a)
 // Per-sample relative squared error ||y - t||^2 / ||t||^2 over the first
 // creal channels and dy = 2 (y - t) / (B ||t||^2) as bf16 (0 on padding), in
 // three deterministic phases: per-(sample, chunk) partial sums, per-sample
-// totals in chunk order, then dy.
+// totals in chunk order, then dy.  Training normally takes the fused route
+// (conv1_k32_kernel<true>, loss_final_kernel, loss_grad_e_kernel); these
+// kernels serve output layers that the 1x1 MFMA kernel does not take.
 constexpr int kLossChunk = 4096;  // elements per partial
 
 __global__ __launch_bounds__(256) void loss_partial_kernel(const float* y, const float* t, int ld, int creal, int n,
@@ -1052,6 +1099,20 @@ __global__ void loss_grad_kernel(const float* y, const float* t, int ld, int cre
     if (i >= (size_t)B * n) return;
     const int b = (int)(i / n), c = (int)(i % n) % ld;
     dy[i] = c < creal ? f2bf(scale[b] * (y[i] - t[i])) : (uint16_t)0;
+}
+
+// dy = scale[b] e as bf16, four per thread, from e = y - t (0 on padding
+// channels) written by the output layer's training epilogue: the same bits
+// as loss_grad_kernel gives from y and t.
+__global__ void loss_grad_e_kernel(const float* e, int n, long long total, const float* scale, uint16_t* dy)
+{
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= total) return;
+    const float sc = scale[i / n];
+    const float4 v = *reinterpret_cast<const float4*>(e + i);
+    const uint32_t lo = (uint32_t)f2bf(sc * v.x) | ((uint32_t)f2bf(sc * v.y) << 16);
+    const uint32_t hi = (uint32_t)f2bf(sc * v.z) | ((uint32_t)f2bf(sc * v.w) << 16);
+    *reinterpret_cast<uint2*>(dy + i) = make_uint2(lo, hi);
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master copy; c1 = 1/(1-b1^t),
@@ -1374,7 +1435,8 @@ void launch_conv(const wdg_cnn_s& n, hipStream_t s, const uint16_t* in, int in_l
     }
     if (ks == 1 && f32out && cout % 16 == 0 && cout <= 16 * kC1MaxOb && cin % 8 == 0 && in_ld % 8 == 0 &&
         in_off % 8 == 0 && out_ld % 4 == 0 && out_off % 4 == 0) {
-        hipLaunchKernelGGL(conv1_k32_kernel, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(conv1_k32_kernel<false>, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, a,
+                           LossArgs{});
         return;
     }
     const int nb = cout / 16;
@@ -1514,7 +1576,17 @@ const uint16_t* bwd_relu(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in
     return n.dpre;
 }
 
-void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B, float* y_out = nullptr)
+// The output layer's fused training epilogue applies (1x1 MFMA path, wave
+// partials of 16 pixels inside one sample).
+bool loss_fused_ok(const wdg_cnn_s& n)
+{
+    return n.out_ld % 16 == 0 && n.out_ld <= 16 * kC1MaxOb && n.L[OUT].cin % 8 == 0 && (n.M * n.M) % 16 == 0;
+}
+
+// With loss (training), the output layer writes e = y - t into n.y and the
+// loss partials instead of y (see conv1_k32_kernel<true>).
+void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B, float* y_out = nullptr,
+             const LossArgs* loss = nullptr)
 {
     const int C1 = n.C[0], C2 = n.C[1], C3 = n.C[2], C4 = n.C[3];
     fwd(n, s, E1A, x, n.in_ld, 0, n.e1a, C1, 0, B);
@@ -1538,6 +1610,13 @@ void forward(wdg_cnn_s& n, hipStream_t s, const uint16_t* x, int B, float* y_out
     fwd(n, s, D1A, n.cat1, C2 + C1, 0, n.d1a, C1, 0, B);
     fwd(n, s, D1B, n.d1a, C1, 0, n.d1, C1, 0, B);
     const Layer& L = n.L[OUT];
+    if (loss) {
+        const ConvArgs a{n.d1, n.wbf + L.w, n.w + L.b, n.y, C1, 0, C1, n.out_ld, 0, n.out_ld, B, n.M, n.M, 1, 0,
+                         nullptr, 0, 0};
+        const long long npix = (long long)B * n.M * n.M;
+        hipLaunchKernelGGL(conv1_k32_kernel<true>, dim3((unsigned)((npix + 63) / 64)), dim3(256), 0, s, a, *loss);
+        return;
+    }
     launch_conv(n, s, n.d1, C1, 0, C1, n.wbf + L.w, n.w + L.b, y_out ? y_out : n.y, n.out_ld, 0, n.out_ld, true, B, n.M,
                 n.M, 1, false);
 }
@@ -1600,14 +1679,26 @@ void refresh_copies(wdg_cnn_s& n, hipStream_t s)
 
 void loss_grad(wdg_cnn_s& n, hipStream_t s, int B, const uint16_t* x, const float* t)
 {
-    forward(n, s, x, B);
     const int nper = n.M * n.M * n.out_ld;
-    const int chunks = (nper + kLossChunk - 1) / kLossChunk;
     double* part = reinterpret_cast<double*>(n.partial);  // free between the weight-gradient passes
-    hipLaunchKernelGGL(loss_partial_kernel, dim3(chunks, B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, nper, part);
-    hipLaunchKernelGGL(loss_final_kernel, dim3(B), dim3(256), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for((long long)B * nper)), dim3(256), 0, s, n.y, t, n.out_ld,
-                       n.out_real, nper, B, n.loss + n.maxB, n.dy);
+    if (loss_fused_ok(n)) {
+        // the output layer writes e = y - t and per-wave partials; dy from e alone
+        const LossArgs la{t, part, n.out_real};
+        forward(n, s, x, B, nullptr, &la);
+        const int chunks = n.M * n.M / 16;
+        hipLaunchKernelGGL(loss_final_kernel, dim3(B), dim3(256), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
+        const long long total = (long long)B * nper;
+        hipLaunchKernelGGL(loss_grad_e_kernel, dim3((unsigned)((total / 4 + 255) / 256)), dim3(256), 0, s, n.y, nper,
+                           total, n.loss + n.maxB, n.dy);
+    } else {
+        forward(n, s, x, B);
+        const int chunks = (nper + kLossChunk - 1) / kLossChunk;
+        hipLaunchKernelGGL(loss_partial_kernel, dim3(chunks, B), dim3(256), 0, s, n.y, t, n.out_ld, n.out_real, nper,
+                           part);
+        hipLaunchKernelGGL(loss_final_kernel, dim3(B), dim3(256), 0, s, part, chunks, B, n.loss, n.loss + n.maxB);
+        hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for((long long)B * nper)), dim3(256), 0, s, n.y, t,
+                           n.out_ld, n.out_real, nper, B, n.loss + n.maxB, n.dy);
+    }
     backward(n, s, x, B);
 }
 
@@ -1692,7 +1783,8 @@ int wdg_hip_cnn_create(int M, int in_ld, int out_channels, int base_width, int m
             maxpart = std::max({maxpart, chunks * nw, tchunks * (nw + L.cout), c1 * (nw + L.cout), bchunks * L.cout});
         }
         // the loss partials (two doubles per sample and chunk) reuse the buffer
-        const long long loss_chunks = ((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk;
+        const long long loss_chunks =
+            std::max(((long long)M * M * n->out_ld + kLossChunk - 1) / kLossChunk, (long long)M * M / 16);
         maxpart = std::max(maxpart, 4LL * max_batch * loss_chunks);
         n->nparam = off;
         n->nflip = foff;
