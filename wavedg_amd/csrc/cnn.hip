@@ -864,6 +864,30 @@ __global__ __launch_bounds__(64 * kRedWaves) void reduce_chunks_kernel(const flo
     }
 }
 
+// The same sums for chunks <= kRedWaves, one thread per output: wave c of
+// reduce_chunks_kernel then holds chunk c alone, so both add
+// 0 + c_0 + c_1 + ... in chunk order and give the same bits, but here every
+// thread has work (layers with many weights and few chunks).
+__global__ __launch_bounds__(256) void reduce_few_chunks_kernel(const float* partial, long long n, int chunks,
+                                                                float* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float tot = 0.f;
+    for (int c = 0; c < chunks; ++c) tot += partial[(size_t)c * n + i];
+    out[i] = tot;
+}
+
+void reduce_chunks(hipStream_t s, const float* partial, long long n, int chunks, float* out)
+{
+    if (chunks <= kRedWaves)
+        hipLaunchKernelGGL(reduce_few_chunks_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n,
+                           chunks, out);
+    else
+        hipLaunchKernelGGL(reduce_chunks_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * kRedWaves), 0, s, partial,
+                           n, chunks, out);
+}
+
 // partial[chunk][c] = sum over the chunk's kBiasChunk pixels of d[p, c].
 // Thread (g, c) of the block (g < G = 256 / C groups) sums pixels g, g + G,
 // ... for channel c (consecutive threads read consecutive channels) with two
@@ -1481,14 +1505,12 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int ntap = L.ks * L.ks;
     const long long nw = (long long)ntap * L.cout * L.cin;
     WgradArgs wa{in, dpre, n.partial, in_ld, in_off, L.cin, L.cout, L.cout, B, H, H, L.ks, 1};
-    const dim3 rblk(64 * kRedWaves);
     if (wgrad_tr_ok(L, H, in_ld, in_off)) {
         const int wchunks = wgrad_tiled_chunks(L, B, H, &wa.tiles_per_chunk);
         hipLaunchKernelGGL(wgrad3_tr_kernel, dim3(wchunks, L.cout / 32, (L.cin + 31) / 32), dim3(64 * kWgWaves), 0, s,
                            wa);
         // rows of [weights][bias]: the bias follows the weights in the parameter vector
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw + L.cout, 64)), rblk, 0, s, n.partial,
-                           nw + L.cout, wchunks, n.g + L.w);
+        reduce_chunks(s, n.partial, nw + L.cout, wchunks, n.g + L.w);
         if (gin)
             launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H,
                         L.ks, false);
@@ -1498,8 +1520,7 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
         const int wchunks = wgrad1_chunks(B, H, &wa.tiles_per_chunk);
         const size_t lds = (size_t)64 * (L.cout + L.cin) * sizeof(uint16_t);
         hipLaunchKernelGGL(wgrad1_tr_kernel, dim3(wchunks), dim3(64 * (L.cout / 16)), lds, s, wa);
-        hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw + L.cout, 64)), rblk, 0, s, n.partial,
-                           nw + L.cout, wchunks, n.g + L.w);
+        reduce_chunks(s, n.partial, nw + L.cout, wchunks, n.g + L.w);
         if (gin)
             launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H,
                         L.ks, false);
@@ -1508,11 +1529,10 @@ void bwd(wdg_cnn_s& n, hipStream_t s, int li, const uint16_t* in, int in_ld, int
     const int wchunks = chunks;
     const int tiles = ntap * (L.cout / 16) * (L.cin / 16);
     hipLaunchKernelGGL(wgrad_kernel, dim3(chunks, (tiles + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, s, wa);
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(nw, 64)), rblk, 0, s, n.partial, nw, wchunks, n.g + L.w);
+    reduce_chunks(s, n.partial, nw, wchunks, n.g + L.w);
     const int bchunks = (int)((npix + kBiasChunk - 1) / kBiasChunk);
     hipLaunchKernelGGL(bias_partial_kernel, dim3(bchunks), dim3(256), 0, s, dpre, L.cout, (int)npix, n.partial);
-    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(blocks_for(L.cout, 64)), rblk, 0, s, n.partial, (long long)L.cout,
-                       bchunks, n.g + L.b);
+    reduce_chunks(s, n.partial, (long long)L.cout, bchunks, n.g + L.b);
     if (gin)
         launch_conv(n, s, dpre, L.cout, 0, L.cout, n.wflip + L.wb, nullptr, gin, L.cin, 0, L.cin, true, B, H, H, L.ks,
                     false);
