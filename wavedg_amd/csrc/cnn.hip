@@ -1167,9 +1167,33 @@ __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long long n)
 // array belongs to the first layer l with i < end[l].
 struct FlipTable {
     long long w[NLAYER_MAX], wb[NLAYER_MAX], end[NLAYER_MAX];
+    long long tend[NLAYER_MAX];  // wflip_tiled_kernel: cumulative 16 x 16 tile counts
     int cout[NLAYER_MAX], cin[NLAYER_MAX], ntap[NLAYER_MAX];
     int nl;
 };
+
+// The same copies through 16 x 16 LDS tiles (every layer's channel counts
+// multiples of 16): a workgroup reads 16 output x 16 input channels of one
+// tap along the input channels and writes them transposed along the output
+// channels, so both sides are contiguous.  Each copy is f2bf of the same
+// weight, so the bits equal wflip_all_kernel's.
+__global__ __launch_bounds__(256) void wflip_tiled_kernel(const float* w, uint16_t* wb, FlipTable t)
+{
+    __shared__ float tile[16][17];
+    const long long b = blockIdx.x;
+    int l = 0;
+    while (b >= t.tend[l]) ++l;  // uniform across the workgroup
+    const long long tb = b - (l ? t.tend[l - 1] : 0);
+    const int cout = t.cout[l], cin = t.cin[l], ntap = t.ntap[l];
+    const int nci = cin >> 4, nco = cout >> 4;
+    const int cit = (int)(tb % nci);
+    const long long r = tb / nci;
+    const int cot = (int)(r % nco), tp = (int)(r / nco);
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    tile[ty][tx] = w[t.w[l] + ((long long)tp * cout + 16 * cot + ty) * cin + 16 * cit + tx];
+    __syncthreads();
+    wb[t.wb[l] + ((long long)(ntap - 1 - tp) * cin + 16 * cit + ty) * cout + 16 * cot + tx] = f2bf(tile[tx][ty]);
+}
 
 __global__ void wflip_all_kernel(const float* w, uint16_t* wb, FlipTable t)
 {
@@ -1679,6 +1703,8 @@ void flip_weights(wdg_cnn_s& n, hipStream_t s)
 {
     FlipTable t{};
     t.nl = NLAYER;
+    bool tiled = true;
+    long long tiles = 0;
     for (int l = 0; l < NLAYER; ++l) {
         const Layer& L = n.L[l];
         t.w[l] = L.w;
@@ -1687,8 +1713,14 @@ void flip_weights(wdg_cnn_s& n, hipStream_t s)
         t.cout[l] = L.cout;
         t.cin[l] = L.cin;
         t.ntap[l] = L.ks * L.ks;
+        tiled = tiled && L.cout % 16 == 0 && L.cin % 16 == 0;
+        tiles += (long long)L.ks * L.ks * (L.cout / 16) * (L.cin / 16);
+        t.tend[l] = tiles;
     }
-    hipLaunchKernelGGL(wflip_all_kernel, dim3(blocks_for(n.nflip)), dim3(256), 0, s, n.w, n.wflip, t);
+    if (tiled)
+        hipLaunchKernelGGL(wflip_tiled_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n.w, n.wflip, t);
+    else
+        hipLaunchKernelGGL(wflip_all_kernel, dim3(blocks_for(n.nflip)), dim3(256), 0, s, n.w, n.wflip, t);
 }
 
 void refresh_copies(wdg_cnn_s& n, hipStream_t s)
