@@ -769,6 +769,17 @@ __global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a, LossArgs l)
     f32x4 acc[kC1MaxOb];
 #pragma unroll
     for (int ob = 0; ob < kC1MaxOb; ++ob) acc[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // training: the target's values for the epilogue, loaded before the
+    // operand loads so that their latency overlaps the MFMAs
+    float4 tpre[LOSS ? kC1MaxOb : 1];
+    if constexpr (LOSS) {
+#pragma unroll
+        for (int ob = 0; ob < kC1MaxOb; ++ob) {
+            if (ob >= nob) break;  // uniform
+            tpre[ob] = pok ? *reinterpret_cast<const float4*>(l.t + p * a.out_ld + a.out_off + 16 * ob + 4 * kq)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
     for (int k0 = 0; k0 < a.cin; k0 += 32) {
         const int kk = k0 + 8 * kq;
         const bool kin = kk < a.cin;
@@ -801,7 +812,7 @@ __global__ __launch_bounds__(256) void conv1_k32_kernel(ConvArgs a, LossArgs l)
             for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q], 0.f);
         if constexpr (LOSS) {
             if (pok) {
-                const float4 t4 = *reinterpret_cast<const float4*>(l.t + p * a.out_ld + a.out_off + c);
+                const float4 t4 = tpre[ob];
                 const float tq[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
